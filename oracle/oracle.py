@@ -1,0 +1,313 @@
+"""ctypes view of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, where it is the checker or the timed CPU
+baseline -- never the thing that is measured or shipped.
+
+Arrays are numpy uint64 views of the reference ABI layouts:
+Fr (n, 4), G1 affine (n, 12) = x[6] || y[6], G1 Jacobian (n, 18).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class Domain(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint32), ("extended_k", ctypes.c_uint32), ("j", ctypes.c_uint32)] + [
+        (name, ctypes.c_uint64 * 4)
+        for name in ("omega", "omega_inv", "extended_omega", "extended_omega_inv",
+                     "ifft_divisor", "extended_ifft_divisor", "g_coset", "g_coset_inv",
+                     "barycentric_weight")
+    ]
+
+    def fr(self, name):
+        return np.array(getattr(self, name)[:], dtype=np.uint64)
+
+
+class Rng(ctypes.Structure):
+    _fields_ = [("key", ctypes.c_uint32 * 8), ("counter", ctypes.c_uint64),
+                ("buf", ctypes.c_uint32 * 64), ("idx", ctypes.c_uint32)]
+
+
+def _declare(L):
+    V = None
+    sig = {
+        "orc_fr_add": (V, [u64p, u64p, u64p]), "orc_fr_sub": (V, [u64p, u64p, u64p]),
+        "orc_fr_mul": (V, [u64p, u64p, u64p]), "orc_fr_neg": (V, [u64p, u64p]),
+        "orc_fr_inv": (V, [u64p, u64p]), "orc_fr_one": (V, [u64p]),
+        "orc_fr_from_u64": (V, [u64p, ctypes.c_uint64]),
+        "orc_fr_to_canonical": (V, [u64p, u64p]),
+        "orc_fr_from_canonical": (ctypes.c_int, [u64p, u64p]),
+        "orc_fr_from_uniform_bytes": (V, [u64p, u8p]),
+        "orc_fr_root_of_unity": (V, [u64p]), "orc_fr_zeta": (V, [u64p]), "orc_fr_delta": (V, [u64p]),
+        "orc_fp_add": (V, [u64p, u64p, u64p]), "orc_fp_sub": (V, [u64p, u64p, u64p]),
+        "orc_fp_mul": (V, [u64p, u64p, u64p]), "orc_fp_inv": (V, [u64p, u64p]),
+        "orc_fp_one": (V, [u64p]),
+        "orc_fp_to_canonical": (V, [u64p, u64p]),
+        "orc_fp_from_canonical": (ctypes.c_int, [u64p, u64p]),
+        "orc_g1_generator": (V, [u64p]),
+        "orc_g1j_to_affine": (V, [u64p, u64p]), "orc_g1j_from_affine": (V, [u64p, u64p]),
+        "orc_g1j_dbl": (V, [u64p, u64p]), "orc_g1j_add": (V, [u64p, u64p, u64p]),
+        "orc_g1j_eq": (ctypes.c_int, [u64p, u64p]),
+        "orc_g1a_is_on_curve": (ctypes.c_int, [u64p]),
+        "orc_g1_mul": (V, [u64p, u64p, u64p]),
+        "orc_g1_batch_to_affine": (V, [u64p, u64p, ctypes.c_size_t]),
+        "orc_g1_compress": (V, [u8p, u64p]),
+        "orc_g1_decompress": (ctypes.c_int, [u64p, u8p]),
+        "orc_g1_from_uncompressed": (ctypes.c_int, [u64p, u8p]),
+        "orc_g1_to_uncompressed": (V, [u8p, u64p]),
+        "orc_msm_naive": (V, [u64p, u64p, u64p, ctypes.c_size_t]),
+        "orc_msm_pippenger": (V, [u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
+        "orc_booth_index": (ctypes.c_int32, [ctypes.c_size_t, ctypes.c_size_t, u8p]),
+        "orc_srs_powers": (V, [u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
+        "orc_g1_fixed_base_batch": (V, [u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
+        "orc_best_fft": (V, [u64p, u64p, ctypes.c_uint32, ctypes.c_int]),
+        "orc_omega": (V, [u64p, ctypes.c_uint32]),
+        "orc_fft": (V, [u64p, ctypes.c_uint32, ctypes.c_int]),
+        "orc_ifft": (V, [u64p, ctypes.c_uint32, ctypes.c_int]),
+        "orc_mul_zeta": (V, [u64p, ctypes.c_size_t, u64p, u64p]),
+        "orc_mul_repeating": (V, [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t]),
+        "orc_mul_scalar": (V, [u64p, ctypes.c_size_t, u64p]),
+        "orc_domain_new": (V, [ctypes.POINTER(Domain), u64p, ctypes.c_uint32, ctypes.c_uint32]),
+        "orc_coeff_to_extended": (V, [u64p, u64p, ctypes.POINTER(Domain), ctypes.c_int]),
+        "orc_extended_to_coeff": (V, [u64p, ctypes.POINTER(Domain), ctypes.c_int]),
+        "orc_rng_seed_from_u64": (V, [ctypes.POINTER(Rng), ctypes.c_uint64]),
+        "orc_rng_next_u64": (ctypes.c_uint64, [ctypes.POINTER(Rng)]),
+        "orc_fr_random_many": (V, [u64p, ctypes.c_size_t, ctypes.POINTER(Rng)]),
+        "orc_chacha20_block": (V, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                   ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def P(a):
+    """uint64 pointer to a C-contiguous numpy array"""
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(u64p)
+
+
+def B(a):
+    assert a.dtype == np.uint8 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(u8p)
+
+
+def fr(n=None):
+    return np.zeros((4,) if n is None else (n, 4), dtype=np.uint64)
+
+
+def g1a(n=None):
+    return np.zeros((12,) if n is None else (n, 12), dtype=np.uint64)
+
+
+def g1j(n=None):
+    return np.zeros((18,) if n is None else (n, 18), dtype=np.uint64)
+
+
+# ---- convenience wrappers -------------------------------------------------
+
+def fr_from_int(v):
+    limbs = np.array([(v >> (64 * i)) & (2**64 - 1) for i in range(4)], dtype=np.uint64)
+    out = fr()
+    if not lib().orc_fr_from_canonical(P(out), P(limbs)):
+        raise ValueError("not reduced")
+    return out
+
+
+def fr_to_int(a):
+    out = fr()
+    lib().orc_fr_to_canonical(P(out), P(np.ascontiguousarray(a, dtype=np.uint64)))
+    return sum(int(out[i]) << (64 * i) for i in range(4))
+
+
+def fr_op(name, *args):
+    out = fr()
+    getattr(lib(), "orc_fr_" + name)(P(out), *[P(np.ascontiguousarray(a)) for a in args])
+    return out
+
+
+def fp_op(name, *args):
+    out = np.zeros(6, dtype=np.uint64)
+    getattr(lib(), "orc_fp_" + name)(P(out), *[P(np.ascontiguousarray(a)) for a in args])
+    return out
+
+
+def generator():
+    g = g1a()
+    lib().orc_g1_generator(P(g))
+    return g
+
+
+def to_affine(j):
+    j = np.ascontiguousarray(j, dtype=np.uint64)
+    if j.ndim == 1:
+        out = g1a()
+        lib().orc_g1j_to_affine(P(out), P(j))
+        return out
+    out = g1a(j.shape[0])
+    lib().orc_g1_batch_to_affine(P(out), P(j), j.shape[0])
+    return out
+
+
+def g1_mul(p_aff, s):
+    out = g1j()
+    lib().orc_g1_mul(P(out), P(np.ascontiguousarray(p_aff)), P(np.ascontiguousarray(s)))
+    return out
+
+
+def random_fr(n, seed):
+    rng = Rng()
+    lib().orc_rng_seed_from_u64(ctypes.byref(rng), seed)
+    out = fr(n)
+    if n:
+        lib().orc_fr_random_many(P(out), n, ctypes.byref(rng))
+    return out
+
+
+def msm(points, scalars, threads=None, naive=False):
+    points = np.ascontiguousarray(points, dtype=np.uint64)
+    scalars = np.ascontiguousarray(scalars, dtype=np.uint64)
+    n = points.shape[0]
+    assert scalars.shape[0] == n
+    out = g1j()
+    if naive:
+        lib().orc_msm_naive(P(out), P(points), P(scalars), n)
+    else:
+        lib().orc_msm_pippenger(P(out), P(points), P(scalars), n, threads or os.cpu_count())
+    return out
+
+
+def fixed_base(ts, threads=None):
+    ts = np.ascontiguousarray(ts, dtype=np.uint64)
+    out = g1a(ts.shape[0])
+    lib().orc_g1_fixed_base_batch(P(out), P(ts), ts.shape[0], threads or os.cpu_count())
+    return out
+
+
+def srs(tau, n, threads=None):
+    out = g1a(n)
+    lib().orc_srs_powers(P(out), P(np.ascontiguousarray(tau)), n, threads or os.cpu_count())
+    return out
+
+
+def omega(log_n):
+    out = fr()
+    lib().orc_omega(P(out), log_n)
+    return out
+
+
+def fft(a, log_n, threads=None):
+    a = np.array(a, dtype=np.uint64, order="C", copy=True)
+    lib().orc_fft(P(a), log_n, threads or os.cpu_count())
+    return a
+
+
+def ifft(a, log_n, threads=None):
+    a = np.array(a, dtype=np.uint64, order="C", copy=True)
+    lib().orc_ifft(P(a), log_n, threads or os.cpu_count())
+    return a
+
+
+def best_fft(a, w, log_n, threads=None):
+    a = np.array(a, dtype=np.uint64, order="C", copy=True)
+    lib().orc_best_fft(P(a), P(np.ascontiguousarray(w)), log_n, threads or os.cpu_count())
+    return a
+
+
+def domain(j, k):
+    d = Domain()
+    ek = k
+    while (1 << ek) < (1 << k) * (j - 1):
+        ek += 1
+    t = fr(1 << (ek - k))
+    lib().orc_domain_new(ctypes.byref(d), P(t), j, k)
+    return d, t
+
+
+def mul_zeta(a, first, second):
+    a = np.array(a, dtype=np.uint64, order="C", copy=True)
+    lib().orc_mul_zeta(P(a), a.shape[0], P(np.ascontiguousarray(first)), P(np.ascontiguousarray(second)))
+    return a
+
+
+def mul_repeating(a, factors):
+    a = np.array(a, dtype=np.uint64, order="C", copy=True)
+    f = np.ascontiguousarray(factors, dtype=np.uint64)
+    lib().orc_mul_repeating(P(a), a.shape[0], P(f), f.shape[0])
+    return a
+
+
+def mul_scalar(a, s):
+    a = np.array(a, dtype=np.uint64, order="C", copy=True)
+    lib().orc_mul_scalar(P(a), a.shape[0], P(np.ascontiguousarray(s)))
+    return a
+
+
+def coeff_to_extended(coeffs, d, threads=None):
+    n = 1 << d.k
+    N = 1 << d.extended_k
+    assert coeffs.shape[0] == n
+    out = fr(N)
+    lib().orc_coeff_to_extended(P(out), P(np.ascontiguousarray(coeffs)), ctypes.byref(d), threads or os.cpu_count())
+    return out
+
+
+def extended_to_coeff(values, d, threads=None):
+    a = np.array(values, dtype=np.uint64, order="C", copy=True)
+    lib().orc_extended_to_coeff(P(a), ctypes.byref(d), threads or os.cpu_count())
+    return a
+
+
+def compress(p_aff):
+    out = np.zeros(48, dtype=np.uint8)
+    lib().orc_g1_compress(B(out), P(np.ascontiguousarray(p_aff)))
+    return bytes(out)
+
+
+def from_uncompressed(buf):
+    b = np.frombuffer(bytes(buf), dtype=np.uint8).copy()
+    out = g1a()
+    ok = lib().orc_g1_from_uncompressed(P(out), B(b))
+    return out if ok else None
+
+
+def load_srs_file(path):
+    """Parse a bls_midnight_2pK file (params.rs:170-262, RawBytes format):
+    u32 k || 2^k uncompressed G1 g || 2^k uncompressed G1 g_lagrange || G2 || sG2.
+    Returns (k, g, g_lagrange) as affine Montgomery arrays."""
+    with open(path, "rb") as f:
+        data = f.read()
+    k = int.from_bytes(data[:4], "little")
+    n = 1 << k
+    body = np.frombuffer(data, dtype=np.uint8, count=2 * n * 96, offset=4).reshape(2 * n, 96)
+    pts = g1a(2 * n)
+    for i in range(2 * n):
+        row = np.ascontiguousarray(body[i])
+        if not lib().orc_g1_from_uncompressed(P(pts[i]), B(row)):
+            raise ValueError("invalid point %d" % i)
+    return k, pts[:n].copy(), pts[n:].copy()
