@@ -1,0 +1,237 @@
+/*
+ * midnight_hip.h -- C ABI of the MI355X (gfx950) backend for the Midnight
+ * prover hot path: BLS12-381 G1 MSM, Fr NTT / coset transforms, and the
+ * quotient numerator on the extended domain.
+ *
+ * Each `midnight_sppark_*` entry point keeps the name, argument order,
+ * value layout and error convention of the reference CUDA backend
+ * (midnight-sppark-msm/src/bls12_381_g1_msm.cu) so that the Rust FFI block in
+ * midnight-sppark-msm/src/lib.rs:763-1362 links against this library
+ * unchanged.  Entry points prefixed `midnight_hip_` have no counterpart in
+ * the reference; each says who it is for.
+ *
+ * Conventions (bls12_381_g1_msm.cu:336-349, lib.rs:17-36):
+ *   RawFr            4 x u64 LE limbs, Montgomery form, R = 2^256
+ *   RawG1Affine      x[6], y[6] Montgomery (R = 2^384); identity = all zero
+ *   RawG1Projective  Jacobian x[6], y[6], z[6]; identity has z = 0.  Callers
+ *                    compare results after normalisation.
+ *   RustError        returned by value; code 0 = success, otherwise a
+ *                    hipError_t value or -1; message is malloc'd (the Rust
+ *                    side frees it with libc free), NULL on success
+ *                    (vendor/sppark-0.1.15/sppark/util/rusterror.h).
+ *   Host pointers are caller-owned and used synchronously.  `void*` handles
+ *   (MSM contexts, device vectors, numerator contexts) are freed with the
+ *   matching *_free function.
+ *   Lanes: the lane is thread-local (bls12_381_g1_msm.cu:48); all work of
+ *   one lane is ordered on that lane's stream.
+ */
+#ifndef MIDNIGHT_HIP_H
+#define MIDNIGHT_HIP_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct RustError {
+    int code;
+    char *message;
+} RustError;
+
+typedef struct RawFr {
+    uint64_t limbs[4];
+} RawFr;
+
+typedef struct RawG1Affine {
+    uint64_t x[6];
+    uint64_t y[6];
+} RawG1Affine;
+
+typedef struct RawG1Projective {
+    uint64_t x[6];
+    uint64_t y[6];
+    uint64_t z[6];
+} RawG1Projective;
+
+/* bls12_381_g1_msm.cu:50-61.  window_bits 0 = automatic; the other knobs
+ * are validated as the reference validates them (:85-110) and mapped onto
+ * this backend's own tuning (window, engines). */
+typedef struct RawMsmConfig {
+    uint32_t domain_k;
+    uint32_t min_msm_scalars;
+    uint32_t unfiltered_min_scalars;
+    uint32_t window_bits;
+    uint32_t sort_grid_divisor;
+    uint32_t sort_window_lanes;
+    uint32_t integration_blocks;
+    uint32_t batch_lanes;
+    uint32_t reuse_event;
+    uint32_t device_reduce;
+} RawMsmConfig;
+
+/* bls12_381_g1_msm.cu:63-68 */
+typedef struct RawMsmSlice {
+    size_t scalar_offset;
+    size_t point_offset;
+    size_t len;
+    size_t nonzero_len;
+} RawMsmSlice;
+
+/* ---------------- lanes ---------------- */
+
+/* replaces bls12_381_g1_msm.cu:134 midnight_sppark_execution_lane_set */
+RustError midnight_sppark_execution_lane_set(uint32_t lane);
+
+/* ---------------- G1 MSM ---------------- */
+
+/* replaces bls12_381_g1_msm.cu:6164 */
+RustError midnight_sppark_bls12_381_g1_msm(RawG1Projective *out, const RawG1Affine *points,
+                                           const RawFr *scalars, size_t len);
+/* replaces bls12_381_g1_msm.cu:6202 */
+RustError midnight_sppark_bls12_381_g1_msm_batch(RawG1Projective *out, const RawG1Affine *points,
+                                                 const RawFr *scalars, size_t npoints, size_t nmsms);
+/* replaces bls12_381_g1_msm.cu:6247 */
+RustError midnight_sppark_bls12_381_g1_msm_context_create_with_config(void **out,
+                                                                      const RawG1Affine *points,
+                                                                      size_t npoints,
+                                                                      RawMsmConfig config);
+/* replaces bls12_381_g1_msm.cu:6271 */
+RustError midnight_sppark_bls12_381_g1_msm_context_create(void **out, const RawG1Affine *points,
+                                                          size_t npoints);
+/* replaces bls12_381_g1_msm.cu:6283 */
+void midnight_sppark_bls12_381_g1_msm_context_free(void *context);
+/* replaces bls12_381_g1_msm.cu:6288 */
+RustError midnight_sppark_bls12_381_g1_msm_context_batch(RawG1Projective *out, void *context,
+                                                         const RawFr *scalars, size_t npoints,
+                                                         size_t nmsms);
+/* replaces bls12_381_g1_msm.cu:6310 */
+RustError midnight_sppark_bls12_381_g1_msm_context_device_batch(RawG1Projective *out,
+                                                                void *context,
+                                                                const void *device_vec,
+                                                                size_t scalar_offset,
+                                                                size_t npoints, size_t nmsms);
+/* replaces bls12_381_g1_msm.cu:6346 */
+RustError midnight_sppark_bls12_381_g1_msm_context_device_ragged(RawG1Projective *out,
+                                                                 void *context,
+                                                                 const void *device_vec,
+                                                                 const RawMsmSlice *slices,
+                                                                 size_t count);
+/* replaces bls12_381_g1_msm.cu:6426 */
+RustError midnight_sppark_bls12_381_g1_msm_context_device_scrambled_batch(
+    RawG1Projective *out, void *context, const void *device_vec, size_t scalar_offset,
+    size_t npoints, size_t nmsms, const void *mask_device_vec);
+
+/* ---------------- Fr NTT / coset ---------------- */
+
+/* replaces bls12_381_g1_msm.cu:6488.  In-place natural-order NTT of 2^log_n
+ * values; forward uses omega = ROOT_OF_UNITY^(2^(32-log_n)), inverse uses
+ * omega^-1 and includes the 1/n factor. */
+RustError midnight_sppark_bls12_381_fr_ntt(RawFr *values, uint32_t log_n, bool inverse);
+/* replaces bls12_381_g1_msm.cu:3447: NTT then v[i] *= scale * {1,first,second}[i % 3] */
+RustError midnight_sppark_bls12_381_fr_ntt_zeta(RawFr *values, uint32_t log_n, bool inverse,
+                                                RawFr scale, RawFr first, RawFr second);
+/* replaces bls12_381_g1_msm.cu:3497: v[i] *= factors[i % factor_len], inverse NTT,
+ * then the ntt_zeta post-processing */
+RustError midnight_sppark_bls12_381_fr_divide_ntt_zeta(RawFr *values, uint32_t log_n,
+                                                       const RawFr *factors, size_t factor_len,
+                                                       RawFr scale, RawFr first, RawFr second);
+/* replaces bls12_381_g1_msm.cu:2542: per poly, a[i] *= {1,first,second}[i % 3],
+ * zero-pad to 2^extended_log_n, forward NTT */
+RustError midnight_sppark_bls12_381_fr_coeff_to_extended_batch(RawFr *out, const RawFr *inputs,
+                                                               size_t input_len, size_t count,
+                                                               uint32_t extended_log_n,
+                                                               RawFr first, RawFr second);
+/* replaces bls12_381_g1_msm.cu:2604 (result stays on device) */
+RustError midnight_sppark_bls12_381_fr_device_coeff_to_extended_batch(
+    void **out, const RawFr *inputs, size_t input_len, size_t count, uint32_t extended_log_n,
+    RawFr first, RawFr second);
+/* replaces bls12_381_g1_msm.cu:2675 */
+RustError midnight_sppark_bls12_381_fr_device_coeff_to_extended_from_device_batch(
+    void **out, const void *source, size_t input_len, size_t count, uint32_t extended_log_n,
+    RawFr first, RawFr second);
+/* replaces bls12_381_g1_msm.cu:3281 */
+RustError midnight_sppark_bls12_381_fr_device_vec_ntt_zeta(void *device_vec, uint32_t log_n,
+                                                           bool inverse, RawFr scale, RawFr first,
+                                                           RawFr second);
+/* replaces bls12_381_g1_msm.cu:3331 */
+RustError midnight_sppark_bls12_381_fr_device_vec_ntt_zeta_batch(void *device_vec,
+                                                                 uint32_t log_n, size_t count,
+                                                                 bool inverse, RawFr scale,
+                                                                 RawFr first, RawFr second);
+/* replaces bls12_381_g1_msm.cu:3385 */
+RustError midnight_sppark_bls12_381_fr_device_vec_divide_ntt_zeta(void *device_vec,
+                                                                  uint32_t log_n,
+                                                                  const RawFr *factors,
+                                                                  size_t factor_len, RawFr scale,
+                                                                  RawFr first, RawFr second);
+
+/* ---------------- Fr vector ops ---------------- */
+
+/* replaces bls12_381_g1_msm.cu:6504 */
+RustError midnight_sppark_bls12_381_fr_mul_assign(RawFr *values, RawFr scalar, size_t len);
+/* replaces bls12_381_g1_msm.cu:6521 */
+RustError midnight_sppark_bls12_381_fr_mul_assign_zeta(RawFr *values, RawFr first, RawFr second,
+                                                       size_t len);
+/* replaces bls12_381_g1_msm.cu:6544 */
+RustError midnight_sppark_bls12_381_fr_mul_assign_repeating(RawFr *values, const RawFr *factors,
+                                                            size_t factor_len, size_t len);
+
+/* ---------------- device vectors ---------------- */
+
+/* replaces bls12_381_g1_msm.cu:2741 */
+RustError midnight_sppark_bls12_381_fr_device_vec_upload(void **out, const RawFr *values,
+                                                         size_t len);
+/* replaces bls12_381_g1_msm.cu:2774 */
+RustError midnight_sppark_bls12_381_fr_device_vec_upload_many(void **out,
+                                                              const RawFr *const *values,
+                                                              const size_t *lengths, size_t count);
+/* replaces bls12_381_g1_msm.cu:3074 */
+RustError midnight_sppark_bls12_381_fr_device_vec_download(RawFr *out, const void *device_vec,
+                                                           size_t len);
+/* replaces bls12_381_g1_msm.cu:3442 */
+void midnight_sppark_bls12_381_fr_device_vec_free(void *device_vec);
+/* replaces bls12_381_g1_msm.cu:3181: split the quotient into piece_count
+ * limbs of poly_len - 1 coefficients padded to poly_len, then add the
+ * blinding telescope (piece i gets +blind_i at index poly_len - 1 and piece
+ * i + 1 gets -blind_i at index 0) */
+RustError midnight_sppark_bls12_381_fr_device_quotient_pieces(void **out,
+                                                              const void *quotient_vec,
+                                                              size_t poly_len, size_t piece_count,
+                                                              const RawFr *blinds,
+                                                              size_t blind_count);
+
+/* ---------------- backend extensions (no reference counterpart) ---------------- */
+
+/* Number of elements of a device vector (for callers and tests that hold
+ * only the opaque handle). */
+size_t midnight_hip_fr_device_vec_len(const void *device_vec);
+
+/* Batched ragged MSM over one resident point set: MSM j uses points
+ * [0, sizes[j]) and scalars at scalar_offsets[j] of the device vector.  All
+ * MSMs of the batch share each stage's launches.  For batch provers that
+ * commit many polynomials of different sizes (many circuits of different k
+ * in one launch). */
+RustError midnight_hip_g1_msm_context_device_multi(RawG1Projective *out, void *context,
+                                                   const void *device_vec,
+                                                   const size_t *scalar_offsets,
+                                                   const size_t *sizes, size_t count);
+
+/* Batched ragged in-place inverse/forward NTTs over one device vector:
+ * transform j covers 2^log_ns[j] values at offsets[j].  For batch provers
+ * (many circuits of different k). */
+RustError midnight_hip_fr_device_vec_ntt_multi(void *device_vec, const size_t *offsets,
+                                               const uint32_t *log_ns, size_t count,
+                                               bool inverse);
+
+/* Measured throughput of the integer multiply-add the field code is built
+ * on (v_mad_u64_u32), in operations per second over the whole device, and
+ * the device clock/CU count used; for roofline reporting. */
+RustError midnight_hip_valu_peak(double *mad_u64_per_s, double *clock_hz, int *num_cus);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

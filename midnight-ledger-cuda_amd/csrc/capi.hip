@@ -1,0 +1,552 @@
+// C ABI: lanes, device vectors, NTT / coset and Fr vector entry points.
+// Argument checks, return codes and messages follow the reference entry
+// points cited at each function (midnight-sppark-msm/src/bls12_381_g1_msm.cu).
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "devvec.hpp"
+#include "frops.hpp"
+#include "ntt.hpp"
+#include "runtime.hpp"
+
+namespace mh {
+
+RustError make_error(int code, const char *msg)
+{
+    RustError e;
+    e.code = code == 0 ? -1 : code;
+    e.message = msg ? strdup(msg) : nullptr;
+    return e;
+}
+
+namespace {
+thread_local uint32_t t_lane = 0;
+hipStream_t g_streams[64][kMaxLanes + 1];
+std::mutex g_stream_mu;
+}  // namespace
+
+uint32_t current_lane() { return t_lane; }
+void set_current_lane(uint32_t lane) { t_lane = lane; }
+
+int device_id()
+{
+    int d = 0;
+    MH_HIP(hipGetDevice(&d));
+    return d;
+}
+
+hipStream_t lane_stream(uint32_t lane)
+{
+    int dev = device_id();
+    std::lock_guard<std::mutex> g(g_stream_mu);
+    hipStream_t &s = g_streams[dev][lane];
+    if (!s) MH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+// per-lane reusable staging/scratch buffers for the host-bridge entry points
+namespace {
+struct LaneScratch {
+    std::mutex mu;
+    DevBuf<Fr> a, b, c;
+};
+LaneScratch g_scratch[64][kMaxLanes + 1];
+}  // namespace
+
+LaneScratch &lane_scratch() { return g_scratch[device_id()][current_lane()]; }
+
+Fr to_fr(const RawFr &r)
+{
+    Fr f;
+    std::memcpy(f.v, r.limbs, 32);
+    return f;
+}
+
+HostFr to_hfr(const RawFr &r)
+{
+    HostFr h;
+    std::memcpy(h.l, r.limbs, 32);
+    return h;
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+#define MH_TRY try {
+#define MH_CATCH                                                     \
+    }                                                                \
+    catch (const hip_error &e) { return make_error(e.code, e.what()); } \
+    catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
+
+extern "C" RustError midnight_sppark_execution_lane_set(uint32_t lane)
+{
+    if (lane > kMaxLanes) return make_error((int)hipErrorInvalidValue, "invalid sppark proof execution lane");
+    MH_TRY
+    set_current_lane(lane);
+    (void)lane_stream(lane);
+    return ok();
+    MH_CATCH
+}
+
+// ---------------------------------------------------------------- device vectors
+
+extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_upload(void **out, const RawFr *values, size_t len)
+{
+    if (!out) return make_error((int)hipErrorInvalidValue, "out is null");
+    *out = nullptr;
+    if (len && !values) return make_error((int)hipErrorInvalidValue, "values is null");
+    MH_TRY
+    auto dv = std::make_unique<DeviceVec>();
+    dv->lane = current_lane();
+    dv->len = len;
+    dv->values.alloc(len);
+    if (len) {
+        hipStream_t st = cur_stream();
+        MH_HIP(hipMemcpyAsync(dv->values.get(), values, len * 32, hipMemcpyHostToDevice, st));
+        MH_HIP(hipStreamSynchronize(st));
+    }
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_upload_many(void **out, const RawFr *const *values,
+                                                                         const size_t *lengths, size_t count)
+{
+    if (!out || (count && (!values || !lengths)))
+        return make_error((int)hipErrorInvalidValue, "invalid multi-slice upload arguments");
+    *out = nullptr;
+    size_t total = 0;
+    for (size_t i = 0; i < count; ++i) {
+        if ((lengths[i] && !values[i]) || lengths[i] > SIZE_MAX - total)
+            return make_error((int)hipErrorInvalidValue, "invalid multi-slice upload source");
+        total += lengths[i];
+    }
+    MH_TRY
+    auto dv = std::make_unique<DeviceVec>();
+    dv->lane = current_lane();
+    dv->len = total;
+    dv->values.alloc(total);
+    hipStream_t st = cur_stream();
+    size_t off = 0;
+    for (size_t i = 0; i < count; ++i) {
+        if (!lengths[i]) continue;
+        MH_HIP(hipMemcpyAsync(dv->values.get() + off, values[i], lengths[i] * 32, hipMemcpyHostToDevice, st));
+        off += lengths[i];
+    }
+    MH_HIP(hipStreamSynchronize(st));
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_download(RawFr *out, const void *device_vec, size_t len)
+{
+    if (!device_vec || (len && !out))
+        return make_error((int)hipErrorInvalidValue, "invalid device vector download arguments");
+    const DeviceVec *dv = static_cast<const DeviceVec *>(device_vec);
+    if (dv->len != len) return make_error((int)hipErrorInvalidValue, "device vector length mismatch");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    if (len) MH_HIP(hipMemcpyAsync(out, dv->values.get(), len * 32, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    return ok();
+    MH_CATCH
+}
+
+extern "C" void midnight_sppark_bls12_381_fr_device_vec_free(void *device_vec)
+{
+    DeviceVec *dv = static_cast<DeviceVec *>(device_vec);
+    if (!dv) return;
+    // pending work of the owning lane may still read the buffer
+    try {
+        (void)hipStreamSynchronize(lane_stream(dv->lane));
+    } catch (...) {
+    }
+    delete dv;
+}
+
+extern "C" size_t midnight_hip_fr_device_vec_len(const void *device_vec)
+{
+    return device_vec ? static_cast<const DeviceVec *>(device_vec)->len : 0;
+}
+
+extern "C" RustError midnight_sppark_bls12_381_fr_device_quotient_pieces(void **out, const void *quotient_vec,
+                                                                         size_t poly_len, size_t piece_count,
+                                                                         const RawFr *blinds, size_t blind_count)
+{
+    if (!out || !quotient_vec || poly_len < 2 || piece_count < 2 || blind_count != piece_count - 1 || !blinds ||
+        piece_count > SIZE_MAX / poly_len)
+        return make_error((int)hipErrorInvalidValue, "invalid resident quotient piece arguments");
+    *out = nullptr;
+    const DeviceVec *q = static_cast<const DeviceVec *>(quotient_vec);
+    if (q->len < (poly_len - 1) * piece_count)
+        return make_error((int)hipErrorInvalidValue, "resident quotient is too short");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<Fr> d_blinds(blind_count);
+    MH_HIP(hipMemcpyAsync(d_blinds.get(), blinds, blind_count * 32, hipMemcpyHostToDevice, st));
+    auto dv = std::make_unique<DeviceVec>();
+    dv->lane = current_lane();
+    dv->len = poly_len * piece_count;
+    dv->values.alloc(dv->len);
+    fr_quotient_pieces(dv->values.get(), q->values.get(), poly_len, piece_count, d_blinds.get(), st);
+    MH_HIP(hipStreamSynchronize(st));
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+// ---------------------------------------------------------------- NTT family
+
+namespace {
+
+// Host-bridge NTT: upload, transform in place on the lane's stream, download.
+RustError host_ntt(RawFr *values, uint32_t log_n, NttJob job, const RawFr *factors, size_t factor_len)
+{
+    const size_t n = size_t(1) << log_n;
+    LaneScratch &ls = lane_scratch();
+    std::lock_guard<std::mutex> g(ls.mu);
+    hipStream_t st = cur_stream();
+    ls.a.ensure(n);
+    ls.b.ensure(ntt_scratch_elems(log_n, 1));
+    MH_HIP(hipMemcpyAsync(ls.a.get(), values, n * 32, hipMemcpyHostToDevice, st));
+    DevBuf<Fr> d_factors;
+    if (factors) {
+        d_factors.alloc(factor_len);
+        MH_HIP(hipMemcpyAsync(d_factors.get(), factors, factor_len * 32, hipMemcpyHostToDevice, st));
+        job.factors = d_factors.get();
+        job.factor_len = factor_len;
+        job.pre = 2;
+    }
+    job.in = ls.a.get();
+    job.out = ls.a.get();
+    job.scratch = ls.b.get();
+    job.in_pstride = job.out_pstride = n;
+    job.log_n = log_n;
+    job.count = 1;
+    ntt_run(job, st);
+    MH_HIP(hipMemcpyAsync(values, ls.a.get(), n * 32, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    return ok();
+}
+
+}  // namespace
+
+extern "C" RustError midnight_sppark_bls12_381_fr_ntt(RawFr *values, uint32_t log_n, bool inverse)
+{
+    if (!values && log_n) return make_error((int)hipErrorInvalidValue, "values is null");
+    MH_TRY
+    NttJob job;
+    job.inverse = inverse;
+    return host_ntt(values, log_n, job, nullptr, 0);
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_fr_ntt_zeta(RawFr *values, uint32_t log_n, bool inverse, RawFr scale,
+                                                           RawFr first, RawFr second)
+{
+    if (log_n == 0) return ok();
+    MH_TRY
+    NttJob job;
+    job.inverse = inverse;
+    job.post_zeta = true;
+    job.scale = to_hfr(scale);
+    job.first = to_hfr(first);
+    job.second = to_hfr(second);
+    return host_ntt(values, log_n, job, nullptr, 0);
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_fr_divide_ntt_zeta(RawFr *values, uint32_t log_n, const RawFr *factors,
+                                                                  size_t factor_len, RawFr scale, RawFr first,
+                                                                  RawFr second)
+{
+    if (log_n == 0 || factor_len == 0) return ok();
+    MH_TRY
+    NttJob job;
+    job.inverse = true;
+    job.post_zeta = true;
+    job.scale = to_hfr(scale);
+    job.first = to_hfr(first);
+    job.second = to_hfr(second);
+    return host_ntt(values, log_n, job, factors, factor_len);
+    MH_CATCH
+}
+
+namespace {
+
+// coeff_to_extended into a fresh device buffer (count polys of 2^ext_log_n)
+void coeff_to_extended_device(Fr *dst, const Fr *src, size_t input_len, size_t count, uint32_t ext_log_n,
+                              const RawFr &first, const RawFr &second, hipStream_t st)
+{
+    DevBuf<Fr> scratch(ntt_scratch_elems(ext_log_n, count));
+    NttJob job;
+    job.in = src;
+    job.out = dst;
+    job.scratch = scratch.get();
+    job.in_pstride = input_len;
+    job.out_pstride = size_t(1) << ext_log_n;
+    job.log_n = ext_log_n;
+    job.count = (uint32_t)count;
+    job.pre = 1;
+    job.in_len = input_len;
+    job.first = to_hfr(first);
+    job.second = to_hfr(second);
+    ntt_run(job, st);
+    MH_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+extern "C" RustError midnight_sppark_bls12_381_fr_coeff_to_extended_batch(RawFr *out, const RawFr *inputs,
+                                                                          size_t input_len, size_t count,
+                                                                          uint32_t extended_log_n, RawFr first,
+                                                                          RawFr second)
+{
+    if (input_len == 0 || count == 0) return ok();
+    size_t output_len = size_t(1) << extended_log_n;
+    if (input_len > output_len) return make_error(-1, "input length exceeds extended domain length");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<Fr> d_in(input_len * count), d_out(output_len * count);
+    MH_HIP(hipMemcpyAsync(d_in.get(), inputs, input_len * count * 32, hipMemcpyHostToDevice, st));
+    coeff_to_extended_device(d_out.get(), d_in.get(), input_len, count, extended_log_n, first, second, st);
+    MH_HIP(hipMemcpyAsync(out, d_out.get(), output_len * count * 32, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_fr_device_coeff_to_extended_batch(void **out, const RawFr *inputs,
+                                                                                 size_t input_len, size_t count,
+                                                                                 uint32_t extended_log_n,
+                                                                                 RawFr first, RawFr second)
+{
+    if (!out) return make_error((int)hipErrorInvalidValue, "out is null");
+    *out = nullptr;
+    if (input_len == 0 || count == 0 || !inputs)
+        return make_error((int)hipErrorInvalidValue, "invalid coefficient batch");
+    size_t output_len = size_t(1) << extended_log_n;
+    if (input_len > output_len)
+        return make_error((int)hipErrorInvalidValue, "input length exceeds extended domain length");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<Fr> d_in(input_len * count);
+    MH_HIP(hipMemcpyAsync(d_in.get(), inputs, input_len * count * 32, hipMemcpyHostToDevice, st));
+    auto dv = std::make_unique<DeviceVec>();
+    dv->lane = current_lane();
+    dv->len = output_len * count;
+    dv->values.alloc(dv->len);
+    coeff_to_extended_device(dv->values.get(), d_in.get(), input_len, count, extended_log_n, first, second, st);
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_fr_device_coeff_to_extended_from_device_batch(
+    void **out, const void *source, size_t input_len, size_t count, uint32_t extended_log_n, RawFr first,
+    RawFr second)
+{
+    if (!out || !source) return make_error((int)hipErrorInvalidValue, "invalid resident coefficient source");
+    *out = nullptr;
+    const DeviceVec *src = static_cast<const DeviceVec *>(source);
+    if (input_len == 0 || count == 0 || src->len != input_len * count)
+        return make_error((int)hipErrorInvalidValue, "resident coefficient source length mismatch");
+    size_t output_len = size_t(1) << extended_log_n;
+    if (input_len > output_len || count > SIZE_MAX / output_len)
+        return make_error((int)hipErrorInvalidValue, "invalid resident extended-domain size");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    auto dv = std::make_unique<DeviceVec>();
+    dv->lane = current_lane();
+    dv->len = output_len * count;
+    dv->values.alloc(dv->len);
+    coeff_to_extended_device(dv->values.get(), src->values.get(), input_len, count, extended_log_n, first, second,
+                             st);
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+namespace {
+
+void device_ntt_zeta(DeviceVec *dv, uint32_t log_n, size_t count, bool inverse, const RawFr &scale,
+                     const RawFr &first, const RawFr &second, const Fr *factors, size_t factor_len)
+{
+    hipStream_t st = cur_stream();
+    const size_t n = size_t(1) << log_n;
+    DevBuf<Fr> scratch(ntt_scratch_elems(log_n, count));
+    NttJob job;
+    job.in = dv->values.get();
+    job.out = dv->values.get();
+    job.scratch = scratch.get();
+    job.in_pstride = job.out_pstride = n;
+    job.log_n = log_n;
+    job.count = (uint32_t)count;
+    job.inverse = inverse;
+    job.post_zeta = true;
+    job.scale = to_hfr(scale);
+    job.first = to_hfr(first);
+    job.second = to_hfr(second);
+    if (factors) {
+        job.pre = 2;
+        job.factors = factors;
+        job.factor_len = factor_len;
+    }
+    ntt_run(job, st);
+    // the scratch buffer is released on return: wait for the passes
+    MH_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_ntt_zeta(void *device_vec, uint32_t log_n, bool inverse,
+                                                                      RawFr scale, RawFr first, RawFr second)
+{
+    if (!device_vec) return make_error((int)hipErrorInvalidValue, "device vector is null");
+    DeviceVec *dv = static_cast<DeviceVec *>(device_vec);
+    if (log_n >= 63 || dv->len != (size_t(1) << log_n))
+        return make_error((int)hipErrorInvalidValue, "device vector length mismatch");
+    if (log_n == 0) return ok();
+    MH_TRY
+    device_ntt_zeta(dv, log_n, 1, inverse, scale, first, second, nullptr, 0);
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_ntt_zeta_batch(void *device_vec, uint32_t log_n,
+                                                                            size_t count, bool inverse, RawFr scale,
+                                                                            RawFr first, RawFr second)
+{
+    if (!device_vec || count == 0) return make_error((int)hipErrorInvalidValue, "invalid resident NTT batch arguments");
+    DeviceVec *dv = static_cast<DeviceVec *>(device_vec);
+    if (log_n >= 63) return make_error((int)hipErrorInvalidValue, "resident NTT batch size overflow");
+    const size_t len = size_t(1) << log_n;
+    if (count > SIZE_MAX / len) return make_error((int)hipErrorInvalidValue, "resident NTT batch size overflow");
+    if (dv->len != len * count) return make_error((int)hipErrorInvalidValue, "resident NTT batch length mismatch");
+    MH_TRY
+    if (log_n == 0) {
+        // sppark's length-1 NTT is the identity; only the post-processing applies
+        hipStream_t st = cur_stream();
+        fr_mul_scalar(dv->values.get(), to_fr(scale), count, st);
+        MH_HIP(hipStreamSynchronize(st));
+        return ok();
+    }
+    device_ntt_zeta(dv, log_n, count, inverse, scale, first, second, nullptr, 0);
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_divide_ntt_zeta(void *device_vec, uint32_t log_n,
+                                                                             const RawFr *factors, size_t factor_len,
+                                                                             RawFr scale, RawFr first, RawFr second)
+{
+    if (!device_vec || !factors || factor_len == 0)
+        return make_error((int)hipErrorInvalidValue, "invalid resident quotient arguments");
+    DeviceVec *dv = static_cast<DeviceVec *>(device_vec);
+    if (log_n >= 63 || dv->len != (size_t(1) << log_n))
+        return make_error((int)hipErrorInvalidValue, "device vector length mismatch");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<Fr> d_factors(factor_len);
+    MH_HIP(hipMemcpyAsync(d_factors.get(), factors, factor_len * 32, hipMemcpyHostToDevice, st));
+    device_ntt_zeta(dv, log_n, 1, true, scale, first, second, d_factors.get(), factor_len);
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_hip_fr_device_vec_ntt_multi(void *device_vec, const size_t *offsets,
+                                                          const uint32_t *log_ns, size_t count, bool inverse)
+{
+    if (!device_vec || (count && (!offsets || !log_ns)))
+        return make_error((int)hipErrorInvalidValue, "invalid batched NTT arguments");
+    DeviceVec *dv = static_cast<DeviceVec *>(device_vec);
+    for (size_t i = 0; i < count; ++i) {
+        if (log_ns[i] > 22 || offsets[i] > dv->len || (size_t(1) << log_ns[i]) > dv->len - offsets[i])
+            return make_error((int)hipErrorInvalidValue, "batched NTT slice exceeds the device vector");
+    }
+    MH_TRY
+    hipStream_t st = cur_stream();
+    // group transforms by size; polys of one size that sit back to back share launches
+    size_t max_scratch = 0;
+    for (size_t i = 0; i < count; ++i) max_scratch = std::max(max_scratch, ntt_scratch_elems(log_ns[i], 1));
+    std::vector<size_t> order(count);
+    for (size_t i = 0; i < count; ++i) order[i] = i;
+    DevBuf<Fr> scratch(max_scratch * 8);
+    size_t i = 0;
+    while (i < count) {
+        // run of consecutive, contiguous, equal-size transforms (up to 8 per launch set)
+        size_t j = i + 1;
+        const size_t n = size_t(1) << log_ns[i];
+        while (j < count && j - i < 8 && log_ns[j] == log_ns[i] && offsets[j] == offsets[j - 1] + n) ++j;
+        NttJob job;
+        job.in = dv->values.get() + offsets[i];
+        job.out = dv->values.get() + offsets[i];
+        job.scratch = scratch.get();
+        job.in_pstride = job.out_pstride = n;
+        job.log_n = log_ns[i];
+        job.count = (uint32_t)(j - i);
+        job.inverse = inverse;
+        ntt_run(job, st);
+        i = j;
+    }
+    MH_HIP(hipStreamSynchronize(st));
+    return ok();
+    MH_CATCH
+}
+
+// ---------------------------------------------------------------- Fr vector ops
+
+namespace {
+
+template <class F>
+RustError host_vector_op(RawFr *values, size_t len, F launch)
+{
+    if (len == 0) return ok();
+    LaneScratch &ls = lane_scratch();
+    std::lock_guard<std::mutex> g(ls.mu);
+    hipStream_t st = cur_stream();
+    ls.a.ensure(len);
+    MH_HIP(hipMemcpyAsync(ls.a.get(), values, len * 32, hipMemcpyHostToDevice, st));
+    launch(ls.a.get(), st);
+    MH_HIP(hipMemcpyAsync(values, ls.a.get(), len * 32, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    return ok();
+}
+
+}  // namespace
+
+extern "C" RustError midnight_sppark_bls12_381_fr_mul_assign(RawFr *values, RawFr scalar, size_t len)
+{
+    MH_TRY
+    Fr s = to_fr(scalar);
+    return host_vector_op(values, len, [&](Fr *d, hipStream_t st) { fr_mul_scalar(d, s, len, st); });
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_fr_mul_assign_zeta(RawFr *values, RawFr first, RawFr second,
+                                                                  size_t len)
+{
+    MH_TRY
+    Fr a = to_fr(first), b = to_fr(second);
+    return host_vector_op(values, len, [&](Fr *d, hipStream_t st) { fr_mul_zeta(d, a, b, len, st); });
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_fr_mul_assign_repeating(RawFr *values, const RawFr *factors,
+                                                                       size_t factor_len, size_t len)
+{
+    if (len == 0 || factor_len == 0) return ok();
+    MH_TRY
+    hipStream_t st0 = cur_stream();
+    DevBuf<Fr> d_f(factor_len);
+    MH_HIP(hipMemcpyAsync(d_f.get(), factors, factor_len * 32, hipMemcpyHostToDevice, st0));
+    return host_vector_op(values, len,
+                          [&](Fr *d, hipStream_t st) { fr_mul_repeating(d, d_f.get(), factor_len, len, st); });
+    MH_CATCH
+}

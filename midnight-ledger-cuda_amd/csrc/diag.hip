@@ -1,0 +1,131 @@
+// Device measurements used for roofline reporting: the sustained rate of
+// v_mad_u64_u32 (the 32x32+64 multiply-add every field product here is
+// built from), and the rate of whole Fp / Fr Montgomery products.
+#include <chrono>
+#include <cstring>
+
+#include "fields.hpp"
+#include "runtime.hpp"
+
+namespace mh {
+
+namespace {
+
+// Eight independent multiply-add chains per thread, kept in registers; the
+// result is written so the compiler keeps every chain.
+__global__ void __launch_bounds__(256) mad_u64_loop(uint64_t *out, uint32_t iters, uint32_t seed)
+{
+    uint32_t a = seed + threadIdx.x, b = seed ^ (blockIdx.x * 2654435761u);
+    uint64_t acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = (uint64_t)(a + k) << 17;
+    for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc[k] = (uint64_t)(uint32_t)acc[k] * b + acc[k];
+        }
+    }
+    uint64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s ^= acc[k];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+template <class C>
+__global__ void __launch_bounds__(256) field_mul_loop(uint32_t *out, uint32_t iters)
+{
+    Fe<C> a = Fe<C>::one(), b = Fe<C>::one();
+    a.v[0] ^= threadIdx.x;
+    b.v[1] ^= blockIdx.x;
+    for (uint32_t i = 0; i < iters; ++i) {
+        a = mul<C>(a, b);
+        b = mul<C>(b, a);
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < C::N; ++k) s ^= a.v[k] ^ b.v[k];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+double time_launch(void (*fn)(hipStream_t), hipStream_t st)
+{
+    hipEvent_t e0, e1;
+    MH_HIP(hipEventCreate(&e0));
+    MH_HIP(hipEventCreate(&e1));
+    fn(st);   // warm-up
+    MH_HIP(hipEventRecord(e0, st));
+    fn(st);
+    MH_HIP(hipEventRecord(e1, st));
+    MH_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms * 1e-3;
+}
+
+uint64_t *g_sink = nullptr;
+int g_blocks = 0;
+constexpr uint32_t kMadIters = 4096;
+constexpr uint32_t kMulIters = 64;
+
+void run_mad(hipStream_t st)
+{
+    hipLaunchKernelGGL(mad_u64_loop, dim3(g_blocks), dim3(256), 0, st, g_sink, kMadIters, 12345u);
+}
+void run_fp(hipStream_t st)
+{
+    hipLaunchKernelGGL(field_mul_loop<FpCfg>, dim3(g_blocks), dim3(g_blocks == 1 ? 64 : 256), 0, st,
+                       reinterpret_cast<uint32_t *>(g_sink), kMulIters);
+}
+void run_fr(hipStream_t st)
+{
+    hipLaunchKernelGGL(field_mul_loop<FrCfg>, dim3(g_blocks), dim3(g_blocks == 1 ? 64 : 256), 0, st,
+                       reinterpret_cast<uint32_t *>(g_sink), kMulIters);
+}
+
+}  // namespace
+
+}  // namespace mh
+
+extern "C" RustError midnight_hip_valu_peak(double *mad_u64_per_s, double *clock_hz, int *num_cus)
+{
+    using namespace mh;
+    try {
+        hipDeviceProp_t prop;
+        MH_HIP(hipGetDeviceProperties(&prop, device_id()));
+        g_blocks = prop.multiProcessorCount * 8;
+        if (!g_sink) MH_HIP(hipMalloc(reinterpret_cast<void **>(&g_sink), (size_t)g_blocks * 256 * 8));
+        hipStream_t st = cur_stream();
+        double t = time_launch(run_mad, st);
+        double ops = (double)g_blocks * 256 * kMadIters * 16 * 8;
+        if (mad_u64_per_s) *mad_u64_per_s = ops / t;
+        if (clock_hz) *clock_hz = (double)prop.clockRate * 1e3;
+        if (num_cus) *num_cus = prop.multiProcessorCount;
+        return ok();
+    } catch (const hip_error &e) {
+        return make_error(e.code, e.what());
+    }
+}
+
+// Whole-product throughput (products per second) for Fp and Fr; with
+// blocks = 1 (one wave) it is the latency of a dependent product chain.
+extern "C" RustError midnight_hip_field_mul_rate(int blocks, double *fp_per_s, double *fr_per_s)
+{
+    using namespace mh;
+    try {
+        hipDeviceProp_t prop;
+        MH_HIP(hipGetDeviceProperties(&prop, device_id()));
+        if (!g_sink) MH_HIP(hipMalloc(reinterpret_cast<void **>(&g_sink), (size_t)prop.multiProcessorCount * 8 * 256 * 8));
+        g_blocks = blocks > 0 ? blocks : prop.multiProcessorCount * 8;
+        if (!g_sink) MH_HIP(hipMalloc(reinterpret_cast<void **>(&g_sink), (size_t)g_blocks * 256 * 8));
+        hipStream_t st = cur_stream();
+        double per = (double)g_blocks * (g_blocks == 1 ? 64 : 256) * kMulIters * 2;
+        if (fp_per_s) *fp_per_s = per / time_launch(run_fp, st);
+        if (fr_per_s) *fr_per_s = per / time_launch(run_fr, st);
+        return ok();
+    } catch (const hip_error &e) {
+        return make_error(e.code, e.what());
+    }
+}

@@ -1,0 +1,305 @@
+// Fr NTT for gfx950: natural order in, natural order out.
+//
+// A transform of n = 2^log_n values is done as one or two LDS passes.
+//   log_n <= 11: one pass; a workgroup holds the whole polynomial in LDS.
+//   log_n <= 22: the four-step split n = n1 * n2 (n1 = 2^ceil(log_n/2)):
+//     pass A, per column j2 (stride-n2 elements): an n1-point NTT, then the
+//             element at row k1 is multiplied by omega_n^(j2*k1);
+//     pass B, per row k1 (contiguous elements): an n2-point NTT whose output
+//             k2 lands at index k1 + n1*k2.
+//   X[k1 + n1 k2] = sum_j2 w_n2^(j2 k2) w_n^(j2 k1) sum_j1 x[j1 n2 + j2] w_n1^(j1 k1)
+//   is the DFT, so the result equals the reference's bit_rev + CT NTT
+//   (vendor/sppark-0.1.15/sppark/ntt/ntt.cuh:217-243) and best_fft
+//   (vendor/midnight-curves-0.3.1/src/fft.rs:30-112) limb for limb.
+// Inside a pass, a workgroup loads C columns of m = 2^t elements into LDS in
+// bit-reversed order, runs t radix-2 DIT stages with one barrier per stage,
+// and writes the C columns back; C is chosen so C*m*32 B <= 64 KiB.  Loads
+// and stores walk the unit-stride dimension fastest, so every wave touches
+// whole runs of C (pass A) or m (pass B) consecutive elements.
+//
+// Pre- and post-operations of the reference's fused entry points are folded
+// into the first load and the last store of each transform:
+//   pre  zeta: a_i *= {1, first, second}[i % 3], zero beyond input_len
+//        (fr_coeff_to_extended_prepare_kernel, bls12_381_g1_msm.cu:1648-1678)
+//   pre  repeat: a_i *= factors[i % factor_len]
+//        (fr_mul_assign_repeating_kernel, :1213-1223)
+//   post scale(+zeta): a_i *= scale * {1, first, second}[i % 3]
+//        (fr_ntt_zeta_postprocess_kernel, :1680-1700)
+#include <mutex>
+#include <vector>
+
+#include "fields.hpp"
+#include "host_fr.hpp"
+#include "ntt.hpp"
+#include "runtime.hpp"
+
+namespace mh {
+
+namespace {
+
+constexpr int kMaxLogLds = 11;     // largest single-pass sub-transform
+constexpr int kTwLog = 22;         // twiddle tables are built for 2^22
+constexpr int kThreads = 256;
+
+struct Tables {
+    Fr *roots[2];   // omega_{2^11}^j, j < 1024
+    Fr *lo[2];      // omega_{2^22}^i, i < 2048
+    Fr *hi[2];      // omega_{2^22}^(2048 i), i < 2048
+};
+
+Tables g_tables[64];
+std::once_flag g_tables_once[64];
+
+void upload(Fr **dst, const std::vector<HostFr> &v)
+{
+    MH_HIP(hipMalloc(reinterpret_cast<void **>(dst), v.size() * 32));
+    MH_HIP(hipMemcpy(*dst, v.data(), v.size() * 32, hipMemcpyHostToDevice));
+}
+
+const Tables &tables()
+{
+    int dev = device_id();
+    std::call_once(g_tables_once[dev], [dev]() {
+        Tables &t = g_tables[dev];
+        for (int dir = 0; dir < 2; ++dir) {
+            HostFr w22 = HostFr::omega(kTwLog);
+            if (dir) w22 = w22.inverse();
+            HostFr w11 = w22;
+            for (int i = kMaxLogLds; i < kTwLog; ++i) w11 = w11.sqr();
+            std::vector<HostFr> roots(1024), lo(2048), hi(2048);
+            HostFr acc = HostFr::one();
+            for (int i = 0; i < 1024; ++i) { roots[i] = acc; acc = acc * w11; }
+            acc = HostFr::one();
+            for (int i = 0; i < 2048; ++i) { lo[i] = acc; acc = acc * w22; }
+            HostFr step = acc;   // w22^2048
+            acc = HostFr::one();
+            for (int i = 0; i < 2048; ++i) { hi[i] = acc; acc = acc * step; }
+            upload(&t.roots[dir], roots);
+            upload(&t.lo[dir], lo);
+            upload(&t.hi[dir], hi);
+        }
+    });
+    return g_tables[dev];
+}
+
+struct PassArgs {
+    const Fr *in;
+    Fr *out;
+    uint64_t in_pstride, out_pstride;   // per polynomial (blockIdx.y)
+    uint32_t in_es, in_cs;              // logical index = j*es + c*cs
+    uint32_t out_es, out_cs;
+    uint32_t log_m, C;
+    uint32_t log_n;                     // whole transform (mid twiddle)
+    uint32_t twiddle;                   // multiply (c, k) by w_n^(c*k)
+    uint32_t pre;                       // 0 none, 1 zeta + zero pad, 2 repeating factors
+    uint32_t post;                      // 0 none, 1 scale, 2 scale * zeta
+    uint64_t in_len;                    // logical indices >= in_len read as zero
+    const Fr *factors;
+    uint64_t factor_len;
+    Fr first, second, scale;
+    const Fr *roots, *lo, *hi;
+};
+
+__device__ __forceinline__ uint32_t bitrev(uint32_t x, uint32_t bits)
+{
+    return __builtin_bitreverse32(x) >> (32 - bits);
+}
+
+__device__ __forceinline__ Fr zeta_factor(const Fr &v, uint64_t idx, const Fr &first, const Fr &second)
+{
+    uint32_t c = (uint32_t)(idx % 3);
+    if (c == 1) return v * first;
+    if (c == 2) return v * second;
+    return v;
+}
+
+__global__ void __launch_bounds__(kThreads) ntt_pass(PassArgs a)
+{
+    extern __shared__ Fr lds[];
+    const uint32_t m = 1u << a.log_m;
+    const uint32_t C = a.C;
+    const uint32_t total = C * m;
+    const uint32_t c0 = blockIdx.x * C;
+    const Fr *in = a.in + (uint64_t)blockIdx.y * a.in_pstride;
+    Fr *out = a.out + (uint64_t)blockIdx.y * a.out_pstride;
+
+    // load (bit-reversed within each column), applying the pre-operation
+    const bool load_cfast = (a.in_cs == 1) && C > 1;
+    for (uint32_t f = threadIdx.x; f < total; f += blockDim.x) {
+        uint32_t c, j;
+        if (load_cfast) { c = f % C; j = f / C; }
+        else { j = f % m; c = f / m; }
+        uint64_t L = (uint64_t)j * a.in_es + (uint64_t)(c0 + c) * a.in_cs;
+        Fr v;
+        if (L < a.in_len) {
+            v = load_fr(in + L);
+            if (a.pre == 1) v = zeta_factor(v, L, a.first, a.second);
+            else if (a.pre == 2) v = v * a.factors[L % a.factor_len];
+        } else {
+            v = Fr::zero();
+        }
+        lds[c * m + bitrev(j, a.log_m)] = v;
+    }
+    __syncthreads();
+
+    // radix-2 DIT stages
+    const uint32_t half = m >> 1;
+    for (uint32_t s = 0; s < a.log_m; ++s) {
+        const uint32_t h = 1u << s;
+        for (uint32_t b = threadIdx.x; b < C * half; b += blockDim.x) {
+            uint32_t c = b / half, bb = b % half;
+            uint32_t low = bb & (h - 1);
+            uint32_t i = ((bb >> s) << (s + 1)) | low;
+            Fr *col = lds + c * m;
+            Fr u = col[i];
+            Fr v = col[i + h];
+            if (low) v = v * a.roots[low << (kMaxLogLds - 1 - s)];
+            col[i] = u + v;
+            col[i + h] = u - v;
+        }
+        __syncthreads();
+    }
+
+    // store, applying the mid twiddle and the post-operation
+    const bool store_cfast = (a.out_cs == 1) && C > 1;
+    const uint32_t tw_shift = kTwLog - a.log_n;
+    for (uint32_t f = threadIdx.x; f < total; f += blockDim.x) {
+        uint32_t c, k;
+        if (store_cfast) { c = f % C; k = f / C; }
+        else { k = f % m; c = f / m; }
+        Fr v = lds[c * m + k];
+        if (a.twiddle) {
+            uint32_t e = ((c0 + c) * k) << tw_shift;
+            if (e) {
+                Fr w = a.lo[e & 2047];
+                if (e >> 11) w = w * a.hi[e >> 11];
+                v = v * w;
+            }
+        }
+        uint64_t L = (uint64_t)k * a.out_es + (uint64_t)(c0 + c) * a.out_cs;
+        if (a.post) {
+            v = v * a.scale;
+            if (a.post == 2) v = zeta_factor(v, L, a.first, a.second);
+        }
+        store_fr(out + L, v);
+    }
+}
+
+uint32_t pick_cols(uint32_t log_m, uint32_t ncols)
+{
+    uint32_t C = 1;
+    while (C * 2 <= ncols && ((C * 2) << log_m) <= 2048) C *= 2;
+    return C;
+}
+
+void launch(const PassArgs &a, uint32_t ncols, uint32_t npolys, hipStream_t st)
+{
+    dim3 grid(ncols / a.C, npolys);
+    size_t lds = (size_t)a.C * (1u << a.log_m) * sizeof(Fr);
+    hipLaunchKernelGGL(ntt_pass, grid, dim3(kThreads), lds, st, a);
+    MH_HIP(hipGetLastError());
+}
+
+Fr to_dev(const HostFr &h)
+{
+    Fr r;
+    std::memcpy(r.v, h.l, 32);
+    return r;
+}
+
+}  // namespace
+
+void ntt_run(const NttJob &job, hipStream_t st)
+{
+    const uint32_t log_n = job.log_n;
+    MH_CHECK(log_n <= 2 * kMaxLogLds, hipErrorInvalidValue, "NTT size above 2^22 is not supported");
+    const Tables &T = tables();
+    const int dir = job.inverse ? 1 : 0;
+    const uint64_t n = 1ull << log_n;
+
+    // post: inverse includes 1/n; ntt_zeta multiplies by scale and the zeta pattern
+    HostFr post_scale = HostFr::one();
+    bool has_post = false;
+    if (job.inverse) {
+        post_scale = HostFr::from_u64(n).inverse();
+        has_post = true;
+    }
+    if (job.post_zeta) {
+        post_scale = post_scale * job.scale;
+        has_post = true;
+    }
+
+    PassArgs base{};
+    base.roots = T.roots[dir];
+    base.lo = T.lo[dir];
+    base.hi = T.hi[dir];
+    base.log_n = log_n;
+    base.first = to_dev(job.first);
+    base.second = to_dev(job.second);
+    base.scale = to_dev(post_scale);
+    base.factors = job.factors;
+    base.factor_len = job.factor_len;
+
+    if (log_n == 0) {
+        // a length-1 transform is the identity; only pre/post factors apply
+    }
+
+    if (log_n <= kMaxLogLds) {
+        PassArgs a = base;
+        a.in = job.in;
+        a.out = job.out;
+        a.in_pstride = job.in_pstride;
+        a.out_pstride = job.out_pstride;
+        a.in_es = 1; a.in_cs = 0;
+        a.out_es = 1; a.out_cs = 0;
+        a.log_m = log_n;
+        a.C = 1;
+        a.twiddle = 0;
+        a.pre = job.pre;
+        a.in_len = job.in_len;
+        a.post = has_post ? (job.post_zeta ? 2 : 1) : 0;
+        launch(a, 1, job.count, st);
+        return;
+    }
+
+    const uint32_t t1 = (log_n + 1) / 2, t2 = log_n / 2;
+    const uint32_t n1 = 1u << t1, n2 = 1u << t2;
+    MH_CHECK(job.scratch != nullptr, hipErrorInvalidValue, "NTT scratch missing");
+
+    PassArgs A = base;
+    A.in = job.in;
+    A.out = job.scratch;
+    A.in_pstride = job.in_pstride;
+    A.out_pstride = n;
+    A.in_es = n2; A.in_cs = 1;
+    A.out_es = n2; A.out_cs = 1;
+    A.log_m = t1;
+    A.C = pick_cols(t1, n2);
+    A.twiddle = 1;
+    A.pre = job.pre;
+    A.in_len = job.in_len;
+    A.post = 0;
+    launch(A, n2, job.count, st);
+
+    PassArgs B = base;
+    B.in = job.scratch;
+    B.out = job.out;
+    B.in_pstride = n;
+    B.out_pstride = job.out_pstride;
+    B.in_es = 1; B.in_cs = n2;
+    B.out_es = n1; B.out_cs = 1;
+    B.log_m = t2;
+    B.C = pick_cols(t2, n1);
+    B.twiddle = 0;
+    B.pre = 0;
+    B.in_len = ~0ull;
+    B.post = has_post ? (job.post_zeta ? 2 : 1) : 0;
+    launch(B, n1, job.count, st);
+}
+
+size_t ntt_scratch_elems(uint32_t log_n, size_t count)
+{
+    return log_n <= kMaxLogLds ? 0 : (size_t(1) << log_n) * count;
+}
+
+}  // namespace mh

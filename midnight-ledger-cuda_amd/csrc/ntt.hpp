@@ -1,0 +1,33 @@
+// Host interface of the Fr NTT passes (ntt.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "fields.hpp"
+#include "host_fr.hpp"
+
+namespace mh {
+
+struct NttJob {
+    const Fr *in = nullptr;      // count polys, in_pstride apart
+    Fr *out = nullptr;           // count polys of 2^log_n, out_pstride apart (may equal in)
+    Fr *scratch = nullptr;       // ntt_scratch_elems(log_n, count) elements
+    uint64_t in_pstride = 0, out_pstride = 0;
+    uint32_t log_n = 0;
+    uint32_t count = 1;
+    bool inverse = false;        // inverse includes the 1/n factor
+    uint32_t pre = 0;            // 0 none, 1 zeta + zero pad past in_len, 2 repeating factors
+    uint64_t in_len = ~0ull;
+    const Fr *factors = nullptr;
+    uint64_t factor_len = 1;
+    bool post_zeta = false;      // multiply by scale * {1, first, second}[i % 3]
+    HostFr scale = HostFr::one();
+    HostFr first = HostFr::one(), second = HostFr::one();
+};
+
+void ntt_run(const NttJob &job, hipStream_t st);
+size_t ntt_scratch_elems(uint32_t log_n, size_t count);
+
+}  // namespace mh

@@ -1,0 +1,88 @@
+// Host runtime shared by the backend's translation units: error type that
+// becomes the reference's RustError at the C boundary, per-lane HIP
+// streams, and RAII device buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/midnight_hip.h"
+
+namespace mh {
+
+// A failed HIP call or a rejected argument.  `code` is the hipError_t value
+// (or -1 for argument errors), as the reference returns cudaError_t codes.
+struct hip_error : std::runtime_error {
+    int code;
+    hip_error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define MH_HIP(expr)                                                              \
+    do {                                                                          \
+        hipError_t mh_e_ = (expr);                                                \
+        if (mh_e_ != hipSuccess)                                                  \
+            throw ::mh::hip_error((int)mh_e_, std::string(#expr) + ": " +         \
+                                                  hipGetErrorString(mh_e_));      \
+    } while (0)
+
+#define MH_CHECK(cond, code, msg)                                                 \
+    do {                                                                          \
+        if (!(cond)) throw ::mh::hip_error((code), (msg));                        \
+    } while (0)
+
+RustError make_error(int code, const char *msg);
+inline RustError ok() { return RustError{0, nullptr}; }
+
+// Proof execution lanes (the reference keeps one private device context per
+// lane, bls12_381_g1_msm.cu:46-146).  Here a lane is one HIP stream; every
+// producer and consumer of a lane's device vectors runs on that stream, so
+// stream order is the only ordering the lane needs.
+constexpr uint32_t kMaxLanes = 8;
+uint32_t current_lane();
+void set_current_lane(uint32_t lane);
+hipStream_t lane_stream(uint32_t lane);
+inline hipStream_t cur_stream() { return lane_stream(current_lane()); }
+int device_id();
+
+// Owning device allocation.
+template <class T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t len = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), len(o.len) { o.ptr = nullptr; o.len = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) { release(); ptr = o.ptr; len = o.len; o.ptr = nullptr; o.len = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t n)
+    {
+        release();
+        if (n) MH_HIP(hipMalloc(reinterpret_cast<void **>(&ptr), n * sizeof(T)));
+        len = n;
+    }
+    // grow-only reuse for scratch buffers
+    void ensure(size_t n)
+    {
+        if (n > len) alloc(n);
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        len = 0;
+    }
+    T *get() const { return ptr; }
+};
+
+inline size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }
+
+}  // namespace mh
