@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark of the BLS12-381 prover hot path on one MI355X.
+
+Default workload: G1 MSM with 2^20 points (headline metric "G1 MSM Mpts/s
+@2^20").  A step is one call of the resident-context batch entry point
+(midnight_sppark_bls12_381_g1_msm_context_device_batch) over
+--msms-per-step MSMs of 2^20 uniform 255-bit scalars against one resident
+point set [t_i]G.  Points and scalars are synthetic, generated on the device
+before timing, so the timed region starts with every input in HBM; each
+call returns its Jacobian results to the host before the next starts.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "midnight-ledger-cuda_amd"))
+
+R_MOD = 0x73eda753299d7d483339d80809a1d80553bda402fffe5bfeffffffff00000001
+R_LIMBS = np.array([(R_MOD >> (64 * i)) & (2**64 - 1) for i in range(4)], dtype=np.uint64)
+
+# Integer-VALU peak: v_mad_u64_u32 issues at 64 lanes/clk/CU (full rate,
+# confirmed by midnight_hip_valu_peak), 256 CUs, 2.4 GHz.
+PEAK_MAC_PER_S = 256 * 64 * 2.4e9
+HBM_PEAK = 8.0e12
+# algorithmic Montgomery-product cost of one Fp product (12 limbs, CIOS)
+MACS_PER_FP_MUL = 300
+# SURVEY 8(d) algorithmic MSM work: min over c of N*ceil(256/c)*(8M+2S) + ceil(256/c)*2^(c-1)*2*(12M+2S)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def msm_algorithmic_macs(n):
+    best = None
+    for c in range(1, 25):
+        w = -(-256 // c)
+        fp = n * w * 10 + w * (1 << (c - 1)) * 2 * 14
+        best = fp if best is None or fp < best else best
+    return best * MACS_PER_FP_MUL
+
+
+def random_canonical(n, rng):
+    """uniform values in [0, r): 255-bit draws, rejection of >= r"""
+    out = np.empty((n, 4), dtype=np.uint64)
+    filled = 0
+    while filled < n:
+        m = int((n - filled) * 1.1) + 16
+        x = rng.integers(0, 2**64, size=(m, 4), dtype=np.uint64, endpoint=False)
+        x[:, 3] &= np.uint64((1 << 63) - 1)
+        # lexicographic x < r on limbs 3..0
+        lt = np.zeros(m, dtype=bool)
+        eq = np.ones(m, dtype=bool)
+        for i in (3, 2, 1, 0):
+            lt |= eq & (x[:, i] < R_LIMBS[i])
+            eq &= x[:, i] == R_LIMBS[i]
+        x = x[lt]
+        take = min(n - filled, x.shape[0])
+        out[filled:filled + take] = x[:take]
+        filled += take
+    return out
+
+
+def skew(vals, rng):
+    """70% in [0, 2^16), 20% in {0, 1}, 10% unchanged (uniform)"""
+    u = rng.random(vals.shape[0])
+    small = u < 0.7
+    zo = (u >= 0.7) & (u < 0.9)
+    vals[small] = 0
+    vals[small, 0] = rng.integers(0, 1 << 16, size=int(small.sum()), dtype=np.uint64)
+    vals[zo] = 0
+    vals[zo, 0] = rng.integers(0, 2, size=int(zo.sum()), dtype=np.uint64)
+    return vals
+
+
+def device_fr(H, canonical):
+    dv = H.DeviceVec.upload(canonical)
+    H.to_montgomery(dv)
+    return dv
+
+
+def cpu_baseline_msm(H, ctx, dv_scalars, log_n, seconds):
+    """Oracle Pippenger (multithreaded C restatement of msm_best) on a
+    bounded sample of the same workload: the first 2^sample_log points and
+    scalars, repeated for about `seconds`."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    sample_log = min(log_n, 18)
+    n = 1 << sample_log
+    pts = ctx.points(n)
+    sc = dv_scalars.download()[:n]
+    threads = os.cpu_count() or 1
+    oracle.msm(pts[:1024], sc[:1024], threads=threads)     # warm-up (table, threads)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.msm(pts, sc, threads=threads)
+        reps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(n * reps / dt / 1e6, 4), "unit": "Mpts/s", "cores": threads, "kind": "port",
+            "sample": "%d x MSM of 2^%d points (first 2^%d of the workload), oracle Pippenger, %d threads, %.1f s"
+                      % (reps, sample_log, sample_log, threads, dt)}
+
+
+def bench_msm(args, H):
+    log_n = args.log_n
+    n = 1 << log_n
+    m = args.msms_per_step
+    rng = np.random.default_rng(args.seed)
+    log("generating %d points and %d x %d scalars on the device" % (n, m, n))
+    t_dv = device_fr(H, random_canonical(n, rng))
+    ctx = H.MsmContext.from_multiples(t_dv)
+    t_dv.free()
+    sc = random_canonical(n * m, rng)
+    if args.skewed:
+        sc = skew(sc, rng)
+    dv = device_fr(H, sc)
+    del sc
+    for _ in range(args.warmup):
+        ctx.device_batch(dv, 0, n, m)
+    H.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.device_batch(dv, 0, n, m)
+    dt = time.perf_counter() - t0
+    acc_ms, acc_cnt = H.profile_read("msm_accumulate")
+    tot_ms, tot_cnt = H.profile_read("msm_total")
+    H.profile_enable(False)
+    entries, window = ctx.last_stats()
+    ms_step = dt / args.steps * 1e3
+    value = n * m * args.steps / dt / 1e6
+    # dominant kernel: bucket accumulation, 8M+2S Fp products per entry
+    acc_avg_s = acc_ms / max(acc_cnt, 1) * 1e-3
+    acc_macs = entries * 10 * MACS_PER_FP_MUL
+    achieved = acc_macs / acc_avg_s if acc_avg_s > 0 else 0.0
+    msm_macs = msm_algorithmic_macs(n) * m
+    res = {
+        "metric": "G1 MSM Mpts/s @2^%d" % log_n,
+        "value": round(value, 3),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic: points [t_i]G and %s scalars, generated on device" % ("skewed" if args.skewed else "uniform"),
+        "config": {"workload": "G1 MSM, 2^%d resident points, %d MSMs per step (%s scalars)"
+                               % (log_n, m, "skewed" if args.skewed else "uniform 255-bit"),
+                   "points": n, "msms_per_step": m, "window_bits": window},
+        "roofline": {"bound": "valu", "kernel": "msm_accumulate",
+                     "achieved": round(achieved / 1e12, 3), "peak": round(PEAK_MAC_PER_S / 1e12, 3),
+                     "unit": "TMAC/s", "frac": round(achieved / PEAK_MAC_PER_S, 4), "traffic": None},
+        "msm_valu_frac": round(msm_macs / (dt / args.steps) / PEAK_MAC_PER_S, 4),
+        "accumulate_ms_avg": round(acc_avg_s * 1e3, 3),
+        "engine_ms_avg": round(tot_ms / max(tot_cnt, 1), 3),
+    }
+    if args.cpu_seconds > 0:
+        log("timing the CPU baseline")
+        res["cpu_baseline"] = cpu_baseline_msm(H, ctx, dv, log_n, args.cpu_seconds)
+    ctx.free()
+    dv.free()
+    return res
+
+
+def bench_ntt(args, H):
+    log_n = args.log_n
+    n = 1 << log_n
+    cnt = args.msms_per_step
+    rng = np.random.default_rng(args.seed)
+    dv = device_fr(H, random_canonical(n * cnt, rng))
+    one = np.array([0x00000001fffffffe, 0x5884b7fa00034802, 0x998c4fefecbc4ff5, 0x1824b159acc5056f], dtype=np.uint64)
+    for _ in range(args.warmup):
+        dv.ntt_zeta_batch(log_n, cnt, False, one, one, one)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        dv.ntt_zeta_batch(log_n, cnt, bool(i & 1), one, one, one)
+    dt = time.perf_counter() - t0
+    ms_step = dt / args.steps * 1e3
+    bytes_alg = 64 * n * cnt
+    res = {
+        "metric": "Fr NTT 2^%d elements/s" % log_n, "value": round(n * cnt * args.steps / dt / 1e6, 3),
+        "unit": "Melem/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic uniform Fr",
+        "config": {"workload": "resident Fr NTT 2^%d, %d polys per step" % (log_n, cnt)},
+        "roofline": {"bound": "hbm", "achieved": round(bytes_alg / (dt / args.steps) / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(bytes_alg / (dt / args.steps) / HBM_PEAK, 4), "traffic": None},
+    }
+    dv.free()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="msm", choices=["msm", "ntt"])
+    ap.add_argument("--log-n", type=int, default=20)
+    ap.add_argument("--msms-per-step", type=int, default=4)
+    ap.add_argument("--skewed", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="CPU-baseline sample length (0 disables)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("RANK", "0")) != 0:
+        return
+    import midnight_hip as H
+    H.lib()
+    res = bench_msm(args, H) if args.workload == "msm" else bench_ntt(args, H)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -226,10 +226,36 @@ RustError midnight_hip_fr_device_vec_ntt_multi(void *device_vec, const size_t *o
                                                const uint32_t *log_ns, size_t count,
                                                bool inverse);
 
+/* In place: canonical Fr values -> Montgomery form (a -> a*R mod r).  For
+ * callers that hold canonical scalars (synthetic workloads, wire formats). */
+RustError midnight_hip_fr_device_vec_to_montgomery(void *device_vec);
+
+/* MSM context over the points [t_i]G, where t_i are the (Montgomery) values
+ * of a resident Fr vector, computed on the device by a fixed-base comb.
+ * For SRS construction ([tau^i]G) and synthetic point sets, without a
+ * host round trip of the points. */
+RustError midnight_hip_g1_msm_context_create_from_multiples(void **out, const void *scalar_device_vec);
+/* Copy the first n context points (RawG1Affine) to the host. */
+RustError midnight_hip_g1_msm_context_points_download(RawG1Affine *out, const void *context, size_t n);
+/* Statistics of the context's last MSM batch: bucket additions performed
+ * (nonzero signed digits) and the window width used.  For throughput and
+ * roofline reporting. */
+RustError midnight_hip_g1_msm_context_last_stats(const void *context, uint64_t *entries,
+                                                 uint32_t *window_bits);
+
 /* Measured throughput of the integer multiply-add the field code is built
  * on (v_mad_u64_u32), in operations per second over the whole device, and
  * the device clock/CU count used; for roofline reporting. */
 RustError midnight_hip_valu_peak(double *mad_u64_per_s, double *clock_hz, int *num_cus);
+/* Whole Fp / Fr Montgomery products per second (blocks = 0: whole device;
+ * blocks = 1: one wave, i.e. the latency of a dependent chain). */
+RustError midnight_hip_field_mul_rate(int blocks, double *fp_per_s, double *fr_per_s);
+/* Per-kernel timing with HIP events on the launching stream (off by
+ * default).  enable(1) clears the registry; read() returns 1 and the total
+ * milliseconds and count for a region name ("msm_accumulate", "msm_total",
+ * ...), 0 if the region has not run. */
+void midnight_hip_profile_enable(int on);
+int midnight_hip_profile_read(const char *name, double *total_ms, uint64_t *count);
 
 #ifdef __cplusplus
 }

@@ -161,6 +161,34 @@ __device__ __forceinline__ Jacobian xyzz_to_jacobian(const Xyzz &p)
     return j;
 }
 
+// a^(p-2) = a^-1 (Fermat); 0 maps to 0
+__device__ __forceinline__ Fp fp_inverse(const Fp &a)
+{
+    // p - 2, 32-bit limbs, little-endian
+    constexpr uint32_t E[12] = {0xffffaaa9u, 0xb9feffffu, 0xb153ffffu, 0x1eabfffeu, 0xf6b0f624u, 0x6730d2a0u,
+                                0xf38512bfu, 0x64774b84u, 0x434bacd7u, 0x4b1ba7b6u, 0x397fe69au, 0x1a0111eau};
+    Fp r = Fp::one();
+    for (int i = 380; i >= 0; --i) {
+        r = sqr<FpCfg>(r);
+        if ((E[i >> 5] >> (i & 31)) & 1u) r = r * a;
+    }
+    return r;
+}
+
+__device__ __forceinline__ Affine xyzz_to_affine(const Xyzz &p)
+{
+    Affine a;
+    if (p.is_inf()) {
+        a.x = Fp::zero();
+        a.y = Fp::zero();
+        return a;
+    }
+    Fp inv = fp_inverse(p.zz * p.zzz);
+    a.x = p.x * (inv * p.zzz);   // X / ZZ
+    a.y = p.y * (inv * p.zz);    // Y / ZZZ
+    return a;
+}
+
 __device__ __forceinline__ Affine load_affine(const uint32_t *p)
 {
     Affine a;

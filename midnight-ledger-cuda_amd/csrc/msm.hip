@@ -1,2 +1,893 @@
-// G1 MSM (filled in by the Pippenger engine).
+// BLS12-381 G1 multi-scalar multiplication for gfx950 (signed-window
+// Pippenger), batched: one pass of every stage serves all MSMs of a batch.
+//
+// Stages (all on the lane's stream):
+//  1. digits   one thread per (msm, point): Montgomery -> canonical scalar,
+//              signed c-bit digits d_w in [-2^(c-1), 2^(c-1)] with carry,
+//              histogram of (msm, window, |d|-1) keys by atomics.
+//  2. scan     exclusive prefix sums of the histogram -> bucket offsets.
+//  3. scatter  digits again; each nonzero digit claims a slot of its bucket
+//              (point index | sign << 31).  Order inside a bucket does not
+//              matter: the bucket sum is the same group element.
+//  4. chunks   every bucket is cut into chunks of at most L entries; chunk
+//              offsets by a second scan; a chunk->bucket map.
+//  5. accumulate  one lane per chunk: serial XYZZ mixed additions over the
+//              chunk's points (8M+2S each), the next point's 96-byte row
+//              prefetched while the current one is added.  Single-chunk
+//              buckets are written straight to the bucket array.
+//  6. bucket sums  one lane per multi-chunk bucket adds its chunk partials.
+//  7. bit planes  S_w = sum_i (i+1) B_i = sum_k 2^k Q_k with
+//              Q_k = sum of the buckets whose weight has bit k set.  Each Q_k
+//              is a plain sum, reduced by two wide GPU passes (groups of G1,
+//              then G2), so no GPU lane runs a long serial chain.
+//  8. host     the (msm, window, plane) partials come back as Jacobian
+//              points; the host sums each plane and runs one doubling chain
+//              over the exponents c*w + k (Horner), which is faster on a CPU
+//              core than on a single GPU lane.
+// The result is the unique group element sum s_i P_i, returned in the
+// reference's Jacobian layout (bls12_381_g1_msm.cu:6164-6486 entry points).
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "curve.hpp"
+#include "devvec.hpp"
+#include "frops.hpp"
+#include "host_g1.hpp"
+#include "msm.hpp"
+#include "msm_context.hpp"
+#include "profile.hpp"
 #include "runtime.hpp"
+
+namespace mh {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kSignBit = 0x80000000u;
+
+struct MsmDesc {
+    const Fr *scalars;   // device
+    uint32_t pt_off;     // first point of this MSM in the point table
+    uint32_t n;          // number of (point, scalar) pairs
+};
+
+struct Geometry {
+    uint32_t c, nwin, B, M;
+    uint64_t keys() const { return (uint64_t)M * nwin * B; }
+};
+
+// signed digit of window w; `carry` flows from window w-1
+__device__ __forceinline__ int32_t window_digit(const uint32_t k[9], uint32_t w, uint32_t c, uint32_t &carry)
+{
+    uint32_t bit = w * c, q = bit >> 5, r = bit & 31;
+    uint64_t pair = (uint64_t)k[q] | ((uint64_t)k[q + 1] << 32);
+    uint32_t raw = (uint32_t)(pair >> r) & ((1u << c) - 1u);
+    raw += carry;
+    uint32_t half = 1u << (c - 1);
+    if (raw > half) {
+        carry = 1;
+        return (int32_t)raw - (int32_t)(1u << c);
+    }
+    carry = 0;
+    return (int32_t)raw;
+}
+
+__device__ __forceinline__ void canonical_scalar(const Fr *s, uint32_t k[9])
+{
+    Fr v = fr_from_mont(load_fr(s));
+#pragma unroll
+    for (int i = 0; i < 8; ++i) k[i] = v.v[i];
+    k[8] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_digits_hist(const MsmDesc *descs, Geometry g, uint32_t *cnt)
+{
+    const uint32_t m = blockIdx.y;
+    const MsmDesc d = descs[m];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    uint32_t k[9];
+    canonical_scalar(d.scalars + i, k);
+    uint32_t carry = 0;
+    const uint64_t base = (uint64_t)m * g.nwin * g.B;
+    for (uint32_t w = 0; w < g.nwin; ++w) {
+        int32_t dg = window_digit(k, w, g.c, carry);
+        if (dg == 0) continue;
+        uint32_t b = (uint32_t)(dg < 0 ? -dg : dg) - 1;
+        atomicAdd(&cnt[base + (uint64_t)w * g.B + b], 1u);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scatter(const MsmDesc *descs, Geometry g, uint32_t *cursor, uint32_t *ent)
+{
+    const uint32_t m = blockIdx.y;
+    const MsmDesc d = descs[m];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    uint32_t k[9];
+    canonical_scalar(d.scalars + i, k);
+    uint32_t carry = 0;
+    const uint64_t base = (uint64_t)m * g.nwin * g.B;
+    const uint32_t pidx = d.pt_off + i;
+    for (uint32_t w = 0; w < g.nwin; ++w) {
+        int32_t dg = window_digit(k, w, g.c, carry);
+        if (dg == 0) continue;
+        uint32_t b = (uint32_t)(dg < 0 ? -dg : dg) - 1;
+        uint32_t pos = atomicAdd(&cursor[base + (uint64_t)w * g.B + b], 1u);
+        ent[pos] = pidx | (dg < 0 ? kSignBit : 0u);
+    }
+}
+
+// ---------------------------------------------------------------- scan
+
+constexpr int kScanItems = 4;
+constexpr uint32_t kScanTile = kBlock * kScanItems;
+
+// exclusive scan of one tile per block; block totals to `sums`
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(const uint32_t *in, uint32_t *out, uint32_t *sums, uint64_t n)
+{
+    __shared__ uint32_t s[kBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems], t = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = base + j < n ? in[base + j] : 0u;
+        t += v[j];
+    }
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t off = 1; off < kBlock; off <<= 1) {
+        uint32_t add = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - t;   // exclusive prefix of this thread
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == kBlock - 1 && sums) sums[blockIdx.x] = s[kBlock - 1];
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_add(uint32_t *out, const uint32_t *sums, uint64_t n)
+{
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    const uint32_t add = sums[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j)
+        if (base + j < n) out[base + j] += add;
+}
+
+// ---------------------------------------------------------------- chunks
+
+__global__ void k_chunk_counts(const uint32_t *cnt, uint32_t *q, uint64_t nkeys, uint32_t L)
+{
+    uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (b < nkeys) q[b] = (cnt[b] + L - 1) / L;
+}
+
+__global__ void k_chunk_map(const uint32_t *q, const uint32_t *co, uint32_t *cmap, uint64_t nkeys)
+{
+    uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (b >= nkeys) return;
+    uint32_t n = q[b], o = co[b];
+    for (uint32_t j = 0; j < n; ++j) cmap[o + j] = (uint32_t)b;
+}
+
+__device__ __forceinline__ Affine fetch_point(const uint32_t *pts, uint32_t e)
+{
+    Affine p = load_affine(pts + (uint64_t)(e & ~kSignBit) * 24);
+    if (e & kSignBit) p = affine_neg(p);
+    return p;
+}
+
+__global__ void __launch_bounds__(kBlock) k_accumulate(const uint32_t *pts, const uint32_t *ent, const uint32_t *cnt,
+                                                       const uint32_t *off, const uint32_t *q, const uint32_t *co,
+                                                       const uint32_t *cmap, uint32_t nchunks, uint32_t L,
+                                                       uint32_t *part, uint32_t *bucket)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nchunks) return;
+    const uint32_t b = cmap[j];
+    const uint32_t r = j - co[b];
+    const uint32_t start = off[b] + r * L;
+    const uint32_t end = min(start + L, off[b] + cnt[b]);
+    Xyzz acc = Xyzz::inf();
+    uint32_t e = ent[start];
+    Affine nxt = fetch_point(pts, e);
+    for (uint32_t x = start; x < end; ++x) {
+        Affine cur = nxt;
+        if (x + 1 < end) nxt = fetch_point(pts, ent[x + 1]);
+        acc = xyzz_add_affine(acc, cur);
+    }
+    if (q[b] == 1) store_xyzz(bucket + (uint64_t)b * 48, acc);
+    else store_xyzz(part + (uint64_t)j * 48, acc);
+}
+
+// buckets finished by the accumulate pass (one chunk) need no more work
+__global__ void k_retire_single(uint32_t *q, uint64_t nkeys, uint32_t *maxq)
+{
+    uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (b >= nkeys) return;
+    uint32_t n = q[b];
+    if (n == 1) q[b] = n = 0;
+    if (n > 1) atomicMax(maxq, n);
+}
+
+__global__ void k_group_counts(const uint32_t *q, uint32_t *q2, uint64_t nkeys, uint32_t G, uint32_t *maxq)
+{
+    uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (b >= nkeys) return;
+    uint32_t n = (q[b] + G - 1) / G;
+    q2[b] = n;
+    if (n > 1) atomicMax(maxq, n);
+}
+
+// one lane per group of up to G partials of one bucket (a level of the tree
+// that keeps very full buckets from becoming one long serial chain)
+__global__ void __launch_bounds__(kBlock) k_reduce_groups(const uint32_t *q, const uint32_t *co, const uint32_t *co2,
+                                                          const uint32_t *cmap2, uint32_t n2, uint32_t G,
+                                                          const uint32_t *part, uint32_t *part2)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n2) return;
+    const uint32_t b = cmap2[j];
+    const uint32_t r = j - co2[b];
+    const uint32_t s0 = co[b] + r * G, s1 = min(s0 + G, co[b] + q[b]);
+    Xyzz acc = Xyzz::inf();
+    for (uint32_t x = s0; x < s1; ++x) acc = xyzz_add(acc, load_xyzz(part + (uint64_t)x * 48));
+    store_xyzz(part2 + (uint64_t)j * 48, acc);
+}
+
+__global__ void __launch_bounds__(kBlock) k_bucket_sums(const uint32_t *q, const uint32_t *co, const uint32_t *part,
+                                                        uint32_t *bucket, uint64_t nkeys)
+{
+    uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (b >= nkeys) return;
+    uint32_t n = q[b];
+    if (n == 0) return;   // empty, or finished by the accumulate pass
+    Xyzz acc = Xyzz::inf();
+    for (uint32_t j = 0; j < n; ++j) acc = xyzz_add(acc, load_xyzz(part + (uint64_t)(co[b] + j) * 48));
+    store_xyzz(bucket + b * 48, acc);
+}
+
+// ---------------------------------------------------------------- bit planes
+
+// Plane k of a window holds the buckets x-1 with bit k of x set, x in [1, B].
+// Its e-th member is x = ((e >> k) << (k + 1)) | (1 << k) | (e & (2^k - 1)).
+struct PlaneLayout {
+    uint32_t c;
+    uint32_t per_window;         // chunks per window
+    uint32_t first[17];          // first chunk of plane k within a window
+    uint32_t members[16];        // members of plane k
+};
+
+__global__ void __launch_bounds__(kBlock) k_plane_partials(const uint32_t *bucket, Geometry g, PlaneLayout pl,
+                                                           uint32_t G1, uint64_t total, uint32_t *pp)
+{
+    uint64_t id = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (id >= total) return;
+    uint64_t mw = id / pl.per_window;
+    uint32_t rem = (uint32_t)(id % pl.per_window);
+    uint32_t k = 0;
+    while (rem >= pl.first[k + 1]) ++k;
+    uint32_t chunk = rem - pl.first[k];
+    uint32_t e0 = chunk * G1, e1 = min(e0 + G1, pl.members[k]);
+    const uint32_t *wb = bucket + mw * g.B * 48;
+    const uint32_t lowmask = (1u << k) - 1u;
+    Xyzz acc = Xyzz::inf();
+    for (uint32_t e = e0; e < e1; ++e) {
+        uint32_t x = ((e >> k) << (k + 1)) | (1u << k) | (e & lowmask);
+        acc = xyzz_add(acc, load_xyzz(wb + (uint64_t)(x - 1) * 48));
+    }
+    store_xyzz(pp + id * 48, acc);
+}
+
+// Sum groups of G consecutive partials within segments of `seg` partials;
+// output segment length ceil(seg / G).  Results as Jacobian when `to_jac`.
+__global__ void __launch_bounds__(kBlock) k_group_sums(const uint32_t *in, uint64_t nseg, const uint32_t *seg_off,
+                                                       const uint32_t *seg_len, const uint32_t *out_off, uint32_t G,
+                                                       uint64_t total_out, uint32_t *out, int to_jac)
+{
+    uint64_t id = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (id >= total_out) return;
+    // locate segment by binary search over out_off (nseg small)
+    uint64_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        uint64_t mid = (lo + hi) / 2;
+        if (out_off[mid] <= id) lo = mid;
+        else hi = mid;
+    }
+    uint32_t local = (uint32_t)(id - out_off[lo]);
+    uint32_t s0 = seg_off[lo] + local * G, s1 = min(s0 + G, seg_off[lo] + seg_len[lo]);
+    Xyzz acc = Xyzz::inf();
+    for (uint32_t x = s0; x < s1; ++x) acc = xyzz_add(acc, load_xyzz(in + (uint64_t)x * 48));
+    if (to_jac) store_jacobian(out + id * 36, xyzz_to_jacobian(acc));
+    else store_xyzz(out + id * 48, acc);
+}
+
+inline uint32_t grid1(uint64_t n) { return (uint32_t)ceil_div(n, kBlock); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- engine
+
+struct MsmEngine::Buffers {
+    DevBuf<uint32_t> cnt, off, cursor, q, co, cmap, ent, scan_tmp;
+    DevBuf<uint32_t> q2, co2, maxq, part2;
+    DevBuf<uint32_t> part, bucket, pp, pp2, pj;
+    DevBuf<uint32_t> seg_off, seg_len, out_off;
+    DevBuf<MsmDesc> descs;
+    DevBuf<Fr> masked;
+};
+
+MsmEngine::MsmEngine() : buf_(new Buffers) {}
+MsmEngine::~MsmEngine() = default;
+
+namespace {
+
+// exclusive scan of n u32 values (device), returns the total
+uint64_t scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t n, DevBuf<uint32_t> &tmp, hipStream_t st)
+{
+    // levels of tile sums
+    std::vector<uint64_t> sizes;
+    uint64_t m = n;
+    while (true) {
+        sizes.push_back(m);
+        if (m <= kScanTile) break;
+        m = ceil_div(m, kScanTile);
+    }
+    // tmp layout: for each level >= 1, sums array and its scan
+    uint64_t need = 0;
+    for (size_t l = 1; l < sizes.size(); ++l) need += 2 * sizes[l];
+    need += 2;
+    tmp.ensure(need);
+    std::vector<uint32_t *> sums(sizes.size(), nullptr), sscan(sizes.size(), nullptr);
+    uint64_t o = 0;
+    for (size_t l = 1; l < sizes.size(); ++l) {
+        sums[l] = tmp.get() + o;
+        o += sizes[l];
+        sscan[l] = tmp.get() + o;
+        o += sizes[l];
+    }
+    uint32_t *total_dev = tmp.get() + o;
+    // up-sweep
+    const uint32_t *src = in;
+    uint32_t *dst = out;
+    for (size_t l = 0; l < sizes.size(); ++l) {
+        uint64_t tiles = ceil_div(sizes[l], kScanTile);
+        uint32_t *s = l + 1 < sizes.size() ? sums[l + 1] : total_dev;
+        hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)tiles), dim3(kBlock), 0, st, src, dst, s, sizes[l]);
+        MH_HIP(hipGetLastError());
+        if (l + 1 < sizes.size()) {
+            src = sums[l + 1];
+            dst = sscan[l + 1];
+        }
+    }
+    // down-sweep
+    for (size_t l = sizes.size() - 1; l-- > 0;) {
+        uint64_t tiles = ceil_div(sizes[l], kScanTile);
+        uint32_t *target = l == 0 ? out : sscan[l];
+        hipLaunchKernelGGL(k_scan_add, dim3((uint32_t)tiles), dim3(kBlock), 0, st, target, sscan[l + 1], sizes[l]);
+        MH_HIP(hipGetLastError());
+    }
+    // total = last exclusive value + last input
+    uint32_t last_out = 0, last_in = 0;
+    MH_HIP(hipMemcpyAsync(&last_out, out + n - 1, 4, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipMemcpyAsync(&last_in, in + n - 1, 4, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    return (uint64_t)last_out + last_in;
+}
+
+uint32_t auto_window(uint64_t n)
+{
+    // measured on MI355X: larger windows cut entries (N * ceil(256/c)) but
+    // grow the bucket planes; c around log2(n) - 7 balances the two
+    int lg = 0;
+    while ((1ull << lg) < n) ++lg;
+    int c = lg - 7;
+    if (c < 8) c = 8;
+    if (c > 16) c = 16;
+    return (uint32_t)c;
+}
+
+}  // namespace
+
+void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::vector<MsmJob> &jobs,
+                    uint32_t window_bits, RawG1Projective *out, hipStream_t st)
+{
+    if (jobs.empty()) return;
+    Buffers &b = *buf_;
+    uint64_t max_n = 0, total_pairs = 0;
+    for (const auto &j : jobs) {
+        MH_CHECK(j.pt_off + j.n <= npoints_total, (int)hipErrorInvalidValue, "MSM range exceeds the point set");
+        max_n = std::max<uint64_t>(max_n, j.n);
+        total_pairs += j.n;
+    }
+    Geometry g;
+    g.c = window_bits ? window_bits : auto_window(max_n);
+    g.nwin = (256 + g.c - 1) / g.c;
+    g.B = 1u << (g.c - 1);
+    g.M = (uint32_t)jobs.size();
+    last_window = g.c;
+    MH_CHECK((uint64_t)g.nwin * total_pairs < (1ull << 32), (int)hipErrorInvalidValue, "MSM batch too large");
+    const uint64_t nkeys = g.keys();
+    ProfileScope prof;
+    prof.begin("msm_total", st);
+
+    // descriptors
+    std::vector<MsmDesc> hd(jobs.size());
+    for (size_t i = 0; i < jobs.size(); ++i) hd[i] = MsmDesc{jobs[i].scalars, (uint32_t)jobs[i].pt_off, (uint32_t)jobs[i].n};
+    b.descs.ensure(hd.size());
+    MH_HIP(hipMemcpyAsync(b.descs.get(), hd.data(), hd.size() * sizeof(MsmDesc), hipMemcpyHostToDevice, st));
+
+    // 1-3: histogram, offsets, scatter
+    b.cnt.ensure(nkeys);
+    b.off.ensure(nkeys);
+    b.cursor.ensure(nkeys);
+    MH_HIP(hipMemsetAsync(b.cnt.get(), 0, nkeys * 4, st));
+    dim3 dgrid(grid1(max_n), g.M);
+    if (max_n) {
+        hipLaunchKernelGGL(k_digits_hist, dgrid, dim3(kBlock), 0, st, b.descs.get(), g, b.cnt.get());
+        MH_HIP(hipGetLastError());
+    }
+    const uint64_t nent = scan_exclusive(b.cnt.get(), b.off.get(), nkeys, b.scan_tmp, st);
+    b.ent.ensure(std::max<uint64_t>(nent, 1));
+    MH_HIP(hipMemcpyAsync(b.cursor.get(), b.off.get(), nkeys * 4, hipMemcpyDeviceToDevice, st));
+    if (max_n) {
+        hipLaunchKernelGGL(k_scatter, dgrid, dim3(kBlock), 0, st, b.descs.get(), g, b.cursor.get(), b.ent.get());
+        MH_HIP(hipGetLastError());
+    }
+
+    // 4-6: chunked accumulation into buckets
+    const uint32_t L = chunk_len;
+    b.q.ensure(nkeys);
+    b.co.ensure(nkeys);
+    hipLaunchKernelGGL(k_chunk_counts, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.cnt.get(), b.q.get(), nkeys, L);
+    MH_HIP(hipGetLastError());
+    const uint64_t nchunks = scan_exclusive(b.q.get(), b.co.get(), nkeys, b.scan_tmp, st);
+    b.bucket.ensure(nkeys * 48);
+    // empty buckets must read as the identity (zz = 0)
+    MH_HIP(hipMemsetAsync(b.bucket.get(), 0, nkeys * 48 * 4, st));
+    if (nchunks) {
+        b.cmap.ensure(nchunks);
+        b.part.ensure(nchunks * 48);
+        hipLaunchKernelGGL(k_chunk_map, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), b.cmap.get(),
+                           nkeys);
+        MH_HIP(hipGetLastError());
+        prof.begin("msm_accumulate", st);
+        hipLaunchKernelGGL(k_accumulate, dim3(grid1(nchunks)), dim3(kBlock), 0, st, d_points, b.ent.get(), b.cnt.get(),
+                           b.off.get(), b.q.get(), b.co.get(), b.cmap.get(), (uint32_t)nchunks, L, b.part.get(),
+                           b.bucket.get());
+        MH_HIP(hipGetLastError());
+        prof.end(st);
+        last_entries = nent;
+        // multi-chunk buckets: tree levels of `kGroup` partials while any
+        // bucket still has more than kGroup partials
+        constexpr uint32_t kGroup = 16;
+        b.maxq.ensure(1);
+        MH_HIP(hipMemsetAsync(b.maxq.get(), 0, 4, st));
+        hipLaunchKernelGGL(k_retire_single, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), nkeys, b.maxq.get());
+        MH_HIP(hipGetLastError());
+        uint32_t maxq = 0;
+        MH_HIP(hipMemcpyAsync(&maxq, b.maxq.get(), 4, hipMemcpyDeviceToHost, st));
+        MH_HIP(hipStreamSynchronize(st));
+        uint32_t *q = b.q.get(), *co = b.co.get(), *part = b.part.get();
+        while (maxq > kGroup) {
+            b.q2.ensure(nkeys);
+            b.co2.ensure(nkeys);
+            MH_HIP(hipMemsetAsync(b.maxq.get(), 0, 4, st));
+            hipLaunchKernelGGL(k_group_counts, dim3(grid1(nkeys)), dim3(kBlock), 0, st, q, b.q2.get(), nkeys, kGroup,
+                               b.maxq.get());
+            MH_HIP(hipGetLastError());
+            const uint64_t n2 = scan_exclusive(b.q2.get(), b.co2.get(), nkeys, b.scan_tmp, st);
+            b.cmap.ensure(n2);
+            b.part2.ensure(n2 * 48);
+            hipLaunchKernelGGL(k_chunk_map, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q2.get(), b.co2.get(),
+                               b.cmap.get(), nkeys);
+            MH_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_reduce_groups, dim3(grid1(n2)), dim3(kBlock), 0, st, q, co, b.co2.get(), b.cmap.get(),
+                               (uint32_t)n2, kGroup, part, b.part2.get());
+            MH_HIP(hipGetLastError());
+            MH_HIP(hipMemcpyAsync(&maxq, b.maxq.get(), 4, hipMemcpyDeviceToHost, st));
+            MH_HIP(hipStreamSynchronize(st));
+            // the reduced level becomes the input of the next
+            std::swap(b.q, b.q2);
+            std::swap(b.co, b.co2);
+            std::swap(b.part, b.part2);
+            q = b.q.get();
+            co = b.co.get();
+            part = b.part.get();
+        }
+        hipLaunchKernelGGL(k_bucket_sums, dim3(grid1(nkeys)), dim3(kBlock), 0, st, q, co, part, b.bucket.get(), nkeys);
+        MH_HIP(hipGetLastError());
+    }
+
+    // 7: bit-plane partial sums per (msm, window, plane)
+    PlaneLayout pl{};
+    pl.c = g.c;
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < g.c; ++k) {
+        pl.members[k] = k + 1 < g.c ? g.B / 2 : 1;
+        pl.first[k] = acc;
+        acc += (pl.members[k] + plane_group - 1) / plane_group;
+    }
+    pl.first[g.c] = acc;
+    for (uint32_t k = g.c + 1; k < 17; ++k) pl.first[k] = 0xffffffffu;
+    pl.per_window = acc;
+    const uint64_t nmw = (uint64_t)g.M * g.nwin;
+    const uint64_t npp = nmw * pl.per_window;
+    b.pp.ensure(npp * 48);
+    hipLaunchKernelGGL(k_plane_partials, dim3(grid1(npp)), dim3(kBlock), 0, st, b.bucket.get(), g, pl, plane_group,
+                       npp, b.pp.get());
+    MH_HIP(hipGetLastError());
+
+    // second pass: groups of plane_group2 within each (msm, window, plane)
+    const uint64_t nseg = nmw * g.c;
+    std::vector<uint32_t> seg_off(nseg), seg_len(nseg), out_off(nseg);
+    uint64_t tot_out = 0;
+    std::vector<uint32_t> plane_out(g.c);
+    for (uint64_t s = 0; s < nseg; ++s) {
+        uint64_t mw = s / g.c;
+        uint32_t k = (uint32_t)(s % g.c);
+        seg_off[s] = (uint32_t)(mw * pl.per_window + pl.first[k]);
+        seg_len[s] = pl.first[k + 1] - pl.first[k];
+        out_off[s] = (uint32_t)tot_out;
+        uint32_t o = (seg_len[s] + plane_group2 - 1) / plane_group2;
+        if (mw == 0) plane_out[k] = o;
+        tot_out += o;
+    }
+    b.seg_off.ensure(nseg);
+    b.seg_len.ensure(nseg);
+    b.out_off.ensure(nseg);
+    MH_HIP(hipMemcpyAsync(b.seg_off.get(), seg_off.data(), nseg * 4, hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(b.seg_len.get(), seg_len.data(), nseg * 4, hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(b.out_off.get(), out_off.data(), nseg * 4, hipMemcpyHostToDevice, st));
+    b.pj.ensure(tot_out * 36);
+    hipLaunchKernelGGL(k_group_sums, dim3(grid1(tot_out)), dim3(kBlock), 0, st, b.pp.get(), nseg, b.seg_off.get(),
+                       b.seg_len.get(), b.out_off.get(), plane_group2, tot_out, b.pj.get(), 1);
+    MH_HIP(hipGetLastError());
+
+    // 8: host: plane sums and one doubling chain per MSM
+    std::vector<HJac> hj(tot_out);
+    static_assert(sizeof(HJac) == 144, "HJac layout");
+    MH_HIP(hipMemcpyAsync(hj.data(), b.pj.get(), tot_out * 144, hipMemcpyDeviceToHost, st));
+    prof.end(st);
+    MH_HIP(hipStreamSynchronize(st));
+    prof.flush();
+
+    auto finish = [&](uint32_t m) {
+        // exponent e = c*w + k  ->  plane sum
+        const uint32_t E = g.nwin * g.c;
+        HJac accj = HJac::identity();
+        for (uint32_t e = E; e-- > 0;) {
+            accj = accj.dbl();
+            uint32_t w = e / g.c, k = e % g.c;
+            uint64_t s = ((uint64_t)m * g.nwin + w) * g.c + k;
+            for (uint32_t x = 0; x < plane_out[k]; ++x) accj = accj + hj[out_off[s] + x];
+        }
+        std::memcpy(&out[m], &accj, sizeof(RawG1Projective));
+    };
+    const uint32_t nthreads = std::min<uint32_t>(g.M, 16);
+    if (nthreads <= 1) {
+        for (uint32_t m = 0; m < g.M; ++m) finish(m);
+    } else {
+        std::vector<std::thread> th;
+        for (uint32_t t = 0; t < nthreads; ++t)
+            th.emplace_back([&, t]() {
+                for (uint32_t m = t; m < g.M; m += nthreads) finish(m);
+            });
+        for (auto &x : th) x.join();
+    }
+}
+
+}  // namespace mh
+
+// ---------------------------------------------------------------- C ABI
+
+using namespace mh;
+
+namespace {
+
+}  // namespace
+
+RawMsmConfig mh::default_msm_config(size_t npoints)
+{
+    uint32_t k = 0;
+    size_t cap = npoints > 1 ? npoints - 1 : 0;
+    while (cap) {
+        ++k;
+        cap >>= 1;
+    }
+    uint32_t wb = k == 0 ? 0 : 10;
+    if (k >= 16 && k <= 18) wb = 12;
+    if (k > 18) wb = 0;
+    return RawMsmConfig{k, 100000, 32768, wb, 3, 2, 1, 4, 0, (uint32_t)(k >= 14 && k <= 18 ? 1 : 0)};
+}
+
+std::unique_ptr<MsmContext> mh::make_context(size_t npoints)
+{
+    auto ctx = std::make_unique<MsmContext>();
+    ctx->lane = current_lane();
+    ctx->config = default_msm_config(npoints);
+    ctx->npoints = npoints;
+    ctx->points.alloc(std::max<size_t>(npoints, 1) * 24);
+    return ctx;
+}
+
+namespace {
+
+// bls12_381_g1_msm.cu:85-110
+bool valid_config(const RawMsmConfig &c)
+{
+    return c.domain_k <= 31 && c.min_msm_scalars >= 1 && c.unfiltered_min_scalars >= 1 &&
+           (c.window_bits == 0 || (c.window_bits >= 10 && c.window_bits <= 18)) && c.sort_grid_divisor >= 1 &&
+           c.sort_grid_divisor <= 8 && c.sort_window_lanes >= 1 && c.sort_window_lanes <= 4 &&
+           (c.integration_blocks == 1 || c.integration_blocks == 2 || c.integration_blocks == 4 ||
+            c.integration_blocks == 8 || c.integration_blocks == 16) &&
+           c.batch_lanes >= 1 && c.batch_lanes <= 16 && c.reuse_event <= 1 && c.device_reduce <= 3;
+}
+
+// The reference's window_bits tunes its sort; this engine picks its own
+// window from the MSM size (window_bits > 16 would overflow a 32-bit
+// histogram of signed digits per point, so the engine caps it).
+uint32_t engine_window(const RawMsmConfig &c)
+{
+    (void)c;
+    return 0;
+}
+
+constexpr size_t kMaxBatchPairs = size_t(1) << 26;   // bounds entry buffers (~ 20 windows x 4 B x this)
+
+void run_batched(MsmContext &ctx, const std::vector<MsmJob> &jobs, RawG1Projective *out, hipStream_t st)
+{
+    size_t i = 0;
+    while (i < jobs.size()) {
+        size_t j = i, pairs = 0;
+        while (j < jobs.size() && (j == i || pairs + jobs[j].n <= kMaxBatchPairs)) {
+            pairs += jobs[j].n;
+            ++j;
+        }
+        std::vector<MsmJob> part(jobs.begin() + i, jobs.begin() + j);
+        ctx.engine.run(ctx.points.get(), ctx.npoints, part, engine_window(ctx.config), out + i, st);
+        i = j;
+    }
+}
+
+RawG1Projective identity_out()
+{
+    RawG1Projective r;
+    HJac id = HJac::identity();
+    std::memcpy(&r, &id, sizeof(r));
+    return r;
+}
+
+}  // namespace
+
+#define MH_TRY try {
+#define MH_CATCH                                                                         \
+    }                                                                                    \
+    catch (const hip_error &e) { return make_error(e.code, e.what()); }                  \
+    catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
+
+extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_create_with_config(void **out,
+                                                                                 const RawG1Affine *points,
+                                                                                 size_t npoints, RawMsmConfig config)
+{
+    if (!out) return make_error((int)hipErrorInvalidValue, "out is null");
+    *out = nullptr;
+    if (!valid_config(config)) return make_error((int)hipErrorInvalidValue, "invalid BLS12-381 MSM config");
+    if (npoints && !points) return make_error((int)hipErrorInvalidValue, "points is null");
+    if (npoints >= (size_t(1) << 31)) return make_error((int)hipErrorInvalidValue, "too many MSM points");
+    MH_TRY
+    auto ctx = make_context(npoints);
+    ctx->config = config;
+    hipStream_t st = cur_stream();
+    if (npoints) MH_HIP(hipMemcpyAsync(ctx->points.get(), points, npoints * 96, hipMemcpyHostToDevice, st));
+    MH_HIP(hipStreamSynchronize(st));
+    *out = ctx.release();
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_create(void **out, const RawG1Affine *points,
+                                                                     size_t npoints)
+{
+    return midnight_sppark_bls12_381_g1_msm_context_create_with_config(out, points, npoints,
+                                                                        default_msm_config(npoints));
+}
+
+extern "C" void midnight_sppark_bls12_381_g1_msm_context_free(void *context)
+{
+    MsmContext *ctx = static_cast<MsmContext *>(context);
+    if (!ctx) return;
+    try {
+        (void)hipStreamSynchronize(lane_stream(ctx->lane));
+    } catch (...) {
+    }
+    delete ctx;
+}
+
+extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_batch(RawG1Projective *out, void *context,
+                                                                    const RawFr *scalars, size_t npoints, size_t nmsms)
+{
+    if (!context) return make_error(-1, "null BLS12-381 G1 MSM context");
+    MsmContext *ctx = static_cast<MsmContext *>(context);
+    if (nmsms == 0) return ok();
+    if (!out || (npoints && !scalars)) return make_error((int)hipErrorInvalidValue, "invalid MSM arguments");
+    if (npoints > ctx->npoints) return make_error((int)hipErrorInvalidValue, "MSM length exceeds the context");
+    MH_TRY
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = cur_stream();
+    if (npoints == 0) {
+        for (size_t i = 0; i < nmsms; ++i) out[i] = identity_out();
+        return ok();
+    }
+    DevBuf<Fr> d_s(npoints * nmsms);
+    MH_HIP(hipMemcpyAsync(d_s.get(), scalars, npoints * nmsms * 32, hipMemcpyHostToDevice, st));
+    std::vector<MsmJob> jobs(nmsms);
+    for (size_t i = 0; i < nmsms; ++i) jobs[i] = MsmJob{d_s.get() + i * npoints, 0, npoints};
+    run_batched(*ctx, jobs, out, st);
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_device_batch(RawG1Projective *out, void *context,
+                                                                           const void *device_vec,
+                                                                           size_t scalar_offset, size_t npoints,
+                                                                           size_t nmsms)
+{
+    if (!context || !device_vec || (nmsms && !out))
+        return make_error(-1, "invalid resident BLS12-381 G1 MSM arguments");
+    MsmContext *ctx = static_cast<MsmContext *>(context);
+    const DeviceVec *dv = static_cast<const DeviceVec *>(device_vec);
+    if (npoints && nmsms > (SIZE_MAX - scalar_offset) / npoints)
+        return make_error((int)hipErrorInvalidValue, "resident MSM range overflow");
+    if (scalar_offset + npoints * nmsms > dv->len)
+        return make_error((int)hipErrorInvalidValue, "resident MSM range exceeds device vector");
+    if (npoints > ctx->npoints) return make_error((int)hipErrorInvalidValue, "MSM length exceeds the context");
+    if (nmsms == 0) return ok();
+    MH_TRY
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = cur_stream();
+    if (npoints == 0) {
+        for (size_t i = 0; i < nmsms; ++i) out[i] = identity_out();
+        return ok();
+    }
+    std::vector<MsmJob> jobs(nmsms);
+    for (size_t i = 0; i < nmsms; ++i) jobs[i] = MsmJob{dv->values.get() + scalar_offset + i * npoints, 0, npoints};
+    run_batched(*ctx, jobs, out, st);
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_device_ragged(RawG1Projective *out, void *context,
+                                                                            const void *device_vec,
+                                                                            const RawMsmSlice *slices, size_t count)
+{
+    if (!context || !device_vec || (count && (!out || !slices)))
+        return make_error(-1, "invalid ragged resident BLS12-381 G1 MSM arguments");
+    if (count == 0) return ok();
+    MsmContext *ctx = static_cast<MsmContext *>(context);
+    const DeviceVec *dv = static_cast<const DeviceVec *>(device_vec);
+    for (size_t i = 0; i < count; ++i) {
+        const RawMsmSlice &s = slices[i];
+        if (s.point_offset > ctx->npoints || s.len > ctx->npoints - s.point_offset || s.scalar_offset > dv->len ||
+            s.len > dv->len - s.scalar_offset || s.nonzero_len > s.len)
+            return make_error((int)hipErrorInvalidValue, "ragged resident MSM slice exceeds its source");
+    }
+    MH_TRY
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = cur_stream();
+    // zero scalars produce no digits, so the nonzero prefix needs no compaction
+    std::vector<MsmJob> jobs;
+    std::vector<size_t> where;
+    for (size_t i = 0; i < count; ++i) {
+        if (slices[i].nonzero_len == 0 || slices[i].len == 0) {
+            out[i] = identity_out();
+            continue;
+        }
+        jobs.push_back(MsmJob{dv->values.get() + slices[i].scalar_offset, slices[i].point_offset, slices[i].len});
+        where.push_back(i);
+    }
+    std::vector<RawG1Projective> res(jobs.size());
+    run_batched(*ctx, jobs, res.data(), st);
+    for (size_t i = 0; i < jobs.size(); ++i) out[where[i]] = res[i];
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_device_scrambled_batch(
+    RawG1Projective *out, void *context, const void *device_vec, size_t scalar_offset, size_t npoints, size_t nmsms,
+    const void *mask_device_vec)
+{
+    if (!context || !device_vec || !mask_device_vec || (nmsms && !out))
+        return make_error(-1, "invalid scrambled resident BLS12-381 G1 MSM arguments");
+    MsmContext *ctx = static_cast<MsmContext *>(context);
+    const DeviceVec *dv = static_cast<const DeviceVec *>(device_vec);
+    const DeviceVec *mask = static_cast<const DeviceVec *>(mask_device_vec);
+    if (mask->len != npoints) return make_error((int)hipErrorInvalidValue, "resident MSM scramble mask length mismatch");
+    if (npoints && nmsms > (SIZE_MAX - scalar_offset) / npoints)
+        return make_error((int)hipErrorInvalidValue, "scrambled resident MSM range overflow");
+    const size_t count = npoints * nmsms;
+    if (scalar_offset + count > dv->len)
+        return make_error((int)hipErrorInvalidValue, "scrambled resident MSM range exceeds device vector");
+    if (npoints > ctx->npoints) return make_error((int)hipErrorInvalidValue, "MSM length exceeds the context");
+    if (count == 0) return ok();
+    MH_TRY
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = cur_stream();
+    // returns MSM(P, s + mask); the caller subtracts MSM(P, mask) (lib.rs:3349-3353)
+    DevBuf<Fr> masked(count);
+    fr_add_mask(masked.get(), dv->values.get() + scalar_offset, mask->values.get(), npoints, count, st);
+    std::vector<MsmJob> jobs(nmsms);
+    for (size_t i = 0; i < nmsms; ++i) jobs[i] = MsmJob{masked.get() + i * npoints, 0, npoints};
+    run_batched(*ctx, jobs, out, st);
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_hip_g1_msm_context_device_multi(RawG1Projective *out, void *context,
+                                                              const void *device_vec, const size_t *scalar_offsets,
+                                                              const size_t *sizes, size_t count)
+{
+    if (!context || !device_vec || (count && (!out || !scalar_offsets || !sizes)))
+        return make_error(-1, "invalid batched MSM arguments");
+    MsmContext *ctx = static_cast<MsmContext *>(context);
+    const DeviceVec *dv = static_cast<const DeviceVec *>(device_vec);
+    for (size_t i = 0; i < count; ++i)
+        if (sizes[i] > ctx->npoints || scalar_offsets[i] > dv->len || sizes[i] > dv->len - scalar_offsets[i])
+            return make_error((int)hipErrorInvalidValue, "batched MSM slice exceeds its source");
+    if (count == 0) return ok();
+    MH_TRY
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = cur_stream();
+    std::vector<MsmJob> jobs;
+    std::vector<size_t> where;
+    for (size_t i = 0; i < count; ++i) {
+        if (sizes[i] == 0) {
+            out[i] = identity_out();
+            continue;
+        }
+        jobs.push_back(MsmJob{dv->values.get() + scalar_offsets[i], 0, sizes[i]});
+        where.push_back(i);
+    }
+    std::vector<RawG1Projective> res(jobs.size());
+    run_batched(*ctx, jobs, res.data(), st);
+    for (size_t i = 0; i < jobs.size(); ++i) out[where[i]] = res[i];
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_g1_msm_batch(RawG1Projective *out, const RawG1Affine *points,
+                                                            const RawFr *scalars, size_t npoints, size_t nmsms)
+{
+    void *h = nullptr;
+    RustError e = midnight_sppark_bls12_381_g1_msm_context_create(&h, points, npoints);
+    if (e.code) return e;
+    e = midnight_sppark_bls12_381_g1_msm_context_batch(out, h, scalars, npoints, nmsms);
+    midnight_sppark_bls12_381_g1_msm_context_free(h);
+    return e;
+}
+
+extern "C" RustError midnight_sppark_bls12_381_g1_msm(RawG1Projective *out, const RawG1Affine *points,
+                                                      const RawFr *scalars, size_t len)
+{
+    return midnight_sppark_bls12_381_g1_msm_batch(out, points, scalars, len, 1);
+}
+
+extern "C" RustError midnight_hip_g1_msm_context_last_stats(const void *context, uint64_t *entries,
+                                                           uint32_t *window_bits)
+{
+    if (!context) return make_error((int)hipErrorInvalidValue, "null context");
+    const MsmContext *ctx = static_cast<const MsmContext *>(context);
+    if (entries) *entries = ctx->engine.last_entries;
+    if (window_bits) *window_bits = ctx->engine.last_window;
+    return ok();
+}

@@ -1,0 +1,43 @@
+// Host interface of the batched Pippenger engine (msm.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <memory>
+#include <vector>
+
+#include "fields.hpp"
+#include "runtime.hpp"
+
+namespace mh {
+
+// One MSM of a batch: n scalars (device, Montgomery) against points
+// [pt_off, pt_off + n) of the engine's point table.
+struct MsmJob {
+    const Fr *scalars;
+    size_t pt_off;
+    size_t n;
+};
+
+class MsmEngine {
+public:
+    MsmEngine();
+    ~MsmEngine();
+    // d_points: RawG1Affine rows (24 u32 each).  window_bits 0 = automatic.
+    void run(const uint32_t *d_points, size_t npoints_total, const std::vector<MsmJob> &jobs, uint32_t window_bits,
+             RawG1Projective *out, hipStream_t st);
+
+    uint32_t chunk_len = 64;      // entries per accumulation lane
+    uint32_t plane_group = 16;    // buckets per lane in the first plane pass
+    uint32_t plane_group2 = 8;    // partials per lane in the second plane pass
+
+    // statistics of the last run (for throughput / roofline reporting)
+    uint64_t last_entries = 0;    // nonzero signed digits = bucket additions
+    uint32_t last_window = 0;
+
+private:
+    struct Buffers;
+    std::unique_ptr<Buffers> buf_;
+};
+
+}  // namespace mh

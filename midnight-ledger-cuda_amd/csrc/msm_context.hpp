@@ -1,0 +1,27 @@
+// Resident MSM point set behind the reference's opaque context handle
+// (msm_context_t, bls12_381_g1_msm.cu:154-329): the points stay in HBM for
+// the context's lifetime and every MSM of the context reuses the engine's
+// scratch buffers.
+#pragma once
+#include <memory>
+#include <mutex>
+
+#include "msm.hpp"
+#include "runtime.hpp"
+
+namespace mh {
+
+struct MsmContext {
+    uint32_t lane = 0;
+    RawMsmConfig config{};
+    size_t npoints = 0;
+    DevBuf<uint32_t> points;   // npoints rows of RawG1Affine (24 u32)
+    std::mutex mu;
+    MsmEngine engine;
+};
+
+RawMsmConfig default_msm_config(size_t npoints);
+std::unique_ptr<MsmContext> make_context(size_t npoints);
+void generator_multiples(uint32_t *out_points, const Fr *scalars, size_t n, hipStream_t st);
+
+}  // namespace mh

@@ -121,6 +121,14 @@ def _declare(L):
             E, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
         "midnight_hip_field_mul_rate": (
             E, [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+        "midnight_hip_fr_device_vec_to_montgomery": (E, [vp]),
+        "midnight_hip_g1_msm_context_create_from_multiples": (E, [vpp, vp]),
+        "midnight_hip_g1_msm_context_points_download": (E, [u64p, vp, c_size]),
+        "midnight_hip_g1_msm_context_last_stats": (
+            E, [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32)]),
+        "midnight_hip_profile_enable": (None, [ctypes.c_int]),
+        "midnight_hip_profile_read": (
+            ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name, None)
@@ -336,7 +344,11 @@ def default_config(npoints):
 class MsmContext:
     """Resident point set (msm_context_t)."""
 
-    def __init__(self, points, config=None):
+    def __init__(self, points, config=None, _handle=None, _npoints=0):
+        if _handle is not None:
+            self.handle = _handle
+            self.npoints = _npoints
+            return
         p = _aff_array(points)
         self.npoints = p.shape[0]
         h = ctypes.c_void_p()
@@ -383,6 +395,26 @@ class MsmContext:
             P(out), self.handle, dvec.handle, scalar_offset, npoints, nmsms, mask.handle))
         return out
 
+    @classmethod
+    def from_multiples(cls, dvec):
+        """Context over the points [t_i]G, t_i the (Montgomery) values of dvec,
+        computed on the device."""
+        h = ctypes.c_void_p()
+        check(lib().midnight_hip_g1_msm_context_create_from_multiples(ctypes.byref(h), dvec.handle))
+        return cls(None, _handle=h, _npoints=len(dvec))
+
+    def points(self, n=None):
+        n = self.npoints if n is None else n
+        out = np.zeros((n, 12), dtype=np.uint64)
+        check(lib().midnight_hip_g1_msm_context_points_download(P(out), self.handle, n))
+        return out
+
+    def last_stats(self):
+        e = ctypes.c_uint64()
+        w = ctypes.c_uint32()
+        check(lib().midnight_hip_g1_msm_context_last_stats(self.handle, ctypes.byref(e), ctypes.byref(w)))
+        return e.value, w.value
+
     def device_multi(self, dvec, scalar_offsets, sizes):
         n = len(sizes)
         offs = (c_size * n)(*scalar_offsets)
@@ -393,6 +425,22 @@ class MsmContext:
 
 
 # ---------------------------------------------------------------- diagnostics
+
+def to_montgomery(dvec):
+    """In place: canonical values -> Montgomery form (a -> a R mod r)."""
+    check(lib().midnight_hip_fr_device_vec_to_montgomery(dvec.handle))
+
+
+def profile_enable(on=True):
+    lib().midnight_hip_profile_enable(1 if on else 0)
+
+
+def profile_read(name):
+    ms = ctypes.c_double()
+    cnt = ctypes.c_uint64()
+    lib().midnight_hip_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+    return ms.value, cnt.value
+
 
 def valu_peak():
     r = ctypes.c_double()

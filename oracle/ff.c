@@ -289,3 +289,14 @@ int orc_fp_from_canonical(orc_fp *r, const uint64_t in[6])
     orc_fp_mul(r, &t, &FP_R2);
     return 1;
 }
+
+/* sum_i a_i * b_i (closed form of an MSM over points [t_i]G) */
+void orc_fr_inner_product(orc_fr *r, const orc_fr *a, const orc_fr *b, size_t n)
+{
+    orc_fr acc = {{0}}, t;
+    for (size_t i = 0; i < n; ++i) {
+        orc_fr_mul(&t, &a[i], &b[i]);
+        orc_fr_add(&acc, &acc, &t);
+    }
+    *r = acc;
+}

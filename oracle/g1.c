@@ -317,3 +317,14 @@ int orc_g1_from_uncompressed(orc_g1a *r, const uint8_t in[96])
     if (!fp_from_be(&r->y, buf + 48)) return 0;
     return orc_g1a_is_on_curve(r);
 }
+
+/* decompress n consecutive 48-byte points; returns the number that failed */
+size_t orc_g1_decompress_many(orc_g1a *out, const uint8_t *in, size_t n, int threads)
+{
+    size_t bad = 0;
+    if (threads < 1) threads = 1;
+    #pragma omp parallel for num_threads(threads) reduction(+ : bad) schedule(static)
+    for (long long i = 0; i < (long long)n; ++i)
+        if (!orc_g1_decompress(&out[i], in + 48 * i)) ++bad;
+    return bad;
+}

@@ -49,6 +49,7 @@ int  orc_fr_eq(const orc_fr *a, const orc_fr *b);
 int  orc_fr_is_zero(const orc_fr *a);
 /* Fq::from_uniform_bytes (fq.rs:704-726): a0*R2 + a1*R3 */
 void orc_fr_from_uniform_bytes(orc_fr *r, const uint8_t bytes[64]);
+void orc_fr_inner_product(orc_fr *r, const orc_fr *a, const orc_fr *b, size_t n);
 /* named constants, Montgomery form (fq.rs:60-135, 476-505, 693-702) */
 void orc_fr_root_of_unity(orc_fr *r);      /* 2^32-th root */
 void orc_fr_zeta(orc_fr *r);               /* cube root of unity ZETA */
@@ -89,6 +90,7 @@ void orc_g1_batch_to_affine(orc_g1a *out, const orc_g1j *in, size_t n);
 /* ZCash serialisation (48-byte compressed, 96-byte uncompressed) */
 void orc_g1_compress(uint8_t out[48], const orc_g1a *p);
 int  orc_g1_decompress(orc_g1a *r, const uint8_t in[48]);
+size_t orc_g1_decompress_many(orc_g1a *out, const uint8_t *in, size_t n, int threads);
 int  orc_g1_from_uncompressed(orc_g1a *r, const uint8_t in[96]);
 void orc_g1_to_uncompressed(uint8_t out[96], const orc_g1a *p);
 

@@ -62,6 +62,7 @@ def _declare(L):
         "orc_fr_to_canonical": (V, [u64p, u64p]),
         "orc_fr_from_canonical": (ctypes.c_int, [u64p, u64p]),
         "orc_fr_from_uniform_bytes": (V, [u64p, u8p]),
+        "orc_fr_inner_product": (V, [u64p, u64p, u64p, ctypes.c_size_t]),
         "orc_fr_root_of_unity": (V, [u64p]), "orc_fr_zeta": (V, [u64p]), "orc_fr_delta": (V, [u64p]),
         "orc_fp_add": (V, [u64p, u64p, u64p]), "orc_fp_sub": (V, [u64p, u64p, u64p]),
         "orc_fp_mul": (V, [u64p, u64p, u64p]), "orc_fp_inv": (V, [u64p, u64p]),
@@ -77,6 +78,7 @@ def _declare(L):
         "orc_g1_batch_to_affine": (V, [u64p, u64p, ctypes.c_size_t]),
         "orc_g1_compress": (V, [u8p, u64p]),
         "orc_g1_decompress": (ctypes.c_int, [u64p, u8p]),
+        "orc_g1_decompress_many": (ctypes.c_size_t, [u64p, u8p, ctypes.c_size_t, ctypes.c_int]),
         "orc_g1_from_uncompressed": (ctypes.c_int, [u64p, u8p]),
         "orc_g1_to_uncompressed": (V, [u8p, u64p]),
         "orc_msm_naive": (V, [u64p, u64p, u64p, ctypes.c_size_t]),
@@ -154,6 +156,28 @@ def fr_op(name, *args):
 def fp_op(name, *args):
     out = np.zeros(6, dtype=np.uint64)
     getattr(lib(), "orc_fp_" + name)(P(out), *[P(np.ascontiguousarray(a)) for a in args])
+    return out
+
+
+def inner_product(a, b):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    out = fr()
+    lib().orc_fr_inner_product(P(out), P(a), P(b), a.shape[0])
+    return out
+
+
+def g1_eq(p, q):
+    """equality of two Jacobian points (any Z)"""
+    return bool(lib().orc_g1j_eq(P(np.ascontiguousarray(p, dtype=np.uint64)),
+                                 P(np.ascontiguousarray(q, dtype=np.uint64))))
+
+
+def decompress(buf):
+    b = np.frombuffer(bytes(buf), dtype=np.uint8).copy()
+    out = g1a()
+    if not lib().orc_g1_decompress(P(out), B(b)):
+        raise ValueError("invalid compressed point")
     return out
 
 
@@ -310,4 +334,18 @@ def load_srs_file(path):
         row = np.ascontiguousarray(body[i])
         if not lib().orc_g1_from_uncompressed(P(pts[i]), B(row)):
             raise ValueError("invalid point %d" % i)
+    return k, pts[:n].copy(), pts[n:].copy()
+
+
+def load_srs_fixture(path):
+    """Read tests/golden/srs_2p14.g1c: u32 k || 2^k compressed g || 2^k compressed g_lagrange."""
+    with open(path, "rb") as f:
+        data = f.read()
+    k = int.from_bytes(data[:4], "little")
+    n = 1 << k
+    raw = np.frombuffer(data, dtype=np.uint8, offset=4, count=2 * n * 48).copy()
+    pts = g1a(2 * n)
+    bad = lib().orc_g1_decompress_many(P(pts), B(raw), 2 * n, os.cpu_count())
+    if bad:
+        raise ValueError("%d invalid points in %s" % (bad, path))
     return k, pts[:n].copy(), pts[n:].copy()

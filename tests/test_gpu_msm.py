@@ -1,0 +1,202 @@
+"""GPU parity: G1 MSM through the C ABI vs the CPU oracle.
+
+Bar: equal group elements (the Jacobian outputs are compared after
+normalisation, as the reference's callers do; the affine result is unique).
+Reference entry points: bls12_381_g1_msm.cu:6164-6486.  Test patterns follow
+midnight-sppark-msm/src/lib.rs:5277-5379 (points (i+1)G, scalars with 1/7
+zeros) and the reference's SRS data file (MSM(g, c) == MSM(g_lagrange,
+NTT(c)), tests/golden/srs_2p14.g1c).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def points_for(O, ts):
+    return O.fixed_base(ts)
+
+
+def closed_form(O, ts, scalars):
+    """MSM([t_i]G, s) = [sum s_i t_i]G"""
+    return O.g1_mul(O.generator(), O.inner_product(scalars, ts))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 64, 100, 1000, 4097])
+def test_msm_small_vs_naive(H, O, n):
+    ts = O.random_fr(n, 500 + n)
+    pts = points_for(O, ts)
+    sc = O.random_fr(n, 600 + n)
+    got = H.g1_msm(pts, sc)
+    ref = O.msm(pts, sc, naive=n <= 100)
+    assert O.g1_eq(got, ref), "n=%d" % n
+    assert O.g1_eq(got, closed_form(O, ts, sc))
+
+
+def test_msm_reference_pattern(H, O):
+    # lib.rs:5277-5379: points (i+1)G, every 7th scalar zero
+    n = 4096
+    ts = np.stack([O.fr_from_int(i + 1) for i in range(n)])
+    pts = points_for(O, ts)
+    sc = O.random_fr(n, 9)
+    sc[::7] = 0
+    got = H.g1_msm(pts, sc)
+    assert O.g1_eq(got, closed_form(O, ts, sc))
+
+
+def test_msm_edge_scalars_and_points(H, O):
+    n = 300
+    ts = O.random_fr(n, 21)
+    pts = points_for(O, ts)
+    sc = O.random_fr(n, 22)
+    r_minus_1 = O.fr_op("neg", O.fr_from_int(1))
+    sc[0] = 0                       # zero scalar
+    sc[1] = O.fr_from_int(1)        # one
+    sc[2] = r_minus_1               # largest scalar
+    sc[3] = r_minus_1
+    pts[4] = 0                      # identity point (encoded as 0,0)
+    pts[5] = pts[6]                 # duplicate point, same scalar -> doubling in a bucket
+    sc[5] = sc[6]
+    ts[4] = 0
+    ts[5] = ts[6]
+    pts[7] = pts[8].copy()          # P and -P with equal scalars -> cancel
+    neg_y = O.fp_op("sub", np.zeros(6, dtype=np.uint64), pts[8][6:])
+    pts[7][6:] = neg_y
+    ts[7] = O.fr_op("neg", ts[8])
+    sc[7] = sc[8]
+    got = H.g1_msm(pts, sc)
+    assert O.g1_eq(got, closed_form(O, ts, sc))
+    assert O.g1_eq(got, O.msm(pts, sc, naive=True))
+
+
+def test_msm_all_zero_and_all_equal(H, O):
+    n = 2000
+    ts = O.random_fr(n, 31)
+    pts = points_for(O, ts)
+    z = np.zeros((n, 4), dtype=np.uint64)
+    got = H.g1_msm(pts, z)
+    assert not got[12:].any()          # identity: z = 0
+    # every scalar equal: one bucket per window holds every point
+    one = np.tile(O.fr_from_int(1), (n, 1))
+    assert O.g1_eq(H.g1_msm(pts, one), closed_form(O, ts, one))
+    big = np.tile(O.random_fr(1, 32)[0], (n, 1))
+    assert O.g1_eq(H.g1_msm(pts, big), closed_form(O, ts, big))
+
+
+def test_msm_srs_fixture_identity(H, O):
+    """Reference data: MSM(g, c) == MSM(g_lagrange, NTT(c)) at k=14."""
+    k, g, gl = O.load_srs_fixture(os.path.join(GOLDEN, "srs_2p14.g1c"))
+    n = 1 << k
+    c = O.random_fr(n, 1234)
+    e = H.fr_ntt(c, k)             # GPU NTT
+    a = H.g1_msm(g, c)             # GPU MSM, monomial basis
+    b = H.g1_msm(gl, e)            # GPU MSM, Lagrange basis
+    assert O.g1_eq(a, b)
+    assert O.g1_eq(a, O.msm(g, c))
+
+
+def test_msm_context_batch_variants(H, O):
+    n, m = 3000, 4
+    ts = O.random_fr(n, 41)
+    pts = points_for(O, ts)
+    sc = O.random_fr(n * m, 42)
+    ctx = H.MsmContext(pts)
+    out = ctx.batch(sc, n, m)
+    for i in range(m):
+        assert O.g1_eq(out[i], closed_form(O, ts, sc[i * n:(i + 1) * n]))
+    # resident scalars
+    dv = H.DeviceVec.upload(np.concatenate([O.random_fr(5, 43), sc]))
+    out2 = ctx.device_batch(dv, 5, n, m)
+    for i in range(m):
+        assert O.g1_eq(out2[i], out[i])
+    # ragged: (scalar_offset, point_offset, len, nonzero_len)
+    slices = [(5, 0, n, n), (5 + n, 100, 500, 500), (5 + 2 * n, 7, 0, 0), (5 + 3 * n, 2000, 1000, 10)]
+    out3 = ctx.device_ragged(dv, slices)
+    allsc = dv.download()
+    for (so, po, ln, nz), res in zip(slices, out3):
+        ref = closed_form(O, ts[po:po + ln], allsc[so:so + ln]) if ln else None
+        if ln == 0:
+            assert not res[12:].any()      # identity: z = 0
+        else:
+            assert O.g1_eq(res, ref)
+    # scrambled: returns MSM(P, s + mask)
+    mask = O.random_fr(n, 44)
+    mdv = H.DeviceVec.upload(mask)
+    out4 = ctx.device_scrambled_batch(dv, 5, n, 2, mdv)
+    for i in range(2):
+        s_plus = np.stack([O.fr_op("add", sc[i * n + j], mask[j]) for j in range(n)])
+        assert O.g1_eq(out4[i], closed_form(O, ts, s_plus))
+    # batched ragged sizes over one resident vector
+    sizes = [n, 1024, 17, 2999]
+    offs = [5, 5 + n, 5 + 2 * n, 5]
+    out5 = ctx.device_multi(dv, offs, sizes)
+    for o, s, res in zip(offs, sizes, out5):
+        assert O.g1_eq(res, closed_form(O, ts[:s], allsc[o:o + s]))
+    ctx.free()
+
+
+def test_msm_batch_entry(H, O):
+    n, m = 1500, 3
+    ts = O.random_fr(n, 51)
+    pts = points_for(O, ts)
+    sc = O.random_fr(n * m, 52)
+    out = H.g1_msm_batch(pts, sc, m)
+    for i in range(m):
+        assert O.g1_eq(out[i], closed_form(O, ts, sc[i * n:(i + 1) * n]))
+
+
+@pytest.mark.parametrize("log_n", [16, 18])
+def test_msm_medium_closed_form(H, O, log_n):
+    n = 1 << log_n
+    ts = O.random_fr(n, 60 + log_n)
+    pts = points_for(O, ts)
+    sc = O.random_fr(n, 70 + log_n)
+    assert O.g1_eq(H.g1_msm(pts, sc), closed_form(O, ts, sc))
+
+
+def test_msm_skewed_mix(H, O):
+    # the skewed scalar mix of the 2^24 workload, at 2^16
+    n = 1 << 16
+    ts = O.random_fr(n, 80)
+    pts = points_for(O, ts)
+    sc = O.random_fr(n, 81)
+    rng = np.random.default_rng(82)
+    u = rng.random(n)
+    small = u < 0.7
+    zo = (u >= 0.7) & (u < 0.9)
+    for i in np.nonzero(small)[0]:
+        sc[i] = O.fr_from_int(int(rng.integers(0, 1 << 16)))
+    for i in np.nonzero(zo)[0]:
+        sc[i] = O.fr_from_int(int(rng.integers(0, 2)))
+    assert O.g1_eq(H.g1_msm(pts, sc), closed_form(O, ts, sc))
+
+
+def test_msm_errors(H, O):
+    # null context (bls12_381_g1_msm.cu:6295-6297)
+    with pytest.raises(H.MidnightHipError):
+        H.check(H.lib().midnight_sppark_bls12_381_g1_msm_context_batch(None, None, None, 1, 1))
+    bad = H.RawMsmConfig(14, 100000, 32768, 9, 3, 2, 1, 4, 0, 1)   # window_bits 9 is invalid
+    with pytest.raises(H.MidnightHipError):
+        H.MsmContext(np.zeros((4, 12), dtype=np.uint64), bad)
+
+
+def test_generator_multiples_and_montgomery(H, O):
+    n = 5000
+    ts = O.random_fr(n, 91)
+    # canonical -> Montgomery on the device
+    can = np.stack([np.array([(O.fr_to_int(t) >> (64 * i)) & (2**64 - 1) for i in range(4)], dtype=np.uint64)
+                    for t in ts])
+    dv = H.DeviceVec.upload(can)
+    H.to_montgomery(dv)
+    assert np.array_equal(dv.download(), ts)
+    ctx = H.MsmContext.from_multiples(dv)
+    assert np.array_equal(ctx.points(), O.fixed_base(ts))
+    sc = O.random_fr(n, 92)
+    assert O.g1_eq(ctx.batch(sc, n, 1)[0], O.g1_mul(O.generator(), O.inner_product(sc, ts)))
+    entries, window = ctx.last_stats()
+    assert window >= 8 and 0 < entries <= n * (-(-256 // window))
+    ctx.free()
