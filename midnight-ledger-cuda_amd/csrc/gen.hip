@@ -134,6 +134,7 @@ extern "C" RustError midnight_hip_g1_msm_context_create_from_multiples(void **ou
         hipStream_t st = cur_stream();
         std::unique_ptr<MsmContext> ctx = make_context(dv->len);
         generator_multiples(ctx->points.get(), dv->values.get(), dv->len, st);
+        build_precomp(*ctx, st);
         MH_HIP(hipStreamSynchronize(st));
         *out = ctx.release();
         return ok();

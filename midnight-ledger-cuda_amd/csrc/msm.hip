@@ -57,8 +57,21 @@ struct MsmDesc {
 
 struct Geometry {
     uint32_t c, nwin, B, M;
-    uint64_t keys() const { return (uint64_t)M * nwin * B; }
+    // kw: bucket sets per MSM (nwin for plain windows; 1 when the points are
+    // precomputed per window, [2^(c w)] P_i at index w * pstride + i)
+    uint32_t kw;
+    uint32_t pstride;
+    uint64_t keys() const { return (uint64_t)M * kw * B; }
 };
+
+// Keys are bucket-major: ((b * M) + m) * kw + w.  All threads work on the
+// same window at the same time, so a window-major layout would send every
+// histogram atomic to one small block of memory (a few L2 channels); with
+// the bucket index outermost they spread over the whole counter array.
+__device__ __forceinline__ uint64_t key_of(const Geometry &g, uint64_t m_kw, uint32_t w, uint32_t b)
+{
+    return (uint64_t)b * g.M * g.kw + m_kw + (g.kw > 1 ? w : 0);
+}
 
 // signed digit of window w; `carry` flows from window w-1
 __device__ __forceinline__ int32_t window_digit(const uint32_t k[9], uint32_t w, uint32_t c, uint32_t &carry)
@@ -93,12 +106,12 @@ __global__ void __launch_bounds__(kBlock) k_digits_hist(const MsmDesc *descs, Ge
     uint32_t k[9];
     canonical_scalar(d.scalars + i, k);
     uint32_t carry = 0;
-    const uint64_t base = (uint64_t)m * g.nwin * g.B;
+    const uint64_t base = (uint64_t)m * g.kw;
     for (uint32_t w = 0; w < g.nwin; ++w) {
         int32_t dg = window_digit(k, w, g.c, carry);
         if (dg == 0) continue;
         uint32_t b = (uint32_t)(dg < 0 ? -dg : dg) - 1;
-        atomicAdd(&cnt[base + (uint64_t)w * g.B + b], 1u);
+        atomicAdd(&cnt[key_of(g, base, w, b)], 1u);
     }
 }
 
@@ -111,14 +124,14 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const MsmDesc *descs, Geomet
     uint32_t k[9];
     canonical_scalar(d.scalars + i, k);
     uint32_t carry = 0;
-    const uint64_t base = (uint64_t)m * g.nwin * g.B;
+    const uint64_t base = (uint64_t)m * g.kw;
     const uint32_t pidx = d.pt_off + i;
     for (uint32_t w = 0; w < g.nwin; ++w) {
         int32_t dg = window_digit(k, w, g.c, carry);
         if (dg == 0) continue;
         uint32_t b = (uint32_t)(dg < 0 ? -dg : dg) - 1;
-        uint32_t pos = atomicAdd(&cursor[base + (uint64_t)w * g.B + b], 1u);
-        ent[pos] = pidx | (dg < 0 ? kSignBit : 0u);
+        uint32_t pos = atomicAdd(&cursor[key_of(g, base, w, b)], 1u);
+        ent[pos] = (pidx + w * g.pstride) | (dg < 0 ? kSignBit : 0u);
     }
 }
 
@@ -279,12 +292,12 @@ __global__ void __launch_bounds__(kBlock) k_plane_partials(const uint32_t *bucke
     while (rem >= pl.first[k + 1]) ++k;
     uint32_t chunk = rem - pl.first[k];
     uint32_t e0 = chunk * G1, e1 = min(e0 + G1, pl.members[k]);
-    const uint32_t *wb = bucket + mw * g.B * 48;
+    const uint64_t stride = (uint64_t)g.M * g.kw;   // keys are bucket-major
     const uint32_t lowmask = (1u << k) - 1u;
     Xyzz acc = Xyzz::inf();
     for (uint32_t e = e0; e < e1; ++e) {
         uint32_t x = ((e >> k) << (k + 1)) | (1u << k) | (e & lowmask);
-        acc = xyzz_add(acc, load_xyzz(wb + (uint64_t)(x - 1) * 48));
+        acc = xyzz_add(acc, load_xyzz(bucket + ((uint64_t)(x - 1) * stride + mw) * 48));
     }
     store_xyzz(pp + id * 48, acc);
 }
@@ -312,9 +325,42 @@ __global__ void __launch_bounds__(kBlock) k_group_sums(const uint32_t *in, uint6
     else store_xyzz(out + id * 48, acc);
 }
 
+// row w * n + i of `out` = [2^(c w)] P_i, by c doublings per window
+__global__ void __launch_bounds__(kBlock) k_precompute(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t c,
+                                                       uint32_t nwin)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Affine a = load_affine(in + (uint64_t)i * 24);
+    store_affine(out + (uint64_t)i * 24, a);
+    Xyzz x = xyzz_from_affine(a);
+    for (uint32_t w = 1; w < nwin; ++w) {
+        for (uint32_t k = 0; k < c; ++k) x = xyzz_dbl(x);
+        store_affine(out + ((uint64_t)w * n + i) * 24, xyzz_to_affine(x));
+    }
+}
+
 inline uint32_t grid1(uint64_t n) { return (uint32_t)ceil_div(n, kBlock); }
 
 }  // namespace
+
+void build_precomp(MsmContext &ctx, hipStream_t st)
+{
+    const size_t n = ctx.npoints;
+    if (n < (size_t(1) << 10)) return;
+    int lg = 0;
+    while ((size_t(1) << lg) < n) ++lg;
+    uint32_t c = (uint32_t)std::min(20, std::max(10, lg - 4));
+    uint32_t nwin = (256 + c - 1) / c;
+    if ((uint64_t)nwin * n >= (1ull << 31)) return;   // entry indices are 31-bit
+    ctx.table.alloc((size_t)nwin * n * 24);
+    hipLaunchKernelGGL(k_precompute, dim3(grid1(n)), dim3(kBlock), 0, st, ctx.points.get(), ctx.table.get(),
+                       (uint32_t)n, c, nwin);
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipStreamSynchronize(st));
+    ctx.pc_c = c;
+    ctx.pc_nwin = nwin;
+}
 
 // ---------------------------------------------------------------- engine
 
@@ -400,7 +446,7 @@ uint32_t auto_window(uint64_t n)
 }  // namespace
 
 void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::vector<MsmJob> &jobs,
-                    uint32_t window_bits, RawG1Projective *out, hipStream_t st)
+                    uint32_t window_bits, RawG1Projective *out, hipStream_t st, const PrecompTable *pc)
 {
     if (jobs.empty()) return;
     Buffers &b = *buf_;
@@ -411,8 +457,19 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
         total_pairs += j.n;
     }
     Geometry g;
-    g.c = window_bits ? window_bits : auto_window(max_n);
-    g.nwin = (256 + g.c - 1) / g.c;
+    if (pc) {
+        // one bucket set per MSM: window w reads the points [2^(c w)] P_i
+        g.c = pc->c;
+        g.nwin = pc->nwin;
+        g.kw = 1;
+        g.pstride = (uint32_t)pc->n;
+        d_points = pc->table;
+    } else {
+        g.c = window_bits ? window_bits : auto_window(max_n);
+        g.nwin = (256 + g.c - 1) / g.c;
+        g.kw = g.nwin;
+        g.pstride = 0;
+    }
     g.B = 1u << (g.c - 1);
     g.M = (uint32_t)jobs.size();
     last_window = g.c;
@@ -521,7 +578,7 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
     pl.first[g.c] = acc;
     for (uint32_t k = g.c + 1; k < 17; ++k) pl.first[k] = 0xffffffffu;
     pl.per_window = acc;
-    const uint64_t nmw = (uint64_t)g.M * g.nwin;
+    const uint64_t nmw = (uint64_t)g.M * g.kw;
     const uint64_t npp = nmw * pl.per_window;
     b.pp.ensure(npp * 48);
     hipLaunchKernelGGL(k_plane_partials, dim3(grid1(npp)), dim3(kBlock), 0, st, b.bucket.get(), g, pl, plane_group,
@@ -564,12 +621,12 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
 
     auto finish = [&](uint32_t m) {
         // exponent e = c*w + k  ->  plane sum
-        const uint32_t E = g.nwin * g.c;
+        const uint32_t E = g.kw * g.c;
         HJac accj = HJac::identity();
         for (uint32_t e = E; e-- > 0;) {
             accj = accj.dbl();
             uint32_t w = e / g.c, k = e % g.c;
-            uint64_t s = ((uint64_t)m * g.nwin + w) * g.c + k;
+            uint64_t s = ((uint64_t)m * g.kw + w) * g.c + k;
             for (uint32_t x = 0; x < plane_out[k]; ++x) accj = accj + hj[out_off[s] + x];
         }
         std::memcpy(&out[m], &accj, sizeof(RawG1Projective));
@@ -655,7 +712,12 @@ void run_batched(MsmContext &ctx, const std::vector<MsmJob> &jobs, RawG1Projecti
             ++j;
         }
         std::vector<MsmJob> part(jobs.begin() + i, jobs.begin() + j);
-        ctx.engine.run(ctx.points.get(), ctx.npoints, part, engine_window(ctx.config), out + i, st);
+        if (ctx.pc_nwin) {
+            PrecompTable pc{ctx.table.get(), ctx.npoints, ctx.pc_c, ctx.pc_nwin};
+            ctx.engine.run(ctx.points.get(), ctx.npoints, part, 0, out + i, st, &pc);
+        } else {
+            ctx.engine.run(ctx.points.get(), ctx.npoints, part, engine_window(ctx.config), out + i, st);
+        }
         i = j;
     }
 }
@@ -676,9 +738,12 @@ RawG1Projective identity_out()
     catch (const hip_error &e) { return make_error(e.code, e.what()); }                  \
     catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
 
-extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_create_with_config(void **out,
-                                                                                 const RawG1Affine *points,
-                                                                                 size_t npoints, RawMsmConfig config)
+namespace {
+
+// Resident contexts precompute per-window point copies (worth it because a
+// context serves many MSMs); the one-shot entry points do not.
+RustError create_context(void **out, const RawG1Affine *points, size_t npoints, RawMsmConfig config,
+                         bool precompute)
 {
     if (!out) return make_error((int)hipErrorInvalidValue, "out is null");
     *out = nullptr;
@@ -690,10 +755,20 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_create_with_config
     ctx->config = config;
     hipStream_t st = cur_stream();
     if (npoints) MH_HIP(hipMemcpyAsync(ctx->points.get(), points, npoints * 96, hipMemcpyHostToDevice, st));
+    if (precompute) build_precomp(*ctx, st);
     MH_HIP(hipStreamSynchronize(st));
     *out = ctx.release();
     return ok();
     MH_CATCH
+}
+
+}  // namespace
+
+extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_create_with_config(void **out,
+                                                                                 const RawG1Affine *points,
+                                                                                 size_t npoints, RawMsmConfig config)
+{
+    return create_context(out, points, npoints, config, true);
 }
 
 extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_create(void **out, const RawG1Affine *points,
@@ -869,7 +944,7 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_batch(RawG1Projective *out
                                                             const RawFr *scalars, size_t npoints, size_t nmsms)
 {
     void *h = nullptr;
-    RustError e = midnight_sppark_bls12_381_g1_msm_context_create(&h, points, npoints);
+    RustError e = create_context(&h, points, npoints, default_msm_config(npoints), false);
     if (e.code) return e;
     e = midnight_sppark_bls12_381_g1_msm_context_batch(out, h, scalars, npoints, nmsms);
     midnight_sppark_bls12_381_g1_msm_context_free(h);

@@ -19,13 +19,23 @@ struct MsmJob {
     size_t n;
 };
 
+// Per-window copies of a point set: row w * n + i holds [2^(c w)] P_i
+// (RawG1Affine), w < nwin = ceil(256 / c).  With it every window of an MSM
+// feeds one shared bucket set, and the window fold disappears.
+struct PrecompTable {
+    const uint32_t *table;
+    size_t n;
+    uint32_t c, nwin;
+};
+
 class MsmEngine {
 public:
     MsmEngine();
     ~MsmEngine();
     // d_points: RawG1Affine rows (24 u32 each).  window_bits 0 = automatic.
+    // With `pc`, the point rows come from the precomputed table instead.
     void run(const uint32_t *d_points, size_t npoints_total, const std::vector<MsmJob> &jobs, uint32_t window_bits,
-             RawG1Projective *out, hipStream_t st);
+             RawG1Projective *out, hipStream_t st, const PrecompTable *pc = nullptr);
 
     uint32_t chunk_len = 64;      // entries per accumulation lane
     uint32_t plane_group = 16;    // buckets per lane in the first plane pass

@@ -16,12 +16,18 @@ struct MsmContext {
     RawMsmConfig config{};
     size_t npoints = 0;
     DevBuf<uint32_t> points;   // npoints rows of RawG1Affine (24 u32)
+    // per-window copies [2^(c w)] P_i (w < pc_nwin), built once per context
+    DevBuf<uint32_t> table;
+    uint32_t pc_c = 0, pc_nwin = 0;
     std::mutex mu;
     MsmEngine engine;
 };
 
 RawMsmConfig default_msm_config(size_t npoints);
 std::unique_ptr<MsmContext> make_context(size_t npoints);
+// Builds the per-window table of a context whose points are on the device
+// (no-op for small point sets, which use plain windows).
+void build_precomp(MsmContext &ctx, hipStream_t st);
 void generator_multiples(uint32_t *out_points, const Fr *scalars, size_t n, hipStream_t st);
 
 }  // namespace mh

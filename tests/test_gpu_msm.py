@@ -200,3 +200,23 @@ def test_generator_multiples_and_montgomery(H, O):
     entries, window = ctx.last_stats()
     assert window >= 8 and 0 < entries <= n * (-(-256 // window))
     ctx.free()
+
+
+@pytest.mark.parametrize("log_n", [10, 14, 17])
+def test_msm_precomputed_context_modes(H, O, log_n):
+    """Resident contexts use per-window point copies; compare against the
+    one-shot (plain window) path and the closed form, with skewed and
+    uniform scalars, several MSMs per call."""
+    n = 1 << log_n
+    ts = O.random_fr(n, 300 + log_n)
+    dv_t = H.DeviceVec.upload(ts)
+    ctx = H.MsmContext.from_multiples(dv_t)
+    sc = O.random_fr(3 * n, 310 + log_n)
+    sc[n:2 * n:3] = 0
+    sc[2 * n:] = np.tile(O.fr_from_int(1), (n, 1))        # one bucket holds every point
+    out = ctx.batch(sc, n, 3)
+    for i in range(3):
+        assert O.g1_eq(out[i], O.g1_mul(O.generator(), O.inner_product(sc[i * n:(i + 1) * n], ts)))
+    pts = ctx.points()
+    assert O.g1_eq(H.g1_msm(pts, sc[:n]), out[0])
+    ctx.free()
