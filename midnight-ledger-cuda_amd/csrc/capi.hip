@@ -284,11 +284,13 @@ namespace {
 void coeff_to_extended_device(Fr *dst, const Fr *src, size_t input_len, size_t count, uint32_t ext_log_n,
                               const RawFr &first, const RawFr &second, hipStream_t st)
 {
-    DevBuf<Fr> scratch(ntt_scratch_elems(ext_log_n, count));
+    LaneScratch &ls = lane_scratch();
+    std::lock_guard<std::mutex> g(ls.mu);
+    ls.c.ensure(ntt_scratch_elems(ext_log_n, count));
     NttJob job;
     job.in = src;
     job.out = dst;
-    job.scratch = scratch.get();
+    job.scratch = ls.c.get();
     job.in_pstride = input_len;
     job.out_pstride = size_t(1) << ext_log_n;
     job.log_n = ext_log_n;
@@ -380,11 +382,14 @@ void device_ntt_zeta(DeviceVec *dv, uint32_t log_n, size_t count, bool inverse, 
 {
     hipStream_t st = cur_stream();
     const size_t n = size_t(1) << log_n;
-    DevBuf<Fr> scratch(ntt_scratch_elems(log_n, count));
+    // reusable per-lane scratch: no allocation per transform
+    LaneScratch &ls = lane_scratch();
+    std::lock_guard<std::mutex> g(ls.mu);
+    ls.c.ensure(ntt_scratch_elems(log_n, count));
     NttJob job;
     job.in = dv->values.get();
     job.out = dv->values.get();
-    job.scratch = scratch.get();
+    job.scratch = ls.c.get();
     job.in_pstride = job.out_pstride = n;
     job.log_n = log_n;
     job.count = (uint32_t)count;
@@ -399,7 +404,7 @@ void device_ntt_zeta(DeviceVec *dv, uint32_t log_n, size_t count, bool inverse, 
         job.factor_len = factor_len;
     }
     ntt_run(job, st);
-    // the scratch buffer is released on return: wait for the passes
+    // callers of the resident entry points see completed transforms
     MH_HIP(hipStreamSynchronize(st));
 }
 

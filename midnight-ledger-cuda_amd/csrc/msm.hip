@@ -97,8 +97,13 @@ __device__ __forceinline__ void canonical_scalar(const Fr *s, uint32_t k[9])
     k[8] = 0;
 }
 
+// The window count is a template parameter so that every digit of a scalar
+// is formed first and all of its atomics are in flight together, instead
+// of one atomic round trip per loop iteration.
+template <uint32_t C>
 __global__ void __launch_bounds__(kBlock) k_digits_hist(const MsmDesc *descs, Geometry g, uint32_t *cnt)
 {
+    constexpr uint32_t NW = (256 + C - 1) / C;
     const uint32_t m = blockIdx.y;
     const MsmDesc d = descs[m];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -107,16 +112,21 @@ __global__ void __launch_bounds__(kBlock) k_digits_hist(const MsmDesc *descs, Ge
     canonical_scalar(d.scalars + i, k);
     uint32_t carry = 0;
     const uint64_t base = (uint64_t)m * g.kw;
-    for (uint32_t w = 0; w < g.nwin; ++w) {
-        int32_t dg = window_digit(k, w, g.c, carry);
-        if (dg == 0) continue;
-        uint32_t b = (uint32_t)(dg < 0 ? -dg : dg) - 1;
+    int32_t dg[NW];
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) dg[w] = window_digit(k, w, C, carry);
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+        if (dg[w] == 0) continue;
+        uint32_t b = (uint32_t)(dg[w] < 0 ? -dg[w] : dg[w]) - 1;
         atomicAdd(&cnt[key_of(g, base, w, b)], 1u);
     }
 }
 
+template <uint32_t C>
 __global__ void __launch_bounds__(kBlock) k_scatter(const MsmDesc *descs, Geometry g, uint32_t *cursor, uint32_t *ent)
 {
+    constexpr uint32_t NW = (256 + C - 1) / C;
     const uint32_t m = blockIdx.y;
     const MsmDesc d = descs[m];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -126,13 +136,101 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const MsmDesc *descs, Geomet
     uint32_t carry = 0;
     const uint64_t base = (uint64_t)m * g.kw;
     const uint32_t pidx = d.pt_off + i;
-    for (uint32_t w = 0; w < g.nwin; ++w) {
-        int32_t dg = window_digit(k, w, g.c, carry);
-        if (dg == 0) continue;
-        uint32_t b = (uint32_t)(dg < 0 ? -dg : dg) - 1;
-        uint32_t pos = atomicAdd(&cursor[key_of(g, base, w, b)], 1u);
-        ent[pos] = (pidx + w * g.pstride) | (dg < 0 ? kSignBit : 0u);
+    int32_t dg[NW];
+    uint32_t pos[NW];
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) dg[w] = window_digit(k, w, C, carry);
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+        uint32_t b = (uint32_t)(dg[w] < 0 ? -dg[w] : dg[w]) - 1;
+        pos[w] = dg[w] ? atomicAdd(&cursor[key_of(g, base, w, b)], 1u) : 0u;
     }
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w)
+        if (dg[w]) ent[pos[w]] = (pidx + w * g.pstride) | (dg[w] < 0 ? kSignBit : 0u);
+}
+
+// Counting sort through LDS.  A block takes a tile of one MSM's scalars and
+// one range of that MSM's bucket keys (local key b * kw + w); it counts the
+// digits of the tile that fall in the range in an LDS histogram, so global
+// memory sees one atomic per (block, bucket) instead of one per digit:
+//   pass 0: cnt[key] += count          (bucket sizes)
+//   pass 1: base = cursor[key] += count, then each digit takes
+//           base + (LDS rank) as its slot.
+// Global atomics on this part are device-scope and are resolved past the
+// per-XCD L2s, so their count is what sets the sort's speed.
+constexpr uint32_t kSortThreads = 1024;
+
+template <uint32_t C>
+__global__ void __launch_bounds__(kSortThreads) k_sort_lds(const MsmDesc *descs, Geometry g, uint32_t tile,
+                                                           uint32_t range_keys, uint32_t *counters, uint32_t *ent,
+                                                           int scatter)
+{
+    constexpr uint32_t NW = (256 + C - 1) / C;
+    extern __shared__ uint32_t hist[];
+    const uint32_t m = blockIdx.y;
+    const MsmDesc d = descs[m];
+    const uint32_t i0 = blockIdx.x * tile;
+    if (i0 >= d.n) return;   // uniform for the whole block
+    const uint32_t i1 = min(i0 + tile, d.n);
+    const uint32_t lo = blockIdx.z * range_keys;
+    const uint64_t mkw = (uint64_t)m * g.kw;
+    for (uint32_t j = threadIdx.x; j < range_keys; j += blockDim.x) hist[j] = 0;
+    __syncthreads();
+    for (int round = 0; round < (scatter ? 2 : 1); ++round) {
+        for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            uint32_t k[9];
+            canonical_scalar(d.scalars + i, k);
+            uint32_t carry = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < NW; ++w) {
+                int32_t dg = window_digit(k, w, C, carry);
+                if (dg == 0) continue;
+                uint32_t b = (uint32_t)(dg < 0 ? -dg : dg) - 1;
+                uint32_t lkey = b * g.kw + (g.kw > 1 ? w : 0) - lo;
+                if (lkey >= range_keys) continue;
+                uint32_t r = atomicAdd(&hist[lkey], 1u);
+                if (round == 1)
+                    ent[r] = (d.pt_off + i + w * g.pstride) | (dg < 0 ? kSignBit : 0u);
+            }
+        }
+        __syncthreads();
+        // flush the histogram: sizes (pass 0) or slot reservation (pass 1)
+        for (uint32_t j = threadIdx.x; j < range_keys; j += blockDim.x) {
+            uint32_t n = hist[j];
+            if (round == 1 || n == 0) continue;
+            uint32_t lkey = lo + j;
+            uint64_t key = ((uint64_t)(lkey / g.kw) * g.M) * g.kw + mkw + lkey % g.kw;
+            uint32_t base = atomicAdd(&counters[key], n);
+            if (scatter) hist[j] = base;
+        }
+        __syncthreads();
+    }
+}
+
+#define MH_WINDOW_CASES(X) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)
+
+void launch_digits(bool scatter, uint32_t c, uint64_t max_n, hipStream_t st, const MsmDesc *descs, const Geometry &g,
+                   uint32_t *counters, uint32_t *ent)
+{
+    const uint32_t keys_per_msm = g.kw * g.B;
+    const uint32_t range_keys = std::min<uint32_t>(keys_per_msm, 16384);
+    const uint32_t ranges = (keys_per_msm + range_keys - 1) / range_keys;
+    const uint32_t tile = max_n >= (1u << 18) ? 16384u : 4096u;
+    dim3 grid((uint32_t)ceil_div(max_n, tile), g.M, ranges);
+    const size_t lds = (size_t)range_keys * 4;
+    switch (c) {
+#define MH_CASE(CC)                                                                                           \
+    case CC:                                                                                                  \
+        hipLaunchKernelGGL(k_sort_lds<CC>, grid, dim3(kSortThreads), lds, st, descs, g, tile, range_keys,     \
+                           counters, ent, scatter ? 1 : 0);                                                   \
+        break;
+        MH_WINDOW_CASES(MH_CASE)
+#undef MH_CASE
+    default:
+        throw hip_error((int)hipErrorInvalidValue, "unsupported MSM window width");
+    }
+    MH_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------- scan
@@ -491,15 +589,13 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
     MH_HIP(hipMemsetAsync(b.cnt.get(), 0, nkeys * 4, st));
     dim3 dgrid(grid1(max_n), g.M);
     if (max_n) {
-        hipLaunchKernelGGL(k_digits_hist, dgrid, dim3(kBlock), 0, st, b.descs.get(), g, b.cnt.get());
-        MH_HIP(hipGetLastError());
+        launch_digits(false, g.c, max_n, st, b.descs.get(), g, b.cnt.get(), nullptr);
     }
     const uint64_t nent = scan_exclusive(b.cnt.get(), b.off.get(), nkeys, b.scan_tmp, st);
     b.ent.ensure(std::max<uint64_t>(nent, 1));
     MH_HIP(hipMemcpyAsync(b.cursor.get(), b.off.get(), nkeys * 4, hipMemcpyDeviceToDevice, st));
     if (max_n) {
-        hipLaunchKernelGGL(k_scatter, dgrid, dim3(kBlock), 0, st, b.descs.get(), g, b.cursor.get(), b.ent.get());
-        MH_HIP(hipGetLastError());
+        launch_digits(true, g.c, max_n, st, b.descs.get(), g, b.cursor.get(), b.ent.get());
     }
 
     // 4-6: chunked accumulation into buckets
