@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "fields.hpp"
+#include "fp28.hpp"
 #include "runtime.hpp"
 
 namespace mh {
@@ -45,6 +46,58 @@ __global__ void __launch_bounds__(256) field_mul_loop(uint32_t *out, uint32_t it
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < C::N; ++k) s ^= a.v[k] ^ b.v[k];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256) fp28_mul_loop(uint32_t *out, uint32_t iters, int square)
+{
+    Fp28 a = fp28_one(), b = fp28_one();
+    a.v[0] ^= threadIdx.x;
+    b.v[1] ^= blockIdx.x;
+    for (uint32_t i = 0; i < iters; ++i) {
+        if (square) {
+            a = fp28_sqr(a);
+            b = fp28_sqr(b);
+        } else {
+            a = fp28_mul(a, b);
+            b = fp28_mul(b, a);
+        }
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 14; ++k) s ^= a.v[k] ^ b.v[k];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// bucket additions with the point operands in registers (no memory traffic)
+__global__ void __launch_bounds__(256) madd_loop(uint32_t *out, uint32_t iters, int radix28)
+{
+    Affine q;
+    q.x = Fp::one();
+    q.y = Fp::one();
+    q.x.v[0] ^= threadIdx.x * 7919u;
+    q.y.v[1] ^= blockIdx.x * 104729u;
+    uint32_t s = 0;
+    if (radix28) {
+        Affine28 q28 = affine28_from_std(q);
+        Xyzz28 acc = xyzz28_from_affine(q28);
+        acc.zz.v[0] ^= 1;
+        for (uint32_t i = 0; i < iters; ++i) {
+            acc = xyzz28_add_affine(acc, q28);
+            q28.x.v[2] ^= acc.x.v[3] & 1;
+        }
+#pragma unroll
+        for (int k = 0; k < 14; ++k) s ^= acc.x.v[k] ^ acc.zzz.v[k];
+    } else {
+        Xyzz acc = xyzz_from_affine(q);
+        acc.zz.v[0] ^= 1;
+        for (uint32_t i = 0; i < iters; ++i) {
+            acc = xyzz_add_affine(acc, q);
+            q.x.v[2] ^= acc.x.v[3] & 1;
+        }
+#pragma unroll
+        for (int k = 0; k < 12; ++k) s ^= acc.x.v[k] ^ acc.zzz.v[k];
+    }
     out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
@@ -124,6 +177,52 @@ extern "C" RustError midnight_hip_field_mul_rate(int blocks, double *fp_per_s, d
         double per = (double)g_blocks * (g_blocks == 1 ? 64 : 256) * kMulIters * 2;
         if (fp_per_s) *fp_per_s = per / time_launch(run_fp, st);
         if (fr_per_s) *fr_per_s = per / time_launch(run_fr, st);
+        return ok();
+    } catch (const hip_error &e) {
+        return make_error(e.code, e.what());
+    }
+}
+
+namespace mh {
+namespace {
+int g_variant = 0;
+constexpr uint32_t kMaddIters = 16;
+void run_fp28(hipStream_t st)
+{
+    hipLaunchKernelGGL(fp28_mul_loop, dim3(g_blocks), dim3(256), 0, st, reinterpret_cast<uint32_t *>(g_sink),
+                       kMulIters, g_variant);
+}
+void run_madd(hipStream_t st)
+{
+    hipLaunchKernelGGL(madd_loop, dim3(g_blocks), dim3(256), 0, st, reinterpret_cast<uint32_t *>(g_sink), kMaddIters,
+                       g_variant);
+}
+}  // namespace
+}  // namespace mh
+
+// Throughput of the two Fp representations in isolation, per second over
+// the whole device: out[0] Fp28 products, out[1] Fp28 squarings,
+// out[2] XYZZ mixed additions (32-bit limbs), out[3] the same in Fp28.
+extern "C" RustError midnight_hip_field_rates28(double out[4])
+{
+    using namespace mh;
+    try {
+        hipDeviceProp_t prop;
+        MH_HIP(hipGetDeviceProperties(&prop, device_id()));
+        if (!g_sink)
+            MH_HIP(hipMalloc(reinterpret_cast<void **>(&g_sink), (size_t)prop.multiProcessorCount * 8 * 256 * 8));
+        g_blocks = prop.multiProcessorCount * 8;
+        hipStream_t st = cur_stream();
+        const double per_mul = (double)g_blocks * 256 * kMulIters * 2;
+        const double per_madd = (double)g_blocks * 256 * kMaddIters;
+        g_variant = 0;
+        out[0] = per_mul / time_launch(run_fp28, st);
+        g_variant = 1;
+        out[1] = per_mul / time_launch(run_fp28, st);
+        g_variant = 0;
+        out[2] = per_madd / time_launch(run_madd, st);
+        g_variant = 1;
+        out[3] = per_madd / time_launch(run_madd, st);
         return ok();
     } catch (const hip_error &e) {
         return make_error(e.code, e.what());

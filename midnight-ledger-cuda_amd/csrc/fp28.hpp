@@ -1,0 +1,405 @@
+// BLS12-381 Fp in radix 2^28 for the MSM accumulation loop.
+//
+// Fourteen 28-bit limbs (392 bits) in Montgomery form with R = 2^392.  The
+// product is finely integrated product scanning (FIPS; Koc, Acar, Kaliski
+// 1996): every column adds 28-bit x 28-bit products into one 64-bit
+// accumulator, which v_mad_u64_u32 does in a single instruction per
+// product with the accumulator itself as the 64-bit addend -- no carry flag,
+// no zero-extension.  A column holds at most 28 products below 2^56, so the
+// accumulator cannot overflow.  With 32-bit limbs every product also needs a
+// carry-propagating add (and a zero high word), about 1.6x the instructions.
+// Values are kept fully reduced in [0, p), so zero tests and equality are
+// limb comparisons.
+//
+// Conversions to and from the 32-bit, R = 2^384 form used everywhere else
+// (the reference's RawG1Affine layout) are a Montgomery product by
+// 2^400 mod p (in) or 2^384 mod p (out) plus a bit repacking.
+#pragma once
+#include "curve.hpp"
+#include "fields.hpp"
+
+namespace mh {
+
+struct Fp28 {
+    uint32_t v[14];
+};
+
+constexpr uint32_t kM28 = 0x0fffffffu;
+constexpr uint32_t kPinv28 = 0x0ffcfffdu;   // -p^-1 mod 2^28
+
+__device__ __forceinline__ uint32_t p28(int i)
+{
+    constexpr uint32_t P[14] = {0xfffaaabu, 0xfefffffu, 0x3ffffb9u, 0xfffeb15u, 0x6241eabu, 0xa0f6b0fu, 0xf6730d2u,
+                                0xf38512bu, 0x4774b84u, 0x4bacd76u, 0xba7b643u, 0xe69a4b1u, 0x1ea397fu, 0x001a011u};
+    return P[i];
+}
+
+__device__ __forceinline__ Fp28 fp28_const(int which)
+{
+    // 0: 2^400 mod p (std -> internal), 1: 2^384 mod p (internal -> std)
+    constexpr uint32_t TO_INT[14] = {0x80e6299u, 0x3500034u, 0xeb12856u, 0xdeb2699u, 0xc988670u,
+                                     0x4ef6697u, 0x70983e8u, 0xa4e6fe9u, 0x3e8a053u, 0xecf271eu,
+                                     0xc20d323u, 0x6eb6385u, 0x47f1286u, 0x00156dau};
+    constexpr uint32_t TO_STD[14] = {0x002fffdu, 0x0900000u, 0xc000276u, 0x000bc40u, 0x8baebf4u,
+                                     0x5753c75u, 0x55f4898u, 0x7052574u, 0x7ce5853u, 0x56ec6d7u,
+                                     0x71a97a2u, 0xe4935c0u, 0xec3fa80u, 0x0015f65u};
+    Fp28 r;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) r.v[i] = which == 0 ? TO_INT[i] : TO_STD[i];
+    return r;
+}
+
+__device__ __forceinline__ Fp28 fp28_one()
+{
+    constexpr uint32_t O[14] = {0x347fcb8u, 0xd800000u, 0x002b119u, 0x0cde6d2u, 0xc7212e0u, 0x83a2090u, 0x037669fu,
+                                0xda0f73eu, 0x9b09b42u, 0x1297bb0u, 0x515d98fu, 0x012ca7cu, 0x659fcfau, 0x000577au};
+    Fp28 r;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) r.v[i] = O[i];
+    return r;
+}
+
+__device__ __forceinline__ Fp28 fp28_zero()
+{
+    Fp28 r;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) r.v[i] = 0;
+    return r;
+}
+
+__device__ __forceinline__ bool fp28_is_zero(const Fp28 &a)
+{
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) o |= a.v[i];
+    return o == 0;
+}
+
+// x - p if x >= p (x < 2p, limbs < 2^28 except the top, which is < 2^19)
+__device__ __forceinline__ void fp28_reduce(Fp28 &x)
+{
+    uint32_t d[14];
+    int32_t bw = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        int32_t t = (int32_t)x.v[i] - (int32_t)p28(i) - bw;
+        bw = (int32_t)((uint32_t)t >> 31);
+        d[i] = (uint32_t)t & kM28;
+    }
+#pragma unroll
+    for (int i = 0; i < 14; ++i) x.v[i] = bw ? x.v[i] : d[i];
+}
+
+// Row-oriented form (CIOS order) with 64-bit lazy accumulators: row i adds
+// a_j * b_i and m * p_j into fourteen independent accumulators (28
+// independent multiply-adds), so consecutive instructions do not wait on
+// one another; only the 28-bit carry of the lowest word moves up per row.
+// Each accumulator collects at most 28 products below 2^56 (< 2^61).
+__device__ __forceinline__ Fp28 fp28_mul(const Fp28 &a, const Fp28 &b)
+{
+    uint64_t t[14];
+#pragma unroll
+    for (int j = 0; j < 14; ++j) t[j] = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+#pragma unroll
+        for (int j = 0; j < 14; ++j) t[j] += (uint64_t)a.v[j] * b.v[i];
+        const uint32_t m = ((uint32_t)t[0] * kPinv28) & kM28;
+#pragma unroll
+        for (int j = 0; j < 14; ++j) t[j] += (uint64_t)m * p28(j);
+        // t[0] is now a multiple of 2^28: shift the row down one limb
+        const uint64_t carry = t[0] >> 28;
+#pragma unroll
+        for (int j = 0; j < 13; ++j) t[j] = t[j + 1];
+        t[0] += carry;
+        t[13] = 0;
+    }
+    Fp28 r;
+    uint64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 14; ++j) {
+        c += t[j];
+        r.v[j] = (uint32_t)c & kM28;
+        c >>= 28;
+    }
+    fp28_reduce(r);
+    return r;
+}
+
+// Column-oriented form (finely integrated product scanning), kept for
+// squaring where the symmetric cross products halve the multiplies.
+__device__ __forceinline__ Fp28 fp28_mul_fips(const Fp28 &a, const Fp28 &b)
+{
+    uint64_t acc = 0;
+    uint32_t m[14];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) {
+        uint64_t e = 0, o = 0, q = 0;
+#pragma unroll
+        for (int i = 0; i <= k; ++i) {
+            if (i & 1) o += (uint64_t)a.v[i] * b.v[k - i];
+            else e += (uint64_t)a.v[i] * b.v[k - i];
+        }
+#pragma unroll
+        for (int i = 0; i < k; ++i) q += (uint64_t)m[i] * p28(k - i);
+        acc += e + o + q;
+        m[k] = ((uint32_t)acc * kPinv28) & kM28;
+        acc += (uint64_t)m[k] * p28(0);
+        acc >>= 28;
+    }
+    Fp28 r;
+#pragma unroll
+    for (int k = 14; k < 27; ++k) {
+        uint64_t e = 0, o = 0, q = 0;
+#pragma unroll
+        for (int i = k - 13; i < 14; ++i) {
+            if (i & 1) o += (uint64_t)a.v[i] * b.v[k - i];
+            else e += (uint64_t)a.v[i] * b.v[k - i];
+        }
+#pragma unroll
+        for (int i = k - 13; i < 14; ++i) q += (uint64_t)m[i] * p28(k - i);
+        acc += e + o + q;
+        r.v[k - 14] = (uint32_t)acc & kM28;
+        acc >>= 28;
+    }
+    r.v[13] = (uint32_t)acc;
+    fp28_reduce(r);
+    return r;
+}
+
+// squaring: cross products once, doubled, plus the diagonal
+__device__ __forceinline__ Fp28 fp28_sqr(const Fp28 &a)
+{
+    uint64_t acc = 0;
+    uint32_t m[14];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) {
+        uint64_t cross = 0, q = 0, q2 = 0;
+#pragma unroll
+        for (int i = 0; 2 * i < k; ++i) cross += (uint64_t)a.v[i] * a.v[k - i];
+        if ((k & 1) == 0) q2 = (uint64_t)a.v[k / 2] * a.v[k / 2];
+#pragma unroll
+        for (int i = 0; i < k; ++i) q += (uint64_t)m[i] * p28(k - i);
+        acc += (cross << 1) + q2 + q;
+        m[k] = ((uint32_t)acc * kPinv28) & kM28;
+        acc += (uint64_t)m[k] * p28(0);
+        acc >>= 28;
+    }
+    Fp28 r;
+#pragma unroll
+    for (int k = 14; k < 27; ++k) {
+        uint64_t cross = 0, q = 0, q2 = 0;
+#pragma unroll
+        for (int i = k - 13; 2 * i < k; ++i) cross += (uint64_t)a.v[i] * a.v[k - i];
+        if ((k & 1) == 0) q2 = (uint64_t)a.v[k / 2] * a.v[k / 2];
+#pragma unroll
+        for (int i = k - 13; i < 14; ++i) q += (uint64_t)m[i] * p28(k - i);
+        acc += (cross << 1) + q2 + q;
+        r.v[k - 14] = (uint32_t)acc & kM28;
+        acc >>= 28;
+    }
+    r.v[13] = (uint32_t)acc;
+    fp28_reduce(r);
+    return r;
+}
+
+__device__ __forceinline__ Fp28 fp28_add(const Fp28 &a, const Fp28 &b)
+{
+    Fp28 r;
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        uint32_t s = a.v[i] + b.v[i] + c;
+        c = s >> 28;
+        r.v[i] = s & kM28;
+    }
+    fp28_reduce(r);
+    return r;
+}
+
+__device__ __forceinline__ Fp28 fp28_sub(const Fp28 &a, const Fp28 &b)
+{
+    Fp28 r;
+    int32_t bw = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        int32_t t = (int32_t)a.v[i] - (int32_t)b.v[i] - bw;
+        bw = (int32_t)((uint32_t)t >> 31);
+        r.v[i] = (uint32_t)t & kM28;
+    }
+    // borrowed: add p back
+    const uint32_t mask = 0u - (uint32_t)bw;
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        uint32_t s = r.v[i] + (p28(i) & mask) + c;
+        c = s >> 28;
+        r.v[i] = s & kM28;
+    }
+    return r;
+}
+
+__device__ __forceinline__ Fp28 fp28_dbl(const Fp28 &a) { return fp28_add(a, a); }
+__device__ __forceinline__ Fp28 fp28_neg(const Fp28 &a) { return fp28_sub(fp28_zero(), a); }
+
+// ---- repacking between 12 x 32-bit and 14 x 28-bit limbs (same integer)
+
+__device__ __forceinline__ Fp28 repack_to28(const Fp &x)
+{
+    Fp28 r;
+#pragma unroll
+    for (int j = 0; j < 14; ++j) {
+        const int bit = 28 * j, q = bit >> 5, s = bit & 31;
+        uint32_t lo = x.v[q];
+        uint32_t hi = q + 1 < 12 ? x.v[q + 1] : 0u;
+        uint64_t pair = (uint64_t)lo | ((uint64_t)hi << 32);
+        r.v[j] = (uint32_t)(pair >> s) & kM28;
+    }
+    return r;
+}
+
+__device__ __forceinline__ Fp repack_to32(const Fp28 &x)
+{
+    Fp r;
+#pragma unroll
+    for (int w = 0; w < 12; ++w) {
+        const int bit = 32 * w, j = bit / 28, s = bit % 28;
+        uint32_t lo = x.v[j] >> s;
+        uint32_t hi = j + 1 < 14 ? (x.v[j + 1] << (28 - s)) : 0u;
+        r.v[w] = lo | hi;
+    }
+    return r;
+}
+
+__device__ __forceinline__ Fp28 fp28_from_std(const Fp &x) { return fp28_mul(repack_to28(x), fp28_const(0)); }
+__device__ __forceinline__ Fp fp28_to_std(const Fp28 &x) { return repack_to32(fp28_mul(x, fp28_const(1))); }
+
+// ---- points
+
+struct Affine28 {
+    Fp28 x, y;
+    __device__ __forceinline__ bool is_inf() const { return fp28_is_zero(x) && fp28_is_zero(y); }
+};
+
+struct Xyzz28 {
+    Fp28 x, y, zz, zzz;
+    __device__ __forceinline__ bool is_inf() const { return fp28_is_zero(zz); }
+};
+
+__device__ __forceinline__ Xyzz28 xyzz28_inf()
+{
+    Xyzz28 r;
+    r.x = fp28_one();
+    r.y = fp28_one();
+    r.zz = fp28_zero();
+    r.zzz = fp28_zero();
+    return r;
+}
+
+__device__ __forceinline__ Xyzz28 xyzz28_from_affine(const Affine28 &a)
+{
+    if (a.is_inf()) return xyzz28_inf();
+    Xyzz28 r;
+    r.x = a.x;
+    r.y = a.y;
+    r.zz = fp28_one();
+    r.zzz = fp28_one();
+    return r;
+}
+
+// mdbl-2008-s-1 on an affine point (not the identity)
+__device__ __forceinline__ Xyzz28 xyzz28_dbl_affine(const Affine28 &q)
+{
+    if (fp28_is_zero(q.y)) return xyzz28_inf();
+    Fp28 u = fp28_dbl(q.y);
+    Fp28 v = fp28_sqr(u);
+    Fp28 w = fp28_mul(u, v);
+    Fp28 s = fp28_mul(q.x, v);
+    Fp28 x2 = fp28_sqr(q.x);
+    Fp28 m = fp28_add(x2, fp28_dbl(x2));
+    Xyzz28 r;
+    r.x = fp28_sub(fp28_sqr(m), fp28_dbl(s));
+    r.y = fp28_sub(fp28_mul(m, fp28_sub(s, r.x)), fp28_mul(w, q.y));
+    r.zz = v;
+    r.zzz = w;
+    return r;
+}
+
+// madd-2008-s: p + q with q affine (identity encoded as 0, 0)
+__device__ __forceinline__ Xyzz28 xyzz28_add_affine(const Xyzz28 &p, const Affine28 &q)
+{
+    if (q.is_inf()) return p;
+    if (p.is_inf()) return xyzz28_from_affine(q);
+    Fp28 u2 = fp28_mul(q.x, p.zz);
+    Fp28 s2 = fp28_mul(q.y, p.zzz);
+    Fp28 pp_ = fp28_sub(u2, p.x);
+    Fp28 r_ = fp28_sub(s2, p.y);
+    if (fp28_is_zero(pp_)) {
+        if (fp28_is_zero(r_)) return xyzz28_dbl_affine(q);
+        return xyzz28_inf();
+    }
+    Fp28 pp = fp28_sqr(pp_);
+    Fp28 ppp = fp28_mul(pp_, pp);
+    Fp28 qq = fp28_mul(p.x, pp);
+    Xyzz28 r;
+    r.x = fp28_sub(fp28_sub(fp28_sqr(r_), ppp), fp28_dbl(qq));
+    r.y = fp28_sub(fp28_mul(r_, fp28_sub(qq, r.x)), fp28_mul(p.y, ppp));
+    r.zz = fp28_mul(p.zz, pp);
+    r.zzz = fp28_mul(p.zzz, ppp);
+    return r;
+}
+
+__device__ __forceinline__ Affine28 load_affine28(const uint32_t *p)
+{
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    Affine28 a;
+    uint32_t t[28];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        uint4 u = q[i];
+        t[4 * i] = u.x;
+        t[4 * i + 1] = u.y;
+        t[4 * i + 2] = u.z;
+        t[4 * i + 3] = u.w;
+    }
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        a.x.v[i] = t[i];
+        a.y.v[i] = t[14 + i];
+    }
+    return a;
+}
+
+__device__ __forceinline__ void store_affine28(uint32_t *p, const Affine28 &a)
+{
+    uint4 *q = reinterpret_cast<uint4 *>(p);
+    uint32_t t[28];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        t[i] = a.x.v[i];
+        t[14 + i] = a.y.v[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 7; ++i) q[i] = make_uint4(t[4 * i], t[4 * i + 1], t[4 * i + 2], t[4 * i + 3]);
+}
+
+__device__ __forceinline__ Affine28 affine28_from_std(const Affine &a)
+{
+    Affine28 r;
+    r.x = fp28_from_std(a.x);
+    r.y = fp28_from_std(a.y);
+    return r;
+}
+
+__device__ __forceinline__ Xyzz xyzz28_to_std(const Xyzz28 &p)
+{
+    if (p.is_inf()) return Xyzz::inf();
+    Xyzz r;
+    r.x = fp28_to_std(p.x);
+    r.y = fp28_to_std(p.y);
+    r.zz = fp28_to_std(p.zz);
+    r.zzz = fp28_to_std(p.zzz);
+    return r;
+}
+
+}  // namespace mh

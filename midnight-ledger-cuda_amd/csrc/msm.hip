@@ -35,6 +35,7 @@
 
 #include "curve.hpp"
 #include "devvec.hpp"
+#include "fp28.hpp"
 #include "frops.hpp"
 #include "host_g1.hpp"
 #include "msm.hpp"
@@ -321,6 +322,39 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(const uint32_t *pts, cons
     else store_xyzz(part + (uint64_t)j * 48, acc);
 }
 
+// Same pass over a precomputed table held in radix 2^28 (28 u32 per point):
+// the bucket sum runs in Fp28 and is converted once at the end.
+__device__ __forceinline__ Affine28 fetch_point28(const uint32_t *pts, uint32_t e)
+{
+    Affine28 p = load_affine28(pts + (uint64_t)(e & ~kSignBit) * 28);
+    if (e & kSignBit) p.y = fp28_neg(p.y);
+    return p;
+}
+
+__global__ void __launch_bounds__(kBlock) k_accumulate28(const uint32_t *pts, const uint32_t *ent, const uint32_t *cnt,
+                                                         const uint32_t *off, const uint32_t *q, const uint32_t *co,
+                                                         const uint32_t *cmap, uint32_t nchunks, uint32_t L,
+                                                         uint32_t *part, uint32_t *bucket)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nchunks) return;
+    const uint32_t b = cmap[j];
+    const uint32_t r = j - co[b];
+    const uint32_t start = off[b] + r * L;
+    const uint32_t end = min(start + L, off[b] + cnt[b]);
+    Xyzz28 acc = xyzz28_inf();
+    // the next row is in flight while the current one is added
+    Affine28 nxt = fetch_point28(pts, ent[start]);
+    for (uint32_t x = start; x < end; ++x) {
+        Affine28 cur = nxt;
+        if (x + 1 < end) nxt = fetch_point28(pts, ent[x + 1]);
+        acc = xyzz28_add_affine(acc, cur);
+    }
+    const Xyzz out = xyzz28_to_std(acc);
+    if (q[b] == 1) store_xyzz(bucket + (uint64_t)b * 48, out);
+    else store_xyzz(part + (uint64_t)j * 48, out);
+}
+
 // buckets finished by the accumulate pass (one chunk) need no more work
 __global__ void k_retire_single(uint32_t *q, uint64_t nkeys, uint32_t *maxq)
 {
@@ -423,18 +457,19 @@ __global__ void __launch_bounds__(kBlock) k_group_sums(const uint32_t *in, uint6
     else store_xyzz(out + id * 48, acc);
 }
 
-// row w * n + i of `out` = [2^(c w)] P_i, by c doublings per window
+// row w * n + i of `out` = [2^(c w)] P_i, by c doublings per window, stored
+// in radix 2^28 (Affine28) for the accumulation pass
 __global__ void __launch_bounds__(kBlock) k_precompute(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t c,
                                                        uint32_t nwin)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Affine a = load_affine(in + (uint64_t)i * 24);
-    store_affine(out + (uint64_t)i * 24, a);
+    store_affine28(out + (uint64_t)i * 28, affine28_from_std(a));
     Xyzz x = xyzz_from_affine(a);
     for (uint32_t w = 1; w < nwin; ++w) {
         for (uint32_t k = 0; k < c; ++k) x = xyzz_dbl(x);
-        store_affine(out + ((uint64_t)w * n + i) * 24, xyzz_to_affine(x));
+        store_affine28(out + ((uint64_t)w * n + i) * 28, affine28_from_std(xyzz_to_affine(x)));
     }
 }
 
@@ -451,7 +486,7 @@ void build_precomp(MsmContext &ctx, hipStream_t st)
     uint32_t c = (uint32_t)std::min(20, std::max(10, lg - 4));
     uint32_t nwin = (256 + c - 1) / c;
     if ((uint64_t)nwin * n >= (1ull << 31)) return;   // entry indices are 31-bit
-    ctx.table.alloc((size_t)nwin * n * 24);
+    ctx.table.alloc((size_t)nwin * n * 28);   // Affine28 rows
     hipLaunchKernelGGL(k_precompute, dim3(grid1(n)), dim3(kBlock), 0, st, ctx.points.get(), ctx.table.get(),
                        (uint32_t)n, c, nwin);
     MH_HIP(hipGetLastError());
@@ -615,9 +650,14 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
                            nkeys);
         MH_HIP(hipGetLastError());
         prof.begin("msm_accumulate", st);
-        hipLaunchKernelGGL(k_accumulate, dim3(grid1(nchunks)), dim3(kBlock), 0, st, d_points, b.ent.get(), b.cnt.get(),
-                           b.off.get(), b.q.get(), b.co.get(), b.cmap.get(), (uint32_t)nchunks, L, b.part.get(),
-                           b.bucket.get());
+        if (pc)
+            hipLaunchKernelGGL(k_accumulate28, dim3(grid1(nchunks)), dim3(kBlock), 0, st, d_points, b.ent.get(),
+                               b.cnt.get(), b.off.get(), b.q.get(), b.co.get(), b.cmap.get(), (uint32_t)nchunks, L,
+                               b.part.get(), b.bucket.get());
+        else
+            hipLaunchKernelGGL(k_accumulate, dim3(grid1(nchunks)), dim3(kBlock), 0, st, d_points, b.ent.get(),
+                               b.cnt.get(), b.off.get(), b.q.get(), b.co.get(), b.cmap.get(), (uint32_t)nchunks, L,
+                               b.part.get(), b.bucket.get());
         MH_HIP(hipGetLastError());
         prof.end(st);
         last_entries = nent;
