@@ -477,28 +477,36 @@ extern "C" RustError midnight_hip_fr_device_vec_ntt_multi(void *device_vec, cons
     }
     MH_TRY
     hipStream_t st = cur_stream();
-    // group transforms by size; polys of one size that sit back to back share launches
+    // Runs of consecutive, back-to-back transforms of one size share their
+    // launches (one grid row per polynomial); callers that lay polynomials
+    // out by size get one launch set per distinct size.
+    struct Run {
+        size_t first, count;
+    };
+    std::vector<Run> runs;
     size_t max_scratch = 0;
-    for (size_t i = 0; i < count; ++i) max_scratch = std::max(max_scratch, ntt_scratch_elems(log_ns[i], 1));
-    std::vector<size_t> order(count);
-    for (size_t i = 0; i < count; ++i) order[i] = i;
-    DevBuf<Fr> scratch(max_scratch * 8);
-    size_t i = 0;
-    while (i < count) {
-        // run of consecutive, contiguous, equal-size transforms (up to 8 per launch set)
+    for (size_t i = 0; i < count;) {
         size_t j = i + 1;
         const size_t n = size_t(1) << log_ns[i];
-        while (j < count && j - i < 8 && log_ns[j] == log_ns[i] && offsets[j] == offsets[j - 1] + n) ++j;
+        while (j < count && j - i < 65535 && log_ns[j] == log_ns[i] && offsets[j] == offsets[j - 1] + n) ++j;
+        runs.push_back(Run{i, j - i});
+        max_scratch = std::max(max_scratch, ntt_scratch_elems(log_ns[i], j - i));
+        i = j;
+    }
+    LaneScratch &ls = lane_scratch();
+    std::lock_guard<std::mutex> g(ls.mu);
+    ls.c.ensure(max_scratch);
+    for (const Run &r : runs) {
+        const size_t n = size_t(1) << log_ns[r.first];
         NttJob job;
-        job.in = dv->values.get() + offsets[i];
-        job.out = dv->values.get() + offsets[i];
-        job.scratch = scratch.get();
+        job.in = dv->values.get() + offsets[r.first];
+        job.out = dv->values.get() + offsets[r.first];
+        job.scratch = ls.c.get();
         job.in_pstride = job.out_pstride = n;
-        job.log_n = log_ns[i];
-        job.count = (uint32_t)(j - i);
+        job.log_n = log_ns[r.first];
+        job.count = (uint32_t)r.count;
         job.inverse = inverse;
         ntt_run(job, st);
-        i = j;
     }
     MH_HIP(hipStreamSynchronize(st));
     return ok();
