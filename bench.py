@@ -266,12 +266,113 @@ def bench_batch(args, H):
     return res
 
 
+QUOTIENT_FR_MULS_PER_ROW = 52      # gate 3, permutation 49 (three sets over 8 columns, chunk 3)
+MACS_PER_FR_MUL = 136              # 2 * 8^2 + 8 32x32->64 products per Montgomery product
+QUOTIENT_BYTES_PER_ROW = 23 * 32   # 5 advice + 3 fixed + 8 sigma + 3 z + l0, l_last, l_active + output
+
+
+def bench_quotient(args, H):
+    """Quotient numerator of the benchmark circuit (k = 20: 5 advice,
+    3 fixed, gate q*(a*b + c - d), 8 permutation columns in 3 sets) on its
+    extended domain, 2^log_n rows (default 22).  A step is one resident
+    numerator_context_eval: inputs resident in HBM, output a new device
+    vector.  Column values are uniform synthetic Fr (the kernel's work does
+    not depend on them)."""
+    import numerator_graph as G
+    log_n = args.log_n if args.log_n != 20 else 22
+    n = 1 << log_n
+    rng = np.random.default_rng(args.seed)
+    log("generating %d x 2^%d synthetic columns" % (8 + 14, log_n))
+    cols = device_fr(H, random_canonical(n * 8, rng))          # advice a..e | z_0..z_2
+    key = [random_canonical(n, rng) for _ in range(14)]         # q s1 s2 | sigma x8 | l0 l_last l_active
+    scal = random_canonical(8, rng)
+    gate = ("prod", ("fixed", 0, 0), ("sum", ("sum", ("prod", ("advice", 0, 0), ("advice", 1, 0)),
+                                              ("advice", 2, 0)), ("neg", ("advice", 3, 0))))
+    graph = G.RawGraphBuffers(G.custom_gates_graph([gate]))
+    perm_cols = [(1, i) for i in range(5)] + [(0, i) for i in range(3)]
+    ctx = H.NumeratorContext(
+        n, n >> 2, fixed=np.concatenate(key[:3]), fixed_lagrange=np.concatenate(key[:3])[: 3 * (n >> 2)],
+        l0=key[11], l_last=key[12], l_active=key[13], permutation_columns=perm_cols,
+        permutation_pk_cosets=np.concatenate(key[3:11]),
+        permutation_pk_lagrange=np.concatenate(key[3:11])[: 8 * (n >> 2)], custom_graph=graph,
+        omega=scal[0], trace_omega=scal[1], g_coset=scal[2], delta_const=scal[3], rot_scale=4,
+        last_rotation=-6, chunk_len=3)
+    del key
+    ch = dict(beta=scal[4], gamma=scal[5], theta=scal[6], trash_challenge=scal[7], y=scal[0])
+
+    def step():
+        out = ctx.eval(cols, 8, advice_offset=0, advice_cols=5, permutation_offset=5, permutation_sets=3, **ch)
+        out.free()
+
+    for _ in range(args.warmup):
+        step()
+    H.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dt = (time.perf_counter() - t0) / args.steps
+    k_ms, k_cnt = H.profile_read("numerator")
+    H.profile_enable(False)
+    t_kernel = k_ms / 1e3 / max(k_cnt, 1)
+    peak, _, _ = H.valu_peak()
+    macs = QUOTIENT_FR_MULS_PER_ROW * MACS_PER_FR_MUL * n
+    bytes_alg = QUOTIENT_BYTES_PER_ROW * n
+    res = {
+        "metric": "quotient numerator Mrows/s @2^%d extended rows" % log_n,
+        "value": round(n / dt / 1e6, 3), "unit": "Mrows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic uniform Fr columns, resident in HBM",
+        "config": {"workload": "numerator of the k=%d circuit (5 advice, 3 fixed, gate q(ab+c-d), "
+                               "8-column permutation in 3 sets) on 2^%d extended rows" % (log_n - 2, log_n),
+                   "rows": n},
+        "roofline": {"bound": "valu", "kernel": "numerator",
+                     "achieved": round(macs / t_kernel / 1e12, 3), "peak": round(peak / 1e12, 3),
+                     "unit": "TMAC/s", "frac": round(macs / t_kernel / peak, 4), "traffic": None},
+        "hbm": {"achieved": round(bytes_alg / t_kernel / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(bytes_alg / t_kernel / HBM_PEAK, 4)},
+        "kernel_ms_avg": round(t_kernel * 1e3, 3),
+    }
+    if args.cpu_seconds > 0:
+        log("timing the CPU baseline")
+        res["cpu_baseline"] = cpu_baseline_quotient(args.cpu_seconds)
+    ctx.free()
+    cols.free()
+    return res
+
+
+def cpu_baseline_quotient(seconds):
+    """Oracle evaluate_numerator (row-parallel C restatement) on a bounded
+    sample: the same circuit shape over 2^18 rows, repeated for about
+    `seconds`."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+    import synth_circuit
+    threads = os.cpu_count() or 1
+    log_s = 18
+    inp = synth_circuit.random_inputs(1 << log_s, 5)
+    oracle.numerator(**dict(inp, n=1024, fixed=inp["fixed"][:3 * 1024], advice=inp["advice"][:5 * 1024],
+                            l0=inp["l0"][:1024], l_last=inp["l_last"][:1024], l_active=inp["l_active"][:1024],
+                            permutation_products=inp["permutation_products"][:3 * 1024],
+                            permutation_pk_cosets=inp["permutation_pk_cosets"][:8 * 1024]), threads=threads)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.numerator(**inp, threads=threads)
+        reps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round((1 << log_s) * reps / dt / 1e6, 4), "unit": "Mrows/s", "cores": threads, "kind": "port",
+            "sample": "%d x numerator over 2^%d rows of the same circuit shape, oracle evaluate_numerator, "
+                      "%d threads, %.1f s" % (reps, log_s, threads, dt)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="msm", choices=["msm", "ntt", "batch"])
+    ap.add_argument("--workload", default="msm", choices=["msm", "ntt", "batch", "quotient"])
     ap.add_argument("--log-n", type=int, default=20)
     ap.add_argument("--msms-per-step", type=int, default=4)
     ap.add_argument("--skewed", action="store_true")
@@ -286,7 +387,7 @@ def main():
         return
     import midnight_hip as H
     H.lib()
-    run = {"msm": bench_msm, "ntt": bench_ntt, "batch": bench_batch}[args.workload]
+    run = {"msm": bench_msm, "ntt": bench_ntt, "batch": bench_batch, "quotient": bench_quotient}[args.workload]
     res = run(args, H)
     print(json.dumps(res), flush=True)
 

@@ -203,6 +203,117 @@ RustError midnight_sppark_bls12_381_fr_device_quotient_pieces(void **out,
                                                               const RawFr *blinds,
                                                               size_t blind_count);
 
+/* ---------------- quotient numerator ---------------- */
+
+/* Compiled expression graph (bls12_381_g1_msm.cu:351-376; built by the
+ * prover's graph compiler, vendor/midnight-proofs-0.8.1/src/plonk/
+ * evaluation.rs:677-800).  Source kinds: 0 constant[a], 1 slot[a],
+ * 2 fixed / 3 advice / 4 instance column a at rotations[b], 5 challenge[a],
+ * 6 beta, 7 gamma, 8 theta, 9 trash challenge, 10 y, 11 previous value
+ * (zero in every evaluation below).  Opcodes: 0 add, 1 sub, 2 mul,
+ * 3 square, 4 double, 5 negate, 6 Horner (value = lhs, then
+ * value = value * rhs + part over horner_parts[parts_offset ..
+ * parts_offset + parts_len]), 7 store.  The result of a graph is the value
+ * of its last calculation.  num_intermediates (<= 256) is the slot count;
+ * a calculation whose target is not below it is rejected.  Out-of-range
+ * constant, column, rotation and challenge indices read as zero. */
+typedef struct RawValueSource {
+    uint32_t kind;
+    uint32_t a;
+    uint32_t b;
+} RawValueSource;
+
+typedef struct RawCalculation {
+    uint32_t op;
+    uint32_t target;
+    uint32_t parts_offset;
+    uint32_t parts_len;
+    RawValueSource lhs;
+    RawValueSource rhs;
+} RawCalculation;
+
+typedef struct RawGraph {
+    const RawFr *constants;
+    size_t constants_len;
+    const int32_t *rotations;
+    size_t rotations_len;
+    const RawCalculation *calculations;
+    size_t calculations_len;
+    const RawValueSource *horner_parts;
+    size_t horner_parts_len;
+    uint32_t num_intermediates;
+} RawGraph;
+
+/* kind 0 fixed, 1 advice, 2 instance (bls12_381_g1_msm.cu:378-381) */
+typedef struct RawColumnRef {
+    uint32_t kind;
+    uint32_t index;
+} RawColumnRef;
+
+/* replaces bls12_381_g1_msm.cu:5196 midnight_sppark_bls12_381_eval_numerator.
+ * Host buffers, column-major (column c at [c*len, (c+1)*len)).  Per row idx
+ * of the evaluated domain (generator omega, rotations scaled by rot_scale,
+ * indices taken modulo len) the value is folded as value = value*y + term:
+ * custom graph (or initial_out[idx] when run_custom == 0), then the
+ * permutation terms of permutation_sets product columns (chunk_len columns
+ * per set, delta_start * omega^idx for the first identity column, times
+ * delta_const per column), then each lookup's five terms, then each trash
+ * term -- the fold order of evaluation.rs:3114-3528. */
+RustError midnight_sppark_bls12_381_eval_numerator(
+    RawFr *out, size_t len, const RawFr *initial_out, uint32_t run_custom,
+    const RawFr *fixed, size_t fixed_cols, const RawFr *advice, size_t advice_cols,
+    const RawFr *instance, size_t instance_cols, const RawFr *challenges, size_t challenge_len,
+    RawFr beta, RawFr gamma, RawFr theta, RawFr trash_challenge, RawFr y, RawFr omega,
+    RawFr delta_start, RawFr delta_const, int32_t rot_scale, int32_t last_rotation,
+    uint32_t chunk_len, const RawFr *l0, const RawFr *l_last, const RawFr *l_active,
+    const RawFr *permutation_products, size_t permutation_sets,
+    const RawColumnRef *permutation_columns, size_t permutation_columns_len,
+    const RawFr *permutation_pk_cosets, size_t permutation_pk_cosets_len,
+    const RawFr *lookup_product_cosets, const RawFr *lookup_input_cosets,
+    const RawFr *lookup_table_cosets, size_t lookup_count, const RawFr *trash_cosets,
+    const uint32_t *trash_selector_fixed_indices, size_t trash_count, RawGraph custom_graph,
+    const RawGraph *lookup_graphs, const RawGraph *trash_graphs);
+
+/* replaces bls12_381_g1_msm.cu:5434: out[g*len + idx] = graph g at row idx */
+RustError midnight_sppark_bls12_381_eval_graph_batch(
+    RawFr *out, size_t len, const RawFr *fixed, size_t fixed_cols, const RawFr *advice,
+    size_t advice_cols, const RawFr *instance, size_t instance_cols, const RawFr *challenges,
+    size_t challenge_len, RawFr beta, RawFr gamma, RawFr theta, RawFr trash_challenge, RawFr y,
+    int32_t rot_scale, const RawGraph *graphs, size_t graph_count);
+
+/* replaces bls12_381_g1_msm.cu:5517: keeps the key-dependent inputs of the
+ * numerator (fixed and sigma cosets, Lagrange polynomials, graphs, domain
+ * constants) resident for many evaluations.  fixed_lagrange /
+ * permutation_pk_lagrange (input_len rows each) and trace_omega are
+ * validated and stored as the reference stores them. */
+RustError midnight_sppark_bls12_381_numerator_context_create(
+    void **out, size_t len, size_t input_len, const RawFr *fixed, const RawFr *fixed_lagrange,
+    size_t fixed_cols, const RawFr *l0, const RawFr *l_last, const RawFr *l_active,
+    const RawColumnRef *permutation_columns, size_t permutation_columns_len,
+    const RawFr *permutation_pk_cosets, const RawFr *permutation_pk_lagrange,
+    size_t permutation_pk_cosets_len, const uint32_t *trash_selector_fixed_indices,
+    size_t trash_count, RawGraph custom_graph, const RawGraph *lookup_graphs,
+    const RawGraph *lookup_input_graphs, const RawGraph *lookup_table_graphs,
+    size_t lookup_count, const RawGraph *trash_graphs, RawFr omega, RawFr trace_omega,
+    RawFr g_coset, RawFr delta_const, int32_t rot_scale, int32_t last_rotation,
+    uint32_t chunk_len);
+
+/* replaces bls12_381_g1_msm.cu:5662 */
+void midnight_sppark_bls12_381_numerator_context_free(void *context);
+
+/* replaces bls12_381_g1_msm.cu:5968: evaluates the numerator from resident
+ * columns.  raw_columns is one device vector of total_columns * len values;
+ * advice, instance, permutation products, lookup (product, input, table)
+ * and trash cosets are column ranges of it.  delta_start = beta * g_coset.
+ * *out receives a new device vector of len values. */
+RustError midnight_sppark_bls12_381_numerator_context_eval(
+    void **out, const void *raw_context, const void *raw_columns, size_t total_columns,
+    size_t advice_offset, size_t advice_cols, size_t instance_offset, size_t instance_cols,
+    const RawFr *challenges, size_t challenge_len, RawFr beta, RawFr gamma, RawFr theta,
+    RawFr trash_challenge, RawFr y, size_t permutation_offset, size_t permutation_sets,
+    size_t lookup_product_offset, size_t lookup_input_offset, size_t lookup_table_offset,
+    size_t lookup_count, size_t trash_offset, size_t trash_count);
+
 /* ---------------- backend extensions (no reference counterpart) ---------------- */
 
 /* Number of elements of a device vector (for callers and tests that hold

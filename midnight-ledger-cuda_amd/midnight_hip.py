@@ -14,6 +14,8 @@ import os
 
 import numpy as np
 
+from numerator_graph import RawColumnRef, RawGraph, empty_raw_graph
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmidnight_hip.so")
 
@@ -126,6 +128,20 @@ def _declare(L):
         "midnight_hip_g1_msm_context_points_download": (E, [u64p, vp, c_size]),
         "midnight_hip_g1_msm_context_last_stats": (
             E, [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32)]),
+        "midnight_sppark_bls12_381_eval_numerator": (
+            E, [u64p, c_size, vp, u32, vp, c_size, vp, c_size, vp, c_size, vp, c_size]
+            + [RawFr] * 8 + [ctypes.c_int32, ctypes.c_int32, u32, vp, vp, vp, vp, c_size, vp, c_size, vp, c_size,
+                             vp, vp, vp, c_size, vp, vp, c_size, RawGraph, vp, vp]),
+        "midnight_sppark_bls12_381_eval_graph_batch": (
+            E, [u64p, c_size, vp, c_size, vp, c_size, vp, c_size, vp, c_size] + [RawFr] * 5
+            + [ctypes.c_int32, vp, c_size]),
+        "midnight_sppark_bls12_381_numerator_context_create": (
+            E, [vpp, c_size, c_size, vp, vp, c_size, vp, vp, vp, vp, c_size, vp, vp, c_size, vp, c_size,
+                RawGraph, vp, vp, vp, c_size, vp, RawFr, RawFr, RawFr, RawFr, ctypes.c_int32, ctypes.c_int32, u32]),
+        "midnight_sppark_bls12_381_numerator_context_free": (None, [vp]),
+        "midnight_sppark_bls12_381_numerator_context_eval": (
+            E, [vpp, vp, vp, c_size, c_size, c_size, c_size, c_size, vp, c_size] + [RawFr] * 5
+            + [c_size] * 8),
         "midnight_hip_profile_enable": (None, [ctypes.c_int]),
         "midnight_hip_profile_read": (
             ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]),
@@ -455,3 +471,155 @@ def field_mul_rate(blocks=0):
     fr = ctypes.c_double()
     check(lib().midnight_hip_field_mul_rate(blocks, ctypes.byref(fp), ctypes.byref(fr)))
     return fp.value, fr.value
+
+
+# ---------------------------------------------------------------- quotient numerator
+
+def _ptr(a):
+    """Data pointer of an optional numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def _cols(a, n):
+    """(cols * n, 4) or (cols, n, 4) Fr array -> (contiguous array, cols)."""
+    if a is None:
+        return None, 0
+    a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 4)
+    if n == 0:
+        return a, 0
+    assert a.shape[0] % n == 0, "column array length is not a multiple of the row count"
+    return a, a.shape[0] // n
+
+
+def _graphs(gs):
+    gs = list(gs)
+    if not gs:
+        return None, None
+    arr = (RawGraph * len(gs))(*[g.as_raw() for g in gs])
+    return arr, ctypes.addressof(arr)
+
+
+def _column_refs(refs):
+    refs = list(refs)
+    if not refs:
+        return None, None
+    arr = (RawColumnRef * len(refs))(*[RawColumnRef(k, i) for k, i in refs])
+    return arr, ctypes.addressof(arr)
+
+
+def eval_numerator(n, *, beta, gamma, theta, trash_challenge, y, omega, delta_start, delta_const,
+                   rot_scale, last_rotation, chunk_len, custom_graph=None, run_custom=True,
+                   initial_out=None, fixed=None, advice=None, instance=None, challenges=None,
+                   l0=None, l_last=None, l_active=None, permutation_products=None,
+                   permutation_columns=(), permutation_pk_cosets=None, lookup_product_cosets=None,
+                   lookup_input_cosets=None, lookup_table_cosets=None, lookup_graphs=(),
+                   trash_cosets=None, trash_selector_fixed_indices=(), trash_graphs=()):
+    """midnight_sppark_bls12_381_eval_numerator on host arrays.  Column
+    arrays are (cols * n, 4) uint64 Montgomery Fr; graphs are
+    numerator_graph.RawGraphBuffers; permutation_columns are (kind, index)
+    pairs.  Returns the (n, 4) numerator values."""
+    out = np.zeros((n, 4), dtype=np.uint64)
+    fixed, nf = _cols(fixed, n)
+    advice, na = _cols(advice, n)
+    instance, ni = _cols(instance, n)
+    ch = None if challenges is None else np.ascontiguousarray(challenges, dtype=np.uint64).reshape(-1, 4)
+    init = None if initial_out is None else np.ascontiguousarray(initial_out, dtype=np.uint64).reshape(-1, 4)
+    lag = [None if x is None else np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, 4)
+           for x in (l0, l_last, l_active)]
+    pp, nsets = _cols(permutation_products, n)
+    pc, ncos = _cols(permutation_pk_cosets, n)
+    lp, nlook = _cols(lookup_product_cosets, n)
+    li, _ = _cols(lookup_input_cosets, n)
+    lt, _ = _cols(lookup_table_cosets, n)
+    tc, ntrash = _cols(trash_cosets, n)
+    col_arr, col_ptr = _column_refs(permutation_columns)
+    sel = np.ascontiguousarray(trash_selector_fixed_indices, dtype=np.uint32) if ntrash else None
+    lg_arr, lg_ptr = _graphs(lookup_graphs)
+    tg_arr, tg_ptr = _graphs(trash_graphs)
+    cg = custom_graph.as_raw() if custom_graph is not None else empty_raw_graph()
+    check(lib().midnight_sppark_bls12_381_eval_numerator(
+        P(out), n, _ptr(init), 1 if run_custom else 0, _ptr(fixed), nf, _ptr(advice), na, _ptr(instance), ni,
+        _ptr(ch), 0 if ch is None else ch.shape[0],
+        RawFr.of(beta), RawFr.of(gamma), RawFr.of(theta), RawFr.of(trash_challenge), RawFr.of(y),
+        RawFr.of(omega), RawFr.of(delta_start), RawFr.of(delta_const), rot_scale, last_rotation, chunk_len,
+        _ptr(lag[0]), _ptr(lag[1]), _ptr(lag[2]), _ptr(pp), nsets, col_ptr, len(permutation_columns),
+        _ptr(pc), ncos, _ptr(lp), _ptr(li), _ptr(lt), nlook, _ptr(tc), _ptr(sel), ntrash, cg, lg_ptr, tg_ptr))
+    return out
+
+
+def eval_graph_batch(n, graphs, *, beta, gamma, theta, trash_challenge, y, rot_scale, fixed=None,
+                     advice=None, instance=None, challenges=None):
+    """midnight_sppark_bls12_381_eval_graph_batch: (len(graphs), n, 4)."""
+    fixed, nf = _cols(fixed, n)
+    advice, na = _cols(advice, n)
+    instance, ni = _cols(instance, n)
+    ch = None if challenges is None else np.ascontiguousarray(challenges, dtype=np.uint64).reshape(-1, 4)
+    g_arr, g_ptr = _graphs(graphs)
+    out = np.zeros((len(graphs) * n, 4), dtype=np.uint64)
+    check(lib().midnight_sppark_bls12_381_eval_graph_batch(
+        P(out), n, _ptr(fixed), nf, _ptr(advice), na, _ptr(instance), ni, _ptr(ch),
+        0 if ch is None else ch.shape[0], RawFr.of(beta), RawFr.of(gamma), RawFr.of(theta),
+        RawFr.of(trash_challenge), RawFr.of(y), rot_scale, g_ptr, len(graphs)))
+    return out.reshape(len(graphs), n, 4)
+
+
+class NumeratorContext:
+    """Resident key data for repeated numerator evaluations
+    (numerator_context_create / _eval / _free)."""
+
+    def __init__(self, n, input_len, *, fixed, fixed_lagrange, l0, l_last, l_active, permutation_columns,
+                 permutation_pk_cosets, permutation_pk_lagrange, custom_graph, omega, trace_omega, g_coset,
+                 delta_const, rot_scale, last_rotation, chunk_len, lookup_graphs=(), lookup_input_graphs=(),
+                 lookup_table_graphs=(), trash_graphs=(), trash_selector_fixed_indices=()):
+        fixed, nf = _cols(fixed, n)
+        fl, _ = _cols(fixed_lagrange, input_len)
+        pc, ncos = _cols(permutation_pk_cosets, n)
+        pl, _ = _cols(permutation_pk_lagrange, input_len)
+        lag = [np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, 4) for x in (l0, l_last, l_active)]
+        col_arr, col_ptr = _column_refs(permutation_columns)
+        ntrash = len(trash_graphs)
+        sel = np.ascontiguousarray(trash_selector_fixed_indices, dtype=np.uint32) if ntrash else None
+        lg = _graphs(lookup_graphs)
+        lig = _graphs(lookup_input_graphs)
+        ltg = _graphs(lookup_table_graphs)
+        tg = _graphs(trash_graphs)
+        h = ctypes.c_void_p()
+        check(lib().midnight_sppark_bls12_381_numerator_context_create(
+            ctypes.byref(h), n, input_len, _ptr(fixed), _ptr(fl), nf, _ptr(lag[0]), _ptr(lag[1]), _ptr(lag[2]),
+            col_ptr, len(permutation_columns), _ptr(pc), _ptr(pl), ncos, _ptr(sel), ntrash,
+            custom_graph.as_raw(), lg[1], lig[1], ltg[1], len(lookup_graphs), tg[1],
+            RawFr.of(omega), RawFr.of(trace_omega), RawFr.of(g_coset), RawFr.of(delta_const),
+            rot_scale, last_rotation, chunk_len))
+        self.handle = h
+        self.n = n
+
+    def eval(self, columns, total_columns, *, beta, gamma, theta, trash_challenge, y, advice_offset=0,
+             advice_cols=0, instance_offset=0, instance_cols=0, challenges=None, permutation_offset=0,
+             permutation_sets=0, lookup_product_offset=0, lookup_input_offset=0, lookup_table_offset=0,
+             lookup_count=0, trash_offset=0, trash_count=0):
+        """Numerator from a DeviceVec of total_columns * n resident values;
+        returns a new DeviceVec of n values."""
+        ch = None if challenges is None else np.ascontiguousarray(challenges, dtype=np.uint64).reshape(-1, 4)
+        h = ctypes.c_void_p()
+        check(lib().midnight_sppark_bls12_381_numerator_context_eval(
+            ctypes.byref(h), self.handle, columns.handle if columns is not None else None, total_columns,
+            advice_offset, advice_cols, instance_offset, instance_cols, _ptr(ch),
+            0 if ch is None else ch.shape[0], RawFr.of(beta), RawFr.of(gamma), RawFr.of(theta),
+            RawFr.of(trash_challenge), RawFr.of(y), permutation_offset, permutation_sets,
+            lookup_product_offset, lookup_input_offset, lookup_table_offset, lookup_count,
+            trash_offset, trash_count))
+        return DeviceVec(h.value)
+
+    def free(self):
+        if self.handle and self.handle.value:
+            lib().midnight_sppark_bls12_381_numerator_context_free(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -137,6 +137,62 @@ void orc_coeff_to_extended(orc_fr *out, const orc_fr *coeffs, const orc_domain *
 /* extended_to_coeff (domain.rs:410): iNTT(ext) * 1/N_ext, zeta^-1 distribute */
 void orc_extended_to_coeff(orc_fr *a, const orc_domain *d, int threads);
 
+/* ---- Quotient numerator (plonk/evaluation.rs) ---- */
+/* Compiled expression graph, same layout as the GPU ABI's RawGraph
+ * (bls12_381_g1_msm.cu:351-376).  Source kinds: 0 constant, 1 intermediate,
+ * 2 fixed, 3 advice, 4 instance (a = column, b = rotation slot), 5 challenge,
+ * 6 beta, 7 gamma, 8 theta, 9 trash challenge, 10 y, 11 previous value.
+ * Opcodes: 0 add, 1 sub, 2 mul, 3 square, 4 double, 5 negate, 6 Horner
+ * (value = lhs; value = value*rhs + part for each part), 7 store. */
+typedef struct { uint32_t kind, a, b; } orc_value_source;
+typedef struct {
+    uint32_t op, target, parts_offset, parts_len;
+    orc_value_source lhs, rhs;
+} orc_calculation;
+typedef struct {
+    const orc_fr *constants; size_t constants_len;
+    const int32_t *rotations; size_t rotations_len;
+    const orc_calculation *calculations; size_t calculations_len;
+    const orc_value_source *horner_parts; size_t horner_parts_len;
+    uint32_t num_intermediates;
+} orc_graph;
+typedef struct { uint32_t kind, index; } orc_column_ref;   /* 0 fixed, 1 advice, 2 instance */
+
+/* Everything one numerator evaluation reads.  Column-major polynomial
+ * arrays: column c occupies [c*len, (c+1)*len). */
+typedef struct {
+    size_t len;
+    int run_custom;                /* 0: start from initial_out, skip gates */
+    const orc_fr *initial_out;
+    const orc_fr *fixed; size_t fixed_cols;
+    const orc_fr *advice; size_t advice_cols;
+    const orc_fr *instance; size_t instance_cols;
+    const orc_fr *challenges; size_t challenge_len;
+    orc_fr beta, gamma, theta, trash_challenge, y;
+    orc_fr omega;                  /* generator of the evaluated domain */
+    orc_fr g_coset;                /* delta_start = beta * g_coset */
+    int32_t rot_scale, last_rotation;
+    uint32_t chunk_len;
+    const orc_fr *l0, *l_last, *l_active;
+    const orc_fr *permutation_products; size_t permutation_sets;
+    const orc_column_ref *permutation_columns; size_t permutation_columns_len;
+    const orc_fr *permutation_pk_cosets;
+    const orc_fr *lookup_product_cosets, *lookup_input_cosets, *lookup_table_cosets;
+    size_t lookup_count;
+    const orc_fr *trash_cosets;
+    const uint32_t *trash_selector_fixed_indices; size_t trash_count;
+    orc_graph custom_graph;
+    const orc_graph *lookup_graphs;
+    const orc_graph *trash_graphs;
+} orc_numerator_inputs;
+
+/* GraphEvaluator::evaluate over rows 0..len (evaluation.rs:3797-3845);
+ * previous[i] is the PreviousValue of row i (NULL: zero). */
+void orc_graph_eval_rows(orc_fr *out, const orc_graph *g, const orc_numerator_inputs *in,
+                         const orc_fr *previous, int threads);
+/* Evaluator::evaluate_numerator for one proof (evaluation.rs:3114-3528) */
+void orc_eval_numerator(orc_fr *out, const orc_numerator_inputs *in, int threads);
+
 /* ---- RNG : rand_core 0.6 seed_from_u64 (PCG32) + rand_chacha 0.3 ChaCha20 ---- */
 typedef struct {
     uint32_t key[8];

@@ -101,6 +101,8 @@ def _declare(L):
         "orc_fr_random_many": (V, [u64p, ctypes.c_size_t, ctypes.POINTER(Rng)]),
         "orc_chacha20_block": (V, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
                                    ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32)]),
+        "orc_eval_numerator": (V, [u64p, ctypes.c_void_p, ctypes.c_int]),
+        "orc_graph_eval_rows": (V, [u64p, ctypes.c_void_p, ctypes.c_void_p, u64p, ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -349,3 +351,164 @@ def load_srs_fixture(path):
     if bad:
         raise ValueError("%d invalid points in %s" % (bad, path))
     return k, pts[:n].copy(), pts[n:].copy()
+
+
+# ---------------------------------------------------------------- quotient numerator
+
+class OrcValueSource(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("a", ctypes.c_uint32), ("b", ctypes.c_uint32)]
+
+
+class OrcGraph(ctypes.Structure):
+    _fields_ = [("constants", ctypes.c_void_p), ("constants_len", ctypes.c_size_t),
+                ("rotations", ctypes.c_void_p), ("rotations_len", ctypes.c_size_t),
+                ("calculations", ctypes.c_void_p), ("calculations_len", ctypes.c_size_t),
+                ("horner_parts", ctypes.c_void_p), ("horner_parts_len", ctypes.c_size_t),
+                ("num_intermediates", ctypes.c_uint32)]
+
+
+class OrcColumnRef(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("index", ctypes.c_uint32)]
+
+
+_FR4 = ctypes.c_uint64 * 4
+
+
+class NumeratorInputs(ctypes.Structure):
+    _fields_ = [
+        ("len", ctypes.c_size_t), ("run_custom", ctypes.c_int), ("initial_out", ctypes.c_void_p),
+        ("fixed", ctypes.c_void_p), ("fixed_cols", ctypes.c_size_t),
+        ("advice", ctypes.c_void_p), ("advice_cols", ctypes.c_size_t),
+        ("instance", ctypes.c_void_p), ("instance_cols", ctypes.c_size_t),
+        ("challenges", ctypes.c_void_p), ("challenge_len", ctypes.c_size_t),
+        ("beta", _FR4), ("gamma", _FR4), ("theta", _FR4), ("trash_challenge", _FR4), ("y", _FR4),
+        ("omega", _FR4), ("g_coset", _FR4),
+        ("rot_scale", ctypes.c_int32), ("last_rotation", ctypes.c_int32), ("chunk_len", ctypes.c_uint32),
+        ("l0", ctypes.c_void_p), ("l_last", ctypes.c_void_p), ("l_active", ctypes.c_void_p),
+        ("permutation_products", ctypes.c_void_p), ("permutation_sets", ctypes.c_size_t),
+        ("permutation_columns", ctypes.c_void_p), ("permutation_columns_len", ctypes.c_size_t),
+        ("permutation_pk_cosets", ctypes.c_void_p),
+        ("lookup_product_cosets", ctypes.c_void_p), ("lookup_input_cosets", ctypes.c_void_p),
+        ("lookup_table_cosets", ctypes.c_void_p), ("lookup_count", ctypes.c_size_t),
+        ("trash_cosets", ctypes.c_void_p), ("trash_selector_fixed_indices", ctypes.c_void_p),
+        ("trash_count", ctypes.c_size_t),
+        ("custom_graph", OrcGraph), ("lookup_graphs", ctypes.c_void_p), ("trash_graphs", ctypes.c_void_p),
+    ]
+
+
+def _orc_graph(g):
+    """Any graph object with as_raw() (RawGraph layout) -> OrcGraph."""
+    if g is None:
+        return OrcGraph()
+    r = g.as_raw()
+    return OrcGraph(r.constants, r.constants_len, r.rotations, r.rotations_len, r.calculations,
+                    r.calculations_len, r.horner_parts, r.horner_parts_len, r.num_intermediates)
+
+
+def _col(a, n):
+    if a is None:
+        return None, 0
+    a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 4)
+    assert a.shape[0] % n == 0
+    return a, a.shape[0] // n
+
+
+def _set4(field, v):
+    v = np.asarray(v, dtype=np.uint64).reshape(4)
+    for i in range(4):
+        field[i] = int(v[i])
+
+
+def numerator(n, *, beta, gamma, theta, trash_challenge, y, omega, g_coset, rot_scale, last_rotation,
+              chunk_len, custom_graph=None, run_custom=True, initial_out=None, fixed=None, advice=None,
+              instance=None, challenges=None, l0=None, l_last=None, l_active=None, permutation_products=None,
+              permutation_columns=(), permutation_pk_cosets=None, lookup_product_cosets=None,
+              lookup_input_cosets=None, lookup_table_cosets=None, lookup_graphs=(), trash_cosets=None,
+              trash_selector_fixed_indices=(), trash_graphs=(), threads=None):
+    """orc_eval_numerator: evaluate_numerator for one proof, over n rows.
+    g_coset multiplies beta into the first identity column's delta."""
+    keep = []
+
+    def ptr(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a)
+        keep.append(a)
+        return a.ctypes.data
+
+    s = NumeratorInputs()
+    s.len = n
+    s.run_custom = 1 if run_custom else 0
+    s.initial_out = ptr(None if initial_out is None else np.asarray(initial_out, dtype=np.uint64))
+    for name, arr in (("fixed", fixed), ("advice", advice), ("instance", instance)):
+        a, c = _col(arr, n)
+        setattr(s, name, ptr(a))
+        setattr(s, name + "_cols", c)
+    if challenges is not None:
+        ch = np.ascontiguousarray(challenges, dtype=np.uint64).reshape(-1, 4)
+        s.challenges = ptr(ch)
+        s.challenge_len = ch.shape[0]
+    for name, v in (("beta", beta), ("gamma", gamma), ("theta", theta), ("trash_challenge", trash_challenge),
+                    ("y", y), ("omega", omega), ("g_coset", g_coset)):
+        _set4(getattr(s, name), v)
+    s.rot_scale, s.last_rotation, s.chunk_len = rot_scale, last_rotation, chunk_len
+    for name, v in (("l0", l0), ("l_last", l_last), ("l_active", l_active)):
+        setattr(s, name, ptr(None if v is None else np.asarray(v, dtype=np.uint64)))
+    pp, nsets = _col(permutation_products, n)
+    s.permutation_products, s.permutation_sets = ptr(pp), nsets
+    refs = list(permutation_columns)
+    if refs:
+        arr = (OrcColumnRef * len(refs))(*[OrcColumnRef(k, i) for k, i in refs])
+        keep.append(arr)
+        s.permutation_columns = ctypes.addressof(arr)
+    s.permutation_columns_len = len(refs)
+    s.permutation_pk_cosets = ptr(_col(permutation_pk_cosets, n)[0])
+    lp, nlook = _col(lookup_product_cosets, n)
+    s.lookup_product_cosets, s.lookup_count = ptr(lp), nlook
+    s.lookup_input_cosets = ptr(_col(lookup_input_cosets, n)[0])
+    s.lookup_table_cosets = ptr(_col(lookup_table_cosets, n)[0])
+    tc, ntrash = _col(trash_cosets, n)
+    s.trash_cosets, s.trash_count = ptr(tc), ntrash
+    if ntrash:
+        s.trash_selector_fixed_indices = ptr(np.asarray(trash_selector_fixed_indices, dtype=np.uint32))
+    s.custom_graph = _orc_graph(custom_graph)
+    for name, gs in (("lookup_graphs", lookup_graphs), ("trash_graphs", trash_graphs)):
+        gs = list(gs)
+        if gs:
+            arr = (OrcGraph * len(gs))(*[_orc_graph(g) for g in gs])
+            keep.append(arr)
+            setattr(s, name, ctypes.addressof(arr))
+    out = np.zeros((n, 4), dtype=np.uint64)
+    lib().orc_eval_numerator(P(out), ctypes.addressof(s), threads or os.cpu_count())
+    return out
+
+
+def graph_rows(n, graph, *, beta=None, gamma=None, theta=None, trash_challenge=None, y=None, rot_scale=1,
+               fixed=None, advice=None, instance=None, challenges=None, previous=None, threads=None):
+    """orc_graph_eval_rows: one graph over n rows (previous: per-row
+    PreviousValue or None for zero)."""
+    z = np.zeros(4, dtype=np.uint64)
+    kw = dict(beta=z if beta is None else beta, gamma=z if gamma is None else gamma,
+              theta=z if theta is None else theta,
+              trash_challenge=z if trash_challenge is None else trash_challenge, y=z if y is None else y)
+    keep = []
+    s = NumeratorInputs()
+    s.len = n
+    for name, arr in (("fixed", fixed), ("advice", advice), ("instance", instance)):
+        a, c = _col(arr, n)
+        keep.append(a)
+        setattr(s, name, None if a is None else a.ctypes.data)
+        setattr(s, name + "_cols", c)
+    if challenges is not None:
+        ch = np.ascontiguousarray(challenges, dtype=np.uint64).reshape(-1, 4)
+        keep.append(ch)
+        s.challenges, s.challenge_len = ch.ctypes.data, ch.shape[0]
+    for name, v in kw.items():
+        _set4(getattr(s, name), v)
+    s.rot_scale = rot_scale
+    g = _orc_graph(graph)
+    prev = None if previous is None else np.ascontiguousarray(previous, dtype=np.uint64).reshape(-1, 4)
+    out = np.zeros((n, 4), dtype=np.uint64)
+    lib().orc_graph_eval_rows(P(out), ctypes.addressof(g), ctypes.addressof(s),
+                              None if prev is None else P(prev), threads or os.cpu_count())
+    return out

@@ -1,2 +1,316 @@
-/* quotient numerator oracle: filled in below */
+/*
+ * Quotient numerator on the extended domain -- CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Restates the reference CPU prover's numerator evaluation
+ * (vendor/midnight-proofs-0.8.1/src/plonk/evaluation.rs):
+ *   - ValueSource::get            evaluation.rs:84-119
+ *   - Calculation::evaluate       evaluation.rs:150-210
+ *   - GraphEvaluator::evaluate    evaluation.rs:3797-3845
+ *   - get_rotation_idx            evaluation.rs:42-45
+ *   - Evaluator::evaluate_numerator, one proof:
+ *       custom gates              evaluation.rs:3190-3225
+ *       permutation               evaluation.rs:3258-3355
+ *       lookups                   evaluation.rs:3382-3470
+ *       trashcans                 evaluation.rs:3473-3525
+ * over the compiled graph encoding that the GPU entry point receives
+ * (RawGraph / RawCalculation / RawValueSource, bls12_381_g1_msm.cu:351-381;
+ * opcode numbers midnight-gpu-prover/src/expression.rs:80-98).
+ *
+ * The phases run one after another over all rows, as the CPU prover does;
+ * each phase folds its terms into the running value with value = value*y + t.
+ *
+ * Parity pinning (tests/test_oracle_quotient.py):
+ *   - the graph evaluator against a direct evaluation of the expression the
+ *     graph was compiled from (gate q*(a*b + c - d), Horner in y, squares,
+ *     doubles, negations, constants, rotations in both directions);
+ *   - the whole numerator evaluated on the base domain of a satisfying
+ *     synthetic circuit (gate + 8-column copy constraints split over
+ *     3 permutation sets) is zero on every row, and a single changed witness
+ *     cell makes exactly the expected rows nonzero;
+ *   - the numerator on the extended coset of that circuit, divided by the
+ *     vanishing polynomial, interpolates to a polynomial whose top
+ *     coefficients (degree >= (n-1)(d-1)) are zero.
+ */
 #include "oracle.h"
+#include <stdlib.h>
+#include <string.h>
+
+static int64_t rot_idx(size_t idx, int32_t rot, int32_t rot_scale, size_t len)
+{
+    /* get_rotation_idx: (idx + rot*rot_scale) rem_euclid size */
+    int64_t m = (int64_t)len;
+    int64_t s = ((int64_t)idx + (int64_t)rot * (int64_t)rot_scale) % m;
+    return s < 0 ? s + m : s;
+}
+
+static const orc_fr *column_base(uint32_t kind, uint32_t index, const orc_numerator_inputs *in)
+{
+    if (kind == 0 && index < in->fixed_cols) return in->fixed + (size_t)index * in->len;
+    if (kind == 1 && index < in->advice_cols) return in->advice + (size_t)index * in->len;
+    if (kind == 2 && index < in->instance_cols) return in->instance + (size_t)index * in->len;
+    return NULL;
+}
+
+static orc_fr source_value(const orc_value_source *s, const orc_graph *g, size_t idx,
+                           const orc_numerator_inputs *in, const orc_fr *inter,
+                           const orc_fr *previous)
+{
+    orc_fr z; memset(&z, 0, sizeof z);
+    switch (s->kind) {
+    case 0: return s->a < g->constants_len ? g->constants[s->a] : z;
+    case 1: return s->a < g->num_intermediates ? inter[s->a] : z;
+    case 2: case 3: case 4: {
+        const orc_fr *base = column_base(s->kind - 2, s->a, in);
+        if (!base || s->b >= g->rotations_len) return z;
+        return base[rot_idx(idx, g->rotations[s->b], in->rot_scale, in->len)];
+    }
+    case 5: return s->a < in->challenge_len ? in->challenges[s->a] : z;
+    case 6: return in->beta;
+    case 7: return in->gamma;
+    case 8: return in->theta;
+    case 9: return in->trash_challenge;
+    case 10: return in->y;
+    case 11: return *previous;
+    default: return z;
+    }
+}
+
+/* GraphEvaluator::evaluate for one row; inter holds num_intermediates
+ * values, zeroed here (compiled graphs never read a slot before writing
+ * it in the same row). */
+static orc_fr graph_eval(const orc_graph *g, size_t idx, const orc_numerator_inputs *in,
+                         orc_fr *inter, const orc_fr *previous)
+{
+    orc_fr last; memset(&last, 0, sizeof last);
+    memset(inter, 0, sizeof(orc_fr) * g->num_intermediates);
+    for (size_t c = 0; c < g->calculations_len; c++) {
+        const orc_calculation *calc = &g->calculations[c];
+        orc_fr v, a, b;
+        memset(&v, 0, sizeof v);
+        switch (calc->op) {
+        case 0: a = source_value(&calc->lhs, g, idx, in, inter, previous);
+                b = source_value(&calc->rhs, g, idx, in, inter, previous);
+                orc_fr_add(&v, &a, &b); break;
+        case 1: a = source_value(&calc->lhs, g, idx, in, inter, previous);
+                b = source_value(&calc->rhs, g, idx, in, inter, previous);
+                orc_fr_sub(&v, &a, &b); break;
+        case 2: a = source_value(&calc->lhs, g, idx, in, inter, previous);
+                b = source_value(&calc->rhs, g, idx, in, inter, previous);
+                orc_fr_mul(&v, &a, &b); break;
+        case 3: a = source_value(&calc->lhs, g, idx, in, inter, previous);
+                orc_fr_sqr(&v, &a); break;
+        case 4: a = source_value(&calc->lhs, g, idx, in, inter, previous);
+                orc_fr_add(&v, &a, &a); break;
+        case 5: a = source_value(&calc->lhs, g, idx, in, inter, previous);
+                orc_fr_neg(&v, &a); break;
+        case 6: {   /* Horner(start, parts, factor) */
+            orc_fr f = source_value(&calc->rhs, g, idx, in, inter, previous);
+            v = source_value(&calc->lhs, g, idx, in, inter, previous);
+            for (uint32_t p = 0; p < calc->parts_len; p++) {
+                size_t pi = (size_t)calc->parts_offset + p;
+                orc_fr part; memset(&part, 0, sizeof part);
+                if (pi < g->horner_parts_len)
+                    part = source_value(&g->horner_parts[pi], g, idx, in, inter, previous);
+                orc_fr_mul(&v, &v, &f);
+                orc_fr_add(&v, &v, &part);
+            }
+            break;
+        }
+        case 7: v = source_value(&calc->lhs, g, idx, in, inter, previous); break;
+        default: break;
+        }
+        if (calc->target < g->num_intermediates) inter[calc->target] = v;
+        last = v;
+    }
+    return last;
+}
+
+void orc_graph_eval_rows(orc_fr *out, const orc_graph *g, const orc_numerator_inputs *in,
+                         const orc_fr *previous, int threads)
+{
+    size_t len = in->len;
+    #pragma omp parallel num_threads(threads > 0 ? threads : 1)
+    {
+        orc_fr *inter = (orc_fr *)calloc(g->num_intermediates ? g->num_intermediates : 1, sizeof(orc_fr));
+        orc_fr zero; memset(&zero, 0, sizeof zero);
+        #pragma omp for schedule(static)
+        for (size_t i = 0; i < len; i++)
+            out[i] = graph_eval(g, i, in, inter, previous ? &previous[i] : &zero);
+        free(inter);
+    }
+}
+
+static inline void fold(orc_fr *v, const orc_fr *y, const orc_fr *t)
+{
+    orc_fr_mul(v, v, y);
+    orc_fr_add(v, v, t);
+}
+
+void orc_eval_numerator(orc_fr *out, const orc_numerator_inputs *in, int threads)
+{
+    const size_t len = in->len;
+    const orc_fr y = in->y, beta = in->beta, gamma = in->gamma;
+    orc_fr one; orc_fr_one(&one);
+    if (threads <= 0) threads = 1;
+
+    /* custom gates (evaluation.rs:3190-3225): the graph's closing Horner
+     * reads PreviousValue, the running value of the row */
+    if (in->run_custom) {
+        /* one proof: the running value is zero when the gates are folded */
+        orc_graph_eval_rows(out, &in->custom_graph, in, NULL, threads);
+    } else {
+        memcpy(out, in->initial_out, len * sizeof(orc_fr));
+    }
+
+    /* permutation (evaluation.rs:3258-3355) */
+    if (in->permutation_sets != 0 && in->chunk_len != 0) {
+        orc_fr delta_start;
+        orc_fr_mul(&delta_start, &beta, &in->g_coset);
+        orc_fr delta; orc_fr_delta(&delta);
+        const size_t sets = in->permutation_sets;
+        const orc_fr *first = in->permutation_products;
+        const orc_fr *lastp = in->permutation_products + (sets - 1) * len;
+        /* parallelize() splits the rows into chunks; beta_term = omega^start
+         * at a chunk start, then *= omega per row (evaluation.rs:3283-3352) */
+        const size_t nchunks = (size_t)threads * 4;
+        const size_t chunk = (len + nchunks - 1) / nchunks;
+        #pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+        for (size_t ch = 0; ch < nchunks; ch++) {
+            size_t start = ch * chunk, end = start + chunk < len ? start + chunk : len;
+            orc_fr beta_term;
+            uint64_t e[4] = {(uint64_t)start, 0, 0, 0};
+            orc_fr_pow(&beta_term, &in->omega, e);
+            for (size_t idx = start; idx < end; idx++) {
+                size_t r_next = (size_t)rot_idx(idx, 1, in->rot_scale, len);
+                size_t r_last = (size_t)rot_idx(idx, in->last_rotation, in->rot_scale, len);
+                orc_fr v = out[idx], t, u;
+
+                orc_fr_sub(&t, &one, &first[idx]);
+                orc_fr_mul(&t, &t, &in->l0[idx]);
+                fold(&v, &y, &t);
+
+                orc_fr_sqr(&t, &lastp[idx]);
+                orc_fr_sub(&t, &t, &lastp[idx]);
+                orc_fr_mul(&t, &t, &in->l_last[idx]);
+                fold(&v, &y, &t);
+
+                for (size_t s = 1; s < sets; s++) {
+                    orc_fr_sub(&t, &in->permutation_products[s * len + idx],
+                               &in->permutation_products[(s - 1) * len + r_last]);
+                    orc_fr_mul(&t, &t, &in->l0[idx]);
+                    fold(&v, &y, &t);
+                }
+
+                orc_fr current_delta;
+                orc_fr_mul(&current_delta, &delta_start, &beta_term);
+                for (size_t s = 0; s < sets; s++) {
+                    const orc_fr *z = in->permutation_products + s * len;
+                    size_t base = s * in->chunk_len;
+                    orc_fr left = z[r_next], right = z[idx];
+                    for (size_t j = 0; j < in->chunk_len && base + j < in->permutation_columns_len; j++) {
+                        const orc_column_ref *c = &in->permutation_columns[base + j];
+                        const orc_fr *col = column_base(c->kind, c->index, in);
+                        if (!col) continue;
+                        const orc_fr *sigma = in->permutation_pk_cosets + (base + j) * len;
+                        orc_fr_mul(&t, &beta, &sigma[idx]);
+                        orc_fr_add(&t, &t, &col[idx]);
+                        orc_fr_add(&t, &t, &gamma);
+                        orc_fr_mul(&left, &left, &t);
+                    }
+                    for (size_t j = 0; j < in->chunk_len && base + j < in->permutation_columns_len; j++) {
+                        const orc_column_ref *c = &in->permutation_columns[base + j];
+                        const orc_fr *col = column_base(c->kind, c->index, in);
+                        if (!col) continue;
+                        orc_fr_add(&t, &col[idx], &current_delta);
+                        orc_fr_add(&t, &t, &gamma);
+                        orc_fr_mul(&right, &right, &t);
+                        orc_fr_mul(&current_delta, &current_delta, &delta);
+                    }
+                    orc_fr_sub(&u, &left, &right);
+                    orc_fr_mul(&u, &u, &in->l_active[idx]);
+                    fold(&v, &y, &u);
+                }
+                out[idx] = v;
+                orc_fr_mul(&beta_term, &beta_term, &in->omega);
+            }
+        }
+    }
+
+    /* lookups (evaluation.rs:3382-3470): all lookups of a row in order */
+    if (in->lookup_count != 0) {
+        const orc_fr zero = {{0, 0, 0, 0}};
+        #pragma omp parallel num_threads(threads)
+        {
+            uint32_t maxi = 1;
+            for (size_t n = 0; n < in->lookup_count; n++)
+                if (in->lookup_graphs[n].num_intermediates > maxi) maxi = in->lookup_graphs[n].num_intermediates;
+            orc_fr *inter = (orc_fr *)calloc(maxi, sizeof(orc_fr));
+            #pragma omp for schedule(static)
+            for (size_t idx = 0; idx < len; idx++) {
+                size_t r_next = (size_t)rot_idx(idx, 1, in->rot_scale, len);
+                size_t r_prev = (size_t)rot_idx(idx, -1, in->rot_scale, len);
+                orc_fr v = out[idx], t, u, w;
+                for (size_t n = 0; n < in->lookup_count; n++) {
+                    const orc_fr *z = in->lookup_product_cosets + n * len;
+                    const orc_fr *ai = in->lookup_input_cosets + n * len;
+                    const orc_fr *st = in->lookup_table_cosets + n * len;
+                    orc_fr table_value = graph_eval(&in->lookup_graphs[n], idx, in, inter, &zero);
+                    orc_fr a_minus_s; orc_fr_sub(&a_minus_s, &ai[idx], &st[idx]);
+
+                    orc_fr_sub(&t, &one, &z[idx]);
+                    orc_fr_mul(&t, &t, &in->l0[idx]);
+                    fold(&v, &y, &t);
+
+                    orc_fr_sqr(&t, &z[idx]);
+                    orc_fr_sub(&t, &t, &z[idx]);
+                    orc_fr_mul(&t, &t, &in->l_last[idx]);
+                    fold(&v, &y, &t);
+
+                    orc_fr_add(&t, &ai[idx], &beta);
+                    orc_fr_mul(&t, &z[r_next], &t);
+                    orc_fr_add(&u, &st[idx], &gamma);
+                    orc_fr_mul(&t, &t, &u);
+                    orc_fr_mul(&w, &z[idx], &table_value);
+                    orc_fr_sub(&t, &t, &w);
+                    orc_fr_mul(&t, &t, &in->l_active[idx]);
+                    fold(&v, &y, &t);
+
+                    orc_fr_mul(&t, &a_minus_s, &in->l0[idx]);
+                    fold(&v, &y, &t);
+
+                    orc_fr_sub(&t, &ai[idx], &ai[r_prev]);
+                    orc_fr_mul(&t, &a_minus_s, &t);
+                    orc_fr_mul(&t, &t, &in->l_active[idx]);
+                    fold(&v, &y, &t);
+                }
+                out[idx] = v;
+            }
+            free(inter);
+        }
+    }
+
+    /* trashcans (evaluation.rs:3473-3525), one pass per trashcan */
+    for (size_t n = 0; n < in->trash_count; n++) {
+        const orc_graph *g = &in->trash_graphs[n];
+        uint32_t sel = in->trash_selector_fixed_indices[n];
+        const orc_fr *trash = in->trash_cosets + n * len;
+        const orc_fr zero = {{0, 0, 0, 0}};
+        if (sel >= in->fixed_cols) continue;
+        #pragma omp parallel num_threads(threads)
+        {
+            orc_fr *inter = (orc_fr *)calloc(g->num_intermediates ? g->num_intermediates : 1, sizeof(orc_fr));
+            #pragma omp for schedule(static)
+            for (size_t idx = 0; idx < len; idx++) {
+                orc_fr compressed = graph_eval(g, idx, in, inter, &zero);
+                orc_fr q = in->fixed[(size_t)sel * len + idx], t;
+                orc_fr_sub(&t, &one, &q);
+                orc_fr_mul(&t, &t, &trash[idx]);
+                orc_fr_sub(&t, &compressed, &t);
+                fold(&out[idx], &y, &t);
+            }
+            free(inter);
+        }
+    }
+}

@@ -1,0 +1,241 @@
+"""Synthetic PLONK circuit for the quotient-numerator tests.
+
+Five advice columns (a, b, c, d, e), three fixed columns (q, s1, s2), one
+gate q * (a * b + c - d) = 0, and copy constraints over all eight columns
+(random cycles of 2-4 cells), split into permutation sets of
+chunk_len = degree - 2 = 3 columns.  Degree 5 (the minimum degree the
+benchmark circuit sets), so the extended domain is 4x the base domain;
+blinding factors = max(3, 1) + 0 + 1 + 1 = 5 (circuit.rs:2502-2530).
+
+Everything is built here with Python integers, following the reference
+prover's steps: permutation keygen sigma values delta^col' * omega^row'
+(permutation/keygen.rs), the grand products of permutation/prover.rs:327-515
+(denominators beta*sigma + gamma + v, numerators delta^j*omega^i*beta +
+gamma + v, shifted prefix product chained over sets, blinded last rows),
+and the Lagrange polynomials l0, l_last, l_active of keygen.  The result is
+a satisfying witness, on which the numerator must vanish on the base
+domain.  Used only by tests.
+"""
+import random
+
+import numpy as np
+
+import oracle as O
+import numerator_graph as G
+
+R = G.FR_MODULUS
+DEGREE = 5
+CHUNK = DEGREE - 2
+BLINDING = 5
+ADVICE = 5
+FIXED = 3
+# permutation columns: advice a..e, then fixed q, s1, s2
+PERM_COLUMNS = [(1, i) for i in range(ADVICE)] + [(0, i) for i in range(FIXED)]
+
+
+def mont(values):
+    return G.to_montgomery_limbs(values)
+
+
+def ints(arr):
+    return G.from_montgomery_limbs(arr)
+
+
+def fr_const(name):
+    out = O.fr()
+    getattr(O.lib(), "orc_fr_" + name)(O.P(out))
+    return ints(out[None, :])[0]
+
+
+def omega_int(k):
+    return ints(O.omega(k)[None, :])[0]
+
+
+def gate_expression():
+    """q * (a * b + c - d), as the expression tree the gate API builds."""
+    q = ("fixed", 0, 0)
+    a, b, c, d = (("advice", i, 0) for i in range(4))
+    return ("prod", q, ("sum", ("sum", ("prod", a, b), c), ("neg", d)))
+
+
+class Circuit:
+    def __init__(self, k, seed=1):
+        rnd = random.Random(seed)
+        self.k = k
+        n = self.n = 1 << k
+        u = self.u = n - BLINDING - 1          # l_last row; rows < u are usable
+        self.omega = omega_int(k)
+        self.delta = fr_const("delta")
+        self.zeta = fr_const("zeta")
+        rv = lambda: rnd.randrange(R)
+
+        a = [rv() for _ in range(n)]
+        b = [rv() for _ in range(n)]
+        c = [rv() for _ in range(n)]
+        d = [(a[i] * b[i] + c[i]) % R if i < u else rv() for i in range(n)]
+        e = [rv() for _ in range(n)]
+        q = [1 if i < u else 0 for i in range(n)]
+        s1 = [rv() if i < u else 0 for i in range(n)]
+        s2 = [rv() if i < u else 0 for i in range(n)]
+        cols = [a, b, c, d, e, q, s1, s2]
+
+        # copy constraints: cycles whose cells hold one value.  Sources are
+        # cells of a..d (gate-constrained), targets cells of e, s1, s2 whose
+        # values are overwritten; q cells (all 1) form cycles of their own.
+        used = set()
+        cycles = []
+        free_targets = [(col, row) for col in (4, 6, 7) for row in range(u)]
+        rnd.shuffle(free_targets)
+        for _ in range(max(4, u // 3)):
+            src = (rnd.randrange(4), rnd.randrange(u))
+            if src in used:
+                continue
+            size = rnd.randrange(1, 4)
+            members = [src]
+            while len(members) < size + 1 and free_targets:
+                t = free_targets.pop()
+                if t in used:
+                    continue
+                cols[t[0]][t[1]] = cols[src[0]][src[1]]
+                members.append(t)
+            if len(members) < 2:
+                continue
+            used.update(members)
+            rnd.shuffle(members)
+            cycles.append(members)
+        qrows = rnd.sample(range(u), min(u, 12))
+        for i in range(0, len(qrows) - 1, 3):
+            cyc = [(5, r) for r in qrows[i:i + 3]]
+            if len(cyc) >= 2:
+                cycles.append(cyc)
+        self.cols = cols
+        self.cycles = cycles
+
+        # sigma mapping; cells outside cycles map to themselves
+        mapping = {}
+        for cyc in cycles:
+            for t, cell in enumerate(cyc):
+                mapping[cell] = cyc[(t + 1) % len(cyc)]
+        ncol = len(PERM_COLUMNS)
+        omega_pows = [pow(self.omega, i, R) for i in range(n)]
+        delta_pows = [pow(self.delta, j, R) for j in range(ncol)]
+        self.sigma = []
+        for j in range(ncol):
+            col = []
+            for i in range(n):
+                jj, ii = mapping.get((j, i), (j, i))
+                col.append(delta_pows[jj] * omega_pows[ii] % R)
+            self.sigma.append(col)
+        self.omega_pows = omega_pows
+        self.delta_pows = delta_pows
+
+        self.beta, self.gamma, self.theta, self.trash, self.y = (rv() for _ in range(5))
+        self.blinds = [rv() for _ in range(BLINDING * self.nsets)]
+        self.z = self.products()
+
+        self.l0 = [1 if i == 0 else 0 for i in range(n)]
+        self.l_last = [1 if i == u else 0 for i in range(n)]
+        self.l_active = [1 if i < u else 0 for i in range(n)]
+
+    @property
+    def nsets(self):
+        return (len(PERM_COLUMNS) + CHUNK - 1) // CHUNK
+
+    def products(self):
+        """Permutation grand products (permutation/prover.rs:355-515)."""
+        n, u = self.n, self.u
+        beta, gamma = self.beta, self.gamma
+        last_z = 1
+        deltaomega_col = 0          # column index of the running delta power
+        blinds = iter(self.blinds)
+        zs = []
+        for s in range(self.nsets):
+            cols = list(range(s * CHUNK, min((s + 1) * CHUNK, len(PERM_COLUMNS))))
+            modified = [1] * n
+            for j in cols:
+                v = self.cols[j]
+                sig = self.sigma[j]
+                for i in range(n):
+                    modified[i] = modified[i] * (beta * sig[i] + gamma + v[i]) % R
+            modified = [pow(m, R - 2, R) for m in modified]
+            for j in cols:
+                v = self.cols[j]
+                dj = self.delta_pows[deltaomega_col]
+                for i in range(n):
+                    modified[i] = modified[i] * (dj * self.omega_pows[i] * beta + gamma + v[i]) % R
+                deltaomega_col += 1
+            z = [last_z]
+            for i in range(1, n):
+                z.append(z[i - 1] * modified[i - 1] % R)
+            for i in range(n - BLINDING, n):
+                z[i] = next(blinds)
+            last_z = z[u]
+            zs.append(z)
+        return zs
+
+    # ---- numerator inputs -------------------------------------------------
+
+    def custom_graph(self):
+        return G.RawGraphBuffers(G.custom_gates_graph([gate_expression()]))
+
+    def base_inputs(self):
+        """evaluate_numerator inputs on the base domain (LagrangeCoeff):
+        rot_scale 1, omega, g_coset 1."""
+        n = self.n
+        flat = lambda cs: mont([x for c in cs for x in c])
+        return dict(
+            n=n, fixed=flat(self.cols[5:]), advice=flat(self.cols[:5]),
+            beta=mont([self.beta])[0], gamma=mont([self.gamma])[0], theta=mont([self.theta])[0],
+            trash_challenge=mont([self.trash])[0], y=mont([self.y])[0], omega=mont([self.omega])[0],
+            g_coset=mont([1])[0], rot_scale=1, last_rotation=-(BLINDING + 1), chunk_len=CHUNK,
+            l0=mont(self.l0), l_last=mont(self.l_last), l_active=mont(self.l_active),
+            permutation_products=flat(self.z), permutation_columns=PERM_COLUMNS,
+            permutation_pk_cosets=flat(self.sigma), custom_graph=self.custom_graph())
+
+    def extended_inputs(self, threads=None):
+        """The same inputs on the extended coset (ExtendedLagrangeCoeff):
+        every column Lagrange -> coefficients -> coset evaluations."""
+        k = self.k
+        dom, t_evals = O.domain(DEGREE, k)
+        self.domain, self.t_evals = dom, t_evals
+        N = 1 << dom.extended_k
+
+        def ext(col):
+            coeffs = O.ifft(mont(col), k, threads)
+            return O.coeff_to_extended(coeffs, dom, threads)
+
+        def flat(cs):
+            return np.concatenate([ext(c) for c in cs])
+
+        base = self.base_inputs()
+        base.update(
+            n=N, fixed=flat(self.cols[5:]), advice=flat(self.cols[:5]),
+            omega=np.array(dom.fr("extended_omega")), g_coset=np.array(dom.fr("g_coset")),
+            rot_scale=1 << (dom.extended_k - k),
+            l0=ext(self.l0), l_last=ext(self.l_last), l_active=ext(self.l_active),
+            permutation_products=flat(self.z), permutation_pk_cosets=flat(self.sigma))
+        return base
+
+
+def random_inputs(n, seed, *, rot_scale=4, sets=3, graph=None):
+    """Unstructured numerator inputs of the synthetic circuit's shape
+    (5 advice, 3 fixed, 8 permutation columns): for parity checks at sizes
+    where building a satisfying witness in Python is too slow."""
+    ncols = len(PERM_COLUMNS)
+    vals = O.random_fr(n * (ADVICE + FIXED + ncols + sets + 3) + 8, seed)
+    off = [0]
+
+    def take(m):
+        a = vals[off[0]:off[0] + m]
+        off[0] += m
+        return a
+
+    adv, fix, sig, zs = take(ADVICE * n), take(FIXED * n), take(ncols * n), take(sets * n)
+    l0, ll, la = take(n), take(n), take(n)
+    sc = take(8)
+    return dict(
+        n=n, fixed=fix, advice=adv, beta=sc[0], gamma=sc[1], theta=sc[2], trash_challenge=sc[3], y=sc[4],
+        omega=sc[5], g_coset=sc[6], rot_scale=rot_scale, last_rotation=-(BLINDING + 1), chunk_len=CHUNK,
+        l0=l0, l_last=ll, l_active=la, permutation_products=zs, permutation_columns=PERM_COLUMNS,
+        permutation_pk_cosets=sig,
+        custom_graph=graph or G.RawGraphBuffers(G.custom_gates_graph([gate_expression()])))

@@ -45,3 +45,17 @@ def test_argument_errors_without_device(H):
     with pytest.raises(H.MidnightHipError, match="null BLS12-381 G1 MSM context"):
         H.check(L.midnight_sppark_bls12_381_g1_msm_context_batch(None, None, None, 1, 1))
     del h
+
+
+def test_numerator_argument_errors_without_device(H):
+    import numpy as np
+    z = np.zeros(4, dtype=np.uint64)
+    # run_custom = 0 needs initial_out (bls12_381_g1_msm.cu:5245-5247)
+    with pytest.raises(H.MidnightHipError, match="initial_out"):
+        H.eval_numerator(16, run_custom=False, beta=z, gamma=z, theta=z, trash_challenge=z, y=z, omega=z,
+                         delta_start=z, delta_const=z, rot_scale=1, last_rotation=-1, chunk_len=3)
+    L = H.lib()
+    err = L.midnight_sppark_bls12_381_numerator_context_eval(
+        None, None, None, 0, 0, 0, 0, 0, None, 0, *([H.RawFr()] * 5), *([0] * 8))
+    assert err.code != 0
+    H.check(H.RustError(0, None))

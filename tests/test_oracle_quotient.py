@@ -1,0 +1,220 @@
+"""Pinning of the numerator oracle (oracle/quotient.c) and of the host-side
+graph compiler (numerator_graph.py), without a GPU.
+
+- The compiled graph of the benchmark gate is the one the reference's
+  builder produces (hand-derived from evaluation.rs:3685-3786 and the slot
+  planner evaluation.rs:419-545).
+- Graph evaluation equals a direct evaluation of the expression trees the
+  graphs were compiled from (a separate tree walker here).
+- On a satisfying synthetic circuit the whole numerator (gate, permutation
+  over three sets) is zero on every row of the base domain; a changed
+  witness cell makes exactly the rows it touches nonzero.
+- On the extended coset, numerator / (X^n - 1) interpolates to a
+  polynomial of degree < (n - 1)(d - 1): its top d - 1 coefficients vanish.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import numerator_graph as G
+import oracle as O
+import synth_circuit as S
+
+R = G.FR_MODULUS
+
+
+def test_gate_graph_matches_reference_builder():
+    g = G.custom_gates_graph([S.gate_expression()])
+    assert g.constants == [0, 1, 2]
+    assert g.rotations == [0]
+    raw = G.RawGraphBuffers(g)
+    assert raw.num_intermediates == 3
+    I = lambda s: (G.INTER, s, 0)
+    expect = [
+        (G.OP_STORE, 0, 0, 0, (G.FIXED, 0, 0), (0, 0, 0)),
+        (G.OP_STORE, 1, 0, 0, (G.ADVICE, 0, 0), (0, 0, 0)),
+        (G.OP_STORE, 2, 0, 0, (G.ADVICE, 1, 0), (0, 0, 0)),
+        (G.OP_MUL, 2, 0, 0, I(1), I(2)),
+        (G.OP_STORE, 1, 0, 0, (G.ADVICE, 2, 0), (0, 0, 0)),
+        (G.OP_ADD, 1, 0, 0, I(2), I(1)),
+        (G.OP_STORE, 2, 0, 0, (G.ADVICE, 3, 0), (0, 0, 0)),
+        (G.OP_SUB, 2, 0, 0, I(1), I(2)),
+        (G.OP_MUL, 2, 0, 0, I(0), I(2)),
+        (G.OP_HORNER, 2, 0, 1, (G.PREV, 0, 0), (G.Y, 0, 0)),
+    ]
+    assert raw.ops == expect
+    assert raw.parts == [I(2)]
+
+
+def test_builder_simplifications():
+    g = G.GraphEvaluator()
+    a = ("advice", 0, 0)
+    assert g.add_expression(("prod", ("constant", 0), a)) == G.ZERO_CONST
+    s = g.add_expression(("prod", ("constant", 1), a))
+    assert g.calculations[-1][0] == ("store", (G.ADVICE, 0, 0))
+    assert g.add_expression(("prod", a, ("constant", 2))) == g.add_calculation(("double", s))
+    assert g.add_expression(("prod", a, a)) == g.add_calculation(("square", s))
+    assert g.add_expression(("neg", ("constant", 5))) == (G.CONST, 3, 0) and g.constants[3] == R - 5
+    assert g.add_expression(("sum", ("constant", 0), ("neg", a))) == g.add_calculation(("negate", s))
+    # a repeated sub-expression is computed once
+    n0 = len(g.calculations)
+    g.add_expression(("prod", a, ("advice", 1, 1)))
+    g.add_expression(("prod", ("advice", 1, 1), a))
+    assert len(g.calculations) == n0 + 2
+
+
+def _walk(e, row, n, rot_scale, cols, challenges):
+    """Direct evaluation of an expression tree at one row."""
+    t = e[0]
+    if t == "constant":
+        return e[1] % R
+    if t in ("fixed", "advice", "instance"):
+        c = cols[t][e[1]]
+        return c[(row + e[2] * rot_scale) % n]
+    if t == "challenge":
+        return challenges[e[1]]
+    if t == "neg":
+        return (-_walk(e[1], row, n, rot_scale, cols, challenges)) % R
+    if t == "sum":
+        return (_walk(e[1], row, n, rot_scale, cols, challenges) + _walk(e[2], row, n, rot_scale, cols, challenges)) % R
+    if t == "prod":
+        return _walk(e[1], row, n, rot_scale, cols, challenges) * _walk(e[2], row, n, rot_scale, cols, challenges) % R
+    if t == "scaled":
+        return _walk(e[1], row, n, rot_scale, cols, challenges) * e[2] % R
+    raise ValueError(t)
+
+
+def _expressions():
+    a, b, c, d = (("advice", i, 0) for i in range(4))
+    a_next, a_prev3 = ("advice", 0, 1), ("advice", 0, -3)
+    q, s1 = ("fixed", 0, 0), ("fixed", 1, -1)
+    inst = ("instance", 0, 2)
+    return [
+        S.gate_expression(),
+        ("sum", ("prod", a_next, a_next), ("neg", ("prod", ("constant", 2), b))),
+        ("scaled", ("sum", a_prev3, ("neg", ("constant", 7))), 123456789),
+        ("prod", s1, ("sum", inst, ("challenge", 1))),
+        ("sum", ("neg", ("sum", c, d)), ("prod", ("constant", 1), ("prod", q, ("constant", 0)))),
+        ("prod", ("sum", a, b), ("sum", b, a)),
+    ]
+
+
+@pytest.mark.parametrize("rot_scale", [1, 2])
+def test_graph_eval_equals_direct_evaluation(rot_scale):
+    n = 64
+    rnd = random.Random(7)
+    cols = {"advice": [[rnd.randrange(R) for _ in range(n)] for _ in range(4)],
+            "fixed": [[rnd.randrange(R) for _ in range(n)] for _ in range(2)],
+            "instance": [[rnd.randrange(R) for _ in range(n)]]}
+    ch = [rnd.randrange(R) for _ in range(2)]
+    y = rnd.randrange(R)
+    prev = [rnd.randrange(R) for _ in range(n)]
+    exprs = _expressions()
+    graph = G.RawGraphBuffers(G.custom_gates_graph(exprs))
+    flat = lambda cs: S.mont([x for c in cs for x in c])
+    got = O.graph_rows(n, graph, y=S.mont([y])[0], rot_scale=rot_scale, fixed=flat(cols["fixed"]),
+                       advice=flat(cols["advice"]), instance=flat(cols["instance"]),
+                       challenges=S.mont(ch), previous=S.mont(prev))
+    got = S.ints(got)
+    for row in range(n):
+        v = prev[row]
+        for e in exprs:
+            v = (v * y + _walk(e, row, n, rot_scale, cols, ch)) % R
+        assert got[row] == v, row
+
+
+def test_lookup_and_trash_graph_shapes():
+    inputs = [("advice", 0, 0), ("prod", ("advice", 1, 0), ("fixed", 0, 0))]
+    tables = [("fixed", 1, 0), ("fixed", 2, 0)]
+    g = G.lookup_graph(inputs, tables)
+    n = 16
+    rnd = random.Random(3)
+    adv = [[rnd.randrange(R) for _ in range(n)] for _ in range(2)]
+    fix = [[rnd.randrange(R) for _ in range(n)] for _ in range(3)]
+    beta, gamma, theta = (rnd.randrange(R) for _ in range(3))
+    got = S.ints(O.graph_rows(n, G.RawGraphBuffers(g), beta=S.mont([beta])[0], gamma=S.mont([gamma])[0],
+                              theta=S.mont([theta])[0], advice=S.mont([x for c in adv for x in c]),
+                              fixed=S.mont([x for c in fix for x in c])))
+    for i in range(n):
+        ci = (adv[0][i] * theta + adv[1][i] * fix[0][i]) % R
+        ct = (fix[1][i] * theta + fix[2][i]) % R
+        assert got[i] == (ci + beta) * (ct + gamma) % R
+    tg = G.trash_graph([("advice", 0, 0), ("advice", 1, 0), ("constant", 5)])
+    tr = rnd.randrange(R)
+    got = S.ints(O.graph_rows(n, G.RawGraphBuffers(tg), trash_challenge=S.mont([tr])[0],
+                              advice=S.mont([x for c in adv for x in c])))
+    for i in range(n):
+        assert got[i] == ((adv[0][i] * tr + adv[1][i]) * tr + 5) % R
+
+
+@pytest.fixture(scope="module")
+def circuit():
+    return S.Circuit(7, seed=11)
+
+
+def test_synthetic_circuit_is_satisfied(circuit):
+    c = circuit
+    assert len(c.cycles) > 8
+    for cyc in c.cycles:
+        vals = {c.cols[j][i] for j, i in cyc}
+        assert len(vals) == 1
+    # the last product of the last set closes the argument
+    assert c.z[-1][c.u] == 1
+    assert c.z[0][0] == 1
+
+
+def test_numerator_vanishes_on_base_domain(circuit):
+    nu = O.numerator(**circuit.base_inputs())
+    assert not nu.any()
+
+
+def test_numerator_flags_broken_gate_row(circuit):
+    c = circuit
+    inp = c.base_inputs()
+    row = next(r for r in range(3, c.u) if all((0, r) not in cyc for cyc in c.cycles))
+    adv = inp["advice"].copy()
+    adv[0 * c.n + row] = S.mont([c.cols[0][row] + 1])[0]
+    inp["advice"] = adv
+    nu = O.numerator(**inp)
+    assert [i for i in range(c.n) if nu[i].any()] == [row]
+
+
+def test_numerator_flags_broken_copy(circuit):
+    c = circuit
+    inp = c.base_inputs()
+    j, row = next(cell for cyc in c.cycles for cell in cyc if cell[0] == 4)
+    adv = inp["advice"].copy()
+    adv[j * c.n + row] = S.mont([c.cols[j][row] + 1])[0]
+    inp["advice"] = adv
+    nu = O.numerator(**inp)
+    assert [i for i in range(c.n) if nu[i].any()] == [row]
+
+
+def test_numerator_flags_wrong_set_chaining(circuit):
+    c = circuit
+    inp = c.base_inputs()
+    z = inp["permutation_products"].copy().reshape(3, c.n, 4)
+    inp["permutation_products"] = np.ascontiguousarray(z[[1, 0, 2]].reshape(-1, 4))
+    nu = O.numerator(**inp)
+    assert nu[0].any()
+
+
+def test_numerator_extended_quotient_degree(circuit):
+    c = circuit
+    inp = c.extended_inputs()
+    n, N = c.n, inp["n"]
+    assert N == 4 * n
+    nu = O.numerator(**inp)
+    h = O.mul_repeating(nu, c.t_evals)
+    coeffs = O.extended_to_coeff(h, c.domain)
+    top = (n - 1) * (S.DEGREE - 1)
+    assert not coeffs[top:].any()
+    assert coeffs[top - 1].any()
+    # a changed advice value breaks divisibility by X^n - 1
+    bad = dict(inp)
+    adv = inp["advice"].copy()
+    adv[5] = O.fr_op("add", adv[5], O.fr_from_int(1))
+    bad["advice"] = adv
+    coeffs = O.extended_to_coeff(O.mul_repeating(O.numerator(**bad), c.t_evals), c.domain)
+    assert coeffs[top:].any()
