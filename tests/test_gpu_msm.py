@@ -220,3 +220,37 @@ def test_msm_precomputed_context_modes(H, O, log_n):
     pts = ctx.points()
     assert O.g1_eq(H.g1_msm(pts, sc[:n]), out[0])
     ctx.free()
+
+
+def _skewed_montgomery(O, sc, seed):
+    """The 2^24 workload's skewed mix, in place on Montgomery scalars:
+    70% in [0, 2^16), 20% in {0, 1}, 10% left uniform."""
+    rng = np.random.default_rng(seed)
+    table = np.stack([O.fr_from_int(v) for v in range(1 << 16)])
+    u = rng.random(sc.shape[0])
+    small = u < 0.7
+    zo = (u >= 0.7) & (u < 0.9)
+    sc[small] = table[rng.integers(0, 1 << 16, size=int(small.sum()))]
+    sc[zo] = table[rng.integers(0, 2, size=int(zo.sum()))]
+    return sc
+
+
+@pytest.mark.slow
+def test_msm_2p24_uniform_and_skewed(H, O):
+    """2^24 resident points [t_i]G, uniform and skewed scalars in one call:
+    each result equals [<s, t>]G."""
+    n = 1 << 24
+    print("MSM 2^24: generating inputs", flush=True)
+    ts = O.random_fr(n, 2401)
+    dv_t = H.DeviceVec.upload(ts)
+    ctx = H.MsmContext.from_multiples(dv_t)
+    dv_t.free()
+    uni = O.random_fr(n, 2402)
+    skw = _skewed_montgomery(O, O.random_fr(n, 2403), 2404)
+    print("MSM 2^24: device batch", flush=True)
+    dv = H.DeviceVec.upload(np.concatenate([uni, skw]))
+    out = ctx.device_batch(dv, 0, n, 2)
+    for i, sc in enumerate((uni, skw)):
+        assert O.g1_eq(out[i], O.g1_mul(O.generator(), O.inner_product(sc, ts))), i
+    dv.free()
+    ctx.free()
