@@ -406,11 +406,13 @@ __global__ void __launch_bounds__(kBlock) k_bucket_sums(const uint32_t *q, const
 
 // Plane k of a window holds the buckets x-1 with bit k of x set, x in [1, B].
 // Its e-th member is x = ((e >> k) << (k + 1)) | (1 << k) | (e & (2^k - 1)).
+constexpr uint32_t kMaxWindow = 20;   // widest window any path selects (MH_WINDOW_CASES)
+
 struct PlaneLayout {
     uint32_t c;
-    uint32_t per_window;         // chunks per window
-    uint32_t first[17];          // first chunk of plane k within a window
-    uint32_t members[16];        // members of plane k
+    uint32_t per_window;                 // chunks per window
+    uint32_t first[kMaxWindow + 1];      // first chunk of plane k within a window
+    uint32_t members[kMaxWindow];        // members of plane k
 };
 
 __global__ void __launch_bounds__(kBlock) k_plane_partials(const uint32_t *bucket, Geometry g, PlaneLayout pl,
@@ -703,6 +705,7 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
     }
 
     // 7: bit-plane partial sums per (msm, window, plane)
+    MH_CHECK(g.c >= 1 && g.c <= kMaxWindow, (int)hipErrorInvalidValue, "unsupported MSM window width");
     PlaneLayout pl{};
     pl.c = g.c;
     uint32_t acc = 0;
@@ -712,7 +715,7 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
         acc += (pl.members[k] + plane_group - 1) / plane_group;
     }
     pl.first[g.c] = acc;
-    for (uint32_t k = g.c + 1; k < 17; ++k) pl.first[k] = 0xffffffffu;
+    for (uint32_t k = g.c + 1; k <= kMaxWindow; ++k) pl.first[k] = 0xffffffffu;
     pl.per_window = acc;
     const uint64_t nmw = (uint64_t)g.M * g.kw;
     const uint64_t npp = nmw * pl.per_window;

@@ -202,11 +202,12 @@ def test_generator_multiples_and_montgomery(H, O):
     ctx.free()
 
 
-@pytest.mark.parametrize("log_n", [10, 14, 17])
+@pytest.mark.parametrize("log_n", [10, 14, 17, 21])
 def test_msm_precomputed_context_modes(H, O, log_n):
     """Resident contexts use per-window point copies; compare against the
     one-shot (plain window) path and the closed form, with skewed and
-    uniform scalars, several MSMs per call."""
+    uniform scalars, several MSMs per call.  2^21 points select a 17-bit
+    window (wider than any one-shot window)."""
     n = 1 << log_n
     ts = O.random_fr(n, 300 + log_n)
     dv_t = H.DeviceVec.upload(ts)
