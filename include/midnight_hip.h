@@ -314,6 +314,33 @@ RustError midnight_sppark_bls12_381_numerator_context_eval(
     size_t lookup_product_offset, size_t lookup_input_offset, size_t lookup_table_offset,
     size_t lookup_count, size_t trash_offset, size_t trash_count);
 
+/* ---------------- grand products ---------------- */
+
+/* replaces bls12_381_g1_msm.cu:4303: in place, values[0] = initial,
+ * values[i] = initial * prod_{j < i} values_in[j] */
+RustError midnight_sppark_bls12_381_fr_shifted_prefix_product(RawFr *values, RawFr initial, size_t len);
+
+/* replaces bls12_381_g1_msm.cu:4342: permutation product columns of
+ * column_count host columns (column-major, len rows each) with their
+ * permuted identity values; sets of chunk_len columns chained through the
+ * row len - blinding_factors - 1, the last blinding_factors rows of set s
+ * taken from blind_values[s * blinding_factors ...]
+ * (plonk/permutation/prover.rs:355-515).  *out receives a device vector of
+ * ceil(column_count / chunk_len) * len values. */
+RustError midnight_sppark_bls12_381_fr_permutation_products(
+    void **out, const RawFr *columns, const RawFr *permutations, size_t len, size_t column_count,
+    size_t chunk_len, RawFr beta, RawFr gamma, RawFr omega, const RawFr *delta_powers,
+    const RawFr *blind_values, size_t blinding_factors);
+
+/* replaces bls12_381_g1_msm.cu:5738: the same products from a numerator
+ * context's Lagrange-form fixed and sigma columns (input_len rows, trace
+ * omega) and resident advice / instance device vectors. */
+RustError midnight_sppark_bls12_381_trace_context_permutation(
+    void **out, const void *raw_context, const void *raw_advice, size_t advice_cols,
+    const void *raw_instance, size_t instance_cols, RawFr beta, RawFr gamma,
+    const RawFr *delta_powers, size_t delta_powers_len, const RawFr *blind_values,
+    size_t blind_values_len, size_t blinding_factors);
+
 /* ---------------- backend extensions (no reference counterpart) ---------------- */
 
 /* Number of elements of a device vector (for callers and tests that hold

@@ -33,8 +33,10 @@
 
 #include "devvec.hpp"
 #include "fields.hpp"
+#include "grandproduct.hpp"
 #include "host_fr.hpp"
 #include "profile.hpp"
+#include "rowpow.hpp"
 #include "runtime.hpp"
 
 namespace mh {
@@ -52,8 +54,6 @@ constexpr int kBlock = 256;
 constexpr uint32_t kMaxSlots = 256;       // MAX_GRAPH_SCRATCH_SLOTS (midnight-gpu-prover expression.rs:23)
 constexpr uint32_t kLdsSlots = 6;         // 6 x 32 B x 256 lanes = 48 KB of LDS per block
 constexpr uint32_t kGlobalSlotBlocks = 1024;
-constexpr int kOmegaBits = 11;
-constexpr uint32_t kOmegaTab = 1u << kOmegaBits;
 
 struct DevGraph {
     const Fr *constants;
@@ -213,10 +213,7 @@ __device__ __forceinline__ Fr eval_graph(const DevGraph &g, size_t idx, const Nu
 
 __device__ __forceinline__ Fr omega_pow(size_t idx, const NumArgs &a)
 {
-    Fr r = load_fr(a.omega_tab + (idx & (kOmegaTab - 1)));
-    for (uint32_t l = 1; l < a.omega_levels; ++l)
-        r = r * load_fr(a.omega_tab + l * kOmegaTab + ((idx >> (kOmegaBits * l)) & (kOmegaTab - 1)));
-    return r;
+    return omega_pow_tab(idx, a.omega_tab, a.omega_levels);
 }
 
 // Permutation terms (numerator_permutation_kernel, bls12_381_g1_msm.cu:674-755;
@@ -419,27 +416,6 @@ struct GraphSet {
     }
 };
 
-// omega^(j * 2^(11 l)) for j < 2048 and l < levels, levels = ceil(log2(len) / 11).
-void omega_table(DevBuf<Fr> &tab, uint32_t &levels, const HostFr &omega, size_t len, hipStream_t st)
-{
-    uint32_t bits = 0;
-    while (bits < 64 && (1ull << bits) < len) ++bits;
-    levels = bits <= kOmegaBits ? 1 : (bits + kOmegaBits - 1) / kOmegaBits;
-    std::vector<HostFr> h((size_t)levels * kOmegaTab);
-    HostFr step = omega;
-    for (uint32_t l = 0; l < levels; ++l) {
-        HostFr acc = HostFr::one();
-        for (uint32_t j = 0; j < kOmegaTab; ++j) {
-            h[(size_t)l * kOmegaTab + j] = acc;
-            acc = acc * step;
-        }
-        step = acc;   // step^2048
-    }
-    tab.alloc(h.size());
-    MH_HIP(hipMemcpyAsync(tab.get(), h.data(), h.size() * 32, hipMemcpyHostToDevice, st));
-    MH_HIP(hipStreamSynchronize(st));
-}
-
 template <class Kernel>
 void launch_rows(NumArgs a, uint32_t slots, Kernel lds_kernel, Kernel global_kernel, hipStream_t st)
 {
@@ -603,7 +579,7 @@ extern "C" RustError midnight_sppark_bls12_381_eval_numerator(
     a.trash_count = (uint32_t)trash_count;
     a.trash_cosets = up(d_tc, trash_cosets, trash_count * len, st);
     a.trash_sel = up_array(d_sel, trash_selector_fixed_indices, trash_count, st);
-    omega_table(d_omega, a.omega_levels, to_hfr(omega), len, st);
+    build_omega_table(d_omega, a.omega_levels, to_hfr(omega), len, st);
     a.omega_tab = d_omega.get();
 
     GraphSet gs;
@@ -723,7 +699,7 @@ extern "C" RustError midnight_sppark_bls12_381_numerator_context_create(
     }
     for (size_t i = 0; i < trash_count; ++i) c->graphs.add(trash_graphs[i], st);
     c->graphs.finish(st);
-    omega_table(c->omega_tab, c->omega_levels, c->omega, len, st);
+    build_omega_table(c->omega_tab, c->omega_levels, c->omega, len, st);
     MH_HIP(hipStreamSynchronize(st));
     *out = c.release();
     return ok();
@@ -822,6 +798,59 @@ extern "C" RustError midnight_sppark_bls12_381_numerator_context_eval(
     a.graphs = c->graphs.table.get();
     run_numerator(a, c->graphs.max_slots, st);
     MH_HIP(hipStreamSynchronize(st));
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_sppark_bls12_381_trace_context_permutation(
+    void **out, const void *raw_context, const void *raw_advice, size_t advice_cols, const void *raw_instance,
+    size_t instance_cols, RawFr beta, RawFr gamma, const RawFr *delta_powers, size_t delta_powers_len,
+    const RawFr *blind_values, size_t blind_values_len, size_t blinding_factors)
+{
+    if (out == nullptr || raw_context == nullptr || delta_powers == nullptr)
+        return make_error((int)hipErrorInvalidValue, "invalid resident permutation arguments");
+    *out = nullptr;
+    const auto *c = reinterpret_cast<const NumeratorContext *>(raw_context);
+    const size_t len = c->input_len;
+    const size_t column_count = c->perm_cols;
+    const size_t chunk_len = c->chunk_len;
+    const auto *advice = reinterpret_cast<const DeviceVec *>(raw_advice);
+    const auto *instance = reinterpret_cast<const DeviceVec *>(raw_instance);
+    auto has_len = [](const DeviceVec *v, size_t expected) {
+        return expected == 0 ? (v == nullptr || v->len == 0) : (v != nullptr && v->len == expected);
+    };
+    if (column_count == 0 || chunk_len == 0 || delta_powers_len != column_count ||
+        !has_len(advice, advice_cols * len) || !has_len(instance, instance_cols * len))
+        return make_error((int)hipErrorInvalidValue, "resident permutation vector length mismatch");
+    const size_t set_count = (column_count + chunk_len - 1) / chunk_len;
+    if (blinding_factors >= len || blind_values_len != set_count * blinding_factors ||
+        (blind_values_len != 0 && blind_values == nullptr))
+        return make_error((int)hipErrorInvalidValue, "resident permutation blind mismatch");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<Fr> d_blinds;
+    if (blind_values_len) {
+        d_blinds.alloc(blind_values_len);
+        MH_HIP(hipMemcpyAsync(d_blinds.get(), blind_values, blind_values_len * 32, hipMemcpyHostToDevice, st));
+    }
+    std::vector<HostFr> dp(column_count);
+    for (size_t j = 0; j < column_count; ++j) dp[j] = to_hfr(delta_powers[j]);
+    PermColumnsView v;
+    v.fixed = c->fixed_lagrange.get();
+    v.fixed_cols = (uint32_t)c->fixed_cols;
+    v.advice = advice_cols ? advice->values.get() : nullptr;
+    v.advice_cols = (uint32_t)advice_cols;
+    v.instance = instance_cols ? instance->values.get() : nullptr;
+    v.instance_cols = (uint32_t)instance_cols;
+    v.refs = c->perm_columns.get();
+    auto dv = std::make_unique<DeviceVec>();
+    dv->lane = current_lane();
+    dv->len = set_count * len;
+    dv->values.alloc(dv->len);
+    permutation_products_device(dv->values.get(), len, column_count, chunk_len, v, c->perm_lagrange.get(),
+                                dp.data(), to_hfr(beta), to_hfr(gamma), c->trace_omega, d_blinds.get(),
+                                blinding_factors, st);
     *out = dv.release();
     return ok();
     MH_CATCH

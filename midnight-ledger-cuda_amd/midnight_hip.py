@@ -142,6 +142,11 @@ def _declare(L):
         "midnight_sppark_bls12_381_numerator_context_eval": (
             E, [vpp, vp, vp, c_size, c_size, c_size, c_size, c_size, vp, c_size] + [RawFr] * 5
             + [c_size] * 8),
+        "midnight_sppark_bls12_381_fr_shifted_prefix_product": (E, [u64p, RawFr, c_size]),
+        "midnight_sppark_bls12_381_fr_permutation_products": (
+            E, [vpp, u64p, u64p, c_size, c_size, c_size, RawFr, RawFr, RawFr, u64p, vp, c_size]),
+        "midnight_sppark_bls12_381_trace_context_permutation": (
+            E, [vpp, vp, vp, c_size, vp, c_size, RawFr, RawFr, u64p, c_size, vp, c_size, c_size]),
         "midnight_hip_profile_enable": (None, [ctypes.c_int]),
         "midnight_hip_profile_read": (
             ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]),
@@ -613,6 +618,19 @@ class NumeratorContext:
             trash_offset, trash_count))
         return DeviceVec(h.value)
 
+    def trace_permutation(self, advice, advice_cols, instance, instance_cols, *, beta, gamma, delta_powers,
+                          blinds, blinding_factors):
+        """trace_context_permutation: permutation products from resident
+        Lagrange advice / instance (DeviceVec or None); returns a DeviceVec."""
+        dp = _fr_array(delta_powers)
+        bl = None if blinds is None or len(blinds) == 0 else _fr_array(blinds)
+        h = ctypes.c_void_p()
+        check(lib().midnight_sppark_bls12_381_trace_context_permutation(
+            ctypes.byref(h), self.handle, advice.handle if advice is not None else None, advice_cols,
+            instance.handle if instance is not None else None, instance_cols, RawFr.of(beta), RawFr.of(gamma),
+            P(dp), dp.shape[0], _ptr(bl), 0 if bl is None else bl.shape[0], blinding_factors))
+        return DeviceVec(h.value)
+
     def free(self):
         if self.handle and self.handle.value:
             lib().midnight_sppark_bls12_381_numerator_context_free(self.handle)
@@ -623,3 +641,26 @@ class NumeratorContext:
             self.free()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- grand products
+
+def fr_shifted_prefix_product(values, initial):
+    """out[0] = initial, out[i] = initial * prod_{j<i} values[j]"""
+    a = np.array(_fr_array(values), copy=True)
+    check(lib().midnight_sppark_bls12_381_fr_shifted_prefix_product(P(a), RawFr.of(initial), a.shape[0]))
+    return a
+
+
+def fr_permutation_products(columns, permutations, n, chunk_len, *, beta, gamma, omega, delta_powers, blinds,
+                            blinding_factors):
+    """Permutation product columns (sets * n values) as a DeviceVec."""
+    cols = _fr_array(columns)
+    perm = _fr_array(permutations)
+    dp = _fr_array(delta_powers)
+    bl = None if blinding_factors == 0 else _fr_array(blinds)
+    h = ctypes.c_void_p()
+    check(lib().midnight_sppark_bls12_381_fr_permutation_products(
+        ctypes.byref(h), P(cols), P(perm), n, cols.shape[0] // n, chunk_len, RawFr.of(beta), RawFr.of(gamma),
+        RawFr.of(omega), P(dp), _ptr(bl), blinding_factors))
+    return DeviceVec(h.value)

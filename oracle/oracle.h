@@ -193,6 +193,16 @@ void orc_graph_eval_rows(orc_fr *out, const orc_graph *g, const orc_numerator_in
 /* Evaluator::evaluate_numerator for one proof (evaluation.rs:3114-3528) */
 void orc_eval_numerator(orc_fr *out, const orc_numerator_inputs *in, int threads);
 
+/* Permutation grand products (permutation/prover.rs:355-515); out gets
+ * ceil(column_count / chunk_len) sets of len values. */
+void orc_permutation_products(orc_fr *out, const orc_fr *columns, const orc_fr *sigmas, size_t len,
+                              size_t column_count, size_t chunk_len, const orc_fr *beta, const orc_fr *gamma,
+                              const orc_fr *omega, const orc_fr *delta_powers, const orc_fr *blinds,
+                              size_t blinding, int threads);
+
+/* shifted_prefix_product (permutation/prover.rs:36-43); out may not alias values */
+void orc_shifted_prefix_product(orc_fr *out, const orc_fr *values, size_t len, const orc_fr *initial);
+
 /* ---- RNG : rand_core 0.6 seed_from_u64 (PCG32) + rand_chacha 0.3 ChaCha20 ---- */
 typedef struct {
     uint32_t key[8];

@@ -102,6 +102,9 @@ def _declare(L):
         "orc_chacha20_block": (V, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
                                    ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32)]),
         "orc_eval_numerator": (V, [u64p, ctypes.c_void_p, ctypes.c_int]),
+        "orc_shifted_prefix_product": (V, [u64p, u64p, ctypes.c_size_t, u64p]),
+        "orc_permutation_products": (V, [u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                                         u64p, u64p, u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
         "orc_graph_eval_rows": (V, [u64p, ctypes.c_void_p, ctypes.c_void_p, u64p, ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -511,4 +514,29 @@ def graph_rows(n, graph, *, beta=None, gamma=None, theta=None, trash_challenge=N
     out = np.zeros((n, 4), dtype=np.uint64)
     lib().orc_graph_eval_rows(P(out), ctypes.addressof(g), ctypes.addressof(s),
                               None if prev is None else P(prev), threads or os.cpu_count())
+    return out
+
+
+def permutation_products(columns, sigmas, n, chunk_len, beta, gamma, omega, delta_powers, blinds, blinding,
+                         threads=None):
+    """orc_permutation_products: (sets * n, 4) grand products."""
+    cols = np.ascontiguousarray(columns, dtype=np.uint64).reshape(-1, 4)
+    sig = np.ascontiguousarray(sigmas, dtype=np.uint64).reshape(-1, 4)
+    ncol = cols.shape[0] // n
+    sets = -(-ncol // chunk_len)
+    dp = np.ascontiguousarray(delta_powers, dtype=np.uint64).reshape(-1, 4)
+    bl = np.ascontiguousarray(blinds, dtype=np.uint64).reshape(-1, 4) if blinding else fr(1)
+    out = fr(sets * n)
+    lib().orc_permutation_products(P(out), P(cols), P(sig), n, ncol, chunk_len,
+                                   P(np.ascontiguousarray(beta, dtype=np.uint64)),
+                                   P(np.ascontiguousarray(gamma, dtype=np.uint64)),
+                                   P(np.ascontiguousarray(omega, dtype=np.uint64)), P(dp), P(bl), blinding,
+                                   threads or os.cpu_count())
+    return out
+
+
+def shifted_prefix_product(values, initial):
+    v = np.ascontiguousarray(values, dtype=np.uint64).reshape(-1, 4)
+    out = fr(v.shape[0])
+    lib().orc_shifted_prefix_product(P(out), P(v), v.shape[0], P(np.ascontiguousarray(initial, dtype=np.uint64)))
     return out

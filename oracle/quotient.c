@@ -314,3 +314,90 @@ void orc_eval_numerator(orc_fr *out, const orc_numerator_inputs *in, int threads
         }
     }
 }
+
+/* Permutation grand products (permutation/prover.rs:355-515), one column
+ * set of chunk_len columns at a time: modified[i] = prod_j (v_j[i] +
+ * delta_j*omega^i*beta + gamma) / prod_j (v_j[i] + beta*sigma_j[i] + gamma)
+ * (denominators batch-inverted; zero inverts to zero), z[0] = last_z,
+ * z[i] = z[i-1] * modified[i-1], the last `blinding` rows replaced by the
+ * set's blinds, last_z = z[len - blinding - 1] (starts at one). */
+void orc_permutation_products(orc_fr *out, const orc_fr *columns, const orc_fr *sigmas, size_t len,
+                              size_t column_count, size_t chunk_len, const orc_fr *beta, const orc_fr *gamma,
+                              const orc_fr *omega, const orc_fr *delta_powers, const orc_fr *blinds,
+                              size_t blinding, int threads)
+{
+    const size_t sets = (column_count + chunk_len - 1) / chunk_len;
+    orc_fr last_z; orc_fr_one(&last_z);
+    orc_fr *num = (orc_fr *)malloc(len * sizeof(orc_fr));
+    orc_fr *den = (orc_fr *)malloc(len * sizeof(orc_fr));
+    orc_fr *pre = (orc_fr *)malloc(len * sizeof(orc_fr));
+    if (threads <= 0) threads = 1;
+    for (size_t s = 0; s < sets; s++) {
+        size_t c0 = s * chunk_len, c1 = c0 + chunk_len < column_count ? c0 + chunk_len : column_count;
+        const size_t nchunks = (size_t)threads * 4;
+        const size_t chunk = (len + nchunks - 1) / nchunks;
+        #pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+        for (size_t ch = 0; ch < nchunks; ch++) {
+          size_t start = ch * chunk, end = start + chunk < len ? start + chunk : len;
+          orc_fr wi;
+          uint64_t e[4] = {(uint64_t)start, 0, 0, 0};
+          orc_fr_pow(&wi, omega, e);
+          for (size_t i = start; i < end; i++, orc_fr_mul(&wi, &wi, omega)) {
+            orc_fr t, n1, d1;
+            orc_fr_one(&n1);
+            orc_fr_one(&d1);
+            for (size_t j = c0; j < c1; j++) {
+                const orc_fr *v = &columns[j * len + i];
+                orc_fr_mul(&t, beta, &sigmas[j * len + i]);
+                orc_fr_add(&t, &t, gamma);
+                orc_fr_add(&t, &t, v);
+                orc_fr_mul(&d1, &d1, &t);
+                orc_fr_mul(&t, &delta_powers[j], &wi);
+                orc_fr_mul(&t, &t, beta);
+                orc_fr_add(&t, &t, gamma);
+                orc_fr_add(&t, &t, v);
+                orc_fr_mul(&n1, &n1, &t);
+            }
+            num[i] = n1;
+            den[i] = d1;
+          }
+        }
+        /* batch inversion of the denominators (ff BatchInvert): zeros stay zero */
+        orc_fr acc; orc_fr_one(&acc);
+        for (size_t i = 0; i < len; i++) {
+            pre[i] = acc;
+            if (!orc_fr_is_zero(&den[i])) orc_fr_mul(&acc, &acc, &den[i]);
+        }
+        orc_fr inv; orc_fr_inv(&inv, &acc);
+        for (size_t i = len; i-- > 0;) {
+            if (orc_fr_is_zero(&den[i])) continue;
+            orc_fr t;
+            orc_fr_mul(&t, &inv, &pre[i]);
+            orc_fr_mul(&inv, &inv, &den[i]);
+            den[i] = t;
+        }
+        orc_fr *z = out + s * len;
+        z[0] = last_z;
+        for (size_t i = 1; i < len; i++) {
+            orc_fr m;
+            orc_fr_mul(&m, &num[i - 1], &den[i - 1]);
+            orc_fr_mul(&z[i], &z[i - 1], &m);
+        }
+        for (size_t i = len - blinding; i < len; i++) z[i] = blinds[s * blinding + (i - (len - blinding))];
+        last_z = z[len - blinding - 1];
+    }
+    free(num); free(den); free(pre);
+}
+
+/* shifted_prefix_product (permutation/prover.rs:36-43): out[0] = initial,
+ * out[i] = out[i-1] * values[i-1] */
+void orc_shifted_prefix_product(orc_fr *out, const orc_fr *values, size_t len, const orc_fr *initial)
+{
+    if (len == 0) return;
+    orc_fr acc = *initial, next;
+    for (size_t i = 0; i < len; i++) {
+        orc_fr_mul(&next, &acc, &values[i]);
+        out[i] = acc;
+        acc = next;
+    }
+}

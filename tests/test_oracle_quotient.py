@@ -218,3 +218,16 @@ def test_numerator_extended_quotient_degree(circuit):
     bad["advice"] = adv
     coeffs = O.extended_to_coeff(O.mul_repeating(O.numerator(**bad), c.t_evals), c.domain)
     assert coeffs[top:].any()
+
+
+def test_oracle_permutation_products_match_prover_restatement(circuit):
+    """The C grand products equal the circuit builder's (a separate Python
+    restatement of permutation/prover.rs), and the last set closes to one
+    at the last usable row."""
+    c = circuit
+    cols = S.mont([x for col in c.cols for x in col])
+    sig = S.mont([x for col in c.sigma for x in col])
+    z = O.permutation_products(cols, sig, c.n, S.CHUNK, S.mont([c.beta])[0], S.mont([c.gamma])[0],
+                               S.mont([c.omega])[0], S.mont(c.delta_pows), S.mont(c.blinds), S.BLINDING)
+    assert np.array_equal(z, S.mont([x for zz in c.z for x in zz]))
+    assert S.ints(z[2 * c.n + c.u][None, :])[0] == 1
