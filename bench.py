@@ -266,6 +266,288 @@ def bench_batch(args, H):
     return res
 
 
+def bench_kzg(args, H):
+    """Plain KZG commitment of 2^log_n Lagrange values (default 2^16): the
+    values are uploaded from the host, inverse-NTT'd to coefficients and
+    committed with one MSM against the SRS [tau^i]G, resident on the device
+    (tau from the seed).  A step is one commitment; the value is
+    commitments per second with the values starting in host memory, as the
+    prover's commit_lagrange receives them."""
+    log_n = args.log_n if args.log_n != 20 else 16
+    n = 1 << log_n
+    rng = np.random.default_rng(args.seed)
+    tau = random_canonical(1, rng)
+    tau_dv = device_fr(H, tau)
+    tau_m = tau_dv.download()[0]
+    tau_dv.free()
+    powers = H.fr_shifted_prefix_product(np.tile(tau_m, (n, 1)), MONT_ONE)
+    pw = H.DeviceVec.upload(powers)
+    ctx = H.MsmContext.from_multiples(pw)
+    pw.free()
+    evals = random_canonical(n, rng)      # any value < r is a valid Montgomery element
+
+    def commit():
+        dv = H.DeviceVec.upload(evals)
+        dv.ntt_zeta(log_n, True, MONT_ONE, MONT_ONE, MONT_ONE)
+        out = ctx.device_batch(dv, 0, n, 1)
+        dv.free()
+        return out
+
+    for _ in range(args.warmup):
+        commit()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        commit()
+        lat.append(time.perf_counter() - t1)
+    dt = (time.perf_counter() - t0) / args.steps
+    lat.sort()
+    res = {
+        "metric": "KZG commit 2^%d commitments/s" % log_n, "value": round(1.0 / dt, 2), "unit": "commits/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic uniform Fr values on the host; SRS [tau^i]G built on the device",
+        "config": {"workload": "KZG commit: upload 2^%d Lagrange values, iNTT, MSM against [tau^i]G" % log_n},
+        "p50_ms": round(lat[len(lat) // 2] * 1e3, 3), "p99_ms": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))] * 1e3, 3),
+    }
+    ctx.free()
+    return res
+
+
+MONT_ONE = np.array([0x00000001fffffffe, 0x5884b7fa00034802, 0x998c4fefecbc4ff5, 0x1824b159acc5056f],
+                    dtype=np.uint64)
+
+
+def synthetic_circuit(H, k, seed):
+    """One instance of the benchmark circuit family at 2^k rows: advice
+    a, b, c uniform, d = a*b + c, e copying a cells of a (copy cycles of two
+    over random rows), fixed q = 1 on usable rows, s1, s2 uniform.  Returns
+    (fixed, advice) as Montgomery arrays and the permutation mapping."""
+    import random as _random
+    R = R_MOD
+    n = 1 << k
+    u = n - 6
+    rnd = _random.Random(seed)
+    a = [rnd.randrange(R) for _ in range(n)]
+    b = [rnd.randrange(R) for _ in range(n)]
+    c = [rnd.randrange(R) for _ in range(n)]
+    d = [(a[i] * b[i] + c[i]) % R if i < u else rnd.randrange(R) for i in range(n)]
+    e = [rnd.randrange(R) for _ in range(n)]
+    mapping = np.zeros((8, n, 2), dtype=np.uint32)
+    mapping[:, :, 0] = np.arange(8, dtype=np.uint32)[:, None]
+    mapping[:, :, 1] = np.arange(n, dtype=np.uint32)[None, :]
+    rows = list(range(u))
+    rnd.shuffle(rows)
+    for t in range(0, u // 4, 1):
+        i, j = rows[2 * t], rows[2 * t + 1]
+        e[i] = a[j]
+        mapping[4, i] = (0, j)
+        mapping[0, j] = (4, i)
+    q = [1 if i < u else 0 for i in range(n)]
+    s1 = [rnd.randrange(R) if i < u else 0 for i in range(n)]
+    s2 = [rnd.randrange(R) if i < u else 0 for i in range(n)]
+
+    def mont(cols):
+        flat = np.array([[(v >> (64 * j)) & (2**64 - 1) for j in range(4)] for col in cols for v in col],
+                        dtype=np.uint64)
+        dv = device_fr(H, flat)
+        out = dv.download()
+        dv.free()
+        return out
+
+    return mont([q, s1, s2]), mont([a, b, c, d, e]), mapping.reshape(-1)
+
+
+def bench_prove(args, H):
+    """One full proof of the benchmark circuit at k = log_n (default 20):
+    5 advice and 3 fixed columns, gate q(ab + c - d), 8-column copy
+    permutation, extended domain 4x (2^22).  A step is one
+    midnight_hip_plonk_prove call: advice upload, commitments, permutation
+    products, numerator and quotient on the 2^22 coset, evaluations and the
+    KZG multi-open, returning the proof bytes.  Key and SRS are resident."""
+    import numerator_graph as G
+    k = args.log_n
+    rng = np.random.default_rng(args.seed)
+    log("building the circuit, SRS and key for k = %d" % k)
+    tau = random_canonical(1, rng)
+    tdv = device_fr(H, tau)
+    tau_m = tdv.download()[0]
+    tdv.free()
+    pw = H.DeviceVec.upload(H.fr_shifted_prefix_product(np.tile(tau_m, (1 << k, 1)), MONT_ONE))
+    srs = H.MsmContext.from_multiples(pw)
+    pw.free()
+    gate = ("prod", ("fixed", 0, 0), ("sum", ("sum", ("prod", ("advice", 0, 0), ("advice", 1, 0)),
+                                             ("advice", 2, 0)), ("neg", ("advice", 3, 0))))
+    graph = G.RawGraphBuffers(G.custom_gates_graph([gate]))
+    perm = [(1, i) for i in range(5)] + [(0, i) for i in range(3)]
+    fixed, adv, mapping = synthetic_circuit(H, k, 2000 + k)
+    pk = H.PlonkKey(k, srs, 5, fixed, 5, [(c, 0) for c in range(5)], [(c, 0) for c in range(3)], perm, mapping,
+                    graph, b"benchmark circuit family")
+    for i in range(args.warmup):
+        pk.prove(adv, 100 + i)
+    H.profile_enable(True)
+    lat = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        t1 = time.perf_counter()
+        proof = pk.prove(adv, 200 + i)
+        lat.append(time.perf_counter() - t1)
+    dt = (time.perf_counter() - t0) / args.steps
+    msm_ms, msm_cnt = H.profile_read("msm_total")
+    acc_ms, _ = H.profile_read("msm_accumulate")
+    num_ms, num_cnt = H.profile_read("numerator")
+    H.profile_enable(False)
+    peak, _, _ = H.valu_peak()
+    n = 1 << k
+    num_macs = QUOTIENT_FR_MULS_PER_ROW * MACS_PER_FR_MUL * 4 * n
+    t_num = num_ms / 1e3 / max(num_cnt, 1)
+    lat.sort()
+    res = {
+        "metric": "k=%d proof latency (proofs/s)" % k, "value": round(1.0 / dt, 3), "unit": "proofs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic satisfying circuit of the benchmark family; SRS [tau^i]G built on the device",
+        "config": {"workload": "full proof, k=%d (2^%d rows), 5 advice / 3 fixed, gate q(ab+c-d), 8-column "
+                               "permutation, 4x extended domain" % (k, k), "proof_bytes": len(proof)},
+        "p50_ms": round(lat[len(lat) // 2] * 1e3, 3),
+        "stage_ms_per_proof": {"msm_total": round(msm_ms / args.steps, 3),
+                               "msm_accumulate": round(acc_ms / args.steps, 3),
+                               "numerator": round(num_ms / args.steps, 3)},
+        "roofline": {"bound": "valu", "kernel": "numerator", "achieved": round(num_macs / t_num / 1e12, 3),
+                     "peak": round(peak / 1e12, 3), "unit": "TMAC/s", "frac": round(num_macs / t_num / peak, 4),
+                     "traffic": None},
+    }
+    if args.cpu_seconds > 0:
+        res["cpu_baseline"] = cpu_baseline_prove(args.cpu_seconds)
+    pk.free()
+    srs.free()
+    return res
+
+
+def cpu_baseline_prove(seconds):
+    """The reference-algorithm CPU prover (oracle MSM / FFT / numerator /
+    products, protocol restated in tests/plonk_ref.py) on a bounded sample:
+    proofs of the same circuit family at k = 14, repeated for about
+    `seconds`."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import plonk_ref
+    import synth_circuit
+    threads = os.cpu_count() or 1
+    c = synth_circuit.Circuit(14, seed=14)
+    key = plonk_ref.CpuKey(c, 12345, threads)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        plonk_ref.cpu_prove(key, c.cols[:5], reps, threads)
+        reps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(reps / dt, 4), "unit": "proofs/s", "cores": threads, "kind": "port",
+            "sample": "%d proofs at k=14 of the same circuit family (CPU reference prover), %d threads, %.1f s"
+                      % (reps, threads, dt)}
+
+
+def bench_serve(args, H):
+    """Serving: a Poisson stream of proof requests for the benchmark circuit
+    family at k in {12, 14, 16, 18} (mix 50/30/15/5 %), SRS of 2^20 points
+    and the four proving keys resident, HBM-budgeted admission.  The same
+    stream is served one proof at a time (one slot) and with continuous
+    batching (--slots); the value is the batched proofs/s, with p50/p99
+    latency (submission to proof) for both."""
+    import threading
+    import numerator_graph as G
+    rng = np.random.default_rng(args.seed)
+    log("building the 2^20 SRS and the keys for k = 12, 14, 16, 18")
+    tau = random_canonical(1, rng)
+    tdv = device_fr(H, tau)
+    tau_m = tdv.download()[0]
+    tdv.free()
+    srs_n = 1 << 20
+    pw = H.DeviceVec.upload(H.fr_shifted_prefix_product(np.tile(tau_m, (srs_n, 1)), MONT_ONE))
+    srs = H.MsmContext.from_multiples(pw)
+    pw.free()
+    gate = ("prod", ("fixed", 0, 0), ("sum", ("sum", ("prod", ("advice", 0, 0), ("advice", 1, 0)),
+                                             ("advice", 2, 0)), ("neg", ("advice", 3, 0))))
+    graph = G.RawGraphBuffers(G.custom_gates_graph([gate]))
+    perm = [(1, i) for i in range(5)] + [(0, i) for i in range(3)]
+    ks = [12, 14, 16, 18]
+    keys, advice = {}, {}
+    for k in ks:
+        fixed, adv, mapping = synthetic_circuit(H, k, 1000 + k)
+        keys[k] = H.PlonkKey(k, srs, 5, fixed, 5, [(c, 0) for c in range(5)], [(c, 0) for c in range(3)], perm,
+                             mapping, graph, b"benchmark circuit family")
+        advice[k] = adv
+    mix = rng.choice(ks, size=args.requests, p=[0.5, 0.3, 0.15, 0.05])
+    # one-at-a-time cost of each k (warm), for the offered load
+    cost = {}
+    for k in ks:
+        keys[k].prove(advice[k], 1)
+        t0 = time.perf_counter()
+        keys[k].prove(advice[k], 2)
+        cost[k] = time.perf_counter() - t0
+    mean_cost = sum(cost[k] * p for k, p in zip(ks, [0.5, 0.3, 0.15, 0.05]))
+    rate = args.load / mean_cost          # requests per second offered
+    gaps = rng.exponential(1.0 / rate, size=args.requests)
+    arrivals = np.cumsum(gaps)
+    log("single-proof ms: %s; offered load %.1f req/s (%.1f x one-at-a-time capacity)"
+        % ({k: round(v * 1e3, 2) for k, v in cost.items()}, rate, args.load))
+
+    def serve(slots):
+        server = H.ProofServer(slots, args.hbm_budget_gb << 30)
+        jobs = [None] * args.requests
+        t_start = time.perf_counter()
+
+        def submitter():
+            for i in range(args.requests):
+                delay = t_start + arrivals[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                jobs[i] = (time.perf_counter(), server.submit(keys[int(mix[i])], advice[int(mix[i])], 10 + i))
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat = []
+        for i in range(args.requests):
+            while jobs[i] is None:
+                time.sleep(0.0005)
+            t_sub, jid = jobs[i]
+            server.wait(jid)
+            lat.append(time.perf_counter() - t_sub)
+        th.join()
+        makespan = time.perf_counter() - t_start
+        server.free()
+        lat = sorted(lat)
+        pct = lambda q: lat[min(len(lat) - 1, int(np.ceil(q * len(lat))) - 1)]   # nearest rank
+        return {"proofs_per_s": round(args.requests / makespan, 2), "p50_ms": round(pct(0.5) * 1e3, 2),
+                "p99_ms": round(pct(0.99) * 1e3, 2), "makespan_s": round(makespan, 3)}
+
+    for _ in range(args.warmup):
+        serve(args.slots)
+    log("serving one proof at a time")
+    single = serve(1)
+    log("serving with %d slots" % args.slots)
+    batched = serve(args.slots)
+    res = {
+        "metric": "serving proofs/s (k in {12,14,16,18}, Poisson arrivals), batched vs one-at-a-time",
+        "value": batched["proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "steps": args.requests,
+        "warmup": args.warmup, "ms_per_step": round(1e3 / batched["proofs_per_s"], 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic satisfying circuits of the benchmark family, Poisson arrivals from the seed",
+        "config": {"workload": "proof server, %d requests, k mix 12/14/16/18 at 50/30/15/5%%, SRS 2^20 resident, "
+                               "%d slots, HBM budget %d GiB, offered load %.1f x one-at-a-time capacity"
+                               % (args.requests, args.slots, args.hbm_budget_gb, args.load)},
+        "batched": batched, "one_at_a_time": single,
+        "speedup": round(batched["proofs_per_s"] / single["proofs_per_s"], 3),
+        "single_proof_ms": {str(k): round(v * 1e3, 3) for k, v in cost.items()},
+    }
+    for k in ks:
+        keys[k].free()
+    srs.free()
+    return res
+
+
 QUOTIENT_FR_MULS_PER_ROW = 52      # gate 3, permutation 49 (three sets over 8 columns, chunk 3)
 MACS_PER_FR_MUL = 136              # 2 * 8^2 + 8 32x32->64 products per Montgomery product
 QUOTIENT_BYTES_PER_ROW = 23 * 32   # 5 advice + 3 fixed + 8 sigma + 3 z + l0, l_last, l_active + output
@@ -372,11 +654,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="msm", choices=["msm", "ntt", "batch", "quotient"])
+    ap.add_argument("--workload", default="msm", choices=["msm", "ntt", "batch", "quotient", "kzg", "serve", "prove"])
     ap.add_argument("--log-n", type=int, default=20)
     ap.add_argument("--msms-per-step", type=int, default=4)
     ap.add_argument("--skewed", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--requests", type=int, default=200, help="serving: requests in the stream")
+    ap.add_argument("--slots", type=int, default=8, help="serving: concurrent proofs")
+    ap.add_argument("--load", type=float, default=4.0, help="serving: offered load / one-at-a-time capacity")
+    ap.add_argument("--hbm-budget-gb", type=int, default=64, help="serving: HBM admission budget")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU-baseline sample length (0 disables)")
     args = ap.parse_args()
@@ -387,7 +673,7 @@ def main():
         return
     import midnight_hip as H
     H.lib()
-    run = {"msm": bench_msm, "ntt": bench_ntt, "batch": bench_batch, "quotient": bench_quotient}[args.workload]
+    run = {"msm": bench_msm, "ntt": bench_ntt, "batch": bench_batch, "quotient": bench_quotient, "kzg": bench_kzg, "serve": bench_serve, "prove": bench_prove}[args.workload]
     res = run(args, H)
     print(json.dumps(res), flush=True)
 

@@ -381,6 +381,53 @@ RustError midnight_hip_g1_msm_context_points_download(RawG1Affine *out, const vo
 RustError midnight_hip_g1_msm_context_last_stats(const void *context, uint64_t *entries,
                                                  uint32_t *window_bits);
 
+/* ---- proof pipeline (create_proof of plonk/prover.rs:1019-1076 for circuits
+ * made of custom gates and a copy permutation; no lookups, trashcans or
+ * instance columns).  For provers that keep a circuit's key and the SRS
+ * resident and want whole proofs from one call. ---- */
+
+/* Proving key.  srs_context: an MSM context whose first srs_points points
+ * are [tau^i]G (>= 2^k); not owned, must outlive the key.  fixed: fixed_cols
+ * columns of 2^k Lagrange values.  advice_queries / fixed_queries: pairs
+ * (column, rotation as int32 bits), in the constraint system's query order.
+ * perm_columns: the permuted columns in order; perm_mapping: for permuted
+ * column j and row i, the pair (j', i') of the next cell of its cycle
+ * (itself when not copied).  gate: the custom-gate graph (Horner in y).
+ * description: bytes hashed into the verifying-key representative. */
+RustError midnight_hip_plonk_keygen(void **out, uint32_t k, void *srs_context, size_t srs_points,
+                                    uint32_t degree, const RawFr *fixed, size_t fixed_cols,
+                                    size_t advice_cols, const uint32_t *advice_queries,
+                                    size_t advice_query_count, const uint32_t *fixed_queries,
+                                    size_t fixed_query_count, const RawColumnRef *perm_columns,
+                                    size_t perm_cols, const uint32_t *perm_mapping, RawGraph gate,
+                                    const char *description);
+void midnight_hip_plonk_key_free(void *pk);
+/* Verifying data: fixed and permutation commitments, transcript representative. */
+RustError midnight_hip_plonk_vk(const void *pk, RawG1Projective *fixed_commitments,
+                                RawG1Projective *permutation_commitments, RawFr *transcript_repr);
+/* One proof: advice = advice_cols columns of 2^k Lagrange values (rows from
+ * 2^k - blinding - 1 on are overwritten with blinding values), blinding
+ * drawn from ChaCha20Rng::seed_from_u64(rng_seed).  *proof_len receives the
+ * proof size; the call fails if proof_capacity is smaller. */
+RustError midnight_hip_plonk_prove(void *pk, const RawFr *advice, size_t advice_len, uint64_t rng_seed,
+                                   uint8_t *proof, size_t proof_capacity, size_t *proof_len);
+
+/* Proof server: `slots` worker threads (lanes 1..slots, at most 8) prove
+ * submitted requests concurrently on the GPU; a request is admitted when a
+ * slot is free and its HBM footprint fits in hbm_budget_bytes beside the
+ * running proofs.  Keys passed to submit must outlive their requests.  For
+ * serving processes (proof-server's worker pool) that batch concurrent
+ * requests of different circuit sizes. */
+RustError midnight_hip_proof_server_create(void **out, uint32_t slots, uint64_t hbm_budget_bytes);
+/* Queues one proof (the advice is copied); *job_id identifies it. */
+RustError midnight_hip_proof_server_submit(void *server, void *pk, const RawFr *advice, size_t advice_len,
+                                           uint64_t seed, uint64_t *job_id);
+/* Waits for a job: proof bytes, and its time queued and proving (ms). */
+RustError midnight_hip_proof_server_wait(void *server, uint64_t job_id, uint8_t *proof, size_t capacity,
+                                         size_t *proof_len, double *queue_ms, double *prove_ms);
+/* Finishes queued and running proofs, then stops the server. */
+void midnight_hip_proof_server_free(void *server);
+
 /* Measured throughput of the integer multiply-add the field code is built
  * on (v_mad_u64_u32), in operations per second over the whole device, and
  * the device clock/CU count used; for roofline reporting. */

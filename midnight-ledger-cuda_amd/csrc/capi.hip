@@ -47,6 +47,24 @@ hipStream_t lane_stream(uint32_t lane)
     return s;
 }
 
+void *dev_alloc(size_t bytes, hipStream_t st)
+{
+    static std::once_flag pool_once[64];
+    const int dev = device_id();
+    std::call_once(pool_once[dev], [dev]() {
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t keep = ~0ull;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+    });
+    void *p = nullptr;
+    MH_HIP(hipMallocAsync(&p, bytes, st));
+    return p;
+}
+
+void dev_free(void *p, hipStream_t st) { (void)hipFreeAsync(p, st); }
+
 // per-lane reusable staging/scratch buffers for the host-bridge entry points
 namespace {
 struct LaneScratch {

@@ -79,6 +79,53 @@ struct HostFr {
 
     HostFr sqr() const { return *this * *this; }
 
+    HostFr operator+(const HostFr &b) const
+    {
+        HostFr r;
+        unsigned __int128 c = 0;
+        for (int i = 0; i < 4; ++i) {
+            c = (unsigned __int128)l[i] + b.l[i] + (uint64_t)(c >> 64);
+            r.l[i] = (uint64_t)c;
+        }
+        if (geq_p(r.l)) sub_p(r.l);   // r < 2^255: no carry out of the top limb
+        return r;
+    }
+    HostFr operator-(const HostFr &b) const
+    {
+        HostFr r;
+        unsigned __int128 br = 0;
+        for (int i = 0; i < 4; ++i) {
+            unsigned __int128 d = (unsigned __int128)l[i] - b.l[i] - (uint64_t)br;
+            r.l[i] = (uint64_t)d;
+            br = (d >> 64) & 1;
+        }
+        if (br) {
+            unsigned __int128 c = 0;
+            for (int i = 0; i < 4; ++i) {
+                c = (unsigned __int128)r.l[i] + P[i] + (uint64_t)(c >> 64);
+                r.l[i] = (uint64_t)c;
+            }
+        }
+        return r;
+    }
+    HostFr neg() const { return zero() - *this; }
+    bool is_zero() const { return (l[0] | l[1] | l[2] | l[3]) == 0; }
+
+    // canonical integer < r  ->  Montgomery form
+    static HostFr from_canonical(const uint64_t c[4])
+    {
+        HostFr t{{c[0], c[1], c[2], c[3]}};
+        HostFr r2;
+        std::memcpy(r2.l, R2, 32);
+        return t * r2;
+    }
+    void to_canonical(uint64_t out[4]) const
+    {
+        const HostFr one_raw{{1, 0, 0, 0}};
+        const HostFr t = *this * one_raw;
+        std::memcpy(out, t.l, 32);
+    }
+
     HostFr pow(const uint64_t e[4]) const
     {
         HostFr acc = one(), base = *this;

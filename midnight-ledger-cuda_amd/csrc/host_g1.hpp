@@ -98,6 +98,24 @@ struct HFp {
     }
     HFp sqr() const { return *this * *this; }
     HFp dbl() const { return *this + *this; }
+    HFp inverse() const   // Fermat: a^(p-2); zero maps to zero
+    {
+        uint64_t e[6];
+        std::memcpy(e, P, 48);
+        e[0] -= 2;
+        HFp acc = one(), base = *this;
+        for (int i = 0; i < 384; ++i) {
+            if ((e[i / 64] >> (i % 64)) & 1) acc = acc * base;
+            base = base.sqr();
+        }
+        return acc;
+    }
+    void to_canonical(uint64_t out[6]) const
+    {
+        const HFp one_raw{{1, 0, 0, 0, 0, 0}};
+        const HFp t = *this * one_raw;
+        std::memcpy(out, t.l, 48);
+    }
 };
 
 struct HJac {

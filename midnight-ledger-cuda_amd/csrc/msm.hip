@@ -841,7 +841,8 @@ uint32_t engine_window(const RawMsmConfig &c)
 
 constexpr size_t kMaxBatchPairs = size_t(1) << 26;   // bounds entry buffers (~ 20 windows x 4 B x this)
 
-void run_batched(MsmContext &ctx, const std::vector<MsmJob> &jobs, RawG1Projective *out, hipStream_t st)
+void run_batched(MsmContext &ctx, MsmEngine &engine, const std::vector<MsmJob> &jobs, RawG1Projective *out,
+                 hipStream_t st)
 {
     size_t i = 0;
     while (i < jobs.size()) {
@@ -853,9 +854,9 @@ void run_batched(MsmContext &ctx, const std::vector<MsmJob> &jobs, RawG1Projecti
         std::vector<MsmJob> part(jobs.begin() + i, jobs.begin() + j);
         if (ctx.pc_nwin) {
             PrecompTable pc{ctx.table.get(), ctx.npoints, ctx.pc_c, ctx.pc_nwin};
-            ctx.engine.run(ctx.points.get(), ctx.npoints, part, 0, out + i, st, &pc);
+            engine.run(ctx.points.get(), ctx.npoints, part, 0, out + i, st, &pc);
         } else {
-            ctx.engine.run(ctx.points.get(), ctx.npoints, part, engine_window(ctx.config), out + i, st);
+            engine.run(ctx.points.get(), ctx.npoints, part, engine_window(ctx.config), out + i, st);
         }
         i = j;
     }
@@ -937,7 +938,8 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_batch(RawG1Project
     if (!out || (npoints && !scalars)) return make_error((int)hipErrorInvalidValue, "invalid MSM arguments");
     if (npoints > ctx->npoints) return make_error((int)hipErrorInvalidValue, "MSM length exceeds the context");
     MH_TRY
-    std::lock_guard<std::mutex> g(ctx->mu);
+    MsmContext::LaneEngine &le = ctx->lane_engine(current_lane());
+    std::lock_guard<std::mutex> g(le.mu);
     hipStream_t st = cur_stream();
     if (npoints == 0) {
         for (size_t i = 0; i < nmsms; ++i) out[i] = identity_out();
@@ -947,7 +949,7 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_batch(RawG1Project
     MH_HIP(hipMemcpyAsync(d_s.get(), scalars, npoints * nmsms * 32, hipMemcpyHostToDevice, st));
     std::vector<MsmJob> jobs(nmsms);
     for (size_t i = 0; i < nmsms; ++i) jobs[i] = MsmJob{d_s.get() + i * npoints, 0, npoints};
-    run_batched(*ctx, jobs, out, st);
+    run_batched(*ctx, le.engine, jobs, out, st);
     return ok();
     MH_CATCH
 }
@@ -968,7 +970,8 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_device_batch(RawG1
     if (npoints > ctx->npoints) return make_error((int)hipErrorInvalidValue, "MSM length exceeds the context");
     if (nmsms == 0) return ok();
     MH_TRY
-    std::lock_guard<std::mutex> g(ctx->mu);
+    MsmContext::LaneEngine &le = ctx->lane_engine(current_lane());
+    std::lock_guard<std::mutex> g(le.mu);
     hipStream_t st = cur_stream();
     if (npoints == 0) {
         for (size_t i = 0; i < nmsms; ++i) out[i] = identity_out();
@@ -976,7 +979,7 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_device_batch(RawG1
     }
     std::vector<MsmJob> jobs(nmsms);
     for (size_t i = 0; i < nmsms; ++i) jobs[i] = MsmJob{dv->values.get() + scalar_offset + i * npoints, 0, npoints};
-    run_batched(*ctx, jobs, out, st);
+    run_batched(*ctx, le.engine, jobs, out, st);
     return ok();
     MH_CATCH
 }
@@ -997,7 +1000,8 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_device_ragged(RawG
             return make_error((int)hipErrorInvalidValue, "ragged resident MSM slice exceeds its source");
     }
     MH_TRY
-    std::lock_guard<std::mutex> g(ctx->mu);
+    MsmContext::LaneEngine &le = ctx->lane_engine(current_lane());
+    std::lock_guard<std::mutex> g(le.mu);
     hipStream_t st = cur_stream();
     // zero scalars produce no digits, so the nonzero prefix needs no compaction
     std::vector<MsmJob> jobs;
@@ -1011,7 +1015,7 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_device_ragged(RawG
         where.push_back(i);
     }
     std::vector<RawG1Projective> res(jobs.size());
-    run_batched(*ctx, jobs, res.data(), st);
+    run_batched(*ctx, le.engine, jobs, res.data(), st);
     for (size_t i = 0; i < jobs.size(); ++i) out[where[i]] = res[i];
     return ok();
     MH_CATCH
@@ -1035,14 +1039,15 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_device_scrambled_b
     if (npoints > ctx->npoints) return make_error((int)hipErrorInvalidValue, "MSM length exceeds the context");
     if (count == 0) return ok();
     MH_TRY
-    std::lock_guard<std::mutex> g(ctx->mu);
+    MsmContext::LaneEngine &le = ctx->lane_engine(current_lane());
+    std::lock_guard<std::mutex> g(le.mu);
     hipStream_t st = cur_stream();
     // returns MSM(P, s + mask); the caller subtracts MSM(P, mask) (lib.rs:3349-3353)
     DevBuf<Fr> masked(count);
     fr_add_mask(masked.get(), dv->values.get() + scalar_offset, mask->values.get(), npoints, count, st);
     std::vector<MsmJob> jobs(nmsms);
     for (size_t i = 0; i < nmsms; ++i) jobs[i] = MsmJob{masked.get() + i * npoints, 0, npoints};
-    run_batched(*ctx, jobs, out, st);
+    run_batched(*ctx, le.engine, jobs, out, st);
     return ok();
     MH_CATCH
 }
@@ -1060,7 +1065,8 @@ extern "C" RustError midnight_hip_g1_msm_context_device_multi(RawG1Projective *o
             return make_error((int)hipErrorInvalidValue, "batched MSM slice exceeds its source");
     if (count == 0) return ok();
     MH_TRY
-    std::lock_guard<std::mutex> g(ctx->mu);
+    MsmContext::LaneEngine &le = ctx->lane_engine(current_lane());
+    std::lock_guard<std::mutex> g(le.mu);
     hipStream_t st = cur_stream();
     std::vector<MsmJob> jobs;
     std::vector<size_t> where;
@@ -1073,7 +1079,7 @@ extern "C" RustError midnight_hip_g1_msm_context_device_multi(RawG1Projective *o
         where.push_back(i);
     }
     std::vector<RawG1Projective> res(jobs.size());
-    run_batched(*ctx, jobs, res.data(), st);
+    run_batched(*ctx, le.engine, jobs, res.data(), st);
     for (size_t i = 0; i < jobs.size(); ++i) out[where[i]] = res[i];
     return ok();
     MH_CATCH
@@ -1101,7 +1107,8 @@ extern "C" RustError midnight_hip_g1_msm_context_last_stats(const void *context,
 {
     if (!context) return make_error((int)hipErrorInvalidValue, "null context");
     const MsmContext *ctx = static_cast<const MsmContext *>(context);
-    if (entries) *entries = ctx->engine.last_entries;
-    if (window_bits) *window_bits = ctx->engine.last_window;
+    MsmContext::LaneEngine &le = const_cast<MsmContext *>(ctx)->lane_engine(current_lane());
+    if (entries) *entries = le.engine.last_entries;
+    if (window_bits) *window_bits = le.engine.last_window;
     return ok();
 }

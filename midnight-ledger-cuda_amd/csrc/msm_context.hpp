@@ -19,8 +19,21 @@ struct MsmContext {
     // per-window copies [2^(c w)] P_i (w < pc_nwin), built once per context
     DevBuf<uint32_t> table;
     uint32_t pc_c = 0, pc_nwin = 0;
-    std::mutex mu;
-    MsmEngine engine;
+    // One engine (scratch buffers, statistics) per lane, so that MSMs of
+    // different lanes over the same points run concurrently; calls of one
+    // lane take turns on its engine.
+    struct LaneEngine {
+        std::mutex mu;
+        MsmEngine engine;
+    };
+    std::mutex mu;   // guards creation of lane engines
+    std::unique_ptr<LaneEngine> engines[kMaxLanes + 1];
+    LaneEngine &lane_engine(uint32_t lane)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        if (!engines[lane]) engines[lane] = std::make_unique<LaneEngine>();
+        return *engines[lane];
+    }
 };
 
 RawMsmConfig default_msm_config(size_t npoints);

@@ -1,0 +1,744 @@
+// Proof pipeline: key generation and create_proof for PLONK circuits made of
+// custom gates and a copy permutation (the benchmark circuit family: 5
+// advice and 3 fixed columns, gate q(ab + c - d), 8 permuted columns),
+// device-resident from the advice upload to the last opening commitment.
+//
+// Stage order, transcript and blinding draws follow the reference prover
+// (vendor/midnight-proofs-0.8.1/src/plonk/prover.rs:68-575 compute_trace,
+// :769-1010 finalise_proof, :1931-2080 evaluations and queries;
+// permutation/prover.rs:327-530, :545-712; vanishing/prover.rs:41-406;
+// poly/kzg/mod.rs:1978-2570 multi_open; transcript/{mod.rs,
+// implementors.rs}):
+//   vk repr -> advice (blinded rows u..n from the RNG) -> commit ->
+//   theta, beta, gamma -> permutation products (blinds from the RNG) ->
+//   commit -> trash challenge -> random polynomial (ChaCha20 seeded from
+//   the RNG) -> commit -> y -> numerator on the extended coset -> h ->
+//   pieces (telescoping blinds) -> commit -> x -> evaluations -> multi-open
+//   (x1, x2, f, x3, q evaluations, x4, pi).
+// Every polynomial stays on the device; the host holds the transcript, the
+// RNG and the few field elements the protocol reads back (evaluations).
+//
+// Deliberate differences from the reference, which the reference cannot be
+// built here to compare against: the verifying-key representative hashes
+// a description string of our own instead of the Rust Debug output of the
+// pinned constraint system; the vanishing argument's random polynomial is
+// drawn as with a one-thread pool (its seeding depends on the thread count
+// in the reference, vanishing/prover.rs:41-75); commitments are computed
+// from coefficients with the monomial SRS (commit_lagrange with the Lagrange
+// SRS gives the same points).
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <set>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "devvec.hpp"
+#include "fields.hpp"
+#include "frops.hpp"
+#include "host_fr.hpp"
+#include "host_g1.hpp"
+#include "hostcrypto.hpp"
+#include "ntt.hpp"
+#include "polyops.hpp"
+#include "runtime.hpp"
+
+namespace mh {
+
+void g1_compress(const RawG1Projective &p, uint8_t out[48])
+{
+    HFp X, Y, Z;
+    std::memcpy(X.l, p.x, 48);
+    std::memcpy(Y.l, p.y, 48);
+    std::memcpy(Z.l, p.z, 48);
+    std::memset(out, 0, 48);
+    if (Z.is_zero()) {
+        out[0] = 0xc0;
+        return;
+    }
+    const HFp zi = Z.inverse();
+    const HFp zi2 = zi.sqr();
+    const HFp x = X * zi2, y = Y * zi2 * zi;
+    uint64_t xc[6], yc[6], nc[6];
+    x.to_canonical(xc);
+    y.to_canonical(yc);
+    (HFp::zero() - y).to_canonical(nc);
+    bool larger = false;   // y > -y as integers
+    for (int i = 5; i >= 0; --i) {
+        if (yc[i] != nc[i]) {
+            larger = yc[i] > nc[i];
+            break;
+        }
+    }
+    for (int i = 0; i < 48; ++i) out[i] = (uint8_t)(xc[5 - i / 8] >> (8 * (7 - i % 8)));
+    out[0] |= 0x80 | (larger ? 0x20 : 0);
+}
+
+namespace {
+
+const uint8_t kTranscriptKey[] = "Domain separator for transcript";
+
+// Fiat-Shamir transcript over Blake2b-512 (transcript/implementors.rs:13-33):
+// absorb = 0x01 || bytes, squeeze = 0x00 then the digest of the running
+// state (which continues), written values are appended to the proof.
+class Transcript {
+public:
+    Transcript() : h_(64, kTranscriptKey, sizeof(kTranscriptKey) - 1, nullptr) {}
+    void absorb(const uint8_t *b, size_t n)
+    {
+        const uint8_t p = 1;
+        h_.update(&p, 1);
+        h_.update(b, n);
+    }
+    void common_scalar(const HostFr &f)
+    {
+        uint8_t b[32];
+        fr_to_bytes(f, b);
+        absorb(b, 32);
+    }
+    void write_scalar(const HostFr &f)
+    {
+        uint8_t b[32];
+        fr_to_bytes(f, b);
+        absorb(b, 32);
+        proof.insert(proof.end(), b, b + 32);
+    }
+    void write_point(const RawG1Projective &p)
+    {
+        uint8_t b[48];
+        g1_compress(p, b);
+        absorb(b, 48);
+        proof.insert(proof.end(), b, b + 48);
+    }
+    HostFr squeeze()
+    {
+        const uint8_t p = 0;
+        h_.update(&p, 1);
+        const std::vector<uint8_t> d = h_.digest();
+        return fr_from_uniform_bytes(d.data());
+    }
+    std::vector<uint8_t> proof;
+
+private:
+    Blake2b h_;
+};
+
+void check(const RustError &e)
+{
+    if (e.code != 0) {
+        std::string m = e.message ? e.message : "backend error";
+        free(e.message);
+        throw hip_error(e.code, m);
+    }
+}
+
+RawFr raw(const HostFr &h)
+{
+    RawFr r;
+    std::memcpy(r.limbs, h.l, 32);
+    return r;
+}
+
+HostFr hpow(const HostFr &a, uint64_t e)
+{
+    const uint64_t ee[4] = {e, 0, 0, 0};
+    return a.pow(ee);
+}
+
+// ZETA (cube root of unity) and DELTA of the scalar field, Montgomery form
+// (vendor/midnight-curves-0.3.1/src/bls12_381/fq.rs:492-500, :693-702)
+const HostFr kZeta{{0x92d9090b093011d2ULL, 0xfc9cbd719d6aa073ULL, 0xc1f14ef0cd65a1a6ULL, 0x017f6d35e72fcdebULL}};
+const HostFr kDelta{{0x70e310d3d146f96aULL, 0x4b64c08919e299e6ULL, 0x51e114186a8b970dULL, 0x6185d06627c067cbULL}};
+
+std::unique_ptr<DeviceVec> new_vec(size_t len)
+{
+    auto v = std::make_unique<DeviceVec>();
+    v->lane = current_lane();
+    v->len = len;
+    v->values.alloc(len);
+    return v;
+}
+
+void ntt_batch(Fr *data, uint32_t log_n, size_t count, bool inverse, DevBuf<Fr> &scratch, hipStream_t st)
+{
+    scratch.ensure(ntt_scratch_elems(log_n, count));
+    NttJob j;
+    j.in = data;
+    j.out = data;
+    j.scratch = scratch.get();
+    j.in_pstride = j.out_pstride = size_t(1) << log_n;
+    j.log_n = log_n;
+    j.count = (uint32_t)count;
+    j.inverse = inverse;
+    ntt_run(j, st);
+}
+
+// coefficients (len n each, count polys) -> values on the extended coset
+void to_extended(Fr *dst, const Fr *src, size_t n, size_t count, uint32_t ext_log_n, DevBuf<Fr> &scratch,
+                 hipStream_t st)
+{
+    scratch.ensure(ntt_scratch_elems(ext_log_n, count));
+    NttJob j;
+    j.in = src;
+    j.out = dst;
+    j.scratch = scratch.get();
+    j.in_pstride = n;
+    j.out_pstride = size_t(1) << ext_log_n;
+    j.log_n = ext_log_n;
+    j.count = (uint32_t)count;
+    j.pre = 1;
+    j.in_len = n;
+    j.first = kZeta;
+    j.second = kZeta.sqr();
+    ntt_run(j, st);
+}
+
+}  // namespace
+
+struct PlonkKey {
+    uint32_t lane = 0;
+    uint32_t k = 0, ext_k = 0, degree = 0;
+    size_t n = 0, ext_n = 0, bf = 0, u = 0;
+    size_t fixed_cols = 0, advice_cols = 0, perm_cols = 0, chunk = 0, sets = 0, pieces = 0;
+    void *srs = nullptr;                                   // MSM context (not owned), >= n points
+    std::vector<std::pair<uint32_t, int32_t>> advice_queries, fixed_queries;
+    std::vector<RawColumnRef> perm_columns;
+    std::unique_ptr<DeviceVec> fixed_coeff, sigma_coeff;  // coefficient form
+    std::vector<RawG1Projective> fixed_com, sigma_com;
+    HostFr omega, omega_inv, ext_omega, repr;
+    std::vector<HostFr> t_inv, delta_powers;
+    void *numctx = nullptr;
+    DevBuf<Fr> scratch_by_lane[kMaxLanes + 1];   // NTT scratch of proofs running on each lane
+    DevBuf<Fr> &scratch() { return scratch_by_lane[current_lane()]; }
+    ~PlonkKey()
+    {
+        if (numctx) midnight_sppark_bls12_381_numerator_context_free(numctx);
+    }
+};
+
+namespace {
+
+void commit(std::vector<RawG1Projective> &out, PlonkKey &pk, const DeviceVec &v, size_t offset_polys, size_t count)
+{
+    out.resize(count);
+    check(midnight_sppark_bls12_381_g1_msm_context_device_batch(out.data(), pk.srs, &v, offset_polys * pk.n, pk.n,
+                                                                count));
+}
+
+HostFr rotate(const PlonkKey &pk, const HostFr &x, int32_t rot)
+{
+    HostFr r = x;
+    const HostFr w = rot >= 0 ? pk.omega : pk.omega_inv;
+    for (int32_t i = 0; i < (rot >= 0 ? rot : -rot); ++i) r = r * w;
+    return r;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------- keygen
+
+std::unique_ptr<PlonkKey> plonk_keygen(uint32_t k, void *srs, size_t srs_points, uint32_t degree, const RawFr *fixed,
+                                       size_t fixed_cols, size_t advice_cols,
+                                       const std::vector<std::pair<uint32_t, int32_t>> &advice_queries,
+                                       const std::vector<std::pair<uint32_t, int32_t>> &fixed_queries,
+                                       const RawColumnRef *perm_columns, size_t perm_cols, const uint32_t *mapping,
+                                       const RawGraph &gate, const std::string &description)
+{
+    hipStream_t st = cur_stream();
+    auto pk = std::make_unique<PlonkKey>();
+    pk->lane = current_lane();
+    pk->k = k;
+    pk->n = size_t(1) << k;
+    pk->degree = degree;
+    const size_t n = pk->n;
+    MH_CHECK(srs_points >= n, (int)hipErrorInvalidValue, "SRS is smaller than the circuit");
+    MH_CHECK(degree >= 3 && perm_cols > 0, (int)hipErrorInvalidValue, "circuit degree below 3 or no permutation");
+    pk->srs = srs;
+    // EvaluationDomain::new(degree, k) (poly/domain.rs:54-159)
+    uint32_t ek = k;
+    while ((size_t(1) << ek) < n * (degree - 1)) ++ek;
+    pk->ext_k = ek;
+    pk->ext_n = size_t(1) << ek;
+    pk->pieces = degree - 1;
+    pk->omega = HostFr::omega(k);
+    pk->omega_inv = pk->omega.inverse();
+    pk->ext_omega = HostFr::omega(ek);
+    {
+        const HostFr orig = hpow(kZeta, n), step = hpow(pk->ext_omega, n);
+        HostFr cur = orig;
+        for (size_t i = 0; i < (size_t(1) << (ek - k)); ++i) {
+            pk->t_inv.push_back((cur - HostFr::one()).inverse());
+            cur = cur * step;
+        }
+    }
+    // blinding factors (plonk/circuit.rs:2502-2530): max(3, queries per advice column) + 1 + 1
+    size_t maxq = 1;
+    for (size_t c = 0; c < advice_cols; ++c) {
+        size_t q = 0;
+        for (auto &aq : advice_queries) q += aq.first == c;
+        maxq = std::max(maxq, q);
+    }
+    pk->bf = std::max<size_t>(3, maxq) + 2;
+    MH_CHECK(n > pk->bf + 1, (int)hipErrorInvalidValue, "circuit too small for its blinding rows");
+    pk->u = n - pk->bf - 1;
+    pk->fixed_cols = fixed_cols;
+    pk->advice_cols = advice_cols;
+    pk->perm_cols = perm_cols;
+    pk->chunk = degree - 2;
+    pk->sets = (perm_cols + pk->chunk - 1) / pk->chunk;
+    pk->advice_queries = advice_queries;
+    pk->fixed_queries = fixed_queries;
+    pk->perm_columns.assign(perm_columns, perm_columns + perm_cols);
+    pk->delta_powers.resize(perm_cols);
+    HostFr d = HostFr::one();
+    for (size_t j = 0; j < perm_cols; ++j, d = d * kDelta) pk->delta_powers[j] = d;
+
+    // sigma Lagrange values delta^j' omega^i' (permutation/keygen.rs)
+    std::vector<HostFr> wpow(n);
+    wpow[0] = HostFr::one();
+    for (size_t i = 1; i < n; ++i) wpow[i] = wpow[i - 1] * pk->omega;
+    std::vector<HostFr> sigma(perm_cols * n);
+    for (size_t j = 0; j < perm_cols; ++j)
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t cj = mapping[2 * (j * n + i)], ri = mapping[2 * (j * n + i) + 1];
+            MH_CHECK(cj < perm_cols && ri < n, (int)hipErrorInvalidValue, "permutation mapping out of range");
+            sigma[j * n + i] = pk->delta_powers[cj] * wpow[ri];
+        }
+    // l0, l_last, l_active on the base domain
+    std::vector<HostFr> lag(3 * n, HostFr::zero());
+    lag[0] = HostFr::one();
+    lag[n + pk->u] = HostFr::one();
+    for (size_t i = 0; i < pk->u; ++i) lag[2 * n + i] = HostFr::one();
+
+    // Lagrange -> coefficients -> commitments; extended cosets
+    DeviceVec fixed_lag, sigma_lag;
+    fixed_lag.len = fixed_cols * n;
+    fixed_lag.values.alloc(std::max<size_t>(fixed_lag.len, 1));
+    sigma_lag.len = perm_cols * n;
+    sigma_lag.values.alloc(sigma_lag.len);
+    if (fixed_cols) MH_HIP(hipMemcpyAsync(fixed_lag.values.get(), fixed, fixed_cols * n * 32, hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(sigma_lag.values.get(), sigma.data(), perm_cols * n * 32, hipMemcpyHostToDevice, st));
+    pk->fixed_coeff = new_vec(std::max<size_t>(fixed_cols, 1) * n);
+    pk->sigma_coeff = new_vec(perm_cols * n);
+    pk->fixed_coeff->len = fixed_cols * n;
+    if (fixed_cols)
+        MH_HIP(hipMemcpyAsync(pk->fixed_coeff->values.get(), fixed_lag.values.get(), fixed_cols * n * 32,
+                              hipMemcpyDeviceToDevice, st));
+    MH_HIP(hipMemcpyAsync(pk->sigma_coeff->values.get(), sigma_lag.values.get(), perm_cols * n * 32,
+                          hipMemcpyDeviceToDevice, st));
+    if (fixed_cols) ntt_batch(pk->fixed_coeff->values.get(), k, fixed_cols, true, pk->scratch(), st);
+    ntt_batch(pk->sigma_coeff->values.get(), k, perm_cols, true, pk->scratch(), st);
+    DevBuf<Fr> lag_d(3 * n);
+    MH_HIP(hipMemcpyAsync(lag_d.get(), lag.data(), 3 * n * 32, hipMemcpyHostToDevice, st));
+    ntt_batch(lag_d.get(), k, 3, true, pk->scratch(), st);
+    MH_HIP(hipStreamSynchronize(st));
+    if (fixed_cols) commit(pk->fixed_com, *pk, *pk->fixed_coeff, 0, fixed_cols);
+    commit(pk->sigma_com, *pk, *pk->sigma_coeff, 0, perm_cols);
+
+    const size_t N = pk->ext_n;
+    DevBuf<Fr> fixed_ext(std::max<size_t>(fixed_cols, 1) * N), sigma_ext(perm_cols * N), lag_ext(3 * N);
+    if (fixed_cols) to_extended(fixed_ext.get(), pk->fixed_coeff->values.get(), n, fixed_cols, ek, pk->scratch(), st);
+    to_extended(sigma_ext.get(), pk->sigma_coeff->values.get(), n, perm_cols, ek, pk->scratch(), st);
+    to_extended(lag_ext.get(), lag_d.get(), n, 3, ek, pk->scratch(), st);
+    MH_HIP(hipStreamSynchronize(st));
+    // the numerator context keeps its own copies; host staging of the cosets
+    std::vector<RawFr> h_fixed_ext(fixed_cols * N), h_sigma_ext(perm_cols * N), h_lag_ext(3 * N),
+        h_fixed_lag(fixed_cols * n);
+    if (fixed_cols) {
+        MH_HIP(hipMemcpyAsync(h_fixed_ext.data(), fixed_ext.get(), fixed_cols * N * 32, hipMemcpyDeviceToHost, st));
+        std::memcpy(h_fixed_lag.data(), fixed, fixed_cols * n * 32);
+    }
+    MH_HIP(hipMemcpyAsync(h_sigma_ext.data(), sigma_ext.get(), perm_cols * N * 32, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipMemcpyAsync(h_lag_ext.data(), lag_ext.get(), 3 * N * 32, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    check(midnight_sppark_bls12_381_numerator_context_create(
+        &pk->numctx, N, n, fixed_cols ? h_fixed_ext.data() : nullptr, fixed_cols ? h_fixed_lag.data() : nullptr,
+        fixed_cols, h_lag_ext.data(), h_lag_ext.data() + N, h_lag_ext.data() + 2 * N, perm_columns, perm_cols,
+        h_sigma_ext.data(), reinterpret_cast<const RawFr *>(sigma.data()), perm_cols, nullptr, 0, gate, nullptr,
+        nullptr, nullptr, 0, nullptr, raw(pk->ext_omega), raw(pk->omega), raw(kZeta), raw(kDelta),
+        (int32_t)(1u << (ek - k)), -(int32_t)(pk->bf + 1), (uint32_t)pk->chunk));
+
+    // verifying-key representative (plonk/mod.rs:371-405): Blake2b-512
+    // personalised "Halo2-Verify-Key" over version, k, the commitments
+    // (96-byte affine Montgomery limbs) and a description of the circuit
+    const uint8_t personal[16] = {'H', 'a', 'l', 'o', '2', '-', 'V', 'e', 'r', 'i', 'f', 'y', '-', 'K', 'e', 'y'};
+    Blake2b hv(64, nullptr, 0, personal);
+    const uint8_t head[2] = {0x03, (uint8_t)k};
+    hv.update(head, 2);
+    auto put_points = [&](const std::vector<RawG1Projective> &pts) {
+        const uint32_t cnt = (uint32_t)pts.size();
+        uint8_t c[4] = {(uint8_t)cnt, (uint8_t)(cnt >> 8), (uint8_t)(cnt >> 16), (uint8_t)(cnt >> 24)};
+        hv.update(c, 4);
+        for (const auto &p : pts) {
+            HFp X, Y, Z;
+            std::memcpy(X.l, p.x, 48);
+            std::memcpy(Y.l, p.y, 48);
+            std::memcpy(Z.l, p.z, 48);
+            uint8_t b[96] = {0};
+            if (!Z.is_zero()) {
+                const HFp zi = Z.inverse(), zi2 = zi.sqr();
+                const HFp x = X * zi2, y = Y * zi2 * zi;
+                std::memcpy(b, x.l, 48);
+                std::memcpy(b + 48, y.l, 48);
+            }
+            hv.update(b, 96);
+        }
+    };
+    put_points(pk->fixed_com);
+    put_points(pk->sigma_com);
+    hv.update(reinterpret_cast<const uint8_t *>(description.data()), description.size());
+    pk->repr = fr_from_uniform_bytes(hv.digest().data());
+    return pk;
+}
+
+// HBM a proof of this key holds beyond the key itself: the extended
+// advice and product cosets with the numerator and quotient, the
+// coefficient-form polynomials and opening temporaries, and MSM scratch.
+size_t plonk_footprint(const PlonkKey &pk)
+{
+    const size_t polys = pk.advice_cols + pk.sets + 1 + pk.pieces;
+    return 32 * (pk.ext_n * (pk.advice_cols + pk.sets + 2) + pk.n * (2 * polys + 12)) + 256 * pk.n;
+}
+
+size_t plonk_advice_len(const PlonkKey &pk) { return pk.advice_cols * pk.n; }
+
+// ---------------------------------------------------------------------- prove
+
+namespace {
+
+struct Query {
+    const Fr *poly;     // device coefficients, pk.n long
+    HostFr point;
+};
+
+}  // namespace
+
+std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t seed)
+{
+    hipStream_t st = cur_stream();
+    const size_t n = pk.n, N = pk.ext_n, A = pk.advice_cols, S = pk.sets;
+    Transcript T;
+
+    // every draw of the prover's RNG, in the reference's order
+    ChaChaRng rng = ChaChaRng::seed_from_u64(seed);
+    std::vector<HostFr> adv(A * n);
+    std::memcpy(adv.data(), advice, A * n * 32);
+    for (size_t c = 0; c < A; ++c)
+        for (size_t r = pk.u; r < n; ++r) adv[c * n + r] = rng.random_fr();   // unusable rows
+    std::vector<HostFr> perm_blinds(S * pk.bf);
+    for (auto &b : perm_blinds) b = rng.random_fr();
+    uint8_t vseed[32];
+    rng.fill_bytes(vseed, 32);
+    std::vector<HostFr> limb_blinds(pk.pieces - 1);
+    for (auto &b : limb_blinds) b = rng.random_fr();
+    // vanishing random polynomial: one ChaCha20 stream (a one-thread pool)
+    std::vector<HostFr> rand_poly(n);
+    std::thread rand_thread([&]() {
+        ChaChaRng r2 = ChaChaRng::from_seed(vseed);
+        for (auto &v : rand_poly) v = r2.random_fr();
+    });
+
+    T.common_scalar(pk.repr);
+
+    // advice: Lagrange values (kept for the permutation), coefficients, commitments
+    DeviceVec adv_lag;
+    adv_lag.lane = current_lane();
+    adv_lag.len = A * n;
+    adv_lag.values.alloc(A * n);
+    MH_HIP(hipMemcpyAsync(adv_lag.values.get(), adv.data(), A * n * 32, hipMemcpyHostToDevice, st));
+    // polys: advice | z sets | random | h pieces (coefficient form, n each)
+    const size_t P_ADV = 0, P_Z = A, P_RAND = A + S, P_H = A + S + 1, NPOLY = A + S + 1 + pk.pieces;
+    auto polys = new_vec(NPOLY * n);
+    Fr *pd = polys->values.get();
+    MH_HIP(hipMemcpyAsync(pd, adv_lag.values.get(), A * n * 32, hipMemcpyDeviceToDevice, st));
+    ntt_batch(pd + P_ADV * n, pk.k, A, true, pk.scratch(), st);
+    std::vector<RawG1Projective> com;
+    commit(com, pk, *polys, P_ADV, A);
+    for (auto &c : com) T.write_point(c);
+
+    const HostFr theta = T.squeeze();
+    const HostFr beta = T.squeeze();
+    const HostFr gamma = T.squeeze();
+
+    // permutation products from the resident Lagrange advice
+    void *zv = nullptr;
+    check(midnight_sppark_bls12_381_trace_context_permutation(
+        &zv, pk.numctx, &adv_lag, A, nullptr, 0, raw(beta), raw(gamma),
+        reinterpret_cast<const RawFr *>(pk.delta_powers.data()), pk.perm_cols,
+        reinterpret_cast<const RawFr *>(perm_blinds.data()), perm_blinds.size(), pk.bf));
+    std::unique_ptr<DeviceVec> z(reinterpret_cast<DeviceVec *>(zv));
+    MH_HIP(hipMemcpyAsync(pd + P_Z * n, z->values.get(), S * n * 32, hipMemcpyDeviceToDevice, st));
+    z.reset();
+    ntt_batch(pd + P_Z * n, pk.k, S, true, pk.scratch(), st);
+    commit(com, pk, *polys, P_Z, S);
+    for (auto &c : com) T.write_point(c);
+
+    const HostFr trash = T.squeeze();
+
+    rand_thread.join();
+    MH_HIP(hipMemcpyAsync(pd + P_RAND * n, rand_poly.data(), n * 32, hipMemcpyHostToDevice, st));
+    commit(com, pk, *polys, P_RAND, 1);
+    T.write_point(com[0]);
+
+    const HostFr y = T.squeeze();
+
+    // numerator on the extended coset of advice | products
+    auto cols = new_vec((A + S) * N);
+    to_extended(cols->values.get(), pd + P_ADV * n, n, A, pk.ext_k, pk.scratch(), st);
+    to_extended(cols->values.get() + A * N, pd + P_Z * n, n, S, pk.ext_k, pk.scratch(), st);
+    MH_HIP(hipStreamSynchronize(st));
+    void *hv = nullptr;
+    check(midnight_sppark_bls12_381_numerator_context_eval(&hv, pk.numctx, cols.get(), A + S, 0, A, 0, 0, nullptr,
+                                                           0, raw(beta), raw(gamma), raw(theta), raw(trash), raw(y),
+                                                           A, S, 0, 0, 0, 0, 0, 0));
+    cols.reset();
+    std::unique_ptr<DeviceVec> h(reinterpret_cast<DeviceVec *>(hv));
+    // h = numerator / (X^n - 1) on the coset, then coefficients (extended_to_coeff)
+    check(midnight_sppark_bls12_381_fr_device_vec_divide_ntt_zeta(
+        h.get(), pk.ext_k, reinterpret_cast<const RawFr *>(pk.t_inv.data()), pk.t_inv.size(), raw(HostFr::one()),
+        raw(kZeta.sqr()), raw(kZeta)));
+    DevBuf<Fr> lb(limb_blinds.size());
+    MH_HIP(hipMemcpyAsync(lb.get(), limb_blinds.data(), limb_blinds.size() * 32, hipMemcpyHostToDevice, st));
+    fr_quotient_pieces(pd + P_H * n, h->values.get(), n, pk.pieces, lb.get(), st);
+    MH_HIP(hipStreamSynchronize(st));
+    h.reset();
+    commit(com, pk, *polys, P_H, pk.pieces);
+    for (auto &c : com) T.write_point(c);
+
+    const HostFr x = T.squeeze();
+
+    // evaluations (write_evals_to_transcript, vanishing / permutation evaluate)
+    auto eval1 = [&](const Fr *poly, const HostFr &pt) {
+        HostFr v;
+        poly_eval_many(&v, poly, n, 1, &pt, 1, st);
+        return v;
+    };
+    for (auto &q : pk.advice_queries) T.write_scalar(eval1(pd + (P_ADV + q.first) * n, rotate(pk, x, q.second)));
+    const Fr *fixed = pk.fixed_coeff->values.get();
+    for (auto &q : pk.fixed_queries) T.write_scalar(eval1(fixed + (size_t)q.first * n, rotate(pk, x, q.second)));
+    T.write_scalar(eval1(pd + P_RAND * n, x));
+    {
+        std::vector<HostFr> sv(pk.perm_cols);
+        poly_eval_many(sv.data(), pk.sigma_coeff->values.get(), n, pk.perm_cols, &x, 1, st);
+        for (auto &v : sv) T.write_scalar(v);
+    }
+    const HostFr x_next = rotate(pk, x, 1), x_last = rotate(pk, x, -(int32_t)(pk.bf + 1));
+    for (size_t s = 0; s < S; ++s) {
+        const HostFr pts[3] = {x, x_next, x_last};
+        HostFr ev[3];
+        poly_eval_many(ev, pd + (P_Z + s) * n, n, 1, pts, s + 1 < S ? 3 : 2, st);
+        T.write_scalar(ev[0]);
+        T.write_scalar(ev[1]);
+        if (s + 1 < S) T.write_scalar(ev[2]);
+    }
+
+    // h(X) for the opening: sum_i x^((n-1) i) h_i(X) (vanishing/prover.rs:321-369)
+    auto hpoly = new_vec(n);
+    {
+        const HostFr sf = hpow(x, n - 1);
+        std::vector<const Fr *> ps;
+        std::vector<HostFr> sc;
+        HostFr p = HostFr::one();
+        for (size_t i = 0; i < pk.pieces; ++i, p = p * sf) {
+            ps.push_back(pd + (P_H + i) * n);
+            sc.push_back(p);
+        }
+        poly_lincomb(hpoly->values.get(), ps, sc, n, HostFr::zero(), st);
+    }
+
+    // queries (compute_queries, prover.rs:1993-2080)
+    std::vector<Query> queries;
+    for (auto &q : pk.advice_queries) queries.push_back({pd + (P_ADV + q.first) * n, rotate(pk, x, q.second)});
+    for (size_t s = 0; s < S; ++s) {
+        queries.push_back({pd + (P_Z + s) * n, x});
+        queries.push_back({pd + (P_Z + s) * n, x_next});
+    }
+    for (size_t s = S - 1; s-- > 0;) queries.push_back({pd + (P_Z + s) * n, x_last});
+    for (auto &q : pk.fixed_queries) queries.push_back({fixed + (size_t)q.first * n, rotate(pk, x, q.second)});
+    for (size_t j = 0; j < pk.perm_cols; ++j) queries.push_back({pk.sigma_coeff->values.get() + j * n, x});
+    queries.push_back({hpoly->values.get(), x});
+    queries.push_back({pd + P_RAND * n, x});
+
+    // multi-open (poly/kzg/mod.rs:1978-2570; utils.rs construct_intermediate_sets)
+    const HostFr x1 = T.squeeze();
+    const HostFr x2 = T.squeeze();
+    std::vector<HostFr> points;
+    std::vector<const Fr *> cpoly;                 // commitment order
+    std::vector<std::vector<size_t>> cpoints;      // point indices per commitment
+    for (auto &q : queries) {
+        size_t pi = 0;
+        while (pi < points.size() && !(points[pi] == q.point)) ++pi;
+        if (pi == points.size()) points.push_back(q.point);
+        size_t ci = 0;
+        while (ci < cpoly.size() && cpoly[ci] != q.poly) ++ci;
+        if (ci == cpoly.size()) {
+            cpoly.push_back(q.poly);
+            cpoints.emplace_back();
+        }
+        MH_CHECK(std::find(cpoints[ci].begin(), cpoints[ci].end(), pi) == cpoints[ci].end(),
+                 (int)hipErrorInvalidValue, "duplicated query");
+        cpoints[ci].push_back(pi);
+    }
+    std::map<std::set<size_t>, size_t> set_index;
+    std::vector<size_t> cset(cpoly.size());
+    for (size_t c = 0; c < cpoly.size(); ++c) {
+        std::set<size_t> ps(cpoints[c].begin(), cpoints[c].end());
+        auto it = set_index.find(ps);
+        if (it == set_index.end()) it = set_index.emplace(ps, set_index.size()).first;
+        cset[c] = it->second;
+    }
+    const size_t nsets = set_index.size();
+    std::vector<std::vector<size_t>> set_points(nsets);
+    for (auto &kv : set_index) set_points[kv.second].assign(kv.first.begin(), kv.first.end());
+    // q_i = sum_j x1^j p_j over the set's commitments, in commitment order
+    auto qv = new_vec(nsets * n);
+    for (size_t s = 0; s < nsets; ++s) {
+        std::vector<const Fr *> ps;
+        std::vector<HostFr> sc;
+        HostFr p = HostFr::one();
+        for (size_t c = 0; c < cpoly.size(); ++c)
+            if (cset[c] == s) {
+                ps.push_back(cpoly[c]);
+                sc.push_back(p);
+                p = p * x1;
+            }
+        poly_lincomb(qv->values.get() + s * n, ps, sc, n, HostFr::zero(), st);
+    }
+    // sets in ascending (size, index) order
+    std::vector<size_t> order(nsets);
+    for (size_t i = 0; i < nsets; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return set_points[a].size() < set_points[b].size(); });
+    // f = sum_i x2^i (q_i - r_i) / prod (X - p), i in sorted order
+    auto fv = new_vec(2 * n);   // [f | work]
+    Fr *f = fv->values.get(), *work = f + n;
+    {
+        std::vector<const Fr *> ps;
+        std::vector<HostFr> sc;
+        HostFr p = HostFr::one();
+        DevBuf<Fr> divided(nsets * n);
+        for (size_t oi = 0; oi < nsets; ++oi) {
+            const size_t s = order[oi];
+            Fr *dst = divided.get() + oi * n;
+            MH_HIP(hipMemcpyAsync(dst, qv->values.get() + s * n, n * 32, hipMemcpyDeviceToDevice, st));
+            for (size_t pi : set_points[s]) poly_kate_divide(dst, dst, n, points[pi], st);
+            ps.push_back(dst);
+            sc.push_back(p);
+            p = p * x2;
+        }
+        poly_lincomb(f, ps, sc, n, HostFr::zero(), st);
+    }
+    commit(com, pk, *fv, 0, 1);
+    T.write_point(com[0]);
+    const HostFr x3 = T.squeeze();
+    std::vector<HostFr> qe(nsets);
+    for (size_t oi = 0; oi < nsets; ++oi) {
+        qe[oi] = eval1(qv->values.get() + order[oi] * n, x3);
+        T.write_scalar(qe[oi]);
+    }
+    const HostFr x4 = T.squeeze();
+    {
+        std::vector<const Fr *> ps;
+        std::vector<HostFr> sc;
+        HostFr p = HostFr::one();
+        for (size_t oi = 0; oi < nsets; ++oi, p = p * x4) {
+            ps.push_back(qv->values.get() + order[oi] * n);
+            sc.push_back(p);
+        }
+        ps.push_back(f);
+        sc.push_back(p);
+        poly_lincomb(work, ps, sc, n, HostFr::zero(), st);
+    }
+    const HostFr v = eval1(work, x3);
+    // pi = (final - v) / (X - x3)
+    {
+        std::vector<const Fr *> ps{work};
+        std::vector<HostFr> sc{HostFr::one()};
+        poly_lincomb(work, ps, sc, n, v.neg(), st);
+    }
+    poly_kate_divide(work, work, n, x3, st);
+    MH_HIP(hipMemcpyAsync(f, work, n * 32, hipMemcpyDeviceToDevice, st));
+    commit(com, pk, *fv, 0, 1);
+    T.write_point(com[0]);
+    MH_HIP(hipStreamSynchronize(st));
+    return T.proof;
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+#define MH_TRY try {
+#define MH_CATCH                                                     \
+    }                                                                \
+    catch (const hip_error &e) { return make_error(e.code, e.what()); } \
+    catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
+
+extern "C" RustError midnight_hip_plonk_keygen(void **out, uint32_t k, void *srs_context, size_t srs_points,
+                                               uint32_t degree, const RawFr *fixed, size_t fixed_cols,
+                                               size_t advice_cols, const uint32_t *advice_queries,
+                                               size_t advice_query_count, const uint32_t *fixed_queries,
+                                               size_t fixed_query_count, const RawColumnRef *perm_columns,
+                                               size_t perm_cols, const uint32_t *perm_mapping, RawGraph gate,
+                                               const char *description)
+{
+    if (!out || !srs_context || k < 3 || k > 26 || (fixed_cols && !fixed) || !perm_columns || !perm_mapping ||
+        (advice_query_count && !advice_queries) || (fixed_query_count && !fixed_queries))
+        return make_error((int)hipErrorInvalidValue, "invalid keygen arguments");
+    *out = nullptr;
+    MH_TRY
+    std::vector<std::pair<uint32_t, int32_t>> aq, fq;
+    for (size_t i = 0; i < advice_query_count; ++i) {
+        MH_CHECK(advice_queries[2 * i] < advice_cols, (int)hipErrorInvalidValue, "advice query column out of range");
+        aq.emplace_back(advice_queries[2 * i], (int32_t)advice_queries[2 * i + 1]);
+    }
+    for (size_t i = 0; i < fixed_query_count; ++i) {
+        MH_CHECK(fixed_queries[2 * i] < fixed_cols, (int)hipErrorInvalidValue, "fixed query column out of range");
+        fq.emplace_back(fixed_queries[2 * i], (int32_t)fixed_queries[2 * i + 1]);
+    }
+    auto pk = plonk_keygen(k, srs_context, srs_points, degree, fixed, fixed_cols, advice_cols, aq, fq, perm_columns,
+                           perm_cols, perm_mapping, gate, description ? description : "");
+    *out = pk.release();
+    return ok();
+    MH_CATCH
+}
+
+extern "C" void midnight_hip_plonk_key_free(void *pk)
+{
+    auto *p = reinterpret_cast<PlonkKey *>(pk);
+    if (!p) return;
+    try {
+        (void)hipStreamSynchronize(lane_stream(p->lane));
+    } catch (...) {
+    }
+    delete p;
+}
+
+extern "C" RustError midnight_hip_plonk_vk(const void *pk, RawG1Projective *fixed_commitments,
+                                           RawG1Projective *permutation_commitments, RawFr *transcript_repr)
+{
+    if (!pk) return make_error((int)hipErrorInvalidValue, "null key");
+    const auto *p = reinterpret_cast<const PlonkKey *>(pk);
+    if (fixed_commitments) std::copy(p->fixed_com.begin(), p->fixed_com.end(), fixed_commitments);
+    if (permutation_commitments) std::copy(p->sigma_com.begin(), p->sigma_com.end(), permutation_commitments);
+    if (transcript_repr) *transcript_repr = raw(p->repr);
+    return ok();
+}
+
+extern "C" RustError midnight_hip_plonk_prove(void *pk, const RawFr *advice, size_t advice_len, uint64_t rng_seed,
+                                              uint8_t *proof, size_t proof_capacity, size_t *proof_len)
+{
+    if (!pk || !advice || !proof_len) return make_error((int)hipErrorInvalidValue, "invalid prove arguments");
+    auto *p = reinterpret_cast<PlonkKey *>(pk);
+    if (advice_len != p->advice_cols * p->n)
+        return make_error((int)hipErrorInvalidValue, "advice length does not match the key");
+    MH_TRY
+    const std::vector<uint8_t> bytes = plonk_prove(*p, advice, rng_seed);
+    *proof_len = bytes.size();
+    if (!proof || proof_capacity < bytes.size()) return make_error((int)hipErrorInvalidValue, "proof buffer too small");
+    std::memcpy(proof, bytes.data(), bytes.size());
+    return ok();
+    MH_CATCH
+}

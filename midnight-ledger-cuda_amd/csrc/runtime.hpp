@@ -47,26 +47,46 @@ hipStream_t lane_stream(uint32_t lane);
 inline hipStream_t cur_stream() { return lane_stream(current_lane()); }
 int device_id();
 
-// Owning device allocation.
+// Stream-ordered device memory: allocations come from the device's default
+// memory pool on the current lane's stream and are returned to it on that
+// stream.  hipFree would synchronise every stream of the device, so proofs
+// running on different lanes would stall each other at every temporary
+// buffer; the pool keeps freed memory for reuse (release threshold raised
+// once per device).
+void *dev_alloc(size_t bytes, hipStream_t st);
+void dev_free(void *p, hipStream_t st);
+
+// Owning device allocation (used on the stream that allocated it).
 template <class T>
 struct DevBuf {
     T *ptr = nullptr;
     size_t len = 0;
+    hipStream_t st = nullptr;
     DevBuf() = default;
     explicit DevBuf(size_t n) { alloc(n); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), len(o.len) { o.ptr = nullptr; o.len = 0; }
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), len(o.len), st(o.st) { o.ptr = nullptr; o.len = 0; }
     DevBuf &operator=(DevBuf &&o) noexcept
     {
-        if (this != &o) { release(); ptr = o.ptr; len = o.len; o.ptr = nullptr; o.len = 0; }
+        if (this != &o) {
+            release();
+            ptr = o.ptr;
+            len = o.len;
+            st = o.st;
+            o.ptr = nullptr;
+            o.len = 0;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
     void alloc(size_t n)
     {
         release();
-        if (n) MH_HIP(hipMalloc(reinterpret_cast<void **>(&ptr), n * sizeof(T)));
+        if (n) {
+            st = cur_stream();
+            ptr = static_cast<T *>(dev_alloc(n * sizeof(T), st));
+        }
         len = n;
     }
     // grow-only reuse for scratch buffers
@@ -76,7 +96,7 @@ struct DevBuf {
     }
     void release()
     {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) dev_free(ptr, st);
         ptr = nullptr;
         len = 0;
     }

@@ -147,6 +147,19 @@ def _declare(L):
             E, [vpp, u64p, u64p, c_size, c_size, c_size, RawFr, RawFr, RawFr, u64p, vp, c_size]),
         "midnight_sppark_bls12_381_trace_context_permutation": (
             E, [vpp, vp, vp, c_size, vp, c_size, RawFr, RawFr, u64p, c_size, vp, c_size, c_size]),
+        "midnight_hip_plonk_keygen": (
+            E, [vpp, u32, vp, c_size, u32, vp, c_size, c_size, vp, c_size, vp, c_size, vp, c_size, vp, RawGraph,
+                ctypes.c_char_p]),
+        "midnight_hip_plonk_key_free": (None, [vp]),
+        "midnight_hip_plonk_vk": (E, [vp, u64p, u64p, u64p]),
+        "midnight_hip_plonk_prove": (E, [vp, u64p, c_size, ctypes.c_uint64, ctypes.c_char_p, c_size,
+                                         ctypes.POINTER(c_size)]),
+        "midnight_hip_proof_server_create": (E, [vpp, u32, ctypes.c_uint64]),
+        "midnight_hip_proof_server_submit": (E, [vp, vp, u64p, c_size, ctypes.c_uint64,
+                                                 ctypes.POINTER(ctypes.c_uint64)]),
+        "midnight_hip_proof_server_wait": (E, [vp, ctypes.c_uint64, ctypes.c_char_p, c_size, ctypes.POINTER(c_size),
+                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+        "midnight_hip_proof_server_free": (None, [vp]),
         "midnight_hip_profile_enable": (None, [ctypes.c_int]),
         "midnight_hip_profile_read": (
             ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]),
@@ -664,3 +677,90 @@ def fr_permutation_products(columns, permutations, n, chunk_len, *, beta, gamma,
         ctypes.byref(h), P(cols), P(perm), n, cols.shape[0] // n, chunk_len, RawFr.of(beta), RawFr.of(gamma),
         RawFr.of(omega), P(dp), _ptr(bl), blinding_factors))
     return DeviceVec(h.value)
+
+
+# ---------------------------------------------------------------- proof pipeline
+
+class PlonkKey:
+    """Proving key of a gates + copy-permutation circuit, resident on the
+    device (midnight_hip_plonk_keygen); proofs with prove()."""
+
+    def __init__(self, k, srs, degree, fixed, advice_cols, advice_queries, fixed_queries, perm_columns,
+                 perm_mapping, gate, description=b""):
+        n = 1 << k
+        fx = _fr_array(fixed) if fixed is not None and len(fixed) else None
+        aq = np.ascontiguousarray([(c, r & 0xFFFFFFFF) for c, r in advice_queries], dtype=np.uint32).reshape(-1)
+        fq = np.ascontiguousarray([(c, r & 0xFFFFFFFF) for c, r in fixed_queries], dtype=np.uint32).reshape(-1)
+        col_arr, col_ptr = _column_refs(perm_columns)
+        mp = np.ascontiguousarray(perm_mapping, dtype=np.uint32)
+        assert mp.size == 2 * len(perm_columns) * n
+        h = ctypes.c_void_p()
+        check(lib().midnight_hip_plonk_keygen(
+            ctypes.byref(h), k, srs.handle, srs.npoints, degree, _ptr(fx), 0 if fx is None else fx.shape[0] // n,
+            advice_cols, _ptr(aq), len(advice_queries), _ptr(fq), len(fixed_queries), col_ptr, len(perm_columns),
+            mp.ctypes.data, gate.as_raw(), description))
+        self.handle = h
+        self.k, self.n = k, n
+        self.advice_cols = advice_cols
+        self.nfixed = 0 if fx is None else fx.shape[0] // n
+        self.nperm = len(perm_columns)
+
+    def vk(self):
+        fc = np.zeros((self.nfixed, 18), dtype=np.uint64)
+        pc = np.zeros((self.nperm, 18), dtype=np.uint64)
+        rp = np.zeros(4, dtype=np.uint64)
+        check(lib().midnight_hip_plonk_vk(self.handle, P(fc) if self.nfixed else None, P(pc), P(rp)))
+        return fc, pc, rp
+
+    def prove(self, advice, seed, capacity=1 << 16):
+        a = _fr_array(advice)
+        buf = ctypes.create_string_buffer(capacity)
+        ln = c_size()
+        check(lib().midnight_hip_plonk_prove(self.handle, P(a), a.shape[0], seed, buf, capacity, ctypes.byref(ln)))
+        return buf.raw[:ln.value]
+
+    def free(self):
+        if self.handle and self.handle.value:
+            lib().midnight_hip_plonk_key_free(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class ProofServer:
+    """Continuous-batching proof server (midnight_hip_proof_server_*)."""
+
+    def __init__(self, slots, hbm_budget_bytes):
+        h = ctypes.c_void_p()
+        check(lib().midnight_hip_proof_server_create(ctypes.byref(h), slots, hbm_budget_bytes))
+        self.handle = h
+
+    def submit(self, pk, advice, seed):
+        a = _fr_array(advice)
+        jid = ctypes.c_uint64()
+        check(lib().midnight_hip_proof_server_submit(self.handle, pk.handle, P(a), a.shape[0], seed,
+                                                     ctypes.byref(jid)))
+        return jid.value
+
+    def wait(self, job, capacity=1 << 16):
+        buf = ctypes.create_string_buffer(capacity)
+        ln = c_size()
+        qms, pms = ctypes.c_double(), ctypes.c_double()
+        check(lib().midnight_hip_proof_server_wait(self.handle, job, buf, capacity, ctypes.byref(ln),
+                                                   ctypes.byref(qms), ctypes.byref(pms)))
+        return buf.raw[:ln.value], qms.value, pms.value
+
+    def free(self):
+        if self.handle and self.handle.value:
+            lib().midnight_hip_proof_server_free(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

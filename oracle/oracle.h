@@ -203,6 +203,11 @@ void orc_permutation_products(orc_fr *out, const orc_fr *columns, const orc_fr *
 /* shifted_prefix_product (permutation/prover.rs:36-43); out may not alias values */
 void orc_shifted_prefix_product(orc_fr *out, const orc_fr *values, size_t len, const orc_fr *initial);
 
+/* ---- coefficient-form polynomials (utils/arithmetic.rs) ---- */
+void orc_poly_eval(orc_fr *out, const orc_fr *poly, size_t n, const orc_fr *point);
+/* q[0 .. n-2] = a / (X - b) without remainder, q[n-1] = 0; q may alias a */
+void orc_kate_division(orc_fr *q, const orc_fr *a, size_t n, const orc_fr *b);
+
 /* ---- RNG : rand_core 0.6 seed_from_u64 (PCG32) + rand_chacha 0.3 ChaCha20 ---- */
 typedef struct {
     uint32_t key[8];

@@ -103,6 +103,12 @@ def _declare(L):
                                    ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32)]),
         "orc_eval_numerator": (V, [u64p, ctypes.c_void_p, ctypes.c_int]),
         "orc_shifted_prefix_product": (V, [u64p, u64p, ctypes.c_size_t, u64p]),
+        "orc_poly_eval": (V, [u64p, u64p, ctypes.c_size_t, u64p]),
+        "orc_g1j_mul": (V, [u64p, u64p, u64p]),
+        "orc_g1j_neg": (V, [u64p, u64p]),
+        "orc_kate_division": (V, [u64p, u64p, ctypes.c_size_t, u64p]),
+        "orc_rng_from_seed": (V, [ctypes.POINTER(Rng), ctypes.POINTER(ctypes.c_uint8)]),
+        "orc_rng_next_u32": (ctypes.c_uint32, [ctypes.POINTER(Rng)]),
         "orc_permutation_products": (V, [u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
                                          u64p, u64p, u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
         "orc_graph_eval_rows": (V, [u64p, ctypes.c_void_p, ctypes.c_void_p, u64p, ctypes.c_int]),
@@ -540,3 +546,42 @@ def shifted_prefix_product(values, initial):
     out = fr(v.shape[0])
     lib().orc_shifted_prefix_product(P(out), P(v), v.shape[0], P(np.ascontiguousarray(initial, dtype=np.uint64)))
     return out
+
+
+def poly_eval(coeffs, point):
+    c = np.ascontiguousarray(coeffs, dtype=np.uint64).reshape(-1, 4)
+    out = fr()
+    lib().orc_poly_eval(P(out), P(c), c.shape[0], P(np.ascontiguousarray(point, dtype=np.uint64)))
+    return out
+
+
+def kate_division(coeffs, point):
+    """(a - a(b)) / (X - b) with the same length (top coefficient zero)"""
+    c = np.array(coeffs, dtype=np.uint64, order="C", copy=True).reshape(-1, 4)
+    lib().orc_kate_division(P(c), P(c), c.shape[0], P(np.ascontiguousarray(point, dtype=np.uint64)))
+    return c
+
+
+class ChaCha:
+    """The oracle's rand_chacha ChaCha20Rng (seed_from_u64 or a 32-byte seed)."""
+
+    def __init__(self, seed=None, seed_bytes=None):
+        self.r = Rng()
+        if seed_bytes is not None:
+            b = (ctypes.c_uint8 * 32)(*seed_bytes)
+            lib().orc_rng_from_seed(ctypes.byref(self.r), b)
+        else:
+            lib().orc_rng_seed_from_u64(ctypes.byref(self.r), seed)
+
+    def random_fr(self, n):
+        out = fr(n)
+        if n:
+            lib().orc_fr_random_many(P(out), n, ctypes.byref(self.r))
+        return out
+
+    def fill_bytes(self, n):
+        out = bytearray()
+        while len(out) < n:
+            w = lib().orc_rng_next_u32(ctypes.byref(self.r))
+            out += int(w).to_bytes(4, "little")
+        return bytes(out[:n])

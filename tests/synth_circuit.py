@@ -239,3 +239,23 @@ def random_inputs(n, seed, *, rot_scale=4, sets=3, graph=None):
         l0=l0, l_last=ll, l_active=la, permutation_products=zs, permutation_columns=PERM_COLUMNS,
         permutation_pk_cosets=sig,
         custom_graph=graph or G.RawGraphBuffers(G.custom_gates_graph([gate_expression()])))
+
+
+DESCRIPTION = (b"synthetic PLONK circuit: advice a b c d e, fixed q s1 s2, gate q*(a*b+c-d), "
+               b"copy permutation over a b c d e q s1 s2, degree 5")
+ADVICE_QUERIES = [(c, 0) for c in range(ADVICE)]
+FIXED_QUERIES = [(c, 0) for c in range(FIXED)]
+
+
+def mapping(c):
+    """(j', i') of the next cell of every permuted cell's cycle, as the
+    keygen's flat uint32 array [2 * (j * n + i)]."""
+    n = c.n
+    m = np.zeros((len(PERM_COLUMNS), n, 2), dtype=np.uint32)
+    m[:, :, 0] = np.arange(len(PERM_COLUMNS), dtype=np.uint32)[:, None]
+    m[:, :, 1] = np.arange(n, dtype=np.uint32)[None, :]
+    for cyc in c.cycles:
+        for t, (j, i) in enumerate(cyc):
+            jj, ii = cyc[(t + 1) % len(cyc)]
+            m[j, i] = (jj, ii)
+    return m.reshape(-1)

@@ -1,0 +1,92 @@
+"""GPU proof pipeline (midnight_hip_plonk_keygen / _prove) vs the
+reference-algorithm CPU prover: identical proof bytes for the same circuit,
+SRS and blinding seed, accepted by the designated verifier.  Stage order and
+transcript: plonk/prover.rs:68-575, :769-1010 (see tests/plonk_ref.py)."""
+import numpy as np
+import pytest
+
+import numerator_graph as G
+import plonk_ref as PR
+import synth_circuit as S
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_key(H, O, c, tau_int, extra_srs=0):
+    n = c.n
+    powers = H.fr_shifted_prefix_product(np.tile(PR.to_fr(tau_int), (n + extra_srs, 1)), O.fr_from_int(1))
+    dv = H.DeviceVec.upload(powers)
+    srs = H.MsmContext.from_multiples(dv)
+    dv.free()
+    fixed = S.mont([v for col in c.cols[5:] for v in col])
+    pk = H.PlonkKey(c.k, srs, S.DEGREE, fixed, S.ADVICE, S.ADVICE_QUERIES, S.FIXED_QUERIES, S.PERM_COLUMNS,
+                    S.mapping(c), c.custom_graph(), S.DESCRIPTION)
+    return pk, srs
+
+
+@pytest.mark.parametrize("k,seed", [(7, 42), (10, 7)])
+def test_gpu_proof_matches_cpu(H, O, k, seed):
+    c = S.Circuit(k, seed=3 + k)
+    tau = 987654321 + k
+    key = PR.CpuKey(c, tau)
+    pk, srs = gpu_key(H, O, c, tau, extra_srs=5)
+    fc, pc, rp = pk.vk()
+    for a, b in zip(fc, key.fixed_com):
+        assert O.g1_eq(a, b)
+    for a, b in zip(pc, key.sigma_com):
+        assert O.g1_eq(a, b)
+    assert PR.to_int(rp) == key.repr
+    print("proof k=%d" % k, flush=True)
+    gpu = pk.prove(S.mont([v for col in c.cols[:5] for v in col]), seed)
+    cpu = PR.cpu_prove(key, c.cols[:5], seed)
+    assert len(gpu) == len(cpu)
+    assert gpu == cpu
+    assert PR.verify((fc, pc, PR.to_int(rp)), gpu, key)
+    pk.free()
+    srs.free()
+
+
+def test_gpu_proof_rejects_bad_inputs(H, O):
+    c = S.Circuit(7, seed=9)
+    pk, srs = gpu_key(H, O, c, 5555)
+    with pytest.raises(H.MidnightHipError):
+        pk.prove(np.zeros((10, 4), dtype=np.uint64), 1)
+    with pytest.raises(H.MidnightHipError):
+        pk.prove(S.mont([v for col in c.cols[:5] for v in col]), 1, capacity=100)
+    pk.free()
+    srs.free()
+
+
+def test_server_results_same_alone_and_in_mix(H, O):
+    """Continuous batching: proofs of different k run concurrently on
+    several lanes; every proof equals the one made alone (and the CPU
+    reference for the small circuits), whatever runs beside it."""
+    circuits = [S.Circuit(k, seed=40 + k) for k in (7, 9, 11, 12)]
+    keys = [gpu_key(H, O, c, 111 + c.k) for c in circuits]
+    adv = [S.mont([v for col in c.cols[:5] for v in col]) for c in circuits]
+    alone = {}
+    for i, (pk, srs) in enumerate(keys):
+        for seed in (1, 2):
+            alone[(i, seed)] = pk.prove(adv[i], seed)
+    cpu = PR.cpu_prove(PR.CpuKey(circuits[0], 111 + circuits[0].k), circuits[0].cols[:5], 1)
+    assert alone[(0, 1)] == cpu
+    server = H.ProofServer(4, 8 << 30)
+    jobs = []
+    for rep in range(3):
+        for i, (pk, srs) in enumerate(keys):
+            for seed in (1, 2):
+                jobs.append(((i, seed), server.submit(pk, adv[i], seed)))
+    for key, jid in jobs:
+        proof, qms, pms = server.wait(jid)
+        assert proof == alone[key], key
+        assert qms >= 0 and pms > 0
+    server.free()
+    # a budget smaller than one proof still serves (one at a time)
+    server = H.ProofServer(3, 1)
+    jids = [server.submit(keys[1][0], adv[1], 2) for _ in range(4)]
+    for jid in jids:
+        assert server.wait(jid)[0] == alone[(1, 2)]
+    server.free()
+    for pk, srs in keys:
+        pk.free()
+        srs.free()
