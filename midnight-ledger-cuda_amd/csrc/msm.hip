@@ -485,7 +485,11 @@ void build_precomp(MsmContext &ctx, hipStream_t st)
     if (n < (size_t(1) << 10)) return;
     int lg = 0;
     while ((size_t(1) << lg) < n) ++lg;
-    uint32_t c = (uint32_t)std::min(20, std::max(10, lg - 4));
+    // c = log2 n - 4, at most 16: wider windows cut the entries
+    // (n * ceil(256/c)) but multiply the digit passes of the LDS sort
+    // (2^(c-1) / 16384 key ranges) and the bucket reduction; at 2^24 points
+    // c = 20 spent as long sorting and reducing as accumulating.
+    uint32_t c = (uint32_t)std::min(16, std::max(10, lg - 4));
     uint32_t nwin = (256 + c - 1) / c;
     if ((uint64_t)nwin * n >= (1ull << 31)) return;   // entry indices are 31-bit
     ctx.table.alloc((size_t)nwin * n * 28);   // Affine28 rows
