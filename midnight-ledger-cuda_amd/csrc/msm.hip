@@ -477,6 +477,17 @@ __global__ void __launch_bounds__(kBlock) k_precompute(const uint32_t *in, uint3
 
 inline uint32_t grid1(uint64_t n) { return (uint32_t)ceil_div(n, kBlock); }
 
+// lanes that give every SIMD of the 256 CUs two waves
+constexpr uint64_t kFillLanes = 256 * 4 * 2 * 64;
+
+// items per lane for `work` serial items: `cap`, or fewer (at least 4) when
+// `cap` per lane would leave the GPU short of kFillLanes lanes
+inline uint32_t fill_group(uint64_t work, uint32_t cap)
+{
+    const uint64_t g = ceil_div(work, kFillLanes);
+    return (uint32_t)std::max<uint64_t>(std::min<uint64_t>(g, cap), std::min<uint32_t>(4, cap));
+}
+
 }  // namespace
 
 void build_precomp(MsmContext &ctx, hipStream_t st)
@@ -640,7 +651,7 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
     }
 
     // 4-6: chunked accumulation into buckets
-    const uint32_t L = chunk_len;
+    const uint32_t L = fill_group(nent, chunk_len);
     b.q.ensure(nkeys);
     b.co.ensure(nkeys);
     hipLaunchKernelGGL(k_chunk_counts, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.cnt.get(), b.q.get(), nkeys, L);
@@ -669,7 +680,7 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
         last_entries = nent;
         // multi-chunk buckets: tree levels of `kGroup` partials while any
         // bucket still has more than kGroup partials
-        constexpr uint32_t kGroup = 16;
+        const uint32_t kGroup = nchunks >= 4 * kFillLanes ? 16 : 4;
         b.maxq.ensure(1);
         MH_HIP(hipMemsetAsync(b.maxq.get(), 0, 4, st));
         hipLaunchKernelGGL(k_retire_single, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), nkeys, b.maxq.get());
@@ -710,54 +721,81 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
 
     // 7: bit-plane partial sums per (msm, window, plane)
     MH_CHECK(g.c >= 1 && g.c <= kMaxWindow, (int)hipErrorInvalidValue, "unsupported MSM window width");
+    const uint64_t nmw = (uint64_t)g.M * g.kw;
+    const uint32_t G1 = fill_group(nmw * ((uint64_t)(g.c - 1) * (g.B / 2) + 1), plane_group);
     PlaneLayout pl{};
     pl.c = g.c;
     uint32_t acc = 0;
     for (uint32_t k = 0; k < g.c; ++k) {
         pl.members[k] = k + 1 < g.c ? g.B / 2 : 1;
         pl.first[k] = acc;
-        acc += (pl.members[k] + plane_group - 1) / plane_group;
+        acc += (pl.members[k] + G1 - 1) / G1;
     }
     pl.first[g.c] = acc;
     for (uint32_t k = g.c + 1; k <= kMaxWindow; ++k) pl.first[k] = 0xffffffffu;
     pl.per_window = acc;
-    const uint64_t nmw = (uint64_t)g.M * g.kw;
     const uint64_t npp = nmw * pl.per_window;
     b.pp.ensure(npp * 48);
-    hipLaunchKernelGGL(k_plane_partials, dim3(grid1(npp)), dim3(kBlock), 0, st, b.bucket.get(), g, pl, plane_group,
-                       npp, b.pp.get());
+    hipLaunchKernelGGL(k_plane_partials, dim3(grid1(npp)), dim3(kBlock), 0, st, b.bucket.get(), g, pl, G1, npp,
+                       b.pp.get());
     MH_HIP(hipGetLastError());
 
-    // second pass: groups of plane_group2 within each (msm, window, plane)
+    // group passes within each segment (msm, window, plane) until one
+    // partial per segment is left; the last pass converts to Jacobian
     const uint64_t nseg = nmw * g.c;
-    std::vector<uint32_t> seg_off(nseg), seg_len(nseg), out_off(nseg);
-    uint64_t tot_out = 0;
-    std::vector<uint32_t> plane_out(g.c);
+    struct Level {
+        uint64_t base, total_out;
+        uint32_t G;
+    };
+    std::vector<Level> levels;
+    std::vector<uint32_t> seg_off, seg_len, out_off;   // all levels, concatenated
+    std::vector<uint32_t> offs(nseg), lens(nseg);
     for (uint64_t s = 0; s < nseg; ++s) {
-        uint64_t mw = s / g.c;
-        uint32_t k = (uint32_t)(s % g.c);
-        seg_off[s] = (uint32_t)(mw * pl.per_window + pl.first[k]);
-        seg_len[s] = pl.first[k + 1] - pl.first[k];
-        out_off[s] = (uint32_t)tot_out;
-        uint32_t o = (seg_len[s] + plane_group2 - 1) / plane_group2;
-        if (mw == 0) plane_out[k] = o;
-        tot_out += o;
+        const uint32_t k = (uint32_t)(s % g.c);
+        offs[s] = (uint32_t)((s / g.c) * pl.per_window + pl.first[k]);
+        lens[s] = pl.first[k + 1] - pl.first[k];
     }
-    b.seg_off.ensure(nseg);
-    b.seg_len.ensure(nseg);
-    b.out_off.ensure(nseg);
-    MH_HIP(hipMemcpyAsync(b.seg_off.get(), seg_off.data(), nseg * 4, hipMemcpyHostToDevice, st));
-    MH_HIP(hipMemcpyAsync(b.seg_len.get(), seg_len.data(), nseg * 4, hipMemcpyHostToDevice, st));
-    MH_HIP(hipMemcpyAsync(b.out_off.get(), out_off.data(), nseg * 4, hipMemcpyHostToDevice, st));
-    b.pj.ensure(tot_out * 36);
-    hipLaunchKernelGGL(k_group_sums, dim3(grid1(tot_out)), dim3(kBlock), 0, st, b.pp.get(), nseg, b.seg_off.get(),
-                       b.seg_len.get(), b.out_off.get(), plane_group2, tot_out, b.pj.get(), 1);
-    MH_HIP(hipGetLastError());
+    uint64_t in_total = npp;
+    for (;;) {
+        const uint32_t maxlen = *std::max_element(lens.begin(), lens.end());
+        const bool last = maxlen <= plane_group2;
+        const uint32_t G = last ? maxlen : (in_total >= (uint64_t)plane_group2 * kFillLanes / 8 ? plane_group2 : 4);
+        Level lv{seg_off.size(), 0, G};
+        for (uint64_t s = 0; s < nseg; ++s) {
+            seg_off.push_back(offs[s]);
+            seg_len.push_back(lens[s]);
+            out_off.push_back((uint32_t)lv.total_out);
+            offs[s] = (uint32_t)lv.total_out;
+            lens[s] = (lens[s] + G - 1) / G;
+            lv.total_out += lens[s];
+        }
+        levels.push_back(lv);
+        in_total = lv.total_out;
+        if (last) break;
+    }
+    b.seg_off.ensure(seg_off.size());
+    b.seg_len.ensure(seg_len.size());
+    b.out_off.ensure(out_off.size());
+    MH_HIP(hipMemcpyAsync(b.seg_off.get(), seg_off.data(), seg_off.size() * 4, hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(b.seg_len.get(), seg_len.data(), seg_len.size() * 4, hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(b.out_off.get(), out_off.data(), out_off.size() * 4, hipMemcpyHostToDevice, st));
+    b.pp2.ensure(std::max<uint64_t>(levels[0].total_out, 1) * 48);
+    b.pj.ensure(nseg * 36);
+    uint32_t *lin = b.pp.get(), *lout = b.pp2.get();
+    for (size_t i = 0; i < levels.size(); ++i) {
+        const Level &lv = levels[i];
+        const bool last = i + 1 == levels.size();
+        hipLaunchKernelGGL(k_group_sums, dim3(grid1(lv.total_out)), dim3(kBlock), 0, st, lin, nseg,
+                           b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base, lv.G,
+                           lv.total_out, last ? b.pj.get() : lout, last ? 1 : 0);
+        MH_HIP(hipGetLastError());
+        std::swap(lin, lout);
+    }
 
-    // 8: host: plane sums and one doubling chain per MSM
-    std::vector<HJac> hj(tot_out);
+    // 8: host: one doubling chain per MSM over its (window, plane) sums
+    std::vector<HJac> hj(nseg);
     static_assert(sizeof(HJac) == 144, "HJac layout");
-    MH_HIP(hipMemcpyAsync(hj.data(), b.pj.get(), tot_out * 144, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipMemcpyAsync(hj.data(), b.pj.get(), nseg * 144, hipMemcpyDeviceToHost, st));
     prof.end(st);
     MH_HIP(hipStreamSynchronize(st));
     prof.flush();
@@ -769,8 +807,7 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
         for (uint32_t e = E; e-- > 0;) {
             accj = accj.dbl();
             uint32_t w = e / g.c, k = e % g.c;
-            uint64_t s = ((uint64_t)m * g.kw + w) * g.c + k;
-            for (uint32_t x = 0; x < plane_out[k]; ++x) accj = accj + hj[out_off[s] + x];
+            accj = accj + hj[((uint64_t)m * g.kw + w) * g.c + k];
         }
         std::memcpy(&out[m], &accj, sizeof(RawG1Projective));
     };
@@ -856,7 +893,12 @@ void run_batched(MsmContext &ctx, MsmEngine &engine, const std::vector<MsmJob> &
             ++j;
         }
         std::vector<MsmJob> part(jobs.begin() + i, jobs.begin() + j);
-        if (ctx.pc_nwin) {
+        size_t part_max = 0;
+        for (const auto &pj : part) part_max = std::max<size_t>(part_max, pj.n);
+        // the table's one bucket set has 2^(c-1) buckets; MSMs much shorter
+        // than that (a small circuit over a large SRS) spend less on plain
+        // windows sized for their own length
+        if (ctx.pc_nwin && part_max >= (size_t(1) << (ctx.pc_c - 1))) {
             PrecompTable pc{ctx.table.get(), ctx.npoints, ctx.pc_c, ctx.pc_nwin};
             engine.run(ctx.points.get(), ctx.npoints, part, 0, out + i, st, &pc);
         } else {

@@ -37,9 +37,13 @@ public:
     void run(const uint32_t *d_points, size_t npoints_total, const std::vector<MsmJob> &jobs, uint32_t window_bits,
              RawG1Projective *out, hipStream_t st, const PrecompTable *pc = nullptr);
 
+    // Upper bounds of the serial work per lane.  A point addition takes the
+    // same time per lane at any occupancy, so batches too small to fill the
+    // GPU get shorter chunks and groups (down to 4): more lanes, a shorter
+    // serial chain.
     uint32_t chunk_len = 64;      // entries per accumulation lane
     uint32_t plane_group = 16;    // buckets per lane in the first plane pass
-    uint32_t plane_group2 = 8;    // partials per lane in the second plane pass
+    uint32_t plane_group2 = 8;    // partials per lane in the later plane passes
 
     // statistics of the last run (for throughput / roofline reporting)
     uint64_t last_entries = 0;    // nonzero signed digits = bucket additions

@@ -32,7 +32,6 @@
 #include <memory>
 #include <set>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "devvec.hpp"
@@ -43,6 +42,7 @@
 #include "hostcrypto.hpp"
 #include "ntt.hpp"
 #include "polyops.hpp"
+#include "rngdev.hpp"
 #include "runtime.hpp"
 
 namespace mh {
@@ -423,22 +423,23 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
 
     // every draw of the prover's RNG, in the reference's order
     ChaChaRng rng = ChaChaRng::seed_from_u64(seed);
-    std::vector<HostFr> adv(A * n);
-    std::memcpy(adv.data(), advice, A * n * 32);
-    for (size_t c = 0; c < A; ++c)
-        for (size_t r = pk.u; r < n; ++r) adv[c * n + r] = rng.random_fr();   // unusable rows
+    const size_t brows = n - pk.u;
+    std::vector<HostFr> adv_blinds(A * brows);   // unusable rows u..n of each advice column
+    for (auto &b : adv_blinds) b = rng.random_fr();
     std::vector<HostFr> perm_blinds(S * pk.bf);
     for (auto &b : perm_blinds) b = rng.random_fr();
     uint8_t vseed[32];
     rng.fill_bytes(vseed, 32);
     std::vector<HostFr> limb_blinds(pk.pieces - 1);
     for (auto &b : limb_blinds) b = rng.random_fr();
-    // vanishing random polynomial: one ChaCha20 stream (a one-thread pool)
-    std::vector<HostFr> rand_poly(n);
-    std::thread rand_thread([&]() {
-        ChaChaRng r2 = ChaChaRng::from_seed(vseed);
-        for (auto &v : rand_poly) v = r2.random_fr();
-    });
+
+    // polys: advice | z sets | random | h pieces (coefficient form, n each)
+    const size_t P_ADV = 0, P_Z = A, P_RAND = A + S, P_H = A + S + 1, NPOLY = A + S + 1 + pk.pieces;
+    auto polys = new_vec(NPOLY * n);
+    Fr *pd = polys->values.get();
+    // vanishing random polynomial: one ChaCha20 stream (as with a one-thread
+    // pool), drawn on the device while the stream is still empty
+    random_fr_device(pd + P_RAND * n, n, vseed, st);
 
     T.common_scalar(pk.repr);
 
@@ -447,12 +448,12 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     adv_lag.lane = current_lane();
     adv_lag.len = A * n;
     adv_lag.values.alloc(A * n);
-    MH_HIP(hipMemcpyAsync(adv_lag.values.get(), adv.data(), A * n * 32, hipMemcpyHostToDevice, st));
-    // polys: advice | z sets | random | h pieces (coefficient form, n each)
-    const size_t P_ADV = 0, P_Z = A, P_RAND = A + S, P_H = A + S + 1, NPOLY = A + S + 1 + pk.pieces;
-    auto polys = new_vec(NPOLY * n);
-    Fr *pd = polys->values.get();
-    MH_HIP(hipMemcpyAsync(pd, adv_lag.values.get(), A * n * 32, hipMemcpyDeviceToDevice, st));
+    Fr *al = adv_lag.values.get();
+    MH_HIP(hipMemcpyAsync(al, advice, A * n * 32, hipMemcpyHostToDevice, st));
+    for (size_t c = 0; c < A && brows; ++c)
+        MH_HIP(hipMemcpyAsync(al + c * n + pk.u, adv_blinds.data() + c * brows, brows * 32, hipMemcpyHostToDevice,
+                              st));
+    MH_HIP(hipMemcpyAsync(pd, al, A * n * 32, hipMemcpyDeviceToDevice, st));
     ntt_batch(pd + P_ADV * n, pk.k, A, true, pk.scratch(), st);
     std::vector<RawG1Projective> com;
     commit(com, pk, *polys, P_ADV, A);
@@ -477,8 +478,6 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
 
     const HostFr trash = T.squeeze();
 
-    rand_thread.join();
-    MH_HIP(hipMemcpyAsync(pd + P_RAND * n, rand_poly.data(), n * 32, hipMemcpyHostToDevice, st));
     commit(com, pk, *polys, P_RAND, 1);
     T.write_point(com[0]);
 

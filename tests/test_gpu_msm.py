@@ -255,3 +255,27 @@ def test_msm_2p24_uniform_and_skewed(H, O):
         assert O.g1_eq(out[i], O.g1_mul(O.generator(), O.inner_product(sc, ts))), i
     dv.free()
     ctx.free()
+
+
+def test_msm_short_on_large_precomputed_context(H, O):
+    """MSMs much shorter than a precomputed context's bucket set take plain
+    windows over the context's points (a small circuit over a large SRS);
+    mixed lengths in one ragged batch, each against the closed form."""
+    n = 1 << 16
+    ts = O.random_fr(n, 990)
+    dv_t = H.DeviceVec.upload(ts)
+    ctx = H.MsmContext.from_multiples(dv_t)
+    sizes = [1, 100, 1 << 10, 1 << 12, 1 << 14, 1 << 16]
+    sc = O.random_fr(sum(sizes), 991)
+    dv = H.DeviceVec.upload(sc)
+    offs = np.cumsum([0] + sizes[:-1]).tolist()
+    out = ctx.device_multi(dv, offs, sizes)
+    for i, (o, m) in enumerate(zip(offs, sizes)):
+        want = O.g1_mul(O.generator(), O.inner_product(sc[o:o + m], ts[:m]))
+        assert O.g1_eq(out[i], want), m
+    for m in (1 << 12, 1 << 15):
+        got = ctx.device_batch(dv, 0, m, 1)[0]
+        assert O.g1_eq(got, O.g1_mul(O.generator(), O.inner_product(sc[:m], ts[:m]))), m
+    ctx.free()
+    dv.free()
+    dv_t.free()
