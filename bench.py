@@ -33,11 +33,35 @@ PEAK_SPEC_MAC_PER_S = 256 * 64 * 2.4e9
 HBM_PEAK = 8.0e12
 # algorithmic Montgomery-product cost of one Fp product (12 limbs, CIOS)
 MACS_PER_FP_MUL = 300
-# SURVEY 8(d) algorithmic MSM work: min over c of N*ceil(256/c)*(8M+2S) + ceil(256/c)*2^(c-1)*2*(12M+2S)
+# algorithmic MSM work: min over c of N*ceil(256/c)*(8M+2S) + ceil(256/c)*2^(c-1)*2*(12M+2S)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def copy_calibration(H, mib):
+    """Device-to-device copies of `mib` MiB by a plain copy kernel
+    (k_copy16, one untimed and one timed launch), run before the timed
+    region so that hardware-counter runs can scale the FETCH_SIZE /
+    WRITE_SIZE they read for the other kernels."""
+    rate = H.copy_bandwidth(mib << 20)
+    log("copy calibration: %d MiB, %.0f GB/s read+write" % (mib, rate / 1e9))
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch of the dominant kernel, from the hardware-counter
+    summary committed under profiles/ (FETCH_SIZE + WRITE_SIZE, each scaled
+    by the counter's reading for a copy of known size); None if absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
 
 
 def msm_algorithmic_macs(n):
@@ -128,6 +152,8 @@ def bench_msm(args, H):
     del sc
     for _ in range(args.warmup):
         ctx.device_batch(dv, 0, n, m)
+    if args.pmc_copy_mib:
+        copy_calibration(H, args.pmc_copy_mib)
     peak, _, _ = H.valu_peak()
     H.profile_enable(True)
     t0 = time.perf_counter()
@@ -164,7 +190,9 @@ def bench_msm(args, H):
         "roofline": {"bound": "valu", "kernel": "msm_accumulate",
                      "achieved": round(achieved / 1e12, 3), "peak": round(peak / 1e12, 3),
                      "peak_spec": round(PEAK_SPEC_MAC_PER_S / 1e12, 3),
-                     "unit": "TMAC/s", "frac": round(achieved / peak, 4), "traffic": None},
+                     "unit": "TMAC/s", "frac": round(achieved / peak, 4),
+                     "traffic": profiled_traffic("msm_accumulate_2p20_traffic.md")
+                     if (log_n, m, args.skewed) == (20, 4, False) else None},
         "msm_valu_frac": round(msm_macs / (dt / args.steps) / peak, 4),
         "accumulate_ms_avg": round(acc_avg_s * 1e3, 3),
         "engine_ms_avg": round(tot_ms / max(tot_cnt, 1), 3),
@@ -663,6 +691,8 @@ def main():
     ap.add_argument("--slots", type=int, default=8, help="serving: concurrent proofs")
     ap.add_argument("--load", type=float, default=4.0, help="serving: offered load / one-at-a-time capacity")
     ap.add_argument("--hbm-budget-gb", type=int, default=64, help="serving: HBM admission budget")
+    ap.add_argument("--pmc-copy-mib", type=int, default=0,
+                    help="msm: run one device copy of this many MiB before timing (counter calibration)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU-baseline sample length (0 disables)")
     args = ap.parse_args()
@@ -671,6 +701,13 @@ def main():
         sys.exit(2)
     if int(os.environ.get("RANK", "0")) != 0:
         return
+    if args.workload == "serve":
+        # one hardware queue per lane stream (the proof server's slots plus
+        # the caller's lane 0); HIP's default of 4 would make lanes share
+        # queues and serialise their kernels.  Read when HIP starts, so it is
+        # set before the library is loaded.
+        want = 16 if args.slots + 1 <= 16 else 32
+        os.environ["GPU_MAX_HW_QUEUES"] = str(max(int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), want))
     import midnight_hip as H
     H.lib()
     run = {"msm": bench_msm, "ntt": bench_ntt, "batch": bench_batch, "quotient": bench_quotient, "kzg": bench_kzg, "serve": bench_serve, "prove": bench_prove}[args.workload]

@@ -432,6 +432,10 @@ void midnight_hip_proof_server_free(void *server);
  * on (v_mad_u64_u32), in operations per second over the whole device, and
  * the device clock/CU count used; for roofline reporting. */
 RustError midnight_hip_valu_peak(double *mad_u64_per_s, double *clock_hz, int *num_cus);
+/* One device-to-device copy of `bytes` (multiple of 16) by a plain copy
+ * kernel: read + write bytes per second.  Also the launch of known size that
+ * hardware byte counters are scaled against. */
+RustError midnight_hip_copy_bandwidth(size_t bytes, double *bytes_per_s);
 /* Whole Fp / Fr Montgomery products per second (blocks = 0: whole device;
  * blocks = 1: one wave, i.e. the latency of a dependent chain). */
 RustError midnight_hip_field_mul_rate(int blocks, double *fp_per_s, double *fr_per_s);

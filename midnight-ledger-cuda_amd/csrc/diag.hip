@@ -1,6 +1,7 @@
 // Device measurements used for roofline reporting: the sustained rate of
 // v_mad_u64_u32 (the 32x32+64 multiply-add every field product here is
 // built from), and the rate of whole Fp / Fr Montgomery products.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -156,6 +157,53 @@ extern "C" RustError midnight_hip_valu_peak(double *mad_u64_per_s, double *clock
         if (mad_u64_per_s) *mad_u64_per_s = ops / t;
         if (clock_hz) *clock_hz = (double)prop.clockRate * 1e3;
         if (num_cus) *num_cus = prop.multiProcessorCount;
+        return ok();
+    } catch (const hip_error &e) {
+        return make_error(e.code, e.what());
+    }
+}
+
+namespace mh {
+namespace {
+
+// plain streaming copy, 16 B per lane and iteration (grid-stride)
+__global__ void __launch_bounds__(256) k_copy16(const uint4 *src, uint4 *dst, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace
+}  // namespace mh
+
+// One device-to-device copy of `bytes` (a multiple of 16) by a plain copy
+// kernel (k_copy16), after one untimed pass; returns its rate.  Also the
+// reference launch of known size for hardware-counter byte scaling.
+extern "C" RustError midnight_hip_copy_bandwidth(size_t bytes, double *bytes_per_s)
+{
+    using namespace mh;
+    if (bytes == 0 || bytes % 16) return make_error((int)hipErrorInvalidValue, "copy size must be a positive multiple of 16");
+    try {
+        hipStream_t st = cur_stream();
+        DevBuf<uint4> a(bytes / 16), b(bytes / 16);
+        const size_t n = bytes / 16;
+        const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+        MH_HIP(hipMemsetAsync(a.get(), 1, bytes, st));
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, a.get(), b.get(), n);
+        MH_HIP(hipGetLastError());
+        hipEvent_t e0, e1;
+        MH_HIP(hipEventCreate(&e0));
+        MH_HIP(hipEventCreate(&e1));
+        MH_HIP(hipEventRecord(e0, st));
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, a.get(), b.get(), n);
+        MH_HIP(hipGetLastError());
+        MH_HIP(hipEventRecord(e1, st));
+        MH_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (bytes_per_s) *bytes_per_s = 2.0 * (double)bytes / (ms * 1e-3);
         return ok();
     } catch (const hip_error &e) {
         return make_error(e.code, e.what());

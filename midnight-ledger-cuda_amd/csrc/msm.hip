@@ -301,11 +301,11 @@ __device__ __forceinline__ Affine fetch_point(const uint32_t *pts, uint32_t e)
 
 __global__ void __launch_bounds__(kBlock) k_accumulate(const uint32_t *pts, const uint32_t *ent, const uint32_t *cnt,
                                                        const uint32_t *off, const uint32_t *q, const uint32_t *co,
-                                                       const uint32_t *cmap, uint32_t nchunks, uint32_t L,
+                                                       const uint32_t *cmap, const uint32_t *totals, uint32_t L,
                                                        uint32_t *part, uint32_t *bucket)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nchunks) return;
+    if (j >= totals[1]) return;
     const uint32_t b = cmap[j];
     const uint32_t r = j - co[b];
     const uint32_t start = off[b] + r * L;
@@ -322,22 +322,26 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(const uint32_t *pts, cons
     else store_xyzz(part + (uint64_t)j * 48, acc);
 }
 
-// Same pass over a precomputed table held in radix 2^28 (28 u32 per point):
-// the bucket sum runs in Fp28 and is converted once at the end.
+// Same pass over a precomputed table held in radix 2^28 (28 u32 per point,
+// rows padded to 32 u32 so that every gathered point is one 128-byte cache
+// line; unpadded, six rows in eight straddle two lines): the bucket sum runs
+// in Fp28 and is converted once at the end.
+constexpr uint32_t kRow28 = 32;
+
 __device__ __forceinline__ Affine28 fetch_point28(const uint32_t *pts, uint32_t e)
 {
-    Affine28 p = load_affine28(pts + (uint64_t)(e & ~kSignBit) * 28);
+    Affine28 p = load_affine28(pts + (uint64_t)(e & ~kSignBit) * kRow28);
     if (e & kSignBit) p.y = fp28_neg(p.y);
     return p;
 }
 
 __global__ void __launch_bounds__(kBlock) k_accumulate28(const uint32_t *pts, const uint32_t *ent, const uint32_t *cnt,
                                                          const uint32_t *off, const uint32_t *q, const uint32_t *co,
-                                                         const uint32_t *cmap, uint32_t nchunks, uint32_t L,
+                                                         const uint32_t *cmap, const uint32_t *totals, uint32_t L,
                                                          uint32_t *part, uint32_t *bucket)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nchunks) return;
+    if (j >= totals[1]) return;
     const uint32_t b = cmap[j];
     const uint32_t r = j - co[b];
     const uint32_t start = off[b] + r * L;
@@ -355,51 +359,75 @@ __global__ void __launch_bounds__(kBlock) k_accumulate28(const uint32_t *pts, co
     else store_xyzz(part + (uint64_t)j * 48, out);
 }
 
-// buckets finished by the accumulate pass (one chunk) need no more work
-__global__ void k_retire_single(uint32_t *q, uint64_t nkeys, uint32_t *maxq)
-{
-    uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (b >= nkeys) return;
-    uint32_t n = q[b];
-    if (n == 1) q[b] = n = 0;
-    if (n > 1) atomicMax(maxq, n);
-}
-
-__global__ void k_group_counts(const uint32_t *q, uint32_t *q2, uint64_t nkeys, uint32_t G, uint32_t *maxq)
-{
-    uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (b >= nkeys) return;
-    uint32_t n = (q[b] + G - 1) / G;
-    q2[b] = n;
-    if (n > 1) atomicMax(maxq, n);
-}
-
-// one lane per group of up to G partials of one bucket (a level of the tree
-// that keeps very full buckets from becoming one long serial chain)
-__global__ void __launch_bounds__(kBlock) k_reduce_groups(const uint32_t *q, const uint32_t *co, const uint32_t *co2,
-                                                          const uint32_t *cmap2, uint32_t n2, uint32_t G,
-                                                          const uint32_t *part, uint32_t *part2)
+// Bucket reduction without read-backs.  After the accumulation, bucket b
+// holds q[b] partials at part[co[b] + r].  Levels l = 1, 2, ... (one lane
+// per chunk) shrink the buckets that still hold more than G partials to
+// ceil(q / G^l), summing groups of G into the other buffer; then one lane
+// per bucket sums the at most G partials left.  The number of levels is
+// fixed by the fullest bucket the batch could have, so nothing is read back
+// to the host; a level with no bucket left to shrink only reads indices.
+__global__ void __launch_bounds__(kBlock) k_tree_level(const uint32_t *q, const uint32_t *co, const uint32_t *cmap,
+                                                       const uint32_t *totals, uint64_t div_prev, uint32_t G,
+                                                       const uint32_t *in, uint32_t *out)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n2) return;
-    const uint32_t b = cmap2[j];
-    const uint32_t r = j - co2[b];
-    const uint32_t s0 = co[b] + r * G, s1 = min(s0 + G, co[b] + q[b]);
+    if (j >= totals[1]) return;
+    const uint32_t b = cmap[j];
+    const uint32_t r = j - co[b];
+    const uint32_t qprev = (uint32_t)(((uint64_t)q[b] + div_prev - 1) / div_prev);
+    if (qprev <= G) return;   // left for the per-bucket pass
+    const uint32_t qcur = (qprev + G - 1) / G;
+    if (r >= qcur) return;
+    const uint32_t s0 = r * G, s1 = min(s0 + G, qprev);
+    const uint64_t base = co[b];
     Xyzz acc = Xyzz::inf();
-    for (uint32_t x = s0; x < s1; ++x) acc = xyzz_add(acc, load_xyzz(part + (uint64_t)x * 48));
-    store_xyzz(part2 + (uint64_t)j * 48, acc);
+    for (uint32_t x = s0; x < s1; ++x) acc = xyzz_add(acc, load_xyzz(in + (base + x) * 48));
+    store_xyzz(out + (base + r) * 48, acc);
 }
 
-__global__ void __launch_bounds__(kBlock) k_bucket_sums(const uint32_t *q, const uint32_t *co, const uint32_t *part,
-                                                        uint32_t *bucket, uint64_t nkeys)
+// Bucket b after the levels above: the levels that shrank it are 1..lv, so
+// its partials are in `part` (lv even) or `part2` (lv odd).  Buckets of one
+// chunk were written by the accumulation.
+__global__ void __launch_bounds__(kBlock) k_tree_final(const uint32_t *q, const uint32_t *co, uint64_t nkeys,
+                                                       uint32_t G, const uint32_t *part, const uint32_t *part2,
+                                                       uint32_t *bucket)
 {
-    uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (b >= nkeys) return;
-    uint32_t n = q[b];
-    if (n == 0) return;   // empty, or finished by the accumulate pass
+    uint32_t rem = q[b];
+    if (rem <= 1) return;
+    uint32_t lv = 0;
+    while (rem > G) {
+        rem = (rem + G - 1) / G;
+        ++lv;
+    }
+    const uint32_t *src = (lv & 1) ? part2 : part;
+    const uint64_t base = co[b];
     Xyzz acc = Xyzz::inf();
-    for (uint32_t j = 0; j < n; ++j) acc = xyzz_add(acc, load_xyzz(part + (uint64_t)(co[b] + j) * 48));
+    for (uint32_t x = 0; x < rem; ++x) acc = xyzz_add(acc, load_xyzz(src + (base + x) * 48));
     store_xyzz(bucket + b * 48, acc);
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// totals[0] = bucket entries, totals[1] = chunks (both scans are exclusive)
+__global__ void k_totals(const uint32_t *cnt, const uint32_t *off, const uint32_t *q, const uint32_t *co,
+                         uint64_t nkeys, uint32_t *totals)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        totals[0] = off[nkeys - 1] + cnt[nkeys - 1];
+        totals[1] = co[nkeys - 1] + q[nkeys - 1];
+    }
 }
 
 // ---------------------------------------------------------------- bit planes
@@ -415,8 +443,8 @@ struct PlaneLayout {
     uint32_t members[kMaxWindow];        // members of plane k
 };
 
-__global__ void __launch_bounds__(kBlock) k_plane_partials(const uint32_t *bucket, Geometry g, PlaneLayout pl,
-                                                           uint32_t G1, uint64_t total, uint32_t *pp)
+__global__ void __launch_bounds__(kBlock) k_plane_partials(const uint32_t *bucket, const uint32_t *cnt, Geometry g,
+                                                           PlaneLayout pl, uint32_t G1, uint64_t total, uint32_t *pp)
 {
     uint64_t id = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (id >= total) return;
@@ -431,7 +459,8 @@ __global__ void __launch_bounds__(kBlock) k_plane_partials(const uint32_t *bucke
     Xyzz acc = Xyzz::inf();
     for (uint32_t e = e0; e < e1; ++e) {
         uint32_t x = ((e >> k) << (k + 1)) | (1u << k) | (e & lowmask);
-        acc = xyzz_add(acc, load_xyzz(bucket + ((uint64_t)(x - 1) * stride + mw) * 48));
+        const uint64_t key = (uint64_t)(x - 1) * stride + mw;
+        if (cnt[key]) acc = xyzz_add(acc, load_xyzz(bucket + key * 48));   // empty buckets are not written
     }
     store_xyzz(pp + id * 48, acc);
 }
@@ -467,11 +496,11 @@ __global__ void __launch_bounds__(kBlock) k_precompute(const uint32_t *in, uint3
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Affine a = load_affine(in + (uint64_t)i * 24);
-    store_affine28(out + (uint64_t)i * 28, affine28_from_std(a));
+    store_affine28(out + (uint64_t)i * kRow28, affine28_from_std(a));
     Xyzz x = xyzz_from_affine(a);
     for (uint32_t w = 1; w < nwin; ++w) {
         for (uint32_t k = 0; k < c; ++k) x = xyzz_dbl(x);
-        store_affine28(out + ((uint64_t)w * n + i) * 28, affine28_from_std(xyzz_to_affine(x)));
+        store_affine28(out + ((uint64_t)w * n + i) * kRow28, affine28_from_std(xyzz_to_affine(x)));
     }
 }
 
@@ -503,7 +532,7 @@ void build_precomp(MsmContext &ctx, hipStream_t st)
     uint32_t c = (uint32_t)std::min(16, std::max(10, lg - 4));
     uint32_t nwin = (256 + c - 1) / c;
     if ((uint64_t)nwin * n >= (1ull << 31)) return;   // entry indices are 31-bit
-    ctx.table.alloc((size_t)nwin * n * 28);   // Affine28 rows
+    ctx.table.alloc((size_t)nwin * n * kRow28);   // Affine28 rows
     hipLaunchKernelGGL(k_precompute, dim3(grid1(n)), dim3(kBlock), 0, st, ctx.points.get(), ctx.table.get(),
                        (uint32_t)n, c, nwin);
     MH_HIP(hipGetLastError());
@@ -515,23 +544,46 @@ void build_precomp(MsmContext &ctx, hipStream_t st)
 // ---------------------------------------------------------------- engine
 
 struct MsmEngine::Buffers {
-    DevBuf<uint32_t> cnt, off, cursor, q, co, cmap, ent, scan_tmp;
-    DevBuf<uint32_t> q2, co2, maxq, part2;
-    DevBuf<uint32_t> part, bucket, pp, pp2, pj;
+    DevBuf<uint32_t> cnt, off, cursor, q, co, cmap, ent, scan_tmp, totals;
+    DevBuf<uint32_t> part, part2, bucket, pp, pp2, pj;
     DevBuf<uint32_t> seg_off, seg_len, out_off;
     DevBuf<MsmDesc> descs;
-    DevBuf<Fr> masked;
+    // pinned host copies of the (window, plane) sums and the totals
+    void *h_pj = nullptr;
+    size_t h_pj_cap = 0;
+    uint32_t *h_totals = nullptr;
+    ~Buffers()
+    {
+        if (h_pj) (void)hipHostFree(h_pj);
+        if (h_totals) (void)hipHostFree(h_totals);
+    }
 };
 
-MsmEngine::MsmEngine() : buf_(new Buffers) {}
+// What prepare() decided for the next enqueue() / finish().
+struct MsmEngine::State {
+    struct Level {
+        uint64_t base, total_out;
+        uint32_t G;
+    };
+    Geometry g{};
+    PlaneLayout pl{};
+    const uint32_t *d_points = nullptr;
+    uint64_t max_n = 0, nkeys = 0, nent_max = 0, nch_max = 0, npp = 0, nseg = 0;
+    uint32_t L = 0, G = 0, tree_levels = 0, G1 = 0;
+    bool precomputed = false, ready = false;
+    std::vector<Level> levels;
+    std::vector<MsmDesc> hd;
+    std::vector<uint32_t> seg_off, seg_len, out_off;
+};
+
+MsmEngine::MsmEngine() : buf_(new Buffers), st_(new State) {}
 MsmEngine::~MsmEngine() = default;
 
 namespace {
 
-// exclusive scan of n u32 values (device), returns the total
-uint64_t scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t n, DevBuf<uint32_t> &tmp, hipStream_t st)
+// levels of tile sums of an exclusive scan of n values
+std::vector<uint64_t> scan_levels(uint64_t n)
 {
-    // levels of tile sums
     std::vector<uint64_t> sizes;
     uint64_t m = n;
     while (true) {
@@ -539,46 +591,48 @@ uint64_t scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t n, DevBuf<ui
         if (m <= kScanTile) break;
         m = ceil_div(m, kScanTile);
     }
-    // tmp layout: for each level >= 1, sums array and its scan
+    return sizes;
+}
+
+// scratch elements scan_launch needs for n values
+uint64_t scan_tmp_elems(uint64_t n)
+{
+    const auto sizes = scan_levels(n);
     uint64_t need = 0;
     for (size_t l = 1; l < sizes.size(); ++l) need += 2 * sizes[l];
-    need += 2;
-    tmp.ensure(need);
+    return need + 2;
+}
+
+// exclusive scan of n u32 values on the device; kernels only (no
+// allocation, no read-back), tmp holds scan_tmp_elems(n)
+void scan_launch(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp, hipStream_t st)
+{
+    const auto sizes = scan_levels(n);
     std::vector<uint32_t *> sums(sizes.size(), nullptr), sscan(sizes.size(), nullptr);
     uint64_t o = 0;
     for (size_t l = 1; l < sizes.size(); ++l) {
-        sums[l] = tmp.get() + o;
+        sums[l] = tmp + o;
         o += sizes[l];
-        sscan[l] = tmp.get() + o;
+        sscan[l] = tmp + o;
         o += sizes[l];
     }
-    uint32_t *total_dev = tmp.get() + o;
+    uint32_t *total_dev = tmp + o;
     // up-sweep
-    const uint32_t *src = in;
-    uint32_t *dst = out;
     for (size_t l = 0; l < sizes.size(); ++l) {
-        uint64_t tiles = ceil_div(sizes[l], kScanTile);
-        uint32_t *s = l + 1 < sizes.size() ? sums[l + 1] : total_dev;
-        hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)tiles), dim3(kBlock), 0, st, src, dst, s, sizes[l]);
+        const uint32_t *src = l == 0 ? in : sums[l];
+        uint32_t *dst = l == 0 ? out : sscan[l];
+        uint32_t *next = l + 1 < sizes.size() ? sums[l + 1] : total_dev;
+        const uint64_t tiles = ceil_div(sizes[l], kScanTile);
+        hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)tiles), dim3(kBlock), 0, st, src, dst, next, sizes[l]);
         MH_HIP(hipGetLastError());
-        if (l + 1 < sizes.size()) {
-            src = sums[l + 1];
-            dst = sscan[l + 1];
-        }
     }
-    // down-sweep
+    // down-sweep: add the scanned tile sums of level l+1 into level l
     for (size_t l = sizes.size() - 1; l-- > 0;) {
-        uint64_t tiles = ceil_div(sizes[l], kScanTile);
+        const uint64_t tiles = ceil_div(sizes[l], kScanTile);
         uint32_t *target = l == 0 ? out : sscan[l];
         hipLaunchKernelGGL(k_scan_add, dim3((uint32_t)tiles), dim3(kBlock), 0, st, target, sscan[l + 1], sizes[l]);
         MH_HIP(hipGetLastError());
     }
-    // total = last exclusive value + last input
-    uint32_t last_out = 0, last_in = 0;
-    MH_HIP(hipMemcpyAsync(&last_out, out + n - 1, 4, hipMemcpyDeviceToHost, st));
-    MH_HIP(hipMemcpyAsync(&last_in, in + n - 1, 4, hipMemcpyDeviceToHost, st));
-    MH_HIP(hipStreamSynchronize(st));
-    return (uint64_t)last_out + last_in;
 }
 
 uint32_t auto_window(uint64_t n)
@@ -595,18 +649,21 @@ uint32_t auto_window(uint64_t n)
 
 }  // namespace
 
-void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::vector<MsmJob> &jobs,
-                    uint32_t window_bits, RawG1Projective *out, hipStream_t st, const PrecompTable *pc)
+void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const std::vector<MsmJob> &jobs,
+                        uint32_t window_bits, hipStream_t st, const PrecompTable *pc)
 {
-    if (jobs.empty()) return;
+    MH_CHECK(!jobs.empty(), (int)hipErrorInvalidValue, "empty MSM batch");
     Buffers &b = *buf_;
+    State &s = *st_;
+    s.ready = false;
     uint64_t max_n = 0, total_pairs = 0;
     for (const auto &j : jobs) {
         MH_CHECK(j.pt_off + j.n <= npoints_total, (int)hipErrorInvalidValue, "MSM range exceeds the point set");
         max_n = std::max<uint64_t>(max_n, j.n);
         total_pairs += j.n;
     }
-    Geometry g;
+    Geometry &g = s.g;
+    s.precomputed = pc != nullptr;
     if (pc) {
         // one bucket set per MSM: window w reads the points [2^(c w)] P_i
         g.c = pc->c;
@@ -620,188 +677,201 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
         g.kw = g.nwin;
         g.pstride = 0;
     }
+    MH_CHECK(g.c >= 8 && g.c <= kMaxWindow, (int)hipErrorInvalidValue, "unsupported MSM window width");
     g.B = 1u << (g.c - 1);
     g.M = (uint32_t)jobs.size();
+    s.d_points = d_points;
+    s.max_n = max_n;
     last_window = g.c;
-    MH_CHECK((uint64_t)g.nwin * total_pairs < (1ull << 32), (int)hipErrorInvalidValue, "MSM batch too large");
-    const uint64_t nkeys = g.keys();
-    ProfileScope prof;
-    prof.begin("msm_total", st);
+    // every scalar gives at most nwin nonzero digits (the top window has a
+    // spare bit, so no carry leaves it)
+    s.nkeys = g.keys();
+    s.nent_max = (uint64_t)g.nwin * total_pairs;
+    MH_CHECK(s.nent_max < (1ull << 32) && s.nkeys < (1ull << 31), (int)hipErrorInvalidValue, "MSM batch too large");
+    s.L = fill_group(s.nent_max, chunk_len);
+    s.nch_max = ceil_div(s.nent_max, s.L) + s.nkeys;
+    MH_CHECK(s.nch_max < (1ull << 32), (int)hipErrorInvalidValue, "MSM batch too large");
+    s.G = s.nch_max >= 4 * kFillLanes ? 16 : 4;
+    // fullest possible bucket: every digit of every scalar of the longest
+    // MSM in one bucket (one bucket set spans all windows when precomputed)
+    const uint64_t bucket_max = pc ? max_n * g.nwin : max_n;
+    const uint64_t q_worst = ceil_div(bucket_max, s.L);
+    s.tree_levels = 0;   // levels until every bucket holds at most G partials
+    for (uint64_t d = s.G; d < q_worst; d *= s.G) ++s.tree_levels;
 
-    // descriptors
-    std::vector<MsmDesc> hd(jobs.size());
-    for (size_t i = 0; i < jobs.size(); ++i) hd[i] = MsmDesc{jobs[i].scalars, (uint32_t)jobs[i].pt_off, (uint32_t)jobs[i].n};
-    b.descs.ensure(hd.size());
-    MH_HIP(hipMemcpyAsync(b.descs.get(), hd.data(), hd.size() * sizeof(MsmDesc), hipMemcpyHostToDevice, st));
+    b.descs.ensure(jobs.size());
+    b.cnt.ensure(s.nkeys);
+    b.off.ensure(s.nkeys);
+    b.cursor.ensure(s.nkeys);
+    b.q.ensure(s.nkeys);
+    b.co.ensure(s.nkeys);
+    b.scan_tmp.ensure(scan_tmp_elems(s.nkeys));
+    b.totals.ensure(4);
+    b.ent.ensure(std::max<uint64_t>(s.nent_max, 1));
+    b.cmap.ensure(s.nch_max);
+    b.part.ensure(s.nch_max * 48);
+    if (s.tree_levels > 0) b.part2.ensure(s.nch_max * 48);
+    b.bucket.ensure(s.nkeys * 48);
 
-    // 1-3: histogram, offsets, scatter
-    b.cnt.ensure(nkeys);
-    b.off.ensure(nkeys);
-    b.cursor.ensure(nkeys);
-    MH_HIP(hipMemsetAsync(b.cnt.get(), 0, nkeys * 4, st));
-    dim3 dgrid(grid1(max_n), g.M);
-    if (max_n) {
-        launch_digits(false, g.c, max_n, st, b.descs.get(), g, b.cnt.get(), nullptr);
-    }
-    const uint64_t nent = scan_exclusive(b.cnt.get(), b.off.get(), nkeys, b.scan_tmp, st);
-    b.ent.ensure(std::max<uint64_t>(nent, 1));
-    MH_HIP(hipMemcpyAsync(b.cursor.get(), b.off.get(), nkeys * 4, hipMemcpyDeviceToDevice, st));
-    if (max_n) {
-        launch_digits(true, g.c, max_n, st, b.descs.get(), g, b.cursor.get(), b.ent.get());
-    }
-
-    // 4-6: chunked accumulation into buckets
-    const uint32_t L = fill_group(nent, chunk_len);
-    b.q.ensure(nkeys);
-    b.co.ensure(nkeys);
-    hipLaunchKernelGGL(k_chunk_counts, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.cnt.get(), b.q.get(), nkeys, L);
-    MH_HIP(hipGetLastError());
-    const uint64_t nchunks = scan_exclusive(b.q.get(), b.co.get(), nkeys, b.scan_tmp, st);
-    b.bucket.ensure(nkeys * 48);
-    // empty buckets must read as the identity (zz = 0)
-    MH_HIP(hipMemsetAsync(b.bucket.get(), 0, nkeys * 48 * 4, st));
-    if (nchunks) {
-        b.cmap.ensure(nchunks);
-        b.part.ensure(nchunks * 48);
-        hipLaunchKernelGGL(k_chunk_map, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), b.cmap.get(),
-                           nkeys);
-        MH_HIP(hipGetLastError());
-        prof.begin("msm_accumulate", st);
-        if (pc)
-            hipLaunchKernelGGL(k_accumulate28, dim3(grid1(nchunks)), dim3(kBlock), 0, st, d_points, b.ent.get(),
-                               b.cnt.get(), b.off.get(), b.q.get(), b.co.get(), b.cmap.get(), (uint32_t)nchunks, L,
-                               b.part.get(), b.bucket.get());
-        else
-            hipLaunchKernelGGL(k_accumulate, dim3(grid1(nchunks)), dim3(kBlock), 0, st, d_points, b.ent.get(),
-                               b.cnt.get(), b.off.get(), b.q.get(), b.co.get(), b.cmap.get(), (uint32_t)nchunks, L,
-                               b.part.get(), b.bucket.get());
-        MH_HIP(hipGetLastError());
-        prof.end(st);
-        last_entries = nent;
-        // multi-chunk buckets: tree levels of `kGroup` partials while any
-        // bucket still has more than kGroup partials
-        const uint32_t kGroup = nchunks >= 4 * kFillLanes ? 16 : 4;
-        b.maxq.ensure(1);
-        MH_HIP(hipMemsetAsync(b.maxq.get(), 0, 4, st));
-        hipLaunchKernelGGL(k_retire_single, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), nkeys, b.maxq.get());
-        MH_HIP(hipGetLastError());
-        uint32_t maxq = 0;
-        MH_HIP(hipMemcpyAsync(&maxq, b.maxq.get(), 4, hipMemcpyDeviceToHost, st));
-        MH_HIP(hipStreamSynchronize(st));
-        uint32_t *q = b.q.get(), *co = b.co.get(), *part = b.part.get();
-        while (maxq > kGroup) {
-            b.q2.ensure(nkeys);
-            b.co2.ensure(nkeys);
-            MH_HIP(hipMemsetAsync(b.maxq.get(), 0, 4, st));
-            hipLaunchKernelGGL(k_group_counts, dim3(grid1(nkeys)), dim3(kBlock), 0, st, q, b.q2.get(), nkeys, kGroup,
-                               b.maxq.get());
-            MH_HIP(hipGetLastError());
-            const uint64_t n2 = scan_exclusive(b.q2.get(), b.co2.get(), nkeys, b.scan_tmp, st);
-            b.cmap.ensure(n2);
-            b.part2.ensure(n2 * 48);
-            hipLaunchKernelGGL(k_chunk_map, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q2.get(), b.co2.get(),
-                               b.cmap.get(), nkeys);
-            MH_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_reduce_groups, dim3(grid1(n2)), dim3(kBlock), 0, st, q, co, b.co2.get(), b.cmap.get(),
-                               (uint32_t)n2, kGroup, part, b.part2.get());
-            MH_HIP(hipGetLastError());
-            MH_HIP(hipMemcpyAsync(&maxq, b.maxq.get(), 4, hipMemcpyDeviceToHost, st));
-            MH_HIP(hipStreamSynchronize(st));
-            // the reduced level becomes the input of the next
-            std::swap(b.q, b.q2);
-            std::swap(b.co, b.co2);
-            std::swap(b.part, b.part2);
-            q = b.q.get();
-            co = b.co.get();
-            part = b.part.get();
-        }
-        hipLaunchKernelGGL(k_bucket_sums, dim3(grid1(nkeys)), dim3(kBlock), 0, st, q, co, part, b.bucket.get(), nkeys);
-        MH_HIP(hipGetLastError());
-    }
-
-    // 7: bit-plane partial sums per (msm, window, plane)
-    MH_CHECK(g.c >= 1 && g.c <= kMaxWindow, (int)hipErrorInvalidValue, "unsupported MSM window width");
+    // bit planes: S = sum_k 2^k Q_k, Q_k = sum of the buckets x with bit k of x
     const uint64_t nmw = (uint64_t)g.M * g.kw;
-    const uint32_t G1 = fill_group(nmw * ((uint64_t)(g.c - 1) * (g.B / 2) + 1), plane_group);
-    PlaneLayout pl{};
+    s.G1 = fill_group(nmw * ((uint64_t)(g.c - 1) * (g.B / 2) + 1), plane_group);
+    PlaneLayout &pl = s.pl;
+    pl = PlaneLayout{};
     pl.c = g.c;
     uint32_t acc = 0;
     for (uint32_t k = 0; k < g.c; ++k) {
         pl.members[k] = k + 1 < g.c ? g.B / 2 : 1;
         pl.first[k] = acc;
-        acc += (pl.members[k] + G1 - 1) / G1;
+        acc += (pl.members[k] + s.G1 - 1) / s.G1;
     }
     pl.first[g.c] = acc;
     for (uint32_t k = g.c + 1; k <= kMaxWindow; ++k) pl.first[k] = 0xffffffffu;
     pl.per_window = acc;
-    const uint64_t npp = nmw * pl.per_window;
-    b.pp.ensure(npp * 48);
-    hipLaunchKernelGGL(k_plane_partials, dim3(grid1(npp)), dim3(kBlock), 0, st, b.bucket.get(), g, pl, G1, npp,
-                       b.pp.get());
-    MH_HIP(hipGetLastError());
+    s.npp = nmw * pl.per_window;
+    b.pp.ensure(s.npp * 48);
 
     // group passes within each segment (msm, window, plane) until one
     // partial per segment is left; the last pass converts to Jacobian
-    const uint64_t nseg = nmw * g.c;
-    struct Level {
-        uint64_t base, total_out;
-        uint32_t G;
-    };
-    std::vector<Level> levels;
-    std::vector<uint32_t> seg_off, seg_len, out_off;   // all levels, concatenated
-    std::vector<uint32_t> offs(nseg), lens(nseg);
-    for (uint64_t s = 0; s < nseg; ++s) {
-        const uint32_t k = (uint32_t)(s % g.c);
-        offs[s] = (uint32_t)((s / g.c) * pl.per_window + pl.first[k]);
-        lens[s] = pl.first[k + 1] - pl.first[k];
+    s.nseg = nmw * g.c;
+    s.levels.clear();
+    s.seg_off.clear();
+    s.seg_len.clear();
+    s.out_off.clear();
+    std::vector<uint32_t> offs(s.nseg), lens(s.nseg);
+    for (uint64_t x = 0; x < s.nseg; ++x) {
+        const uint32_t k = (uint32_t)(x % g.c);
+        offs[x] = (uint32_t)((x / g.c) * pl.per_window + pl.first[k]);
+        lens[x] = pl.first[k + 1] - pl.first[k];
     }
-    uint64_t in_total = npp;
+    uint64_t in_total = s.npp;
     for (;;) {
         const uint32_t maxlen = *std::max_element(lens.begin(), lens.end());
         const bool last = maxlen <= plane_group2;
         const uint32_t G = last ? maxlen : (in_total >= (uint64_t)plane_group2 * kFillLanes / 8 ? plane_group2 : 4);
-        Level lv{seg_off.size(), 0, G};
-        for (uint64_t s = 0; s < nseg; ++s) {
-            seg_off.push_back(offs[s]);
-            seg_len.push_back(lens[s]);
-            out_off.push_back((uint32_t)lv.total_out);
-            offs[s] = (uint32_t)lv.total_out;
-            lens[s] = (lens[s] + G - 1) / G;
-            lv.total_out += lens[s];
+        State::Level lv{s.seg_off.size(), 0, G};
+        for (uint64_t x = 0; x < s.nseg; ++x) {
+            s.seg_off.push_back(offs[x]);
+            s.seg_len.push_back(lens[x]);
+            s.out_off.push_back((uint32_t)lv.total_out);
+            offs[x] = (uint32_t)lv.total_out;
+            lens[x] = (lens[x] + G - 1) / G;
+            lv.total_out += lens[x];
         }
-        levels.push_back(lv);
+        s.levels.push_back(lv);
         in_total = lv.total_out;
         if (last) break;
     }
-    b.seg_off.ensure(seg_off.size());
-    b.seg_len.ensure(seg_len.size());
-    b.out_off.ensure(out_off.size());
-    MH_HIP(hipMemcpyAsync(b.seg_off.get(), seg_off.data(), seg_off.size() * 4, hipMemcpyHostToDevice, st));
-    MH_HIP(hipMemcpyAsync(b.seg_len.get(), seg_len.data(), seg_len.size() * 4, hipMemcpyHostToDevice, st));
-    MH_HIP(hipMemcpyAsync(b.out_off.get(), out_off.data(), out_off.size() * 4, hipMemcpyHostToDevice, st));
-    b.pp2.ensure(std::max<uint64_t>(levels[0].total_out, 1) * 48);
-    b.pj.ensure(nseg * 36);
+    b.seg_off.ensure(s.seg_off.size());
+    b.seg_len.ensure(s.seg_len.size());
+    b.out_off.ensure(s.out_off.size());
+    b.pp2.ensure(std::max<uint64_t>(s.levels[0].total_out, 1) * 48);
+    b.pj.ensure(s.nseg * 36);
+
+    // uploads of the descriptors and the plane-pass layout
+    s.hd.resize(jobs.size());
+    for (size_t i = 0; i < jobs.size(); ++i)
+        s.hd[i] = MsmDesc{jobs[i].scalars, (uint32_t)jobs[i].pt_off, (uint32_t)jobs[i].n};
+    MH_HIP(hipMemcpyAsync(b.descs.get(), s.hd.data(), s.hd.size() * sizeof(MsmDesc), hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(b.seg_off.get(), s.seg_off.data(), s.seg_off.size() * 4, hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(b.seg_len.get(), s.seg_len.data(), s.seg_len.size() * 4, hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(b.out_off.get(), s.out_off.data(), s.out_off.size() * 4, hipMemcpyHostToDevice, st));
+    static_assert(sizeof(HJac) == 144, "HJac layout");
+    if (b.h_pj_cap < s.nseg) {
+        if (b.h_pj) MH_HIP(hipHostFree(b.h_pj));
+        b.h_pj = nullptr;
+        b.h_pj_cap = 0;
+        MH_HIP(hipHostMalloc(&b.h_pj, s.nseg * 144));
+        b.h_pj_cap = s.nseg;
+    }
+    if (!b.h_totals) MH_HIP(hipHostMalloc(reinterpret_cast<void **>(&b.h_totals), 16));
+    s.ready = true;
+}
+
+void MsmEngine::enqueue(hipStream_t st, ProfileScope *prof)
+{
+    Buffers &b = *buf_;
+    State &s = *st_;
+    MH_CHECK(s.ready, -1, "MSM engine used before prepare");
+    const Geometry &g = s.g;
+    const uint64_t nkeys = s.nkeys;
+    if (prof) prof->begin("msm_total", st);
+
+    // 1-3: histogram, offsets, scatter
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)std::min<uint64_t>(grid1(nkeys), 4096)), dim3(kBlock), 0, st,
+                       b.cnt.get(), nkeys, 0u);
+    MH_HIP(hipGetLastError());
+    if (s.max_n) launch_digits(false, g.c, s.max_n, st, b.descs.get(), g, b.cnt.get(), nullptr);
+    scan_launch(b.cnt.get(), b.off.get(), nkeys, b.scan_tmp.get(), st);
+    hipLaunchKernelGGL(k_copy_u32, dim3((uint32_t)std::min<uint64_t>(grid1(nkeys), 4096)), dim3(kBlock), 0, st,
+                       b.cursor.get(), (const uint32_t *)b.off.get(), nkeys);
+    MH_HIP(hipGetLastError());
+    if (s.max_n) launch_digits(true, g.c, s.max_n, st, b.descs.get(), g, b.cursor.get(), b.ent.get());
+
+    // 4-6: chunked accumulation into buckets, then the bucket tree
+    hipLaunchKernelGGL(k_chunk_counts, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.cnt.get(), b.q.get(), nkeys, s.L);
+    MH_HIP(hipGetLastError());
+    scan_launch(b.q.get(), b.co.get(), nkeys, b.scan_tmp.get(), st);
+    hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, st, b.cnt.get(), b.off.get(), b.q.get(), b.co.get(), nkeys,
+                       b.totals.get());
+    MH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_chunk_map, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), b.cmap.get(),
+                       nkeys);
+    MH_HIP(hipGetLastError());
+    if (prof) prof->begin("msm_accumulate", st);
+    if (s.precomputed)
+        hipLaunchKernelGGL(k_accumulate28, dim3(grid1(s.nch_max)), dim3(kBlock), 0, st, s.d_points, b.ent.get(),
+                           b.cnt.get(), b.off.get(), b.q.get(), b.co.get(), b.cmap.get(), b.totals.get(), s.L,
+                           b.part.get(), b.bucket.get());
+    else
+        hipLaunchKernelGGL(k_accumulate, dim3(grid1(s.nch_max)), dim3(kBlock), 0, st, s.d_points, b.ent.get(),
+                           b.cnt.get(), b.off.get(), b.q.get(), b.co.get(), b.cmap.get(), b.totals.get(), s.L,
+                           b.part.get(), b.bucket.get());
+    MH_HIP(hipGetLastError());
+    if (prof) prof->end(st);
+    uint32_t *tin = b.part.get(), *tout = b.part2.get();
+    uint64_t div = 1;
+    for (uint32_t l = 0; l < s.tree_levels; ++l) {
+        hipLaunchKernelGGL(k_tree_level, dim3(grid1(s.nch_max)), dim3(kBlock), 0, st, b.q.get(), b.co.get(),
+                           b.cmap.get(), b.totals.get(), div, s.G, (const uint32_t *)tin, tout);
+        MH_HIP(hipGetLastError());
+        div *= s.G;
+        std::swap(tin, tout);
+    }
+    hipLaunchKernelGGL(k_tree_final, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), nkeys, s.G,
+                       (const uint32_t *)b.part.get(), (const uint32_t *)b.part2.get(), b.bucket.get());
+    MH_HIP(hipGetLastError());
+
+    // 7: bit-plane partial sums and group passes down to one per segment
+    hipLaunchKernelGGL(k_plane_partials, dim3(grid1(s.npp)), dim3(kBlock), 0, st, b.bucket.get(), b.cnt.get(), g, s.pl,
+                       s.G1, s.npp, b.pp.get());
+    MH_HIP(hipGetLastError());
     uint32_t *lin = b.pp.get(), *lout = b.pp2.get();
-    for (size_t i = 0; i < levels.size(); ++i) {
-        const Level &lv = levels[i];
-        const bool last = i + 1 == levels.size();
-        hipLaunchKernelGGL(k_group_sums, dim3(grid1(lv.total_out)), dim3(kBlock), 0, st, lin, nseg,
+    for (size_t i = 0; i < s.levels.size(); ++i) {
+        const State::Level &lv = s.levels[i];
+        const bool last = i + 1 == s.levels.size();
+        hipLaunchKernelGGL(k_group_sums, dim3(grid1(lv.total_out)), dim3(kBlock), 0, st, lin, s.nseg,
                            b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base, lv.G,
                            lv.total_out, last ? b.pj.get() : lout, last ? 1 : 0);
         MH_HIP(hipGetLastError());
         std::swap(lin, lout);
     }
+    MH_HIP(hipMemcpyAsync(b.h_pj, b.pj.get(), s.nseg * 144, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipMemcpyAsync(b.h_totals, b.totals.get(), 8, hipMemcpyDeviceToHost, st));
+    if (prof) prof->end(st);
+}
 
-    // 8: host: one doubling chain per MSM over its (window, plane) sums
-    std::vector<HJac> hj(nseg);
-    static_assert(sizeof(HJac) == 144, "HJac layout");
-    MH_HIP(hipMemcpyAsync(hj.data(), b.pj.get(), nseg * 144, hipMemcpyDeviceToHost, st));
-    prof.end(st);
-    MH_HIP(hipStreamSynchronize(st));
-    prof.flush();
-
-    auto finish = [&](uint32_t m) {
-        // exponent e = c*w + k  ->  plane sum
+void MsmEngine::finish(RawG1Projective *out)
+{
+    const Buffers &b = *buf_;
+    const State &s = *st_;
+    MH_CHECK(s.ready, -1, "MSM engine used before prepare");
+    const Geometry &g = s.g;
+    last_entries = b.h_totals[0];
+    const HJac *hj = static_cast<const HJac *>(b.h_pj);
+    // 8: host: one doubling chain per MSM over its (window, plane) sums,
+    // exponent e = c*w + k
+    auto chain = [&](uint32_t m) {
         const uint32_t E = g.kw * g.c;
         HJac accj = HJac::identity();
         for (uint32_t e = E; e-- > 0;) {
@@ -813,15 +883,27 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
     };
     const uint32_t nthreads = std::min<uint32_t>(g.M, 16);
     if (nthreads <= 1) {
-        for (uint32_t m = 0; m < g.M; ++m) finish(m);
+        for (uint32_t m = 0; m < g.M; ++m) chain(m);
     } else {
         std::vector<std::thread> th;
         for (uint32_t t = 0; t < nthreads; ++t)
             th.emplace_back([&, t]() {
-                for (uint32_t m = t; m < g.M; m += nthreads) finish(m);
+                for (uint32_t m = t; m < g.M; m += nthreads) chain(m);
             });
         for (auto &x : th) x.join();
     }
+}
+
+void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::vector<MsmJob> &jobs,
+                    uint32_t window_bits, RawG1Projective *out, hipStream_t st, const PrecompTable *pc)
+{
+    if (jobs.empty()) return;
+    prepare(d_points, npoints_total, jobs, window_bits, st, pc);
+    ProfileScope prof;
+    enqueue(st, profiling() ? &prof : nullptr);
+    MH_HIP(hipStreamSynchronize(st));
+    prof.flush();
+    finish(out);
 }
 
 }  // namespace mh

@@ -11,6 +11,8 @@
 
 namespace mh {
 
+class ProfileScope;
+
 // One MSM of a batch: n scalars (device, Montgomery) against points
 // [pt_off, pt_off + n) of the engine's point table.
 struct MsmJob {
@@ -37,6 +39,18 @@ public:
     void run(const uint32_t *d_points, size_t npoints_total, const std::vector<MsmJob> &jobs, uint32_t window_bits,
              RawG1Projective *out, hipStream_t st, const PrecompTable *pc = nullptr);
 
+    // run() in three parts, for callers that capture the device work in a
+    // graph: prepare() sizes the buffers for the worst case of the batch and
+    // uploads its descriptors (stream-ordered copies, no kernels); enqueue()
+    // only launches kernels and copies the (window, plane) sums to pinned
+    // host memory -- no allocation, no synchronisation, no memset -- and may
+    // be replayed; finish() runs the doubling chains once the stream has
+    // synchronised.  The buffers stay valid until the next prepare().
+    void prepare(const uint32_t *d_points, size_t npoints_total, const std::vector<MsmJob> &jobs,
+                 uint32_t window_bits, hipStream_t st, const PrecompTable *pc = nullptr);
+    void enqueue(hipStream_t st, ProfileScope *prof);
+    void finish(RawG1Projective *out);
+
     // Upper bounds of the serial work per lane.  A point addition takes the
     // same time per lane at any occupancy, so batches too small to fill the
     // GPU get shorter chunks and groups (down to 4): more lanes, a shorter
@@ -51,7 +65,9 @@ public:
 
 private:
     struct Buffers;
+    struct State;
     std::unique_ptr<Buffers> buf_;
+    std::unique_ptr<State> st_;
 };
 
 }  // namespace mh

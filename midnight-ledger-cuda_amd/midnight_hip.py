@@ -121,6 +121,7 @@ def _declare(L):
             E, [vp, ctypes.POINTER(c_size), ctypes.POINTER(u32), c_size, b]),
         "midnight_hip_valu_peak": (
             E, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
+        "midnight_hip_copy_bandwidth": (E, [c_size, ctypes.POINTER(ctypes.c_double)]),
         "midnight_hip_field_mul_rate": (
             E, [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "midnight_hip_fr_device_vec_to_montgomery": (E, [vp]),
@@ -482,6 +483,13 @@ def valu_peak():
     cus = ctypes.c_int()
     check(lib().midnight_hip_valu_peak(ctypes.byref(r), ctypes.byref(clk), ctypes.byref(cus)))
     return r.value, clk.value, cus.value
+
+
+def copy_bandwidth(nbytes):
+    """read + write bytes/s of one device copy of nbytes by a plain kernel"""
+    r = ctypes.c_double()
+    check(lib().midnight_hip_copy_bandwidth(nbytes, ctypes.byref(r)))
+    return r.value
 
 
 def field_mul_rate(blocks=0):
