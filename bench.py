@@ -238,8 +238,11 @@ def bench_batch(args, H):
     """256 independent instances, sizes 2^k with k log-uniform in [10, 18]
     (k = floor(10 + 9 u), u from a per-instance u64 seed): each does an
     iNTT of 2^k values and an MSM of 2^k points over a shared 2^18 point set.
-    A step runs all instances batched (one launch set per NTT size, one MSM
-    batch); the same work one instance at a time is timed beside it."""
+    A step is one run of a batch plan: the ragged iNTT (one launch per pass
+    for all instances) and one MSM batch, captured once in a HIP graph and
+    replayed, with the MSM configuration autotuned when the plan is made.
+    The same work without the graph (eager launches) and one instance at a
+    time is timed beside it."""
     count = 256
     ks = []
     for j in range(count):
@@ -260,7 +263,16 @@ def bench_batch(args, H):
     k_list = [ks[j] for j in order]
     n_list = [1 << k for k in k_list]
 
+    t0 = time.perf_counter()
+    plan = H.BatchPlan(ctx, dv, o_list, k_list, H.BatchPlan.GRAPH | H.BatchPlan.AUTOTUNE)
+    plan_s = time.perf_counter() - t0
+    info = plan.info()
+    log("batch plan: %s (made in %.2f s)" % (info, plan_s))
+
     def batched():
+        plan.run()
+
+    def eager():
         dv.ntt_multi(o_list, k_list, True)
         ctx.device_multi(dv, o_list, n_list)
 
@@ -275,17 +287,24 @@ def bench_batch(args, H):
     for _ in range(args.steps):
         batched()
     dt = (time.perf_counter() - t0) / args.steps
+    eager()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eager()
+    dt_eager = (time.perf_counter() - t0) / args.steps
     singles()
     t0 = time.perf_counter()
     singles()
     dt1 = time.perf_counter() - t0
+    plan.free()
     res = {
         "metric": "batched MSM+iNTT instances/s (256 instances, k in [10,18])",
         "value": round(count / dt, 2), "unit": "instances/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic uniform Fr, points [t_i]G",
         "config": {"workload": "256 x (iNTT 2^k + MSM 2^k), k = floor(10 + 9u) per instance seed, shared 2^18 points",
-                   "total_points": total},
+                   "total_points": total, "plan": info},
+        "eager_batched_ms": round(dt_eager * 1e3, 3),
         "one_at_a_time_ms": round(dt1 * 1e3, 3),
         "batched_speedup": round(dt1 / dt, 3),
     }

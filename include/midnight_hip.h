@@ -358,11 +358,36 @@ RustError midnight_hip_g1_msm_context_device_multi(RawG1Projective *out, void *c
                                                    const size_t *sizes, size_t count);
 
 /* Batched ragged in-place inverse/forward NTTs over one device vector:
- * transform j covers 2^log_ns[j] values at offsets[j].  For batch provers
+ * transform j covers 2^log_ns[j] values at offsets[j] (slices must not
+ * overlap).  One launch per NTT pass for the whole set.  For batch provers
  * (many circuits of different k). */
 RustError midnight_hip_fr_device_vec_ntt_multi(void *device_vec, const size_t *offsets,
                                                const uint32_t *log_ns, size_t count,
                                                bool inverse);
+
+/* Batch plan: instances i < count, each an inverse NTT of the 2^log_ns[i]
+ * values at offsets[i] of the device vector (in place) followed by the MSM
+ * of those coefficients against points [0, 2^log_ns[i]) of the context.
+ * A run does every instance: the ragged NTT (one launch per pass), then one
+ * batched MSM.  flags: 1 = capture the run's device work once in a HIP graph
+ * and replay it; 2 = autotune the MSM configuration (precomputed table with
+ * chunk lengths 64/32, plain windows of the automatic width +-1) by timing
+ * it on the batch at creation.  The reference picks MSM windows from a
+ * per-k table (Bls12_381MsmConfig::for_k, midnight-sppark-msm/src/lib.rs:
+ * 159-203) and runs instances one by one.  The plan refers to the context
+ * and the device vector, which must outlive it; runs on the creating lane's
+ * stream. */
+RustError midnight_hip_batch_plan_create(void **out, void *msm_context, void *device_vec,
+                                         const size_t *offsets, const uint32_t *log_ns, size_t count,
+                                         uint32_t flags);
+/* Runs the plan once; out[i] = instance i's MSM (Jacobian). */
+RustError midnight_hip_batch_plan_run(void *plan, RawG1Projective *out);
+/* The plan's MSM window, chunk length, whether it uses the precomputed
+ * table, whether it replays a graph, and the tuned MSM time (ms, 0 if not
+ * tuned). */
+RustError midnight_hip_batch_plan_info(const void *plan, uint32_t *window_bits, uint32_t *chunk_len,
+                                       uint32_t *precomputed, uint32_t *graph, float *tuned_ms);
+void midnight_hip_batch_plan_free(void *plan);
 
 /* In place: canonical Fr values -> Montgomery form (a -> a*R mod r).  For
  * callers that hold canonical scalars (synthetic workloads, wire formats). */

@@ -1,6 +1,7 @@
 // C ABI: lanes, device vectors, NTT / coset and Fr vector entry points.
 // Argument checks, return codes and messages follow the reference entry
 // points cited at each function (midnight-sppark-msm/src/bls12_381_g1_msm.cu).
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -493,39 +494,21 @@ extern "C" RustError midnight_hip_fr_device_vec_ntt_multi(void *device_vec, cons
         if (log_ns[i] > 22 || offsets[i] > dv->len || (size_t(1) << log_ns[i]) > dv->len - offsets[i])
             return make_error((int)hipErrorInvalidValue, "batched NTT slice exceeds the device vector");
     }
+    {
+        std::vector<std::pair<size_t, size_t>> spans(count);
+        for (size_t i = 0; i < count; ++i) spans[i] = {offsets[i], size_t(1) << log_ns[i]};
+        std::sort(spans.begin(), spans.end());
+        for (size_t i = 1; i < count; ++i)
+            if (spans[i].first < spans[i - 1].first + spans[i - 1].second)
+                return make_error((int)hipErrorInvalidValue, "batched NTT slices overlap");
+    }
     MH_TRY
     hipStream_t st = cur_stream();
-    // Runs of consecutive, back-to-back transforms of one size share their
-    // launches (one grid row per polynomial); callers that lay polynomials
-    // out by size get one launch set per distinct size.
-    struct Run {
-        size_t first, count;
-    };
-    std::vector<Run> runs;
-    size_t max_scratch = 0;
-    for (size_t i = 0; i < count;) {
-        size_t j = i + 1;
-        const size_t n = size_t(1) << log_ns[i];
-        while (j < count && j - i < 65535 && log_ns[j] == log_ns[i] && offsets[j] == offsets[j - 1] + n) ++j;
-        runs.push_back(Run{i, j - i});
-        max_scratch = std::max(max_scratch, ntt_scratch_elems(log_ns[i], j - i));
-        i = j;
-    }
-    LaneScratch &ls = lane_scratch();
-    std::lock_guard<std::mutex> g(ls.mu);
-    ls.c.ensure(max_scratch);
-    for (const Run &r : runs) {
-        const size_t n = size_t(1) << log_ns[r.first];
-        NttJob job;
-        job.in = dv->values.get() + offsets[r.first];
-        job.out = dv->values.get() + offsets[r.first];
-        job.scratch = ls.c.get();
-        job.in_pstride = job.out_pstride = n;
-        job.log_n = log_ns[r.first];
-        job.count = (uint32_t)r.count;
-        job.inverse = inverse;
-        ntt_run(job, st);
-    }
+    // one launch per pass for the whole set (runs of consecutive transforms
+    // of one size share a descriptor)
+    RaggedNtt set;
+    set.prepare(dv->values.get(), offsets, log_ns, count, inverse, st);
+    set.enqueue(st);
     MH_HIP(hipStreamSynchronize(st));
     return ok();
     MH_CATCH

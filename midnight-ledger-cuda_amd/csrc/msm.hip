@@ -649,6 +649,8 @@ uint32_t auto_window(uint64_t n)
 
 }  // namespace
 
+uint32_t msm_auto_window(uint64_t n) { return auto_window(n); }
+
 void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const std::vector<MsmJob> &jobs,
                         uint32_t window_bits, hipStream_t st, const PrecompTable *pc)
 {

@@ -30,6 +30,9 @@ struct PrecompTable {
     uint32_t c, nwin;
 };
 
+// window width the engine picks for plain windows over MSMs of n points
+uint32_t msm_auto_window(uint64_t n);
+
 class MsmEngine {
 public:
     MsmEngine();

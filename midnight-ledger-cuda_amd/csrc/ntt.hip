@@ -25,6 +25,9 @@
 //        (fr_mul_assign_repeating_kernel, :1213-1223)
 //   post scale(+zeta): a_i *= scale * {1, first, second}[i % 3]
 //        (fr_ntt_zeta_postprocess_kernel, :1680-1700)
+#include <algorithm>
+#include <cstring>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -113,15 +116,15 @@ __device__ __forceinline__ Fr zeta_factor(const Fr &v, uint64_t idx, const Fr &f
     return v;
 }
 
-__global__ void __launch_bounds__(kThreads) ntt_pass(PassArgs a)
+// one workgroup of a pass: columns [bx * C, bx * C + C) of polynomial by
+__device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32_t by, Fr *lds)
 {
-    extern __shared__ Fr lds[];
     const uint32_t m = 1u << a.log_m;
     const uint32_t C = a.C;
     const uint32_t total = C * m;
-    const uint32_t c0 = blockIdx.x * C;
-    const Fr *in = a.in + (uint64_t)blockIdx.y * a.in_pstride;
-    Fr *out = a.out + (uint64_t)blockIdx.y * a.out_pstride;
+    const uint32_t c0 = bx * C;
+    const Fr *in = a.in + (uint64_t)by * a.in_pstride;
+    Fr *out = a.out + (uint64_t)by * a.out_pstride;
 
     // load (bit-reversed within each column), applying the pre-operation
     const bool load_cfast = (a.in_cs == 1) && C > 1;
@@ -185,6 +188,34 @@ __global__ void __launch_bounds__(kThreads) ntt_pass(PassArgs a)
     }
 }
 
+__global__ void __launch_bounds__(kThreads) ntt_pass(PassArgs a)
+{
+    extern __shared__ Fr lds[];
+    pass_body(a, blockIdx.x, blockIdx.y, lds);
+}
+
+// A pass of every transform of a ragged set in one launch: descriptor d owns
+// blocks [first_block, first_block + gx * npolys) of the grid.
+struct RaggedDesc {
+    PassArgs a;
+    uint32_t gx, first_block;
+};
+
+__global__ void __launch_bounds__(kThreads) ntt_pass_ragged(const RaggedDesc *d, uint32_t nd)
+{
+    extern __shared__ Fr lds[];
+    uint32_t lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (d[mid].first_block <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t local = blockIdx.x - d[lo].first_block;
+    const uint32_t gx = d[lo].gx;
+    const PassArgs a = d[lo].a;
+    pass_body(a, local % gx, local / gx, lds);
+}
+
 uint32_t pick_cols(uint32_t log_m, uint32_t ncols)
 {
     uint32_t C = 1;
@@ -209,7 +240,15 @@ Fr to_dev(const HostFr &h)
 
 }  // namespace
 
-void ntt_run(const NttJob &job, hipStream_t st)
+namespace {
+
+struct PassLaunch {
+    PassArgs a;
+    uint32_t ncols, npolys;
+};
+
+// the one (log n <= 11) or two passes of a job
+int job_passes(const NttJob &job, PassLaunch out[2])
 {
     const uint32_t log_n = job.log_n;
     MH_CHECK(log_n <= 2 * kMaxLogLds, hipErrorInvalidValue, "NTT size above 2^22 is not supported");
@@ -240,11 +279,8 @@ void ntt_run(const NttJob &job, hipStream_t st)
     base.factors = job.factors;
     base.factor_len = job.factor_len;
 
-    if (log_n == 0) {
-        // a length-1 transform is the identity; only pre/post factors apply
-    }
-
     if (log_n <= kMaxLogLds) {
+        // (a length-1 transform is the identity; only pre/post factors apply)
         PassArgs a = base;
         a.in = job.in;
         a.out = job.out;
@@ -258,8 +294,8 @@ void ntt_run(const NttJob &job, hipStream_t st)
         a.pre = job.pre;
         a.in_len = job.in_len;
         a.post = has_post ? (job.post_zeta ? 2 : 1) : 0;
-        launch(a, 1, job.count, st);
-        return;
+        out[0] = PassLaunch{a, 1, job.count};
+        return 1;
     }
 
     const uint32_t t1 = (log_n + 1) / 2, t2 = log_n / 2;
@@ -279,7 +315,7 @@ void ntt_run(const NttJob &job, hipStream_t st)
     A.pre = job.pre;
     A.in_len = job.in_len;
     A.post = 0;
-    launch(A, n2, job.count, st);
+    out[0] = PassLaunch{A, n2, job.count};
 
     PassArgs B = base;
     B.in = job.scratch;
@@ -294,7 +330,98 @@ void ntt_run(const NttJob &job, hipStream_t st)
     B.pre = 0;
     B.in_len = ~0ull;
     B.post = has_post ? (job.post_zeta ? 2 : 1) : 0;
-    launch(B, n1, job.count, st);
+    out[1] = PassLaunch{B, n1, job.count};
+    return 2;
+}
+
+}  // namespace
+
+void ntt_run(const NttJob &job, hipStream_t st)
+{
+    PassLaunch p[2];
+    const int np = job_passes(job, p);
+    for (int i = 0; i < np; ++i) launch(p[i].a, p[i].ncols, p[i].npolys, st);
+}
+
+// ---------------------------------------------------------------- ragged sets
+
+struct RaggedNtt::Impl {
+    DevBuf<RaggedDesc> descs[2];
+    uint32_t nd[2] = {0, 0}, blocks[2] = {0, 0};
+    size_t lds[2] = {0, 0};
+    std::vector<RaggedDesc> host[2];
+    DevBuf<Fr> scratch;
+};
+
+RaggedNtt::RaggedNtt() : impl_(new Impl) {}
+RaggedNtt::~RaggedNtt() = default;
+
+void RaggedNtt::prepare(Fr *base, const size_t *offsets, const uint32_t *log_ns, size_t count, bool inverse,
+                        hipStream_t st)
+{
+    Impl &im = *impl_;
+    // runs of consecutive, back-to-back transforms of one size share a
+    // descriptor (one grid row per polynomial)
+    struct Run {
+        size_t first, count;
+    };
+    std::vector<Run> runs;
+    size_t scratch = 0;
+    for (size_t i = 0; i < count;) {
+        size_t j = i + 1;
+        const size_t n = size_t(1) << log_ns[i];
+        while (j < count && log_ns[j] == log_ns[i] && offsets[j] == offsets[j - 1] + n) ++j;
+        runs.push_back(Run{i, j - i});
+        scratch += ntt_scratch_elems(log_ns[i], j - i);
+        i = j;
+    }
+    im.scratch.ensure(std::max<size_t>(scratch, 1));
+    for (int s = 0; s < 2; ++s) {
+        im.host[s].clear();
+        im.blocks[s] = 0;
+        im.lds[s] = 0;
+    }
+    size_t soff = 0;
+    for (const Run &r : runs) {
+        const uint32_t log_n = log_ns[r.first];
+        NttJob job;
+        job.in = base + offsets[r.first];
+        job.out = base + offsets[r.first];
+        job.in_pstride = job.out_pstride = uint64_t(1) << log_n;
+        job.log_n = log_n;
+        job.count = (uint32_t)r.count;
+        job.inverse = inverse;
+        job.scratch = im.scratch.get() + soff;
+        soff += ntt_scratch_elems(log_n, r.count);
+        PassLaunch p[2];
+        const int np = job_passes(job, p);
+        for (int s = 0; s < np; ++s) {
+            const uint32_t gx = p[s].ncols / p[s].a.C;
+            const uint64_t nb = (uint64_t)gx * p[s].npolys;
+            MH_CHECK((uint64_t)im.blocks[s] + nb < (1ull << 31), hipErrorInvalidValue, "ragged NTT set too large");
+            im.host[s].push_back(RaggedDesc{p[s].a, gx, im.blocks[s]});
+            im.blocks[s] += (uint32_t)nb;
+            im.lds[s] = std::max<size_t>(im.lds[s], (size_t)p[s].a.C * (1u << p[s].a.log_m) * sizeof(Fr));
+        }
+    }
+    for (int s = 0; s < 2; ++s) {
+        im.nd[s] = (uint32_t)im.host[s].size();
+        if (!im.nd[s]) continue;
+        im.descs[s].ensure(im.nd[s]);
+        MH_HIP(hipMemcpyAsync(im.descs[s].get(), im.host[s].data(), im.nd[s] * sizeof(RaggedDesc),
+                              hipMemcpyHostToDevice, st));
+    }
+}
+
+void RaggedNtt::enqueue(hipStream_t st)
+{
+    const Impl &im = *impl_;
+    for (int s = 0; s < 2; ++s) {
+        if (!im.nd[s]) continue;
+        hipLaunchKernelGGL(ntt_pass_ragged, dim3(im.blocks[s]), dim3(kThreads), im.lds[s], st,
+                           (const RaggedDesc *)im.descs[s].get(), im.nd[s]);
+        MH_HIP(hipGetLastError());
+    }
 }
 
 size_t ntt_scratch_elems(uint32_t log_n, size_t count)

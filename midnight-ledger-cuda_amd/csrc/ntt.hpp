@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 
 #include "fields.hpp"
 #include "host_fr.hpp"
@@ -29,5 +30,25 @@ struct NttJob {
 
 void ntt_run(const NttJob &job, hipStream_t st);
 size_t ntt_scratch_elems(uint32_t log_n, size_t count);
+
+// In-place transforms of mixed sizes (polynomial i: 2^log_ns[i] values at
+// base + offsets[i]) with one launch per pass for the whole set: every
+// transform's first pass in one launch, the second passes of those above
+// 2^11 in another.  prepare() uploads the pass descriptors and sizes the
+// set's own scratch (stream-ordered); enqueue() only launches, so it can be
+// captured in a graph and replayed.  The buffers stay valid until the next
+// prepare().
+class RaggedNtt {
+public:
+    RaggedNtt();
+    ~RaggedNtt();
+    void prepare(Fr *base, const size_t *offsets, const uint32_t *log_ns, size_t count, bool inverse,
+                 hipStream_t st);
+    void enqueue(hipStream_t st);
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
 
 }  // namespace mh

@@ -122,6 +122,13 @@ def _declare(L):
         "midnight_hip_valu_peak": (
             E, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
         "midnight_hip_copy_bandwidth": (E, [c_size, ctypes.POINTER(ctypes.c_double)]),
+        "midnight_hip_batch_plan_create": (
+            E, [vpp, vp, vp, ctypes.POINTER(c_size), ctypes.POINTER(u32), c_size, u32]),
+        "midnight_hip_batch_plan_run": (E, [vp, u64p]),
+        "midnight_hip_batch_plan_info": (
+            E, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32),
+                ctypes.POINTER(ctypes.c_float)]),
+        "midnight_hip_batch_plan_free": (None, [vp]),
         "midnight_hip_field_mul_rate": (
             E, [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "midnight_hip_fr_device_vec_to_montgomery": (E, [vp]),
@@ -457,6 +464,49 @@ class MsmContext:
         out = np.zeros((n, 18), dtype=np.uint64)
         check(lib().midnight_hip_g1_msm_context_device_multi(P(out), self.handle, dvec.handle, offs, sz, n))
         return out
+
+
+class BatchPlan:
+    """Batched iNTT + MSM plan over a resident context and device vector:
+    instance i = inverse NTT (in place) of the 2^log_ns[i] values at
+    offsets[i], then their MSM against points [0, 2^log_ns[i])."""
+
+    GRAPH = 1
+    AUTOTUNE = 2
+
+    def __init__(self, ctx, dvec, offsets, log_ns, flags=GRAPH | AUTOTUNE):
+        n = len(offsets)
+        self.count = n
+        offs = (c_size * n)(*offsets)
+        logs = (ctypes.c_uint32 * n)(*log_ns)
+        h = ctypes.c_void_p()
+        check(lib().midnight_hip_batch_plan_create(ctypes.byref(h), ctx.handle, dvec.handle, offs, logs, n, flags))
+        self.handle = h
+        self._refs = (ctx, dvec)   # the plan refers to both
+
+    def run(self):
+        out = np.zeros((self.count, 18), dtype=np.uint64)
+        check(lib().midnight_hip_batch_plan_run(self.handle, P(out)))
+        return out
+
+    def info(self):
+        w, L, pc, g = (ctypes.c_uint32() for _ in range(4))
+        ms = ctypes.c_float()
+        check(lib().midnight_hip_batch_plan_info(self.handle, ctypes.byref(w), ctypes.byref(L), ctypes.byref(pc),
+                                                 ctypes.byref(g), ctypes.byref(ms)))
+        return {"window_bits": w.value, "chunk_len": L.value, "precomputed": bool(pc.value), "graph": bool(g.value),
+                "tuned_msm_ms": round(ms.value, 3)}
+
+    def free(self):
+        if self.handle and self.handle.value:
+            lib().midnight_hip_batch_plan_free(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- diagnostics
