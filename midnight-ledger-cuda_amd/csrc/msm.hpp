@@ -53,6 +53,13 @@ public:
                  uint32_t window_bits, hipStream_t st, const PrecompTable *pc = nullptr);
     void enqueue(hipStream_t st, ProfileScope *prof);
     void finish(RawG1Projective *out);
+    // enqueue() in two parts: through the accumulation, then the bucket
+    // tree with `levels` levels (at most the batch's worst case; fewer is
+    // still exact, the last pass sums what is left), the planes and the
+    // copy-out.  The chunk count of the fullest bucket is totals[3] on the
+    // device after enqueue_front().
+    void enqueue_front(hipStream_t st, ProfileScope *prof);
+    void enqueue_back(hipStream_t st, uint32_t levels, ProfileScope *prof);
 
     // Upper bounds of the serial work per lane.  A point addition takes the
     // same time per lane at any occupancy, so batches too small to fill the
