@@ -203,6 +203,77 @@ RustError midnight_sppark_bls12_381_fr_device_quotient_pieces(void **out,
                                                               const RawFr *blinds,
                                                               size_t blind_count);
 
+/* replaces bls12_381_g1_msm.cu:2878: device_vec[indices[i]] = values[i] */
+RustError midnight_sppark_bls12_381_fr_device_vec_scatter_host(void *device_vec, const uint32_t *indices,
+                                                               const RawFr *values, size_t count);
+/* replaces bls12_381_g1_msm.cu:2916: new vector = prefix ++ suffix (host) */
+RustError midnight_sppark_bls12_381_fr_device_vec_concat_host(void **out, const void *prefix_device_vec,
+                                                              const RawFr *suffix, size_t suffix_len);
+/* replaces bls12_381_g1_msm.cu:2964: new vector = sources in order ++ suffix (host) */
+RustError midnight_sppark_bls12_381_fr_device_vec_concat_many_host(void **out,
+                                                                   const void *const *source_device_vecs,
+                                                                   size_t source_count, const RawFr *suffix,
+                                                                   size_t suffix_len);
+/* replaces bls12_381_g1_msm.cu:3025: new vector = polynomials indices[i] (poly_len each) of the source */
+RustError midnight_sppark_bls12_381_fr_device_vec_gather_polys(void **out, const void *source_device_vec,
+                                                               size_t poly_len, const uint32_t *indices,
+                                                               size_t index_count);
+/* replaces bls12_381_g1_msm.cu:3101: per polynomial of [scalar_offset, +poly_len*count), starts[i] = first
+ * nonzero index (poly_len if none) and nonzero_counts[i] (optional); out receives the suffixes
+ * poly[starts[i]..] back to back (out == NULL and out_capacity == 0: only *out_len) */
+RustError midnight_sppark_bls12_381_fr_device_vec_download_nonzero_suffixes(
+    RawFr *out, size_t out_capacity, size_t *out_len, size_t *starts, size_t *nonzero_counts,
+    const void *device_vec, size_t scalar_offset, size_t poly_len, size_t count);
+
+/* ---------------- polynomial combinations and multi-open ---------------- */
+
+/* (source, index): source 0 dynamic, 1 fixed, 2 extra (bls12_381_g1_msm.cu:383-386) */
+typedef struct RawDevicePolyRef {
+    uint32_t source;
+    uint32_t index;
+} RawDevicePolyRef;
+
+/* replaces bls12_381_g1_msm.cu:3558: out = sum_j scalars[j] * polys[j] (host arrays) */
+RustError midnight_sppark_bls12_381_fr_poly_lincomb(RawFr *out, const RawFr *polys, size_t poly_len,
+                                                    size_t poly_count, const RawFr *scalars);
+/* replaces bls12_381_g1_msm.cu:3234: new device vector sum_j scalars[j] * polys[j] */
+RustError midnight_sppark_bls12_381_fr_device_poly_lincomb(void **out, const void *polys_vec, size_t poly_len,
+                                                           size_t poly_count, const RawFr *scalars,
+                                                           size_t scalar_count);
+/* replaces bls12_381_g1_msm.cu:3985: out[j] = poly_j(point), contiguous polynomials */
+RustError midnight_sppark_bls12_381_fr_device_eval_polys(RawFr *out, const void *device_vec, size_t poly_len,
+                                                         size_t poly_count, RawFr point);
+/* replaces bls12_381_g1_msm.cu:3600: q_out[k] = sum over items t of q_offsets[k]..q_offsets[k+1] of
+ * q_coefficients[t] * polys[q_poly_indices[t]]; eval_out[i] = polys[query_poly_indices[i]] at
+ * points[query_point_indices[i]] (poly/kzg/mod.rs:1978-2570) */
+RustError midnight_sppark_bls12_381_fr_multiopen_prepare(
+    RawFr *q_out, RawFr *eval_out, const RawFr *polys, size_t poly_len, size_t poly_count,
+    const uint32_t *q_offsets, size_t q_offsets_len, const uint32_t *q_poly_indices, const RawFr *q_coefficients,
+    size_t q_items_len, const RawFr *points, size_t point_count, const uint32_t *query_poly_indices,
+    const uint32_t *query_point_indices, size_t query_count);
+/* replaces bls12_381_g1_msm.cu:3723: the same over resident polynomials (poly_refs name a source vector
+ * and a polynomial index); q polynomials to q_out (host) and/or *q_device_out */
+RustError midnight_sppark_bls12_381_fr_multiopen_prepare_device(
+    RawFr *q_out, void **q_device_out, RawFr *eval_out, const void *dynamic_source, const void *fixed_source,
+    const void *extra_source, size_t poly_len, const RawDevicePolyRef *poly_refs, size_t poly_count,
+    const uint32_t *q_offsets, size_t q_offsets_len, const uint32_t *q_poly_indices, const RawFr *q_coefficients,
+    size_t q_items_len, const RawFr *points, size_t point_count, const uint32_t *query_poly_indices,
+    const uint32_t *query_point_indices, size_t query_count);
+/* replaces bls12_381_g1_msm.cu:3873: f = sum_k s_k * (q_k divided by (X - a) for each point a of
+ * points[point_offsets[k]..point_offsets[k+1]], remainders dropped) */
+RustError midnight_sppark_bls12_381_fr_multiopen_f_poly_device(
+    void **out, const void *q_device_vec, size_t poly_len, const uint32_t *point_offsets,
+    size_t point_offsets_len, const RawFr *points, size_t point_count, const RawFr *combination_scalars,
+    size_t scalar_count);
+/* replaces bls12_381_g1_msm.cu:4025: pi = (final - final(point)) / (X - point) with
+ * final = sum_k s_k q_k + s_last f; poly_len - 1 coefficients */
+RustError midnight_sppark_bls12_381_fr_multiopen_pi_device(void **out, const void *q_device_vec,
+                                                           const void *f_device_vec, size_t poly_len,
+                                                           const RawFr *combination_scalars, size_t scalar_count,
+                                                           RawFr point);
+/* replaces bls12_381_g1_msm.cu:6582: values[i] = 1 / values[i] (0 stays 0) */
+RustError midnight_sppark_bls12_381_fr_invert_assign(RawFr *values, size_t len);
+
 /* ---------------- quotient numerator ---------------- */
 
 /* Compiled expression graph (bls12_381_g1_msm.cu:351-376; built by the
@@ -331,6 +402,27 @@ RustError midnight_sppark_bls12_381_fr_permutation_products(
     void **out, const RawFr *columns, const RawFr *permutations, size_t len, size_t column_count,
     size_t chunk_len, RawFr beta, RawFr gamma, RawFr omega, const RawFr *delta_powers,
     const RawFr *blind_values, size_t blinding_factors);
+
+/* replaces bls12_381_g1_msm.cu:4451: lookup product columns: z[0] = 1, z[i] = prod_{j < i}
+ * (ci_j + beta)(ct_j + gamma) / ((pi_j + beta)(pt_j + gamma)) (zero denominator: zero fraction), the last
+ * blinding_factors rows of lookup l from blind_values[l * blinding_factors ...]; inputs lookup-major
+ * (lookup l at [l*len, (l+1)*len)) (plonk/lookup/prover.rs:1249-1340) */
+RustError midnight_sppark_bls12_381_fr_lookup_products(
+    void **out, const RawFr *compressed_inputs, const RawFr *compressed_tables, const RawFr *permuted_inputs,
+    const RawFr *permuted_tables, size_t len, size_t lookup_count, RawFr beta, RawFr gamma,
+    const RawFr *blind_values, size_t blinding_factors);
+/* replaces bls12_381_g1_msm.cu:4545: the same from one device vector holding compressed inputs,
+ * compressed tables, permuted inputs, permuted tables (4 * lookup_count * len values) */
+RustError midnight_sppark_bls12_381_fr_lookup_products_device(void **out, const void *trace_lagrange, size_t len,
+                                                              size_t lookup_count, RawFr beta, RawFr gamma,
+                                                              const RawFr *blind_values, size_t blinding_factors);
+/* replaces bls12_381_g1_msm.cu:5667: each trash graph of a numerator context on the Lagrange domain
+ * (input_len rows, rotations unscaled) from the context's fixed columns and resident advice / instance;
+ * *out receives trash_count * input_len values */
+RustError midnight_sppark_bls12_381_trace_context_trash(void **out, const void *context, const void *advice,
+                                                        size_t advice_cols, const void *instance,
+                                                        size_t instance_cols, const RawFr *challenges,
+                                                        size_t challenge_len, RawFr trash_challenge);
 
 /* replaces bls12_381_g1_msm.cu:5738: the same products from a numerator
  * context's Lagrange-form fixed and sigma columns (input_len rows, trace

@@ -189,6 +189,212 @@ extern "C" void midnight_sppark_bls12_381_fr_device_vec_free(void *device_vec)
     delete dv;
 }
 
+namespace {
+
+__global__ void k_scatter(Fr *out, const uint32_t *idx, const Fr *vals, size_t n)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) store_fr(out + idx[i], load_fr(vals + i));
+}
+
+// per polynomial (block): index of the first nonzero value (len if none) and
+// the number of nonzero values
+__global__ __launch_bounds__(256) void k_nonzero_stats(uint64_t *starts, uint64_t *counts, const Fr *v, size_t len)
+{
+    __shared__ unsigned long long first;
+    __shared__ unsigned long long live;
+    if (threadIdx.x == 0) {
+        first = len;
+        live = 0;
+    }
+    __syncthreads();
+    const Fr *p = v + (size_t)blockIdx.x * len;
+    unsigned long long f = len, c = 0;
+    for (size_t i = threadIdx.x; i < len; i += blockDim.x)
+        if (!load_fr(p + i).is_zero()) {
+            if (i < f) f = i;
+            ++c;
+        }
+    atomicMin(&first, f);
+    atomicAdd(&live, c);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        starts[blockIdx.x] = first;
+        counts[blockIdx.x] = live;
+    }
+}
+
+std::unique_ptr<DeviceVec> alloc_vec(size_t len)
+{
+    auto dv = std::make_unique<DeviceVec>();
+    dv->lane = current_lane();
+    dv->len = len;
+    dv->values.alloc(std::max<size_t>(len, 1));
+    return dv;
+}
+
+}  // namespace
+
+// bls12_381_g1_msm.cu:2878-2914
+extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_scatter_host(void *device_vec, const uint32_t *indices,
+                                                                          const RawFr *values, size_t count)
+{
+    if (device_vec == nullptr || (count != 0 && (indices == nullptr || values == nullptr)))
+        return make_error((int)hipErrorInvalidValue, "invalid device scatter arguments");
+    auto *dv = static_cast<DeviceVec *>(device_vec);
+    for (size_t i = 0; i < count; ++i)
+        if (indices[i] >= dv->len) return make_error((int)hipErrorInvalidValue, "device scatter index out of range");
+    if (count == 0) return ok();
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<uint32_t> d_idx(count);
+    DevBuf<Fr> d_vals(count);
+    MH_HIP(hipMemcpyAsync(d_idx.get(), indices, count * 4, hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(d_vals.get(), values, count * 32, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)ceil_div(count, 256)), dim3(256), 0, st, dv->values.get(),
+                       (const uint32_t *)d_idx.get(), (const Fr *)d_vals.get(), count);
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipStreamSynchronize(st));
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:2916-2962
+extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_concat_host(void **out, const void *prefix_device_vec,
+                                                                         const RawFr *suffix, size_t suffix_len)
+{
+    if (out == nullptr || prefix_device_vec == nullptr || (suffix_len != 0 && suffix == nullptr))
+        return make_error((int)hipErrorInvalidValue, "invalid device vector concat arguments");
+    *out = nullptr;
+    const auto *prefix = static_cast<const DeviceVec *>(prefix_device_vec);
+    if (suffix_len > SIZE_MAX - prefix->len)
+        return make_error((int)hipErrorInvalidValue, "device vector concat length overflow");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    auto dv = alloc_vec(prefix->len + suffix_len);
+    if (prefix->len)
+        MH_HIP(hipMemcpyAsync(dv->values.get(), prefix->values.get(), prefix->len * 32, hipMemcpyDeviceToDevice, st));
+    if (suffix_len)
+        MH_HIP(hipMemcpyAsync(dv->values.get() + prefix->len, suffix, suffix_len * 32, hipMemcpyHostToDevice, st));
+    MH_HIP(hipStreamSynchronize(st));
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:2964-3023
+extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_concat_many_host(void **out,
+                                                                              const void *const *source_device_vecs,
+                                                                              size_t source_count, const RawFr *suffix,
+                                                                              size_t suffix_len)
+{
+    if (out == nullptr || source_count == 0 || source_device_vecs == nullptr || (suffix_len != 0 && suffix == nullptr))
+        return make_error((int)hipErrorInvalidValue, "invalid multi-device vector concat arguments");
+    *out = nullptr;
+    size_t total = suffix_len;
+    for (size_t i = 0; i < source_count; ++i) {
+        if (source_device_vecs[i] == nullptr)
+            return make_error((int)hipErrorInvalidValue, "null device vector concat source");
+        const auto *src = static_cast<const DeviceVec *>(source_device_vecs[i]);
+        if (src->len > SIZE_MAX - total)
+            return make_error((int)hipErrorInvalidValue, "multi-device vector concat length overflow");
+        total += src->len;
+    }
+    MH_TRY
+    hipStream_t st = cur_stream();
+    auto dv = alloc_vec(total);
+    size_t off = 0;
+    for (size_t i = 0; i < source_count; ++i) {
+        const auto *src = static_cast<const DeviceVec *>(source_device_vecs[i]);
+        if (src->len)
+            MH_HIP(hipMemcpyAsync(dv->values.get() + off, src->values.get(), src->len * 32, hipMemcpyDeviceToDevice,
+                                  st));
+        off += src->len;
+    }
+    if (suffix_len)
+        MH_HIP(hipMemcpyAsync(dv->values.get() + off, suffix, suffix_len * 32, hipMemcpyHostToDevice, st));
+    MH_HIP(hipStreamSynchronize(st));
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:3025-3072
+extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_gather_polys(void **out, const void *source_device_vec,
+                                                                          size_t poly_len, const uint32_t *indices,
+                                                                          size_t index_count)
+{
+    if (out == nullptr || source_device_vec == nullptr || poly_len == 0 || indices == nullptr || index_count == 0 ||
+        index_count > SIZE_MAX / poly_len)
+        return make_error((int)hipErrorInvalidValue, "invalid device polynomial gather arguments");
+    *out = nullptr;
+    const auto *src = static_cast<const DeviceVec *>(source_device_vec);
+    if (src->len % poly_len != 0)
+        return make_error((int)hipErrorInvalidValue, "device polynomial gather source shape mismatch");
+    const size_t source_count = src->len / poly_len;
+    for (size_t i = 0; i < index_count; ++i)
+        if (indices[i] >= source_count)
+            return make_error((int)hipErrorInvalidValue, "device polynomial gather index out of range");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    auto dv = alloc_vec(index_count * poly_len);
+    for (size_t i = 0; i < index_count; ++i)
+        MH_HIP(hipMemcpyAsync(dv->values.get() + i * poly_len, src->values.get() + (size_t)indices[i] * poly_len,
+                              poly_len * 32, hipMemcpyDeviceToDevice, st));
+    MH_HIP(hipStreamSynchronize(st));
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:3101-3179
+extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_download_nonzero_suffixes(
+    RawFr *out, size_t out_capacity, size_t *out_len, size_t *starts, size_t *nonzero_counts, const void *device_vec,
+    size_t scalar_offset, size_t poly_len, size_t count)
+{
+    if (out_len == nullptr || starts == nullptr || device_vec == nullptr || poly_len == 0 || count == 0 ||
+        count > SIZE_MAX / poly_len || (out_capacity != 0 && out == nullptr))
+        return make_error((int)hipErrorInvalidValue, "invalid nonzero suffix download arguments");
+    *out_len = 0;
+    const auto *dv = static_cast<const DeviceVec *>(device_vec);
+    const size_t required = poly_len * count;
+    if (scalar_offset > dv->len || required > dv->len - scalar_offset)
+        return make_error((int)hipErrorInvalidValue, "nonzero suffix range exceeds device vector");
+    if (count > 0x7fffffffu) return make_error((int)hipErrorInvalidValue, "too many polynomials");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<uint64_t> d_stats(2 * count);
+    const Fr *base = dv->values.get() + scalar_offset;
+    hipLaunchKernelGGL(k_nonzero_stats, dim3((unsigned)count), dim3(256), 0, st, d_stats.get(),
+                       d_stats.get() + count, base, poly_len);
+    MH_HIP(hipGetLastError());
+    std::vector<uint64_t> h(2 * count);
+    MH_HIP(hipMemcpyAsync(h.data(), d_stats.get(), 2 * count * 8, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    size_t total = 0;
+    for (size_t i = 0; i < count; ++i) {
+        starts[i] = (size_t)h[i];
+        if (nonzero_counts) nonzero_counts[i] = (size_t)h[count + i];
+        total += poly_len - starts[i];
+    }
+    if (out == nullptr && out_capacity == 0) {
+        *out_len = total;
+        return ok();
+    }
+    if (total > out_capacity) return make_error((int)hipErrorInvalidValue, "nonzero suffix output is too small");
+    size_t cursor = 0;
+    for (size_t i = 0; i < count; ++i) {
+        const size_t n = poly_len - starts[i];
+        if (n)
+            MH_HIP(hipMemcpyAsync(out + cursor, base + i * poly_len + starts[i], n * 32, hipMemcpyDeviceToHost, st));
+        cursor += n;
+    }
+    MH_HIP(hipStreamSynchronize(st));
+    *out_len = total;
+    return ok();
+    MH_CATCH
+}
+
 extern "C" size_t midnight_hip_fr_device_vec_len(const void *device_vec)
 {
     return device_vec ? static_cast<const DeviceVec *>(device_vec)->len : 0;
@@ -534,6 +740,22 @@ RustError host_vector_op(RawFr *values, size_t len, F launch)
 }
 
 }  // namespace
+
+// bls12_381_g1_msm.cu:6582-6610: values[i] = 1 / values[i], 0 stays 0
+extern "C" RustError midnight_sppark_bls12_381_fr_invert_assign(RawFr *values, size_t len)
+{
+    if (len == 0) return ok();
+    if (!values) return make_error((int)hipErrorInvalidValue, "values is null");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<Fr> d(len);
+    MH_HIP(hipMemcpyAsync(d.get(), values, len * 32, hipMemcpyHostToDevice, st));
+    fr_batch_invert(d.get(), len, st);
+    MH_HIP(hipMemcpyAsync(values, d.get(), len * 32, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    return ok();
+    MH_CATCH
+}
 
 extern "C" RustError midnight_sppark_bls12_381_fr_mul_assign(RawFr *values, RawFr scalar, size_t len)
 {

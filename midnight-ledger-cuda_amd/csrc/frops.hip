@@ -1,6 +1,7 @@
 // Element-wise Fr kernels behind the reference's vector helpers
 // (bls12_381_g1_msm.cu:1188-1223 mul_assign / zeta / repeating,
-// :1613-1646 quotient pieces, :1761-1772 scramble-mask add).  All are
+// :1613-1646 quotient pieces, :1761-1772 scramble-mask add, :1225 and
+// :6582 invert_assign).  All are
 // HBM-bound streaming passes: one 32-byte read and write per element,
 // vectorised as two 16-byte accesses, grid-stride over the vector.
 #include "fields.hpp"
@@ -75,6 +76,45 @@ __global__ void k_fill(Fr *v, Fr x, size_t n)
         store_fr(v + i, x);
 }
 
+// x^(r - 2) = x^-1 for x != 0 (Fermat), left-to-right over the bits of r - 2
+__device__ __forceinline__ Fr fr_inverse(const Fr &x)
+{
+    constexpr uint64_t E[4] = {0xfffffffeffffffffULL, 0x53bda402fffe5bfeULL, 0x3339d80809a1d805ULL,
+                               0x73eda753299d7d48ULL};
+    Fr r = x;   // bit 254, the top bit of r - 2
+    for (int bit = 253; bit >= 0; --bit) {
+        r = r * r;
+        if ((E[bit >> 6] >> (bit & 63)) & 1) r = r * x;
+    }
+    return r;
+}
+
+// Montgomery's batch inversion over runs of kInvRun elements per lane: one
+// Fermat inversion per run and three products per element; zeros stay zero
+// (halo2's batch_invert and the reference's 1/0 both give 0).
+constexpr uint32_t kInvRun = 32;
+
+__global__ __launch_bounds__(256) void k_batch_invert(Fr *v, Fr *tmp, size_t n)
+{
+    const size_t lane = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t b = lane * kInvRun;
+    if (b >= n) return;
+    const size_t e = b + kInvRun < n ? b + kInvRun : n;
+    Fr acc = Fr::one();
+    for (size_t i = b; i < e; ++i) {
+        const Fr x = load_fr(v + i);
+        store_fr(tmp + i, acc);
+        if (!x.is_zero()) acc = acc * x;
+    }
+    Fr inv = fr_inverse(acc);
+    for (size_t i = e; i-- > b;) {
+        const Fr x = load_fr(v + i);
+        if (x.is_zero()) continue;
+        store_fr(v + i, inv * load_fr(tmp + i));
+        inv = inv * x;
+    }
+}
+
 }  // namespace
 
 void fr_mul_scalar(Fr *v, const Fr &s, size_t n, hipStream_t st)
@@ -114,6 +154,16 @@ void fr_quotient_pieces(Fr *out, const Fr *quotient, size_t poly_len, size_t pie
     hipLaunchKernelGGL(k_blind_pieces, dim3((unsigned)ceil_div(pieces, kBlock)), dim3(kBlock), 0, st, out, blinds,
                        poly_len, pieces);
     MH_HIP(hipGetLastError());
+}
+
+void fr_batch_invert(Fr *v, size_t n, hipStream_t st)
+{
+    if (!n) return;
+    DevBuf<Fr> tmp(n);
+    hipLaunchKernelGGL(k_batch_invert, dim3((unsigned)ceil_div(ceil_div(n, (size_t)kInvRun), 256)), dim3(256), 0, st,
+                       v, tmp.get(), n);
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipStreamSynchronize(st));
 }
 
 void fr_fill(Fr *v, const Fr &x, size_t n, hipStream_t st)

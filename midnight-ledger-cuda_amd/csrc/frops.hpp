@@ -16,4 +16,7 @@ void fr_quotient_pieces(Fr *out, const Fr *quotient, size_t poly_len, size_t pie
                         hipStream_t st);
 void fr_fill(Fr *v, const Fr &x, size_t n, hipStream_t st);
 
+// in place: v[i] = 1 / v[i], zeros stay zero
+void fr_batch_invert(Fr *v, size_t n, hipStream_t st);
+
 }  // namespace mh

@@ -1,7 +1,8 @@
-// Grand products: the permutation argument's product columns and shifted
-// prefix products (bls12_381_g1_msm.cu:1446-1591 fraction / prefix /
+// Grand products: the permutation and lookup arguments' product columns and
+// shifted prefix products (bls12_381_g1_msm.cu:1446-1591 fraction / prefix /
 // blind-tail kernels, :4303 fr_shifted_prefix_product, :4342
-// fr_permutation_products, :5738 trace_context_permutation).
+// fr_permutation_products, :4451 / :4545 fr_lookup_products(_device),
+// :5738 trace_context_permutation).
 //
 // The reference divides per row (one field inversion per row) and runs a
 // generic prefix product.  Here no row is inverted: with
@@ -222,6 +223,24 @@ __global__ __launch_bounds__(256) void k_perm_fraction(PermArgs a)
     store_fr(a.den + (size_t)s * a.len + i, den);
 }
 
+// lookup l, row i: num = (ci + beta)(ct + gamma), den = (pi + beta)(pt + gamma)
+// (a zero denominator gives a zero fraction, as batch inversion leaves 0)
+__global__ __launch_bounds__(256) void k_lookup_fraction(PermArgs a, const Fr *ci, const Fr *ct, const Fr *pi,
+                                                         const Fr *pt)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= a.len) return;
+    const size_t o = (size_t)blockIdx.y * a.len + i;
+    Fr num = (load_fr(ci + o) + a.beta) * (load_fr(ct + o) + a.gamma);
+    Fr den = (load_fr(pi + o) + a.beta) * (load_fr(pt + o) + a.gamma);
+    if (den.is_zero()) {
+        den = Fr::one();
+        num = Fr::zero();
+    }
+    store_fr(a.num + o, num);
+    store_fr(a.den + o, den);
+}
+
 // Per set: N_{u-1}, S_u and D = S_0 (u = last usable row).
 __global__ void k_perm_points(PermArgs a, size_t u, uint32_t sets, Fr *points)
 {
@@ -360,6 +379,67 @@ void permutation_products_device(Fr *out, size_t len, size_t column_count, size_
     MH_HIP(hipStreamSynchronize(st));
 }
 
+void lookup_products_device(Fr *out, const Fr *ci, const Fr *ct, const Fr *pi, const Fr *pt, size_t len,
+                            size_t count, const HostFr &beta, const HostFr &gamma, const Fr *blinds, size_t blinding,
+                            hipStream_t st)
+{
+    MH_CHECK(len > 0 && count > 0 && blinding < len && count <= 65535, (int)hipErrorInvalidValue,
+             "invalid lookup product arguments");
+    const size_t tiles = ceil_div(len, kScanTile);
+    auto &sc = gp_scratch();
+    sc.num.ensure(count * len);
+    sc.den.ensure(count * len);
+    sc.num_tot.ensure(count * tiles);
+    sc.den_tot.ensure(count * tiles);
+    sc.small.ensure(3 * count + 2 * count);
+    Fr *d_points = sc.small.get();
+    Fr *d_consts = d_points + 3 * count;
+    PermArgs a{};
+    a.beta = fr_of(beta);
+    a.gamma = fr_of(gamma);
+    a.len = len;
+    a.num = sc.num.get();
+    a.den = sc.den.get();
+    a.num_tot = sc.num_tot.get();
+    a.den_tot = sc.den_tot.get();
+    a.tiles = tiles;
+    const dim3 rows((unsigned)ceil_div(len, 256), (unsigned)count);
+    hipLaunchKernelGGL(k_lookup_fraction, rows, dim3(256), 0, st, a, ci, ct, pi, pt);
+    MH_HIP(hipGetLastError());
+    for (size_t s0 = 0; s0 < count; s0 += kMaxScanJobs / 2) {
+        ScanJobs jobs{};
+        int cnt = 0;
+        for (size_t l = s0; l < std::min(count, s0 + kMaxScanJobs / 2); ++l) {
+            jobs.j[cnt++] = ScanJob{a.num + l * len, a.num_tot + l * tiles, len, 0};
+            jobs.j[cnt++] = ScanJob{a.den + l * len, a.den_tot + l * tiles, len, 1};
+        }
+        run_scans(jobs, cnt, st);
+    }
+    // z[k] = N_{k-1} * S_k / D per lookup: read back D (u = 0 gives it twice)
+    for (size_t l0 = 0; l0 < count; l0 += 1024) {
+        const size_t n = std::min<size_t>(1024, count - l0);
+        PermArgs b = a;
+        b.num = a.num + l0 * len;
+        b.den = a.den + l0 * len;
+        b.num_tot = a.num_tot + l0 * tiles;
+        b.den_tot = a.den_tot + l0 * tiles;
+        hipLaunchKernelGGL(k_perm_points, dim3(1), dim3((unsigned)(64 * ceil_div(n, 64))), 0, st, b, (size_t)0,
+                           (uint32_t)n, d_points + 3 * l0);
+        MH_HIP(hipGetLastError());
+    }
+    std::vector<HostFr> pts(3 * count), consts(2 * count);
+    MH_HIP(hipMemcpyAsync(pts.data(), d_points, 3 * count * 32, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    for (size_t l = 0; l < count; ++l) {
+        consts[2 * l] = HostFr::one();
+        consts[2 * l + 1] = pts[3 * l + 2].inverse();   // D != 0: zero rows were replaced
+    }
+    MH_HIP(hipMemcpyAsync(d_consts, consts.data(), 2 * count * 32, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_perm_finish, rows, dim3(256), 0, st, a, out, (const Fr *)d_consts, blinds, blinding);
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipStreamSynchronize(st));
+}
+
 void shifted_prefix_product_device(Fr *out, const Fr *in, size_t len, const HostFr &initial, hipStream_t st)
 {
     if (len == 0) return;
@@ -439,6 +519,73 @@ extern "C" RustError midnight_sppark_bls12_381_fr_permutation_products(
     dv->values.alloc(dv->len);
     permutation_products_device(dv->values.get(), len, column_count, chunk_len, v, d_perm.get(), dp.data(),
                                 to_hfr(beta), to_hfr(gamma), to_hfr(omega), d_blinds.get(), blinding_factors, st);
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:4451-4543
+extern "C" RustError midnight_sppark_bls12_381_fr_lookup_products(
+    void **out, const RawFr *compressed_inputs, const RawFr *compressed_tables, const RawFr *permuted_inputs,
+    const RawFr *permuted_tables, size_t len, size_t lookup_count, RawFr beta, RawFr gamma,
+    const RawFr *blind_values, size_t blinding_factors)
+{
+    if (out == nullptr || len == 0 || lookup_count == 0 || compressed_inputs == nullptr ||
+        compressed_tables == nullptr || permuted_inputs == nullptr || permuted_tables == nullptr ||
+        blinding_factors >= len || (blinding_factors != 0 && blind_values == nullptr))
+        return make_error((int)hipErrorInvalidValue, "invalid lookup product arguments");
+    *out = nullptr;
+    if (lookup_count > SIZE_MAX / len) return make_error((int)hipErrorInvalidValue, "lookup product size overflow");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    const size_t total = lookup_count * len;
+    DevBuf<Fr> d(4 * total), d_blinds(std::max<size_t>(lookup_count * blinding_factors, 1));
+    MH_HIP(hipMemcpyAsync(d.get(), compressed_inputs, total * 32, hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(d.get() + total, compressed_tables, total * 32, hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(d.get() + 2 * total, permuted_inputs, total * 32, hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(d.get() + 3 * total, permuted_tables, total * 32, hipMemcpyHostToDevice, st));
+    if (blinding_factors)
+        MH_HIP(hipMemcpyAsync(d_blinds.get(), blind_values, lookup_count * blinding_factors * 32,
+                              hipMemcpyHostToDevice, st));
+    auto dv = std::make_unique<DeviceVec>();
+    dv->lane = current_lane();
+    dv->len = total;
+    dv->values.alloc(total);
+    lookup_products_device(dv->values.get(), d.get(), d.get() + total, d.get() + 2 * total, d.get() + 3 * total,
+                           len, lookup_count, to_hfr(beta), to_hfr(gamma), d_blinds.get(), blinding_factors, st);
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:4545-4634
+extern "C" RustError midnight_sppark_bls12_381_fr_lookup_products_device(void **out, const void *trace_lagrange,
+                                                                         size_t len, size_t lookup_count, RawFr beta,
+                                                                         RawFr gamma, const RawFr *blind_values,
+                                                                         size_t blinding_factors)
+{
+    if (out == nullptr || len == 0 || lookup_count == 0 || trace_lagrange == nullptr || blinding_factors >= len ||
+        (blinding_factors != 0 && blind_values == nullptr))
+        return make_error((int)hipErrorInvalidValue, "invalid resident lookup product arguments");
+    *out = nullptr;
+    if (lookup_count > SIZE_MAX / len)
+        return make_error((int)hipErrorInvalidValue, "resident lookup product size overflow");
+    const size_t total = lookup_count * len;
+    const auto *trace = static_cast<const DeviceVec *>(trace_lagrange);
+    if (trace->len != total * 4) return make_error((int)hipErrorInvalidValue, "resident lookup trace length mismatch");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<Fr> d_blinds(std::max<size_t>(lookup_count * blinding_factors, 1));
+    if (blinding_factors)
+        MH_HIP(hipMemcpyAsync(d_blinds.get(), blind_values, lookup_count * blinding_factors * 32,
+                              hipMemcpyHostToDevice, st));
+    const Fr *b = trace->values.get();
+    auto dv = std::make_unique<DeviceVec>();
+    dv->lane = current_lane();
+    dv->len = total;
+    dv->values.alloc(total);
+    lookup_products_device(dv->values.get(), b, b + total, b + 2 * total, b + 3 * total, len, lookup_count,
+                           to_hfr(beta), to_hfr(gamma), d_blinds.get(), blinding_factors, st);
     *out = dv.release();
     return ok();
     MH_CATCH

@@ -25,6 +25,13 @@ void permutation_products_device(Fr *out, size_t len, size_t column_count, size_
                                  const HostFr &beta, const HostFr &gamma, const HostFr &omega, const Fr *blinds,
                                  size_t blinding, hipStream_t st);
 
+// Lookup product columns, lookup-major [count][len]: z[0] = 1,
+// z[k] = prod_{j < k} (ci_j + beta)(ct_j + gamma) / ((pi_j + beta)(pt_j + gamma)),
+// the last `blinding` rows of lookup l from blinds[l * blinding ...].
+void lookup_products_device(Fr *out, const Fr *ci, const Fr *ct, const Fr *pi, const Fr *pt, size_t len,
+                            size_t count, const HostFr &beta, const HostFr &gamma, const Fr *blinds, size_t blinding,
+                            hipStream_t st);
+
 // out[0] = initial, out[i] = initial * prod_{j < i} in[j] (in and out may alias).
 void shifted_prefix_product_device(Fr *out, const Fr *in, size_t len, const HostFr &initial, hipStream_t st);
 

@@ -803,6 +803,58 @@ extern "C" RustError midnight_sppark_bls12_381_numerator_context_eval(
     MH_CATCH
 }
 
+// bls12_381_g1_msm.cu:5667-5736: the context's trash graphs over its
+// Lagrange domain (input_len rows, rotations unscaled), fixed columns in
+// Lagrange form, no fold: out[t * input_len + row]
+extern "C" RustError midnight_sppark_bls12_381_trace_context_trash(void **out, const void *raw_context,
+                                                                   const void *raw_advice, size_t advice_cols,
+                                                                   const void *raw_instance, size_t instance_cols,
+                                                                   const RawFr *challenges, size_t challenge_len,
+                                                                   RawFr trash_challenge)
+{
+    if (out == nullptr || raw_context == nullptr || (challenge_len != 0 && challenges == nullptr))
+        return make_error((int)hipErrorInvalidValue, "invalid resident trash arguments");
+    *out = nullptr;
+    const auto *c = reinterpret_cast<const NumeratorContext *>(raw_context);
+    const size_t len = c->input_len;
+    const auto *advice = reinterpret_cast<const DeviceVec *>(raw_advice);
+    const auto *instance = reinterpret_cast<const DeviceVec *>(raw_instance);
+    auto has_len = [](const DeviceVec *v, size_t expected) {
+        return expected == 0 ? (v == nullptr || v->len == 0) : (v != nullptr && v->len == expected);
+    };
+    if (c->trash_count == 0 || !has_len(advice, advice_cols * len) || !has_len(instance, instance_cols * len))
+        return make_error((int)hipErrorInvalidValue, "resident trash vector length mismatch");
+    if (advice_cols > 0xffffffffu || instance_cols > 0xffffffffu || challenge_len > 0xffffffffu)
+        return make_error((int)hipErrorInvalidValue, "trash argument count too large");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<Fr> d_ch;
+    auto dv = std::make_unique<DeviceVec>();
+    dv->lane = current_lane();
+    dv->len = c->trash_count * len;
+    dv->values.alloc(dv->len);
+    NumArgs a{};
+    a.out = dv->values.get();
+    a.len = len;
+    a.fixed = c->fixed_lagrange.get();
+    a.advice = advice_cols ? advice->values.get() : nullptr;
+    a.instance = instance_cols ? instance->values.get() : nullptr;
+    a.challenges = up(d_ch, challenges, challenge_len, st);
+    a.fixed_cols = (uint32_t)c->fixed_cols;
+    a.advice_cols = (uint32_t)advice_cols;
+    a.instance_cols = (uint32_t)instance_cols;
+    a.challenge_len = (uint32_t)challenge_len;
+    a.trash = to_fr(trash_challenge);
+    a.rot_scale = 1;
+    a.graphs = c->graphs.table.get() + 1 + c->lookup_count;   // custom, lookups, then trash
+    a.graph_count = (uint32_t)c->trash_count;
+    launch_rows(a, c->graphs.max_slots, k_eval_graphs<true>, k_eval_graphs<false>, st);
+    MH_HIP(hipStreamSynchronize(st));
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
 extern "C" RustError midnight_sppark_bls12_381_trace_context_permutation(
     void **out, const void *raw_context, const void *raw_advice, size_t advice_cols, const void *raw_instance,
     size_t instance_cols, RawFr beta, RawFr gamma, const RawFr *delta_powers, size_t delta_powers_len,

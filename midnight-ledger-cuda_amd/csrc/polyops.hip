@@ -16,8 +16,10 @@
 // coefficient again (a != 0; the opening points are random challenges).
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <vector>
 
+#include "devvec.hpp"
 #include "fields.hpp"
 #include "polyops.hpp"
 #include "rowpow.hpp"
@@ -71,6 +73,70 @@ __global__ __launch_bounds__(kBlock) void k_eval(const Fr *polys, size_t len, co
         for (int w = 0; w < 8; ++w) s.v[w] = sh[w][0];
         store_fr(partial + ((size_t)poly * npts + pt) * gridDim.x + blockIdx.x, s);
     }
+}
+
+// partial[query * nblk + block]: query q evaluates polys[qpoly[q]] at
+// point qpt[q] (power tables tabs[pt])
+__global__ __launch_bounds__(kBlock) void k_eval_queries(const Fr *const *polys, const uint32_t *qpoly,
+                                                         const uint32_t *qpt, size_t len, const Fr *tabs,
+                                                         uint32_t levels, Fr *partial)
+{
+    __shared__ uint32_t sh[8][kBlock];
+    const uint32_t q = blockIdx.y;
+    const Fr *p = polys[qpoly[q]];
+    const Fr *tab = tabs + (size_t)qpt[q] * levels * kOmegaTab;
+    const size_t start = ((size_t)blockIdx.x * kBlock + threadIdx.x) * kChunk;
+    Fr acc = Fr::zero();
+    if (start < len) {
+        const size_t end = std::min(start + (size_t)kChunk, len);
+        const Fr x = load_fr(tab + 1);
+        for (size_t i = end; i-- > start;) acc = acc * x + load_fr(p + i);
+        acc = acc * omega_pow_tab(start, tab, levels);
+    }
+#pragma unroll
+    for (int w = 0; w < 8; ++w) sh[w][threadIdx.x] = acc.v[w];
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            Fr a, b;
+#pragma unroll
+            for (int w = 0; w < 8; ++w) {
+                a.v[w] = sh[w][threadIdx.x];
+                b.v[w] = sh[w][threadIdx.x + off];
+            }
+            a = a + b;
+#pragma unroll
+            for (int w = 0; w < 8; ++w) sh[w][threadIdx.x] = a.v[w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        Fr s;
+#pragma unroll
+        for (int w = 0; w < 8; ++w) s.v[w] = sh[w][0];
+        store_fr(partial + (size_t)q * gridDim.x + blockIdx.x, s);
+    }
+}
+
+// out[q * len + i] = sum over items t of q: coefficients[t] * polys[poly_idx[t]][i]
+__global__ __launch_bounds__(kBlock) void k_q_polys(Fr *out, const Fr *const *polys, const uint32_t *offsets,
+                                                    const uint32_t *poly_idx, const Fr *coefficients, size_t len,
+                                                    size_t q_count)
+{
+    const size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (id >= len * q_count) return;
+    const size_t q = id / len, i = id - q * len;
+    Fr acc = Fr::zero();
+    for (uint32_t t = offsets[q]; t < offsets[q + 1]; ++t)
+        acc = acc + load_fr(polys[poly_idx[t]] + i) * load_fr(coefficients + t);
+    store_fr(out + id, acc);
+}
+
+// q[i] = p[i + 1], q[len - 1] = 0 (division by X)
+__global__ __launch_bounds__(kBlock) void k_shift_down(Fr *q, const Fr *p, size_t len)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < len) store_fr(q + i, i + 1 < len ? load_fr(p + i + 1) : Fr::zero());
 }
 
 // out[i] = constant (i == 0) + sum_j s_j p_j[i]
@@ -222,8 +288,21 @@ void poly_lincomb(Fr *out, const std::vector<const Fr *> &polys, const std::vect
 
 void poly_kate_divide(Fr *q, const Fr *p, size_t len, const HostFr &a, hipStream_t st)
 {
-    MH_CHECK(!a.is_zero(), (int)hipErrorInvalidValue, "division point is zero");
-    if (len < 2) return;
+    if (len == 0) return;
+    if (a.is_zero()) {   // division by X: a shift (through a copy, q may alias p)
+        DevBuf<Fr> w(len);
+        MH_HIP(hipMemcpyAsync(w.get(), p, len * 32, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_shift_down, dim3((unsigned)ceil_div(len, kBlock)), dim3(kBlock), 0, st, q,
+                           (const Fr *)w.get(), len);
+        MH_HIP(hipGetLastError());
+        MH_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    if (len < 2) {   // the quotient of a constant is zero
+        MH_HIP(hipMemsetAsync(q, 0, 32, st));
+        MH_HIP(hipStreamSynchronize(st));
+        return;
+    }
     DevBuf<Fr> tab, inv_tab, w(len);
     uint32_t levels = 1, levels2 = 1;
     build_omega_table(tab, levels, a, len, st);
@@ -244,4 +323,324 @@ void poly_kate_divide(Fr *q, const Fr *p, size_t len, const HostFr &a, hipStream
     MH_HIP(hipStreamSynchronize(st));
 }
 
+void poly_eval_queries(HostFr *out, const std::vector<const Fr *> &polys, size_t len, const uint32_t *qpoly,
+                       const uint32_t *qpt, size_t nq, const HostFr *points, size_t npts, hipStream_t st)
+{
+    if (nq == 0) return;
+    if (len == 0) {
+        for (size_t q = 0; q < nq; ++q) out[q] = HostFr::zero();
+        return;
+    }
+    std::vector<DevBuf<Fr>> tabs(npts);
+    uint32_t levels = 1;
+    for (size_t p = 0; p < npts; ++p) build_omega_table(tabs[p], levels, points[p], len, st);
+    DevBuf<Fr> all(std::max<size_t>((size_t)npts * levels * kOmegaTab, 1));
+    for (size_t p = 0; p < npts; ++p)
+        MH_HIP(hipMemcpyAsync(all.get() + p * levels * kOmegaTab, tabs[p].get(), levels * kOmegaTab * 32,
+                              hipMemcpyDeviceToDevice, st));
+    DevBuf<const Fr *> d_polys(polys.size());
+    DevBuf<uint32_t> d_q(2 * nq);
+    MH_HIP(hipMemcpyAsync(d_polys.get(), polys.data(), polys.size() * sizeof(Fr *), hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(d_q.get(), qpoly, nq * 4, hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(d_q.get() + nq, qpt, nq * 4, hipMemcpyHostToDevice, st));
+    const size_t nblk = ceil_div(len, (size_t)kBlock * kChunk);
+    MH_CHECK(nq < 65536, (int)hipErrorInvalidValue, "too many evaluation queries");
+    DevBuf<Fr> partial(nq * nblk);
+    hipLaunchKernelGGL(k_eval_queries, dim3((unsigned)nblk, (unsigned)nq), dim3(kBlock), 0, st,
+                       (const Fr *const *)d_polys.get(), (const uint32_t *)d_q.get(),
+                       (const uint32_t *)d_q.get() + nq, len, (const Fr *)all.get(), levels, partial.get());
+    MH_HIP(hipGetLastError());
+    std::vector<HostFr> h(nq * nblk);
+    MH_HIP(hipMemcpyAsync(h.data(), partial.get(), h.size() * 32, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    for (size_t q = 0; q < nq; ++q) {
+        HostFr sum = HostFr::zero();
+        for (size_t b = 0; b < nblk; ++b) sum = sum + h[q * nblk + b];
+        out[q] = sum;
+    }
+}
+
+void poly_q_combine(Fr *out, const std::vector<const Fr *> &polys, size_t len, const uint32_t *offsets,
+                    size_t q_count, const uint32_t *poly_idx, const HostFr *coefficients, size_t items,
+                    hipStream_t st)
+{
+    if (len == 0 || q_count == 0) return;
+    DevBuf<const Fr *> d_polys(std::max<size_t>(polys.size(), 1));
+    DevBuf<uint32_t> d_idx(q_count + 1 + std::max<size_t>(items, 1));
+    DevBuf<Fr> d_coef(std::max<size_t>(items, 1));
+    if (!polys.empty())
+        MH_HIP(hipMemcpyAsync(d_polys.get(), polys.data(), polys.size() * sizeof(Fr *), hipMemcpyHostToDevice, st));
+    MH_HIP(hipMemcpyAsync(d_idx.get(), offsets, (q_count + 1) * 4, hipMemcpyHostToDevice, st));
+    if (items) {
+        MH_HIP(hipMemcpyAsync(d_idx.get() + q_count + 1, poly_idx, items * 4, hipMemcpyHostToDevice, st));
+        MH_HIP(hipMemcpyAsync(d_coef.get(), coefficients, items * 32, hipMemcpyHostToDevice, st));
+    }
+    hipLaunchKernelGGL(k_q_polys, dim3((unsigned)ceil_div(len * q_count, kBlock)), dim3(kBlock), 0, st, out,
+                       (const Fr *const *)d_polys.get(), (const uint32_t *)d_idx.get(),
+                       (const uint32_t *)d_idx.get() + q_count + 1, (const Fr *)d_coef.get(), len, q_count);
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipStreamSynchronize(st));
+}
+
 }  // namespace mh
+
+// ---------------------------------------------------------------- C ABI
+
+using namespace mh;
+
+#define MH_TRY try {
+#define MH_CATCH                                                     \
+    }                                                                \
+    catch (const hip_error &e) { return make_error(e.code, e.what()); } \
+    catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
+
+namespace {
+
+std::unique_ptr<DeviceVec> new_devvec(size_t len)
+{
+    auto dv = std::make_unique<DeviceVec>();
+    dv->lane = current_lane();
+    dv->len = len;
+    dv->values.alloc(std::max<size_t>(len, 1));
+    return dv;
+}
+
+std::vector<HostFr> host_frs(const RawFr *v, size_t n)
+{
+    std::vector<HostFr> h(n);
+    for (size_t i = 0; i < n; ++i) h[i] = to_hfr(v[i]);
+    return h;
+}
+
+// the q-polynomial and query arguments shared by both multi-open preparations
+const char *check_multiopen(size_t poly_count, const uint32_t *q_offsets, size_t q_offsets_len,
+                            const uint32_t *q_poly_indices, size_t q_items_len, size_t point_count,
+                            const uint32_t *query_poly_indices, const uint32_t *query_point_indices,
+                            size_t query_count, bool resident)
+{
+    const size_t q_count = q_offsets_len - 1;
+    if (q_offsets[0] != 0 || q_offsets[q_count] != q_items_len)
+        return resident ? "invalid resident multi-open q offsets" : "invalid multi-open q offsets";
+    for (size_t i = 0; i < q_count; ++i)
+        if (q_offsets[i] > q_offsets[i + 1])
+            return resident ? "non-monotonic resident q offsets" : "non-monotonic multi-open q offsets";
+    for (size_t i = 0; i < q_items_len; ++i)
+        if (q_poly_indices[i] >= poly_count)
+            return resident ? "resident q polynomial index out of range" : "multi-open q polynomial index out of range";
+    for (size_t i = 0; i < query_count; ++i)
+        if (query_poly_indices[i] >= poly_count || query_point_indices[i] >= point_count)
+            return resident ? "resident query index out of range" : "multi-open query index out of range";
+    return nullptr;
+}
+
+}  // namespace
+
+// bls12_381_g1_msm.cu:3558-3598: out = sum_j scalars[j] * polys[j] (host arrays)
+extern "C" RustError midnight_sppark_bls12_381_fr_poly_lincomb(RawFr *out, const RawFr *polys, size_t poly_len,
+                                                               size_t poly_count, const RawFr *scalars)
+{
+    if (poly_len == 0 || poly_count == 0) return ok();
+    if (!out || !polys || !scalars) return make_error((int)hipErrorInvalidValue, "invalid polynomial combination");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<Fr> d_polys(poly_len * poly_count), d_out(poly_len);
+    MH_HIP(hipMemcpyAsync(d_polys.get(), polys, poly_len * poly_count * 32, hipMemcpyHostToDevice, st));
+    std::vector<const Fr *> ptrs(poly_count);
+    for (size_t j = 0; j < poly_count; ++j) ptrs[j] = d_polys.get() + j * poly_len;
+    poly_lincomb(d_out.get(), ptrs, host_frs(scalars, poly_count), poly_len, HostFr::zero(), st);
+    MH_HIP(hipMemcpyAsync(out, d_out.get(), poly_len * 32, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:3234-3278: new device vector sum_j scalars[j] * polys[j]
+extern "C" RustError midnight_sppark_bls12_381_fr_device_poly_lincomb(void **out, const void *polys_vec,
+                                                                      size_t poly_len, size_t poly_count,
+                                                                      const RawFr *scalars, size_t scalar_count)
+{
+    if (out == nullptr || polys_vec == nullptr || poly_len == 0 || poly_count == 0 || scalar_count != poly_count ||
+        scalars == nullptr || poly_count > SIZE_MAX / poly_len)
+        return make_error((int)hipErrorInvalidValue, "invalid resident polynomial combination");
+    *out = nullptr;
+    const auto *polys = static_cast<const DeviceVec *>(polys_vec);
+    if (polys->len != poly_len * poly_count)
+        return make_error((int)hipErrorInvalidValue, "resident polynomial combination shape mismatch");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    auto dv = new_devvec(poly_len);
+    std::vector<const Fr *> ptrs(poly_count);
+    for (size_t j = 0; j < poly_count; ++j) ptrs[j] = polys->values.get() + j * poly_len;
+    poly_lincomb(dv->values.get(), ptrs, host_frs(scalars, scalar_count), poly_len, HostFr::zero(), st);
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:3985-4023: out[j] = poly_j(point) for a contiguous set
+extern "C" RustError midnight_sppark_bls12_381_fr_device_eval_polys(RawFr *out, const void *device_vec,
+                                                                    size_t poly_len, size_t poly_count, RawFr point)
+{
+    if (out == nullptr || device_vec == nullptr || poly_len == 0 || poly_count == 0 ||
+        poly_count > SIZE_MAX / poly_len)
+        return make_error((int)hipErrorInvalidValue, "invalid resident polynomial evaluation arguments");
+    const auto *polys = static_cast<const DeviceVec *>(device_vec);
+    if (polys->len != poly_len * poly_count)
+        return make_error((int)hipErrorInvalidValue, "resident polynomial evaluation shape mismatch");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    const HostFr x = to_hfr(point);
+    poly_eval_many(reinterpret_cast<HostFr *>(out), polys->values.get(), poly_len, poly_count, &x, 1, st);
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:3600-3721: q polynomials q[k] = sum of coefficient *
+// poly over q k's items, and the query evaluations, from host polynomials
+extern "C" RustError midnight_sppark_bls12_381_fr_multiopen_prepare(
+    RawFr *q_out, RawFr *eval_out, const RawFr *polys, size_t poly_len, size_t poly_count, const uint32_t *q_offsets,
+    size_t q_offsets_len, const uint32_t *q_poly_indices, const RawFr *q_coefficients, size_t q_items_len,
+    const RawFr *points, size_t point_count, const uint32_t *query_poly_indices, const uint32_t *query_point_indices,
+    size_t query_count)
+{
+    if (poly_len == 0 || poly_count == 0 || q_offsets_len < 2 || q_out == nullptr || eval_out == nullptr ||
+        polys == nullptr || q_offsets == nullptr || q_poly_indices == nullptr || q_coefficients == nullptr ||
+        points == nullptr || point_count == 0 || query_poly_indices == nullptr || query_point_indices == nullptr)
+        return make_error((int)hipErrorInvalidValue, "invalid multi-open preparation arguments");
+    if (const char *e = check_multiopen(poly_count, q_offsets, q_offsets_len, q_poly_indices, q_items_len,
+                                        point_count, query_poly_indices, query_point_indices, query_count, false))
+        return make_error((int)hipErrorInvalidValue, e);
+    const size_t q_count = q_offsets_len - 1;
+    if (poly_count > SIZE_MAX / poly_len || q_count > SIZE_MAX / poly_len || point_count > SIZE_MAX / poly_len)
+        return make_error((int)hipErrorInvalidValue, "multi-open preparation size overflow");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<Fr> d_polys(poly_len * poly_count), d_q(poly_len * q_count);
+    MH_HIP(hipMemcpyAsync(d_polys.get(), polys, poly_len * poly_count * 32, hipMemcpyHostToDevice, st));
+    std::vector<const Fr *> ptrs(poly_count);
+    for (size_t j = 0; j < poly_count; ++j) ptrs[j] = d_polys.get() + j * poly_len;
+    poly_q_combine(d_q.get(), ptrs, poly_len, q_offsets, q_count, q_poly_indices,
+                   host_frs(q_coefficients, q_items_len).data(), q_items_len, st);
+    const auto pts = host_frs(points, point_count);
+    poly_eval_queries(reinterpret_cast<HostFr *>(eval_out), ptrs, poly_len, query_poly_indices, query_point_indices,
+                      query_count, pts.data(), point_count, st);
+    MH_HIP(hipMemcpyAsync(q_out, d_q.get(), poly_len * q_count * 32, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:3723-3871: the same from resident polynomials named by
+// (source, index) in three device vectors (0 dynamic, 1 fixed, 2 extra); the
+// q polynomials go to the host (q_out) and/or a new device vector
+extern "C" RustError midnight_sppark_bls12_381_fr_multiopen_prepare_device(
+    RawFr *q_out, void **q_device_out, RawFr *eval_out, const void *dynamic_source, const void *fixed_source,
+    const void *extra_source, size_t poly_len, const RawDevicePolyRef *poly_refs, size_t poly_count,
+    const uint32_t *q_offsets, size_t q_offsets_len, const uint32_t *q_poly_indices, const RawFr *q_coefficients,
+    size_t q_items_len, const RawFr *points, size_t point_count, const uint32_t *query_poly_indices,
+    const uint32_t *query_point_indices, size_t query_count)
+{
+    if (poly_len == 0 || poly_count == 0 || q_offsets_len < 2 || (q_out == nullptr && q_device_out == nullptr) ||
+        eval_out == nullptr || dynamic_source == nullptr || fixed_source == nullptr || extra_source == nullptr ||
+        poly_refs == nullptr || q_offsets == nullptr || q_poly_indices == nullptr || q_coefficients == nullptr ||
+        points == nullptr || point_count == 0 || query_poly_indices == nullptr || query_point_indices == nullptr)
+        return make_error((int)hipErrorInvalidValue, "invalid resident multi-open arguments");
+    if (q_device_out) *q_device_out = nullptr;
+    const DeviceVec *sources[3] = {static_cast<const DeviceVec *>(dynamic_source),
+                                   static_cast<const DeviceVec *>(fixed_source),
+                                   static_cast<const DeviceVec *>(extra_source)};
+    for (size_t i = 0; i < poly_count; ++i) {
+        const RawDevicePolyRef r = poly_refs[i];
+        if (r.source > 2 || (size_t)r.index >= sources[r.source]->len / poly_len)
+            return make_error((int)hipErrorInvalidValue, "resident polynomial reference out of range");
+    }
+    if (const char *e = check_multiopen(poly_count, q_offsets, q_offsets_len, q_poly_indices, q_items_len,
+                                        point_count, query_poly_indices, query_point_indices, query_count, true))
+        return make_error((int)hipErrorInvalidValue, e);
+    const size_t q_count = q_offsets_len - 1;
+    MH_TRY
+    hipStream_t st = cur_stream();
+    std::vector<const Fr *> ptrs(poly_count);
+    for (size_t j = 0; j < poly_count; ++j)
+        ptrs[j] = sources[poly_refs[j].source]->values.get() + (size_t)poly_refs[j].index * poly_len;
+    auto q = new_devvec(poly_len * q_count);
+    poly_q_combine(q->values.get(), ptrs, poly_len, q_offsets, q_count, q_poly_indices,
+                   host_frs(q_coefficients, q_items_len).data(), q_items_len, st);
+    const auto pts = host_frs(points, point_count);
+    poly_eval_queries(reinterpret_cast<HostFr *>(eval_out), ptrs, poly_len, query_poly_indices, query_point_indices,
+                      query_count, pts.data(), point_count, st);
+    if (q_out) MH_HIP(hipMemcpyAsync(q_out, q->values.get(), poly_len * q_count * 32, hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    if (q_device_out) *q_device_out = q.release();
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:3873-3983: f = sum_k scalars[k] * (q_k divided by
+// (X - a) for every point a of q_k's set, remainders dropped)
+extern "C" RustError midnight_sppark_bls12_381_fr_multiopen_f_poly_device(
+    void **out, const void *q_device_vec, size_t poly_len, const uint32_t *point_offsets, size_t point_offsets_len,
+    const RawFr *points, size_t point_count, const RawFr *combination_scalars, size_t scalar_count)
+{
+    if (out == nullptr || q_device_vec == nullptr || poly_len < 2 || point_offsets == nullptr ||
+        point_offsets_len < 2 || points == nullptr || combination_scalars == nullptr)
+        return make_error((int)hipErrorInvalidValue, "invalid resident f-polynomial arguments");
+    *out = nullptr;
+    const size_t q_count = point_offsets_len - 1;
+    const auto *q_polys = static_cast<const DeviceVec *>(q_device_vec);
+    if (q_count != scalar_count || q_count > SIZE_MAX / poly_len || q_polys->len != q_count * poly_len ||
+        point_offsets[0] != 0 || point_offsets[q_count] != point_count)
+        return make_error((int)hipErrorInvalidValue, "resident f-polynomial shape mismatch");
+    for (size_t i = 0; i < q_count; ++i)
+        if (point_offsets[i] > point_offsets[i + 1])
+            return make_error((int)hipErrorInvalidValue, "non-monotonic f-polynomial point offsets");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    DevBuf<Fr> divided(q_count * poly_len);
+    MH_HIP(hipMemcpyAsync(divided.get(), q_polys->values.get(), q_count * poly_len * 32, hipMemcpyDeviceToDevice,
+                          st));
+    std::vector<const Fr *> ptrs(q_count);
+    for (size_t k = 0; k < q_count; ++k) {
+        Fr *d = divided.get() + k * poly_len;
+        for (uint32_t p = point_offsets[k]; p < point_offsets[k + 1]; ++p) poly_kate_divide(d, d, poly_len, to_hfr(points[p]), st);
+        ptrs[k] = d;
+    }
+    auto f = new_devvec(poly_len);
+    poly_lincomb(f->values.get(), ptrs, host_frs(combination_scalars, scalar_count), poly_len, HostFr::zero(), st);
+    *out = f.release();
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:4025-4110: final = sum_k s_k q_k + s_q f; pi = the
+// quotient of (final - final(point)) by (X - point): poly_len - 1 values
+extern "C" RustError midnight_sppark_bls12_381_fr_multiopen_pi_device(void **out, const void *q_device_vec,
+                                                                      const void *f_device_vec, size_t poly_len,
+                                                                      const RawFr *combination_scalars,
+                                                                      size_t scalar_count, RawFr point)
+{
+    if (out == nullptr || q_device_vec == nullptr || f_device_vec == nullptr || poly_len < 2 ||
+        combination_scalars == nullptr || scalar_count < 2)
+        return make_error((int)hipErrorInvalidValue, "invalid resident pi-polynomial arguments");
+    *out = nullptr;
+    const size_t q_count = scalar_count - 1;
+    const auto *q_polys = static_cast<const DeviceVec *>(q_device_vec);
+    const auto *f_poly = static_cast<const DeviceVec *>(f_device_vec);
+    if (q_count > SIZE_MAX / poly_len || q_polys->len != q_count * poly_len || f_poly->len != poly_len)
+        return make_error((int)hipErrorInvalidValue, "resident pi-polynomial shape mismatch");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    std::vector<const Fr *> ptrs(scalar_count);
+    for (size_t k = 0; k < q_count; ++k) ptrs[k] = q_polys->values.get() + k * poly_len;
+    ptrs[q_count] = f_poly->values.get();
+    DevBuf<Fr> fin(poly_len);
+    poly_lincomb(fin.get(), ptrs, host_frs(combination_scalars, scalar_count), poly_len, HostFr::zero(), st);
+    // the constant term does not enter the quotient, so final(point) need not be subtracted
+    poly_kate_divide(fin.get(), fin.get(), poly_len, to_hfr(point), st);
+    auto pi = new_devvec(poly_len - 1);
+    MH_HIP(hipMemcpyAsync(pi->values.get(), fin.get(), (poly_len - 1) * 32, hipMemcpyDeviceToDevice, st));
+    MH_HIP(hipStreamSynchronize(st));
+    *out = pi.release();
+    return ok();
+    MH_CATCH
+}

@@ -153,6 +153,33 @@ def _declare(L):
         "midnight_sppark_bls12_381_fr_shifted_prefix_product": (E, [u64p, RawFr, c_size]),
         "midnight_sppark_bls12_381_fr_permutation_products": (
             E, [vpp, u64p, u64p, c_size, c_size, c_size, RawFr, RawFr, RawFr, u64p, vp, c_size]),
+        "midnight_sppark_bls12_381_fr_device_vec_scatter_host": (E, [vp, ctypes.POINTER(u32), vp, c_size]),
+        "midnight_sppark_bls12_381_fr_device_vec_concat_host": (E, [vpp, vp, vp, c_size]),
+        "midnight_sppark_bls12_381_fr_device_vec_concat_many_host": (E, [vpp, vpp, c_size, vp, c_size]),
+        "midnight_sppark_bls12_381_fr_device_vec_gather_polys": (E, [vpp, vp, c_size, ctypes.POINTER(u32), c_size]),
+        "midnight_sppark_bls12_381_fr_device_vec_download_nonzero_suffixes": (
+            E, [u64p, c_size, ctypes.POINTER(c_size), ctypes.POINTER(c_size), ctypes.POINTER(c_size), vp, c_size,
+                c_size, c_size]),
+        "midnight_sppark_bls12_381_fr_poly_lincomb": (E, [u64p, u64p, c_size, c_size, u64p]),
+        "midnight_sppark_bls12_381_fr_device_poly_lincomb": (E, [vpp, vp, c_size, c_size, u64p, c_size]),
+        "midnight_sppark_bls12_381_fr_device_eval_polys": (E, [u64p, vp, c_size, c_size, RawFr]),
+        "midnight_sppark_bls12_381_fr_multiopen_prepare": (
+            E, [u64p, u64p, u64p, c_size, c_size, ctypes.POINTER(u32), c_size, ctypes.POINTER(u32), u64p, c_size,
+                u64p, c_size, ctypes.POINTER(u32), ctypes.POINTER(u32), c_size]),
+        "midnight_sppark_bls12_381_fr_multiopen_prepare_device": (
+            E, [vp, vpp, u64p, vp, vp, vp, c_size, ctypes.POINTER(RawDevicePolyRef), c_size, ctypes.POINTER(u32),
+                c_size, ctypes.POINTER(u32), u64p, c_size, u64p, c_size, ctypes.POINTER(u32), ctypes.POINTER(u32),
+                c_size]),
+        "midnight_sppark_bls12_381_fr_multiopen_f_poly_device": (
+            E, [vpp, vp, c_size, ctypes.POINTER(u32), c_size, u64p, c_size, u64p, c_size]),
+        "midnight_sppark_bls12_381_fr_multiopen_pi_device": (E, [vpp, vp, vp, c_size, u64p, c_size, RawFr]),
+        "midnight_sppark_bls12_381_fr_invert_assign": (E, [u64p, c_size]),
+        "midnight_sppark_bls12_381_fr_lookup_products": (
+            E, [vpp, u64p, u64p, u64p, u64p, c_size, c_size, RawFr, RawFr, vp, c_size]),
+        "midnight_sppark_bls12_381_fr_lookup_products_device": (
+            E, [vpp, vp, c_size, c_size, RawFr, RawFr, vp, c_size]),
+        "midnight_sppark_bls12_381_trace_context_trash": (
+            E, [vpp, vp, vp, c_size, vp, c_size, vp, c_size, RawFr]),
         "midnight_sppark_bls12_381_trace_context_permutation": (
             E, [vpp, vp, vp, c_size, vp, c_size, RawFr, RawFr, u64p, c_size, vp, c_size, c_size]),
         "midnight_hip_plonk_keygen": (
@@ -192,6 +219,16 @@ def check(err):
 def P(a):
     assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"], "need C-contiguous uint64"
     return a.ctypes.data_as(u64p)
+
+
+class RawDevicePolyRef(ctypes.Structure):
+    """(source, index): source 0 dynamic, 1 fixed, 2 extra device vector."""
+    _fields_ = [("source", ctypes.c_uint32), ("index", ctypes.c_uint32)]
+
+
+def _u32(values):
+    a = np.ascontiguousarray(values, dtype=np.uint32).reshape(-1)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
 
 
 def _fr_array(values):
@@ -320,6 +357,60 @@ class DeviceVec:
         offs = (c_size * n)(*offsets)
         logs = (ctypes.c_uint32 * n)(*log_ns)
         check(lib().midnight_hip_fr_device_vec_ntt_multi(self.handle, offs, logs, n, inverse))
+
+    def scatter_host(self, indices, values):
+        idx, pidx = _u32(indices)
+        v = _fr_array(values) if len(idx) else np.zeros((0, 4), dtype=np.uint64)
+        check(lib().midnight_sppark_bls12_381_fr_device_vec_scatter_host(self.handle, pidx, _ptr(v), len(idx)))
+
+    def concat_host(self, suffix):
+        sfx = _fr_array(suffix) if len(suffix) else None
+        h = ctypes.c_void_p()
+        check(lib().midnight_sppark_bls12_381_fr_device_vec_concat_host(
+            ctypes.byref(h), self.handle, _ptr(sfx), 0 if sfx is None else sfx.shape[0]))
+        return DeviceVec(h.value)
+
+    @classmethod
+    def concat_many_host(cls, vecs, suffix):
+        arr = (ctypes.c_void_p * len(vecs))(*[v.handle.value for v in vecs])
+        sfx = _fr_array(suffix) if len(suffix) else None
+        h = ctypes.c_void_p()
+        check(lib().midnight_sppark_bls12_381_fr_device_vec_concat_many_host(
+            ctypes.byref(h), ctypes.cast(arr, ctypes.POINTER(ctypes.c_void_p)), len(vecs), _ptr(sfx),
+            0 if sfx is None else sfx.shape[0]))
+        return cls(h.value)
+
+    def gather_polys(self, poly_len, indices):
+        idx, pidx = _u32(indices)
+        h = ctypes.c_void_p()
+        check(lib().midnight_sppark_bls12_381_fr_device_vec_gather_polys(
+            ctypes.byref(h), self.handle, poly_len, pidx, len(idx)))
+        return DeviceVec(h.value)
+
+    def download_nonzero_suffixes(self, scalar_offset, poly_len, count):
+        """(concatenated suffixes, starts, nonzero counts)"""
+        starts = (c_size * count)()
+        counts = (c_size * count)()
+        total = c_size()
+        check(lib().midnight_sppark_bls12_381_fr_device_vec_download_nonzero_suffixes(
+            None, 0, ctypes.byref(total), starts, counts, self.handle, scalar_offset, poly_len, count))
+        out = np.zeros((max(total.value, 1), 4), dtype=np.uint64)
+        check(lib().midnight_sppark_bls12_381_fr_device_vec_download_nonzero_suffixes(
+            P(out), out.shape[0], ctypes.byref(total), starts, counts, self.handle, scalar_offset, poly_len, count))
+        return out[:total.value], list(starts), list(counts)
+
+    def poly_lincomb(self, poly_len, poly_count, scalars):
+        sc = _fr_array(scalars)
+        h = ctypes.c_void_p()
+        check(lib().midnight_sppark_bls12_381_fr_device_poly_lincomb(
+            ctypes.byref(h), self.handle, poly_len, poly_count, P(sc), sc.shape[0]))
+        return DeviceVec(h.value)
+
+    def eval_polys(self, poly_len, poly_count, point):
+        out = np.zeros((poly_count, 4), dtype=np.uint64)
+        check(lib().midnight_sppark_bls12_381_fr_device_eval_polys(P(out), self.handle, poly_len, poly_count,
+                                                                   RawFr.of(point)))
+        return out
 
     def quotient_pieces(self, poly_len, piece_count, blinds):
         bl = _fr_array(blinds)
@@ -702,6 +793,16 @@ class NumeratorContext:
             P(dp), dp.shape[0], _ptr(bl), 0 if bl is None else bl.shape[0], blinding_factors))
         return DeviceVec(h.value)
 
+    def trace_trash(self, advice, advice_cols, instance, instance_cols, challenges, trash_challenge):
+        """trace_context_trash: the trash graphs on the Lagrange domain; a DeviceVec."""
+        ch = None if challenges is None or len(challenges) == 0 else _fr_array(challenges)
+        h = ctypes.c_void_p()
+        check(lib().midnight_sppark_bls12_381_trace_context_trash(
+            ctypes.byref(h), self.handle, advice.handle if advice is not None else None, advice_cols,
+            instance.handle if instance is not None else None, instance_cols, _ptr(ch),
+            0 if ch is None else ch.shape[0], RawFr.of(trash_challenge)))
+        return DeviceVec(h.value)
+
     def free(self):
         if self.handle and self.handle.value:
             lib().midnight_sppark_bls12_381_numerator_context_free(self.handle)
@@ -734,6 +835,97 @@ def fr_permutation_products(columns, permutations, n, chunk_len, *, beta, gamma,
     check(lib().midnight_sppark_bls12_381_fr_permutation_products(
         ctypes.byref(h), P(cols), P(perm), n, cols.shape[0] // n, chunk_len, RawFr.of(beta), RawFr.of(gamma),
         RawFr.of(omega), P(dp), _ptr(bl), blinding_factors))
+    return DeviceVec(h.value)
+
+
+def fr_invert_assign(values):
+    a = np.array(_fr_array(values), copy=True)
+    check(lib().midnight_sppark_bls12_381_fr_invert_assign(P(a), a.shape[0]))
+    return a
+
+
+def fr_lookup_products(ci, ct, pi, pt, n, count, *, beta, gamma, blinds, blinding_factors):
+    """Lookup product columns (count * n values) as a DeviceVec."""
+    arrs = [_fr_array(x) for x in (ci, ct, pi, pt)]
+    bl = None if blinding_factors == 0 else _fr_array(blinds)
+    h = ctypes.c_void_p()
+    check(lib().midnight_sppark_bls12_381_fr_lookup_products(
+        ctypes.byref(h), *[P(a) for a in arrs], n, count, RawFr.of(beta), RawFr.of(gamma), _ptr(bl),
+        blinding_factors))
+    return DeviceVec(h.value)
+
+
+def fr_lookup_products_device(trace, n, count, *, beta, gamma, blinds, blinding_factors):
+    bl = None if blinding_factors == 0 else _fr_array(blinds)
+    h = ctypes.c_void_p()
+    check(lib().midnight_sppark_bls12_381_fr_lookup_products_device(
+        ctypes.byref(h), trace.handle, n, count, RawFr.of(beta), RawFr.of(gamma), _ptr(bl), blinding_factors))
+    return DeviceVec(h.value)
+
+
+def fr_poly_lincomb(polys, poly_len, scalars):
+    pl = _fr_array(polys)
+    sc = _fr_array(scalars)
+    out = np.zeros((poly_len, 4), dtype=np.uint64)
+    check(lib().midnight_sppark_bls12_381_fr_poly_lincomb(P(out), P(pl), poly_len, sc.shape[0], P(sc)))
+    return out
+
+
+def fr_multiopen_prepare(polys, poly_len, q_offsets, q_poly_indices, q_coefficients, points, query_poly_indices,
+                         query_point_indices):
+    """(q polynomials (q_count * poly_len), evaluations) from host polynomials."""
+    pl = _fr_array(polys)
+    qo, pqo = _u32(q_offsets)
+    qi, pqi = _u32(q_poly_indices)
+    qc = _fr_array(q_coefficients) if len(q_coefficients) else np.zeros((1, 4), dtype=np.uint64)
+    pts = _fr_array(points)
+    qp, pqp = _u32(query_poly_indices)
+    qx, pqx = _u32(query_point_indices)
+    q_count = len(qo) - 1
+    q_out = np.zeros((q_count * poly_len, 4), dtype=np.uint64)
+    ev = np.zeros((max(len(qp), 1), 4), dtype=np.uint64)
+    check(lib().midnight_sppark_bls12_381_fr_multiopen_prepare(
+        P(q_out), P(ev), P(pl), poly_len, pl.shape[0] // poly_len, pqo, len(qo), pqi, P(qc), len(qi), P(pts),
+        pts.shape[0], pqp, pqx, len(qp)))
+    return q_out, ev[:len(qp)]
+
+
+def fr_multiopen_prepare_device(dynamic, fixed, extra, poly_len, poly_refs, q_offsets, q_poly_indices,
+                                q_coefficients, points, query_poly_indices, query_point_indices, host_q=True):
+    """(q polynomials on the host or None, q DeviceVec, evaluations) from resident polynomials;
+    poly_refs: (source, index) pairs."""
+    refs = (RawDevicePolyRef * len(poly_refs))(*[RawDevicePolyRef(a, b) for a, b in poly_refs])
+    qo, pqo = _u32(q_offsets)
+    qi, pqi = _u32(q_poly_indices)
+    qc = _fr_array(q_coefficients) if len(q_coefficients) else np.zeros((1, 4), dtype=np.uint64)
+    pts = _fr_array(points)
+    qp, pqp = _u32(query_poly_indices)
+    qx, pqx = _u32(query_point_indices)
+    q_count = len(qo) - 1
+    q_out = np.zeros((q_count * poly_len, 4), dtype=np.uint64) if host_q else None
+    ev = np.zeros((max(len(qp), 1), 4), dtype=np.uint64)
+    h = ctypes.c_void_p()
+    check(lib().midnight_sppark_bls12_381_fr_multiopen_prepare_device(
+        _ptr(q_out), ctypes.byref(h), P(ev), dynamic.handle, fixed.handle, extra.handle, poly_len, refs,
+        len(poly_refs), pqo, len(qo), pqi, P(qc), len(qi), P(pts), pts.shape[0], pqp, pqx, len(qp)))
+    return q_out, DeviceVec(h.value), ev[:len(qp)]
+
+
+def fr_multiopen_f_poly_device(q_vec, poly_len, point_offsets, points, scalars):
+    po, ppo = _u32(point_offsets)
+    pts = _fr_array(points)
+    sc = _fr_array(scalars)
+    h = ctypes.c_void_p()
+    check(lib().midnight_sppark_bls12_381_fr_multiopen_f_poly_device(
+        ctypes.byref(h), q_vec.handle, poly_len, ppo, len(po), P(pts), pts.shape[0], P(sc), sc.shape[0]))
+    return DeviceVec(h.value)
+
+
+def fr_multiopen_pi_device(q_vec, f_vec, poly_len, scalars, point):
+    sc = _fr_array(scalars)
+    h = ctypes.c_void_p()
+    check(lib().midnight_sppark_bls12_381_fr_multiopen_pi_device(
+        ctypes.byref(h), q_vec.handle, f_vec.handle, poly_len, P(sc), sc.shape[0], RawFr.of(point)))
     return DeviceVec(h.value)
 
 

@@ -200,6 +200,12 @@ void orc_permutation_products(orc_fr *out, const orc_fr *columns, const orc_fr *
                               const orc_fr *omega, const orc_fr *delta_powers, const orc_fr *blinds,
                               size_t blinding, int threads);
 
+/* Lookup product columns (lookup/prover.rs:1075-1178); inputs and out lookup-major, count * len values;
+ * blinds: count * blinding values. */
+void orc_lookup_products(orc_fr *out, const orc_fr *ci, const orc_fr *ct, const orc_fr *pi, const orc_fr *pt,
+                         size_t len, size_t count, const orc_fr *beta, const orc_fr *gamma, const orc_fr *blinds,
+                         size_t blinding);
+
 /* shifted_prefix_product (permutation/prover.rs:36-43); out may not alias values */
 void orc_shifted_prefix_product(orc_fr *out, const orc_fr *values, size_t len, const orc_fr *initial);
 

@@ -103,6 +103,8 @@ def _declare(L):
                                    ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32)]),
         "orc_eval_numerator": (V, [u64p, ctypes.c_void_p, ctypes.c_int]),
         "orc_shifted_prefix_product": (V, [u64p, u64p, ctypes.c_size_t, u64p]),
+        "orc_lookup_products": (V, [u64p, u64p, u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_size_t, u64p, u64p,
+                                    u64p, ctypes.c_size_t]),
         "orc_poly_eval": (V, [u64p, u64p, ctypes.c_size_t, u64p]),
         "orc_g1j_mul": (V, [u64p, u64p, u64p]),
         "orc_g1j_neg": (V, [u64p, u64p]),
@@ -538,6 +540,16 @@ def permutation_products(columns, sigmas, n, chunk_len, beta, gamma, omega, delt
                                    P(np.ascontiguousarray(gamma, dtype=np.uint64)),
                                    P(np.ascontiguousarray(omega, dtype=np.uint64)), P(dp), P(bl), blinding,
                                    threads or os.cpu_count())
+    return out
+
+
+def lookup_products(ci, ct, pi, pt, n, count, beta, gamma, blinds, blinding):
+    """orc_lookup_products: count * n values (lookup-major)"""
+    arrs = [np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, 4) for x in (ci, ct, pi, pt)]
+    bl = np.ascontiguousarray(blinds, dtype=np.uint64).reshape(-1, 4) if blinding else np.zeros((1, 4), np.uint64)
+    out = np.zeros((n * count, 4), dtype=np.uint64)
+    lib().orc_lookup_products(P(out), *[P(a) for a in arrs], n, count, P(np.ascontiguousarray(beta)),
+                              P(np.ascontiguousarray(gamma)), P(bl), blinding)
     return out
 
 

@@ -31,7 +31,10 @@
  *     cell makes exactly the expected rows nonzero;
  *   - the numerator on the extended coset of that circuit, divided by the
  *     vanishing polynomial, interpolates to a polynomial whose top
- *     coefficients (degree >= (n-1)(d-1)) are zero.
+ *     coefficients (degree >= (n-1)(d-1)) are zero;
+ *   - lookup products against Python integers on a tiny input (with a zero
+ *     denominator and the blinded tail), and closure to one at the last
+ *     usable row when the permuted columns permute the compressed ones.
  */
 #include "oracle.h"
 #include <stdlib.h>
@@ -400,4 +403,38 @@ void orc_shifted_prefix_product(orc_fr *out, const orc_fr *values, size_t len, c
         out[i] = acc;
         acc = next;
     }
+}
+
+/* Lookup product columns: vendor/midnight-proofs-0.8.1/src/plonk/lookup/
+ * prover.rs:1075-1178 (construct_product).  Per lookup: denominators
+ * (beta + a'_i)(gamma + s'_i), batch-inverted (zero stays zero), times
+ * (a_i + beta)(s_i + gamma); z = 1, f_0, f_0 f_1, ... for the first
+ * len - blinding rows, then the blinds.  Inputs lookup-major. */
+void orc_lookup_products(orc_fr *out, const orc_fr *ci, const orc_fr *ct, const orc_fr *pi, const orc_fr *pt,
+                         size_t len, size_t count, const orc_fr *beta, const orc_fr *gamma, const orc_fr *blinds,
+                         size_t blinding)
+{
+    orc_fr *f = (orc_fr *)malloc(len * sizeof(orc_fr));
+    for (size_t l = 0; l < count; l++) {
+        const size_t o = l * len;
+        for (size_t i = 0; i < len; i++) {
+            orc_fr a, b;
+            orc_fr_add(&a, beta, &pi[o + i]);
+            orc_fr_add(&b, gamma, &pt[o + i]);
+            orc_fr_mul(&f[i], &a, &b);
+            orc_fr_inv(&f[i], &f[i]);                 /* 0 -> 0, as batch_invert */
+            orc_fr_add(&a, &ci[o + i], beta);
+            orc_fr_mul(&f[i], &f[i], &a);
+            orc_fr_add(&b, &ct[o + i], gamma);
+            orc_fr_mul(&f[i], &f[i], &b);
+        }
+        orc_fr acc;
+        orc_fr_one(&acc);
+        for (size_t i = 0; i + blinding < len; i++) {
+            out[o + i] = acc;
+            orc_fr_mul(&acc, &acc, &f[i]);
+        }
+        for (size_t t = 0; t < blinding; t++) out[o + len - blinding + t] = blinds[l * blinding + t];
+    }
+    free(f);
 }

@@ -231,3 +231,56 @@ def test_oracle_permutation_products_match_prover_restatement(circuit):
                                S.mont([c.omega])[0], S.mont(c.delta_pows), S.mont(c.blinds), S.BLINDING)
     assert np.array_equal(z, S.mont([x for zz in c.z for x in zz]))
     assert S.ints(z[2 * c.n + c.u][None, :])[0] == 1
+
+
+R_MOD = 0x73eda753299d7d483339d80809a1d80553bda402fffe5bfeffffffff00000001
+
+
+def test_oracle_lookup_products_brute_force(O):
+    """orc_lookup_products against Python integers on a tiny input, with a
+    zero denominator (a fraction of zero) and the blinded tail."""
+    n, count, bf = 8, 2, 2
+    vals = [[random.Random(700 + k).randrange(R_MOD) for _ in range(n * count)] for k in range(4)]
+    beta, gamma = 11, 13
+    vals[2][3] = (R_MOD - beta) % R_MOD        # permuted input + beta == 0 on row 3 of lookup 0
+    blinds = [random.Random(710).randrange(R_MOD) for _ in range(count * bf)]
+    arrs = [np.array([O.fr_from_int(v) for v in col]) for col in vals]
+    got = O.lookup_products(*arrs, n, count, O.fr_from_int(beta), O.fr_from_int(gamma),
+                            np.array([O.fr_from_int(b) for b in blinds]), bf)
+    for l in range(count):
+        z = [1]
+        for i in range(n - bf - 1):
+            o = l * n + i
+            den = (vals[2][o] + beta) * (vals[3][o] + gamma) % R_MOD
+            num = (vals[0][o] + beta) * (vals[1][o] + gamma) % R_MOD
+            f = num * pow(den, R_MOD - 2, R_MOD) % R_MOD if den else 0
+            z.append(z[-1] * f % R_MOD)
+        z += blinds[l * bf:(l + 1) * bf]
+        assert [O.fr_to_int(got[l * n + i]) for i in range(n)] == z, l
+    assert O.fr_to_int(got[4]) == 0   # the product is zero after the zero denominator
+
+
+def test_oracle_lookup_products_close_on_permutations(O):
+    """When the permuted columns are permutations of the compressed ones
+    over the usable rows, every lookup's product returns to one at the last
+    usable row (the lookup argument's l_last check)."""
+    n, count, bf = 64, 3, 5
+    u = n - bf - 1
+    rng = random.Random(720)
+    ci, ct, pi, pt = ([] for _ in range(4))
+    for l in range(count):
+        a = [rng.randrange(R_MOD) for _ in range(n)]
+        s = [rng.randrange(R_MOD) for _ in range(n)]
+        pa, ps = a[:u], s[:u]
+        rng.shuffle(pa)
+        rng.shuffle(ps)
+        ci += a
+        ct += s
+        pi += pa + a[u:]
+        pt += ps + s[u:]
+    arrs = [np.array([O.fr_from_int(v) for v in col]) for col in (ci, ct, pi, pt)]
+    z = O.lookup_products(*arrs, n, count, O.fr_from_int(5), O.fr_from_int(7), O.random_fr(count * bf, 721), bf)
+    for l in range(count):
+        assert O.fr_to_int(z[l * n]) == 1
+        assert O.fr_to_int(z[l * n + u]) == 1, l
+        assert O.fr_to_int(z[l * n + u // 2]) != 1
