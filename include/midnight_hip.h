@@ -416,6 +416,23 @@ RustError midnight_sppark_bls12_381_fr_lookup_products(
 RustError midnight_sppark_bls12_381_fr_lookup_products_device(void **out, const void *trace_lagrange, size_t len,
                                                               size_t lookup_count, RawFr beta, RawFr gamma,
                                                               const RawFr *blind_values, size_t blinding_factors);
+/* replaces bls12_381_g1_msm.cu:6617: values sorted in place by canonical value (Fq's Ord) */
+RustError midnight_sppark_bls12_381_fr_sort_canonical(RawFr *values, size_t len);
+/* replaces bls12_381_g1_msm.cu:5139: *out = device vector [compressed inputs | compressed tables |
+ * permuted inputs | permuted tables] (lookup-major blocks of len rows): the permuted columns of
+ * permute_expression_pair (plonk/lookup/prover.rs:1573-1690) over the first usable_rows rows, then the
+ * blinds (blind_count = len - usable_rows per lookup); code -1 when an input is not in its table */
+RustError midnight_sppark_bls12_381_fr_lookup_permute_batch(void **out, const RawFr *compressed_inputs,
+                                                            const RawFr *compressed_tables, size_t len,
+                                                            size_t lookup_count, size_t usable_rows,
+                                                            const RawFr *input_blinds, const RawFr *table_blinds,
+                                                            size_t blind_count);
+/* replaces bls12_381_g1_msm.cu:5849: the same with the compressed columns evaluated from a numerator
+ * context's lookup input / table graphs (theta) on its Lagrange domain */
+RustError midnight_sppark_bls12_381_trace_context_lookup_permute(
+    void **out, const void *context, const void *advice, size_t advice_cols, const void *instance,
+    size_t instance_cols, const RawFr *challenges, size_t challenge_len, RawFr theta, size_t usable_rows,
+    const RawFr *input_blinds, const RawFr *table_blinds, size_t blind_count);
 /* replaces bls12_381_g1_msm.cu:5667: each trash graph of a numerator context on the Lagrange domain
  * (input_len rows, rotations unscaled) from the context's fixed columns and resident advice / instance;
  * *out receives trash_count * input_len values */

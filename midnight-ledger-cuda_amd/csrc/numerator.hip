@@ -34,6 +34,7 @@
 #include "devvec.hpp"
 #include "fields.hpp"
 #include "grandproduct.hpp"
+#include "lookup.hpp"
 #include "host_fr.hpp"
 #include "profile.hpp"
 #include "rowpow.hpp"
@@ -699,6 +700,8 @@ extern "C" RustError midnight_sppark_bls12_381_numerator_context_create(
     }
     for (size_t i = 0; i < trash_count; ++i) c->graphs.add(trash_graphs[i], st);
     c->graphs.finish(st);
+    c->lookup_input_graphs.finish(st);   // descriptor tables for trace_context_lookup_permute
+    c->lookup_table_graphs.finish(st);
     build_omega_table(c->omega_tab, c->omega_levels, c->omega, len, st);
     MH_HIP(hipStreamSynchronize(st));
     *out = c.release();
@@ -826,6 +829,8 @@ extern "C" RustError midnight_sppark_bls12_381_trace_context_trash(void **out, c
         return make_error((int)hipErrorInvalidValue, "resident trash vector length mismatch");
     if (advice_cols > 0xffffffffu || instance_cols > 0xffffffffu || challenge_len > 0xffffffffu)
         return make_error((int)hipErrorInvalidValue, "trash argument count too large");
+    if (!c->graphs.table.get() || c->graphs.graphs.size() < 1 + c->lookup_count + c->trash_count)
+        return make_error(-1, "numerator context holds no trash graphs");
     MH_TRY
     hipStream_t st = cur_stream();
     DevBuf<Fr> d_ch;
@@ -850,6 +855,81 @@ extern "C" RustError midnight_sppark_bls12_381_trace_context_trash(void **out, c
     a.graph_count = (uint32_t)c->trash_count;
     launch_rows(a, c->graphs.max_slots, k_eval_graphs<true>, k_eval_graphs<false>, st);
     MH_HIP(hipStreamSynchronize(st));
+    *out = dv.release();
+    return ok();
+    MH_CATCH
+}
+
+// bls12_381_g1_msm.cu:5849-5966: the lookups' compressed input and table
+// columns from the context's lookup input / table graphs on its Lagrange
+// domain (theta, rotations unscaled), then their permuted columns: a device
+// vector [compressed inputs | compressed tables | permuted inputs | permuted
+// tables], lookup-major blocks of input_len rows
+extern "C" RustError midnight_sppark_bls12_381_trace_context_lookup_permute(
+    void **out, const void *raw_context, const void *raw_advice, size_t advice_cols, const void *raw_instance,
+    size_t instance_cols, const RawFr *challenges, size_t challenge_len, RawFr theta, size_t usable_rows,
+    const RawFr *input_blinds, const RawFr *table_blinds, size_t blind_count)
+{
+    if (out == nullptr || raw_context == nullptr || (challenge_len != 0 && challenges == nullptr))
+        return make_error((int)hipErrorInvalidValue, "invalid resident lookup permutation arguments");
+    *out = nullptr;
+    const auto *c = reinterpret_cast<const NumeratorContext *>(raw_context);
+    const size_t len = c->input_len;
+    const size_t lookup_count = c->lookup_count;
+    const auto *advice = reinterpret_cast<const DeviceVec *>(raw_advice);
+    const auto *instance = reinterpret_cast<const DeviceVec *>(raw_instance);
+    auto has_len = [](const DeviceVec *v, size_t expected) {
+        return expected == 0 ? (v == nullptr || v->len == 0) : (v != nullptr && v->len == expected);
+    };
+    if (lookup_count == 0 || usable_rows == 0 || usable_rows > len || blind_count != len - usable_rows ||
+        c->lookup_input_graphs.graphs.size() != lookup_count || c->lookup_table_graphs.graphs.size() != lookup_count ||
+        !has_len(advice, advice_cols * len) || !has_len(instance, instance_cols * len) ||
+        (blind_count != 0 && (input_blinds == nullptr || table_blinds == nullptr)))
+        return make_error((int)hipErrorInvalidValue, "resident lookup permutation shape mismatch");
+    if (lookup_count > SIZE_MAX / len || lookup_count * len > SIZE_MAX / 4)
+        return make_error((int)hipErrorInvalidValue, "resident lookup permutation size overflow");
+    if (advice_cols > 0xffffffffu || instance_cols > 0xffffffffu || challenge_len > 0xffffffffu ||
+        usable_rows > 0xffffffffu)
+        return make_error((int)hipErrorInvalidValue, "lookup permutation argument count too large");
+    if (!c->lookup_input_graphs.table.get() || !c->lookup_table_graphs.table.get() ||
+        (!c->fixed_lagrange.get() && c->fixed_cols))
+        return make_error(-1, "numerator context holds no lookup compression graphs");
+    MH_TRY
+    hipStream_t st = cur_stream();
+    const size_t group = lookup_count * len;
+    DevBuf<Fr> d_ch;
+    auto dv = std::make_unique<DeviceVec>();
+    dv->lane = current_lane();
+    dv->len = 4 * group;
+    dv->values.alloc(dv->len);
+    NumArgs a{};
+    a.len = len;
+    a.fixed = c->fixed_lagrange.get();
+    a.advice = advice_cols ? advice->values.get() : nullptr;
+    a.instance = instance_cols ? instance->values.get() : nullptr;
+    a.challenges = up(d_ch, challenges, challenge_len, st);
+    a.fixed_cols = (uint32_t)c->fixed_cols;
+    a.advice_cols = (uint32_t)advice_cols;
+    a.instance_cols = (uint32_t)instance_cols;
+    a.challenge_len = (uint32_t)challenge_len;
+    a.theta = to_fr(theta);
+    a.rot_scale = 1;
+    a.graph_count = (uint32_t)lookup_count;
+    a.out = dv->values.get();
+    a.graphs = c->lookup_input_graphs.table.get();
+    launch_rows(a, c->lookup_input_graphs.max_slots, k_eval_graphs<true>, k_eval_graphs<false>, st);
+    a.out = dv->values.get() + group;
+    a.graphs = c->lookup_table_graphs.table.get();
+    launch_rows(a, c->lookup_table_graphs.max_slots, k_eval_graphs<true>, k_eval_graphs<false>, st);
+    DevBuf<Fr> blinds(std::max<size_t>(2 * lookup_count * blind_count, 1));
+    if (blind_count) {
+        MH_HIP(hipMemcpyAsync(blinds.get(), input_blinds, lookup_count * blind_count * 32, hipMemcpyHostToDevice, st));
+        MH_HIP(hipMemcpyAsync(blinds.get() + lookup_count * blind_count, table_blinds, lookup_count * blind_count * 32,
+                              hipMemcpyHostToDevice, st));
+    }
+    if (!lookup_permute_device(dv->values.get(), len, lookup_count, usable_rows, blinds.get(),
+                               blinds.get() + lookup_count * blind_count, blind_count, st))
+        return make_error(-1, "lookup input is not contained in the lookup table");
     *out = dv.release();
     return ok();
     MH_CATCH

@@ -178,6 +178,11 @@ def _declare(L):
             E, [vpp, u64p, u64p, u64p, u64p, c_size, c_size, RawFr, RawFr, vp, c_size]),
         "midnight_sppark_bls12_381_fr_lookup_products_device": (
             E, [vpp, vp, c_size, c_size, RawFr, RawFr, vp, c_size]),
+        "midnight_sppark_bls12_381_fr_sort_canonical": (E, [u64p, c_size]),
+        "midnight_sppark_bls12_381_fr_lookup_permute_batch": (
+            E, [vpp, u64p, u64p, c_size, c_size, c_size, vp, vp, c_size]),
+        "midnight_sppark_bls12_381_trace_context_lookup_permute": (
+            E, [vpp, vp, vp, c_size, vp, c_size, vp, c_size, RawFr, c_size, vp, vp, c_size]),
         "midnight_sppark_bls12_381_trace_context_trash": (
             E, [vpp, vp, vp, c_size, vp, c_size, vp, c_size, RawFr]),
         "midnight_sppark_bls12_381_trace_context_permutation": (
@@ -762,6 +767,7 @@ class NumeratorContext:
             rot_scale, last_rotation, chunk_len))
         self.handle = h
         self.n = n
+        self.lookup_count = len(lookup_graphs)
 
     def eval(self, columns, total_columns, *, beta, gamma, theta, trash_challenge, y, advice_offset=0,
              advice_cols=0, instance_offset=0, instance_cols=0, challenges=None, permutation_offset=0,
@@ -791,6 +797,21 @@ class NumeratorContext:
             ctypes.byref(h), self.handle, advice.handle if advice is not None else None, advice_cols,
             instance.handle if instance is not None else None, instance_cols, RawFr.of(beta), RawFr.of(gamma),
             P(dp), dp.shape[0], _ptr(bl), 0 if bl is None else bl.shape[0], blinding_factors))
+        return DeviceVec(h.value)
+
+    def trace_lookup_permute(self, advice, advice_cols, instance, instance_cols, challenges, theta, usable_rows,
+                             input_blinds, table_blinds):
+        """trace_context_lookup_permute: DeviceVec [compressed inputs | tables | permuted inputs | tables]."""
+        ch = None if challenges is None or len(challenges) == 0 else _fr_array(challenges)
+        ib = _blinds_or_empty(input_blinds)
+        tb = _blinds_or_empty(table_blinds)
+        blind_count = np.ascontiguousarray(input_blinds, dtype=np.uint64).reshape(-1, 4).shape[0]
+        h = ctypes.c_void_p()
+        check(lib().midnight_sppark_bls12_381_trace_context_lookup_permute(
+            ctypes.byref(h), self.handle, advice.handle if advice is not None else None, advice_cols,
+            instance.handle if instance is not None else None, instance_cols, _ptr(ch),
+            0 if ch is None else ch.shape[0], RawFr.of(theta), usable_rows, _ptr(ib), _ptr(tb),
+            blind_count // max(self.lookup_count, 1)))
         return DeviceVec(h.value)
 
     def trace_trash(self, advice, advice_cols, instance, instance_cols, challenges, trash_challenge):
@@ -835,6 +856,29 @@ def fr_permutation_products(columns, permutations, n, chunk_len, *, beta, gamma,
     check(lib().midnight_sppark_bls12_381_fr_permutation_products(
         ctypes.byref(h), P(cols), P(perm), n, cols.shape[0] // n, chunk_len, RawFr.of(beta), RawFr.of(gamma),
         RawFr.of(omega), P(dp), _ptr(bl), blinding_factors))
+    return DeviceVec(h.value)
+
+
+def fr_sort_canonical(values):
+    a = np.array(_fr_array(values), copy=True)
+    check(lib().midnight_sppark_bls12_381_fr_sort_canonical(P(a), a.shape[0]))
+    return a
+
+
+def _blinds_or_empty(b):
+    a = np.ascontiguousarray(b, dtype=np.uint64).reshape(-1, 4)
+    return a if a.shape[0] else np.zeros((1, 4), dtype=np.uint64)
+
+
+def fr_lookup_permute_batch(inputs, tables, n, count, usable_rows, input_blinds, table_blinds):
+    """DeviceVec [compressed inputs | tables | permuted inputs | permuted tables]"""
+    ci = _fr_array(inputs)
+    ct = _fr_array(tables)
+    ib = _blinds_or_empty(input_blinds)
+    tb = _blinds_or_empty(table_blinds)
+    h = ctypes.c_void_p()
+    check(lib().midnight_sppark_bls12_381_fr_lookup_permute_batch(
+        ctypes.byref(h), P(ci), P(ct), n, count, usable_rows, _ptr(ib), _ptr(tb), n - usable_rows))
     return DeviceVec(h.value)
 
 

@@ -168,6 +168,16 @@ def lookup_graph(inputs, tables):
     return g
 
 
+def compression_graph(exprs):
+    """A lookup's input or table compression, Horner in theta from constant
+    zero (evaluation.rs:2743-2766): the graphs of
+    numerator_context_create's lookup_input_graphs / lookup_table_graphs."""
+    g = GraphEvaluator()
+    parts = tuple(g.add_expression(x) for x in exprs)
+    g.add_calculation(("horner", ZERO_CONST, parts, vs(THETA)))
+    return g
+
+
 def trash_graph(constraints):
     """Trash-argument graph: constraints folded in the trash challenge
     (evaluation.rs:2801-2817)."""

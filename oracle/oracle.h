@@ -206,6 +206,11 @@ void orc_lookup_products(orc_fr *out, const orc_fr *ci, const orc_fr *ct, const 
                          size_t len, size_t count, const orc_fr *beta, const orc_fr *gamma, const orc_fr *blinds,
                          size_t blinding);
 
+/* values sorted by canonical value (Fq's Ord) */
+void orc_fr_sort_canonical(orc_fr *out, const orc_fr *v, size_t n);
+/* permute_expression_pair (lookup/prover.rs:1573-1690) over `usable` rows; 0 if an input is not in the table */
+int orc_lookup_permute(orc_fr *pin, orc_fr *ptab, const orc_fr *input, const orc_fr *table, size_t usable);
+
 /* shifted_prefix_product (permutation/prover.rs:36-43); out may not alias values */
 void orc_shifted_prefix_product(orc_fr *out, const orc_fr *values, size_t len, const orc_fr *initial);
 

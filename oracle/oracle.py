@@ -103,6 +103,8 @@ def _declare(L):
                                    ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32)]),
         "orc_eval_numerator": (V, [u64p, ctypes.c_void_p, ctypes.c_int]),
         "orc_shifted_prefix_product": (V, [u64p, u64p, ctypes.c_size_t, u64p]),
+        "orc_fr_sort_canonical": (V, [u64p, u64p, ctypes.c_size_t]),
+        "orc_lookup_permute": (ctypes.c_int, [u64p, u64p, u64p, u64p, ctypes.c_size_t]),
         "orc_lookup_products": (V, [u64p, u64p, u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_size_t, u64p, u64p,
                                     u64p, ctypes.c_size_t]),
         "orc_poly_eval": (V, [u64p, u64p, ctypes.c_size_t, u64p]),
@@ -541,6 +543,24 @@ def permutation_products(columns, sigmas, n, chunk_len, beta, gamma, omega, delt
                                    P(np.ascontiguousarray(omega, dtype=np.uint64)), P(dp), P(bl), blinding,
                                    threads or os.cpu_count())
     return out
+
+
+def sort_canonical(values):
+    v = np.ascontiguousarray(values, dtype=np.uint64).reshape(-1, 4)
+    out = np.zeros_like(v)
+    lib().orc_fr_sort_canonical(P(out), P(v), v.shape[0])
+    return out
+
+
+def lookup_permute(inputs, table, usable):
+    """(permuted input, permuted table) over the first `usable` rows, or None if an input is not in the table"""
+    a = np.ascontiguousarray(inputs, dtype=np.uint64).reshape(-1, 4)
+    t = np.ascontiguousarray(table, dtype=np.uint64).reshape(-1, 4)
+    pin = np.zeros((usable, 4), dtype=np.uint64)
+    ptab = np.zeros((usable, 4), dtype=np.uint64)
+    if not lib().orc_lookup_permute(P(pin), P(ptab), P(a), P(t), usable):
+        return None
+    return pin, ptab
 
 
 def lookup_products(ci, ct, pi, pt, n, count, beta, gamma, blinds, blinding):

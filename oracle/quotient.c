@@ -438,3 +438,80 @@ void orc_lookup_products(orc_fr *out, const orc_fr *ci, const orc_fr *ct, const 
     }
     free(f);
 }
+
+/* permute_expression_pair: vendor/midnight-proofs-0.8.1/src/plonk/lookup/
+ * prover.rs:1573-1690, over the usable rows.  Inputs sorted (Fq's Ord, the
+ * canonical integer order); each first occurrence copies its value to the
+ * permuted table and consumes one equal table entry, walking the sorted
+ * table with one pointer; the unconsumed table entries, ascending, fill the
+ * repeated input rows taken from the last one back.  Returns 0 when an
+ * input is not in the table. */
+typedef struct {
+    uint64_t c[4];
+    orc_fr v;
+} orc_keyed;
+
+static int keyed_cmp(const void *pa, const void *pb)
+{
+    const orc_keyed *a = (const orc_keyed *)pa, *b = (const orc_keyed *)pb;
+    for (int j = 3; j >= 0; j--) {
+        if (a->c[j] < b->c[j]) return -1;
+        if (a->c[j] > b->c[j]) return 1;
+    }
+    return 0;
+}
+
+static orc_keyed *keyed_sorted(const orc_fr *v, size_t n)
+{
+    orc_keyed *k = (orc_keyed *)malloc((n ? n : 1) * sizeof(orc_keyed));
+    for (size_t i = 0; i < n; i++) {
+        orc_fr_to_canonical(k[i].c, &v[i]);
+        k[i].v = v[i];
+    }
+    qsort(k, n, sizeof(orc_keyed), keyed_cmp);   /* equal keys are equal values: stability is moot */
+    return k;
+}
+
+void orc_fr_sort_canonical(orc_fr *out, const orc_fr *v, size_t n)
+{
+    orc_keyed *k = keyed_sorted(v, n);
+    for (size_t i = 0; i < n; i++) out[i] = k[i].v;
+    free(k);
+}
+
+int orc_lookup_permute(orc_fr *pin, orc_fr *ptab, const orc_fr *input, const orc_fr *table, size_t usable)
+{
+    orc_keyed *in = keyed_sorted(input, usable), *tb = keyed_sorted(table, usable);
+    size_t *repeated = (size_t *)malloc((usable ? usable : 1) * sizeof(size_t));
+    unsigned char *removed = (unsigned char *)calloc(usable ? usable : 1, 1);
+    size_t nrep = 0, t = 0;
+    int ok = 1;
+    for (size_t r = 0; r < usable; r++) {
+        pin[r] = in[r].v;
+        if (r == 0 || keyed_cmp(&in[r], &in[r - 1]) != 0) {
+            ptab[r] = in[r].v;
+            while (t < usable && keyed_cmp(&tb[t], &in[r]) < 0) t++;
+            if (t == usable || keyed_cmp(&tb[t], &in[r]) != 0) {
+                ok = 0;
+                break;
+            }
+            removed[t++] = 1;
+        } else {
+            repeated[nrep++] = r;
+        }
+    }
+    for (size_t p = 0; ok && p < usable; p++) {
+        if (removed[p]) continue;
+        if (nrep == 0) {
+            ok = 0;
+            break;
+        }
+        ptab[repeated[--nrep]] = tb[p].v;
+    }
+    if (ok && nrep != 0) ok = 0;
+    free(in);
+    free(tb);
+    free(repeated);
+    free(removed);
+    return ok;
+}

@@ -1,6 +1,7 @@
 """GPU parity of the reference's device-vector, polynomial, multi-open,
 inversion, lookup-product and trash entry points (bls12_381_g1_msm.cu:
-2878-3179, 3234, 3558-4110, 4451-4634, 5667, 6582) through the C ABI.
+2878-3179, 3234, 3558-4110, 4451-4634, 5139, 5667, 5849, 6582, 6617) through
+the C ABI.
 
 Bar: limb-identical to the oracle (oracle/poly.c eval / kate division,
 oracle/quotient.c lookup products and graph rows, Fr inversion) or, for
@@ -231,4 +232,91 @@ def test_trace_context_trash(H, O):
         ctx.trace_trash(adv, 4, None, 0, challenges, trash_ch)   # advice length mismatch
     out.free()
     adv.free()
+    ctx.free()
+
+
+def test_sort_canonical(H, O):
+    v = O.random_fr(20000, 880)
+    v[100:200] = v[:100]                                  # duplicates
+    v[5] = 0
+    assert np.array_equal(H.fr_sort_canonical(v), O.sort_canonical(v))
+    small = O.random_fr(1, 881)
+    assert np.array_equal(H.fr_sort_canonical(small), small)
+
+
+def _lookup_case(O, n, count, seed, usable, table_rows=300):
+    """inputs drawn (with heavy repetition) from the usable rows of each table"""
+    tables = O.random_fr(n * count, seed)
+    rng = np.random.default_rng(seed)
+    rows = min(table_rows, usable)
+    inputs = np.concatenate([tables[l * n + rng.integers(0, rows, n)] for l in range(count)])
+    return inputs, tables
+
+
+def _want_trace(O, inputs, tables, n, count, usable, ib, tb):
+    bc = n - usable
+    pin, ptab = [], []
+    for l in range(count):
+        a, t = O.lookup_permute(inputs[l * n:l * n + usable], tables[l * n:l * n + usable], usable)
+        pin.append(np.concatenate([a, ib[l * bc:(l + 1) * bc]]))
+        ptab.append(np.concatenate([t, tb[l * bc:(l + 1) * bc]]))
+    return np.concatenate([inputs, tables] + pin + ptab)
+
+
+@pytest.mark.parametrize("n,count", [(256, 1), (1 << 14, 3)])
+def test_lookup_permute_batch(H, O, n, count):
+    usable = n - 6
+    inputs, tables = _lookup_case(O, n, count, 890 + count, usable)
+    ib = O.random_fr(count * 6, 893)
+    tb = O.random_fr(count * 6, 894)
+    dv = H.fr_lookup_permute_batch(inputs, tables, n, count, usable, ib, tb)
+    assert np.array_equal(dv.download(), _want_trace(O, inputs, tables, n, count, usable, ib, tb))
+    dv.free()
+    bad = inputs.copy()
+    bad[10] = O.random_fr(1, 895)[0]                        # not in the table
+    with pytest.raises(H.MidnightHipError) as e:
+        H.fr_lookup_permute_batch(bad, tables, n, count, usable, ib, tb)
+    assert e.value.code == -1
+
+
+def test_trace_context_lookup_permute(H, O):
+    n_ext, n = 4096, 1024
+    usable = n - 6
+    theta = O.random_fr(1, 900)[0]
+    fixed_lag = O.random_fr(3 * n, 901)
+    advice = O.random_fr(5 * n, 902)
+    # lookup 0: (advice 0, advice 1) in (fixed 0, fixed 1); lookup 1: advice 2 in fixed 2
+    rng = np.random.default_rng(903)
+    rows = rng.integers(0, 200, n)
+    advice[0:n] = fixed_lag[rows]
+    advice[n:2 * n] = fixed_lag[n + rows]
+    advice[2 * n:3 * n] = fixed_lag[2 * n + rng.integers(0, 50, n)]
+    inputs = [G.RawGraphBuffers(G.compression_graph([("advice", 0, 0), ("advice", 1, 0)])),
+              G.RawGraphBuffers(G.compression_graph([("advice", 2, 0)]))]
+    tables = [G.RawGraphBuffers(G.compression_graph([("fixed", 0, 0), ("fixed", 1, 0)])),
+              G.RawGraphBuffers(G.compression_graph([("fixed", 2, 0)]))]
+    filler = O.random_fr(n_ext * 3, 904)
+    ctx = H.NumeratorContext(
+        n_ext, n, fixed=filler, fixed_lagrange=fixed_lag, l0=filler[:n_ext], l_last=filler[:n_ext],
+        l_active=filler[:n_ext], permutation_columns=[], permutation_pk_cosets=None, permutation_pk_lagrange=None,
+        custom_graph=G.RawGraphBuffers(G.custom_gates_graph([("advice", 0, 0)])), omega=O.omega(12),
+        trace_omega=O.omega(10), g_coset=O.fr_from_int(7), delta_const=O.fr_from_int(1), rot_scale=4,
+        last_rotation=-6, chunk_len=3, lookup_graphs=inputs, lookup_input_graphs=inputs,
+        lookup_table_graphs=tables)
+    adv = H.DeviceVec.upload(advice)
+    ib = O.random_fr(2 * 6, 905)
+    tb = O.random_fr(2 * 6, 906)
+    out = ctx.trace_lookup_permute(adv, 5, None, 0, None, theta, usable, ib, tb)
+    ci = np.concatenate([O.graph_rows(n, g, theta=theta, fixed=fixed_lag, advice=advice) for g in inputs])
+    ct = np.concatenate([O.graph_rows(n, g, theta=theta, fixed=fixed_lag, advice=advice) for g in tables])
+    assert np.array_equal(out.download(), _want_trace(O, ci, ct, n, 2, usable, ib, tb))
+    # the products of the permuted trace close to one at the last usable row
+    beta, gamma = O.random_fr(2, 907)
+    z = H.fr_lookup_products_device(out, n, 2, beta=beta, gamma=gamma, blinds=O.random_fr(10, 908),
+                                    blinding_factors=5)
+    zz = z.download()
+    one = O.fr_from_int(1)
+    assert np.array_equal(zz[usable], one) and np.array_equal(zz[n + usable], one)
+    for x in (z, out, adv):
+        x.free()
     ctx.free()

@@ -284,3 +284,29 @@ def test_oracle_lookup_products_close_on_permutations(O):
         assert O.fr_to_int(z[l * n]) == 1
         assert O.fr_to_int(z[l * n + u]) == 1, l
         assert O.fr_to_int(z[l * n + u // 2]) != 1
+
+
+def test_oracle_lookup_permute(O):
+    """orc_lookup_permute: a hand-derived case of permute_expression_pair
+    (ascending leftovers fill the repeated rows from the last one back), the
+    lookup argument's row property on a random case, and the error for an
+    input missing from the table."""
+    f = O.fr_from_int
+    pin, ptab = O.lookup_permute(np.array([f(3), f(1), f(3), f(2), f(3)]),
+                                 np.array([f(1), f(2), f(3), f(4), f(5)]), 5)
+    assert [O.fr_to_int(x) for x in pin] == [1, 2, 3, 3, 3]
+    assert [O.fr_to_int(x) for x in ptab] == [1, 2, 3, 5, 4]
+    assert O.lookup_permute(np.array([f(6)] * 5), np.array([f(1), f(2), f(3), f(4), f(5)]), 5) is None
+    rng = random.Random(730)
+    n = 300
+    table = [rng.randrange(R_MOD) for _ in range(n)]
+    inputs = [table[rng.randrange(40)] for _ in range(n)]     # heavy repetition, all in the table
+    pin, ptab = O.lookup_permute(np.array([f(x) for x in inputs]), np.array([f(x) for x in table]), n)
+    pi = [O.fr_to_int(x) for x in pin]
+    pt = [O.fr_to_int(x) for x in ptab]
+    assert pi == sorted(inputs)
+    assert sorted(pt) == sorted(table)
+    for r in range(n):
+        assert pi[r] == pt[r] or (r > 0 and pi[r] == pi[r - 1]), r
+    s = O.sort_canonical(np.array([f(x) for x in table]))
+    assert [O.fr_to_int(x) for x in s] == sorted(table)
