@@ -450,6 +450,33 @@ RustError midnight_sppark_bls12_381_trace_context_permutation(
     const RawFr *delta_powers, size_t delta_powers_len, const RawFr *blind_values,
     size_t blind_values_len, size_t blinding_factors);
 
+/* ---------------- Poseidon hash chains (width 3, 8 full + 60 partial rounds, x^5) ---------------- */
+
+/* replaces bls12_381_g1_msm.cu:4121 (lib.rs:1029): out[i] = previous, where each of the len hashes
+ * permutes [previous, tail0, input_len], adds tail1 to lane 0 and permutes again, previous starting
+ * at initial.  mds: 9 values (row-major), round_constants: 68 * 3; len == 0 is a no-op. */
+RustError midnight_sppark_bls12_381_fr_poseidon_hash_chain(RawFr *out, size_t len, RawFr initial, RawFr tail0,
+                                                          RawFr tail1, RawFr input_len, const RawFr *mds,
+                                                          size_t mds_len, const RawFr *round_constants,
+                                                          size_t round_constants_len);
+/* replaces bls12_381_g1_msm.cu:4192 (lib.rs:1042): the same chain's circuit witness, 256 values per
+ * hash (128 per permutation: per full round the S-box cubes and the next state, per batch of six
+ * partial rounds five last-lane values and the state) */
+RustError midnight_sppark_bls12_381_fr_poseidon_hash_chain_witness(RawFr *out, size_t len, RawFr initial,
+                                                                  RawFr tail0, RawFr tail1, RawFr input_len,
+                                                                  const RawFr *mds, size_t mds_len,
+                                                                  const RawFr *round_constants,
+                                                                  size_t round_constants_len);
+/* replaces bls12_381_g1_msm.cu:2820 (lib.rs:854): a device vector of 8 * domain_len values (eight
+ * advice columns, column-major) holding the chain witness placed by layout ((column << 8) | row, 128
+ * entries) at rows 2 + 41 hash + 21 permutation, plus the input rows taken from metadata = {prefix[3],
+ * binding, initial, tail0, tail1, input_len, round_constants[0..3]}; needs 2 + 41 hash_count <
+ * domain_len, every layout column below 8 and every layout row inside the domain */
+RustError midnight_sppark_bls12_381_fr_poseidon_advice_from_witness(void **out, const RawFr *witness,
+                                                                   size_t hash_count, size_t domain_len,
+                                                                   const RawFr *metadata, size_t metadata_len,
+                                                                   const uint16_t *layout, size_t layout_len);
+
 /* ---------------- backend extensions (no reference counterpart) ---------------- */
 
 /* Number of elements of a device vector (for callers and tests that hold

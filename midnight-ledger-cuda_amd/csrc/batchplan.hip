@@ -99,12 +99,6 @@ struct BatchPlan {
 
 using namespace mh;
 
-#define MH_TRY try {
-#define MH_CATCH                                                                         \
-    }                                                                                    \
-    catch (const hip_error &e) { return make_error(e.code, e.what()); }                  \
-    catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
-
 extern "C" RustError midnight_hip_batch_plan_create(void **out, void *msm_context, void *device_vec,
                                                     const size_t *offsets, const uint32_t *log_ns, size_t count,
                                                     uint32_t flags)

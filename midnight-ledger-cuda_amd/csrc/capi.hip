@@ -95,12 +95,6 @@ HostFr to_hfr(const RawFr &r)
 
 using namespace mh;
 
-#define MH_TRY try {
-#define MH_CATCH                                                     \
-    }                                                                \
-    catch (const hip_error &e) { return make_error(e.code, e.what()); } \
-    catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
-
 extern "C" RustError midnight_sppark_execution_lane_set(uint32_t lane)
 {
     if (lane > kMaxLanes) return make_error((int)hipErrorInvalidValue, "invalid sppark proof execution lane");

@@ -460,12 +460,6 @@ void shifted_prefix_product_device(Fr *out, const Fr *in, size_t len, const Host
 
 using namespace mh;
 
-#define MH_TRY try {
-#define MH_CATCH                                                     \
-    }                                                                \
-    catch (const hip_error &e) { return make_error(e.code, e.what()); } \
-    catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
-
 extern "C" RustError midnight_sppark_bls12_381_fr_shifted_prefix_product(RawFr *values, RawFr initial, size_t len)
 {
     if (len == 0) return ok();

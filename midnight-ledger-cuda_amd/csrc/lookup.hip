@@ -280,12 +280,6 @@ bool lookup_permute_device(Fr *trace, size_t len, size_t lookup_count, size_t us
 
 using namespace mh;
 
-#define MH_TRY try {
-#define MH_CATCH                                                     \
-    }                                                                \
-    catch (const hip_error &e) { return make_error(e.code, e.what()); } \
-    catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
-
 // bls12_381_g1_msm.cu:6617-6676: values sorted by canonical value, in place
 extern "C" RustError midnight_sppark_bls12_381_fr_sort_canonical(RawFr *values, size_t len)
 {

@@ -1082,12 +1082,6 @@ RawG1Projective identity_out()
 
 }  // namespace
 
-#define MH_TRY try {
-#define MH_CATCH                                                                         \
-    }                                                                                    \
-    catch (const hip_error &e) { return make_error(e.code, e.what()); }                  \
-    catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
-
 namespace {
 
 // Resident contexts precompute per-window point copies (worth it because a

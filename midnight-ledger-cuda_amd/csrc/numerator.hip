@@ -500,12 +500,6 @@ struct NumeratorContext {
 
 using namespace mh;
 
-#define MH_TRY try {
-#define MH_CATCH                                                     \
-    }                                                                \
-    catch (const hip_error &e) { return make_error(e.code, e.what()); } \
-    catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
-
 extern "C" RustError midnight_sppark_bls12_381_eval_numerator(
     RawFr *out, size_t len, const RawFr *initial_out, uint32_t run_custom, const RawFr *fixed,
     size_t fixed_cols, const RawFr *advice, size_t advice_cols, const RawFr *instance, size_t instance_cols,

@@ -388,12 +388,6 @@ void poly_q_combine(Fr *out, const std::vector<const Fr *> &polys, size_t len, c
 
 using namespace mh;
 
-#define MH_TRY try {
-#define MH_CATCH                                                     \
-    }                                                                \
-    catch (const hip_error &e) { return make_error(e.code, e.what()); } \
-    catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
-
 namespace {
 
 std::unique_ptr<DeviceVec> new_devvec(size_t len)

@@ -669,12 +669,6 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
 
 using namespace mh;
 
-#define MH_TRY try {
-#define MH_CATCH                                                     \
-    }                                                                \
-    catch (const hip_error &e) { return make_error(e.code, e.what()); } \
-    catch (const std::bad_alloc &) { return make_error((int)hipErrorOutOfMemory, "host allocation failed"); }
-
 extern "C" RustError midnight_hip_plonk_keygen(void **out, uint32_t k, void *srs_context, size_t srs_points,
                                                uint32_t degree, const RawFr *fixed, size_t fixed_cols,
                                                size_t advice_cols, const uint32_t *advice_queries,

@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <new>
 #include <stdexcept>
 #include <string>
 
@@ -35,6 +36,16 @@ struct hip_error : std::runtime_error {
 
 RustError make_error(int code, const char *msg);
 inline RustError ok() { return RustError{0, nullptr}; }
+
+// Body of a C-ABI entry point: HIP failures and rejected arguments become
+// their error code, host allocation failures hipErrorOutOfMemory.
+#define MH_TRY try {
+#define MH_CATCH                                                                            \
+    }                                                                                       \
+    catch (const ::mh::hip_error &e) { return ::mh::make_error(e.code, e.what()); }         \
+    catch (const std::bad_alloc &) {                                                        \
+        return ::mh::make_error((int)hipErrorOutOfMemory, "host allocation failed");        \
+    }
 
 // Proof execution lanes (the reference keeps one private device context per
 // lane, bls12_381_g1_msm.cu:46-146).  Here a lane is one HIP stream; every

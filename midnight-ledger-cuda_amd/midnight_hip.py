@@ -179,6 +179,12 @@ def _declare(L):
         "midnight_sppark_bls12_381_fr_lookup_products_device": (
             E, [vpp, vp, c_size, c_size, RawFr, RawFr, vp, c_size]),
         "midnight_sppark_bls12_381_fr_sort_canonical": (E, [u64p, c_size]),
+        "midnight_sppark_bls12_381_fr_poseidon_hash_chain": (
+            E, [u64p, c_size, RawFr, RawFr, RawFr, RawFr, u64p, c_size, u64p, c_size]),
+        "midnight_sppark_bls12_381_fr_poseidon_hash_chain_witness": (
+            E, [u64p, c_size, RawFr, RawFr, RawFr, RawFr, u64p, c_size, u64p, c_size]),
+        "midnight_sppark_bls12_381_fr_poseidon_advice_from_witness": (
+            E, [vpp, u64p, c_size, c_size, u64p, c_size, ctypes.POINTER(ctypes.c_uint16), c_size]),
         "midnight_sppark_bls12_381_fr_lookup_permute_batch": (
             E, [vpp, u64p, u64p, c_size, c_size, c_size, vp, vp, c_size]),
         "midnight_sppark_bls12_381_trace_context_lookup_permute": (
@@ -863,6 +869,44 @@ def fr_sort_canonical(values):
     a = np.array(_fr_array(values), copy=True)
     check(lib().midnight_sppark_bls12_381_fr_sort_canonical(P(a), a.shape[0]))
     return a
+
+
+POSEIDON_HASH_WITNESS_LEN = 256
+
+
+def _poseidon_chain(fn, per_hash, length, initial, tail0, tail1, input_len, mds, round_constants):
+    m = _fr_array(mds)
+    rc = _fr_array(round_constants)
+    out = np.zeros((max(length * per_hash, 1), 4), dtype=np.uint64)
+    check(fn(P(out), length, RawFr.of(initial), RawFr.of(tail0), RawFr.of(tail1), RawFr.of(input_len),
+             P(m), m.shape[0], P(rc), rc.shape[0]))
+    return out[:length * per_hash]
+
+
+def fr_poseidon_hash_chain(length, initial, tail0, tail1, input_len, mds, round_constants):
+    """(length, 4) chain outputs: hash i permutes [previous, tail0, input_len], adds tail1 to lane 0
+    and permutes again; previous starts at `initial`."""
+    return _poseidon_chain(lib().midnight_sppark_bls12_381_fr_poseidon_hash_chain, 1, length, initial, tail0,
+                           tail1, input_len, mds, round_constants)
+
+
+def fr_poseidon_hash_chain_witness(length, initial, tail0, tail1, input_len, mds, round_constants):
+    """(length * 256, 4) circuit witness of the same chain"""
+    return _poseidon_chain(lib().midnight_sppark_bls12_381_fr_poseidon_hash_chain_witness,
+                           POSEIDON_HASH_WITNESS_LEN, length, initial, tail0, tail1, input_len, mds,
+                           round_constants)
+
+
+def fr_poseidon_advice_from_witness(witness, hash_count, domain_len, metadata, layout):
+    """DeviceVec of 8 * domain_len advice values (column-major) built from a chain witness"""
+    w = _fr_array(witness)
+    meta = _fr_array(metadata)
+    lay = np.ascontiguousarray(layout, dtype=np.uint16).reshape(-1)
+    h = ctypes.c_void_p()
+    check(lib().midnight_sppark_bls12_381_fr_poseidon_advice_from_witness(
+        ctypes.byref(h), P(w), hash_count, domain_len, P(meta), meta.shape[0],
+        lay.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), lay.shape[0]))
+    return DeviceVec(h.value)
 
 
 def _blinds_or_empty(b):

@@ -211,6 +211,17 @@ void orc_fr_sort_canonical(orc_fr *out, const orc_fr *v, size_t n);
 /* permute_expression_pair (lookup/prover.rs:1573-1690) over `usable` rows; 0 if an input is not in the table */
 int orc_lookup_permute(orc_fr *pin, orc_fr *ptab, const orc_fr *input, const orc_fr *table, size_t usable);
 
+/* ---- Poseidon (bls12_381_g1_msm.cu:2018-2216): width 3, 8 full + 60 partial rounds, x^5 ---- */
+/* permutation in place; w (128 values) or NULL */
+void orc_poseidon_permutation(orc_fr state[3], const orc_fr *mds, const orc_fr *rc, orc_fr *w);
+/* dependent chain of len two-permutation hashes; out (len) and witness (len * 256) may be NULL */
+void orc_poseidon_hash_chain(orc_fr *out, orc_fr *witness, size_t len, const orc_fr *initial,
+                             const orc_fr *tail0, const orc_fr *tail1, const orc_fr *input_len,
+                             const orc_fr *mds, const orc_fr *rc);
+/* 8 * domain_len advice values from the chain witness, metadata (11) and packed layout (128) */
+void orc_poseidon_advice_from_witness(orc_fr *out, const orc_fr *witness, size_t hash_count, size_t domain_len,
+                                      const orc_fr *metadata, const uint16_t *layout);
+
 /* shifted_prefix_product (permutation/prover.rs:36-43); out may not alias values */
 void orc_shifted_prefix_product(orc_fr *out, const orc_fr *values, size_t len, const orc_fr *initial);
 

@@ -105,6 +105,11 @@ def _declare(L):
         "orc_shifted_prefix_product": (V, [u64p, u64p, ctypes.c_size_t, u64p]),
         "orc_fr_sort_canonical": (V, [u64p, u64p, ctypes.c_size_t]),
         "orc_lookup_permute": (ctypes.c_int, [u64p, u64p, u64p, u64p, ctypes.c_size_t]),
+        "orc_poseidon_permutation": (V, [u64p, u64p, u64p, ctypes.c_void_p]),
+        "orc_poseidon_hash_chain": (V, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, u64p, u64p, u64p,
+                                        u64p, u64p, u64p]),
+        "orc_poseidon_advice_from_witness": (V, [u64p, u64p, ctypes.c_size_t, ctypes.c_size_t, u64p,
+                                                 ctypes.POINTER(ctypes.c_uint16)]),
         "orc_lookup_products": (V, [u64p, u64p, u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_size_t, u64p, u64p,
                                     u64p, ctypes.c_size_t]),
         "orc_poly_eval": (V, [u64p, u64p, ctypes.c_size_t, u64p]),
@@ -561,6 +566,43 @@ def lookup_permute(inputs, table, usable):
     if not lib().orc_lookup_permute(P(pin), P(ptab), P(a), P(t), usable):
         return None
     return pin, ptab
+
+
+def _fr_arr(x, count=None):
+    a = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, 4)
+    if count is not None and a.shape[0] != count:
+        raise ValueError(f"expected {count} field elements, got {a.shape[0]}")
+    return a
+
+
+def poseidon_permutation(state, mds, rc, witness=False):
+    """orc_poseidon_permutation: the permuted (3, 4) state, and its 128 witness values when asked"""
+    st = np.array(_fr_arr(state, 3), copy=True)
+    w = np.zeros((128, 4), dtype=np.uint64) if witness else None
+    lib().orc_poseidon_permutation(P(st), P(_fr_arr(mds, 9)), P(_fr_arr(rc, 204)),
+                                   None if w is None else w.ctypes.data)
+    return (st, w) if witness else st
+
+
+def poseidon_hash_chain(length, initial, tail0, tail1, input_len, mds, rc, witness=False):
+    """orc_poseidon_hash_chain: (length, 4) outputs, or (length * 256, 4) witness values"""
+    out = np.zeros((length, 4), dtype=np.uint64)
+    w = np.zeros((length * 256, 4), dtype=np.uint64) if witness else None
+    args = [np.ascontiguousarray(v, dtype=np.uint64) for v in (initial, tail0, tail1, input_len)]
+    lib().orc_poseidon_hash_chain(out.ctypes.data, None if w is None else w.ctypes.data, length,
+                                  *[P(a) for a in args], P(_fr_arr(mds, 9)), P(_fr_arr(rc, 204)))
+    return w if witness else out
+
+
+def poseidon_advice_from_witness(witness, hash_count, domain_len, metadata, layout):
+    """orc_poseidon_advice_from_witness: (8 * domain_len, 4) column-major advice values"""
+    w = _fr_arr(witness, hash_count * 256)
+    lay = np.ascontiguousarray(layout, dtype=np.uint16)
+    assert lay.shape == (128,)
+    out = np.zeros((8 * domain_len, 4), dtype=np.uint64)
+    lib().orc_poseidon_advice_from_witness(P(out), P(w), hash_count, domain_len, P(_fr_arr(metadata, 11)),
+                                           lay.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
+    return out
 
 
 def lookup_products(ci, ct, pi, pt, n, count, beta, gamma, blinds, blinding):
