@@ -701,15 +701,89 @@ def cpu_baseline_quotient(seconds):
                       "%d threads, %.1f s" % (reps, log_s, threads, dt)}
 
 
+POSEIDON_ROUNDS = 68
+POSEIDON_STEPS_PER_ROUND = 3      # dependent Fr products per round on the chain kernel's critical path
+
+
+def bench_poseidon(args, H):
+    """Poseidon hash chain (width 3, 8 full + 60 partial rounds, x^5) of
+    --chain-len dependent hashes through fr_poseidon_hash_chain_witness, the
+    form the circuit-witness preprocessing calls: a step is one call --
+    parameters uploaded, the chain, its 256 witness values per hash copied to
+    the host.  MDS and round constants are uniform synthetic Fr (the work
+    does not depend on them).  The chain is sequential, so the bound is the
+    latency of one wavefront's dependent Fr products: `roofline` compares the
+    critical-path products per second (3 per round, 136 rounds per hash)
+    with the single-wave dependent-product rate measured on the same device."""
+    length = args.chain_len
+    rng = np.random.default_rng(args.seed)
+    # values in [0, r) are the Montgomery forms of uniform field elements
+    mds_m, rc_m, init_m = random_canonical(9, rng), random_canonical(204, rng), random_canonical(4, rng)
+
+    def step():
+        return H.fr_poseidon_hash_chain_witness(length, init_m[0], init_m[1], init_m[2], init_m[3], mds_m, rc_m)
+
+    for _ in range(args.warmup):
+        step()
+    H.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dt = (time.perf_counter() - t0) / args.steps
+    k_ms, k_cnt = H.profile_read("poseidon_chain")
+    H.profile_enable(False)
+    t_kernel = k_ms / 1e3 / max(k_cnt, 1)
+    _, fr_rate = H.field_mul_rate(1)
+    lat = 64.0 / fr_rate                        # seconds per dependent product on one wave
+    products = POSEIDON_STEPS_PER_ROUND * POSEIDON_ROUNDS * 2 * length
+    res = {
+        "metric": "Poseidon hash-chain hashes/s (chain of %d, with witness)" % length,
+        "value": round(length / dt, 1), "unit": "hashes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic uniform Fr parameters and chain inputs",
+        "config": {"workload": "Poseidon chain of %d hashes (2 permutations each), witness to host" % length,
+                   "hashes": length},
+        "roofline": {"bound": "valu-latency", "kernel": "k_poseidon_chain",
+                     "achieved": round(products / t_kernel / 1e6, 3), "peak": round(1.0 / lat / 1e6, 3),
+                     "unit": "M dependent Fr products/s", "frac": round(products * lat / t_kernel, 4),
+                     "traffic": None},
+        "kernel_ms_avg": round(t_kernel * 1e3, 3),
+        "us_per_hash": round(t_kernel / length * 1e6, 2),
+        "fr_product_latency_ns": round(lat * 1e9, 1),
+    }
+    if args.cpu_seconds > 0:
+        log("timing the CPU baseline")
+        res["cpu_baseline"] = cpu_baseline_poseidon(args.cpu_seconds, mds_m, rc_m, init_m)
+    return res
+
+
+def cpu_baseline_poseidon(seconds, mds_m, rc_m, init_m):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    sample = 256
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        O.poseidon_hash_chain(sample, init_m[0], init_m[1], init_m[2], init_m[3], mds_m, rc_m, witness=True)
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    return {"value": round(reps * sample / dt, 1), "unit": "hashes/s", "cores": 1, "kind": "port",
+            "sample": "%d chains of %d hashes with witness, oracle poseidon_hash_chain, 1 thread, %.1f s"
+                      % (reps, sample, dt)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="msm", choices=["msm", "ntt", "batch", "quotient", "kzg", "serve", "prove"])
+    ap.add_argument("--workload", default="msm", choices=["msm", "ntt", "batch", "quotient", "kzg", "serve", "prove",
+                                                         "poseidon"])
     ap.add_argument("--log-n", type=int, default=20)
     ap.add_argument("--msms-per-step", type=int, default=4)
     ap.add_argument("--skewed", action="store_true")
+    ap.add_argument("--chain-len", type=int, default=1024, help="poseidon: hashes in the chain")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--requests", type=int, default=200, help="serving: requests in the stream")
     ap.add_argument("--slots", type=int, default=8, help="serving: concurrent proofs")
@@ -734,7 +808,8 @@ def main():
         os.environ["GPU_MAX_HW_QUEUES"] = str(max(int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), want))
     import midnight_hip as H
     H.lib()
-    run = {"msm": bench_msm, "ntt": bench_ntt, "batch": bench_batch, "quotient": bench_quotient, "kzg": bench_kzg, "serve": bench_serve, "prove": bench_prove}[args.workload]
+    run = {"msm": bench_msm, "ntt": bench_ntt, "batch": bench_batch, "quotient": bench_quotient, "kzg": bench_kzg, "serve": bench_serve, "prove": bench_prove,
+           "poseidon": bench_poseidon}[args.workload]
     res = run(args, H)
     print(json.dumps(res), flush=True)
 

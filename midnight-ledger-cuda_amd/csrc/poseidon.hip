@@ -5,19 +5,25 @@
 // Width-3 Poseidon with 4 + 4 full rounds around 60 partial rounds, S-box
 // x^5, caller-supplied MDS (row-major 3 x 3) and round constants (68 x 3).
 // A chain is strictly sequential -- every hash starts from the previous
-// output -- so one wavefront runs it and the question is the latency of a
-// round, not throughput.  Lane k < 9 owns one MDS product
-// mds[l][c] * sbox(state[c]) (l = k / 3, c = k % 3) and keeps its own copy
-// of state[c]: the S-box of every lane of column c runs at once, the nine
-// products run at once, and one LDS exchange gives each lane its next
-// state[c] = rc[r + 1][c] + sum_j prod[c][j].  A round then costs the wave
-// four field multiplications (x^2, x^3, x^5, product) where one lane would
-// issue twelve to eighteen.  The 204 round constants sit in LDS.
+// output -- so one wavefront runs it and what counts is the latency of a
+// round.  Every lane keeps its own copy of one state element state[c]:
+//   lanes 0..8   (l = k / 3, c = k % 3) own the MDS product mds[l][c] * s_c,
+//   lanes 9..11  (c = k - 9) square state[c] for the product lanes and form
+//                the S-box cubes the witness records.
+// With the product taken as ((m x) x^2) x^2 = m x^5, a full round is three
+// multiplication steps that every lane runs at once -- (m x | x^2),
+// (m x^3 | x^3), (m x^5) -- with one LDS exchange of x^2 between steps one
+// and two, where forming x^5 first and multiplying by m after is four; five
+// needs three steps in any order (2^2 < 5).  A second LDS exchange sums the
+// products: state'[c] = rc[r + 1][c] + sum_j prod[c][j].  Partial rounds
+// take the same steps with only column 2 nonlinear.  The 204 round
+// constants sit in LDS.
 #include <cstring>
 #include <memory>
 
 #include "devvec.hpp"
 #include "fields.hpp"
+#include "profile.hpp"
 #include "runtime.hpp"
 
 namespace mh {
@@ -35,26 +41,32 @@ constexpr size_t kLayoutLen = 128;
 
 __device__ __forceinline__ bool is_full_round(int r) { return r < 4 || r >= 64; }
 
-// One permutation on the nine product lanes.  `x` is this lane's copy of
-// state[c]; `w` (nullable) receives the permutation's 128 witness values,
-// written by the three lanes of row l = 0.
-__device__ __forceinline__ void permute(Fr &x, const Fr &m, int l, int c, int k, const Fr *s_rc,
+// One permutation.  `x` is this lane's copy of state[c]; `w` (nullable)
+// receives the permutation's 128 witness values: the cubes from the
+// squaring lanes, the post-MDS states from the product lanes of row 0.
+__device__ __forceinline__ void permute(Fr &x, const Fr &m, int l, int c, int k, const Fr *s_rc, Fr *s_sq,
                                         Fr (*s_prod)[9], Fr *w)
 {
     x = x + s_rc[c];
     size_t off = 0;
-    const bool writer = w != nullptr && l == 0;
+    const bool product_lane = k < 9;
+    const bool writer = w != nullptr && product_lane && l == 0;
     for (int r = 0; r < kRounds; ++r) {
         const bool full = is_full_round(r);
-        Fr s = x;
-        if (full || c == 2) {
-            Fr sq = x * x;
-            Fr cube = sq * x;
-            if (writer && full) store_fr(w + off + c, cube);
-            s = cube * sq;
+        const bool nonlinear = full || c == 2;
+        // step 1: m x on the product lanes, x^2 on the squaring lanes
+        Fr y = (product_lane ? m : x) * x;
+        if (!product_lane) store_fr(s_sq + c, y);
+        __syncthreads();
+        if (nonlinear) {
+            // step 2: m x^3 | x^3;  step 3: m x^5
+            const Fr sq = product_lane ? s_sq[c] : y;
+            y = y * (product_lane ? sq : x);
+            if (!product_lane && full && w != nullptr) store_fr(w + off + c, y);
+            if (product_lane) y = y * sq;
         }
         Fr *prod = s_prod[r & 1];
-        store_fr(prod + k, m * s);
+        if (product_lane) store_fr(prod + k, y);
         __syncthreads();
         Fr nx = s_rc[(r + 1) * kWidth + c];
         nx = nx + prod[c * kWidth + 0];
@@ -77,7 +89,7 @@ __device__ __forceinline__ void permute(Fr &x, const Fr &m, int l, int c, int k,
     }
 }
 
-// One block of 64 threads (a single wavefront); lanes 9..63 only help load
+// One block of 64 threads (a single wavefront); lanes 12..63 only help load
 // the round constants.
 __global__ __launch_bounds__(64) void k_poseidon_chain(Fr *out, Fr *witness, Fr initial, Fr tail0, Fr tail1,
                                                        Fr input_len, const Fr *mds, const Fr *rc, size_t len)
@@ -86,20 +98,21 @@ __global__ __launch_bounds__(64) void k_poseidon_chain(Fr *out, Fr *witness, Fr 
     // gets no constant)
     __shared__ Fr s_rc[(kRounds + 1) * kWidth];
     __shared__ Fr s_prod[2][9];
+    __shared__ Fr s_sq[kWidth];
     const int k = threadIdx.x;
     for (int i = k; i < (kRounds + 1) * kWidth; i += blockDim.x)
         s_rc[i] = i < kRounds * kWidth ? load_fr(rc + i) : Fr::zero();
     __syncthreads();
-    if (k >= 9) return;
-    const int l = k / kWidth, c = k % kWidth;
-    const Fr m = load_fr(mds + l * kWidth + c);
+    if (k >= 12) return;
+    const int l = k < 9 ? k / kWidth : 0, c = k < 9 ? k % kWidth : k - 9;
+    const Fr m = k < 9 ? load_fr(mds + l * kWidth + c) : Fr::zero();
     Fr previous = initial;
     for (size_t index = 0; index < len; ++index) {
         Fr x = c == 2 ? input_len : Fr::zero() + (c == 0 ? previous : tail0);
         Fr *w = witness ? witness + index * kHashWitness : nullptr;
-        permute(x, m, l, c, k, s_rc, s_prod, w);
+        permute(x, m, l, c, k, s_rc, s_sq, s_prod, w);
         if (c == 0) x = x + tail1;
-        permute(x, m, l, c, k, s_rc, s_prod, w ? w + kPermWitness : nullptr);
+        permute(x, m, l, c, k, s_rc, s_sq, s_prod, w ? w + kPermWitness : nullptr);
         previous = x;  // meaningful on the c == 0 lanes
         if (k == 0 && out) store_fr(out + index, previous);
     }
@@ -171,13 +184,17 @@ void run_chain(RawFr *out, size_t len, const RawFr &initial, const RawFr &tail0,
     DevBuf<Fr> d_out(n_out), d_params(kWidth * kWidth + kRounds * kWidth);
     MH_HIP(hipMemcpyAsync(d_params.get(), mds, kWidth * kWidth * 32, hipMemcpyHostToDevice, st));
     MH_HIP(hipMemcpyAsync(d_params.get() + kWidth * kWidth, rc, kRounds * kWidth * 32, hipMemcpyHostToDevice, st));
+    ProfileScope prof;
+    prof.begin("poseidon_chain", st);
     hipLaunchKernelGGL(k_poseidon_chain, dim3(1), dim3(64), 0, st, with_witness ? nullptr : d_out.get(),
                        with_witness ? d_out.get() : nullptr, to_fr(initial), to_fr(tail0), to_fr(tail1),
                        to_fr(input_len), (const Fr *)d_params.get(), (const Fr *)d_params.get() + kWidth * kWidth,
                        len);
     MH_HIP(hipGetLastError());
+    prof.end(st);
     MH_HIP(hipMemcpyAsync(out, d_out.get(), n_out * 32, hipMemcpyDeviceToHost, st));
     MH_HIP(hipStreamSynchronize(st));
+    prof.flush();
 }
 
 }  // namespace
