@@ -95,6 +95,7 @@ __device__ __forceinline__ void fp28_reduce(Fp28 &x)
 // independent multiply-adds), so consecutive instructions do not wait on
 // one another; only the 28-bit carry of the lowest word moves up per row.
 // Each accumulator collects at most 28 products below 2^56 (< 2^61).
+template <bool kReduce = true>
 __device__ __forceinline__ Fp28 fp28_mul(const Fp28 &a, const Fp28 &b)
 {
     uint64_t t[14];
@@ -122,7 +123,7 @@ __device__ __forceinline__ Fp28 fp28_mul(const Fp28 &a, const Fp28 &b)
         r.v[j] = (uint32_t)c & kM28;
         c >>= 28;
     }
-    fp28_reduce(r);
+    if (kReduce) fp28_reduce(r);
     return r;
 }
 
@@ -168,6 +169,7 @@ __device__ __forceinline__ Fp28 fp28_mul_fips(const Fp28 &a, const Fp28 &b)
 }
 
 // squaring: cross products once, doubled, plus the diagonal
+template <bool kReduce = true>
 __device__ __forceinline__ Fp28 fp28_sqr(const Fp28 &a)
 {
     uint64_t acc = 0;
@@ -199,7 +201,7 @@ __device__ __forceinline__ Fp28 fp28_sqr(const Fp28 &a)
         acc >>= 28;
     }
     r.v[13] = (uint32_t)acc;
-    fp28_reduce(r);
+    if (kReduce) fp28_reduce(r);
     return r;
 }
 
@@ -346,6 +348,109 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_affine(const Xyzz28 &p, const Affin
     r.y = fp28_sub(fp28_mul(r_, fp28_sub(qq, r.x)), fp28_mul(p.y, ppp));
     r.zz = fp28_mul(p.zz, pp);
     r.zzz = fp28_mul(p.zzz, ppp);
+    return r;
+}
+
+// ---- loose arithmetic for the accumulation loop.  Values lie in [0, 2p)
+// instead of [0, p): a Montgomery product of two operands below 2p is
+// a b / R + m p / R < 4 p^2 / 2^392 + p < 2p (p < 2^381), so products skip
+// the final subtraction (56 of the ~583 instructions of a product); sums and
+// differences are brought back below 2p with one correction by 2p, as the
+// exact forms are with p.  A loose value is zero mod p iff it is 0 or p.
+
+__device__ __forceinline__ uint32_t p2_28(int i)   // 2p
+{
+    constexpr uint32_t P2[14] = {0xfff5556u, 0xfdfffffu, 0x7ffff73u, 0xfffd62au, 0xc483d57u, 0x41ed61eu, 0xece61a5u,
+                                 0xe70a257u, 0x8ee9709u, 0x9759aecu, 0x74f6c86u, 0xcd34963u, 0x3d472ffu, 0x0034022u};
+    return P2[i];
+}
+
+// x - 2p if x >= 2p (x < 4p)
+__device__ __forceinline__ void fp28_reduce2(Fp28 &x)
+{
+    uint32_t d[14];
+    int32_t bw = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        int32_t t = (int32_t)x.v[i] - (int32_t)p2_28(i) - bw;
+        bw = (int32_t)((uint32_t)t >> 31);
+        d[i] = (uint32_t)t & kM28;
+    }
+#pragma unroll
+    for (int i = 0; i < 14; ++i) x.v[i] = bw ? x.v[i] : d[i];
+}
+
+__device__ __forceinline__ Fp28 fp28_add_lz(const Fp28 &a, const Fp28 &b)
+{
+    Fp28 r;
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        uint32_t s = a.v[i] + b.v[i] + c;
+        c = s >> 28;
+        r.v[i] = s & kM28;
+    }
+    fp28_reduce2(r);
+    return r;
+}
+
+__device__ __forceinline__ Fp28 fp28_sub_lz(const Fp28 &a, const Fp28 &b)
+{
+    Fp28 r;
+    int32_t bw = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        int32_t t = (int32_t)a.v[i] - (int32_t)b.v[i] - bw;
+        bw = (int32_t)((uint32_t)t >> 31);
+        r.v[i] = (uint32_t)t & kM28;
+    }
+    // borrowed: add 2p back
+    const uint32_t mask = 0u - (uint32_t)bw;
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        uint32_t s = r.v[i] + (p2_28(i) & mask) + c;
+        c = s >> 28;
+        r.v[i] = s & kM28;
+    }
+    return r;
+}
+
+__device__ __forceinline__ bool fp28_is_zero_lz(const Fp28 &a)
+{
+    uint32_t z = 0, e = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        z |= a.v[i];
+        e |= a.v[i] ^ p28(i);
+    }
+    return z == 0 || e == 0;
+}
+
+// madd-2008-s on loose coordinates: p (accumulator, loose, identity when
+// zz == 0 exactly) + q (affine, reduced).  The identity is only ever made
+// by xyzz28_inf, and zz of a sum is zz * PP with PP nonzero, so the exact
+// zero test on zz stays valid.
+__device__ __forceinline__ Xyzz28 xyzz28_add_affine_lz(const Xyzz28 &p, const Affine28 &q)
+{
+    if (q.is_inf()) return p;
+    if (p.is_inf()) return xyzz28_from_affine(q);
+    Fp28 u2 = fp28_mul<false>(q.x, p.zz);
+    Fp28 s2 = fp28_mul<false>(q.y, p.zzz);
+    Fp28 pp_ = fp28_sub_lz(u2, p.x);
+    Fp28 r_ = fp28_sub_lz(s2, p.y);
+    if (fp28_is_zero_lz(pp_)) {
+        if (fp28_is_zero_lz(r_)) return xyzz28_dbl_affine(q);
+        return xyzz28_inf();
+    }
+    Fp28 pp = fp28_sqr<false>(pp_);
+    Fp28 ppp = fp28_mul<false>(pp_, pp);
+    Fp28 qq = fp28_mul<false>(p.x, pp);
+    Xyzz28 r;
+    r.x = fp28_sub_lz(fp28_sub_lz(fp28_sqr<false>(r_), ppp), fp28_add_lz(qq, qq));
+    r.y = fp28_sub_lz(fp28_mul<false>(r_, fp28_sub_lz(qq, r.x)), fp28_mul<false>(p.y, ppp));
+    r.zz = fp28_mul<false>(p.zz, pp);
+    r.zzz = fp28_mul<false>(p.zzz, ppp);
     return r;
 }
 

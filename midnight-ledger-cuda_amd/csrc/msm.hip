@@ -364,8 +364,9 @@ __global__ void __launch_bounds__(kBlock) k_accumulate28(const uint32_t *pts, co
     for (uint32_t x = start; x < end; ++x) {
         Affine28 cur = nxt;
         if (x + 1 < end) nxt = fetch_point28(pts, ent[x + 1]);
-        acc = xyzz28_add_affine(acc, cur);
+        acc = xyzz28_add_affine_lz(acc, cur);
     }
+    // the conversion product fully reduces the loose coordinates
     const Xyzz out = xyzz28_to_std(acc);
     if (q[b] == 1) store_xyzz(bucket + (uint64_t)b * 48, out);
     else store_xyzz(part + (uint64_t)j * 48, out);

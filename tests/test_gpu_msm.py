@@ -223,6 +223,29 @@ def test_msm_precomputed_context_modes(H, O, log_n):
     ctx.free()
 
 
+def test_msm_precomputed_repeats_and_cancellations(H, O):
+    """Precomputed-context accumulation (loose coordinates in [0, 2p)) on
+    buckets that repeat one point (doubling inside a chunk: the difference
+    of equal x coordinates may come out as p rather than 0) and that
+    alternate Q and -Q (the running sum returns to the identity mid-chunk
+    and restarts)."""
+    n = 1 << 12
+    ts = O.random_fr(n, 330)
+    sc = O.random_fr(n, 331)
+    t0, s0 = ts[0].copy(), sc[0].copy()
+    ts[:1024] = t0                                      # one point 1024 times, one scalar
+    sc[:1024] = s0
+    t1, s1 = ts[1024].copy(), sc[1024].copy()
+    neg_t1 = O.fr_op("neg", t1)
+    ts[1024:2048:2] = neg_t1                            # -Q, Q, -Q, Q, ... with equal scalars
+    ts[1025:2048:2] = t1
+    sc[1024:2048] = s1
+    ctx = H.MsmContext.from_multiples(H.DeviceVec.upload(ts))
+    out = ctx.batch(sc, n, 1)
+    assert O.g1_eq(out[0], O.g1_mul(O.generator(), O.inner_product(sc, ts)))
+    ctx.free()
+
+
 def _skewed_montgomery(O, sc, seed):
     """The 2^24 workload's skewed mix, in place on Montgomery scalars:
     70% in [0, 2^16), 20% in {0, 1}, 10% left uniform."""
