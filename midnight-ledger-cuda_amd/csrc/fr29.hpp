@@ -1,0 +1,118 @@
+// BLS12-381 Fr products in radix 2^29 for the NTT.
+//
+// Nine 29-bit limbs (261 bits).  The product is CIOS Montgomery
+// multiplication with 64-bit lazy accumulators, as the radix-2^28 Fp code
+// does it: a 29 x 29-bit product is below 2^58 and every accumulator
+// collects at most 18 of them (< 2^62.2), so each product is one
+// v_mad_u64_u32 with the accumulator as addend -- no carry flags, no zero
+// high words.  r = 1 mod 2^32 makes the per-row quotient m = -t0 mod 2^29
+// (no multiply) and the lowest limb of r equal to 1 (no multiply).
+//
+// The Montgomery radix of this product is 2^261, not the 2^256 of the rest
+// of the code.  The NTT multiplies a data value in the usual form a 2^256
+// by a twiddle stored as w 2^261 mod r:
+//   mont29(a 2^256, w 2^261) = a w 2^256 2^261 / 2^261 = (a w) 2^256,
+// exactly the usual form of the product, so data never changes
+// representation -- only the twiddle tables (and the NTT's constant
+// factors) are scaled by 2^5 once on the host.  The result is fully
+// reduced, so it is limb-identical to the 32-bit product.
+#pragma once
+#include "fields.hpp"
+
+namespace mh {
+
+struct Fr29 {
+    uint32_t v[9];
+};
+
+constexpr uint32_t kM29 = 0x1fffffffu;
+
+__device__ __forceinline__ uint32_t r29(int i)
+{
+    constexpr uint32_t R[9] = {0x00000001u, 0x1ffffff8u, 0x1f96ffbfu, 0x1b4805ffu, 0x1d80553bu,
+                               0x0c0404d0u, 0x1520cce7u, 0x0a6533afu, 0x0073eda7u};
+    return R[i];
+}
+
+// same integer, 8 x 32-bit limbs -> 9 x 29-bit limbs
+__device__ __forceinline__ Fr29 fr29_from32(const Fr &x)
+{
+    Fr29 r;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int bit = 29 * j, q = bit >> 5, s = bit & 31;
+        const uint32_t lo = x.v[q];
+        const uint32_t hi = q + 1 < 8 ? x.v[q + 1] : 0u;
+        r.v[j] = (uint32_t)((((uint64_t)hi << 32) | lo) >> s) & kM29;
+    }
+    return r;
+}
+
+// 9 x 29-bit limbs (value below 2^256) -> 8 x 32-bit limbs
+__device__ __forceinline__ Fr fr29_to32(const Fr29 &x)
+{
+    Fr r;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+        const int bit = 32 * w, j = bit / 29, s = bit % 29;
+        uint64_t acc = (uint64_t)(x.v[j] >> s);
+        int have = 29 - s;
+        if (j + 1 < 9) acc |= (uint64_t)x.v[j + 1] << have;
+        if (have + 29 < 32 && j + 2 < 9) acc |= (uint64_t)x.v[j + 2] << (have + 29);
+        r.v[w] = (uint32_t)acc;
+    }
+    return r;
+}
+
+// a * b / 2^261 mod r, fully reduced; a < 2^261 (any 9-limb value), b < r
+// (then the running value stays below a b / 2^261 + r < 2r).
+__device__ __forceinline__ Fr29 fr29_mul(const Fr29 &a, const Fr29 &b)
+{
+    uint64_t t[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) t[j] = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) t[j] += (uint64_t)a.v[j] * b.v[i];
+        const uint32_t m = (0u - (uint32_t)t[0]) & kM29;
+        t[0] += m;   // r_0 = 1
+#pragma unroll
+        for (int j = 1; j < 9; ++j) t[j] += (uint64_t)m * r29(j);
+        // t[0] is a multiple of 2^29: shift the row down one limb
+        const uint64_t carry = t[0] >> 29;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) t[j] = t[j + 1];
+        t[0] += carry;
+        t[8] = 0;
+    }
+    Fr29 r;
+    uint64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        c += t[j];
+        r.v[j] = (uint32_t)c & kM29;
+        c >>= 29;
+    }
+    // r < 2 r_mod: subtract the modulus once if r >= modulus
+    uint32_t d[9];
+    int32_t bw = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        int32_t x = (int32_t)r.v[j] - (int32_t)r29(j) - bw;
+        bw = (int32_t)((uint32_t)x >> 31);
+        d[j] = (uint32_t)x & kM29;
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) r.v[j] = bw ? r.v[j] : d[j];
+    return r;
+}
+
+// (a w) in the usual form, for a in the usual form (8 x 32) and w stored as
+// w 2^261 mod r (8 x 32, fully reduced)
+__device__ __forceinline__ Fr fr_mul_scaled(const Fr &a, const Fr &w261)
+{
+    return fr29_to32(fr29_mul(fr29_from32(a), fr29_from32(w261)));
+}
+
+}  // namespace mh

@@ -23,6 +23,26 @@ def test_ntt_forward_inverse(H, O, log_n):
     assert np.array_equal(H.fr_ntt(fwd, log_n, inverse=True), x)
 
 
+@pytest.mark.parametrize("log_n", [4, 12, 15])
+def test_ntt_extreme_values(H, O, log_n):
+    """Inputs at the ends of the field (0, 1, r - 1, r - 2) and runs of
+    them: the twiddle products (radix 2^29, twiddles scaled by 2^5) take
+    their largest operands here."""
+    n = 1 << log_n
+    x = O.random_fr(n, 2000 + log_n)
+    one = O.fr_from_int(1)
+    r1 = O.fr_op("neg", one)
+    r2 = O.fr_op("sub", r1, one)
+    x[: n // 4] = r1
+    x[n // 4: n // 4 + 7] = [0, 0, 0, 0]
+    x[n // 2: n // 2 + 5] = one
+    x[3 * n // 4:: 3] = r2
+    for inv in (False, True):
+        got = H.fr_ntt(x, log_n, inverse=inv)
+        want = O.ifft(x, log_n) if inv else O.fft(x, log_n)
+        assert np.array_equal(got, want), "log_n=%d inverse=%s" % (log_n, inv)
+
+
 def test_ntt_2p22_roundtrip_and_spot(H, O):
     log_n = 22
     n = 1 << log_n
