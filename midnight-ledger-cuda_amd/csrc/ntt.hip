@@ -129,12 +129,36 @@ __device__ __forceinline__ Fr zeta_factor(const Fr &v, uint64_t idx, const Fr &f
     return v;
 }
 
+// LDS holds a pass's C * m elements as two planes of 16-byte halves
+// (element e: low half at plane[e], high half at plane[P + e]), so lanes
+// touching consecutive elements touch consecutive 16-byte words: a wave's
+// ds_read_b128 then covers whole bank rows instead of colliding 8 ways, as
+// contiguous 32-byte elements do.
+struct LdsPlanes {
+    uint4 *plane;
+    uint32_t P;
+    __device__ __forceinline__ Fr get(uint32_t e) const
+    {
+        const uint4 lo = plane[e], hi = plane[P + e];
+        Fr v;
+        v.v[0] = lo.x; v.v[1] = lo.y; v.v[2] = lo.z; v.v[3] = lo.w;
+        v.v[4] = hi.x; v.v[5] = hi.y; v.v[6] = hi.z; v.v[7] = hi.w;
+        return v;
+    }
+    __device__ __forceinline__ void put(uint32_t e, const Fr &v) const
+    {
+        plane[e] = make_uint4(v.v[0], v.v[1], v.v[2], v.v[3]);
+        plane[P + e] = make_uint4(v.v[4], v.v[5], v.v[6], v.v[7]);
+    }
+};
+
 // one workgroup of a pass: columns [bx * C, bx * C + C) of polynomial by
-__device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32_t by, Fr *lds)
+__device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32_t by, Fr *lds_mem)
 {
     const uint32_t m = 1u << a.log_m;
     const uint32_t C = a.C;
     const uint32_t total = C * m;
+    const LdsPlanes lds{reinterpret_cast<uint4 *>(lds_mem), total};
     const uint32_t c0 = bx * C;
     const Fr *in = a.in + (uint64_t)by * a.in_pstride;
     Fr *out = a.out + (uint64_t)by * a.out_pstride;
@@ -154,7 +178,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
         } else {
             v = Fr::zero();
         }
-        lds[c * m + bitrev(j, a.log_m)] = v;
+        lds.put(c * m + bitrev(j, a.log_m), v);
     }
     __syncthreads();
 
@@ -166,12 +190,12 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
             uint32_t c = b / half, bb = b % half;
             uint32_t low = bb & (h - 1);
             uint32_t i = ((bb >> s) << (s + 1)) | low;
-            Fr *col = lds + c * m;
-            Fr u = col[i];
-            Fr v = col[i + h];
+            const uint32_t e = c * m + i;
+            Fr u = lds.get(e);
+            Fr v = lds.get(e + h);
             if (low) v = fr_mul_scaled(v, a.roots[low << (kMaxLogLds - 1 - s)]);
-            col[i] = u + v;
-            col[i + h] = u - v;
+            lds.put(e, u + v);
+            lds.put(e + h, u - v);
         }
         __syncthreads();
     }
@@ -183,7 +207,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
         uint32_t c, k;
         if (store_cfast) { c = f % C; k = f / C; }
         else { k = f % m; c = f / m; }
-        Fr v = lds[c * m + k];
+        Fr v = lds.get(c * m + k);
         if (a.twiddle) {
             uint32_t e = ((c0 + c) * k) << tw_shift;
             if (e) {
