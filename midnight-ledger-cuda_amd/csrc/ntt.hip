@@ -49,7 +49,7 @@ namespace {
 
 constexpr int kMaxLogLds = 11;     // largest single-pass sub-transform
 constexpr int kTwLog = 22;         // twiddle tables are built for 2^22
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;   // 8 waves: two 64 KiB workgroups per CU give 4 waves per SIMD
 
 struct Tables {
     // each stored as value * 2^261 mod r (the radix-2^29 product's form)
