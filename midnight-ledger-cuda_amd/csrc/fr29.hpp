@@ -115,4 +115,10 @@ __device__ __forceinline__ Fr fr_mul_scaled(const Fr &a, const Fr &w261)
     return fr29_to32(fr29_mul(fr29_from32(a), fr29_from32(w261)));
 }
 
+// the same with w already repacked (one twiddle, several products)
+__device__ __forceinline__ Fr fr_mul_scaled29(const Fr &a, const Fr29 &w261)
+{
+    return fr29_to32(fr29_mul(fr29_from32(a), w261));
+}
+
 }  // namespace mh

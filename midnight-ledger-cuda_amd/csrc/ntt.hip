@@ -182,15 +182,44 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
     }
     __syncthreads();
 
-    // radix-2 DIT stages
-    const uint32_t half = m >> 1;
-    for (uint32_t s = 0; s < a.log_m; ++s) {
+    // DIT stages, two at a time: a thread takes the four elements
+    // e, e + h, e + 2h, e + 3h (h = 2^s, bits s and s+1 of e clear), does
+    // stage s on (e, e+h) and (e+2h, e+3h) -- one twiddle w_2h^low for both
+    // -- and stage s+1 on (e, e+2h) and (e+h, e+3h) -- twiddles w_4h^low and
+    // w_4h^(low+h) -- with one LDS round trip and one barrier for the pair.
+    // An odd last stage is a radix-2 one.
+    uint32_t s = 0;
+    for (; s + 1 < a.log_m; s += 2) {
         const uint32_t h = 1u << s;
+        const uint32_t quarter = m >> 2;
+        for (uint32_t g = threadIdx.x; g < C * quarter; g += blockDim.x) {
+            const uint32_t c = g / quarter, gg = g % quarter;
+            const uint32_t low = gg & (h - 1);
+            const uint32_t e = c * m + (((gg >> s) << (s + 2)) | low);
+            Fr x0 = lds.get(e), x1 = lds.get(e + h), x2 = lds.get(e + 2 * h), x3 = lds.get(e + 3 * h);
+            if (low) {
+                const Fr29 w = fr29_from32(a.roots[low << (kMaxLogLds - 1 - s)]);
+                x1 = fr_mul_scaled29(x1, w);
+                x3 = fr_mul_scaled29(x3, w);
+            }
+            const Fr y0 = x0 + x1, y1 = x0 - x1;
+            Fr y2 = x2 + x3, y3 = x2 - x3;
+            if (low) y2 = fr_mul_scaled(y2, a.roots[low << (kMaxLogLds - 2 - s)]);
+            y3 = fr_mul_scaled(y3, a.roots[(low + h) << (kMaxLogLds - 2 - s)]);
+            lds.put(e, y0 + y2);
+            lds.put(e + 2 * h, y0 - y2);
+            lds.put(e + h, y1 + y3);
+            lds.put(e + 3 * h, y1 - y3);
+        }
+        __syncthreads();
+    }
+    if (s < a.log_m) {
+        const uint32_t h = 1u << s;
+        const uint32_t half = m >> 1;
         for (uint32_t b = threadIdx.x; b < C * half; b += blockDim.x) {
-            uint32_t c = b / half, bb = b % half;
-            uint32_t low = bb & (h - 1);
-            uint32_t i = ((bb >> s) << (s + 1)) | low;
-            const uint32_t e = c * m + i;
+            const uint32_t c = b / half, bb = b % half;
+            const uint32_t low = bb & (h - 1);
+            const uint32_t e = c * m + (((bb >> s) << (s + 1)) | low);
             Fr u = lds.get(e);
             Fr v = lds.get(e + h);
             if (low) v = fr_mul_scaled(v, a.roots[low << (kMaxLogLds - 1 - s)]);
