@@ -217,7 +217,13 @@ void launch_digits(bool scatter, uint32_t c, uint64_t max_n, hipStream_t st, con
     const uint32_t keys_per_msm = g.kw * g.B;
     const uint32_t range_keys = std::min<uint32_t>(keys_per_msm, 16384);
     const uint32_t ranges = (keys_per_msm + range_keys - 1) / range_keys;
-    const uint32_t tile = max_n >= (1u << 18) ? 16384u : 4096u;
+    // tiles of up to 16384 scalars (each block flushes one global atomic per
+    // touched bucket, so bigger tiles mean fewer atomics), but at least ~512
+    // blocks so that a single large MSM still fills the device
+    uint64_t tile64 = 16384;
+    while (tile64 > 4096 && ceil_div(max_n, tile64) * g.M * ranges < 512) tile64 >>= 1;
+    if (max_n < (1u << 18)) tile64 = 4096;
+    const uint32_t tile = (uint32_t)tile64;
     dim3 grid((uint32_t)ceil_div(max_n, tile), g.M, ranges);
     const size_t lds = (size_t)range_keys * 4;
     switch (c) {
