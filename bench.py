@@ -25,10 +25,10 @@ sys.path.insert(0, os.path.join(ROOT, "midnight-ledger-cuda_amd"))
 R_MOD = 0x73eda753299d7d483339d80809a1d80553bda402fffe5bfeffffffff00000001
 R_LIMBS = np.array([(R_MOD >> (64 * i)) & (2**64 - 1) for i in range(4)], dtype=np.uint64)
 
-# Integer-VALU peak.  Spec: v_mad_u64_u32 at 64 lanes/clk/CU x 256 CUs x
-# 2.4 GHz.  The roofline uses the rate measured live on the box by
-# midnight_hip_valu_peak (lanes/clk/CU x CUs x clock, as the roofline of
-# this path is defined), which is below the spec figure.
+# Integer-VALU peak: v_mad_u64_u32 at 64 lanes/clk/CU x 256 CUs x 2.4 GHz
+# (the published CU count and peak clock).  `roofline.peak` and `frac` use
+# this figure; the rate measured live on the box by midnight_hip_valu_peak
+# (below it) is reported beside it as `peak_measured`.
 PEAK_SPEC_MAC_PER_S = 256 * 64 * 2.4e9
 HBM_PEAK = 8.0e12
 # algorithmic Montgomery-product cost of one Fp product (12 limbs, CIOS)
@@ -154,7 +154,8 @@ def bench_msm(args, H):
         ctx.device_batch(dv, 0, n, m)
     if args.pmc_copy_mib:
         copy_calibration(H, args.pmc_copy_mib)
-    peak, _, _ = H.valu_peak()
+    peak_measured, _, _ = H.valu_peak()
+    peak = PEAK_SPEC_MAC_PER_S
     H.profile_enable(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -189,7 +190,7 @@ def bench_msm(args, H):
                    "points": n, "msms_per_step": m, "window_bits": window},
         "roofline": {"bound": "valu", "kernel": "msm_accumulate",
                      "achieved": round(achieved / 1e12, 3), "peak": round(peak / 1e12, 3),
-                     "peak_spec": round(PEAK_SPEC_MAC_PER_S / 1e12, 3),
+                     "peak_measured": round(peak_measured / 1e12, 3),
                      "unit": "TMAC/s", "frac": round(achieved / peak, 4),
                      "traffic": profiled_traffic("msm_accumulate_2p20_traffic.md")
                      if (log_n, m, args.skewed) == (20, 4, False) else None},
@@ -222,7 +223,8 @@ def bench_ntt(args, H):
     # algorithmic work: (n/2) log2 n Fr products of 2*8^2 + 8 = 136 MACs; bytes: one read and one write
     bytes_alg = 64 * n * cnt
     macs = 68 * n * log_n * cnt
-    peak, _, _ = H.valu_peak()
+    peak_measured, _, _ = H.valu_peak()
+    peak = PEAK_SPEC_MAC_PER_S
     t_step = dt / args.steps
     res = {
         "metric": "Fr NTT 2^%d elements/s" % log_n, "value": round(n * cnt * args.steps / dt / 1e6, 3),
@@ -231,6 +233,7 @@ def bench_ntt(args, H):
         "dtype": "u32", "data": "synthetic uniform Fr",
         "config": {"workload": "resident Fr NTT 2^%d, %d polys per step" % (log_n, cnt)},
         "roofline": {"bound": "valu", "achieved": round(macs / t_step / 1e12, 3), "peak": round(peak / 1e12, 3),
+                     "peak_measured": round(peak_measured / 1e12, 3),
                      "unit": "TMAC/s", "frac": round(macs / t_step / peak, 4), "traffic": None},
         "hbm": {"achieved": round(bytes_alg / t_step / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(bytes_alg / t_step / HBM_PEAK, 4)},
@@ -450,7 +453,8 @@ def bench_prove(args, H):
     acc_ms, _ = H.profile_read("msm_accumulate")
     num_ms, num_cnt = H.profile_read("numerator")
     H.profile_enable(False)
-    peak, _, _ = H.valu_peak()
+    peak_measured, _, _ = H.valu_peak()
+    peak = PEAK_SPEC_MAC_PER_S
     n = 1 << k
     num_macs = QUOTIENT_FR_MULS_PER_ROW * MACS_PER_FR_MUL * 4 * n
     t_num = num_ms / 1e3 / max(num_cnt, 1)
@@ -467,8 +471,8 @@ def bench_prove(args, H):
                                "msm_accumulate": round(acc_ms / args.steps, 3),
                                "numerator": round(num_ms / args.steps, 3)},
         "roofline": {"bound": "valu", "kernel": "numerator", "achieved": round(num_macs / t_num / 1e12, 3),
-                     "peak": round(peak / 1e12, 3), "unit": "TMAC/s", "frac": round(num_macs / t_num / peak, 4),
-                     "traffic": None},
+                     "peak": round(peak / 1e12, 3), "peak_measured": round(peak_measured / 1e12, 3),
+                     "unit": "TMAC/s", "frac": round(num_macs / t_num / peak, 4), "traffic": None},
     }
     if args.cpu_seconds > 0:
         res["cpu_baseline"] = cpu_baseline_prove(args.cpu_seconds)
@@ -648,7 +652,8 @@ def bench_quotient(args, H):
     k_ms, k_cnt = H.profile_read("numerator")
     H.profile_enable(False)
     t_kernel = k_ms / 1e3 / max(k_cnt, 1)
-    peak, _, _ = H.valu_peak()
+    peak_measured, _, _ = H.valu_peak()
+    peak = PEAK_SPEC_MAC_PER_S
     macs = QUOTIENT_FR_MULS_PER_ROW * MACS_PER_FR_MUL * n
     bytes_alg = QUOTIENT_BYTES_PER_ROW * n
     res = {
@@ -661,6 +666,7 @@ def bench_quotient(args, H):
                    "rows": n},
         "roofline": {"bound": "valu", "kernel": "numerator",
                      "achieved": round(macs / t_kernel / 1e12, 3), "peak": round(peak / 1e12, 3),
+                     "peak_measured": round(peak_measured / 1e12, 3),
                      "unit": "TMAC/s", "frac": round(macs / t_kernel / peak, 4), "traffic": None},
         "hbm": {"achieved": round(bytes_alg / t_kernel / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(bytes_alg / t_kernel / HBM_PEAK, 4)},
