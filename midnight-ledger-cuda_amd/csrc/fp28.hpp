@@ -507,4 +507,106 @@ __device__ __forceinline__ Xyzz xyzz28_to_std(const Xyzz28 &p)
     return r;
 }
 
+// ---- loose XYZZ points in memory and the general loose addition, for the
+// bucket reduction that follows the accumulation: partial sums stay in
+// radix 2^28 (loose, [0, 2p)) until the last group pass converts them.
+
+constexpr uint32_t kXyzz28Words = 56;   // 4 x 14 limbs, 224 B (14 x 16 B)
+
+__device__ __forceinline__ Xyzz28 load_xyzz28(const uint32_t *p)
+{
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    uint32_t t[56];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        const uint4 u = q[i];
+        t[4 * i] = u.x;
+        t[4 * i + 1] = u.y;
+        t[4 * i + 2] = u.z;
+        t[4 * i + 3] = u.w;
+    }
+    Xyzz28 a;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        a.x.v[i] = t[i];
+        a.y.v[i] = t[14 + i];
+        a.zz.v[i] = t[28 + i];
+        a.zzz.v[i] = t[42 + i];
+    }
+    return a;
+}
+
+__device__ __forceinline__ void store_xyzz28(uint32_t *p, const Xyzz28 &a)
+{
+    uint32_t t[56];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        t[i] = a.x.v[i];
+        t[14 + i] = a.y.v[i];
+        t[28 + i] = a.zz.v[i];
+        t[42 + i] = a.zzz.v[i];
+    }
+    uint4 *q = reinterpret_cast<uint4 *>(p);
+#pragma unroll
+    for (int i = 0; i < 14; ++i) q[i] = make_uint4(t[4 * i], t[4 * i + 1], t[4 * i + 2], t[4 * i + 3]);
+}
+
+// 32-bit form -> loose radix 2^28 (one product per coordinate); the
+// identity keeps zz = 0 exactly
+__device__ __forceinline__ Xyzz28 xyzz28_from_std_lz(const Xyzz &p)
+{
+    if (p.is_inf()) return xyzz28_inf();
+    Xyzz28 r;
+    r.x = fp28_mul<false>(repack_to28(p.x), fp28_const(0));
+    r.y = fp28_mul<false>(repack_to28(p.y), fp28_const(0));
+    r.zz = fp28_mul<false>(repack_to28(p.zz), fp28_const(0));
+    r.zzz = fp28_mul<false>(repack_to28(p.zzz), fp28_const(0));
+    return r;
+}
+
+// dbl-2008-s-1 on loose coordinates
+__device__ __forceinline__ Xyzz28 xyzz28_dbl_lz(const Xyzz28 &p)
+{
+    if (p.is_inf() || fp28_is_zero_lz(p.y)) return xyzz28_inf();
+    Fp28 u = fp28_add_lz(p.y, p.y);
+    Fp28 v = fp28_sqr<false>(u);
+    Fp28 w = fp28_mul<false>(u, v);
+    Fp28 s = fp28_mul<false>(p.x, v);
+    Fp28 x2 = fp28_sqr<false>(p.x);
+    Fp28 m = fp28_add_lz(x2, fp28_add_lz(x2, x2));
+    Xyzz28 r;
+    r.x = fp28_sub_lz(fp28_sqr<false>(m), fp28_add_lz(s, s));
+    r.y = fp28_sub_lz(fp28_mul<false>(m, fp28_sub_lz(s, r.x)), fp28_mul<false>(w, p.y));
+    r.zz = fp28_mul<false>(v, p.zz);
+    r.zzz = fp28_mul<false>(w, p.zzz);
+    return r;
+}
+
+// add-2008-s on loose coordinates (the identity only as zz = 0 exactly: zz
+// of a sum is zz1 zz2 PP and of a double v zz, both nonzero mod p)
+__device__ __forceinline__ Xyzz28 xyzz28_add_lz(const Xyzz28 &p, const Xyzz28 &q)
+{
+    if (q.is_inf()) return p;
+    if (p.is_inf()) return q;
+    Fp28 u1 = fp28_mul<false>(p.x, q.zz);
+    Fp28 u2 = fp28_mul<false>(q.x, p.zz);
+    Fp28 s1 = fp28_mul<false>(p.y, q.zzz);
+    Fp28 s2 = fp28_mul<false>(q.y, p.zzz);
+    Fp28 pp_ = fp28_sub_lz(u2, u1);
+    Fp28 r_ = fp28_sub_lz(s2, s1);
+    if (fp28_is_zero_lz(pp_)) {
+        if (fp28_is_zero_lz(r_)) return xyzz28_dbl_lz(p);
+        return xyzz28_inf();
+    }
+    Fp28 pp = fp28_sqr<false>(pp_);
+    Fp28 ppp = fp28_mul<false>(pp_, pp);
+    Fp28 qq = fp28_mul<false>(u1, pp);
+    Xyzz28 r;
+    r.x = fp28_sub_lz(fp28_sub_lz(fp28_sqr<false>(r_), ppp), fp28_add_lz(qq, qq));
+    r.y = fp28_sub_lz(fp28_mul<false>(r_, fp28_sub_lz(qq, r.x)), fp28_mul<false>(s1, ppp));
+    r.zz = fp28_mul<false>(fp28_mul<false>(p.zz, q.zz), pp);
+    r.zzz = fp28_mul<false>(fp28_mul<false>(p.zzz, q.zzz), ppp);
+    return r;
+}
+
 }  // namespace mh

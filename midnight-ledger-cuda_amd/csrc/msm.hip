@@ -336,8 +336,9 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(const uint32_t *pts, cons
         if (x + 1 < end) nxt = fetch_point(pts, ent[x + 1]);
         acc = xyzz_add_affine(acc, cur);
     }
-    if (q[b] == 1) store_xyzz(bucket + (uint64_t)b * 48, acc);
-    else store_xyzz(part + (uint64_t)j * 48, acc);
+    const Xyzz28 out = xyzz28_from_std_lz(acc);
+    if (q[b] == 1) store_xyzz28(bucket + (uint64_t)b * kXyzz28Words, out);
+    else store_xyzz28(part + (uint64_t)j * kXyzz28Words, out);
 }
 
 // Same pass over a precomputed table held in radix 2^28 (28 u32 per point,
@@ -372,10 +373,9 @@ __global__ void __launch_bounds__(kBlock) k_accumulate28(const uint32_t *pts, co
         if (x + 1 < end) nxt = fetch_point28(pts, ent[x + 1]);
         acc = xyzz28_add_affine_lz(acc, cur);
     }
-    // the conversion product fully reduces the loose coordinates
-    const Xyzz out = xyzz28_to_std(acc);
-    if (q[b] == 1) store_xyzz(bucket + (uint64_t)b * 48, out);
-    else store_xyzz(part + (uint64_t)j * 48, out);
+    // the reduction kernels take the loose radix-2^28 sums as they are
+    if (q[b] == 1) store_xyzz28(bucket + (uint64_t)b * kXyzz28Words, acc);
+    else store_xyzz28(part + (uint64_t)j * kXyzz28Words, acc);
 }
 
 // Bucket reduction.  After the accumulation, bucket b holds q[b] partials
@@ -423,9 +423,9 @@ __global__ void __launch_bounds__(kBlock) k_tree_level(const uint32_t *q, const 
     const uint32_t qprev = level_len(q[b], div);
     const uint32_t s0 = r * G, s1 = min(s0 + G, qprev);
     const uint64_t base = co[b];
-    Xyzz acc = Xyzz::inf();
-    for (uint32_t x = s0; x < s1; ++x) acc = xyzz_add(acc, load_xyzz(in + (base + x) * 48));
-    store_xyzz(out + (base + r) * 48, acc);
+    Xyzz28 acc = xyzz28_inf();
+    for (uint32_t x = s0; x < s1; ++x) acc = xyzz28_add_lz(acc, load_xyzz28(in + (base + x) * kXyzz28Words));
+    store_xyzz28(out + (base + r) * kXyzz28Words, acc);
 }
 
 // Bucket b after `levels` levels: the levels that shrank it are the first
@@ -446,9 +446,9 @@ __global__ void __launch_bounds__(kBlock) k_tree_final(const uint32_t *q, const 
     }
     const uint32_t *src = (lv & 1) ? part2 : part;
     const uint64_t base = co[b];
-    Xyzz acc = Xyzz::inf();
-    for (uint32_t x = 0; x < rem; ++x) acc = xyzz_add(acc, load_xyzz(src + (base + x) * 48));
-    store_xyzz(bucket + b * 48, acc);
+    Xyzz28 acc = xyzz28_inf();
+    for (uint32_t x = 0; x < rem; ++x) acc = xyzz28_add_lz(acc, load_xyzz28(src + (base + x) * kXyzz28Words));
+    store_xyzz28(bucket + b * kXyzz28Words, acc);
 }
 
 // zeroes the bucket counters and the totals
@@ -501,13 +501,14 @@ __global__ void __launch_bounds__(kBlock) k_plane_partials(const uint32_t *bucke
     uint32_t e0 = chunk * G1, e1 = min(e0 + G1, pl.members[k]);
     const uint64_t stride = (uint64_t)g.M * g.kw;   // keys are bucket-major
     const uint32_t lowmask = (1u << k) - 1u;
-    Xyzz acc = Xyzz::inf();
+    Xyzz28 acc = xyzz28_inf();
     for (uint32_t e = e0; e < e1; ++e) {
         uint32_t x = ((e >> k) << (k + 1)) | (1u << k) | (e & lowmask);
         const uint64_t key = (uint64_t)(x - 1) * stride + mw;
-        if (cnt[key]) acc = xyzz_add(acc, load_xyzz(bucket + key * 48));   // empty buckets are not written
+        // empty buckets are not written
+        if (cnt[key]) acc = xyzz28_add_lz(acc, load_xyzz28(bucket + key * kXyzz28Words));
     }
-    store_xyzz(pp + id * 48, acc);
+    store_xyzz28(pp + id * kXyzz28Words, acc);
 }
 
 // Sum groups of G consecutive partials within segments of `seg` partials;
@@ -527,10 +528,11 @@ __global__ void __launch_bounds__(kBlock) k_group_sums(const uint32_t *in, uint6
     }
     uint32_t local = (uint32_t)(id - out_off[lo]);
     uint32_t s0 = seg_off[lo] + local * G, s1 = min(s0 + G, seg_off[lo] + seg_len[lo]);
-    Xyzz acc = Xyzz::inf();
-    for (uint32_t x = s0; x < s1; ++x) acc = xyzz_add(acc, load_xyzz(in + (uint64_t)x * 48));
-    if (to_jac) store_jacobian(out + id * 36, xyzz_to_jacobian(acc));
-    else store_xyzz(out + id * 48, acc);
+    Xyzz28 acc = xyzz28_inf();
+    for (uint32_t x = s0; x < s1; ++x) acc = xyzz28_add_lz(acc, load_xyzz28(in + (uint64_t)x * kXyzz28Words));
+    // the last pass converts (fully reduced) to the 32-bit Jacobian form
+    if (to_jac) store_jacobian(out + id * 36, xyzz_to_jacobian(xyzz28_to_std(acc)));
+    else store_xyzz28(out + id * kXyzz28Words, acc);
 }
 
 // row w * n + i of `out` = [2^(c w)] P_i, by c doublings per window, stored
@@ -757,14 +759,14 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
     b.totals.ensure(4);
     b.ent.ensure(std::max<uint64_t>(s.nent_max, 1));
     b.cmap.ensure(s.nch_max);
-    b.part.ensure(s.nch_max * 48);
+    b.part.ensure(s.nch_max * kXyzz28Words);
     if (s.tree_levels > 0) {
-        b.part2.ensure(s.nch_max * 48);
+        b.part2.ensure(s.nch_max * kXyzz28Words);
         b.lc.ensure(s.nkeys);
         b.lo.ensure(s.nkeys);
         b.lmap.ensure(s.nch_max / s.G + s.nkeys);
     }
-    b.bucket.ensure(s.nkeys * 48);
+    b.bucket.ensure(s.nkeys * kXyzz28Words);
 
     // bit planes: S = sum_k 2^k Q_k, Q_k = sum of the buckets x with bit k of x
     const uint64_t nmw = (uint64_t)g.M * g.kw;
@@ -782,7 +784,7 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
     for (uint32_t k = g.c + 1; k <= kMaxWindow; ++k) pl.first[k] = 0xffffffffu;
     pl.per_window = acc;
     s.npp = nmw * pl.per_window;
-    b.pp.ensure(s.npp * 48);
+    b.pp.ensure(s.npp * kXyzz28Words);
 
     // group passes within each segment (msm, window, plane) until one
     // partial per segment is left; the last pass converts to Jacobian
@@ -818,7 +820,7 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
     b.seg_off.ensure(s.seg_off.size());
     b.seg_len.ensure(s.seg_len.size());
     b.out_off.ensure(s.out_off.size());
-    b.pp2.ensure(std::max<uint64_t>(s.levels[0].total_out, 1) * 48);
+    b.pp2.ensure(std::max<uint64_t>(s.levels[0].total_out, 1) * kXyzz28Words);
     b.pj.ensure(s.nseg * 36);
 
     // uploads of the descriptors and the plane-pass layout
