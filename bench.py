@@ -720,7 +720,8 @@ def bench_poseidon(args, H):
     does not depend on them).  The chain is sequential, so the bound is the
     latency of one wavefront's dependent Fr products: `roofline` compares the
     critical-path products per second (3 per round, 136 rounds per hash)
-    with the single-wave dependent-product rate measured on the same device."""
+    with the single-wave dependent-product rate of the radix-2^29 product
+    the kernel uses, measured on the same device."""
     length = args.chain_len
     rng = np.random.default_rng(args.seed)
     # values in [0, r) are the Montgomery forms of uniform field elements
@@ -739,8 +740,7 @@ def bench_poseidon(args, H):
     k_ms, k_cnt = H.profile_read("poseidon_chain")
     H.profile_enable(False)
     t_kernel = k_ms / 1e3 / max(k_cnt, 1)
-    _, fr_rate = H.field_mul_rate(1)
-    lat = 64.0 / fr_rate                        # seconds per dependent product on one wave
+    lat = H.fr29_product_latency()              # seconds per dependent (radix-2^29) product on one wave
     products = POSEIDON_STEPS_PER_ROUND * POSEIDON_ROUNDS * 2 * length
     res = {
         "metric": "Poseidon hash-chain hashes/s (chain of %d, with witness)" % length,

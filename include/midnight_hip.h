@@ -600,6 +600,9 @@ RustError midnight_hip_copy_bandwidth(size_t bytes, double *bytes_per_s);
 /* Whole Fp / Fr Montgomery products per second (blocks = 0: whole device;
  * blocks = 1: one wave, i.e. the latency of a dependent chain). */
 RustError midnight_hip_field_mul_rate(int blocks, double *fp_per_s, double *fr_per_s);
+/* Latency in seconds of one dependent radix-2^29 Fr product on a single
+ * wave (the product the Poseidon chain kernel issues). */
+RustError midnight_hip_fr29_product_latency(double *seconds);
 /* Per-kernel timing with HIP events on the launching stream (off by
  * default).  enable(1) clears the registry; read() returns 1 and the total
  * milliseconds and count for a region name ("msm_accumulate", "msm_total",

@@ -7,6 +7,7 @@
 
 #include "fields.hpp"
 #include "fp28.hpp"
+#include "fr29.hpp"
 #include "runtime.hpp"
 
 namespace mh {
@@ -102,6 +103,20 @@ __global__ void __launch_bounds__(256) madd_loop(uint32_t *out, uint32_t iters, 
     out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// two interleaved dependent chains of radix-2^29 Fr products on one wave
+__global__ void __launch_bounds__(64) fr29_chain_loop(uint32_t *out, uint32_t iters)
+{
+    Fr one = Fr::one();
+    one.v[0] ^= threadIdx.x;   // still below r
+    Fr29 a = fr29_from32(one), b = a;
+    b.v[1] ^= 7u;
+    for (uint32_t i = 0; i < iters; ++i) {
+        a = fr29_mul(a, b);
+        b = fr29_mul(b, a);
+    }
+    out[threadIdx.x] = a.v[0] ^ b.v[0];
+}
+
 double time_launch(void (*fn)(hipStream_t), hipStream_t st)
 {
     hipEvent_t e0, e1;
@@ -137,6 +152,10 @@ void run_fr(hipStream_t st)
 {
     hipLaunchKernelGGL(field_mul_loop<FrCfg>, dim3(g_blocks), dim3(g_blocks == 1 ? 64 : 256), 0, st,
                        reinterpret_cast<uint32_t *>(g_sink), kMulIters);
+}
+void run_fr29(hipStream_t st)
+{
+    hipLaunchKernelGGL(fr29_chain_loop, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t *>(g_sink), kMulIters);
 }
 
 }  // namespace
@@ -219,7 +238,8 @@ extern "C" RustError midnight_hip_field_mul_rate(int blocks, double *fp_per_s, d
         hipDeviceProp_t prop;
         MH_HIP(hipGetDeviceProperties(&prop, device_id()));
         if (!g_sink) MH_HIP(hipMalloc(reinterpret_cast<void **>(&g_sink), (size_t)prop.multiProcessorCount * 8 * 256 * 8));
-        g_blocks = blocks > 0 ? blocks : prop.multiProcessorCount * 8;
+        // g_sink holds one u64 per thread of multiProcessorCount * 8 blocks
+        g_blocks = blocks > 0 ? std::min(blocks, prop.multiProcessorCount * 8) : prop.multiProcessorCount * 8;
         if (!g_sink) MH_HIP(hipMalloc(reinterpret_cast<void **>(&g_sink), (size_t)g_blocks * 256 * 8));
         hipStream_t st = cur_stream();
         double per = (double)g_blocks * (g_blocks == 1 ? 64 : 256) * kMulIters * 2;
@@ -271,6 +291,24 @@ extern "C" RustError midnight_hip_field_rates28(double out[4])
         out[2] = per_madd / time_launch(run_madd, st);
         g_variant = 1;
         out[3] = per_madd / time_launch(run_madd, st);
+        return ok();
+    } catch (const hip_error &e) {
+        return make_error(e.code, e.what());
+    }
+}
+
+// Latency of one dependent radix-2^29 Fr product on a single wave (the
+// product the Poseidon chain issues), in seconds.
+extern "C" RustError midnight_hip_fr29_product_latency(double *seconds)
+{
+    using namespace mh;
+    try {
+        hipDeviceProp_t prop;
+        MH_HIP(hipGetDeviceProperties(&prop, device_id()));
+        if (!g_sink)
+            MH_HIP(hipMalloc(reinterpret_cast<void **>(&g_sink), (size_t)prop.multiProcessorCount * 8 * 256 * 8));
+        const double t = time_launch(run_fr29, cur_stream());
+        if (seconds) *seconds = t / (2.0 * kMulIters);
         return ok();
     } catch (const hip_error &e) {
         return make_error(e.code, e.what());

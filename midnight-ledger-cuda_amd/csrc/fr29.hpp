@@ -64,6 +64,37 @@ __device__ __forceinline__ Fr fr29_to32(const Fr29 &x)
     return r;
 }
 
+// r - modulus if r >= modulus (r < 2 modulus, limbs below 2^29)
+__device__ __forceinline__ void fr29_reduce(Fr29 &r)
+{
+    uint32_t d[9];
+    int32_t bw = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        int32_t x = (int32_t)r.v[j] - (int32_t)r29(j) - bw;
+        bw = (int32_t)((uint32_t)x >> 31);
+        d[j] = (uint32_t)x & kM29;
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) r.v[j] = bw ? r.v[j] : d[j];
+}
+
+// a + b mod r for fully reduced a, b (any Montgomery radix: the sum of two
+// representations is the representation of the sum)
+__device__ __forceinline__ Fr29 fr29_add(const Fr29 &a, const Fr29 &b)
+{
+    Fr29 r;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const uint32_t s = a.v[j] + b.v[j] + c;
+        c = s >> 29;
+        r.v[j] = s & kM29;
+    }
+    fr29_reduce(r);
+    return r;
+}
+
 // a * b / 2^261 mod r, fully reduced; a < 2^261 (any 9-limb value), b < r
 // (then the running value stays below a b / 2^261 + r < 2r).
 __device__ __forceinline__ Fr29 fr29_mul(const Fr29 &a, const Fr29 &b)
@@ -94,17 +125,7 @@ __device__ __forceinline__ Fr29 fr29_mul(const Fr29 &a, const Fr29 &b)
         r.v[j] = (uint32_t)c & kM29;
         c >>= 29;
     }
-    // r < 2 r_mod: subtract the modulus once if r >= modulus
-    uint32_t d[9];
-    int32_t bw = 0;
-#pragma unroll
-    for (int j = 0; j < 9; ++j) {
-        int32_t x = (int32_t)r.v[j] - (int32_t)r29(j) - bw;
-        bw = (int32_t)((uint32_t)x >> 31);
-        d[j] = (uint32_t)x & kM29;
-    }
-#pragma unroll
-    for (int j = 0; j < 9; ++j) r.v[j] = bw ? r.v[j] : d[j];
+    fr29_reduce(r);   // r < 2 modulus
     return r;
 }
 

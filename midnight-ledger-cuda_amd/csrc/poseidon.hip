@@ -18,11 +18,23 @@
 // products: state'[c] = rc[r + 1][c] + sum_j prod[c][j].  Partial rounds
 // take the same steps with only column 2 nonlinear.  The 204 round
 // constants sit in LDS.
+//
+// Products are the radix-2^29 ones of fr29.hpp (~0.6x the instructions of
+// 32-bit CIOS; issue count is what bounds a single wave).  Their Montgomery
+// radix is 2^261, so the chain keeps every value as v 2^261 mod r: the
+// parameters and inputs are scaled by 2^5 on the host, products of two such
+// values stay in that form, sums do not depend on the radix, and one
+// data-parallel pass after the chain brings outputs and witness back to
+// the usual 2^256 form (a product by 2^256 mod r).
+#include <algorithm>
 #include <cstring>
 #include <memory>
+#include <vector>
 
 #include "devvec.hpp"
 #include "fields.hpp"
+#include "fr29.hpp"
+#include "host_fr.hpp"
 #include "profile.hpp"
 #include "runtime.hpp"
 
@@ -42,12 +54,13 @@ constexpr size_t kLayoutLen = 128;
 __device__ __forceinline__ bool is_full_round(int r) { return r < 4 || r >= 64; }
 
 // One permutation.  `x` is this lane's copy of state[c]; `w` (nullable)
-// receives the permutation's 128 witness values: the cubes from the
-// squaring lanes, the post-MDS states from the product lanes of row 0.
-__device__ __forceinline__ void permute(Fr &x, const Fr &m, int l, int c, int k, const Fr *s_rc, Fr *s_sq,
-                                        Fr (*s_prod)[9], Fr *w)
+// receives the permutation's 128 witness values (in the 2^261 form): the
+// cubes from the squaring lanes, the post-MDS states from the product lanes
+// of row 0.
+__device__ __forceinline__ void permute(Fr29 &x, const Fr29 &m, int l, int c, int k, const Fr29 *s_rc,
+                                        Fr29 *s_sq, Fr29 (*s_prod)[9], Fr *w)
 {
-    x = x + s_rc[c];
+    x = fr29_add(x, s_rc[c]);
     size_t off = 0;
     const bool product_lane = k < 9;
     const bool writer = w != nullptr && product_lane && l == 0;
@@ -55,32 +68,32 @@ __device__ __forceinline__ void permute(Fr &x, const Fr &m, int l, int c, int k,
         const bool full = is_full_round(r);
         const bool nonlinear = full || c == 2;
         // step 1: m x on the product lanes, x^2 on the squaring lanes
-        Fr y = (product_lane ? m : x) * x;
-        if (!product_lane) store_fr(s_sq + c, y);
+        Fr29 y = fr29_mul(product_lane ? m : x, x);
+        if (!product_lane) s_sq[c] = y;
         __syncthreads();
         if (nonlinear) {
             // step 2: m x^3 | x^3;  step 3: m x^5
-            const Fr sq = product_lane ? s_sq[c] : y;
-            y = y * (product_lane ? sq : x);
-            if (!product_lane && full && w != nullptr) store_fr(w + off + c, y);
-            if (product_lane) y = y * sq;
+            const Fr29 sq = product_lane ? s_sq[c] : y;
+            y = fr29_mul(y, product_lane ? sq : x);
+            if (!product_lane && full && w != nullptr) store_fr(w + off + c, fr29_to32(y));
+            if (product_lane) y = fr29_mul(y, sq);
         }
-        Fr *prod = s_prod[r & 1];
-        if (product_lane) store_fr(prod + k, y);
+        Fr29 *prod = s_prod[r & 1];
+        if (product_lane) prod[k] = y;
         __syncthreads();
-        Fr nx = s_rc[(r + 1) * kWidth + c];
-        nx = nx + prod[c * kWidth + 0];
-        nx = nx + prod[c * kWidth + 1];
-        nx = nx + prod[c * kWidth + 2];
+        Fr29 nx = s_rc[(r + 1) * kWidth + c];
+        nx = fr29_add(nx, prod[c * kWidth + 0]);
+        nx = fr29_add(nx, prod[c * kWidth + 1]);
+        nx = fr29_add(nx, prod[c * kWidth + 2]);
         if (writer) {
             if (full) {
-                store_fr(w + off + 3 + c, nx);
+                store_fr(w + off + 3 + c, fr29_to32(nx));
             } else {
                 int in_batch = (r - 4) % 6;
                 if (in_batch < 5) {
-                    if (c == 2) store_fr(w + off + in_batch, nx);
+                    if (c == 2) store_fr(w + off + in_batch, fr29_to32(nx));
                 } else {
-                    store_fr(w + off + 5 + c, nx);
+                    store_fr(w + off + 5 + c, fr29_to32(nx));
                 }
             }
         }
@@ -90,32 +103,40 @@ __device__ __forceinline__ void permute(Fr &x, const Fr &m, int l, int c, int k,
 }
 
 // One block of 64 threads (a single wavefront); lanes 12..63 only help load
-// the round constants.
+// the round constants.  Parameters and inputs arrive in the 2^261 form.
 __global__ __launch_bounds__(64) void k_poseidon_chain(Fr *out, Fr *witness, Fr initial, Fr tail0, Fr tail1,
                                                        Fr input_len, const Fr *mds, const Fr *rc, size_t len)
 {
     // round constants with three zeros after the last round (its MDS output
     // gets no constant)
-    __shared__ Fr s_rc[(kRounds + 1) * kWidth];
-    __shared__ Fr s_prod[2][9];
-    __shared__ Fr s_sq[kWidth];
+    __shared__ Fr29 s_rc[(kRounds + 1) * kWidth];
+    __shared__ Fr29 s_prod[2][9];
+    __shared__ Fr29 s_sq[kWidth];
     const int k = threadIdx.x;
     for (int i = k; i < (kRounds + 1) * kWidth; i += blockDim.x)
-        s_rc[i] = i < kRounds * kWidth ? load_fr(rc + i) : Fr::zero();
+        s_rc[i] = fr29_from32(i < kRounds * kWidth ? load_fr(rc + i) : Fr::zero());
     __syncthreads();
     if (k >= 12) return;
     const int l = k < 9 ? k / kWidth : 0, c = k < 9 ? k % kWidth : k - 9;
-    const Fr m = k < 9 ? load_fr(mds + l * kWidth + c) : Fr::zero();
-    Fr previous = initial;
+    const Fr29 m = fr29_from32(k < 9 ? load_fr(mds + l * kWidth + c) : Fr::zero());
+    const Fr29 t0 = fr29_from32(tail0), t1 = fr29_from32(tail1), il = fr29_from32(input_len);
+    Fr29 previous = fr29_from32(initial);
     for (size_t index = 0; index < len; ++index) {
-        Fr x = c == 2 ? input_len : Fr::zero() + (c == 0 ? previous : tail0);
+        Fr29 x = c == 2 ? il : (c == 0 ? previous : t0);
         Fr *w = witness ? witness + index * kHashWitness : nullptr;
         permute(x, m, l, c, k, s_rc, s_sq, s_prod, w);
-        if (c == 0) x = x + tail1;
+        if (c == 0) x = fr29_add(x, t1);
         permute(x, m, l, c, k, s_rc, s_sq, s_prod, w ? w + kPermWitness : nullptr);
         previous = x;  // meaningful on the c == 0 lanes
-        if (k == 0 && out) store_fr(out + index, previous);
+        if (k == 0 && out) store_fr(out + index, fr29_to32(previous));
     }
+}
+
+// v 2^261 -> v 2^256 in place: the radix-2^29 product by 2^256 mod r
+__global__ void k_from261(Fr *v, size_t n, Fr r256)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        store_fr(v + i, fr_mul_scaled(load_fr(v + i), r256));
 }
 
 // Witness scatter: value idx of hash h = idx / 256 goes to column
@@ -176,22 +197,39 @@ bool chain_args_ok(const RawFr *out, const RawFr *mds, size_t mds_len, const Raw
 
 // Runs the chain; `with_witness` selects the witness stream (len * 256
 // values) instead of the outputs (len values) as what is copied to `out`.
+// v (usual 2^256 form) -> v 2^261 mod r, which is the usual form of 32 v
+Fr scaled261(const RawFr &v)
+{
+    const HostFr h = to_hfr(v) * HostFr::from_u64(32);
+    Fr r;
+    std::memcpy(r.v, h.l, 32);
+    return r;
+}
+
 void run_chain(RawFr *out, size_t len, const RawFr &initial, const RawFr &tail0, const RawFr &tail1,
                const RawFr &input_len, const RawFr *mds, const RawFr *rc, bool with_witness)
 {
     hipStream_t st = cur_stream();
     const size_t n_out = with_witness ? len * kHashWitness : len;
-    DevBuf<Fr> d_out(n_out), d_params(kWidth * kWidth + kRounds * kWidth);
-    MH_HIP(hipMemcpyAsync(d_params.get(), mds, kWidth * kWidth * 32, hipMemcpyHostToDevice, st));
-    MH_HIP(hipMemcpyAsync(d_params.get() + kWidth * kWidth, rc, kRounds * kWidth * 32, hipMemcpyHostToDevice, st));
+    constexpr size_t kParams = kWidth * kWidth + kRounds * kWidth;
+    std::vector<Fr> params(kParams);
+    for (size_t i = 0; i < kParams; ++i) params[i] = scaled261(i < 9 ? mds[i] : rc[i - 9]);
+    DevBuf<Fr> d_out(n_out), d_params(kParams);
+    MH_HIP(hipMemcpyAsync(d_params.get(), params.data(), kParams * 32, hipMemcpyHostToDevice, st));
     ProfileScope prof;
     prof.begin("poseidon_chain", st);
     hipLaunchKernelGGL(k_poseidon_chain, dim3(1), dim3(64), 0, st, with_witness ? nullptr : d_out.get(),
-                       with_witness ? d_out.get() : nullptr, to_fr(initial), to_fr(tail0), to_fr(tail1),
-                       to_fr(input_len), (const Fr *)d_params.get(), (const Fr *)d_params.get() + kWidth * kWidth,
-                       len);
+                       with_witness ? d_out.get() : nullptr, scaled261(initial), scaled261(tail0),
+                       scaled261(tail1), scaled261(input_len), (const Fr *)d_params.get(),
+                       (const Fr *)d_params.get() + kWidth * kWidth, len);
     MH_HIP(hipGetLastError());
     prof.end(st);
+    Fr r256;
+    const HostFr one = HostFr::one();
+    std::memcpy(r256.v, one.l, 32);
+    hipLaunchKernelGGL(k_from261, dim3((unsigned)std::min<size_t>(ceil_div(n_out, 256), 8192)), dim3(256), 0, st,
+                       d_out.get(), n_out, r256);
+    MH_HIP(hipGetLastError());
     MH_HIP(hipMemcpyAsync(out, d_out.get(), n_out * 32, hipMemcpyDeviceToHost, st));
     MH_HIP(hipStreamSynchronize(st));
     prof.flush();

@@ -122,6 +122,7 @@ def _declare(L):
         "midnight_hip_valu_peak": (
             E, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
         "midnight_hip_copy_bandwidth": (E, [c_size, ctypes.POINTER(ctypes.c_double)]),
+        "midnight_hip_fr29_product_latency": (E, [ctypes.POINTER(ctypes.c_double)]),
         "midnight_hip_batch_plan_create": (
             E, [vpp, vp, vp, ctypes.POINTER(c_size), ctypes.POINTER(u32), c_size, u32]),
         "midnight_hip_batch_plan_run": (E, [vp, u64p]),
@@ -642,6 +643,13 @@ def copy_bandwidth(nbytes):
     r = ctypes.c_double()
     check(lib().midnight_hip_copy_bandwidth(nbytes, ctypes.byref(r)))
     return r.value
+
+
+def fr29_product_latency():
+    """seconds per dependent radix-2^29 Fr product on one wave"""
+    t = ctypes.c_double()
+    check(lib().midnight_hip_fr29_product_latency(ctypes.byref(t)))
+    return t.value
 
 
 def field_mul_rate(blocks=0):
