@@ -209,6 +209,197 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_lds(const MsmDesc *descs,
     }
 }
 
+// ---------------------------------------------------------------- two-pass placement
+//
+// After the count pass and the scan (off = bucket starts) the digits are
+// placed in two passes whose HBM writes are long contiguous runs:
+//   pass 1 (k_place_coarse): a block forms the digits of up to 1024
+//     scalars, counting-sorts them in LDS by coarse bin (F consecutive keys,
+//     at most kCoarseBins bins), reserves its slice of each bin's region of
+//     `tmp` -- the same index space as `ent`: bin cb spans off[cb F] ... --
+//     with one atomic per bin, and writes its (key, entry) pairs bin by bin;
+//   pass 2 (k_place_fine): a block takes up to kFineChunk pairs of one bin,
+//     counting-sorts them by key in LDS, reserves each key's slice of its
+//     bucket with one atomic, and writes the entries key by key.
+// Placing each digit straight into its bucket (one 4-byte store per digit,
+// runs of ~8 per block and key) moved ~10x the entry bytes through HBM.
+constexpr uint32_t kPlaceThreads = 1024;
+constexpr uint32_t kCoarseBins = 1024;   // at most
+constexpr uint32_t kStage1 = 16384;      // pairs staged by a pass-1 block
+constexpr uint32_t kFineChunk = 16384;   // pairs per pass-2 block
+constexpr uint32_t kMaxFine = 4096;      // keys per coarse bin (pass-2 LDS)
+constexpr uint32_t kNoKey = 0xffffffffu;
+
+// In-place exclusive scan of a[0, n) in LDS by a block of kPlaceThreads
+// (n <= 4 * kPlaceThreads); `wsum` holds 16 wave sums.  Returns the total.
+__device__ uint32_t lds_exclusive_scan(uint32_t *a, uint32_t n, uint32_t *wsum)
+{
+    const uint32_t t = threadIdx.x, per = (n + kPlaceThreads - 1) / kPlaceThreads;
+    uint32_t v[4], s = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        const uint32_t idx = t * per + i;
+        v[i] = (i < per && idx < n) ? a[idx] : 0u;
+        s += v[i];
+    }
+    uint32_t x = s;   // inclusive scan of s within the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if ((int)(t & 63) >= o) x += y;
+    }
+    const uint32_t wave = t >> 6;
+    if ((t & 63) == 63) wsum[wave] = x;
+    __syncthreads();
+    if (t < 16) {
+        uint32_t w = wsum[t];
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const uint32_t y = __shfl_up(w, o, 64);
+            if ((int)t >= o) w += y;
+        }
+        wsum[t] = w;
+    }
+    __syncthreads();
+    uint32_t run = (wave ? wsum[wave - 1] : 0u) + x - s;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        const uint32_t idx = t * per + i;
+        if (i < per && idx < n) a[idx] = run;
+        run += v[i];
+    }
+    const uint32_t total = wsum[kPlaceThreads / 64 - 1];
+    __syncthreads();
+    return total;
+}
+
+template <uint32_t C>
+__global__ void __launch_bounds__(kPlaceThreads) k_place_coarse(const MsmDesc *descs, Geometry g, uint32_t tile,
+                                                               uint32_t fshift, uint32_t nbins, uint32_t *ccursor,
+                                                               uint64_t *tmp)
+{
+    constexpr uint32_t NW = (256 + C - 1) / C;
+    __shared__ uint32_t hist[kCoarseBins];
+    __shared__ uint32_t delta[kCoarseBins];
+    __shared__ uint64_t stage[kStage1];
+    __shared__ uint32_t wsum[16];
+    const uint32_t m = blockIdx.y;
+    const MsmDesc d = descs[m];
+    const uint32_t i0 = blockIdx.x * tile;
+    if (i0 >= d.n) return;   // uniform for the whole block
+    const uint32_t t = threadIdx.x;
+    for (uint32_t j = t; j < nbins; j += kPlaceThreads) hist[j] = 0;
+    __syncthreads();
+    const uint32_t i = i0 + t;
+    const bool live = t < tile && i < d.n;
+    const uint64_t mkw = (uint64_t)m * g.kw;
+    uint32_t key[NW], val[NW];
+    uint32_t k[9];
+    if (live) canonical_scalar(d.scalars + i, k);
+    uint32_t carry = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+        const int32_t dg = live ? window_digit(k, w, C, carry) : 0;
+        key[w] = kNoKey;
+        if (dg == 0) continue;
+        const uint32_t b = (uint32_t)(dg < 0 ? -dg : dg) - 1;
+        key[w] = (uint32_t)key_of(g, mkw, w, b);
+        val[w] = (d.pt_off + i + w * g.pstride) | (dg < 0 ? kSignBit : 0u);
+        atomicAdd(&hist[key[w] >> fshift], 1u);
+    }
+    __syncthreads();
+    const uint32_t total = lds_exclusive_scan(hist, nbins, wsum);
+    for (uint32_t j = t; j < nbins; j += kPlaceThreads) {
+        const uint32_t st = hist[j], n = (j + 1 < nbins ? hist[j + 1] : total) - st;
+        delta[j] = n ? atomicAdd(&ccursor[j], n) - st : 0u;   // mod 2^32: delta + p is the slot
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+        if (key[w] == kNoKey) continue;
+        const uint32_t r = atomicAdd(&hist[key[w] >> fshift], 1u);
+        stage[r] = ((uint64_t)key[w] << 32) | val[w];
+    }
+    __syncthreads();
+    for (uint32_t p = t; p < total; p += kPlaceThreads) {
+        const uint64_t v = stage[p];
+        tmp[delta[(uint32_t)(v >> 32) >> fshift] + p] = v;
+    }
+}
+
+// per coarse bin: its pass-2 chunk count, and the bin's cursor = its first
+// bucket's start
+__global__ void k_bin_setup(const uint32_t *off, const uint32_t *cnt, uint64_t nkeys, uint32_t fshift, uint32_t nbins,
+                            uint32_t *ccursor, uint32_t *nch2)
+{
+    const uint32_t cb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cb >= nbins) return;
+    const uint64_t k0 = (uint64_t)cb << fshift, kl = min<uint64_t>(k0 + (1ull << fshift), nkeys) - 1;
+    const uint32_t bs = off[k0], be = off[kl] + cnt[kl];
+    ccursor[cb] = bs;
+    nch2[cb] = (be - bs + kFineChunk - 1) / kFineChunk;
+}
+
+// pass-2 block -> coarse bin map; t2[0] = pass-2 blocks
+__global__ void k_fine_map(const uint32_t *nch2, const uint32_t *cst2, uint32_t nbins, uint32_t *cmap2, uint32_t *t2)
+{
+    const uint32_t cb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cb >= nbins) return;
+    const uint32_t n = nch2[cb], o = cst2[cb];
+    for (uint32_t x = 0; x < n; ++x) cmap2[o + x] = cb;
+    if (cb == nbins - 1) t2[0] = o + n;
+}
+
+__global__ void __launch_bounds__(kPlaceThreads) k_place_fine(const uint64_t *tmp, const uint32_t *off,
+                                                             const uint32_t *cnt, const uint32_t *cmap2,
+                                                             const uint32_t *cst2, const uint32_t *t2,
+                                                             uint64_t nkeys, uint32_t fshift, uint32_t *cursor,
+                                                             uint32_t *ent)
+{
+    constexpr uint32_t PER = kFineChunk / kPlaceThreads;
+    __shared__ uint32_t fh[kMaxFine];
+    __shared__ uint32_t fdelta[kMaxFine];
+    __shared__ uint32_t sval[kFineChunk];
+    __shared__ uint16_t skey[kFineChunk];
+    __shared__ uint32_t wsum[16];
+    const uint32_t j = blockIdx.x;
+    if (j >= t2[0]) return;   // uniform for the whole block
+    const uint32_t t = threadIdx.x;
+    const uint32_t cb = cmap2[j];
+    const uint32_t r = j - cst2[cb];
+    const uint64_t k0 = (uint64_t)cb << fshift;
+    const uint32_t nk = (uint32_t)min<uint64_t>(1ull << fshift, nkeys - k0);
+    const uint32_t bs = off[k0], be = off[k0 + nk - 1] + cnt[k0 + nk - 1];
+    const uint32_t s0 = bs + r * kFineChunk, n = min(be - s0, kFineChunk);
+    for (uint32_t x = t; x < nk; x += kPlaceThreads) fh[x] = 0;
+    __syncthreads();
+    uint64_t v[PER];
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q) {
+        const uint32_t idx = t + q * kPlaceThreads;
+        v[q] = idx < n ? tmp[s0 + idx] : 0ull;
+        if (idx < n) atomicAdd(&fh[(uint32_t)(v[q] >> 32) - (uint32_t)k0], 1u);
+    }
+    __syncthreads();
+    const uint32_t total = lds_exclusive_scan(fh, nk, wsum);
+    for (uint32_t x = t; x < nk; x += kPlaceThreads) {
+        const uint32_t st = fh[x], c = (x + 1 < nk ? fh[x + 1] : total) - st;
+        fdelta[x] = c ? atomicAdd(&cursor[k0 + x], c) - st : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q) {
+        const uint32_t idx = t + q * kPlaceThreads;
+        if (idx >= n) continue;
+        const uint32_t fk = (uint32_t)(v[q] >> 32) - (uint32_t)k0;
+        const uint32_t rr = atomicAdd(&fh[fk], 1u);
+        sval[rr] = (uint32_t)v[q];
+        skey[rr] = (uint16_t)fk;
+    }
+    __syncthreads();
+    for (uint32_t p = t; p < n; p += kPlaceThreads) ent[fdelta[skey[p]] + p] = sval[p];
+}
+
 #define MH_WINDOW_CASES(X) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)
 
 void launch_digits(bool scatter, uint32_t c, uint64_t max_n, hipStream_t st, const MsmDesc *descs, const Geometry &g,
@@ -231,6 +422,26 @@ void launch_digits(bool scatter, uint32_t c, uint64_t max_n, hipStream_t st, con
     case CC:                                                                                                  \
         hipLaunchKernelGGL(k_sort_lds<CC>, grid, dim3(kSortThreads), lds, st, descs, g, tile, range_keys,     \
                            counters, ent, scatter ? 1 : 0);                                                   \
+        break;
+        MH_WINDOW_CASES(MH_CASE)
+#undef MH_CASE
+    default:
+        throw hip_error((int)hipErrorInvalidValue, "unsupported MSM window width");
+    }
+    MH_HIP(hipGetLastError());
+}
+
+void launch_place_coarse(uint32_t c, uint64_t max_n, hipStream_t st, const MsmDesc *descs, const Geometry &g,
+                         uint32_t fshift, uint32_t nbins, uint32_t *ccursor, uint64_t *tmp)
+{
+    const uint32_t nw = (256 + c - 1) / c;
+    const uint32_t tile = std::min<uint32_t>(kPlaceThreads, kStage1 / nw);
+    dim3 grid((uint32_t)ceil_div(max_n, tile), g.M);
+    switch (c) {
+#define MH_CASE(CC)                                                                                           \
+    case CC:                                                                                                  \
+        hipLaunchKernelGGL(k_place_coarse<CC>, grid, dim3(kPlaceThreads), 0, st, descs, g, tile, fshift, nbins, \
+                           ccursor, tmp);                                                                     \
         break;
         MH_WINDOW_CASES(MH_CASE)
 #undef MH_CASE
@@ -592,6 +803,8 @@ void build_precomp(MsmContext &ctx, hipStream_t st)
 
 struct MsmEngine::Buffers {
     DevBuf<uint32_t> cnt, off, cursor, q, co, cmap, ent, scan_tmp, totals;
+    DevBuf<uint64_t> tmp;                          // (key, entry) pairs between the placement passes
+    DevBuf<uint32_t> ccursor, nch2, cst2, cmap2, t2;   // coarse bins: cursors, pass-2 chunks, map, total
     DevBuf<uint32_t> lc, lo, lmap;   // tree levels: groups per bucket, their offsets, lane -> bucket
     DevBuf<uint32_t> part, part2, bucket, pp, pp2, pj;
     DevBuf<uint32_t> seg_off, seg_len, out_off;
@@ -617,6 +830,10 @@ struct MsmEngine::State {
     PlaneLayout pl{};
     const uint32_t *d_points = nullptr;
     uint64_t max_n = 0, nkeys = 0, nent_max = 0, nch_max = 0, npp = 0, nseg = 0;
+    // two-pass placement: 2^fshift keys per coarse bin, nbins bins, grid of pass 2
+    bool two_pass = false;
+    uint32_t fshift = 0, nbins = 0;
+    uint64_t fine_blocks = 0;
     uint32_t L = 0, G = 0, tree_levels = 0, G1 = 0;
     bool precomputed = false, ready = false;
     std::vector<Level> levels;
@@ -758,6 +975,19 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
     b.scan_tmp.ensure(scan_tmp_elems(s.nkeys));
     b.totals.ensure(4);
     b.ent.ensure(std::max<uint64_t>(s.nent_max, 1));
+    s.fshift = 0;
+    while ((s.nkeys + (1ull << s.fshift) - 1) >> s.fshift > kCoarseBins) ++s.fshift;
+    s.two_pass = (1ull << s.fshift) <= kMaxFine;
+    if (s.two_pass) {
+        s.nbins = (uint32_t)((s.nkeys + (1ull << s.fshift) - 1) >> s.fshift);
+        s.fine_blocks = ceil_div(s.nent_max, kFineChunk) + s.nbins;
+        b.tmp.ensure(std::max<uint64_t>(s.nent_max, 1));
+        b.ccursor.ensure(s.nbins);
+        b.nch2.ensure(s.nbins);
+        b.cst2.ensure(s.nbins);
+        b.cmap2.ensure(s.fine_blocks);
+        b.t2.ensure(1);
+    }
     b.cmap.ensure(s.nch_max);
     b.part.ensure(s.nch_max * kXyzz28Words);
     if (s.tree_levels > 0) {
@@ -861,7 +1091,24 @@ void MsmEngine::enqueue_front(hipStream_t st, ProfileScope *prof)
     hipLaunchKernelGGL(k_copy_u32, dim3((uint32_t)std::min<uint64_t>(grid1(nkeys), 4096)), dim3(kBlock), 0, st,
                        b.cursor.get(), (const uint32_t *)b.off.get(), nkeys);
     MH_HIP(hipGetLastError());
-    if (s.max_n) launch_digits(true, g.c, s.max_n, st, b.descs.get(), g, b.cursor.get(), b.ent.get());
+    if (s.max_n && s.two_pass) {
+        hipLaunchKernelGGL(k_bin_setup, dim3(grid1(s.nbins)), dim3(kBlock), 0, st, (const uint32_t *)b.off.get(),
+                           (const uint32_t *)b.cnt.get(), nkeys, s.fshift, s.nbins, b.ccursor.get(), b.nch2.get());
+        MH_HIP(hipGetLastError());
+        scan_launch(b.nch2.get(), b.cst2.get(), s.nbins, b.scan_tmp.get(), st);
+        hipLaunchKernelGGL(k_fine_map, dim3(grid1(s.nbins)), dim3(kBlock), 0, st, (const uint32_t *)b.nch2.get(),
+                           (const uint32_t *)b.cst2.get(), s.nbins, b.cmap2.get(), b.t2.get());
+        MH_HIP(hipGetLastError());
+        launch_place_coarse(g.c, s.max_n, st, b.descs.get(), g, s.fshift, s.nbins, b.ccursor.get(), b.tmp.get());
+        hipLaunchKernelGGL(k_place_fine, dim3((uint32_t)s.fine_blocks), dim3(kPlaceThreads), 0, st,
+                           (const uint64_t *)b.tmp.get(), (const uint32_t *)b.off.get(),
+                           (const uint32_t *)b.cnt.get(), (const uint32_t *)b.cmap2.get(),
+                           (const uint32_t *)b.cst2.get(), (const uint32_t *)b.t2.get(), nkeys, s.fshift,
+                           b.cursor.get(), b.ent.get());
+        MH_HIP(hipGetLastError());
+    } else if (s.max_n) {
+        launch_digits(true, g.c, s.max_n, st, b.descs.get(), g, b.cursor.get(), b.ent.get());
+    }
 
     // 4-6: chunked accumulation into buckets
     hipLaunchKernelGGL(k_chunk_counts, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.cnt.get(), b.q.get(), nkeys, s.L,
