@@ -406,7 +406,10 @@ void launch_digits(bool scatter, uint32_t c, uint64_t max_n, hipStream_t st, con
                    uint32_t *counters, uint32_t *ent)
 {
     const uint32_t keys_per_msm = g.kw * g.B;
-    const uint32_t range_keys = std::min<uint32_t>(keys_per_msm, 16384);
+    // up to 32768 keys (a 128 KiB LDS histogram; gfx950 gives a workgroup
+    // 160 KiB): a 16-bit window's 2^15 buckets in one range, so every block
+    // derives its digits once
+    const uint32_t range_keys = std::min<uint32_t>(keys_per_msm, 32768);
     const uint32_t ranges = (keys_per_msm + range_keys - 1) / range_keys;
     // tiles of up to 16384 scalars (each block flushes one global atomic per
     // touched bucket, so bigger tiles mean fewer atomics), but at least ~512
