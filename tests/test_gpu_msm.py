@@ -149,6 +149,21 @@ def test_msm_batch_entry(H, O):
         assert O.g1_eq(out[i], closed_form(O, ts, sc[i * n:(i + 1) * n]))
 
 
+def test_msm_batch_many_small_msms(H, O):
+    """1100 one-shot MSMs of 64 shared points: plain 8-bit windows give
+    1100 x 32 x 128 = 4.5M bucket keys, more than the two-pass placement
+    covers (4096 keys per coarse bin x 1024 bins), so the direct placement
+    pass runs."""
+    n, m = 64, 1100
+    ts = O.random_fr(n, 53)
+    pts = points_for(O, ts)
+    sc = O.random_fr(n * m, 54)
+    out = H.g1_msm_batch(pts, sc, m)
+    for i in range(0, m, 7):
+        assert O.g1_eq(out[i], closed_form(O, ts, sc[i * n:(i + 1) * n]))
+    assert O.g1_eq(out[m - 1], closed_form(O, ts, sc[(m - 1) * n:]))
+
+
 @pytest.mark.parametrize("log_n", [16, 18])
 def test_msm_medium_closed_form(H, O, log_n):
     n = 1 << log_n
