@@ -25,7 +25,7 @@ RustError make_error(int code, const char *msg)
 
 namespace {
 thread_local uint32_t t_lane = 0;
-hipStream_t g_streams[64][kMaxLanes + 1];
+hipStream_t g_streams[64][kLaneCount];
 std::mutex g_stream_mu;
 }  // namespace
 
@@ -39,12 +39,24 @@ int device_id()
     return d;
 }
 
+// Streams are created once per (device, lane) and live until the process
+// ends, so each thread caches the ones it has seen: the shared table and
+// its mutex are touched once per thread and lane, not on every call.
 hipStream_t lane_stream(uint32_t lane)
 {
-    int dev = device_id();
+    MH_CHECK(lane < kLaneCount, (int)hipErrorInvalidValue, "invalid execution lane");
+    const int dev = device_id();
+    thread_local int t_dev = -1;
+    thread_local hipStream_t t_cache[kLaneCount] = {};
+    if (t_dev != dev) {
+        for (auto &x : t_cache) x = nullptr;
+        t_dev = dev;
+    }
+    if (t_cache[lane]) return t_cache[lane];
     std::lock_guard<std::mutex> g(g_stream_mu);
     hipStream_t &s = g_streams[dev][lane];
     if (!s) MH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    t_cache[lane] = s;
     return s;
 }
 
@@ -72,7 +84,7 @@ struct LaneScratch {
     std::mutex mu;
     DevBuf<Fr> a, b, c;
 };
-LaneScratch g_scratch[64][kMaxLanes + 1];
+LaneScratch g_scratch[64][kLaneCount];
 }  // namespace
 
 LaneScratch &lane_scratch() { return g_scratch[device_id()][current_lane()]; }

@@ -287,7 +287,7 @@ struct GpScratch {
 
 GpScratch &gp_scratch()
 {
-    static GpScratch s[64][kMaxLanes + 1];
+    static GpScratch s[64][kLaneCount];
     return s[device_id()][current_lane()];
 }
 

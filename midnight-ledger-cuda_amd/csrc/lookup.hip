@@ -157,7 +157,7 @@ struct LookupScratch {
 
 LookupScratch &lookup_scratch()
 {
-    static LookupScratch s[64][kMaxLanes + 1];
+    static LookupScratch s[64][kLaneCount];
     return s[device_id()][current_lane()];
 }
 

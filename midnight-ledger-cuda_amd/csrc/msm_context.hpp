@@ -27,7 +27,7 @@ struct MsmContext {
         MsmEngine engine;
     };
     std::mutex mu;   // guards creation of lane engines
-    std::unique_ptr<LaneEngine> engines[kMaxLanes + 1];
+    std::unique_ptr<LaneEngine> engines[kLaneCount];
     LaneEngine &lane_engine(uint32_t lane)
     {
         std::lock_guard<std::mutex> g(mu);

@@ -475,7 +475,7 @@ const T *up_array(DevBuf<T> &buf, const T *src, size_t n, hipStream_t st)
 // Global slot files are per lane: calls of one lane run in stream order.
 LaneSlotFile &numerator_slot_file()
 {
-    static LaneSlotFile files[64][kMaxLanes + 1];
+    static LaneSlotFile files[64][kLaneCount];
     return files[device_id()][current_lane()];
 }
 

@@ -210,7 +210,7 @@ struct PlonkKey {
     HostFr omega, omega_inv, ext_omega, repr;
     std::vector<HostFr> t_inv, delta_powers;
     void *numctx = nullptr;
-    DevBuf<Fr> scratch_by_lane[kMaxLanes + 1];   // NTT scratch of proofs running on each lane
+    DevBuf<Fr> scratch_by_lane[kLaneCount];   // NTT scratch of proofs running on each lane
     DevBuf<Fr> &scratch() { return scratch_by_lane[current_lane()]; }
     ~PlonkKey()
     {

@@ -52,6 +52,11 @@ inline RustError ok() { return RustError{0, nullptr}; }
 // producer and consumer of a lane's device vectors runs on that stream, so
 // stream order is the only ordering the lane needs.
 constexpr uint32_t kMaxLanes = 8;
+// Lanes that exist internally: 0 .. kMaxLanes are the ones callers select
+// (as in the reference); the proof server may run up to kServerSlots
+// workers on lanes 1 .. kServerSlots, beyond the public range.
+constexpr uint32_t kServerSlots = 16;
+constexpr uint32_t kLaneCount = kServerSlots + 1;
 uint32_t current_lane();
 void set_current_lane(uint32_t lane);
 hipStream_t lane_stream(uint32_t lane);

@@ -140,8 +140,8 @@ using namespace mh;
 
 extern "C" RustError midnight_hip_proof_server_create(void **out, uint32_t slots, uint64_t hbm_budget_bytes)
 {
-    if (!out || slots == 0 || slots > kMaxLanes)
-        return make_error((int)hipErrorInvalidValue, "slots must be between 1 and the lane count");
+    if (!out || slots == 0 || slots > kServerSlots)
+        return make_error((int)hipErrorInvalidValue, "slots must be between 1 and 16");
     *out = nullptr;
     try {
         auto s = std::make_unique<ProofServer>();
