@@ -542,17 +542,19 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(const uint32_t *pts, cons
     const uint32_t r = j - co[b];
     const uint32_t start = off[b] + r * L;
     const uint32_t end = min(start + L, off[b] + cnt[b]);
-    Xyzz acc = Xyzz::inf();
+    // each point is converted to radix 2^28 (fully reduced: the rare doubling
+    // path takes it as canonical) and added in loose coordinates, as in the
+    // precomputed pass
+    Xyzz28 acc = xyzz28_inf();
     uint32_t e = ent[start];
     Affine nxt = fetch_point(pts, e);
     for (uint32_t x = start; x < end; ++x) {
         Affine cur = nxt;
         if (x + 1 < end) nxt = fetch_point(pts, ent[x + 1]);
-        acc = xyzz_add_affine(acc, cur);
+        acc = xyzz28_add_affine_lz(acc, affine28_from_std(cur));
     }
-    const Xyzz28 out = xyzz28_from_std_lz(acc);
-    if (q[b] == 1) store_xyzz28(bucket + (uint64_t)b * kXyzz28Words, out);
-    else store_xyzz28(part + (uint64_t)j * kXyzz28Words, out);
+    if (q[b] == 1) store_xyzz28(bucket + (uint64_t)b * kXyzz28Words, acc);
+    else store_xyzz28(part + (uint64_t)j * kXyzz28Words, acc);
 }
 
 // Same pass over a precomputed table held in radix 2^28 (28 u32 per point,
