@@ -81,6 +81,15 @@ def test_server_results_same_alone_and_in_mix(H, O):
         assert proof == alone[key], key
         assert qms >= 0 and pms > 0
     server.free()
+    # 16 slots: workers on lanes 9 .. 16 too, beyond the lanes callers select
+    server = H.ProofServer(16, 8 << 30)
+    jobs = [((i, seed), server.submit(pk, adv[i], seed))
+            for rep in range(3) for i, (pk, srs) in enumerate(keys) for seed in (1, 2)]
+    for key, jid in jobs:
+        assert server.wait(jid)[0] == alone[key], key
+    server.free()
+    with pytest.raises(H.MidnightHipError):
+        H.ProofServer(17, 8 << 30)
     # a budget smaller than one proof still serves (one at a time)
     server = H.ProofServer(3, 1)
     jids = [server.submit(keys[1][0], adv[1], 2) for _ in range(4)]
