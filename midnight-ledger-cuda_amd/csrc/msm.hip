@@ -98,59 +98,6 @@ __device__ __forceinline__ void canonical_scalar(const Fr *s, uint32_t k[9])
     k[8] = 0;
 }
 
-// The window count is a template parameter so that every digit of a scalar
-// is formed first and all of its atomics are in flight together, instead
-// of one atomic round trip per loop iteration.
-template <uint32_t C>
-__global__ void __launch_bounds__(kBlock) k_digits_hist(const MsmDesc *descs, Geometry g, uint32_t *cnt)
-{
-    constexpr uint32_t NW = (256 + C - 1) / C;
-    const uint32_t m = blockIdx.y;
-    const MsmDesc d = descs[m];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.n) return;
-    uint32_t k[9];
-    canonical_scalar(d.scalars + i, k);
-    uint32_t carry = 0;
-    const uint64_t base = (uint64_t)m * g.kw;
-    int32_t dg[NW];
-#pragma unroll
-    for (uint32_t w = 0; w < NW; ++w) dg[w] = window_digit(k, w, C, carry);
-#pragma unroll
-    for (uint32_t w = 0; w < NW; ++w) {
-        if (dg[w] == 0) continue;
-        uint32_t b = (uint32_t)(dg[w] < 0 ? -dg[w] : dg[w]) - 1;
-        atomicAdd(&cnt[key_of(g, base, w, b)], 1u);
-    }
-}
-
-template <uint32_t C>
-__global__ void __launch_bounds__(kBlock) k_scatter(const MsmDesc *descs, Geometry g, uint32_t *cursor, uint32_t *ent)
-{
-    constexpr uint32_t NW = (256 + C - 1) / C;
-    const uint32_t m = blockIdx.y;
-    const MsmDesc d = descs[m];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.n) return;
-    uint32_t k[9];
-    canonical_scalar(d.scalars + i, k);
-    uint32_t carry = 0;
-    const uint64_t base = (uint64_t)m * g.kw;
-    const uint32_t pidx = d.pt_off + i;
-    int32_t dg[NW];
-    uint32_t pos[NW];
-#pragma unroll
-    for (uint32_t w = 0; w < NW; ++w) dg[w] = window_digit(k, w, C, carry);
-#pragma unroll
-    for (uint32_t w = 0; w < NW; ++w) {
-        uint32_t b = (uint32_t)(dg[w] < 0 ? -dg[w] : dg[w]) - 1;
-        pos[w] = dg[w] ? atomicAdd(&cursor[key_of(g, base, w, b)], 1u) : 0u;
-    }
-#pragma unroll
-    for (uint32_t w = 0; w < NW; ++w)
-        if (dg[w]) ent[pos[w]] = (pidx + w * g.pstride) | (dg[w] < 0 ? kSignBit : 0u);
-}
-
 // Counting sort through LDS.  A block takes a tile of one MSM's scalars and
 // one range of that MSM's bucket keys (local key b * kw + w); it counts the
 // digits of the tile that fall in the range in an LDS histogram, so global
