@@ -609,4 +609,71 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_lz(const Xyzz28 &p, const Xyzz28 &q
     return r;
 }
 
+// ------------------------------------------------- four lanes per addition
+//
+// A pass with too few partials to give every SIMD a wave is bound by the
+// latency of one lane's chain of additions, not by issue.  Four lanes (a DPP
+// quad) then share one addition: add-2008-s has its fourteen products in
+// four dependent steps of at most four, so each lane runs one product per
+// step and the quad broadcasts the four results.  Every lane of a quad holds
+// the same operands and results, so the quad takes every branch together,
+// and each product is the same Montgomery product of the same operands as
+// in xyzz28_add_lz (a square as a product of equal operands gives the same
+// limbs): the sum is limb-identical to the one-lane form.
+
+// lane `kLane` of each quad -> all four lanes of the quad
+template <int kLane>
+__device__ __forceinline__ Fp28 fp28_quad_bcast(const Fp28 &a)
+{
+    Fp28 r;
+#pragma unroll
+    for (int i = 0; i < 14; ++i)
+        r.v[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)a.v[i], kLane * 0x55, 0xf, 0xf, false);
+    return r;
+}
+
+// operand of quad lane `sub`: masks and ORs per limb (a chain of selects
+// is turned into a stack table indexed by `sub`)
+__device__ __forceinline__ Fp28 fp28_sel4(uint32_t sub, const Fp28 &a0, const Fp28 &a1, const Fp28 &a2,
+                                          const Fp28 &a3)
+{
+    const uint32_t m0 = 0u - (uint32_t)(sub == 0), m1 = 0u - (uint32_t)(sub == 1);
+    const uint32_t m2 = 0u - (uint32_t)(sub == 2), m3 = 0u - (uint32_t)(sub == 3);
+    Fp28 r;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) r.v[i] = (a0.v[i] & m0) | (a1.v[i] & m1) | (a2.v[i] & m2) | (a3.v[i] & m3);
+    return r;
+}
+
+// xyzz28_add_lz on a quad; `sub` is the lane's place in its quad
+__device__ __forceinline__ Xyzz28 xyzz28_add_quad(const Xyzz28 &p, const Xyzz28 &q, uint32_t sub)
+{
+    if (q.is_inf()) return p;
+    if (p.is_inf()) return q;
+    // u1 = x1 zz2, u2 = x2 zz1, s1 = y1 zzz2, s2 = y2 zzz1
+    Fp28 t = fp28_mul<false>(fp28_sel4(sub, p.x, q.x, p.y, q.y), fp28_sel4(sub, q.zz, p.zz, q.zzz, p.zzz));
+    const Fp28 u1 = fp28_quad_bcast<0>(t), s1 = fp28_quad_bcast<2>(t);
+    const Fp28 pp_ = fp28_sub_lz(fp28_quad_bcast<1>(t), u1);
+    const Fp28 r_ = fp28_sub_lz(fp28_quad_bcast<3>(t), s1);
+    if (fp28_is_zero_lz(pp_)) {
+        if (fp28_is_zero_lz(r_)) return xyzz28_dbl_lz(p);
+        return xyzz28_inf();
+    }
+    // PP = P^2, R^2, zz1 zz2, zzz1 zzz2
+    t = fp28_mul<false>(fp28_sel4(sub, pp_, r_, p.zz, p.zzz), fp28_sel4(sub, pp_, r_, q.zz, q.zzz));
+    const Fp28 pp = fp28_quad_bcast<0>(t), r2 = fp28_quad_bcast<1>(t);
+    const Fp28 zz12 = fp28_quad_bcast<2>(t), zzz12 = fp28_quad_bcast<3>(t);
+    // PPP = P PP, Q = u1 PP, zz3 = zz1 zz2 PP (lane 3 repeats lane 2)
+    t = fp28_mul<false>(fp28_sel4(sub, pp_, u1, zz12, zz12), pp);
+    const Fp28 ppp = fp28_quad_bcast<0>(t), qq = fp28_quad_bcast<1>(t);
+    Xyzz28 r;
+    r.zz = fp28_quad_bcast<2>(t);
+    r.x = fp28_sub_lz(fp28_sub_lz(r2, ppp), fp28_add_lz(qq, qq));
+    // R (Q - x3), s1 PPP, zzz3 = zzz1 zzz2 PPP
+    t = fp28_mul<false>(fp28_sel4(sub, r_, s1, zzz12, zzz12), fp28_sel4(sub, fp28_sub_lz(qq, r.x), ppp, ppp, ppp));
+    r.y = fp28_sub_lz(fp28_quad_bcast<0>(t), fp28_quad_bcast<1>(t));
+    r.zzz = fp28_quad_bcast<2>(t);
+    return r;
+}
+
 }  // namespace mh

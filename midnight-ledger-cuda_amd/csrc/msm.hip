@@ -575,11 +575,23 @@ __global__ void k_level_map(const uint32_t *lc, const uint32_t *lo, uint64_t nke
     if (b == nkeys - 1) totals[2] = o + n;
 }
 
+// One sum per lane (kLanes = 1) or per quad of lanes (kLanes = 4, for
+// passes too small to give every SIMD a wave); `sub` is the lane's place
+// in its quad and only its first lane stores.
+template <int kLanes>
+__device__ __forceinline__ Xyzz28 msm_add(const Xyzz28 &p, const Xyzz28 &q, uint32_t sub)
+{
+    if constexpr (kLanes == 4) return xyzz28_add_quad(p, q, sub);
+    else return xyzz28_add_lz(p, q);
+}
+
+template <int kLanes>
 __global__ void __launch_bounds__(kBlock) k_tree_level(const uint32_t *q, const uint32_t *co, const uint32_t *lmap,
                                                        const uint32_t *lo, const uint32_t *totals, uint64_t div,
                                                        uint32_t G, const uint32_t *in, uint32_t *out)
 {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = t / kLanes, sub = t % kLanes;
     if (j >= totals[2]) return;
     const uint32_t b = lmap[j];
     const uint32_t r = j - lo[b];
@@ -587,18 +599,22 @@ __global__ void __launch_bounds__(kBlock) k_tree_level(const uint32_t *q, const 
     const uint32_t s0 = r * G, s1 = min(s0 + G, qprev);
     const uint64_t base = co[b];
     Xyzz28 acc = xyzz28_inf();
-    for (uint32_t x = s0; x < s1; ++x) acc = xyzz28_add_lz(acc, load_xyzz28(in + (base + x) * kXyzz28Words));
-    store_xyzz28(out + (base + r) * kXyzz28Words, acc);
+    for (uint32_t x = s0; x < s1; ++x)
+        acc = msm_add<kLanes>(acc, load_xyzz28(in + (base + x) * kXyzz28Words), sub);
+    if (sub == 0) store_xyzz28(out + (base + r) * kXyzz28Words, acc);
 }
 
 // Bucket b after `levels` levels: the levels that shrank it are the first
 // lv, so its partials are in `part` (lv even) or `part2` (lv odd).  Buckets
 // of one chunk were written by the accumulation.
+template <int kLanes>
 __global__ void __launch_bounds__(kBlock) k_tree_final(const uint32_t *q, const uint32_t *co, uint64_t nkeys,
                                                        uint32_t G, uint32_t levels, const uint32_t *part,
                                                        const uint32_t *part2, uint32_t *bucket)
 {
-    const uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t b = t / kLanes;
+    const uint32_t sub = (uint32_t)(t % kLanes);
     if (b >= nkeys) return;
     uint32_t rem = q[b];
     if (rem <= 1) return;
@@ -610,8 +626,8 @@ __global__ void __launch_bounds__(kBlock) k_tree_final(const uint32_t *q, const 
     const uint32_t *src = (lv & 1) ? part2 : part;
     const uint64_t base = co[b];
     Xyzz28 acc = xyzz28_inf();
-    for (uint32_t x = 0; x < rem; ++x) acc = xyzz28_add_lz(acc, load_xyzz28(src + (base + x) * kXyzz28Words));
-    store_xyzz28(bucket + b * kXyzz28Words, acc);
+    for (uint32_t x = 0; x < rem; ++x) acc = msm_add<kLanes>(acc, load_xyzz28(src + (base + x) * kXyzz28Words), sub);
+    if (sub == 0) store_xyzz28(bucket + b * kXyzz28Words, acc);
 }
 
 // zeroes the bucket counters and the totals
@@ -651,10 +667,13 @@ struct PlaneLayout {
     uint32_t members[kMaxWindow];        // members of plane k
 };
 
+template <int kLanes>
 __global__ void __launch_bounds__(kBlock) k_plane_partials(const uint32_t *bucket, const uint32_t *cnt, Geometry g,
                                                            PlaneLayout pl, uint32_t G1, uint64_t total, uint32_t *pp)
 {
-    uint64_t id = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t id = t / kLanes;
+    const uint32_t sub = (uint32_t)(t % kLanes);
     if (id >= total) return;
     uint64_t mw = id / pl.per_window;
     uint32_t rem = (uint32_t)(id % pl.per_window);
@@ -669,18 +688,21 @@ __global__ void __launch_bounds__(kBlock) k_plane_partials(const uint32_t *bucke
         uint32_t x = ((e >> k) << (k + 1)) | (1u << k) | (e & lowmask);
         const uint64_t key = (uint64_t)(x - 1) * stride + mw;
         // empty buckets are not written
-        if (cnt[key]) acc = xyzz28_add_lz(acc, load_xyzz28(bucket + key * kXyzz28Words));
+        if (cnt[key]) acc = msm_add<kLanes>(acc, load_xyzz28(bucket + key * kXyzz28Words), sub);
     }
-    store_xyzz28(pp + id * kXyzz28Words, acc);
+    if (sub == 0) store_xyzz28(pp + id * kXyzz28Words, acc);
 }
 
 // Sum groups of G consecutive partials within segments of `seg` partials;
 // output segment length ceil(seg / G).  Results as Jacobian when `to_jac`.
+template <int kLanes>
 __global__ void __launch_bounds__(kBlock) k_group_sums(const uint32_t *in, uint64_t nseg, const uint32_t *seg_off,
                                                        const uint32_t *seg_len, const uint32_t *out_off, uint32_t G,
                                                        uint64_t total_out, uint32_t *out, int to_jac)
 {
-    uint64_t id = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t id = t / kLanes;
+    const uint32_t sub = (uint32_t)(t % kLanes);
     if (id >= total_out) return;
     // locate segment by binary search over out_off (nseg small)
     uint64_t lo = 0, hi = nseg;
@@ -692,7 +714,8 @@ __global__ void __launch_bounds__(kBlock) k_group_sums(const uint32_t *in, uint6
     uint32_t local = (uint32_t)(id - out_off[lo]);
     uint32_t s0 = seg_off[lo] + local * G, s1 = min(s0 + G, seg_off[lo] + seg_len[lo]);
     Xyzz28 acc = xyzz28_inf();
-    for (uint32_t x = s0; x < s1; ++x) acc = xyzz28_add_lz(acc, load_xyzz28(in + (uint64_t)x * kXyzz28Words));
+    for (uint32_t x = s0; x < s1; ++x) acc = msm_add<kLanes>(acc, load_xyzz28(in + (uint64_t)x * kXyzz28Words), sub);
+    if (sub) return;
     // the last pass converts (fully reduced) to the 32-bit Jacobian form
     if (to_jac) store_jacobian(out + id * 36, xyzz_to_jacobian(xyzz28_to_std(acc)));
     else store_xyzz28(out + id * kXyzz28Words, acc);
@@ -1092,6 +1115,8 @@ void MsmEngine::enqueue_back(hipStream_t st, uint32_t levels, ProfileScope *prof
     State &s = *st_;
     const Geometry &g = s.g;
     const uint64_t nkeys = s.nkeys;
+    // passes of fewer sums than this run each sum on a quad of lanes
+    const auto quad = [this](uint64_t sums) { return sums < quad_below; };
     levels = std::min(levels, s.tree_levels);
 
     // bucket tree: dense levels, then one lane per bucket
@@ -1106,27 +1131,46 @@ void MsmEngine::enqueue_back(hipStream_t st, uint32_t levels, ProfileScope *prof
                            (const uint32_t *)b.lo.get(), nkeys, b.lmap.get(), b.totals.get());
         MH_HIP(hipGetLastError());
         const uint64_t lanes = s.nch_max / (div * s.G) + nkeys;   // bound on the level's groups
-        hipLaunchKernelGGL(k_tree_level, dim3(grid1(lanes)), dim3(kBlock), 0, st, b.q.get(), b.co.get(),
-                           b.lmap.get(), b.lo.get(), b.totals.get(), div, s.G, (const uint32_t *)tin, tout);
+        if (quad(lanes))
+            hipLaunchKernelGGL(k_tree_level<4>, dim3(grid1(4 * lanes)), dim3(kBlock), 0, st, b.q.get(), b.co.get(),
+                               b.lmap.get(), b.lo.get(), b.totals.get(), div, s.G, (const uint32_t *)tin, tout);
+        else
+            hipLaunchKernelGGL(k_tree_level<1>, dim3(grid1(lanes)), dim3(kBlock), 0, st, b.q.get(), b.co.get(),
+                               b.lmap.get(), b.lo.get(), b.totals.get(), div, s.G, (const uint32_t *)tin, tout);
         MH_HIP(hipGetLastError());
         div *= s.G;
         std::swap(tin, tout);
     }
-    hipLaunchKernelGGL(k_tree_final, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), nkeys, s.G,
-                       levels, (const uint32_t *)b.part.get(), (const uint32_t *)b.part2.get(), b.bucket.get());
+    if (quad(nkeys))
+        hipLaunchKernelGGL(k_tree_final<4>, dim3(grid1(4 * nkeys)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), nkeys,
+                           s.G, levels, (const uint32_t *)b.part.get(), (const uint32_t *)b.part2.get(),
+                           b.bucket.get());
+    else
+        hipLaunchKernelGGL(k_tree_final<1>, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), nkeys,
+                           s.G, levels, (const uint32_t *)b.part.get(), (const uint32_t *)b.part2.get(),
+                           b.bucket.get());
     MH_HIP(hipGetLastError());
 
     // 7: bit-plane partial sums and group passes down to one per segment
-    hipLaunchKernelGGL(k_plane_partials, dim3(grid1(s.npp)), dim3(kBlock), 0, st, b.bucket.get(), b.cnt.get(), g, s.pl,
-                       s.G1, s.npp, b.pp.get());
+    if (quad(s.npp))
+        hipLaunchKernelGGL(k_plane_partials<4>, dim3(grid1(4 * s.npp)), dim3(kBlock), 0, st, b.bucket.get(),
+                           b.cnt.get(), g, s.pl, s.G1, s.npp, b.pp.get());
+    else
+        hipLaunchKernelGGL(k_plane_partials<1>, dim3(grid1(s.npp)), dim3(kBlock), 0, st, b.bucket.get(), b.cnt.get(),
+                           g, s.pl, s.G1, s.npp, b.pp.get());
     MH_HIP(hipGetLastError());
     uint32_t *lin = b.pp.get(), *lout = b.pp2.get();
     for (size_t i = 0; i < s.levels.size(); ++i) {
         const State::Level &lv = s.levels[i];
         const bool last = i + 1 == s.levels.size();
-        hipLaunchKernelGGL(k_group_sums, dim3(grid1(lv.total_out)), dim3(kBlock), 0, st, lin, s.nseg,
-                           b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base, lv.G,
-                           lv.total_out, last ? b.pj.get() : lout, last ? 1 : 0);
+        if (quad(lv.total_out))
+            hipLaunchKernelGGL(k_group_sums<4>, dim3(grid1(4 * lv.total_out)), dim3(kBlock), 0, st, lin, s.nseg,
+                               b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base,
+                               lv.G, lv.total_out, last ? b.pj.get() : lout, last ? 1 : 0);
+        else
+            hipLaunchKernelGGL(k_group_sums<1>, dim3(grid1(lv.total_out)), dim3(kBlock), 0, st, lin, s.nseg,
+                               b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base,
+                               lv.G, lv.total_out, last ? b.pj.get() : lout, last ? 1 : 0);
         MH_HIP(hipGetLastError());
         std::swap(lin, lout);
     }

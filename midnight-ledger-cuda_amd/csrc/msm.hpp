@@ -68,6 +68,11 @@ public:
     uint32_t chunk_len = 64;      // entries per accumulation lane
     uint32_t plane_group = 16;    // buckets per lane in the first plane pass
     uint32_t plane_group2 = 8;    // partials per lane in the later plane passes
+    // Reduction passes of fewer sums than this give each sum a quad of
+    // lanes (four products per step of the addition instead of one): about
+    // a third of the latency per addition for a third more issue, a gain
+    // while the pass leaves SIMDs without a wave (0 = never).
+    uint64_t quad_below = 3 * 16384;
 
     // statistics of the last run (for throughput / roofline reporting)
     uint64_t last_entries = 0;    // nonzero signed digits = bucket additions
