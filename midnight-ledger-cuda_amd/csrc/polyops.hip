@@ -30,22 +30,22 @@ namespace mh {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kChunk = 16;              // coefficients per lane in evaluations
+constexpr size_t kEvalFill = 256 * 4 * 2 * 64;   // lanes that give every SIMD two waves
 constexpr int kScanItems = 8;
 constexpr size_t kScanTile = (size_t)kBlock * kScanItems;
 
 // partial[(poly * npts + pt) * nblk + block] = sum over the block's rows
-__global__ __launch_bounds__(kBlock) void k_eval(const Fr *polys, size_t len, const Fr *tabs, uint32_t levels,
-                                                 uint32_t npts, Fr *partial)
+__global__ __launch_bounds__(kBlock) void k_eval(const Fr *polys, size_t len, uint32_t chunk, const Fr *tabs,
+                                                 uint32_t levels, uint32_t npts, Fr *partial)
 {
     __shared__ uint32_t sh[8][kBlock];
     const uint32_t poly = blockIdx.y, pt = blockIdx.z;
     const Fr *p = polys + (size_t)poly * len;
     const Fr *tab = tabs + (size_t)pt * levels * kOmegaTab;
-    const size_t start = ((size_t)blockIdx.x * kBlock + threadIdx.x) * kChunk;
+    const size_t start = ((size_t)blockIdx.x * kBlock + threadIdx.x) * chunk;
     Fr acc = Fr::zero();
     if (start < len) {
-        const size_t end = std::min(start + (size_t)kChunk, len);
+        const size_t end = std::min(start + (size_t)chunk, len);
         const Fr x = load_fr(tab + 1);   // x^1
         for (size_t i = end; i-- > start;) acc = acc * x + load_fr(p + i);
         acc = acc * omega_pow_tab(start, tab, levels);
@@ -78,17 +78,17 @@ __global__ __launch_bounds__(kBlock) void k_eval(const Fr *polys, size_t len, co
 // partial[query * nblk + block]: query q evaluates polys[qpoly[q]] at
 // point qpt[q] (power tables tabs[pt])
 __global__ __launch_bounds__(kBlock) void k_eval_queries(const Fr *const *polys, const uint32_t *qpoly,
-                                                         const uint32_t *qpt, size_t len, const Fr *tabs,
-                                                         uint32_t levels, Fr *partial)
+                                                         const uint32_t *qpt, size_t len, uint32_t chunk,
+                                                         const Fr *tabs, uint32_t levels, Fr *partial)
 {
     __shared__ uint32_t sh[8][kBlock];
     const uint32_t q = blockIdx.y;
     const Fr *p = polys[qpoly[q]];
     const Fr *tab = tabs + (size_t)qpt[q] * levels * kOmegaTab;
-    const size_t start = ((size_t)blockIdx.x * kBlock + threadIdx.x) * kChunk;
+    const size_t start = ((size_t)blockIdx.x * kBlock + threadIdx.x) * chunk;
     Fr acc = Fr::zero();
     if (start < len) {
-        const size_t end = std::min(start + (size_t)kChunk, len);
+        const size_t end = std::min(start + (size_t)chunk, len);
         const Fr x = load_fr(tab + 1);
         for (size_t i = end; i-- > start;) acc = acc * x + load_fr(p + i);
         acc = acc * omega_pow_tab(start, tab, levels);
@@ -239,6 +239,17 @@ __global__ __launch_bounds__(kBlock) void k_kate_finish(Fr *q, const Fr *s, cons
     store_fr(q + i, sk * omega_pow_tab(k, inv_tab, levels));
 }
 
+// Coefficients per lane of an evaluation: 16, or fewer (down to 4) while
+// the evaluations would leave the GPU short of lanes -- each lane's Horner
+// chain is a run of dependent products, so a small evaluation is bound by
+// its length, not by issue.
+size_t eval_chunk(size_t len, size_t sums)
+{
+    size_t c = 16;
+    while (c > 4 && sums * ceil_div(len, c) < kEvalFill) c /= 2;
+    return c;
+}
+
 }  // namespace
 
 void poly_eval_many(HostFr *out, const Fr *polys, size_t len, size_t count, const HostFr *points, size_t npts,
@@ -252,10 +263,11 @@ void poly_eval_many(HostFr *out, const Fr *polys, size_t len, size_t count, cons
     for (size_t p = 0; p < npts; ++p)
         MH_HIP(hipMemcpyAsync(all.get() + p * levels * kOmegaTab, tabs[p].get(), levels * kOmegaTab * 32,
                               hipMemcpyDeviceToDevice, st));
-    const size_t nblk = ceil_div(len, (size_t)kBlock * kChunk);
+    const size_t chunk = eval_chunk(len, count * npts);
+    const size_t nblk = ceil_div(len, (size_t)kBlock * chunk);
     DevBuf<Fr> partial(count * npts * nblk);
     hipLaunchKernelGGL(k_eval, dim3((unsigned)nblk, (unsigned)count, (unsigned)npts), dim3(kBlock), 0, st, polys, len,
-                       (const Fr *)all.get(), levels, (uint32_t)npts, partial.get());
+                       (uint32_t)chunk, (const Fr *)all.get(), levels, (uint32_t)npts, partial.get());
     MH_HIP(hipGetLastError());
     std::vector<HostFr> h(count * npts * nblk);
     MH_HIP(hipMemcpyAsync(h.data(), partial.get(), h.size() * 32, hipMemcpyDeviceToHost, st));
@@ -343,12 +355,14 @@ void poly_eval_queries(HostFr *out, const std::vector<const Fr *> &polys, size_t
     MH_HIP(hipMemcpyAsync(d_polys.get(), polys.data(), polys.size() * sizeof(Fr *), hipMemcpyHostToDevice, st));
     MH_HIP(hipMemcpyAsync(d_q.get(), qpoly, nq * 4, hipMemcpyHostToDevice, st));
     MH_HIP(hipMemcpyAsync(d_q.get() + nq, qpt, nq * 4, hipMemcpyHostToDevice, st));
-    const size_t nblk = ceil_div(len, (size_t)kBlock * kChunk);
+    const size_t chunk = eval_chunk(len, nq);
+    const size_t nblk = ceil_div(len, (size_t)kBlock * chunk);
     MH_CHECK(nq < 65536, (int)hipErrorInvalidValue, "too many evaluation queries");
     DevBuf<Fr> partial(nq * nblk);
     hipLaunchKernelGGL(k_eval_queries, dim3((unsigned)nblk, (unsigned)nq), dim3(kBlock), 0, st,
                        (const Fr *const *)d_polys.get(), (const uint32_t *)d_q.get(),
-                       (const uint32_t *)d_q.get() + nq, len, (const Fr *)all.get(), levels, partial.get());
+                       (const uint32_t *)d_q.get() + nq, len, (uint32_t)chunk, (const Fr *)all.get(), levels,
+                       partial.get());
     MH_HIP(hipGetLastError());
     std::vector<HostFr> h(nq * nblk);
     MH_HIP(hipMemcpyAsync(h.data(), partial.get(), h.size() * 32, hipMemcpyDeviceToHost, st));
