@@ -933,7 +933,12 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
     s.L = fill_group(s.nent_max, chunk_len);
     s.nch_max = ceil_div(s.nent_max, s.L) + s.nkeys;
     MH_CHECK(s.nch_max < (1ull << 32), (int)hipErrorInvalidValue, "MSM batch too large");
-    s.G = s.nch_max >= 4 * kFillLanes ? 16 : 4;
+    // partials per group in the bucket tree.  Sized for the worst case, the
+    // tree's levels are launched whether or not a bucket needs them (on
+    // uniform scalars the upper levels find no work), so few levels of
+    // wide groups beat many narrow ones: small MSMs run their groups on
+    // lane quads, and a proof service is bound by launches per proof.
+    s.G = 16;
     // fullest possible bucket: every digit of every scalar of the longest
     // MSM in one bucket (one bucket set spans all windows when precomputed)
     const uint64_t bucket_max = pc ? max_n * g.nwin : max_n;
