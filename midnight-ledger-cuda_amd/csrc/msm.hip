@@ -814,6 +814,9 @@ struct MsmEngine::State {
     std::vector<Level> levels;
     std::vector<MsmDesc> hd;
     std::vector<uint32_t> seg_off, seg_len, out_off;
+    // the layout last uploaded and the device buffers it went to
+    std::vector<uint32_t> up_seg_off, up_seg_len, up_out_off;
+    const void *up_dev[3] = {nullptr, nullptr, nullptr};
 };
 
 MsmEngine::MsmEngine() : buf_(new Buffers), st_(new State) {}
@@ -1038,9 +1041,19 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
     for (size_t i = 0; i < jobs.size(); ++i)
         s.hd[i] = MsmDesc{jobs[i].scalars, (uint32_t)jobs[i].pt_off, (uint32_t)jobs[i].n};
     MH_HIP(hipMemcpyAsync(b.descs.get(), s.hd.data(), s.hd.size() * sizeof(MsmDesc), hipMemcpyHostToDevice, st));
-    MH_HIP(hipMemcpyAsync(b.seg_off.get(), s.seg_off.data(), s.seg_off.size() * 4, hipMemcpyHostToDevice, st));
-    MH_HIP(hipMemcpyAsync(b.seg_len.get(), s.seg_len.data(), s.seg_len.size() * 4, hipMemcpyHostToDevice, st));
-    MH_HIP(hipMemcpyAsync(b.out_off.get(), s.out_off.data(), s.out_off.size() * 4, hipMemcpyHostToDevice, st));
+    // the plane-pass layout depends on the geometry only: the MSMs of one
+    // proof (same size, same engine) upload it once
+    const void *dev_now[3] = {b.seg_off.get(), b.seg_len.get(), b.out_off.get()};
+    if (s.seg_off != s.up_seg_off || s.seg_len != s.up_seg_len || s.out_off != s.up_out_off ||
+        std::memcmp(dev_now, s.up_dev, sizeof(dev_now)) != 0) {
+        MH_HIP(hipMemcpyAsync(b.seg_off.get(), s.seg_off.data(), s.seg_off.size() * 4, hipMemcpyHostToDevice, st));
+        MH_HIP(hipMemcpyAsync(b.seg_len.get(), s.seg_len.data(), s.seg_len.size() * 4, hipMemcpyHostToDevice, st));
+        MH_HIP(hipMemcpyAsync(b.out_off.get(), s.out_off.data(), s.out_off.size() * 4, hipMemcpyHostToDevice, st));
+        s.up_seg_off = s.seg_off;
+        s.up_seg_len = s.seg_len;
+        s.up_out_off = s.out_off;
+        std::memcpy(s.up_dev, dev_now, sizeof(dev_now));
+    }
     static_assert(sizeof(HJac) == 144, "HJac layout");
     if (b.h_pj_cap < s.nseg) {
         if (b.h_pj) MH_HIP(hipHostFree(b.h_pj));

@@ -256,13 +256,9 @@ void poly_eval_many(HostFr *out, const Fr *polys, size_t len, size_t count, cons
                     hipStream_t st)
 {
     if (len == 0 || count == 0 || npts == 0) return;
-    std::vector<DevBuf<Fr>> tabs(npts);
     uint32_t levels = 1;
-    for (size_t p = 0; p < npts; ++p) build_omega_table(tabs[p], levels, points[p], len, st);
-    DevBuf<Fr> all((size_t)npts * levels * kOmegaTab);
-    for (size_t p = 0; p < npts; ++p)
-        MH_HIP(hipMemcpyAsync(all.get() + p * levels * kOmegaTab, tabs[p].get(), levels * kOmegaTab * 32,
-                              hipMemcpyDeviceToDevice, st));
+    DevBuf<Fr> all;
+    build_omega_tables(all, levels, points, npts, len, st);
     const size_t chunk = eval_chunk(len, count * npts);
     const size_t nblk = ceil_div(len, (size_t)kBlock * chunk);
     DevBuf<Fr> partial(count * npts * nblk);
@@ -343,13 +339,9 @@ void poly_eval_queries(HostFr *out, const std::vector<const Fr *> &polys, size_t
         for (size_t q = 0; q < nq; ++q) out[q] = HostFr::zero();
         return;
     }
-    std::vector<DevBuf<Fr>> tabs(npts);
     uint32_t levels = 1;
-    for (size_t p = 0; p < npts; ++p) build_omega_table(tabs[p], levels, points[p], len, st);
-    DevBuf<Fr> all(std::max<size_t>((size_t)npts * levels * kOmegaTab, 1));
-    for (size_t p = 0; p < npts; ++p)
-        MH_HIP(hipMemcpyAsync(all.get() + p * levels * kOmegaTab, tabs[p].get(), levels * kOmegaTab * 32,
-                              hipMemcpyDeviceToDevice, st));
+    DevBuf<Fr> all;
+    build_omega_tables(all, levels, points, npts, len, st);
     DevBuf<const Fr *> d_polys(polys.size());
     DevBuf<uint32_t> d_q(2 * nq);
     MH_HIP(hipMemcpyAsync(d_polys.get(), polys.data(), polys.size() * sizeof(Fr *), hipMemcpyHostToDevice, st));

@@ -43,4 +43,30 @@ inline void build_omega_table(DevBuf<Fr> &tab, uint32_t &levels, const HostFr &o
     MH_HIP(hipStreamSynchronize(st));
 }
 
+// The tables of npts points back to back (point p at p * levels *
+// kOmegaTab), with one upload and one synchronisation for all of them.
+inline void build_omega_tables(DevBuf<Fr> &tab, uint32_t &levels, const HostFr *points, size_t npts, size_t len,
+                               hipStream_t st)
+{
+    uint32_t bits = 0;
+    while (bits < 64 && (1ull << bits) < len) ++bits;
+    levels = bits <= (uint32_t)kOmegaBits ? 1 : (bits + kOmegaBits - 1) / kOmegaBits;
+    const size_t per = (size_t)levels * kOmegaTab;
+    std::vector<HostFr> h(std::max<size_t>(npts * per, 1));
+    for (size_t p = 0; p < npts; ++p) {
+        HostFr step = points[p];
+        for (uint32_t l = 0; l < levels; ++l) {
+            HostFr acc = HostFr::one();
+            for (uint32_t j = 0; j < kOmegaTab; ++j) {
+                h[p * per + (size_t)l * kOmegaTab + j] = acc;
+                acc = acc * step;
+            }
+            step = acc;
+        }
+    }
+    tab.alloc(h.size());
+    MH_HIP(hipMemcpyAsync(tab.get(), h.data(), h.size() * 32, hipMemcpyHostToDevice, st));
+    MH_HIP(hipStreamSynchronize(st));
+}
+
 }  // namespace mh
