@@ -311,13 +311,14 @@ void poly_kate_divide(Fr *q, const Fr *p, size_t len, const HostFr &a, hipStream
         MH_HIP(hipStreamSynchronize(st));
         return;
     }
-    DevBuf<Fr> tab, inv_tab, w(len);
-    uint32_t levels = 1, levels2 = 1;
-    build_omega_table(tab, levels, a, len, st);
-    build_omega_table(inv_tab, levels2, a.inverse(), len, st);
+    // the power tables of a and a^-1, back to back (one upload)
+    DevBuf<Fr> tabs, w(len);
+    uint32_t levels = 1;
+    const HostFr pts[2] = {a, a.inverse()};
+    build_omega_tables(tabs, levels, pts, 2, len, st);
+    const Fr *tab = tabs.get(), *inv_tab = tabs.get() + (size_t)levels * kOmegaTab;
     const unsigned grid = (unsigned)ceil_div(len, kBlock);
-    hipLaunchKernelGGL(k_scale_powers, dim3(grid), dim3(kBlock), 0, st, w.get(), p, len, (const Fr *)tab.get(),
-                       levels);
+    hipLaunchKernelGGL(k_scale_powers, dim3(grid), dim3(kBlock), 0, st, w.get(), p, len, tab, levels);
     MH_HIP(hipGetLastError());
     const size_t tiles = ceil_div(len, kScanTile);
     DevBuf<Fr> totals(tiles);
@@ -326,7 +327,7 @@ void poly_kate_divide(Fr *q, const Fr *p, size_t len, const HostFr &a, hipStream
     hipLaunchKernelGGL(k_suffix_totals, dim3(1), dim3(1), 0, st, totals.get(), tiles);
     MH_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_kate_finish, dim3(grid), dim3(kBlock), 0, st, q, (const Fr *)w.get(),
-                       (const Fr *)totals.get(), len, (const Fr *)inv_tab.get(), levels2);
+                       (const Fr *)totals.get(), len, inv_tab, levels);
     MH_HIP(hipGetLastError());
     MH_HIP(hipStreamSynchronize(st));
 }
