@@ -150,6 +150,24 @@ __global__ __launch_bounds__(kBlock) void k_lincomb(Fr *out, const Fr *const *po
     store_fr(out + i, acc);
 }
 
+// the same with up to kArgPolys polynomials and scalars passed by value in
+// the kernel arguments (no upload, so no synchronisation on host arrays)
+constexpr uint32_t kArgPolys = 16;
+struct LincombArgs {
+    const Fr *polys[kArgPolys];
+    Fr scalars[kArgPolys];
+};
+
+__global__ __launch_bounds__(kBlock) void k_lincomb_args(Fr *out, LincombArgs a, uint32_t count, size_t len,
+                                                         Fr constant)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    Fr acc = i == 0 ? constant : Fr::zero();
+    for (uint32_t j = 0; j < count; ++j) acc = acc + a.scalars[j] * load_fr(a.polys[j] + i);
+    store_fr(out + i, acc);
+}
+
 // w[j] = p[j] * a^j
 __global__ __launch_bounds__(kBlock) void k_scale_powers(Fr *w, const Fr *p, size_t len, const Fr *tab,
                                                          uint32_t levels)
@@ -280,6 +298,19 @@ void poly_lincomb(Fr *out, const std::vector<const Fr *> &polys, const std::vect
 {
     MH_CHECK(polys.size() == scalars.size(), (int)hipErrorInvalidValue, "lincomb shape mismatch");
     if (len == 0) return;
+    if (polys.size() <= kArgPolys) {
+        LincombArgs a{};
+        for (size_t j = 0; j < polys.size(); ++j) {
+            a.polys[j] = polys[j];
+            std::memcpy(a.scalars[j].v, scalars[j].l, 32);
+        }
+        Fr c;
+        std::memcpy(c.v, constant.l, 32);
+        hipLaunchKernelGGL(k_lincomb_args, dim3((unsigned)ceil_div(len, kBlock)), dim3(kBlock), 0, st, out, a,
+                           (uint32_t)polys.size(), len, c);
+        MH_HIP(hipGetLastError());
+        return;
+    }
     DevBuf<const Fr *> d_polys(std::max<size_t>(polys.size(), 1));
     DevBuf<Fr> d_sc(std::max<size_t>(scalars.size(), 1));
     if (!polys.empty()) {
