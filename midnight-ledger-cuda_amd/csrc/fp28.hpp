@@ -676,4 +676,31 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_quad(const Xyzz28 &p, const Xyzz28 
     return r;
 }
 
+// xyzz28_add_affine_lz on a quad: its ten products in four steps
+// (u2 s2 | PP R^2 | PPP Q zz3 | R(Q-X3) y1PPP zzz3)
+__device__ __forceinline__ Xyzz28 xyzz28_add_affine_quad(const Xyzz28 &p, const Affine28 &q, uint32_t sub)
+{
+    if (q.is_inf()) return p;
+    if (p.is_inf()) return xyzz28_from_affine(q);
+    Fp28 t = fp28_mul<false>(fp28_sel4(sub, q.x, q.y, q.x, q.y), fp28_sel4(sub, p.zz, p.zzz, p.zz, p.zzz));
+    const Fp28 pp_ = fp28_sub_lz(fp28_quad_bcast<0>(t), p.x);
+    const Fp28 r_ = fp28_sub_lz(fp28_quad_bcast<1>(t), p.y);
+    if (fp28_is_zero_lz(pp_)) {
+        if (fp28_is_zero_lz(r_)) return xyzz28_dbl_affine(q);
+        return xyzz28_inf();
+    }
+    const Fp28 sq = fp28_sel4(sub, pp_, r_, pp_, r_);
+    t = fp28_mul<false>(sq, sq);
+    const Fp28 pp = fp28_quad_bcast<0>(t), r2 = fp28_quad_bcast<1>(t);
+    t = fp28_mul<false>(fp28_sel4(sub, pp_, p.x, p.zz, p.zz), pp);
+    const Fp28 ppp = fp28_quad_bcast<0>(t), qq = fp28_quad_bcast<1>(t);
+    Xyzz28 r;
+    r.zz = fp28_quad_bcast<2>(t);
+    r.x = fp28_sub_lz(fp28_sub_lz(r2, ppp), fp28_add_lz(qq, qq));
+    t = fp28_mul<false>(fp28_sel4(sub, r_, p.y, p.zzz, p.zzz), fp28_sel4(sub, fp28_sub_lz(qq, r.x), ppp, ppp, ppp));
+    r.y = fp28_sub_lz(fp28_quad_bcast<0>(t), fp28_quad_bcast<1>(t));
+    r.zzz = fp28_quad_bcast<2>(t);
+    return r;
+}
+
 }  // namespace mh

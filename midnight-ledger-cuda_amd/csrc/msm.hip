@@ -517,12 +517,15 @@ __device__ __forceinline__ Affine28 fetch_point28(const uint32_t *pts, uint32_t 
     return p;
 }
 
+// kLanes = 4: each chunk on a lane quad (small MSMs that cannot fill the GPU)
+template <int kLanes>
 __global__ void __launch_bounds__(kBlock) k_accumulate28(const uint32_t *pts, const uint32_t *ent, const uint32_t *cnt,
                                                          const uint32_t *off, const uint32_t *q, const uint32_t *co,
                                                          const uint32_t *cmap, const uint32_t *totals, uint32_t L,
                                                          uint32_t *part, uint32_t *bucket)
 {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = t / kLanes, sub = t % kLanes;
     if (j >= totals[1]) return;
     const uint32_t b = cmap[j];
     const uint32_t r = j - co[b];
@@ -534,8 +537,10 @@ __global__ void __launch_bounds__(kBlock) k_accumulate28(const uint32_t *pts, co
     for (uint32_t x = start; x < end; ++x) {
         Affine28 cur = nxt;
         if (x + 1 < end) nxt = fetch_point28(pts, ent[x + 1]);
-        acc = xyzz28_add_affine_lz(acc, cur);
+        if constexpr (kLanes == 4) acc = xyzz28_add_affine_quad(acc, cur, sub);
+        else acc = xyzz28_add_affine_lz(acc, cur);
     }
+    if (sub) return;
     // the reduction kernels take the loose radix-2^28 sums as they are
     if (q[b] == 1) store_xyzz28(bucket + (uint64_t)b * kXyzz28Words, acc);
     else store_xyzz28(part + (uint64_t)j * kXyzz28Words, acc);
@@ -1115,14 +1120,20 @@ void MsmEngine::enqueue_front(hipStream_t st, ProfileScope *prof)
                        nkeys);
     MH_HIP(hipGetLastError());
     if (prof) prof->begin("msm_accumulate", st);
-    if (s.precomputed)
-        hipLaunchKernelGGL(k_accumulate28, dim3(grid1(s.nch_max)), dim3(kBlock), 0, st, s.d_points, b.ent.get(),
-                           b.cnt.get(), b.off.get(), b.q.get(), b.co.get(), b.cmap.get(), b.totals.get(), s.L,
-                           b.part.get(), b.bucket.get());
-    else
+    if (s.precomputed) {
+        if (s.nch_max < quad_below)
+            hipLaunchKernelGGL(k_accumulate28<4>, dim3(grid1(4 * s.nch_max)), dim3(kBlock), 0, st, s.d_points,
+                               b.ent.get(), b.cnt.get(), b.off.get(), b.q.get(), b.co.get(), b.cmap.get(),
+                               b.totals.get(), s.L, b.part.get(), b.bucket.get());
+        else
+            hipLaunchKernelGGL(k_accumulate28<1>, dim3(grid1(s.nch_max)), dim3(kBlock), 0, st, s.d_points,
+                               b.ent.get(), b.cnt.get(), b.off.get(), b.q.get(), b.co.get(), b.cmap.get(),
+                               b.totals.get(), s.L, b.part.get(), b.bucket.get());
+    } else {
         hipLaunchKernelGGL(k_accumulate, dim3(grid1(s.nch_max)), dim3(kBlock), 0, st, s.d_points, b.ent.get(),
                            b.cnt.get(), b.off.get(), b.q.get(), b.co.get(), b.cmap.get(), b.totals.get(), s.L,
                            b.part.get(), b.bucket.get());
+    }
     MH_HIP(hipGetLastError());
     if (prof) prof->end(st);
 }
