@@ -569,9 +569,25 @@ RustError midnight_hip_plonk_vk(const void *pk, RawG1Projective *fixed_commitmen
 /* One proof: advice = advice_cols columns of 2^k Lagrange values (rows from
  * 2^k - blinding - 1 on are overwritten with blinding values), blinding
  * drawn from ChaCha20Rng::seed_from_u64(rng_seed).  *proof_len receives the
- * proof size; the call fails if proof_capacity is smaller. */
+ * proof size; a null proof or a proof_capacity below it is rejected before
+ * any work is done (the size is fixed by the key). */
 RustError midnight_hip_plonk_prove(void *pk, const RawFr *advice, size_t advice_len, uint64_t rng_seed,
                                    uint8_t *proof, size_t proof_capacity, size_t *proof_len);
+/* The length in bytes of every proof of this key. */
+RustError midnight_hip_plonk_proof_size(const void *pk, size_t *proof_len);
+/* Thread count of the reference prover whose draws proofs of this key
+ * reproduce (default 1): the vanishing argument's random polynomial takes one
+ * ChaCha20Rng seed per chunk of that many workers (vanishing/prover.rs:41-75,
+ * split as utils/arithmetic.rs:138-190 parallelize), which shifts every later
+ * draw of the prover's RNG. */
+RustError midnight_hip_plonk_set_rng_threads(void *pk, uint32_t threads);
+/* vanishing::Argument::sample_random (vanishing/prover.rs:41-75) once its
+ * seeds are drawn: a new device vector of 2^log_n Fr, chunk c of the
+ * parallelize split of 2^log_n rows over `threads` workers holding the first
+ * Fq::random values of ChaCha20Rng::from_seed(seeds + 32 c); seed_count must
+ * equal the chunk count, min(threads, 2^log_n). */
+RustError midnight_hip_fr_sample_random_poly(void **out, uint32_t log_n, size_t threads, const uint8_t *seeds,
+                                             size_t seed_count);
 
 /* Proof server: `slots` worker threads (lanes 1..slots, at most 8) prove
  * submitted requests concurrently on the GPU; a request is admitted when a
@@ -583,7 +599,9 @@ RustError midnight_hip_proof_server_create(void **out, uint32_t slots, uint64_t 
 /* Queues one proof (the advice is copied); *job_id identifies it. */
 RustError midnight_hip_proof_server_submit(void *server, void *pk, const RawFr *advice, size_t advice_len,
                                            uint64_t seed, uint64_t *job_id);
-/* Waits for a job: proof bytes, and its time queued and proving (ms). */
+/* Waits for a job: proof bytes, and its time queued and proving (ms).  A
+ * null or too small proof buffer is an error that sets *proof_len and keeps
+ * the finished job, so the caller can wait again with a large enough one. */
 RustError midnight_hip_proof_server_wait(void *server, uint64_t job_id, uint8_t *proof, size_t capacity,
                                          size_t *proof_len, double *queue_ms, double *prove_ms);
 /* Finishes queued and running proofs, then stops the server. */

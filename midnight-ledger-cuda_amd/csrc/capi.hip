@@ -576,6 +576,8 @@ extern "C" RustError midnight_sppark_bls12_381_fr_device_coeff_to_extended_batch
     dv->len = output_len * count;
     dv->values.alloc(dv->len);
     coeff_to_extended_device(dv->values.get(), d_in.get(), input_len, count, extended_log_n, first, second, st);
+    // the handle may be consumed on another lane: it is returned complete
+    MH_HIP(hipStreamSynchronize(st));
     *out = dv.release();
     return ok();
     MH_CATCH
@@ -601,6 +603,8 @@ extern "C" RustError midnight_sppark_bls12_381_fr_device_coeff_to_extended_from_
     dv->values.alloc(dv->len);
     coeff_to_extended_device(dv->values.get(), src->values.get(), input_len, count, extended_log_n, first, second,
                              st);
+    // the handle may be consumed on another lane: it is returned complete
+    MH_HIP(hipStreamSynchronize(st));
     *out = dv.release();
     return ok();
     MH_CATCH

@@ -504,6 +504,8 @@ extern "C" RustError midnight_sppark_bls12_381_fr_device_poly_lincomb(void **out
     std::vector<const Fr *> ptrs(poly_count);
     for (size_t j = 0; j < poly_count; ++j) ptrs[j] = polys->values.get() + j * poly_len;
     poly_lincomb(dv->values.get(), ptrs, host_frs(scalars, scalar_count), poly_len, HostFr::zero(), st);
+    // the handle may be consumed on another lane: it is returned complete
+    MH_HIP(hipStreamSynchronize(st));
     *out = dv.release();
     return ok();
     MH_CATCH
@@ -639,6 +641,8 @@ extern "C" RustError midnight_sppark_bls12_381_fr_multiopen_f_poly_device(
     }
     auto f = new_devvec(poly_len);
     poly_lincomb(f->values.get(), ptrs, host_frs(combination_scalars, scalar_count), poly_len, HostFr::zero(), st);
+    // the handle may be consumed on another lane: it is returned complete
+    MH_HIP(hipStreamSynchronize(st));
     *out = f.release();
     return ok();
     MH_CATCH

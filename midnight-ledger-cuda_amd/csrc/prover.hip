@@ -18,12 +18,13 @@
 // Every polynomial stays on the device; the host holds the transcript, the
 // RNG and the few field elements the protocol reads back (evaluations).
 //
-// Deliberate differences from the reference, which the reference cannot be
-// built here to compare against: the verifying-key representative hashes
-// a description string of our own instead of the Rust Debug output of the
-// pinned constraint system; the vanishing argument's random polynomial is
-// drawn as with a one-thread pool (its seeding depends on the thread count
-// in the reference, vanishing/prover.rs:41-75); commitments are computed
+// The verifying-key representative hashes the caller's description bytes
+// where the reference hashes the Debug text of the pinned domain and
+// constraint system (plonk/mod.rs:371-405): a caller that passes that text
+// gets the reference's representative.  The vanishing argument's random
+// polynomial is seeded per chunk of the reference prover's thread pool
+// (vanishing/prover.rs:41-75); the thread count is a key setting
+// (midnight_hip_plonk_set_rng_threads, default 1).  Commitments are computed
 // from coefficients with the monomial SRS (commit_lagrange with the Lagrange
 // SRS gives the same points).
 #include <algorithm>
@@ -210,6 +211,7 @@ struct PlonkKey {
     HostFr omega, omega_inv, ext_omega, repr;
     std::vector<HostFr> t_inv, delta_powers;
     void *numctx = nullptr;
+    uint32_t rng_threads = 1;   // thread count of the reference prover whose random draws are reproduced
     DevBuf<Fr> scratch_by_lane[kLaneCount];   // NTT scratch of proofs running on each lane
     DevBuf<Fr> &scratch() { return scratch_by_lane[current_lane()]; }
     ~PlonkKey()
@@ -404,6 +406,37 @@ size_t plonk_footprint(const PlonkKey &pk)
 
 size_t plonk_advice_len(const PlonkKey &pk) { return pk.advice_cols * pk.n; }
 
+// Length of every proof of this key: the commitments and evaluations that
+// plonk_prove writes, with the multi-open's point sets formed as it forms
+// them (queries at distinct rotations of x are distinct points).
+size_t plonk_proof_size(const PlonkKey &pk)
+{
+    const size_t A = pk.advice_cols, S = pk.sets;
+    const int32_t last = -(int32_t)(pk.bf + 1);
+    // (polynomial id, rotation) in the prover's query order
+    std::vector<std::pair<long, int32_t>> q;
+    for (auto &a : pk.advice_queries) q.emplace_back((long)a.first, a.second);
+    for (size_t s = 0; s < S; ++s) {
+        q.emplace_back((long)(A + s), 0);
+        q.emplace_back((long)(A + s), 1);
+    }
+    for (size_t s = S - 1; s-- > 0;) q.emplace_back((long)(A + s), last);
+    const long fixed0 = (long)(A + S), sigma0 = fixed0 + (long)pk.fixed_cols, hid = sigma0 + (long)pk.perm_cols;
+    for (auto &f : pk.fixed_queries) q.emplace_back(fixed0 + f.first, f.second);
+    for (size_t j = 0; j < pk.perm_cols; ++j) q.emplace_back(sigma0 + (long)j, 0);
+    q.emplace_back(hid, 0);
+    q.emplace_back(hid + 1, 0);
+    const int64_t n = (int64_t)pk.n;
+    std::map<long, std::set<int64_t>> rots;
+    for (auto &e : q) rots[e.first].insert(((int64_t)e.second % n + n) % n);
+    std::set<std::set<int64_t>> sets;
+    for (auto &kv : rots) sets.insert(kv.second);
+    const size_t points = A + S + 1 + pk.pieces + 2;
+    const size_t scalars = pk.advice_queries.size() + pk.fixed_queries.size() + 1 + pk.perm_cols + 3 * S - 1 +
+                           sets.size();
+    return 48 * points + 32 * scalars;
+}
+
 // ---------------------------------------------------------------------- prove
 
 namespace {
@@ -428,8 +461,12 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     for (auto &b : adv_blinds) b = rng.random_fr();
     std::vector<HostFr> perm_blinds(S * pk.bf);
     for (auto &b : perm_blinds) b = rng.random_fr();
-    uint8_t vseed[32];
-    rng.fill_bytes(vseed, 32);
+    // vanishing random polynomial: one ChaCha20Rng seed per chunk of the
+    // reference prover's thread pool (vanishing/prover.rs:47-66)
+    std::vector<size_t> voff, vlen;
+    parallelize_chunks(n, pk.rng_threads, voff, vlen);
+    std::vector<uint8_t> vseeds(32 * voff.size());
+    for (size_t c = 0; c < voff.size(); ++c) rng.fill_bytes(vseeds.data() + 32 * c, 32);
     std::vector<HostFr> limb_blinds(pk.pieces - 1);
     for (auto &b : limb_blinds) b = rng.random_fr();
 
@@ -437,9 +474,8 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     const size_t P_ADV = 0, P_Z = A, P_RAND = A + S, P_H = A + S + 1, NPOLY = A + S + 1 + pk.pieces;
     auto polys = new_vec(NPOLY * n);
     Fr *pd = polys->values.get();
-    // vanishing random polynomial: one ChaCha20 stream (as with a one-thread
-    // pool), drawn on the device while the stream is still empty
-    random_fr_device(pd + P_RAND * n, n, vseed, st);
+    // the random polynomial's chunks, drawn on the device while the stream is still empty
+    random_fr_streams_device(pd + P_RAND * n, voff.data(), vlen.data(), vseeds.data(), voff.size(), st);
 
     T.common_scalar(pk.repr);
 
@@ -728,10 +764,29 @@ extern "C" RustError midnight_hip_plonk_prove(void *pk, const RawFr *advice, siz
     if (advice_len != p->advice_cols * p->n)
         return make_error((int)hipErrorInvalidValue, "advice length does not match the key");
     MH_TRY
+    // the length is fixed by the key: a short buffer is rejected before any work
+    const size_t want = plonk_proof_size(*p);
+    *proof_len = want;
+    if (!proof || proof_capacity < want) return make_error((int)hipErrorInvalidValue, "proof buffer too small");
     const std::vector<uint8_t> bytes = plonk_prove(*p, advice, rng_seed);
-    *proof_len = bytes.size();
-    if (!proof || proof_capacity < bytes.size()) return make_error((int)hipErrorInvalidValue, "proof buffer too small");
+    MH_CHECK(bytes.size() == want, -1, "proof length differs from the key's proof size");
     std::memcpy(proof, bytes.data(), bytes.size());
+    return ok();
+    MH_CATCH
+}
+
+extern "C" RustError midnight_hip_plonk_set_rng_threads(void *pk, uint32_t threads)
+{
+    if (!pk || threads == 0) return make_error((int)hipErrorInvalidValue, "invalid prover thread count");
+    reinterpret_cast<PlonkKey *>(pk)->rng_threads = threads;
+    return ok();
+}
+
+extern "C" RustError midnight_hip_plonk_proof_size(const void *pk, size_t *proof_len)
+{
+    if (!pk || !proof_len) return make_error((int)hipErrorInvalidValue, "invalid proof size arguments");
+    MH_TRY
+    *proof_len = plonk_proof_size(*reinterpret_cast<const PlonkKey *>(pk));
     return ok();
     MH_CATCH
 }

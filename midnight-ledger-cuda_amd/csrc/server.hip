@@ -185,7 +185,9 @@ extern "C" RustError midnight_hip_proof_server_submit(void *server, void *pk, co
 }
 
 // Blocks until the job is finished; copies the proof and its queueing and
-// proving times, and forgets the job.
+// proving times, and forgets the job.  A null or too small proof buffer
+// returns an error with *proof_len set and keeps the finished job, so the
+// caller can wait on it again with a buffer of that size.
 extern "C" RustError midnight_hip_proof_server_wait(void *server, uint64_t job_id, uint8_t *proof, size_t capacity,
                                                     size_t *proof_len, double *queue_ms, double *prove_ms)
 {
@@ -198,13 +200,16 @@ extern "C" RustError midnight_hip_proof_server_wait(void *server, uint64_t job_i
         if (it == s->jobs.end()) return make_error((int)hipErrorInvalidValue, "unknown job");
         job = it->second;
         s->cv_done.wait(lk, [&]() { return job->state >= 2; });
+        if (queue_ms) *queue_ms = job->t_start - job->t_submit;
+        if (prove_ms) *prove_ms = job->t_end - job->t_start;
+        if (job->state == 2) {
+            *proof_len = job->proof.size();
+            if (!proof || capacity < job->proof.size())
+                return make_error((int)hipErrorInvalidValue, "proof buffer too small");
+        }
         s->jobs.erase(job_id);
     }
-    if (queue_ms) *queue_ms = job->t_start - job->t_submit;
-    if (prove_ms) *prove_ms = job->t_end - job->t_start;
     if (job->state == 3) return make_error(job->err_code, job->err.c_str());
-    *proof_len = job->proof.size();
-    if (!proof || capacity < job->proof.size()) return make_error((int)hipErrorInvalidValue, "proof buffer too small");
     std::memcpy(proof, job->proof.data(), job->proof.size());
     return ok();
 }

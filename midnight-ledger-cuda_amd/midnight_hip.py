@@ -201,6 +201,9 @@ def _declare(L):
         "midnight_hip_plonk_vk": (E, [vp, u64p, u64p, u64p]),
         "midnight_hip_plonk_prove": (E, [vp, u64p, c_size, ctypes.c_uint64, ctypes.c_char_p, c_size,
                                          ctypes.POINTER(c_size)]),
+        "midnight_hip_plonk_proof_size": (E, [vp, ctypes.POINTER(c_size)]),
+        "midnight_hip_plonk_set_rng_threads": (E, [vp, u32]),
+        "midnight_hip_fr_sample_random_poly": (E, [vpp, u32, c_size, ctypes.c_char_p, c_size]),
         "midnight_hip_proof_server_create": (E, [vpp, u32, ctypes.c_uint64]),
         "midnight_hip_proof_server_submit": (E, [vp, vp, u64p, c_size, ctypes.c_uint64,
                                                  ctypes.POINTER(ctypes.c_uint64)]),
@@ -1025,6 +1028,15 @@ def fr_multiopen_pi_device(q_vec, f_vec, poly_len, scalars, point):
     return DeviceVec(h.value)
 
 
+def fr_sample_random_poly(log_n, threads, seeds):
+    """device vector of 2^log_n random Fr, one ChaCha20Rng seed (32 bytes of
+    `seeds`) per parallelize chunk of `threads` workers"""
+    h = ctypes.c_void_p()
+    check(lib().midnight_hip_fr_sample_random_poly(ctypes.byref(h), log_n, threads, bytes(seeds),
+                                                   len(seeds) // 32))
+    return DeviceVec(h.value)
+
+
 # ---------------------------------------------------------------- proof pipeline
 
 class PlonkKey:
@@ -1058,9 +1070,19 @@ class PlonkKey:
         check(lib().midnight_hip_plonk_vk(self.handle, P(fc) if self.nfixed else None, P(pc), P(rp)))
         return fc, pc, rp
 
-    def prove(self, advice, seed, capacity=1 << 16):
+    def set_rng_threads(self, threads):
+        check(lib().midnight_hip_plonk_set_rng_threads(self.handle, threads))
+
+    def proof_size(self):
+        ln = c_size()
+        check(lib().midnight_hip_plonk_proof_size(self.handle, ctypes.byref(ln)))
+        return ln.value
+
+    def prove(self, advice, seed, capacity=None):
         a = _fr_array(advice)
-        buf = ctypes.create_string_buffer(capacity)
+        if capacity is None:
+            capacity = self.proof_size()
+        buf = ctypes.create_string_buffer(max(capacity, 1))
         ln = c_size()
         check(lib().midnight_hip_plonk_prove(self.handle, P(a), a.shape[0], seed, buf, capacity, ctypes.byref(ln)))
         return buf.raw[:ln.value]
@@ -1093,7 +1115,7 @@ class ProofServer:
         return jid.value
 
     def wait(self, job, capacity=1 << 16):
-        buf = ctypes.create_string_buffer(capacity)
+        buf = ctypes.create_string_buffer(max(capacity, 1))
         ln = c_size()
         qms, pms = ctypes.c_double(), ctypes.c_double()
         check(lib().midnight_hip_proof_server_wait(self.handle, job, buf, capacity, ctypes.byref(ln),

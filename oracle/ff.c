@@ -300,3 +300,13 @@ void orc_fr_inner_product(orc_fr *r, const orc_fr *a, const orc_fr *b, size_t n)
     }
     *r = acc;
 }
+
+/* out[i] = a[i] * b[i] + c[i], element by element (test data builders) */
+void orc_fr_mul_add_many(orc_fr *out, const orc_fr *a, const orc_fr *b, const orc_fr *c, size_t n)
+{
+    for (size_t i = 0; i < n; ++i) {
+        orc_fr t;
+        orc_fr_mul(&t, &a[i], &b[i]);
+        orc_fr_add(&out[i], &t, &c[i]);
+    }
+}

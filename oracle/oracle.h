@@ -50,6 +50,7 @@ int  orc_fr_is_zero(const orc_fr *a);
 /* Fq::from_uniform_bytes (fq.rs:704-726): a0*R2 + a1*R3 */
 void orc_fr_from_uniform_bytes(orc_fr *r, const uint8_t bytes[64]);
 void orc_fr_inner_product(orc_fr *r, const orc_fr *a, const orc_fr *b, size_t n);
+void orc_fr_mul_add_many(orc_fr *out, const orc_fr *a, const orc_fr *b, const orc_fr *c, size_t n);
 /* named constants, Montgomery form (fq.rs:60-135, 476-505, 693-702) */
 void orc_fr_root_of_unity(orc_fr *r);      /* 2^32-th root */
 void orc_fr_zeta(orc_fr *r);               /* cube root of unity ZETA */
@@ -240,6 +241,8 @@ typedef struct {
 void orc_rng_seed_from_u64(orc_rng *r, uint64_t seed);
 void orc_rng_from_seed(orc_rng *r, const uint8_t seed[32]);
 uint32_t orc_rng_next_u32(orc_rng *r);
+uint64_t orc_pcg32_advance(uint64_t state, uint64_t inc);
+uint32_t orc_pcg32_output(uint64_t state);
 uint64_t orc_rng_next_u64(orc_rng *r);
 /* Fq::random (fq.rs:373-387): rejection sampling of 255-bit values */
 void orc_fr_random(orc_fr *r, orc_rng *rng);

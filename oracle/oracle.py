@@ -63,6 +63,9 @@ def _declare(L):
         "orc_fr_from_canonical": (ctypes.c_int, [u64p, u64p]),
         "orc_fr_from_uniform_bytes": (V, [u64p, u8p]),
         "orc_fr_inner_product": (V, [u64p, u64p, u64p, ctypes.c_size_t]),
+        "orc_fr_mul_add_many": (V, [u64p, u64p, u64p, u64p, ctypes.c_size_t]),
+        "orc_pcg32_advance": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64]),
+        "orc_pcg32_output": (ctypes.c_uint32, [ctypes.c_uint64]),
         "orc_fr_root_of_unity": (V, [u64p]), "orc_fr_zeta": (V, [u64p]), "orc_fr_delta": (V, [u64p]),
         "orc_fp_add": (V, [u64p, u64p, u64p]), "orc_fp_sub": (V, [u64p, u64p, u64p]),
         "orc_fp_mul": (V, [u64p, u64p, u64p]), "orc_fp_inv": (V, [u64p, u64p]),
@@ -128,6 +131,12 @@ def _declare(L):
         f.argtypes = args
 
 
+def _threads(threads):
+    """worker threads of a parallel oracle call: the caller's count, else
+    the host's cores up to 16 (a GPU host's share)"""
+    return threads or min(16, os.cpu_count() or 1)
+
+
 def P(a):
     """uint64 pointer to a C-contiguous numpy array"""
     assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
@@ -176,6 +185,14 @@ def fr_op(name, *args):
 def fp_op(name, *args):
     out = np.zeros(6, dtype=np.uint64)
     getattr(lib(), "orc_fp_" + name)(P(out), *[P(np.ascontiguousarray(a)) for a in args])
+    return out
+
+
+def mul_add(a, b, c):
+    """a * b + c element by element (Montgomery arrays of one length)"""
+    a, b, c = (np.ascontiguousarray(x, dtype=np.uint64) for x in (a, b, c))
+    out = fr(a.shape[0])
+    lib().orc_fr_mul_add_many(P(out), P(a), P(b), P(c), a.shape[0])
     return out
 
 
@@ -242,20 +259,20 @@ def msm(points, scalars, threads=None, naive=False):
     if naive:
         lib().orc_msm_naive(P(out), P(points), P(scalars), n)
     else:
-        lib().orc_msm_pippenger(P(out), P(points), P(scalars), n, threads or os.cpu_count())
+        lib().orc_msm_pippenger(P(out), P(points), P(scalars), n, _threads(threads))
     return out
 
 
 def fixed_base(ts, threads=None):
     ts = np.ascontiguousarray(ts, dtype=np.uint64)
     out = g1a(ts.shape[0])
-    lib().orc_g1_fixed_base_batch(P(out), P(ts), ts.shape[0], threads or os.cpu_count())
+    lib().orc_g1_fixed_base_batch(P(out), P(ts), ts.shape[0], _threads(threads))
     return out
 
 
 def srs(tau, n, threads=None):
     out = g1a(n)
-    lib().orc_srs_powers(P(out), P(np.ascontiguousarray(tau)), n, threads or os.cpu_count())
+    lib().orc_srs_powers(P(out), P(np.ascontiguousarray(tau)), n, _threads(threads))
     return out
 
 
@@ -267,19 +284,19 @@ def omega(log_n):
 
 def fft(a, log_n, threads=None):
     a = np.array(a, dtype=np.uint64, order="C", copy=True)
-    lib().orc_fft(P(a), log_n, threads or os.cpu_count())
+    lib().orc_fft(P(a), log_n, _threads(threads))
     return a
 
 
 def ifft(a, log_n, threads=None):
     a = np.array(a, dtype=np.uint64, order="C", copy=True)
-    lib().orc_ifft(P(a), log_n, threads or os.cpu_count())
+    lib().orc_ifft(P(a), log_n, _threads(threads))
     return a
 
 
 def best_fft(a, w, log_n, threads=None):
     a = np.array(a, dtype=np.uint64, order="C", copy=True)
-    lib().orc_best_fft(P(a), P(np.ascontiguousarray(w)), log_n, threads or os.cpu_count())
+    lib().orc_best_fft(P(a), P(np.ascontiguousarray(w)), log_n, _threads(threads))
     return a
 
 
@@ -317,13 +334,13 @@ def coeff_to_extended(coeffs, d, threads=None):
     N = 1 << d.extended_k
     assert coeffs.shape[0] == n
     out = fr(N)
-    lib().orc_coeff_to_extended(P(out), P(np.ascontiguousarray(coeffs)), ctypes.byref(d), threads or os.cpu_count())
+    lib().orc_coeff_to_extended(P(out), P(np.ascontiguousarray(coeffs)), ctypes.byref(d), _threads(threads))
     return out
 
 
 def extended_to_coeff(values, d, threads=None):
     a = np.array(values, dtype=np.uint64, order="C", copy=True)
-    lib().orc_extended_to_coeff(P(a), ctypes.byref(d), threads or os.cpu_count())
+    lib().orc_extended_to_coeff(P(a), ctypes.byref(d), _threads(threads))
     return a
 
 
@@ -497,7 +514,7 @@ def numerator(n, *, beta, gamma, theta, trash_challenge, y, omega, g_coset, rot_
             keep.append(arr)
             setattr(s, name, ctypes.addressof(arr))
     out = np.zeros((n, 4), dtype=np.uint64)
-    lib().orc_eval_numerator(P(out), ctypes.addressof(s), threads or os.cpu_count())
+    lib().orc_eval_numerator(P(out), ctypes.addressof(s), _threads(threads))
     return out
 
 
@@ -528,7 +545,7 @@ def graph_rows(n, graph, *, beta=None, gamma=None, theta=None, trash_challenge=N
     prev = None if previous is None else np.ascontiguousarray(previous, dtype=np.uint64).reshape(-1, 4)
     out = np.zeros((n, 4), dtype=np.uint64)
     lib().orc_graph_eval_rows(P(out), ctypes.addressof(g), ctypes.addressof(s),
-                              None if prev is None else P(prev), threads or os.cpu_count())
+                              None if prev is None else P(prev), _threads(threads))
     return out
 
 
@@ -546,7 +563,7 @@ def permutation_products(columns, sigmas, n, chunk_len, beta, gamma, omega, delt
                                    P(np.ascontiguousarray(beta, dtype=np.uint64)),
                                    P(np.ascontiguousarray(gamma, dtype=np.uint64)),
                                    P(np.ascontiguousarray(omega, dtype=np.uint64)), P(dp), P(bl), blinding,
-                                   threads or os.cpu_count())
+                                   _threads(threads))
     return out
 
 

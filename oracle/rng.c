@@ -12,9 +12,21 @@
  *   - ChaCha20: DJB ChaCha with 20 rounds, 64-bit block counter in words
  *     12-13, stream id 0 in words 14-15, refilled four blocks at a time;
  *   - next_u64: two consecutive u32 words, low word first.
- * Pinned by: the RFC 7539 section 2.3.2 block-function test vector (the block
- * function); the rand_chacha output stream itself has no vector in the
- * reference, so the stream layout is "parity unpinned".
+ * Pinned by:
+ *   - the RFC 7539 section 2.3.2 block-function test vector (block function,
+ *     key/counter/nonce word layout);
+ *   - the RFC 7539 appendix A.1 keystream vectors #1 and #2 (all-zero key and
+ *     nonce, block counters 0 and 1) read through the stream: from_seed of a
+ *     zero seed yields exactly those 32 words, next_u64 pairs them low word
+ *     first, and a u64 straddling the 64-word refill takes word 63 as its low
+ *     half and the next refill's word 0 as its high half;
+ *   - O'Neill's pcg32 demo outputs (pcg32_srandom_r(42, 54)) for the LCG step
+ *     and the XSH-RR output permutation seed_from_u64 is built from.
+ * Still restated only from rand_core 0.6.4's published source, with no
+ * vector anywhere to check them against: seed_from_u64's increment
+ * 11634580027462260723 and its advance-then-output order (parity unpinned
+ * for seed_from_u64's exact seed bytes; every draw from a given 32-byte seed
+ * is pinned by the vectors above).
  */
 #include "oracle.h"
 #include <string.h>
@@ -68,15 +80,28 @@ void orc_rng_from_seed(orc_rng *r, const uint8_t seed[32])
     r->idx = 64;
 }
 
+/* PCG32's 64-bit LCG step with increment inc (the multiplier of pcg32) */
+uint64_t orc_pcg32_advance(uint64_t state, uint64_t inc)
+{
+    return state * 6364136223846793005ULL + inc;
+}
+
+/* PCG32's XSH-RR output permutation of a 64-bit state */
+uint32_t orc_pcg32_output(uint64_t state)
+{
+    uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
+    uint32_t rot = (uint32_t)(state >> 59);
+    return (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+}
+
 void orc_rng_seed_from_u64(orc_rng *r, uint64_t state)
 {
-    const uint64_t MUL = 6364136223846793005ULL, INC = 11634580027462260723ULL;
+    const uint64_t INC = 11634580027462260723ULL;
     uint8_t seed[32];
     for (int i = 0; i < 8; ++i) {
-        state = state * MUL + INC;
-        uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
-        uint32_t rot = (uint32_t)(state >> 59);
-        uint32_t x = (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+        /* rand_core advances the state before each output */
+        state = orc_pcg32_advance(state, INC);
+        uint32_t x = orc_pcg32_output(state);
         seed[4 * i] = (uint8_t)x;
         seed[4 * i + 1] = (uint8_t)(x >> 8);
         seed[4 * i + 2] = (uint8_t)(x >> 16);

@@ -122,6 +122,7 @@ class CpuKey:
         self.tau = tau_int
         n, k = self.n, self.k
         self.srs = O.srs(to_fr(tau_int), n, threads)
+        self.nsets = c.nsets
         self.dom, self.t_evals = O.domain(S.DEGREE, k)
         self.N = 1 << self.dom.extended_k
         self.bf, self.u = S.BLINDING, c.u
@@ -149,6 +150,22 @@ class CpuKey:
         self.repr = int.from_bytes(hv.digest(), "little") % R
 
 
+class VerifierKey:
+    """What the designated verifier reads of a key, without the prover's
+    polynomials: sizes, domain constants and the SRS trapdoor (for the GPU
+    keys of circuits too large for CpuKey)."""
+
+    def __init__(self, k, tau_int):
+        self.k, self.n = k, 1 << k
+        self.tau = tau_int
+        self.bf = S.BLINDING
+        self.u = self.n - self.bf - 1
+        self.pieces = S.DEGREE - 1
+        self.omega = S.omega_int(k)
+        self.delta = S.fr_const("delta")
+        self.nsets = (len(S.PERM_COLUMNS) + S.CHUNK - 1) // S.CHUNK
+
+
 # ------------------------------------------------------------------ prover
 
 def _rot(key, x, rot):
@@ -169,8 +186,41 @@ def _eval(p, x_int):
     return to_int(O.poly_eval(p, to_fr(x_int)))
 
 
-def cpu_prove(key, advice_cols, seed, threads=None):
-    """advice_cols: list of 5 lists of n ints (Lagrange values)."""
+def parallelize_chunks(n, threads):
+    """(offset, length) of the chunks utils/arithmetic.rs:138-190
+    parallelize cuts n rows into for `threads` workers: n % threads chunks of
+    n // threads + 1 rows, then chunks of n // threads rows (none when that
+    is 0), in row order."""
+    base, cutoff = divmod(n, threads)
+    out = [(c * (base + 1), base + 1) for c in range(cutoff)]
+    if base:
+        out += [(cutoff * (base + 1) + c * base, base) for c in range(threads - cutoff)]
+    return out
+
+
+def sample_random_seed_offsets(n, threads):
+    """The offsets vanishing/prover.rs:50-66 pairs with its seeds, in the
+    order the seeds are drawn: (0..).step_by(chunk + 1).take(n % threads)
+    chained with (n % threads) * (chunk + 1) stepping by chunk when chunk != 0,
+    at most `threads` of them (zip stops drawing seeds with the offsets)."""
+    chunk = n // threads
+    first = [i * (chunk + 1) for i in range(n % threads)]
+    rest = [(n % threads) * (chunk + 1) + i * chunk for i in range(threads)] if chunk != 0 else []
+    return (first + rest)[:threads]
+
+
+def sample_random(rng, n, rng_threads):
+    """vanishing::Argument::sample_random: one 32-byte seed per chunk drawn
+    from the prover's RNG, each chunk filled from its own ChaCha20Rng."""
+    chunks = parallelize_chunks(n, rng_threads)
+    seeds = [rng.fill_bytes(32) for _ in sample_random_seed_offsets(n, rng_threads)]
+    assert len(seeds) == len(chunks)
+    return np.concatenate([O.ChaCha(seed_bytes=sd).random_fr(ln) for sd, (_, ln) in zip(seeds, chunks)])
+
+
+def cpu_prove(key, advice_cols, seed, threads=None, rng_threads=1):
+    """advice_cols: list of 5 lists of n ints (Lagrange values); rng_threads:
+    the thread count of the reference prover whose random draws to follow."""
     c, n, k, A = key.c, key.n, key.k, S.ADVICE
     Sn = c.nsets
     rng = O.ChaCha(seed=seed)
@@ -178,9 +228,8 @@ def cpu_prove(key, advice_cols, seed, threads=None):
     for col in range(A):
         adv[col * n + key.u:(col + 1) * n] = rng.random_fr(n - key.u)
     perm_blinds = rng.random_fr(Sn * key.bf)
-    vseed = rng.fill_bytes(32)
+    rand_poly = sample_random(rng, n, rng_threads)
     limb_blinds = rng.random_fr(key.pieces - 1)
-    rand_poly = O.ChaCha(seed_bytes=vseed).random_fr(n)
 
     T = Transcript()
     T.common_scalar(key.repr)
@@ -319,7 +368,7 @@ def verify(vk, proof, key):
     """vk: (fixed commitments, permutation commitments, transcript repr int).
     Returns True when the proof verifies."""
     fixed_com, sigma_com, repr_int = vk
-    n, A, Sn, bf = key.n, S.ADVICE, key.c.nsets, key.bf
+    n, A, Sn, bf = key.n, S.ADVICE, key.nsets, key.bf
     try:
         T = Transcript(proof)
         T.common_scalar(repr_int)

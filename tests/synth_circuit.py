@@ -58,6 +58,11 @@ def gate_expression():
     return ("prod", q, ("sum", ("sum", ("prod", a, b), c), ("neg", d)))
 
 
+def gate_graph():
+    """the compiled custom-gate graph of the family"""
+    return G.RawGraphBuffers(G.custom_gates_graph([gate_expression()]))
+
+
 class Circuit:
     def __init__(self, k, seed=1):
         rnd = random.Random(seed)
@@ -176,7 +181,7 @@ class Circuit:
     # ---- numerator inputs -------------------------------------------------
 
     def custom_graph(self):
-        return G.RawGraphBuffers(G.custom_gates_graph([gate_expression()]))
+        return gate_graph()
 
     def base_inputs(self):
         """evaluate_numerator inputs on the base domain (LagrangeCoeff):
@@ -215,6 +220,39 @@ class Circuit:
             l0=ext(self.l0), l_last=ext(self.l_last), l_active=ext(self.l_active),
             permutation_products=flat(self.z), permutation_pk_cosets=flat(self.sigma))
         return base
+
+
+class FastCircuit:
+    """The same circuit family at sizes where Python integers are too slow
+    (k up to 20): Montgomery numpy columns from the oracle's seeded RNG, a
+    satisfying witness d = a*b + c on the usable rows, and copy cycles of
+    two that make a quarter of the usable rows of e equal to rows of a.
+    Provides what the GPU key and prover take (fixed, advice, mapping)."""
+
+    def __init__(self, k, seed=1):
+        self.k = k
+        n = self.n = 1 << k
+        u = self.u = n - BLINDING - 1
+        vals = O.random_fr(7 * n, seed)
+        a, b, c, d, e, s1, s2 = (vals[i * n:(i + 1) * n].copy() for i in range(7))
+        d[:u] = O.mul_add(a[:u], b[:u], c[:u])
+        one = mont([1])[0]
+        q = np.zeros((n, 4), dtype=np.uint64)
+        q[:u] = one
+        s1[u:] = 0
+        s2[u:] = 0
+        m = np.zeros((len(PERM_COLUMNS), n, 2), dtype=np.uint32)
+        m[:, :, 0] = np.arange(len(PERM_COLUMNS), dtype=np.uint32)[:, None]
+        m[:, :, 1] = np.arange(n, dtype=np.uint32)[None, :]
+        rows = np.random.default_rng(seed).permutation(u)
+        pairs = u // 4
+        ei, aj = rows[:pairs], rows[pairs:2 * pairs]
+        e[ei] = a[aj]
+        m[4, ei, 0], m[4, ei, 1] = 0, aj
+        m[0, aj, 0], m[0, aj, 1] = 4, ei
+        self.fixed = np.concatenate([q, s1, s2])
+        self.advice = np.concatenate([a, b, c, d, e])
+        self.mapping = m.reshape(-1)
 
 
 def random_inputs(n, seed, *, rot_scale=4, sets=3, graph=None):

@@ -24,7 +24,7 @@ def gpu_key(H, O, c, tau_int, extra_srs=0):
     return pk, srs
 
 
-@pytest.mark.parametrize("k,seed", [(7, 42), (10, 7)])
+@pytest.mark.parametrize("k,seed", [(7, 42), (10, 7), (12, 11), (14, 3)])
 def test_gpu_proof_matches_cpu(H, O, k, seed):
     c = S.Circuit(k, seed=3 + k)
     tau = 987654321 + k
@@ -99,3 +99,43 @@ def test_server_results_same_alone_and_in_mix(H, O):
     for pk, srs in keys:
         pk.free()
         srs.free()
+
+
+@pytest.mark.parametrize("rng_threads", [1, 5, 16])
+def test_gpu_proof_thread_seeded_random_poly(H, O, rng_threads):
+    """The vanishing argument's random polynomial drawn per chunk of an
+    rng_threads pool (vanishing/prover.rs:41-75): GPU and CPU prover give the
+    same bytes, accepted by the verifier."""
+    c = S.Circuit(8, seed=21)
+    tau = 24681357
+    key = PR.CpuKey(c, tau)
+    pk, srs = gpu_key(H, O, c, tau)
+    pk.set_rng_threads(rng_threads)
+    gpu = pk.prove(S.mont([v for col in c.cols[:5] for v in col]), 5)
+    assert gpu == PR.cpu_prove(key, c.cols[:5], 5, rng_threads=rng_threads)
+    assert PR.verify((key.fixed_com, key.sigma_com, key.repr), gpu, key)
+    with pytest.raises(H.MidnightHipError):
+        pk.set_rng_threads(0)
+    pk.free()
+    srs.free()
+
+
+@pytest.mark.parametrize("log_n,threads", [(3, 1), (3, 8), (3, 11), (10, 5), (12, 16), (16, 7), (20, 1), (20, 48)])
+def test_sample_random_poly_matches_oracle(H, O, log_n, threads):
+    """Device ChaCha20Rng draws: chunk c of the parallelize split equals the
+    oracle's Fq::random stream of seed c, across candidate blocks, refills
+    and chunk boundaries."""
+    n = 1 << log_n
+    chunks = PR.parallelize_chunks(n, threads)
+    rng = O.ChaCha(seed=1000 + log_n + threads)
+    seeds = b"".join(rng.fill_bytes(32) for _ in chunks)
+    got = H.fr_sample_random_poly(log_n, threads, seeds)
+    vals = got.download()
+    got.free()
+    for ci, (o, ln) in enumerate(chunks):
+        if n > 4096 and ci not in (0, 1, len(chunks) - 1):
+            continue
+        want = O.ChaCha(seed_bytes=seeds[32 * ci:32 * ci + 32]).random_fr(ln)
+        assert np.array_equal(vals[o:o + ln], want), (ci, o, ln)
+    with pytest.raises(H.MidnightHipError):
+        H.fr_sample_random_poly(log_n, threads, seeds[:-32])      # one seed short

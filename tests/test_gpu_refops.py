@@ -95,6 +95,61 @@ def test_poly_lincomb_and_eval(H, O):
     dv.free()
 
 
+@pytest.mark.parametrize("m", [16, 17, 40])
+def test_poly_lincomb_argument_and_table_paths(H, O, m):
+    """Up to 16 polynomials travel in the kernel arguments, more through an
+    uploaded pointer / scalar table: both sides of the boundary."""
+    n = 257
+    polys = O.random_fr(n * m, 850 + m)
+    sc = O.random_fr(m, 900 + m)
+    want = _lincomb(O, polys, sc, n)
+    assert np.array_equal(H.fr_poly_lincomb(polys, n, sc), want)
+    dv = H.DeviceVec.upload(polys)
+    lc = dv.poly_lincomb(n, m, sc)
+    assert np.array_equal(lc.download(), want)
+    lc.free()
+    dv.free()
+
+
+def test_device_vectors_cross_lanes(H, O):
+    """A device vector returned on one lane is complete: consumed at once on
+    another lane (the reference synchronises before releasing a handle,
+    bls12_381_g1_msm.cu:3269) it holds the same values as on its own lane."""
+    n, m = 1 << 17, 12
+    polys = O.random_fr(n * m, 870)
+    sc = O.random_fr(m, 871)
+    try:
+        H.set_lane(0)
+        dv0 = H.DeviceVec.upload(polys)
+        want = dv0.poly_lincomb(n, m, sc).download()
+        rows = [0, 1, n // 2, n - 1]
+        p = polys.reshape(m, n, 4)
+        for i in rows:
+            assert np.array_equal(want[i], O.inner_product(np.ascontiguousarray(p[:, i]), sc)), i
+        zeta = O.fr_from_int(1)
+        want_ext = H.device_coeff_to_extended_from_device_batch(dv0, n, m, 19, zeta, zeta).download()
+        q_dv0 = H.DeviceVec.upload(polys[:3 * n])
+        pts = O.random_fr(3, 872)
+        want_f = H.fr_multiopen_f_poly_device(q_dv0, n, [0, 1, 2, 3], pts, sc[:3]).download()
+        for rep in range(2):
+            H.set_lane(1)
+            dv1 = H.DeviceVec.upload(polys)
+            lc = dv1.poly_lincomb(n, m, sc)
+            ext = H.device_coeff_to_extended_from_device_batch(dv1, n, m, 19, zeta, zeta)
+            q_dv1 = H.DeviceVec.upload(polys[:3 * n])
+            f = H.fr_multiopen_f_poly_device(q_dv1, n, [0, 1, 2, 3], pts, sc[:3])
+            H.set_lane(0)
+            assert np.array_equal(lc.download(), want), rep
+            assert np.array_equal(ext.download(), want_ext), rep
+            assert np.array_equal(f.download(), want_f), rep
+            for x in (lc, ext, f, q_dv1, dv1):
+                x.free()
+        dv0.free()
+        q_dv0.free()
+    finally:
+        H.set_lane(0)
+
+
 def _multiopen_case(O, n, seed):
     polys = O.random_fr(n * 6, seed)
     q_offsets = [0, 3, 4, 7]

@@ -6,7 +6,9 @@
 - ZCash compressed round trip, and the reference SRS data file
   (transient-crypto/static/bls_midnight_2p14 re-encoded in
   tests/golden/srs_2p14.g1c): g[0] is the generator, every point on-curve
-- ChaCha20 block function: RFC 7539 section 2.3.2 test vector
+- ChaCha20 block function: RFC 7539 section 2.3.2 test vector; the
+  ChaCha20Rng stream: RFC 7539 appendix A.1 keystream vectors #1, #2; the
+  PCG32 step and output of seed_from_u64: the pcg32 demo outputs
 """
 import os
 import random
@@ -130,3 +132,85 @@ def test_chacha20_rfc7539_block(O):
     want = [0xe4e7f110, 0x15593bd1, 0x1fdd0f50, 0xc47120a3, 0xc7f4d1c7, 0x0368c033, 0x9aaa2204, 0x4e6cd4c3,
             0x466482d2, 0x09aa9f07, 0x05d7c214, 0xa2028bd9, 0xd19c12b5, 0xb94e16de, 0xe883d0cb, 0x4e3c50a2]
     assert list(out) == want
+
+
+# RFC 7539 appendix A.1, ChaCha20 block function test vectors #1 and #2:
+# all-zero key and nonce, block counters 0 and 1 (serialized keystream)
+RFC7539_A1_TV1 = bytes.fromhex(
+    "76b8e0ada0f13d90405d6ae55386bd28bdd219b8a08ded1aa836efcc8b770dc7"
+    "da41597c5157488d7724e03fb8d84a376a43b8f41518a11cc387b669b2ee6586")
+RFC7539_A1_TV2 = bytes.fromhex(
+    "9f07e7be5551387a98ba977c732d080dcb0f29a048e3656912c6533e32ee7aed"
+    "29b721769ce64e43d57133b074d839d531ed1f28510afb45ace10a1f4b794d6f")
+
+
+def _zero_seed_rng(O):
+    import ctypes
+    r = O.Rng()
+    O.lib().orc_rng_from_seed(ctypes.byref(r), (ctypes.c_uint8 * 32)())
+    return r
+
+
+def test_chacha20_rng_stream_rfc7539_a1(O):
+    """ChaCha20Rng::from_seed([0; 32]): its first 32 words are the RFC 7539
+    A.1 keystream of blocks 0 and 1, little-endian; next_u64 pairs them low
+    word first; fill_bytes serializes the words."""
+    import ctypes
+    ks = RFC7539_A1_TV1 + RFC7539_A1_TV2
+    words = [int.from_bytes(ks[4 * i:4 * i + 4], "little") for i in range(32)]
+    r = _zero_seed_rng(O)
+    assert [O.lib().orc_rng_next_u32(ctypes.byref(r)) for _ in range(32)] == words
+    r = _zero_seed_rng(O)
+    assert [O.lib().orc_rng_next_u64(ctypes.byref(r)) for _ in range(16)] == \
+        [words[2 * i] | words[2 * i + 1] << 32 for i in range(16)]
+    rng = O.ChaCha(seed_bytes=bytes(32))
+    assert rng.fill_bytes(64) == RFC7539_A1_TV1
+    assert rng.fill_bytes(64) == RFC7539_A1_TV2
+
+
+def test_chacha20_rng_u64_across_refill(O):
+    """A u64 read at an odd word position: inside a refill it takes two
+    consecutive words; at the last word of a 64-word refill it takes that
+    word as its low half and the next refill's first word as its high half
+    (rand_core BlockRng::next_u64), so the u64 stream is the word stream."""
+    import ctypes
+    ref = _zero_seed_rng(O)
+    words = [O.lib().orc_rng_next_u32(ctypes.byref(ref)) for _ in range(200)]
+    for skip in (1, 3, 61, 63, 65, 127):
+        r = _zero_seed_rng(O)
+        for _ in range(skip):
+            O.lib().orc_rng_next_u32(ctypes.byref(r))
+        got = [O.lib().orc_rng_next_u64(ctypes.byref(r)) for _ in range(4)]
+        assert got == [words[skip + 2 * i] | words[skip + 2 * i + 1] << 32 for i in range(4)], skip
+    # words 16..31 are RFC 7539 A.1 block 1: the 64-bit block counter counts through
+    assert words[16] == int.from_bytes(RFC7539_A1_TV2[:4], "little")
+
+
+def test_pcg32_demo_outputs(O):
+    """The LCG step and XSH-RR output that seed_from_u64 expands a u64 with,
+    against O'Neill's pcg32 demo: pcg32_srandom_r(42, 54) (inc = 54 * 2 + 1;
+    state 0, step, add 42, step), then outputs taken from the state before
+    each step."""
+    lib = O.lib()
+    inc = (54 << 1) | 1
+    state = lib.orc_pcg32_advance(0, inc)
+    state = lib.orc_pcg32_advance((state + 42) & (2**64 - 1), inc)
+    out = []
+    for _ in range(6):
+        out.append(lib.orc_pcg32_output(state))
+        state = lib.orc_pcg32_advance(state, inc)
+    assert out == [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
+
+
+def test_seed_from_u64_expansion(O):
+    """seed_from_u64(s): eight PCG32 outputs of a state advanced before each
+    output by the rand_core increment, serialized little-endian, as the
+    ChaCha20 key."""
+    lib = O.lib()
+    inc = 11634580027462260723
+    for s in (0, 1, 42, 2**64 - 1):
+        st, seed = s, b""
+        for _ in range(8):
+            st = lib.orc_pcg32_advance(st, inc)
+            seed += lib.orc_pcg32_output(st).to_bytes(4, "little")
+        assert np.array_equal(O.ChaCha(seed=s).random_fr(5), O.ChaCha(seed_bytes=seed).random_fr(5)), s

@@ -85,3 +85,79 @@ def test_transcript_blake2b():
     h = hashlib.blake2b(digest_size=64, key=PR.KEY)
     h.update(b"\x01abc\x00")
     assert ch == int.from_bytes(h.digest(), "little") % R
+
+
+def test_parallelize_chunks_reference_example():
+    """utils/arithmetic.rs:145-151: 40 items on 12 threads are cut into
+    4, 4, 4, 4, 3, 3, 3, 3, 3, 3, 3, 3."""
+    chunks = PR.parallelize_chunks(40, 12)
+    assert [ln for _, ln in chunks] == [4] * 4 + [3] * 8
+    assert [o for o, _ in chunks] == [0, 4, 8, 12, 16, 19, 22, 25, 28, 31, 34, 37]
+
+
+@pytest.mark.parametrize("n,threads", [(128, 1), (128, 5), (128, 16), (1024, 5), (1024, 12), (5, 8), (8, 8),
+                                       (1 << 20, 48), (1 << 20, 7)])
+def test_sample_random_seeds_cover_the_chunks(n, threads):
+    """Every chunk parallelize hands out gets its own seed: the offsets of
+    vanishing/prover.rs:50-66, in seed order, are the chunk offsets in row
+    order, the chunks partition the rows and differ in length by at most 1."""
+    chunks = PR.parallelize_chunks(n, threads)
+    assert PR.sample_random_seed_offsets(n, threads) == [o for o, _ in chunks]
+    assert len(chunks) == min(n, threads)
+    pos = 0
+    for o, ln in chunks:
+        assert o == pos and ln >= 1
+        pos += ln
+    assert pos == n
+    assert max(ln for _, ln in chunks) - min(ln for _, ln in chunks) <= 1
+
+
+def test_sample_random_draw_order():
+    """The polynomial of a pool of T threads takes T 32-byte seeds from the
+    prover's RNG: the draws after it (the quotient-limb blinds) start 8 T
+    words later; with one thread the polynomial is one stream."""
+    n = 64
+    for threads in (1, 3, 16):
+        rng = O.ChaCha(seed=5)
+        poly = PR.sample_random(rng, n, threads)
+        assert poly.shape == (n, 4)
+        after = rng.random_fr(2)
+        ref = O.ChaCha(seed=5)
+        seeds = [ref.fill_bytes(32) for _ in range(threads)]
+        assert np.array_equal(ref.random_fr(2), after)
+        o, ln = PR.parallelize_chunks(n, threads)[-1]
+        assert np.array_equal(poly[o:], O.ChaCha(seed_bytes=seeds[-1]).random_fr(ln))
+    one = PR.sample_random(O.ChaCha(seed=9), n, 1)
+    assert np.array_equal(one, O.ChaCha(seed_bytes=O.ChaCha(seed=9).fill_bytes(32)).random_fr(n))
+
+
+def test_fast_circuit_satisfies_its_constraints():
+    """The numpy-built circuit of the family: the gate q (a b + c - d) is
+    zero on every row and every copy cycle holds one value."""
+    fc = S.FastCircuit(9, seed=4)
+    n = fc.n
+    cols = np.concatenate([fc.advice, fc.fixed]).reshape(8, n, 4)
+    a, b, c, d, e, q, s1, s2 = (S.ints(cols[i]) for i in range(8))
+    for i in range(n):
+        assert q[i] * (a[i] * b[i] + c[i] - d[i]) % R == 0, i
+    m = fc.mapping.reshape(8, n, 2)
+    moved = 0
+    for j in range(8):
+        for i in range(n):
+            jj, ii = int(m[j, i, 0]), int(m[j, i, 1])
+            vals = [a, b, c, d, e, q, s1, s2]
+            assert vals[j][i] == vals[jj][ii], (j, i)
+            # the mapping is a permutation: the target maps back in a cycle of two
+            assert (int(m[jj, ii, 0]), int(m[jj, ii, 1])) == (j, i)
+            moved += (jj, ii) != (j, i)
+    assert moved == 2 * (fc.u // 4)
+
+
+def test_cpu_proof_with_thread_seeded_random_poly(setup):
+    """A proof whose random polynomial follows a 5-thread pool is accepted
+    and differs from the one-thread proof (every later draw moves)."""
+    c, key = setup[:2]
+    p1 = PR.cpu_prove(key, c.cols[:5], 3)
+    p5 = PR.cpu_prove(key, c.cols[:5], 3, rng_threads=5)
+    assert p1 != p5
+    assert PR.verify((key.fixed_com, key.sigma_com, key.repr), p5, key)
