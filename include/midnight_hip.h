@@ -156,6 +156,14 @@ RustError midnight_sppark_bls12_381_fr_device_coeff_to_extended_from_device_batc
 RustError midnight_sppark_bls12_381_fr_device_vec_ntt_zeta(void *device_vec, uint32_t log_n,
                                                            bool inverse, RawFr scale, RawFr first,
                                                            RawFr second);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation): midnight_sppark_bls12_381_fr_device_vec_ntt_zeta with
+ * the same result, its four-step staging taken from a static 32 MiB scratch
+ * array per GPU instead of an allocation.  log_n at most 20 (larger is an
+ * error).  Calls that overlap on one GPU share that array and race. */
+RustError midnight_hip_fr_device_vec_ntt_zeta_noalloc(void *device_vec, uint32_t log_n, bool inverse,
+                                                      RawFr scale, RawFr first, RawFr second);
 /* replaces bls12_381_g1_msm.cu:3331 */
 RustError midnight_sppark_bls12_381_fr_device_vec_ntt_zeta_batch(void *device_vec,
                                                                  uint32_t log_n, size_t count,

@@ -659,6 +659,71 @@ extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_ntt_zeta(void *devi
     MH_CATCH
 }
 
+// --------------------------------------------------- no-allocation variant
+//
+// The four-step transform stages the 2^log_n values between its two passes
+// in a scratch buffer.  For callers that cannot allocate device memory, the
+// variant below takes that buffer from this static array instead of the
+// lane's grow-only scratch: 2^20 Fr (32 MiB) per GPU, held until the process
+// ends.  Every call on one GPU shares it, so calls that overlap on one GPU
+// (different lanes or threads at once) race; calls on one lane are ordered.
+// The pass kernels overwrite every scratch element before reading it, so
+// nothing needs clearing between calls.
+constexpr uint32_t kNoAllocMaxLog = 20;
+static __device__ __align__(256) uint32_t scratch[(size_t(1) << kNoAllocMaxLog) * 8];
+
+namespace {
+
+// device address of `scratch` on the current device, looked up once per device
+Fr *noalloc_scratch()
+{
+    static std::once_flag once[64];
+    static Fr *ptr[64];
+    const int dev = device_id();
+    std::call_once(once[dev], [dev]() {
+        void *p = nullptr;
+        MH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[dev] = static_cast<Fr *>(p);
+    });
+    return ptr[dev];
+}
+
+}  // namespace
+
+// midnight_sppark_bls12_381_fr_device_vec_ntt_zeta with no allocation: the
+// same transform of the same vector (limb-identical), its staging in the
+// static scratch array; log_n at most 20.
+extern "C" RustError midnight_hip_fr_device_vec_ntt_zeta_noalloc(void *device_vec, uint32_t log_n, bool inverse,
+                                                                 RawFr scale, RawFr first, RawFr second)
+{
+    if (!device_vec) return make_error((int)hipErrorInvalidValue, "device vector is null");
+    DeviceVec *dv = static_cast<DeviceVec *>(device_vec);
+    if (log_n >= 63 || dv->len != (size_t(1) << log_n))
+        return make_error((int)hipErrorInvalidValue, "device vector length mismatch");
+    if (log_n > kNoAllocMaxLog)
+        return make_error((int)hipErrorInvalidValue, "transform above 2^20, the no-allocation scratch limit");
+    if (log_n == 0) return ok();
+    MH_TRY
+    Fr *scr = noalloc_scratch();
+    hipStream_t st = cur_stream();
+    NttJob job;
+    job.in = dv->values.get();
+    job.out = dv->values.get();
+    job.scratch = ntt_scratch_elems(log_n, 1) ? scr : nullptr;
+    job.in_pstride = job.out_pstride = size_t(1) << log_n;
+    job.log_n = log_n;
+    job.count = 1;
+    job.inverse = inverse;
+    job.post_zeta = true;
+    job.scale = to_hfr(scale);
+    job.first = to_hfr(first);
+    job.second = to_hfr(second);
+    ntt_run(job, st);
+    MH_HIP(hipStreamSynchronize(st));
+    return ok();
+    MH_CATCH
+}
+
 extern "C" RustError midnight_sppark_bls12_381_fr_device_vec_ntt_zeta_batch(void *device_vec, uint32_t log_n,
                                                                             size_t count, bool inverse, RawFr scale,
                                                                             RawFr first, RawFr second)

@@ -58,42 +58,43 @@ struct Tables {
     Fr *hi[2];      // omega_{2^22}^(2048 i), i < 2048
 };
 
+// The tables live in static device memory ([direction][table][entry], 32-bit
+// words): filled once per device by a copy, so no transform ever allocates.
+constexpr size_t kTwEntries = 1024 + 2048 + 2048;
+__device__ __align__(256) uint32_t g_twiddles[2 * kTwEntries * 8];
+
 Tables g_tables[64];
 std::once_flag g_tables_once[64];
 
 // c -> c * 2^261 mod r, which is the usual (2^256) form of 32 c
 HostFr scaled29(const HostFr &c) { return c * HostFr::from_u64(32); }
 
-void upload(Fr **dst, const std::vector<HostFr> &v)
-{
-    std::vector<HostFr> sc(v.size());
-    for (size_t i = 0; i < v.size(); ++i) sc[i] = scaled29(v[i]);
-    MH_HIP(hipMalloc(reinterpret_cast<void **>(dst), sc.size() * 32));
-    MH_HIP(hipMemcpy(*dst, sc.data(), sc.size() * 32, hipMemcpyHostToDevice));
-}
-
 const Tables &tables()
 {
     int dev = device_id();
     std::call_once(g_tables_once[dev], [dev]() {
         Tables &t = g_tables[dev];
+        Fr *base = nullptr;
+        MH_HIP(hipGetSymbolAddress(reinterpret_cast<void **>(&base), HIP_SYMBOL(g_twiddles)));
+        std::vector<HostFr> host(2 * kTwEntries);
         for (int dir = 0; dir < 2; ++dir) {
             HostFr w22 = HostFr::omega(kTwLog);
             if (dir) w22 = w22.inverse();
             HostFr w11 = w22;
             for (int i = kMaxLogLds; i < kTwLog; ++i) w11 = w11.sqr();
-            std::vector<HostFr> roots(1024), lo(2048), hi(2048);
+            HostFr *roots = host.data() + dir * kTwEntries, *lo = roots + 1024, *hi = lo + 2048;
             HostFr acc = HostFr::one();
-            for (int i = 0; i < 1024; ++i) { roots[i] = acc; acc = acc * w11; }
+            for (int i = 0; i < 1024; ++i) { roots[i] = scaled29(acc); acc = acc * w11; }
             acc = HostFr::one();
-            for (int i = 0; i < 2048; ++i) { lo[i] = acc; acc = acc * w22; }
+            for (int i = 0; i < 2048; ++i) { lo[i] = scaled29(acc); acc = acc * w22; }
             HostFr step = acc;   // w22^2048
             acc = HostFr::one();
-            for (int i = 0; i < 2048; ++i) { hi[i] = acc; acc = acc * step; }
-            upload(&t.roots[dir], roots);
-            upload(&t.lo[dir], lo);
-            upload(&t.hi[dir], hi);
+            for (int i = 0; i < 2048; ++i) { hi[i] = scaled29(acc); acc = acc * step; }
+            t.roots[dir] = base + dir * kTwEntries;
+            t.lo[dir] = t.roots[dir] + 1024;
+            t.hi[dir] = t.lo[dir] + 2048;
         }
+        MH_HIP(hipMemcpy(base, host.data(), host.size() * 32, hipMemcpyHostToDevice));
     });
     return g_tables[dev];
 }

@@ -203,6 +203,7 @@ def _declare(L):
                                          ctypes.POINTER(c_size)]),
         "midnight_hip_plonk_proof_size": (E, [vp, ctypes.POINTER(c_size)]),
         "midnight_hip_plonk_set_rng_threads": (E, [vp, u32]),
+        "midnight_hip_fr_device_vec_ntt_zeta_noalloc": (E, [vp, u32, ctypes.c_bool, RawFr, RawFr, RawFr]),
         "midnight_hip_fr_sample_random_poly": (E, [vpp, u32, c_size, ctypes.c_char_p, c_size]),
         "midnight_hip_proof_server_create": (E, [vpp, u32, ctypes.c_uint64]),
         "midnight_hip_proof_server_submit": (E, [vp, vp, u64p, c_size, ctypes.c_uint64,
@@ -356,6 +357,10 @@ class DeviceVec:
 
     def ntt_zeta(self, log_n, inverse, scale, first, second):
         check(lib().midnight_sppark_bls12_381_fr_device_vec_ntt_zeta(
+            self.handle, log_n, inverse, RawFr.of(scale), RawFr.of(first), RawFr.of(second)))
+
+    def ntt_zeta_noalloc(self, log_n, inverse, scale, first, second):
+        check(lib().midnight_hip_fr_device_vec_ntt_zeta_noalloc(
             self.handle, log_n, inverse, RawFr.of(scale), RawFr.of(first), RawFr.of(second)))
 
     def ntt_zeta_batch(self, log_n, count, inverse, scale, first, second):
