@@ -9,6 +9,15 @@ point set [t_i]G.  Points and scalars are synthetic, generated on the device
 before timing, so the timed region starts with every input in HBM; each
 call returns its Jacobian results to the host before the next starts.
 
+The default run also reports, in the same JSON line:
+- "one_shot": the same 2^20 MSM through the one-shot entry point (host
+  inputs uploaded by the call);
+- "serving": the proof server on the serving mix (k in {12, 14, 16, 18} at
+  50/30/15/5 %, SRS 2^20 resident), batched and one proof at a time on the
+  same Poisson streams at 4x and at 0.9x the one-at-a-time capacity, with
+  p50/p99 latency and the CPU reference prover's proofs/s on the mix;
+- "cpu_baseline": msm_best (batch-affine buckets) on the same 2^20 MSM.
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -112,28 +121,35 @@ def device_fr(H, canonical):
     return dv
 
 
+def cpu_threads():
+    """host threads for the CPU baselines: all cores, at most 16 (one GPU's
+    share of its host)"""
+    return min(16, os.cpu_count() or 1)
+
+
 def cpu_baseline_msm(H, ctx, dv_scalars, log_n, seconds):
-    """Oracle Pippenger (multithreaded C restatement of msm_best) on a
-    bounded sample of the same workload: the first 2^sample_log points and
-    scalars, repeated for about `seconds`."""
+    """msm_best (the reference's CPU MSM above 2^19: Booth windows of
+    ceil(ln n) bits with the batch-affine bucket schedule, restated in
+    oracle/msm.c) on the workload's first MSM at full size (up to 2^20
+    points), repeated for about `seconds` on all host threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    sample_log = min(log_n, 18)
+    sample_log = min(log_n, 20)
     n = 1 << sample_log
     pts = ctx.points(n)
     sc = dv_scalars.download()[:n]
-    threads = os.cpu_count() or 1
-    oracle.msm(pts[:1024], sc[:1024], threads=threads)     # warm-up (table, threads)
+    threads = cpu_threads()
+    oracle.msm(pts[:16384], sc[:16384], threads=threads, best=True)     # warm-up (threads)
     reps, t0 = 0, time.perf_counter()
     while True:
-        oracle.msm(pts, sc, threads=threads)
+        oracle.msm(pts, sc, threads=threads, best=True)
         reps += 1
         if time.perf_counter() - t0 >= seconds:
             break
     dt = time.perf_counter() - t0
     return {"value": round(n * reps / dt / 1e6, 4), "unit": "Mpts/s", "cores": threads, "kind": "port",
-            "sample": "%d x MSM of 2^%d points (first 2^%d of the workload), oracle Pippenger, %d threads, %.1f s"
-                      % (reps, sample_log, sample_log, threads, dt)}
+            "sample": "%d x MSM of 2^%d points (the workload's first MSM), msm_best with batch-affine buckets "
+                      "restated in C, %d threads, %.1f s" % (reps, sample_log, threads, dt)}
 
 
 def bench_msm(args, H):
@@ -198,11 +214,26 @@ def bench_msm(args, H):
         "accumulate_ms_avg": round(acc_avg_s * 1e3, 3),
         "engine_ms_avg": round(tot_ms / max(tot_cnt, 1), 3),
     }
+    if log_n <= 22:
+        # the same MSM through the one-shot entry point (host points and
+        # scalars, uploaded by the call; no resident context or table)
+        pts = ctx.points(n)
+        sc_h = dv.download()[:n]
+        H.g1_msm(pts[:4096], sc_h[:4096])
+        reps, t0 = 3, time.perf_counter()
+        for _ in range(reps):
+            H.g1_msm(pts, sc_h)
+        dt1 = (time.perf_counter() - t0) / reps
+        res["one_shot"] = {"mpts_per_s": round(n / dt1 / 1e6, 3), "ms_per_msm": round(dt1 * 1e3, 3),
+                           "entry": "midnight_sppark_bls12_381_g1_msm (host inputs, PCIe upload included)"}
+        del pts, sc_h
     if args.cpu_seconds > 0:
         log("timing the CPU baseline")
         res["cpu_baseline"] = cpu_baseline_msm(H, ctx, dv, log_n, args.cpu_seconds)
     ctx.free()
     dv.free()
+    if args.serving:
+        res["serving"] = serving_report(args, H, sorted({args.load, 0.9}, reverse=True))
     return res
 
 
@@ -376,42 +407,33 @@ MONT_ONE = np.array([0x00000001fffffffe, 0x5884b7fa00034802, 0x998c4fefecbc4ff5,
 
 def synthetic_circuit(H, k, seed):
     """One instance of the benchmark circuit family at 2^k rows: advice
-    a, b, c uniform, d = a*b + c, e copying a cells of a (copy cycles of two
-    over random rows), fixed q = 1 on usable rows, s1, s2 uniform.  Returns
-    (fixed, advice) as Montgomery arrays and the permutation mapping."""
-    import random as _random
-    R = R_MOD
+    a, b, c uniform, d = a*b + c on the usable rows (products on the
+    device), e copying a quarter of the usable rows of a (copy cycles of
+    two), fixed q = 1 on usable rows, s1, s2 uniform there.  Returns (fixed,
+    advice) as Montgomery arrays and the permutation mapping."""
     n = 1 << k
     u = n - 6
-    rnd = _random.Random(seed)
-    a = [rnd.randrange(R) for _ in range(n)]
-    b = [rnd.randrange(R) for _ in range(n)]
-    c = [rnd.randrange(R) for _ in range(n)]
-    d = [(a[i] * b[i] + c[i]) % R if i < u else rnd.randrange(R) for i in range(n)]
-    e = [rnd.randrange(R) for _ in range(n)]
+    rng = np.random.default_rng(seed)
+    dv = device_fr(H, random_canonical(7 * n, rng))
+    vals = dv.download()
+    dv.free()
+    a, b, c, d, e, s1, s2 = (vals[i * n:(i + 1) * n].copy() for i in range(7))
+    ab = H.fr_mul_assign_repeating(a[:u], b[:u])                    # a_i * b_i
+    d[:u] = H.fr_poly_lincomb(np.concatenate([ab, c[:u]]), u, np.tile(MONT_ONE, (2, 1)))
+    q = np.zeros((n, 4), dtype=np.uint64)
+    q[:u] = MONT_ONE
+    s1[u:] = 0
+    s2[u:] = 0
     mapping = np.zeros((8, n, 2), dtype=np.uint32)
     mapping[:, :, 0] = np.arange(8, dtype=np.uint32)[:, None]
     mapping[:, :, 1] = np.arange(n, dtype=np.uint32)[None, :]
-    rows = list(range(u))
-    rnd.shuffle(rows)
-    for t in range(0, u // 4, 1):
-        i, j = rows[2 * t], rows[2 * t + 1]
-        e[i] = a[j]
-        mapping[4, i] = (0, j)
-        mapping[0, j] = (4, i)
-    q = [1 if i < u else 0 for i in range(n)]
-    s1 = [rnd.randrange(R) if i < u else 0 for i in range(n)]
-    s2 = [rnd.randrange(R) if i < u else 0 for i in range(n)]
-
-    def mont(cols):
-        flat = np.array([[(v >> (64 * j)) & (2**64 - 1) for j in range(4)] for col in cols for v in col],
-                        dtype=np.uint64)
-        dv = device_fr(H, flat)
-        out = dv.download()
-        dv.free()
-        return out
-
-    return mont([q, s1, s2]), mont([a, b, c, d, e]), mapping.reshape(-1)
+    rows = rng.permutation(u)
+    pairs = u // 4
+    ei, aj = rows[:pairs], rows[pairs:2 * pairs]
+    e[ei] = a[aj]
+    mapping[4, ei, 0], mapping[4, ei, 1] = 0, aj
+    mapping[0, aj, 0], mapping[0, aj, 1] = 4, ei
+    return np.concatenate([q, s1, s2]), np.concatenate([a, b, c, d, e]), mapping.reshape(-1)
 
 
 def bench_prove(args, H):
@@ -485,17 +507,17 @@ def cpu_baseline_prove(seconds):
     """The reference-algorithm CPU prover (oracle MSM / FFT / numerator /
     products, protocol restated in tests/plonk_ref.py) on a bounded sample:
     proofs of the same circuit family at k = 14, repeated for about
-    `seconds`."""
+    `seconds` (the key is built outside the timing)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import plonk_ref
     import synth_circuit
-    threads = os.cpu_count() or 1
-    c = synth_circuit.Circuit(14, seed=14)
-    key = plonk_ref.CpuKey(c, 12345, threads)
+    threads = cpu_threads()
+    fc = synth_circuit.FastCircuit(14, seed=14)
+    key = plonk_ref.CpuKey(fc, 12345, threads)
     reps, t0 = 0, time.perf_counter()
     while True:
-        plonk_ref.cpu_prove(key, c.cols[:5], reps, threads)
+        plonk_ref.cpu_prove(key, fc.advice, reps, threads)
         reps += 1
         if time.perf_counter() - t0 >= seconds:
             break
@@ -505,17 +527,16 @@ def cpu_baseline_prove(seconds):
                       % (reps, threads, dt)}
 
 
-def bench_serve(args, H):
-    """Serving: a Poisson stream of proof requests for the benchmark circuit
-    family at k in {12, 14, 16, 18} (mix 50/30/15/5 %), SRS of 2^20 points
-    and the four proving keys resident, HBM-budgeted admission.  The same
-    stream is served one proof at a time (one slot) and with continuous
-    batching (--slots); the value is the batched proofs/s, with p50/p99
-    latency (submission to proof) for both."""
-    import threading
+SERVE_KS = [12, 14, 16, 18]
+SERVE_MIX = [0.5, 0.3, 0.15, 0.05]
+
+
+def serving_setup(args, H):
+    """SRS of 2^20 points [tau^i]G and the four proving keys of the serving
+    mix, resident; the one-at-a-time cost of each k (warm)."""
     import numerator_graph as G
     rng = np.random.default_rng(args.seed)
-    log("building the 2^20 SRS and the keys for k = 12, 14, 16, 18")
+    log("serving: building the 2^20 SRS and the keys for k = 12, 14, 16, 18")
     tau = random_canonical(1, rng)
     tdv = device_fr(H, tau)
     tau_m = tdv.download()[0]
@@ -528,79 +549,148 @@ def bench_serve(args, H):
                                              ("advice", 2, 0)), ("neg", ("advice", 3, 0))))
     graph = G.RawGraphBuffers(G.custom_gates_graph([gate]))
     perm = [(1, i) for i in range(5)] + [(0, i) for i in range(3)]
-    ks = [12, 14, 16, 18]
-    keys, advice = {}, {}
-    for k in ks:
+    keys, advice, cost = {}, {}, {}
+    for k in SERVE_KS:
         fixed, adv, mapping = synthetic_circuit(H, k, 1000 + k)
         keys[k] = H.PlonkKey(k, srs, 5, fixed, 5, [(c, 0) for c in range(5)], [(c, 0) for c in range(3)], perm,
                              mapping, graph, b"benchmark circuit family")
         advice[k] = adv
-    mix = rng.choice(ks, size=args.requests, p=[0.5, 0.3, 0.15, 0.05])
-    # one-at-a-time cost of each k (warm), for the offered load
-    cost = {}
-    for k in ks:
+    for k in SERVE_KS:
         keys[k].prove(advice[k], 1)
         t0 = time.perf_counter()
-        keys[k].prove(advice[k], 2)
-        cost[k] = time.perf_counter() - t0
-    mean_cost = sum(cost[k] * p for k, p in zip(ks, [0.5, 0.3, 0.15, 0.05]))
-    rate = args.load / mean_cost          # requests per second offered
-    gaps = rng.exponential(1.0 / rate, size=args.requests)
-    arrivals = np.cumsum(gaps)
-    log("single-proof ms: %s; offered load %.1f req/s (%.1f x one-at-a-time capacity)"
-        % ({k: round(v * 1e3, 2) for k, v in cost.items()}, rate, args.load))
+        for r in range(3):
+            keys[k].prove(advice[k], 2 + r)
+        cost[k] = (time.perf_counter() - t0) / 3
+    mean_cost = sum(cost[k] * p for k, p in zip(SERVE_KS, SERVE_MIX))
+    log("serving: single-proof ms %s, one-at-a-time capacity %.1f proofs/s"
+        % ({k: round(v * 1e3, 2) for k, v in cost.items()}, 1.0 / mean_cost))
+    return {"srs": srs, "keys": keys, "advice": advice, "cost": cost, "mean_cost": mean_cost}
 
-    def serve(slots):
-        server = H.ProofServer(slots, args.hbm_budget_gb << 30)
-        jobs = [None] * args.requests
-        t_start = time.perf_counter()
 
-        def submitter():
-            for i in range(args.requests):
-                delay = t_start + arrivals[i] - time.perf_counter()
-                if delay > 0:
-                    time.sleep(delay)
-                jobs[i] = (time.perf_counter(), server.submit(keys[int(mix[i])], advice[int(mix[i])], 10 + i))
+def serve_stream(args, H, st, slots, load, seed):
+    """One Poisson stream of args.requests requests (k mix 50/30/15/5 %) at
+    `load` x the one-at-a-time capacity, served with `slots` concurrent
+    proofs; latency is submission to proof (nearest-rank percentiles)."""
+    import threading
+    rng = np.random.default_rng(seed)
+    mix = rng.choice(SERVE_KS, size=args.requests, p=SERVE_MIX)
+    rate = load / st["mean_cost"]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=args.requests))
+    keys, advice = st["keys"], st["advice"]
+    server = H.ProofServer(slots, args.hbm_budget_gb << 30)
+    jobs = [None] * args.requests
+    t_start = time.perf_counter()
 
-        th = threading.Thread(target=submitter)
-        th.start()
-        lat = []
+    def submitter():
         for i in range(args.requests):
-            while jobs[i] is None:
-                time.sleep(0.0005)
-            t_sub, jid = jobs[i]
-            server.wait(jid)
-            lat.append(time.perf_counter() - t_sub)
-        th.join()
-        makespan = time.perf_counter() - t_start
-        server.free()
-        lat = sorted(lat)
-        pct = lambda q: lat[min(len(lat) - 1, int(np.ceil(q * len(lat))) - 1)]   # nearest rank
-        return {"proofs_per_s": round(args.requests / makespan, 2), "p50_ms": round(pct(0.5) * 1e3, 2),
-                "p99_ms": round(pct(0.99) * 1e3, 2), "makespan_s": round(makespan, 3)}
+            delay = t_start + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            jobs[i] = (time.perf_counter(), server.submit(keys[int(mix[i])], advice[int(mix[i])], 10 + i))
 
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat = []
+    for i in range(args.requests):
+        while jobs[i] is None:
+            time.sleep(0.0005)
+        t_sub, jid = jobs[i]
+        server.wait(jid)
+        lat.append(time.perf_counter() - t_sub)
+    th.join()
+    makespan = time.perf_counter() - t_start
+    server.free()
+    lat = sorted(lat)
+    pct = lambda q: lat[min(len(lat) - 1, int(np.ceil(q * len(lat))) - 1)]   # nearest rank
+    return {"proofs_per_s": round(args.requests / makespan, 2), "p50_ms": round(pct(0.5) * 1e3, 2),
+            "p99_ms": round(pct(0.99) * 1e3, 2), "makespan_s": round(makespan, 3),
+            "offered_per_s": round(rate, 2)}
+
+
+def serving_report(args, H, loads):
+    """Batched (args.slots) and one-at-a-time serving of the same Poisson
+    streams at each offered load (a multiple of the one-at-a-time
+    capacity), and the CPU reference prover on the same mix."""
+    st = serving_setup(args, H)
+    out = {"slots": args.slots, "requests": args.requests, "hbm_budget_gib": args.hbm_budget_gb,
+           "k_mix": {str(k): p for k, p in zip(SERVE_KS, SERVE_MIX)},
+           "single_proof_ms": {str(k): round(v * 1e3, 3) for k, v in st["cost"].items()},
+           "one_at_a_time_capacity_per_s": round(1.0 / st["mean_cost"], 2), "loads": {}}
     for _ in range(args.warmup):
-        serve(args.slots)
-    log("serving one proof at a time")
-    single = serve(1)
-    log("serving with %d slots" % args.slots)
-    batched = serve(args.slots)
+        serve_stream(args, H, st, args.slots, max(loads), args.seed)
+    for load in loads:
+        log("serving at %.1f x capacity: one at a time" % load)
+        single = serve_stream(args, H, st, 1, load, args.seed + 1)
+        log("serving at %.1f x capacity: %d slots" % (load, args.slots))
+        batched = serve_stream(args, H, st, args.slots, load, args.seed + 1)
+        out["loads"]["%.1f" % load] = {
+            "batched": batched, "one_at_a_time": single,
+            "speedup": round(batched["proofs_per_s"] / single["proofs_per_s"], 3),
+            "p99_ratio": round(batched["p99_ms"] / single["p99_ms"], 3)}
+    for k in SERVE_KS:
+        st["keys"][k].free()
+    st["srs"].free()
+    if args.cpu_seconds > 0:
+        log("serving: timing the CPU reference prover on the mix")
+        out["cpu_baseline"] = cpu_baseline_serving(args.cpu_seconds)
+    return out
+
+
+def cpu_baseline_serving(seconds):
+    """The reference-algorithm CPU prover (oracle MSM / FFT / numerator /
+    products, protocol in tests/plonk_ref.py) on the serving mix: proofs of
+    the same circuit family at k = 12, 14 and 16 (each key built outside
+    the timing); the k = 18 cost is the k = 16 cost x 4 (row count).  The
+    value is 1 / sum_k p_k t_k, proofs/s one at a time on all threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import plonk_ref
+    import synth_circuit
+    threads = cpu_threads()
+    t = {}
+    budget = {12: 0.2 * seconds, 14: 0.3 * seconds, 16: 0.0}
+    for k in (12, 14, 16):
+        fc = synth_circuit.FastCircuit(k, seed=k)
+        key = plonk_ref.CpuKey(fc, 12345, threads)
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            plonk_ref.cpu_prove(key, fc.advice, reps, threads)
+            reps += 1
+            if time.perf_counter() - t0 >= budget[k]:
+                break
+        t[k] = (time.perf_counter() - t0) / reps
+    t[18] = 4 * t[16]
+    mean = sum(t[k] * p for k, p in zip(SERVE_KS, SERVE_MIX))
+    return {"value": round(1.0 / mean, 4), "unit": "proofs/s", "cores": threads, "kind": "port",
+            "sample": "CPU prover on the serving mix, %d threads: seconds per proof k=12 %.3f, k=14 %.3f, "
+                      "k=16 %.3f measured, k=18 %.3f = 4 x k=16; mix-weighted mean %.3f s"
+                      % (threads, t[12], t[14], t[16], t[18], mean)}
+
+
+def bench_serve(args, H):
+    """Serving: a Poisson stream of proof requests for the benchmark circuit
+    family at k in {12, 14, 16, 18} (mix 50/30/15/5 %), SRS of 2^20 points
+    and the four proving keys resident, HBM-budgeted admission.  The same
+    stream is served one proof at a time (one slot) and with continuous
+    batching (--slots); the value is the batched proofs/s at --load, with
+    p50/p99 latency (submission to proof) for both, and the same at 0.9 x
+    the one-at-a-time capacity."""
+    rep = serving_report(args, H, sorted({args.load, 0.9}, reverse=True))
+    main = rep["loads"]["%.1f" % args.load]
     res = {
         "metric": "serving proofs/s (k in {12,14,16,18}, Poisson arrivals), batched vs one-at-a-time",
-        "value": batched["proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "steps": args.requests,
-        "warmup": args.warmup, "ms_per_step": round(1e3 / batched["proofs_per_s"], 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "value": main["batched"]["proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "steps": args.requests,
+        "warmup": args.warmup, "ms_per_step": round(1e3 / main["batched"]["proofs_per_s"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic satisfying circuits of the benchmark family, Poisson arrivals from the seed",
         "config": {"workload": "proof server, %d requests, k mix 12/14/16/18 at 50/30/15/5%%, SRS 2^20 resident, "
                                "%d slots, HBM budget %d GiB, offered load %.1f x one-at-a-time capacity"
                                % (args.requests, args.slots, args.hbm_budget_gb, args.load)},
-        "batched": batched, "one_at_a_time": single,
-        "speedup": round(batched["proofs_per_s"] / single["proofs_per_s"], 3),
-        "single_proof_ms": {str(k): round(v * 1e3, 3) for k, v in cost.items()},
+        "batched": main["batched"], "one_at_a_time": main["one_at_a_time"], "speedup": main["speedup"],
+        "serving": rep,
     }
-    for k in ks:
-        keys[k].free()
-    srs.free()
+    if "cpu_baseline" in rep:
+        res["cpu_baseline"] = rep["cpu_baseline"]
     return res
 
 
@@ -688,7 +778,7 @@ def cpu_baseline_quotient(seconds):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle
     import synth_circuit
-    threads = os.cpu_count() or 1
+    threads = cpu_threads()
     log_s = 18
     inp = synth_circuit.random_inputs(1 << log_s, 5)
     oracle.numerator(**dict(inp, n=1024, fixed=inp["fixed"][:3 * 1024], advice=inp["advice"][:5 * 1024],
@@ -799,13 +889,17 @@ def main():
                     help="msm: run one device copy of this many MiB before timing (counter calibration)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU-baseline sample length (0 disables)")
+    ap.add_argument("--no-serving", dest="serving", action="store_false",
+                    help="msm: skip the serving section of the default run")
     args = ap.parse_args()
+    if args.workload != "msm" or (args.log_n, args.msms_per_step, args.skewed) != (20, 4, False):
+        args.serving = False
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("RANK", "0")) != 0:
         return
-    if args.workload == "serve":
+    if args.workload == "serve" or args.serving:
         # one hardware queue per lane stream (the proof server's slots plus
         # the caller's lane 0); HIP's default of 4 would make lanes share
         # queues and serialise their kernels.  Read when HIP starts, so it is

@@ -124,7 +124,7 @@ static const uint64_t FP_P[6] = {
     0xb9feffffffffaaabULL, 0x1eabfffeb153ffffULL, 0x6730d2a0f6b0f624ULL,
     0x64774b84f38512bfULL, 0x4b1ba7b6434bacd7ULL, 0x1a0111ea397fe69aULL};
 static uint64_t FP_INV;
-static orc_fp FP_R, FP_R2;
+static orc_fp FP_R, FP_R2, FP_R3;
 
 __attribute__((constructor)) static void orc_ff_init(void)
 {
@@ -135,6 +135,7 @@ __attribute__((constructor)) static void orc_ff_init(void)
     FP_INV = neg_inv64(FP_P[0]);
     pow2_mod(FP_R.l, 384, FP_P, 6);
     pow2_mod(FP_R2.l, 768, FP_P, 6);
+    pow2_mod(FP_R3.l, 1152, FP_P, 6);
 }
 
 void orc_fr_add(orc_fr *r, const orc_fr *a, const orc_fr *b) { mod_add(r->l, a->l, b->l, ORC_FR_MODULUS.l, 4); }
@@ -254,6 +255,47 @@ static void fp_pow(orc_fp *r, const orc_fp *a, const uint64_t e[6])
     *r = acc;
 }
 
+/* variable-time inverse by the binary extended Euclidean algorithm (for
+ * the CPU baseline's batch inversions; orc_fp_inv pins it): x = aR as an
+ * integer, x^-1 = a^-1 R^-1, and a Montgomery product with R^3 gives a^-1 R */
+void orc_fp_inv_vartime(orc_fp *r, const orc_fp *a)
+{
+    if (orc_fp_is_zero(a)) { memset(r, 0, sizeof(*r)); return; }
+    uint64_t u[6], v[6], x1[6] = {1, 0, 0, 0, 0, 0}, x2[6] = {0};
+    memcpy(u, a->l, 48);
+    memcpy(v, FP_P, 48);
+    const uint64_t one[6] = {1, 0, 0, 0, 0, 0};
+    for (;;) {
+        if (memcmp(u, one, 48) == 0 || memcmp(v, one, 48) == 0) break;
+        while (!(u[0] & 1)) {
+            for (int i = 0; i < 5; ++i) u[i] = (u[i] >> 1) | (u[i + 1] << 63);
+            u[5] >>= 1;
+            uint64_t c = 0;
+            if (x1[0] & 1) c = add_n(x1, x1, FP_P, 6);
+            for (int i = 0; i < 5; ++i) x1[i] = (x1[i] >> 1) | (x1[i + 1] << 63);
+            x1[5] = (x1[5] >> 1) | (c << 63);
+        }
+        while (!(v[0] & 1)) {
+            for (int i = 0; i < 5; ++i) v[i] = (v[i] >> 1) | (v[i + 1] << 63);
+            v[5] >>= 1;
+            uint64_t c = 0;
+            if (x2[0] & 1) c = add_n(x2, x2, FP_P, 6);
+            for (int i = 0; i < 5; ++i) x2[i] = (x2[i] >> 1) | (x2[i + 1] << 63);
+            x2[5] = (x2[5] >> 1) | (c << 63);
+        }
+        if (geq_n(u, v, 6)) {
+            sub_n(u, u, v, 6);
+            mod_sub(x1, x1, x2, FP_P, 6);
+        } else {
+            sub_n(v, v, u, 6);
+            mod_sub(x2, x2, x1, FP_P, 6);
+        }
+    }
+    orc_fp t;
+    memcpy(t.l, memcmp(u, one, 48) == 0 ? x1 : x2, 48);
+    orc_fp_mul(r, &t, &FP_R3);
+}
+
 void orc_fp_inv(orc_fp *r, const orc_fp *a)
 {
     uint64_t e[6], two[6] = {2, 0, 0, 0, 0, 0};
@@ -308,5 +350,18 @@ void orc_fr_mul_add_many(orc_fr *out, const orc_fr *a, const orc_fr *b, const or
         orc_fr t;
         orc_fr_mul(&t, &a[i], &b[i]);
         orc_fr_add(&out[i], &t, &c[i]);
+    }
+}
+
+/* out[i] = sum_j s[j] * polys[j * n + i] (m polynomials of n coefficients) */
+void orc_fr_lincomb(orc_fr *out, const orc_fr *polys, const orc_fr *s, size_t m, size_t n)
+{
+    for (size_t i = 0; i < n; ++i) {
+        orc_fr acc = {{0}}, t;
+        for (size_t j = 0; j < m; ++j) {
+            orc_fr_mul(&t, &s[j], &polys[j * n + i]);
+            orc_fr_add(&acc, &acc, &t);
+        }
+        out[i] = acc;
     }
 }

@@ -150,6 +150,163 @@ void orc_msm_pippenger(orc_g1j *out, const orc_g1a *pts, const orc_fr *sc, size_
     free(bytes);
 }
 
+/* ---------------------------------------------------------------------
+ * orc_msm_best: vendor/midnight-curves-0.3.1/src/msm.rs:477-560 msm_best
+ * with its batch-affine bucket schedule (msm.rs:52-430), the CPU baseline's
+ * algorithm.  Window c = ceil(ln n) (n < 8104 gives c < 10, where msm_best
+ * takes msm_parallel: orc_msm_pippenger here); identity bases filtered;
+ * NUM_BITS / c + 1 windows spread over threads; per window, each nonzero
+ * Booth digit either goes to an affine bucket through a schedule of up to
+ * BATCH = 64 pending additions (a bucket that is still empty takes the point
+ * outright; a full schedule runs batch_add: one inversion for all its
+ * additions, Montgomery's trick) or, when its bucket already has a pending
+ * addition in the schedule, into a Jacobian bucket; then summation by parts
+ * over Jacobian + affine buckets and c * w doublings.
+ */
+#define MSMB_BATCH 64
+
+typedef struct { size_t base, buck; int sign; } msmb_sched;
+
+/* batch_add (msm.rs:58-142) */
+static void msmb_batch_add(orc_g1a *ab, uint8_t *ainf, const msmb_sched *set, size_t size, const orc_g1a *bases)
+{
+    orc_fp t[MSMB_BATCH], z[MSMB_BATCH], acc;
+    orc_fp_one(&acc);
+    for (size_t i = 0; i < size; ++i) {
+        const size_t b = set[i].buck;
+        const orc_g1a *p = &bases[set[i].base];
+        if (ainf[b]) continue;
+        if (orc_fp_eq(&ab[b].x, &p->x)) {
+            if (orc_fp_eq(&ab[b].y, &p->y) ^ !set[i].sign) {
+                /* doubling: lambda = 3 x^2 / 2 y */
+                orc_fp x2, x3;
+                orc_fp_sqr(&x2, &p->x);
+                orc_fp_add(&x3, &x2, &x2);
+                orc_fp_add(&x3, &x3, &x2);
+                orc_fp_add(&z[i], &ab[b].y, &ab[b].y);
+                orc_fp_mul(&t[i], &acc, &x3);
+                orc_fp_mul(&acc, &acc, &z[i]);
+                continue;
+            }
+            ainf[b] = 1;   /* P + (-P) */
+            continue;
+        }
+        orc_fp dy;
+        orc_fp_sub(&z[i], &ab[b].x, &p->x);
+        if (set[i].sign) orc_fp_sub(&dy, &ab[b].y, &p->y);
+        else orc_fp_add(&dy, &ab[b].y, &p->y);
+        orc_fp_mul(&t[i], &acc, &dy);
+        orc_fp_mul(&acc, &acc, &z[i]);
+    }
+    orc_fp_inv_vartime(&acc, &acc);
+    for (size_t i = size; i-- > 0;) {
+        const size_t b = set[i].buck;
+        const orc_g1a *p = &bases[set[i].base];
+        if (ainf[b]) continue;
+        orc_fp lambda, x, y, d;
+        orc_fp_mul(&lambda, &acc, &t[i]);
+        orc_fp_mul(&acc, &acc, &z[i]);
+        orc_fp_sqr(&x, &lambda);
+        orc_fp_add(&d, &ab[b].x, &p->x);
+        orc_fp_sub(&x, &x, &d);
+        orc_fp_sub(&d, &p->x, &x);
+        orc_fp_mul(&y, &lambda, &d);
+        if (set[i].sign) orc_fp_sub(&y, &y, &p->y);
+        else orc_fp_add(&y, &y, &p->y);
+        ab[b].x = x;
+        ab[b].y = y;
+    }
+}
+
+void orc_msm_best(orc_g1j *out, const orc_g1a *pts, const orc_fr *sc, size_t n, int threads)
+{
+    if (threads < 1) threads = 1;
+    orc_g1j_identity(out);
+    if (n == 0) return;
+    size_t c = n < 4 ? 1 : n < 32 ? 3 : (size_t)ceil(log((double)n));
+    if (c < 10) { orc_msm_pippenger(out, pts, sc, n, threads); return; }
+    const size_t nb = (size_t)1 << (c - 1);
+    const size_t nwin = 255 / c + 1;
+
+    /* filter identities; scalars to little-endian bytes */
+    size_t *idx = (size_t *)malloc(n * sizeof(size_t));
+    size_t m = 0;
+    for (size_t i = 0; i < n; ++i)
+        if (!orc_g1a_is_identity(&pts[i])) idx[m++] = i;
+    uint8_t *bytes = (uint8_t *)malloc(m * 32);
+    orc_g1a *bases = (orc_g1a *)malloc(m * sizeof(orc_g1a));
+    #pragma omp parallel for num_threads(threads) schedule(static)
+    for (long long k = 0; k < (long long)m; ++k) {
+        uint64_t can[4];
+        orc_fr_to_canonical(can, &sc[idx[k]]);
+        for (int l = 0; l < 4; ++l)
+            for (int b = 0; b < 8; ++b) bytes[32 * k + 8 * l + b] = (uint8_t)(can[l] >> (8 * b));
+        bases[k] = pts[idx[k]];
+    }
+    orc_g1j *acc = (orc_g1j *)malloc(nwin * sizeof(orc_g1j));
+
+    #pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+    for (long long wi = (long long)nwin - 1; wi >= 0; --wi) {
+        const size_t w = (size_t)wi;
+        orc_g1j *jb = (orc_g1j *)malloc(nb * sizeof(orc_g1j));
+        orc_g1a *ab = (orc_g1a *)malloc(nb * sizeof(orc_g1a));
+        uint8_t *ainf = (uint8_t *)malloc(nb);
+        for (size_t b = 0; b < nb; ++b) orc_g1j_identity(&jb[b]);
+        memset(ainf, 1, nb);
+        msmb_sched set[MSMB_BATCH];
+        size_t ptr = 0;
+        for (size_t k = 0; k < m; ++k) {
+            const int32_t d = orc_booth_index(w, c, bytes + 32 * k);
+            if (d == 0) continue;
+            const int sign = d > 0;
+            const size_t b = (size_t)(d > 0 ? d : -d) - 1;
+            int pending = 0;
+            for (size_t q = 0; q < ptr; ++q)
+                if (set[q].buck == b) { pending = 1; break; }
+            if (pending) {
+                jadd_aff(&jb[b], &jb[b], &bases[k], !sign);   /* greedy Jacobian accumulation */
+                continue;
+            }
+            if (ainf[b]) {
+                ab[b] = bases[k];
+                if (!sign) orc_fp_neg(&ab[b].y, &ab[b].y);
+                ainf[b] = 0;
+            } else {
+                set[ptr].base = k;
+                set[ptr].buck = b;
+                set[ptr].sign = sign;
+                if (++ptr == MSMB_BATCH) {
+                    msmb_batch_add(ab, ainf, set, ptr, bases);
+                    ptr = 0;
+                }
+            }
+        }
+        if (ptr) msmb_batch_add(ab, ainf, set, ptr, bases);
+        orc_g1j run, a, s;
+        orc_g1j_identity(&run);
+        orc_g1j_identity(&a);
+        for (size_t b = nb; b-- > 0;) {
+            s = jb[b];
+            if (!ainf[b]) jadd_aff(&s, &s, &ab[b], 0);
+            orc_g1j_add(&run, &run, &s);
+            orc_g1j_add(&a, &a, &run);
+        }
+        for (size_t k = 0; k < c * w; ++k) orc_g1j_dbl(&a, &a);
+        acc[w] = a;
+        free(jb);
+        free(ab);
+        free(ainf);
+    }
+    orc_g1j total;
+    orc_g1j_identity(&total);
+    for (size_t w = 0; w < nwin; ++w) orc_g1j_add(&total, &total, &acc[w]);
+    *out = total;
+    free(acc);
+    free(bases);
+    free(bytes);
+    free(idx);
+}
+
 /* fixed-base comb: table[j][b] = [b * 256^j] G, j < 32, b < 256 */
 static orc_g1a *g_table;
 

@@ -51,6 +51,7 @@ int  orc_fr_is_zero(const orc_fr *a);
 void orc_fr_from_uniform_bytes(orc_fr *r, const uint8_t bytes[64]);
 void orc_fr_inner_product(orc_fr *r, const orc_fr *a, const orc_fr *b, size_t n);
 void orc_fr_mul_add_many(orc_fr *out, const orc_fr *a, const orc_fr *b, const orc_fr *c, size_t n);
+void orc_fr_lincomb(orc_fr *out, const orc_fr *polys, const orc_fr *s, size_t m, size_t n);
 /* named constants, Montgomery form (fq.rs:60-135, 476-505, 693-702) */
 void orc_fr_root_of_unity(orc_fr *r);      /* 2^32-th root */
 void orc_fr_zeta(orc_fr *r);               /* cube root of unity ZETA */
@@ -63,6 +64,7 @@ void orc_fp_neg(orc_fp *r, const orc_fp *a);
 void orc_fp_mul(orc_fp *r, const orc_fp *a, const orc_fp *b);
 void orc_fp_sqr(orc_fp *r, const orc_fp *a);
 void orc_fp_inv(orc_fp *r, const orc_fp *a);
+void orc_fp_inv_vartime(orc_fp *r, const orc_fp *a);
 void orc_fp_one(orc_fp *r);
 void orc_fp_to_canonical(uint64_t out[6], const orc_fp *a);
 int  orc_fp_from_canonical(orc_fp *r, const uint64_t in[6]);
@@ -102,6 +104,7 @@ void orc_msm_naive(orc_g1j *out, const orc_g1a *pts, const orc_fr *sc, size_t n)
  * (restates midnight-curves msm.rs:477 msm_best; c = ceil(ln n)) */
 void orc_msm_pippenger(orc_g1j *out, const orc_g1a *pts, const orc_fr *sc,
                        size_t n, int threads);
+void orc_msm_best(orc_g1j *out, const orc_g1a *pts, const orc_fr *sc, size_t n, int threads);
 /* Booth digit of window w (msm.rs:13 get_booth_index); canonical LE bytes */
 int32_t orc_booth_index(size_t window_index, size_t window_size, const uint8_t el[32]);
 /* SRS helper: out[i] = [tau^i]G for i < n (threaded) */

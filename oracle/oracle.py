@@ -64,6 +64,7 @@ def _declare(L):
         "orc_fr_from_uniform_bytes": (V, [u64p, u8p]),
         "orc_fr_inner_product": (V, [u64p, u64p, u64p, ctypes.c_size_t]),
         "orc_fr_mul_add_many": (V, [u64p, u64p, u64p, u64p, ctypes.c_size_t]),
+        "orc_fr_lincomb": (V, [u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_size_t]),
         "orc_pcg32_advance": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64]),
         "orc_pcg32_output": (ctypes.c_uint32, [ctypes.c_uint64]),
         "orc_fr_root_of_unity": (V, [u64p]), "orc_fr_zeta": (V, [u64p]), "orc_fr_delta": (V, [u64p]),
@@ -86,6 +87,8 @@ def _declare(L):
         "orc_g1_to_uncompressed": (V, [u8p, u64p]),
         "orc_msm_naive": (V, [u64p, u64p, u64p, ctypes.c_size_t]),
         "orc_msm_pippenger": (V, [u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
+        "orc_msm_best": (V, [u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
+        "orc_fp_inv_vartime": (V, [u64p, u64p]),
         "orc_booth_index": (ctypes.c_int32, [ctypes.c_size_t, ctypes.c_size_t, u8p]),
         "orc_srs_powers": (V, [u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
         "orc_g1_fixed_base_batch": (V, [u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
@@ -188,6 +191,16 @@ def fp_op(name, *args):
     return out
 
 
+def lincomb(polys, scalars, n):
+    """sum_j scalars[j] * polys[j], coefficient by coefficient (m x n values)"""
+    p = np.ascontiguousarray(polys, dtype=np.uint64).reshape(-1, 4)
+    sc = np.ascontiguousarray(scalars, dtype=np.uint64).reshape(-1, 4)
+    assert p.shape[0] == sc.shape[0] * n
+    out = fr(n)
+    lib().orc_fr_lincomb(P(out), P(p), P(sc), sc.shape[0], n)
+    return out
+
+
 def mul_add(a, b, c):
     """a * b + c element by element (Montgomery arrays of one length)"""
     a, b, c = (np.ascontiguousarray(x, dtype=np.uint64) for x in (a, b, c))
@@ -250,7 +263,9 @@ def random_fr(n, seed):
     return out
 
 
-def msm(points, scalars, threads=None, naive=False):
+def msm(points, scalars, threads=None, naive=False, best=False):
+    """naive: double-and-add per point; best: msm_best with its batch-affine
+    schedule; else Pippenger with Jacobian buckets"""
     points = np.ascontiguousarray(points, dtype=np.uint64)
     scalars = np.ascontiguousarray(scalars, dtype=np.uint64)
     n = points.shape[0]
@@ -258,6 +273,8 @@ def msm(points, scalars, threads=None, naive=False):
     out = g1j()
     if naive:
         lib().orc_msm_naive(P(out), P(points), P(scalars), n)
+    elif best:
+        lib().orc_msm_best(P(out), P(points), P(scalars), n, _threads(threads))
     else:
         lib().orc_msm_pippenger(P(out), P(points), P(scalars), n, _threads(threads))
     return out

@@ -130,8 +130,9 @@ class CpuKey:
         self.omega = c.omega
         self.zeta = S.fr_const("zeta")
         self.delta = S.fr_const("delta")
-        self.fixed_lag = [S.mont(col) for col in c.cols[5:]]
-        self.sigma_lag = [S.mont(col) for col in c.sigma]
+        m = lambda col: col if isinstance(col, np.ndarray) else S.mont(col)
+        self.fixed_lag = [m(col) for col in c.cols[5:]]
+        self.sigma_lag = [m(col) for col in c.sigma]
         self.fixed_coeff = [O.ifft(v, k, threads) for v in self.fixed_lag]
         self.sigma_coeff = [O.ifft(v, k, threads) for v in self.sigma_lag]
         self.fixed_com = [O.msm(self.srs, p, threads) for p in self.fixed_coeff]
@@ -139,7 +140,7 @@ class CpuKey:
         ext = lambda v: O.coeff_to_extended(O.ifft(v, k, threads), self.dom, threads)
         self.fixed_ext = np.concatenate([O.coeff_to_extended(p, self.dom, threads) for p in self.fixed_coeff])
         self.sigma_ext = np.concatenate([O.coeff_to_extended(p, self.dom, threads) for p in self.sigma_coeff])
-        self.l0_ext, self.l_last_ext, self.l_active_ext = (ext(S.mont(v)) for v in (c.l0, c.l_last, c.l_active))
+        self.l0_ext, self.l_last_ext, self.l_active_ext = (ext(m(v)) for v in (c.l0, c.l_last, c.l_active))
         hv = hashlib.blake2b(digest_size=64, person=b"Halo2-Verify-Key")
         hv.update(bytes([3, k]))
         for coms in (self.fixed_com, self.sigma_com):
@@ -174,12 +175,9 @@ def _rot(key, x, rot):
 
 
 def _lincomb(polys, scalars, n):
-    acc = [0] * n
-    for p, s in zip(polys, scalars):
-        pi = S.ints(p)
-        for i in range(n):
-            acc[i] = (acc[i] + s * pi[i]) % R
-    return S.mont(acc)
+    """sum_j scalars[j] polys[j] (scalars as integers) by the oracle"""
+    return O.lincomb(np.concatenate([np.asarray(p).reshape(n, 4) for p in polys]),
+                     S.mont([s % R for s in scalars]), n)
 
 
 def _eval(p, x_int):
@@ -224,7 +222,10 @@ def cpu_prove(key, advice_cols, seed, threads=None, rng_threads=1):
     c, n, k, A = key.c, key.n, key.k, S.ADVICE
     Sn = c.nsets
     rng = O.ChaCha(seed=seed)
-    adv = S.mont([v for col in advice_cols for v in col]).copy()
+    if isinstance(advice_cols, np.ndarray):     # Montgomery limbs, column after column
+        adv = np.array(advice_cols, dtype=np.uint64, copy=True).reshape(A * n, 4)
+    else:
+        adv = S.mont([v for col in advice_cols for v in col]).copy()
     for col in range(A):
         adv[col * n + key.u:(col + 1) * n] = rng.random_fr(n - key.u)
     perm_blinds = rng.random_fr(Sn * key.bf)
@@ -258,15 +259,16 @@ def cpu_prove(key, advice_cols, seed, threads=None, rng_threads=1):
                      l_last=key.l_last_ext, l_active=key.l_active_ext, permutation_products=z_ext,
                      permutation_columns=S.PERM_COLUMNS, permutation_pk_cosets=key.sigma_ext,
                      custom_graph=c.custom_graph(), threads=threads)
-    h = S.ints(O.extended_to_coeff(O.mul_repeating(nu, key.t_evals), key.dom, threads))
-    h = h[:(n - 1) * key.pieces]
-    limbs = [h[i * (n - 1):(i + 1) * (n - 1)] for i in range(key.pieces)]
-    lb = S.ints(limb_blinds)
+    # h truncated to (n - 1) pieces of n - 1 coefficients, each padded to n with
+    # the blinding telescope (vanishing/prover.rs:155-182, 304-318)
+    h = O.extended_to_coeff(O.mul_repeating(nu, key.t_evals), key.dom, threads)
+    pieces = np.zeros((key.pieces * n, 4), dtype=np.uint64)
+    for i in range(key.pieces):
+        pieces[i * n:i * n + n - 1] = h[i * (n - 1):(i + 1) * (n - 1)]
     for i in range(1, key.pieces):
-        limbs[i - 1].append(lb[i - 1])
-        limbs[i][0] = (limbs[i][0] - lb[i - 1]) % R
-    limbs[-1].append(0)
-    h_pieces = [S.mont(l) for l in limbs]
+        pieces[(i - 1) * n + n - 1] = limb_blinds[i - 1]
+        pieces[i * n] = O.fr_op("sub", pieces[i * n], limb_blinds[i - 1])
+    h_pieces = [pieces[i * n:(i + 1) * n] for i in range(key.pieces)]
     for p in h_pieces:
         T.write_point(O.msm(key.srs, p, threads))
     x = T.squeeze()

@@ -253,6 +253,28 @@ class FastCircuit:
         self.fixed = np.concatenate([q, s1, s2])
         self.advice = np.concatenate([a, b, c, d, e])
         self.mapping = m.reshape(-1)
+        self.omega = omega_int(k)
+        self.delta = fr_const("delta")
+        self.delta_pows = [pow(self.delta, j, R) for j in range(len(PERM_COLUMNS))]
+        # what the CPU reference prover's key reads, as Montgomery arrays
+        self.cols = [a, b, c, d, e, q, s1, s2]
+        one = mont([1])[0]
+        wpow = O.shifted_prefix_product(np.tile(mont([self.omega])[0], (n, 1)), one)
+        dpow = mont(self.delta_pows)
+        zero = np.zeros((n, 4), dtype=np.uint64)
+        self.sigma = [O.mul_add(dpow[m[j, :, 0]], wpow[m[j, :, 1]], zero) for j in range(len(PERM_COLUMNS))]
+        lag = np.zeros((3, n, 4), dtype=np.uint64)
+        lag[0, 0] = one
+        lag[1, u] = one
+        lag[2, :u] = one
+        self.l0, self.l_last, self.l_active = lag[0], lag[1], lag[2]
+
+    @property
+    def nsets(self):
+        return (len(PERM_COLUMNS) + CHUNK - 1) // CHUNK
+
+    def custom_graph(self):
+        return gate_graph()
 
 
 def random_inputs(n, seed, *, rot_scale=4, sets=3, graph=None):

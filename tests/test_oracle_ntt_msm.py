@@ -105,3 +105,38 @@ def test_srs_identity_on_reference_data(O):
     assert O.g1_eq(O.msm(g, c), O.msm(gl, O.fft(c, k)))
     # a deliberately wrong NTT direction must fail the identity
     assert not O.g1_eq(O.msm(g, c), O.msm(gl, O.ifft(c, k)))
+
+
+def test_fp_inverse_vartime(O):
+    """The binary-EEA inverse the batch-affine schedule uses equals the
+    Fermat inverse, and a * a^-1 = 1."""
+    one = np.zeros(6, dtype=np.uint64)
+    O.lib().orc_fp_one(O.P(one))
+    pts = O.fixed_base(O.random_fr(40, 555))
+    for p in pts:
+        x = np.ascontiguousarray(p[:6])
+        assert np.array_equal(O.fp_op("inv_vartime", x), O.fp_op("inv", x))
+        assert np.array_equal(O.fp_op("mul", x, O.fp_op("inv_vartime", x)), one)
+    assert np.array_equal(O.fp_op("inv_vartime", one), one)
+
+
+@pytest.mark.parametrize("n", [8200, 20000])
+def test_msm_best_batch_affine(O, n):
+    """msm_best's batch-affine schedule (c = ceil(ln n) >= 10) against the
+    closed form, with doublings (a point twice under equal scalars), P + (-P)
+    cancellations (a point and its negation under one scalar) and heavy
+    buckets (runs of one scalar) in the mix."""
+    ts = O.random_fr(n, 300 + n)
+    sc = O.random_fr(n, 400 + n)
+    r1 = O.fr_op("neg", O.fr_from_int(1))
+    ts[100:160] = ts[0]                     # one point many times, one scalar: doublings
+    sc[100:160] = sc[0]
+    ts[200:260] = O.fr_op("neg", ts[1])     # -P beside P under one scalar: cancellations
+    sc[200:260] = sc[1]
+    sc[500:1500] = sc[2]                    # a heavy bucket in every window
+    sc[2000] = 0
+    sc[2001] = r1
+    pts = O.fixed_base(ts)
+    got = O.msm(pts, sc, best=True)
+    assert O.g1_eq(got, O.g1_mul(O.generator(), O.inner_product(sc, ts)))
+    assert O.g1_eq(got, O.msm(pts, sc))

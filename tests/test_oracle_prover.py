@@ -161,3 +161,18 @@ def test_cpu_proof_with_thread_seeded_random_poly(setup):
     p5 = PR.cpu_prove(key, c.cols[:5], 3, rng_threads=5)
     assert p1 != p5
     assert PR.verify((key.fixed_com, key.sigma_com, key.repr), p5, key)
+
+
+def test_cpu_prover_on_fast_circuit():
+    """The CPU reference prover's key built from the numpy circuit (sigma
+    from the mapping by vectorised products): its proof verifies, and a
+    witness that breaks one copy cycle does not."""
+    fc = S.FastCircuit(8, seed=12)
+    key = PR.CpuKey(fc, 97531)
+    vk = (key.fixed_com, key.sigma_com, key.repr)
+    proof = PR.cpu_prove(key, fc.advice, 4)
+    assert PR.verify(vk, proof, key)
+    bad = fc.advice.copy().reshape(5, fc.n, 4)
+    row = int(np.nonzero((fc.mapping.reshape(8, fc.n, 2)[4, :, 0] == 0))[0][0])   # a copied cell of e
+    bad[4, row] = O.fr_op("add", bad[4, row], O.fr_from_int(1))
+    assert not PR.verify(vk, PR.cpu_prove(key, bad.reshape(-1, 4), 4), key)
