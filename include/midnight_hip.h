@@ -126,6 +126,14 @@ RustError midnight_sppark_bls12_381_g1_msm_context_device_scrambled_batch(
 
 /* ---------------- Fr NTT / coset ---------------- */
 
+/* Test hooks (bls12_381_g1_msm.cu:6379-6424, the reference's "test-hooks"
+ * feature; Rust tests lib.rs:5088-5209): hold a device-vector producer back
+ * by a host callback that sleeps on the lane's stream.  mask_delay applies
+ * to the calling thread's next scrambled batches (0 turns it off);
+ * delayed_scale multiplies a device vector by `scale` behind the delay and
+ * returns before it has run. */
+void midnight_sppark_test_mask_delay(uint32_t delay_ms);
+RustError midnight_sppark_test_delayed_scale(void *raw_device, RawFr scale, uint32_t delay_ms);
 /* replaces bls12_381_g1_msm.cu:6488.  In-place natural-order NTT of 2^log_n
  * values; forward uses omega = ROOT_OF_UNITY^(2^(32-log_n)), inverse uses
  * omega^-1 and includes the 1/n factor. */
@@ -596,6 +604,24 @@ RustError midnight_hip_plonk_set_rng_threads(void *pk, uint32_t threads);
  * equal the chunk count, min(threads, 2^log_n). */
 RustError midnight_hip_fr_sample_random_poly(void **out, uint32_t log_n, size_t threads, const uint8_t *seeds,
                                              size_t seed_count);
+
+/* KZG parameter files (ParamsKZG::read_custom, poly/kzg/params.rs:187-262):
+ * u32 k || 2^k G1 g || 2^k G1 g_lagrange || G2 || [tau]G2, parsed and checked
+ * on the device into two resident MSM contexts.  format follows SerdeFormat
+ * (utils/helpers.rs:16-28): 0 Processed (compressed G1, subgroup-checked),
+ * 1 RawBytes (uncompressed G1, checked below p, on the curve and in the
+ * subgroup), 2 RawBytesUnchecked (uncompressed, coordinates below p only).
+ * An invalid point fails the load.  The G2 elements are kept as read. */
+RustError midnight_hip_srs_load(void **out, const uint8_t *bytes, size_t len, uint32_t format);
+RustError midnight_hip_srs_load_file(void **out, const char *path, uint32_t format);
+/* k, and the MSM contexts of g and g_lagrange (owned by the SRS: use them
+ * with every midnight_sppark_bls12_381_g1_msm_context_* entry point, never
+ * free them; committing Lagrange values against g_lagrange is
+ * KZGCommitmentScheme::commit_lagrange, poly/kzg/mod.rs:1464-1536). */
+RustError midnight_hip_srs_info(const void *srs, uint32_t *k, void **g_context, void **g_lagrange_context);
+/* The file's G2 || [tau]G2 bytes. */
+RustError midnight_hip_srs_g2(const void *srs, uint8_t *out, size_t capacity, size_t *len);
+void midnight_hip_srs_free(void *srs);
 
 /* Proof server: `slots` worker threads (lanes 1..slots, at most 8) prove
  * submitted requests concurrently on the GPU; a request is admitted when a

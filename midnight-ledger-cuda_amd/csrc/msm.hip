@@ -27,6 +27,7 @@
 // The result is the unique group element sum s_i P_i, returned in the
 // reference's Jacobian layout (bls12_381_g1_msm.cu:6164-6486 entry points).
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -1506,6 +1507,45 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_device_ragged(RawG
     MH_CATCH
 }
 
+// ------------------------------------------------------------- test hooks
+//
+// bls12_381_g1_msm.cu:6379-6424 (feature "test-hooks"): hold a producer back
+// on its lane's stream with a host callback that sleeps, without blocking
+// other streams, so a test observes the ordering of producer and consumer
+// rather than a lucky schedule.  Every kernel still runs to completion.
+namespace {
+
+thread_local uint32_t t_mask_delay_ms = 0;
+
+void sleep_ms_callback(void *ms)
+{
+    std::this_thread::sleep_for(std::chrono::milliseconds(reinterpret_cast<uintptr_t>(ms)));
+}
+
+void delay_stream(hipStream_t st, uint32_t ms)
+{
+    MH_HIP(hipLaunchHostFunc(st, sleep_ms_callback, reinterpret_cast<void *>(static_cast<uintptr_t>(ms))));
+}
+
+}  // namespace
+
+extern "C" void midnight_sppark_test_mask_delay(uint32_t delay_ms) { t_mask_delay_ms = delay_ms; }
+
+// scale a device vector in place behind a delay of delay_ms on the lane's
+// stream; returns at once (the scaling is still pending)
+extern "C" RustError midnight_sppark_test_delayed_scale(void *raw_device, RawFr scale, uint32_t delay_ms)
+{
+    if (!raw_device) return make_error((int)hipErrorInvalidValue, "device vector is null");
+    auto *dv = static_cast<DeviceVec *>(raw_device);
+    MH_TRY
+    hipStream_t st = cur_stream();
+    MH_HIP(hipStreamSynchronize(st));
+    delay_stream(st, delay_ms);
+    fr_mul_scalar(dv->values.get(), to_fr(scale), dv->len, st);
+    return ok();
+    MH_CATCH
+}
+
 extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_device_scrambled_batch(
     RawG1Projective *out, void *context, const void *device_vec, size_t scalar_offset, size_t npoints, size_t nmsms,
     const void *mask_device_vec)
@@ -1529,6 +1569,15 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_device_scrambled_b
     hipStream_t st = cur_stream();
     // returns MSM(P, s + mask); the caller subtracts MSM(P, mask) (lib.rs:3349-3353)
     DevBuf<Fr> masked(count);
+    if (t_mask_delay_ms) {
+        // test hook: the masked scalars start as zeros and their producer is
+        // held back, so an MSM that read them early would be wrong
+        Fr zero;
+        std::memset(zero.v, 0, sizeof(zero.v));
+        fr_fill(masked.get(), zero, count, st);
+        MH_HIP(hipStreamSynchronize(st));
+        delay_stream(st, t_mask_delay_ms);
+    }
     fr_add_mask(masked.get(), dv->values.get() + scalar_offset, mask->values.get(), npoints, count, st);
     std::vector<MsmJob> jobs(nmsms);
     for (size_t i = 0; i < nmsms; ++i) jobs[i] = MsmJob{masked.get() + i * npoints, 0, npoints};

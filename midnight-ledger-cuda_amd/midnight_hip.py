@@ -203,6 +203,13 @@ def _declare(L):
                                          ctypes.POINTER(c_size)]),
         "midnight_hip_plonk_proof_size": (E, [vp, ctypes.POINTER(c_size)]),
         "midnight_hip_plonk_set_rng_threads": (E, [vp, u32]),
+        "midnight_hip_srs_load": (E, [vpp, ctypes.c_char_p, c_size, u32]),
+        "midnight_hip_srs_load_file": (E, [vpp, ctypes.c_char_p, u32]),
+        "midnight_hip_srs_info": (E, [vp, ctypes.POINTER(u32), vpp, vpp]),
+        "midnight_hip_srs_g2": (E, [vp, ctypes.c_char_p, c_size, ctypes.POINTER(c_size)]),
+        "midnight_hip_srs_free": (None, [vp]),
+        "midnight_sppark_test_mask_delay": (None, [u32]),
+        "midnight_sppark_test_delayed_scale": (E, [vp, RawFr, u32]),
         "midnight_hip_fr_device_vec_ntt_zeta_noalloc": (E, [vp, u32, ctypes.c_bool, RawFr, RawFr, RawFr]),
         "midnight_hip_fr_sample_random_poly": (E, [vpp, u32, c_size, ctypes.c_char_p, c_size]),
         "midnight_hip_proof_server_create": (E, [vpp, u32, ctypes.c_uint64]),
@@ -455,6 +462,16 @@ def device_coeff_to_extended_from_device_batch(src, input_len, count, ext_log_n,
     return DeviceVec(h.value)
 
 
+def test_mask_delay(ms):
+    """delay (ms) of the calling thread's scrambled-batch mask producer"""
+    lib().midnight_sppark_test_mask_delay(ms)
+
+
+def test_delayed_scale(dvec, scale, ms):
+    """scale a device vector behind a delay on the lane's stream; returns at once"""
+    check(lib().midnight_sppark_test_delayed_scale(dvec.handle, RawFr.of(scale), ms))
+
+
 # ---------------------------------------------------------------- MSM
 
 def _aff_array(points):
@@ -497,7 +514,8 @@ def default_config(npoints):
 class MsmContext:
     """Resident point set (msm_context_t)."""
 
-    def __init__(self, points, config=None, _handle=None, _npoints=0):
+    def __init__(self, points, config=None, _handle=None, _npoints=0, _owned=True):
+        self.owned = _owned
         if _handle is not None:
             self.handle = _handle
             self.npoints = _npoints
@@ -513,9 +531,9 @@ class MsmContext:
         self.handle = h
 
     def free(self):
-        if self.handle and self.handle.value:
+        if self.handle and self.handle.value and self.owned:
             lib().midnight_sppark_bls12_381_g1_msm_context_free(self.handle)
-            self.handle = ctypes.c_void_p()
+        self.handle = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -1040,6 +1058,46 @@ def fr_sample_random_poly(log_n, threads, seeds):
     check(lib().midnight_hip_fr_sample_random_poly(ctypes.byref(h), log_n, threads, bytes(seeds),
                                                    len(seeds) // 32))
     return DeviceVec(h.value)
+
+
+class Srs:
+    """KZG parameters (bls_midnight_2pK files) resident as two MSM contexts:
+    g ([tau^i]G) and g_lagrange.  format: 0 Processed, 1 RawBytes,
+    2 RawBytesUnchecked."""
+
+    PROCESSED, RAW_BYTES, RAW_BYTES_UNCHECKED = 0, 1, 2
+
+    def __init__(self, data=None, path=None, format=1):
+        h = ctypes.c_void_p()
+        if path is not None:
+            check(lib().midnight_hip_srs_load_file(ctypes.byref(h), os.fsencode(path), format))
+        else:
+            data = bytes(data)
+            check(lib().midnight_hip_srs_load(ctypes.byref(h), data, len(data), format))
+        self.handle = h
+        k = ctypes.c_uint32()
+        g, gl = ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().midnight_hip_srs_info(h, ctypes.byref(k), ctypes.byref(g), ctypes.byref(gl)))
+        self.k = k.value
+        self.g = MsmContext(None, _handle=g, _npoints=1 << self.k, _owned=False)
+        self.g_lagrange = MsmContext(None, _handle=gl, _npoints=1 << self.k, _owned=False)
+
+    def g2(self):
+        ln = c_size()
+        buf = ctypes.create_string_buffer(1024)
+        check(lib().midnight_hip_srs_g2(self.handle, buf, 1024, ctypes.byref(ln)))
+        return buf.raw[:ln.value]
+
+    def free(self):
+        if self.handle and self.handle.value:
+            lib().midnight_hip_srs_free(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- proof pipeline

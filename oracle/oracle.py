@@ -89,6 +89,7 @@ def _declare(L):
         "orc_msm_pippenger": (V, [u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
         "orc_msm_best": (V, [u64p, u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
         "orc_fp_inv_vartime": (V, [u64p, u64p]),
+        "orc_fp_sqrt": (ctypes.c_int, [u64p, u64p]),
         "orc_booth_index": (ctypes.c_int32, [ctypes.c_size_t, ctypes.c_size_t, u8p]),
         "orc_srs_powers": (V, [u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
         "orc_g1_fixed_base_batch": (V, [u64p, u64p, ctypes.c_size_t, ctypes.c_int]),
@@ -365,6 +366,19 @@ def compress(p_aff):
     out = np.zeros(48, dtype=np.uint8)
     lib().orc_g1_compress(B(out), P(np.ascontiguousarray(p_aff)))
     return bytes(out)
+
+
+def to_uncompressed(p_aff):
+    out = np.zeros(96, dtype=np.uint8)
+    lib().orc_g1_to_uncompressed(B(out), P(np.ascontiguousarray(p_aff)))
+    return bytes(out)
+
+
+def fp_sqrt(a):
+    """(root, is_square) of a Montgomery Fp element"""
+    out = np.zeros(6, dtype=np.uint64)
+    ok = lib().orc_fp_sqrt(P(out), P(np.ascontiguousarray(a, dtype=np.uint64)))
+    return out, bool(ok)
 
 
 def from_uncompressed(buf):

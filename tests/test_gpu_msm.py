@@ -317,3 +317,60 @@ def test_msm_short_on_large_precomputed_context(H, O):
     ctx.free()
     dv.free()
     dv_t.free()
+
+
+def _reference_points(O, n):
+    """points (i + 1) G, as the reference's ordering tests use"""
+    ts = np.stack([O.fr_from_int(i + 1) for i in range(n)])
+    return ts, points_for(O, ts)
+
+
+@pytest.mark.parametrize("ragged,sparse", [(False, False), (True, False), (True, True)])
+def test_device_msm_waits_for_delayed_input_producer(H, O, ragged, sparse):
+    """lib.rs:5110-5199: the scalars' producer (a scaling by 2) is held back
+    100 ms on the lane's stream and has not run when the MSM is issued; the
+    MSM must see the doubled scalars."""
+    n, m = 4096, 3
+    ts, pts = _reference_points(O, n)
+    ctx = H.MsmContext(pts)
+    sc = np.stack([O.fr_from_int(0 if sparse and i % 7 == 0 else i * 31 + 7) for i in range(n * m)])
+    dv = H.DeviceVec.upload(sc)
+    H.test_delayed_scale(dv, O.fr_from_int(2), 100)
+    start = 17 if ragged else 0
+    if ragged:
+        slices = []
+        for b in range(m):
+            part = sc[b * n + start:(b + 1) * n]
+            slices.append((b * n + start, start, n - start, int(np.count_nonzero(part.any(axis=1)))))
+        got = ctx.device_ragged(dv, slices)
+    else:
+        got = ctx.device_batch(dv, 0, n, m)
+    two = O.fr_from_int(2)
+    for b in range(m):
+        doubled = np.stack([O.fr_op("mul", x, two) for x in sc[b * n + start:(b + 1) * n]])
+        assert O.g1_eq(got[b], closed_form(O, ts[start:], doubled)), (ragged, sparse, b)
+    ctx.free()
+    dv.free()
+
+
+def test_scrambled_msm_waits_for_delayed_mask_producer(H, O):
+    """lib.rs:5088-5108: the masked scalars start zeroed and their producer
+    is held back 100 ms; the scrambled MSM must return MSM(P, s + mask)."""
+    n, m = 4096, 3
+    ts, pts = _reference_points(O, n)
+    ctx = H.MsmContext(pts)
+    sc = O.random_fr(n * m, 61)
+    mask = O.random_fr(n, 62)
+    dv = H.DeviceVec.upload(sc)
+    mdv = H.DeviceVec.upload(mask)
+    try:
+        H.test_mask_delay(100)
+        got = ctx.device_scrambled_batch(dv, 0, n, m, mdv)
+    finally:
+        H.test_mask_delay(0)
+    for b in range(m):
+        s_plus = np.stack([O.fr_op("add", sc[b * n + j], mask[j]) for j in range(n)])
+        assert O.g1_eq(got[b], closed_form(O, ts, s_plus)), b
+    ctx.free()
+    dv.free()
+    mdv.free()

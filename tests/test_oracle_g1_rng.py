@@ -214,3 +214,16 @@ def test_seed_from_u64_expansion(O):
             st = lib.orc_pcg32_advance(st, inc)
             seed += lib.orc_pcg32_output(st).to_bytes(4, "little")
         assert np.array_equal(O.ChaCha(seed=s).random_fr(5), O.ChaCha(seed_bytes=seed).random_fr(5)), s
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SRS), reason="reference tree not present")
+def test_srs_file_rebuilt_from_fixture(O):
+    """g, g_lagrange (uncompressed) and the G2 tail of the fixtures rebuild
+    the reference's data file byte for byte."""
+    k, g, gl = O.load_srs_fixture(os.path.join(GOLDEN, "srs_2p14.g1c"))
+    with open(os.path.join(GOLDEN, "srs_2p14_g2.bin"), "rb") as f:
+        tail = f.read()
+    rebuilt = k.to_bytes(4, "little") + b"".join(O.to_uncompressed(p) for p in g) + \
+        b"".join(O.to_uncompressed(p) for p in gl) + tail
+    with open(REF_SRS, "rb") as f:
+        assert f.read() == rebuilt
