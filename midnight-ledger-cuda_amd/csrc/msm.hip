@@ -42,6 +42,7 @@
 #include "msm.hpp"
 #include "msm_context.hpp"
 #include "profile.hpp"
+#include "rngdev.hpp"
 #include "runtime.hpp"
 
 namespace mh {
@@ -758,26 +759,109 @@ inline uint32_t fill_group(uint64_t work, uint32_t cap)
 
 }  // namespace
 
+namespace {
+
+// the table window of the fixed rule: c = log2 n - 4, at most 16: wider
+// windows cut the entries (n * ceil(256/c)) but multiply the digit passes of
+// the LDS sort (2^(c-1) / 16384 key ranges) and the bucket reduction; at
+// 2^24 points c = 20 spent as long sorting and reducing as accumulating.
+uint32_t rule_table_window(size_t n)
+{
+    int lg = 0;
+    while ((size_t(1) << lg) < n) ++lg;
+    return (uint32_t)std::min(16, std::max(10, lg - 4));
+}
+
+// the per-window table of window c into `table`; false when its entries
+// would not fit 31-bit indices
+bool build_table(const MsmContext &ctx, uint32_t c, DevBuf<uint32_t> &table, hipStream_t st)
+{
+    const size_t n = ctx.npoints;
+    const uint32_t nwin = (256 + c - 1) / c;
+    if ((uint64_t)nwin * n >= (1ull << 31)) return false;   // entry indices are 31-bit
+    table.alloc((size_t)nwin * n * kRow28);   // Affine28 rows
+    hipLaunchKernelGGL(k_precompute, dim3(grid1(n)), dim3(kBlock), 0, st, ctx.points.get(), table.get(),
+                       (uint32_t)n, c, nwin);
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipStreamSynchronize(st));
+    return true;
+}
+
+// milliseconds of one batch with the table `pc` (best of two enqueues)
+float time_table(MsmEngine &eng, const MsmContext &ctx, const std::vector<MsmJob> &jobs, const PrecompTable &pc,
+                 hipStream_t st)
+{
+    eng.prepare(ctx.points.get(), ctx.npoints, jobs, 0, st, &pc);
+    hipEvent_t a, b;
+    MH_HIP(hipEventCreate(&a));
+    MH_HIP(hipEventCreate(&b));
+    float best = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {
+        MH_HIP(hipEventRecord(a, st));
+        eng.enqueue(st, nullptr);
+        MH_HIP(hipEventRecord(b, st));
+        MH_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        MH_HIP(hipEventElapsedTime(&ms, a, b));
+        best = std::min(best, ms);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return best;
+}
+
+}  // namespace
+
+// Builds the context's per-window table.  A config with window_bits set
+// gets that window (10..16; the reference's 17 and 18 are capped at 16, the
+// widest key range of the LDS digit sort).  Otherwise contexts of 2^16
+// points or more are autotuned here, once: one MSM of uniform scalars over
+// the whole point set is timed with the rule's window and one bit either
+// side (within 10..16), each with accumulation chunks of 64 and 32 entries,
+// and the fastest table and chunk length are kept (the reference's
+// counterpart is its hand-measured per-k table, lib.rs:159-203).  MSM
+// results do not depend on the choice.
 void build_precomp(MsmContext &ctx, hipStream_t st)
 {
     const size_t n = ctx.npoints;
     if (n < (size_t(1) << 10)) return;
-    int lg = 0;
-    while ((size_t(1) << lg) < n) ++lg;
-    // c = log2 n - 4, at most 16: wider windows cut the entries
-    // (n * ceil(256/c)) but multiply the digit passes of the LDS sort
-    // (2^(c-1) / 16384 key ranges) and the bucket reduction; at 2^24 points
-    // c = 20 spent as long sorting and reducing as accumulating.
-    uint32_t c = (uint32_t)std::min(16, std::max(10, lg - 4));
-    uint32_t nwin = (256 + c - 1) / c;
-    if ((uint64_t)nwin * n >= (1ull << 31)) return;   // entry indices are 31-bit
-    ctx.table.alloc((size_t)nwin * n * kRow28);   // Affine28 rows
-    hipLaunchKernelGGL(k_precompute, dim3(grid1(n)), dim3(kBlock), 0, st, ctx.points.get(), ctx.table.get(),
-                       (uint32_t)n, c, nwin);
-    MH_HIP(hipGetLastError());
+    const uint32_t wb = ctx.window_bits;
+    const uint32_t c0 = wb ? std::min<uint32_t>(16, std::max<uint32_t>(10, wb)) : rule_table_window(n);
+    if (wb || n < (size_t(1) << 16)) {
+        if (build_table(ctx, c0, ctx.table, st)) {
+            ctx.pc_c = c0;
+            ctx.pc_nwin = (256 + c0 - 1) / c0;
+        }
+        return;
+    }
+    DevBuf<Fr> sc(n);
+    uint8_t seed[32] = {0};
+    seed[0] = 0x6d;   // any fixed seed: uniform scalars
+    random_fr_device(sc.get(), n, seed, st);
+    const std::vector<MsmJob> jobs{MsmJob{sc.get(), 0, n}};
+    MsmEngine eng;
+    float best = 1e30f;
+    for (uint32_t c = std::max<uint32_t>(10, c0 - 1); c <= std::min<uint32_t>(16, c0 + 1); ++c) {
+        DevBuf<uint32_t> table;
+        if (!build_table(ctx, c, table, st)) continue;
+        const uint32_t nwin = (256 + c - 1) / c;
+        const PrecompTable pc{table.get(), n, c, nwin};
+        bool keep = false;
+        for (uint32_t L : {64u, 32u}) {
+            eng.chunk_len = L;
+            const float ms = time_table(eng, ctx, jobs, pc, st);
+            if (ms < best) {
+                best = ms;
+                ctx.chunk_len = L;
+                ctx.pc_c = c;
+                ctx.pc_nwin = nwin;
+                keep = true;
+            }
+        }
+        if (keep) ctx.table = std::move(table);
+    }
+    ctx.tuned_ms = best;
     MH_HIP(hipStreamSynchronize(st));
-    ctx.pc_c = c;
-    ctx.pc_nwin = nwin;
 }
 
 // ---------------------------------------------------------------- engine
@@ -1317,14 +1401,9 @@ bool valid_config(const RawMsmConfig &c)
            c.batch_lanes >= 1 && c.batch_lanes <= 16 && c.reuse_event <= 1 && c.device_reduce <= 3;
 }
 
-// The reference's window_bits tunes its sort; this engine picks its own
-// window from the MSM size (window_bits > 16 would overflow a 32-bit
-// histogram of signed digits per point, so the engine caps it).
-uint32_t engine_window(const RawMsmConfig &c)
-{
-    (void)c;
-    return 0;
-}
+// Plain windows follow the window_bits of a caller's config when it is set
+// (0 lets the engine pick its width from the MSM size).
+uint32_t engine_window(const MsmContext &ctx) { return ctx.window_bits; }
 
 constexpr size_t kMaxBatchPairs = size_t(1) << 26;   // bounds entry buffers (~ 20 windows x 4 B x this)
 
@@ -1344,11 +1423,12 @@ void run_batched(MsmContext &ctx, MsmEngine &engine, const std::vector<MsmJob> &
         // the table's one bucket set has 2^(c-1) buckets; MSMs much shorter
         // than that (a small circuit over a large SRS) spend less on plain
         // windows sized for their own length
+        engine.chunk_len = ctx.chunk_len;
         if (ctx.pc_nwin && part_max >= (size_t(1) << (ctx.pc_c - 1))) {
             PrecompTable pc{ctx.table.get(), ctx.npoints, ctx.pc_c, ctx.pc_nwin};
             engine.run(ctx.points.get(), ctx.npoints, part, 0, out + i, st, &pc);
         } else {
-            engine.run(ctx.points.get(), ctx.npoints, part, engine_window(ctx.config), out + i, st);
+            engine.run(ctx.points.get(), ctx.npoints, part, engine_window(ctx), out + i, st);
         }
         i = j;
     }
@@ -1369,7 +1449,7 @@ namespace {
 // Resident contexts precompute per-window point copies (worth it because a
 // context serves many MSMs); the one-shot entry points do not.
 RustError create_context(void **out, const RawG1Affine *points, size_t npoints, RawMsmConfig config,
-                         bool precompute)
+                         bool precompute, bool caller_config)
 {
     if (!out) return make_error((int)hipErrorInvalidValue, "out is null");
     *out = nullptr;
@@ -1379,6 +1459,7 @@ RustError create_context(void **out, const RawG1Affine *points, size_t npoints, 
     MH_TRY
     auto ctx = make_context(npoints);
     ctx->config = config;
+    if (caller_config) ctx->window_bits = config.window_bits;
     hipStream_t st = cur_stream();
     if (npoints) MH_HIP(hipMemcpyAsync(ctx->points.get(), points, npoints * 96, hipMemcpyHostToDevice, st));
     if (precompute) build_precomp(*ctx, st);
@@ -1394,14 +1475,26 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_create_with_config
                                                                                  const RawG1Affine *points,
                                                                                  size_t npoints, RawMsmConfig config)
 {
-    return create_context(out, points, npoints, config, true);
+    return create_context(out, points, npoints, config, true, true);
 }
 
 extern "C" RustError midnight_sppark_bls12_381_g1_msm_context_create(void **out, const RawG1Affine *points,
                                                                      size_t npoints)
 {
-    return midnight_sppark_bls12_381_g1_msm_context_create_with_config(out, points, npoints,
-                                                                        default_msm_config(npoints));
+    return create_context(out, points, npoints, default_msm_config(npoints), true, false);
+}
+
+// The table window, accumulation chunk bound and autotuned MSM time (ms; 0
+// when the context was not tuned) of a context.
+extern "C" RustError midnight_hip_g1_msm_context_tuning(const void *context, uint32_t *window_bits,
+                                                        uint32_t *chunk_len, float *tuned_ms)
+{
+    if (!context) return make_error((int)hipErrorInvalidValue, "context is null");
+    const MsmContext *ctx = static_cast<const MsmContext *>(context);
+    if (window_bits) *window_bits = ctx->pc_nwin ? ctx->pc_c : ctx->window_bits;
+    if (chunk_len) *chunk_len = ctx->chunk_len;
+    if (tuned_ms) *tuned_ms = ctx->tuned_ms;
+    return ok();
 }
 
 extern "C" void midnight_sppark_bls12_381_g1_msm_context_free(void *context)
@@ -1623,7 +1716,7 @@ extern "C" RustError midnight_sppark_bls12_381_g1_msm_batch(RawG1Projective *out
                                                             const RawFr *scalars, size_t npoints, size_t nmsms)
 {
     void *h = nullptr;
-    RustError e = create_context(&h, points, npoints, default_msm_config(npoints), false);
+    RustError e = create_context(&h, points, npoints, default_msm_config(npoints), false, false);
     if (e.code) return e;
     e = midnight_sppark_bls12_381_g1_msm_context_batch(out, h, scalars, npoints, nmsms);
     midnight_sppark_bls12_381_g1_msm_context_free(h);

@@ -19,6 +19,9 @@ struct MsmContext {
     // per-window copies [2^(c w)] P_i (w < pc_nwin), built once per context
     DevBuf<uint32_t> table;
     uint32_t pc_c = 0, pc_nwin = 0;
+    uint32_t window_bits = 0;  // a caller config's window (0: autotuned table / automatic plain windows)
+    uint32_t chunk_len = 64;   // accumulation chunk bound of this context's MSMs
+    float tuned_ms = 0;        // autotuned MSM time at creation (0: not tuned)
     // One engine (scratch buffers, statistics) per lane, so that MSMs of
     // different lanes over the same points run concurrently; calls of one
     // lane take turns on its engine.
@@ -39,7 +42,8 @@ struct MsmContext {
 RawMsmConfig default_msm_config(size_t npoints);
 std::unique_ptr<MsmContext> make_context(size_t npoints);
 // Builds the per-window table of a context whose points are on the device
-// (no-op for small point sets, which use plain windows).
+// (no-op for small point sets, which use plain windows): the config's
+// window_bits when set, else autotuned for contexts of 2^16 points or more.
 void build_precomp(MsmContext &ctx, hipStream_t st);
 void generator_multiples(uint32_t *out_points, const Fr *scalars, size_t n, hipStream_t st);
 

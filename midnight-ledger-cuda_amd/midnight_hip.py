@@ -208,6 +208,8 @@ def _declare(L):
         "midnight_hip_srs_info": (E, [vp, ctypes.POINTER(u32), vpp, vpp]),
         "midnight_hip_srs_g2": (E, [vp, ctypes.c_char_p, c_size, ctypes.POINTER(c_size)]),
         "midnight_hip_srs_free": (None, [vp]),
+        "midnight_hip_g1_msm_context_tuning": (E, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32),
+                                                   ctypes.POINTER(ctypes.c_float)]),
         "midnight_sppark_test_mask_delay": (None, [u32]),
         "midnight_sppark_test_delayed_scale": (E, [vp, RawFr, u32]),
         "midnight_hip_fr_device_vec_ntt_zeta_noalloc": (E, [vp, u32, ctypes.c_bool, RawFr, RawFr, RawFr]),
@@ -579,6 +581,13 @@ class MsmContext:
         out = np.zeros((n, 12), dtype=np.uint64)
         check(lib().midnight_hip_g1_msm_context_points_download(P(out), self.handle, n))
         return out
+
+    def tuning(self):
+        """(table or plain window, chunk length, autotuned ms)"""
+        w, c, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_float()
+        check(lib().midnight_hip_g1_msm_context_tuning(self.handle, ctypes.byref(w), ctypes.byref(c),
+                                                       ctypes.byref(ms)))
+        return w.value, c.value, ms.value
 
     def last_stats(self):
         e = ctypes.c_uint64()

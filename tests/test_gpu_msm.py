@@ -374,3 +374,37 @@ def test_scrambled_msm_waits_for_delayed_mask_producer(H, O):
     ctx.free()
     dv.free()
     mdv.free()
+
+
+def test_msm_context_window_bits_and_autotune(H, O):
+    """A context made without a config is autotuned at creation (window of
+    the size rule or one bit either side, chunks of 64 or 32); a caller's
+    window_bits is used as given (tables capped at 16, plain windows as
+    given, .cu:50-110 validity); every choice gives the closed-form MSM."""
+    n = 1 << 16
+    ts = O.random_fr(n, 71)
+    pts = points_for(O, ts)
+    sc = O.random_fr(2 * n, 72)
+    auto = H.MsmContext(pts)
+    w, L, ms = auto.tuning()
+    assert w in (11, 12, 13) and L in (32, 64) and ms > 0
+    want = [closed_form(O, ts, sc[i * n:(i + 1) * n]) for i in range(2)]
+    got = auto.batch(sc, n, 2)
+    assert all(O.g1_eq(a, b) for a, b in zip(got, want))
+    auto.free()
+    for wb in (10, 13, 17):
+        cfg = H.default_config(n)
+        cfg.window_bits = wb
+        ctx = H.MsmContext(pts, config=cfg)
+        w, L, ms = ctx.tuning()
+        assert w == min(wb, 16) and ms == 0
+        got = ctx.batch(sc, n, 2)
+        assert all(O.g1_eq(a, b) for a, b in zip(got, want)), wb
+        # a short MSM runs plain windows of the configured width
+        short = ctx.batch(sc[:300], 300, 1)[0]
+        assert O.g1_eq(short, closed_form(O, ts[:300], sc[:300])), wb
+        ctx.free()
+    cfg = H.default_config(n)
+    cfg.window_bits = 9                       # outside the reference's valid range
+    with pytest.raises(H.MidnightHipError):
+        H.MsmContext(pts, config=cfg)
