@@ -181,6 +181,7 @@ def bench_msm(args, H):
     tot_ms, tot_cnt = H.profile_read("msm_total")
     H.profile_enable(False)
     entries, window = ctx.last_stats()
+    tuned_w, tuned_l, tuned_ms = ctx.tuning()
     ms_step = dt / args.steps * 1e3
     value = n * m * args.steps / dt / 1e6
     # dominant kernel: bucket accumulation, 8M+2S Fp products per entry
@@ -203,7 +204,8 @@ def bench_msm(args, H):
         "data": "synthetic: points [t_i]G and %s scalars, generated on device" % ("skewed" if args.skewed else "uniform"),
         "config": {"workload": "G1 MSM, 2^%d resident points, %d MSMs per step (%s scalars)"
                                % (log_n, m, "skewed" if args.skewed else "uniform 255-bit"),
-                   "points": n, "msms_per_step": m, "window_bits": window},
+                   "points": n, "msms_per_step": m, "window_bits": window,
+                   "autotuned": {"table_window": tuned_w, "chunk_len": tuned_l, "tuning_ms": round(tuned_ms, 3)}},
         "roofline": {"bound": "valu", "kernel": "msm_accumulate",
                      "achieved": round(achieved / 1e12, 3), "peak": round(peak / 1e12, 3),
                      "peak_measured": round(peak_measured / 1e12, 3),

@@ -559,8 +559,8 @@ RustError midnight_hip_g1_msm_context_points_download(RawG1Affine *out, const vo
  * bound and autotuned MSM time in ms (0 when not tuned) of a context.
  * Contexts made without a caller config and of 2^16 points or more are
  * autotuned when created (the table window of the size rule and one bit
- * either side, chunks of 64 and 32 entries, timed on one MSM of uniform
- * scalars); with a config, a nonzero window_bits is used as given (table
+ * either side, chunks of 64 and 32 entries, timed on a batch of four MSMs
+ * of uniform scalars); with a config, a nonzero window_bits is used as given (table
  * windows capped at 16).  MSM results never depend on the choice. */
 RustError midnight_hip_g1_msm_context_tuning(const void *context, uint32_t *window_bits, uint32_t *chunk_len,
                                             float *tuned_ms);

@@ -787,26 +787,20 @@ bool build_table(const MsmContext &ctx, uint32_t c, DevBuf<uint32_t> &table, hip
     return true;
 }
 
-// milliseconds of one batch with the table `pc` (best of two enqueues)
+// milliseconds of one batch with the table `pc`, as callers run it (the
+// eager path: launches, the read-back of the fullest bucket, the host
+// doubling chains); best of two after a warm-up
 float time_table(MsmEngine &eng, const MsmContext &ctx, const std::vector<MsmJob> &jobs, const PrecompTable &pc,
                  hipStream_t st)
 {
-    eng.prepare(ctx.points.get(), ctx.npoints, jobs, 0, st, &pc);
-    hipEvent_t a, b;
-    MH_HIP(hipEventCreate(&a));
-    MH_HIP(hipEventCreate(&b));
+    std::vector<RawG1Projective> out(jobs.size());
     float best = 1e30f;
-    for (int rep = 0; rep < 2; ++rep) {
-        MH_HIP(hipEventRecord(a, st));
-        eng.enqueue(st, nullptr);
-        MH_HIP(hipEventRecord(b, st));
-        MH_HIP(hipEventSynchronize(b));
-        float ms = 0;
-        MH_HIP(hipEventElapsedTime(&ms, a, b));
-        best = std::min(best, ms);
+    for (int rep = 0; rep < 3; ++rep) {
+        const auto t0 = std::chrono::steady_clock::now();
+        eng.run(ctx.points.get(), ctx.npoints, jobs, 0, out.data(), st, &pc);
+        const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (rep) best = std::min(best, ms);
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     return best;
 }
 
@@ -815,8 +809,8 @@ float time_table(MsmEngine &eng, const MsmContext &ctx, const std::vector<MsmJob
 // Builds the context's per-window table.  A config with window_bits set
 // gets that window (10..16; the reference's 17 and 18 are capped at 16, the
 // widest key range of the LDS digit sort).  Otherwise contexts of 2^16
-// points or more are autotuned here, once: one MSM of uniform scalars over
-// the whole point set is timed with the rule's window and one bit either
+// points or more are autotuned here, once: a batch of four MSMs of uniform
+// scalars over the whole point set is timed with the rule's window and one bit either
 // side (within 10..16), each with accumulation chunks of 64 and 32 entries,
 // and the fastest table and chunk length are kept (the reference's
 // counterpart is its hand-measured per-k table, lib.rs:159-203).  MSM
@@ -838,7 +832,9 @@ void build_precomp(MsmContext &ctx, hipStream_t st)
     uint8_t seed[32] = {0};
     seed[0] = 0x6d;   // any fixed seed: uniform scalars
     random_fr_device(sc.get(), n, seed, st);
-    const std::vector<MsmJob> jobs{MsmJob{sc.get(), 0, n}};
+    // a batch of four MSMs (the batches a prover commits: advice columns,
+    // quotient pieces), which is what the chunk length trades against
+    const std::vector<MsmJob> jobs(4, MsmJob{sc.get(), 0, n});
     MsmEngine eng;
     float best = 1e30f;
     for (uint32_t c = std::max<uint32_t>(10, c0 - 1); c <= std::min<uint32_t>(16, c0 + 1); ++c) {
