@@ -129,6 +129,172 @@ __device__ __forceinline__ Fr29 fr29_mul(const Fr29 &a, const Fr29 &b)
     return r;
 }
 
+// a - b mod r for fully reduced a, b
+__device__ __forceinline__ Fr29 fr29_sub(const Fr29 &a, const Fr29 &b)
+{
+    Fr29 r;
+    int32_t bw = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int32_t x = (int32_t)a.v[j] - (int32_t)b.v[j] - bw;
+        bw = (int32_t)((uint32_t)x >> 31);
+        r.v[j] = (uint32_t)x & kM29;
+    }
+    // a borrow out of the top limb: the difference wrapped by 2^261; add r
+    const uint32_t mask = 0u - (uint32_t)bw;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const uint32_t s = r.v[j] + (r29(j) & mask) + c;
+        c = s >> 29;
+        r.v[j] = s & kM29;
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Loose values: any representative below 2r (limbs below 2^29).  Products of
+// loose values skip the final subtraction: for a, b < 2r the running value is
+// below a b / 2^261 + r < 4r^2 / 2^261 + r < 1.06 r.  Sums and differences
+// are brought back below 2r by one conditional subtraction of 2r.  A loose
+// value is fully reduced by one fr29_reduce.
+
+__device__ __forceinline__ uint32_t r2x29(int i)   // 2r in 29-bit limbs
+{
+    constexpr uint32_t R2[9] = {0x00000002u, 0x1ffffff0u, 0x1f2dff7fu, 0x16900bffu, 0x1b00aa77u,
+                                0x180809a1u, 0x0a4199ceu, 0x14ca675fu, 0x00e7db4eu};
+    return R2[i];
+}
+
+// r - 2r if r >= 2r (r < 4r, limbs below 2^29)
+__device__ __forceinline__ void fr29_reduce_2r(Fr29 &r)
+{
+    uint32_t d[9];
+    int32_t bw = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        int32_t x = (int32_t)r.v[j] - (int32_t)r2x29(j) - bw;
+        bw = (int32_t)((uint32_t)x >> 31);
+        d[j] = (uint32_t)x & kM29;
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) r.v[j] = bw ? r.v[j] : d[j];
+}
+
+// a * b / 2^261 mod r for loose a, b; loose result
+__device__ __forceinline__ Fr29 fr29_mul_loose(const Fr29 &a, const Fr29 &b)
+{
+    uint64_t t[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) t[j] = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) t[j] += (uint64_t)a.v[j] * b.v[i];
+        const uint32_t m = (0u - (uint32_t)t[0]) & kM29;
+        t[0] += m;   // r_0 = 1
+#pragma unroll
+        for (int j = 1; j < 9; ++j) t[j] += (uint64_t)m * r29(j);
+        const uint64_t carry = t[0] >> 29;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) t[j] = t[j + 1];
+        t[0] += carry;
+        t[8] = 0;
+    }
+    Fr29 r;
+    uint64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        c += t[j];
+        r.v[j] = (uint32_t)c & kM29;
+        c >>= 29;
+    }
+    return r;
+}
+
+// a + b for loose a, b; loose result
+__device__ __forceinline__ Fr29 fr29_add_loose(const Fr29 &a, const Fr29 &b)
+{
+    Fr29 r;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const uint32_t s = a.v[j] + b.v[j] + c;
+        c = s >> 29;
+        r.v[j] = s & kM29;
+    }
+    fr29_reduce_2r(r);
+    return r;
+}
+
+// a - b (as a - b + 2r) for loose a, b; loose result
+__device__ __forceinline__ Fr29 fr29_sub_loose(const Fr29 &a, const Fr29 &b)
+{
+    Fr29 r;
+    int32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int32_t s = (int32_t)a.v[j] + (int32_t)r2x29(j) - (int32_t)b.v[j] + c;
+        c = s >> 29;   // arithmetic: -1, 0 or 1
+        r.v[j] = (uint32_t)s & kM29;
+    }
+    fr29_reduce_2r(r);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// The scaled form: a value v kept as v 2^261 mod r (fully reduced, 9 limbs),
+// the form in which fr29_mul is closed: mont29(a 2^261, b 2^261) = (a b)
+// 2^261.  Sums and differences of scaled values are scaled values.  Data in
+// the usual form (v 2^256, 8 x 32 bits) enters by a multiplication by 32
+// (a 5-bit shift and a reduction) and leaves by one product with 2^256 mod r.
+
+// 2^261 mod r: the scaled form of one
+__device__ __forceinline__ Fr29 s29_one()
+{
+    constexpr uint32_t O[9] = {0x1fffffbau, 0x0000022fu, 0x1cb61180u, 0x0a4e5c00u, 0x0ee8b1a2u,
+                               0x16e6aedfu, 0x1907f8bbu, 0x0853ddf7u, 0x004d043fu};
+    Fr29 r;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) r.v[j] = O[j];
+    return r;
+}
+
+// usual form (8 x 32, fully reduced) -> scaled form: 32 x mod r
+__device__ __forceinline__ Fr29 s29_from_usual(const Fr &x)
+{
+    // V = 32 x < 32 r < 2^260: bits [29 j - 5, 29 j + 24) of x in limb j
+    Fr29 v;
+    v.v[0] = (x.v[0] << 5) & kM29;
+#pragma unroll
+    for (int j = 1; j < 9; ++j) {
+        const int bit = 29 * j - 5, q = bit >> 5, s = bit & 31;
+        const uint32_t lo = x.v[q];
+        const uint32_t hi = q + 1 < 8 ? x.v[q + 1] : 0u;
+        v.v[j] = (uint32_t)((((uint64_t)hi << 32) | lo) >> s) & kM29;
+    }
+    // q = floor(v8 M / 2^48) with M = floor(2^48 / (r8 + 1)), r8 the top
+    // limb of r: floor(V / r) - 2 <= q <= floor(V / r), so V - q r < 3 r
+    const uint32_t q = (uint32_t)(((uint64_t)v.v[8] * 0x2355091ull) >> 48);
+    int64_t carry = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int64_t t = (int64_t)v.v[j] - (int64_t)q * (int64_t)r29(j) + carry;
+        v.v[j] = (uint32_t)t & kM29;
+        carry = t >> 29;   // arithmetic: the borrow into the next limb
+    }
+    fr29_reduce(v);
+    fr29_reduce(v);
+    return v;
+}
+
+// scaled form (loose or fully reduced) -> usual form (8 x 32, fully reduced):
+// one product by 2^256 mod r
+__device__ __forceinline__ Fr s29_to_usual(const Fr29 &v)
+{
+    return fr29_to32(fr29_mul(v, fr29_from32(Fr::one())));
+}
+
 // (a w) in the usual form, for a in the usual form (8 x 32) and w stored as
 // w 2^261 mod r (8 x 32, fully reduced)
 __device__ __forceinline__ Fr fr_mul_scaled(const Fr &a, const Fr &w261)

@@ -334,6 +334,10 @@ int job_passes(const NttJob &job, PassLaunch out[2])
         post_scale = post_scale * job.scale;
         has_post = true;
     }
+    if (job.post_mul) {
+        post_scale = post_scale * job.mul;
+        has_post = true;
+    }
 
     PassArgs base{};
     base.roots = T.roots[dir];

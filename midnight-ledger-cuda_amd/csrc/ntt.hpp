@@ -26,6 +26,8 @@ struct NttJob {
     bool post_zeta = false;      // multiply by scale * {1, first, second}[i % 3]
     HostFr scale = HostFr::one();
     HostFr first = HostFr::one(), second = HostFr::one();
+    bool post_mul = false;       // also multiply every output by `mul`
+    HostFr mul = HostFr::one();
 };
 
 void ntt_run(const NttJob &job, hipStream_t st);

@@ -33,8 +33,10 @@
 
 #include "devvec.hpp"
 #include "fields.hpp"
+#include "fr29.hpp"
 #include "grandproduct.hpp"
 #include "lookup.hpp"
+#include "numerator.hpp"
 #include "host_fr.hpp"
 #include "profile.hpp"
 #include "rowpow.hpp"
@@ -53,24 +55,32 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kMaxSlots = 256;       // MAX_GRAPH_SCRATCH_SLOTS (midnight-gpu-prover expression.rs:23)
-constexpr uint32_t kLdsSlots = 6;         // 6 x 32 B x 256 lanes = 48 KB of LDS per block
+constexpr uint32_t kLdsSlots = 4;         // 4 x 36 B x 256 lanes = 36 KB of LDS per block
 constexpr uint32_t kGlobalSlotBlocks = 1024;
 
 struct DevGraph {
-    const Fr *constants;
+    const Fr *constants;   // scaled form (c 2^261 mod r), converted when the graph is uploaded
     const int32_t *rotations;
     const RawCalculation *calcs;
     const RawValueSource *parts;
     uint32_t constants_len, rotations_len, calcs_len, parts_len, num_slots;
 };
 
+// Which inputs of an evaluation are already stored in the scaled form
+// (v 2^261 mod r); the others are in the usual form and converted as they
+// are loaded.
+enum : uint32_t {
+    kScFixed = 1, kScAdvice = 2, kScInstance = 4, kScProducts = 8, kScCosets = 16, kScLagrange = 32,
+    kScLookup = 64, kScTrash = 128, kScInitial = 256
+};
+
 struct NumArgs {
-    Fr *out;
+    Fr *out;                  // usual form
     const Fr *initial;
     size_t len;
-    const Fr *fixed, *advice, *instance, *challenges;
+    const Fr *fixed, *advice, *instance, *challenges;   // challenges: scaled
     uint32_t fixed_cols, advice_cols, instance_cols, challenge_len;
-    Fr beta, gamma, theta, trash, y, delta_start, delta_const;
+    Fr beta, gamma, theta, trash, y, delta_start, delta_const;   // scaled
     int32_t rot_scale, last_rotation;
     uint32_t chunk_len, run_custom;
     const Fr *l0, *l_last, *l_active;
@@ -78,19 +88,46 @@ struct NumArgs {
     const RawColumnRef *perm_columns;
     const Fr *perm_cosets;
     uint32_t perm_sets, perm_cols;
-    const Fr *omega_tab;
+    const Fr *omega_tab;      // scaled
     uint32_t omega_levels;
     uint32_t lookup_count, trash_count;
     const Fr *lookup_products, *lookup_inputs, *lookup_tables;
     const Fr *trash_cosets;
     const uint32_t *trash_sel;
     const DevGraph *graphs;   // [custom, lookup 0.., trash 0..]
-    uint32_t *slots_global;   // [slot * 8 + word][slot_stride] when not in LDS
+    uint32_t *slots_global;   // [slot * 9 + limb][slot_stride] when not in LDS
     size_t slot_stride;
     uint32_t graph_count;     // graphs evaluated by k_eval_graphs
+    uint32_t scaled;          // kSc* flags
 };
 
 // ---------------------------------------------------------------- device
+//
+// Every value inside the kernels is in the scaled form of fr29.hpp: nine
+// 29-bit limbs holding v 2^261 mod r, loose (any representative below 2r),
+// so that each product is one radix-2^29 Montgomery product (v_mad_u64_u32
+// on 64-bit accumulators, no carry chains, no final subtraction) and stays
+// in that form.  Results leave fully reduced in the usual form, so they are
+// limb-identical to the reference's.
+
+using S = Fr29;
+
+__device__ __forceinline__ S s_zero()
+{
+    S r;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) r.v[j] = 0;
+    return r;
+}
+__device__ __forceinline__ S s_add(const S &a, const S &b) { return fr29_add_loose(a, b); }
+__device__ __forceinline__ S s_sub(const S &a, const S &b) { return fr29_sub_loose(a, b); }
+__device__ __forceinline__ S s_mul(const S &a, const S &b) { return fr29_mul_loose(a, b); }
+__device__ __forceinline__ S s_const(const Fr &c) { return fr29_from32(c); }   // already scaled
+__device__ __forceinline__ S s_load(const Fr *p, bool scaled)
+{
+    const Fr x = load_fr(p);
+    return scaled ? fr29_from32(x) : s29_from_usual(x);
+}
 
 __device__ __forceinline__ size_t rotated(size_t idx, int32_t rot, int32_t scale, size_t len)
 {
@@ -102,15 +139,26 @@ __device__ __forceinline__ size_t rotated(size_t idx, int32_t rot, int32_t scale
     return (size_t)s;
 }
 
-__device__ __forceinline__ const Fr *column_base(uint32_t kind, uint32_t index, const NumArgs &a)
+// column `index` of kind (0 fixed, 1 advice, 2 instance), or nullptr; sets
+// whether that kind is stored scaled
+__device__ __forceinline__ const Fr *column_base(uint32_t kind, uint32_t index, const NumArgs &a, bool &scaled)
 {
-    if (kind == 0 && index < a.fixed_cols) return a.fixed + (size_t)index * a.len;
-    if (kind == 1 && index < a.advice_cols) return a.advice + (size_t)index * a.len;
-    if (kind == 2 && index < a.instance_cols) return a.instance + (size_t)index * a.len;
+    if (kind == 0 && index < a.fixed_cols) {
+        scaled = a.scaled & kScFixed;
+        return a.fixed + (size_t)index * a.len;
+    }
+    if (kind == 1 && index < a.advice_cols) {
+        scaled = a.scaled & kScAdvice;
+        return a.advice + (size_t)index * a.len;
+    }
+    if (kind == 2 && index < a.instance_cols) {
+        scaled = a.scaled & kScInstance;
+        return a.instance + (size_t)index * a.len;
+    }
     return nullptr;
 }
 
-// Per-lane slot file: word w of slot s at [(s * 8 + w) * stride + lane].
+// Per-lane slot file: limb w of slot s at [(s * 9 + w) * stride + lane].
 // Slots<true> addresses the block's dynamic LDS (stride = kBlock lanes),
 // Slots<false> the global slot file (stride = lanes of the whole grid).
 extern __shared__ uint32_t lds_slots[];
@@ -121,17 +169,17 @@ struct Slots;
 template <>
 struct Slots<true> {
     uint32_t lane;
-    __device__ __forceinline__ Fr load(uint32_t s) const
+    __device__ __forceinline__ S load(uint32_t s) const
     {
-        Fr r;
+        S r;
 #pragma unroll
-        for (int w = 0; w < 8; ++w) r.v[w] = lds_slots[(s * 8 + w) * kBlock + lane];
+        for (int w = 0; w < 9; ++w) r.v[w] = lds_slots[(s * 9 + w) * kBlock + lane];
         return r;
     }
-    __device__ __forceinline__ void store(uint32_t s, const Fr &v) const
+    __device__ __forceinline__ void store(uint32_t s, const S &v) const
     {
 #pragma unroll
-        for (int w = 0; w < 8; ++w) lds_slots[(s * 8 + w) * kBlock + lane] = v.v[w];
+        for (int w = 0; w < 9; ++w) lds_slots[(s * 9 + w) * kBlock + lane] = v.v[w];
     }
 };
 
@@ -140,71 +188,80 @@ struct Slots<false> {
     uint32_t *base;
     size_t stride;
     size_t lane;
-    __device__ __forceinline__ Fr load(uint32_t s) const
+    __device__ __forceinline__ S load(uint32_t s) const
     {
-        Fr r;
+        S r;
 #pragma unroll
-        for (int w = 0; w < 8; ++w) r.v[w] = base[((size_t)s * 8 + w) * stride + lane];
+        for (int w = 0; w < 9; ++w) r.v[w] = base[((size_t)s * 9 + w) * stride + lane];
         return r;
     }
-    __device__ __forceinline__ void store(uint32_t s, const Fr &v) const
+    __device__ __forceinline__ void store(uint32_t s, const S &v) const
     {
 #pragma unroll
-        for (int w = 0; w < 8; ++w) base[((size_t)s * 8 + w) * stride + lane] = v.v[w];
+        for (int w = 0; w < 9; ++w) base[((size_t)s * 9 + w) * stride + lane] = v.v[w];
     }
 };
 
 template <bool kLds>
-__device__ __forceinline__ Fr source_value(const RawValueSource &s, const DevGraph &g, size_t idx, const NumArgs &a,
-                           const Slots<kLds> &sl)
+__device__ __forceinline__ S source_value(const RawValueSource &s, const DevGraph &g, size_t idx, const NumArgs &a,
+                                          const Slots<kLds> &sl)
 {
     switch (s.kind) {
-    case 0: return s.a < g.constants_len ? load_fr(g.constants + s.a) : Fr::zero();
-    case 1: return s.a < g.num_slots ? sl.load(s.a) : Fr::zero();
+    case 0: return s.a < g.constants_len ? s_const(load_fr(g.constants + s.a)) : s_zero();
+    case 1: return s.a < g.num_slots ? sl.load(s.a) : s_zero();
     case 2:
     case 3:
     case 4: {
-        const Fr *base = column_base(s.kind - 2, s.a, a);
-        if (!base || s.b >= g.rotations_len) return Fr::zero();
-        return load_fr(base + rotated(idx, g.rotations[s.b], a.rot_scale, a.len));
+        bool scaled = false;
+        const Fr *base = column_base(s.kind - 2, s.a, a, scaled);
+        if (!base || s.b >= g.rotations_len) return s_zero();
+        return s_load(base + rotated(idx, g.rotations[s.b], a.rot_scale, a.len), scaled);
     }
-    case 5: return s.a < a.challenge_len ? load_fr(a.challenges + s.a) : Fr::zero();
-    case 6: return a.beta;
-    case 7: return a.gamma;
-    case 8: return a.theta;
-    case 9: return a.trash;
-    case 10: return a.y;
-    default: return Fr::zero();   // 11: previous value, zero in every evaluation
+    case 5: return s.a < a.challenge_len ? s_const(load_fr(a.challenges + s.a)) : s_zero();
+    case 6: return s_const(a.beta);
+    case 7: return s_const(a.gamma);
+    case 8: return s_const(a.theta);
+    case 9: return s_const(a.trash);
+    case 10: return s_const(a.y);
+    default: return s_zero();   // 11: previous value, zero in every evaluation
     }
 }
 
 template <bool kLds>
-__device__ __forceinline__ Fr eval_graph(const DevGraph &g, size_t idx, const NumArgs &a, const Slots<kLds> &sl)
+__device__ __forceinline__ S eval_graph(const DevGraph &g, size_t idx, const NumArgs &a, const Slots<kLds> &sl)
 {
-    for (uint32_t s = 0; s < g.num_slots; ++s) sl.store(s, Fr::zero());
-    Fr last = Fr::zero();
+    for (uint32_t s = 0; s < g.num_slots; ++s) sl.store(s, s_zero());
+    S last = s_zero();
     for (uint32_t c = 0; c < g.calcs_len; ++c) {
         const RawCalculation calc = g.calcs[c];
-        Fr v;
+        S v;
         switch (calc.op) {
-        case 0: v = source_value(calc.lhs, g, idx, a, sl) + source_value(calc.rhs, g, idx, a, sl); break;
-        case 1: v = source_value(calc.lhs, g, idx, a, sl) - source_value(calc.rhs, g, idx, a, sl); break;
-        case 2: v = source_value(calc.lhs, g, idx, a, sl) * source_value(calc.rhs, g, idx, a, sl); break;
-        case 3: v = sqr<FrCfg>(source_value(calc.lhs, g, idx, a, sl)); break;
-        case 4: v = dbl<FrCfg>(source_value(calc.lhs, g, idx, a, sl)); break;
-        case 5: v = neg<FrCfg>(source_value(calc.lhs, g, idx, a, sl)); break;
+        case 0: v = s_add(source_value(calc.lhs, g, idx, a, sl), source_value(calc.rhs, g, idx, a, sl)); break;
+        case 1: v = s_sub(source_value(calc.lhs, g, idx, a, sl), source_value(calc.rhs, g, idx, a, sl)); break;
+        case 2: v = s_mul(source_value(calc.lhs, g, idx, a, sl), source_value(calc.rhs, g, idx, a, sl)); break;
+        case 3: {
+            const S x = source_value(calc.lhs, g, idx, a, sl);
+            v = s_mul(x, x);
+            break;
+        }
+        case 4: {
+            const S x = source_value(calc.lhs, g, idx, a, sl);
+            v = s_add(x, x);
+            break;
+        }
+        case 5: v = s_sub(s_zero(), source_value(calc.lhs, g, idx, a, sl)); break;
         case 6: {
-            const Fr f = source_value(calc.rhs, g, idx, a, sl);
+            const S f = source_value(calc.rhs, g, idx, a, sl);
             v = source_value(calc.lhs, g, idx, a, sl);
             for (uint32_t p = 0; p < calc.parts_len; ++p) {
                 const uint32_t pi = calc.parts_offset + p;
-                const Fr part = pi < g.parts_len ? source_value(g.parts[pi], g, idx, a, sl) : Fr::zero();
-                v = v * f + part;
+                const S part = pi < g.parts_len ? source_value(g.parts[pi], g, idx, a, sl) : s_zero();
+                v = s_add(s_mul(v, f), part);
             }
             break;
         }
         case 7: v = source_value(calc.lhs, g, idx, a, sl); break;
-        default: v = Fr::zero(); break;
+        default: v = s_zero(); break;
         }
         if (calc.target < g.num_slots) sl.store(calc.target, v);
         last = v;
@@ -212,50 +269,58 @@ __device__ __forceinline__ Fr eval_graph(const DevGraph &g, size_t idx, const Nu
     return last;
 }
 
-__device__ __forceinline__ Fr omega_pow(size_t idx, const NumArgs &a)
+// omega^idx from the scaled two- or three-level table (rowpow.hpp layout)
+__device__ __forceinline__ S omega_pow(size_t idx, const NumArgs &a)
 {
-    return omega_pow_tab(idx, a.omega_tab, a.omega_levels);
+    S r = s_const(load_fr(a.omega_tab + (idx & (kOmegaTab - 1))));
+    for (uint32_t l = 1; l < a.omega_levels; ++l)
+        r = s_mul(r, s_const(load_fr(a.omega_tab + l * kOmegaTab + ((idx >> (kOmegaBits * l)) & (kOmegaTab - 1)))));
+    return r;
 }
 
 // Permutation terms (numerator_permutation_kernel, bls12_381_g1_msm.cu:674-755;
 // evaluation.rs:3283-3352).
-__device__ __forceinline__ Fr permutation_terms(Fr v, size_t idx, const NumArgs &a)
+__device__ __forceinline__ S permutation_terms(S v, size_t idx, const NumArgs &a)
 {
     const size_t len = a.len;
     const size_t r_next = rotated(idx, 1, a.rot_scale, len);
     const size_t r_last = rotated(idx, a.last_rotation, a.rot_scale, len);
-    const Fr one = Fr::one();
-    const Fr l0 = load_fr(a.l0 + idx);
+    const bool zs = a.scaled & kScProducts, ls = a.scaled & kScLagrange, cs = a.scaled & kScCosets;
+    const S one = s29_one();
+    const S y = s_const(a.y);
+    const S l0 = s_load(a.l0 + idx, ls);
     const Fr *first = a.perm_products;
     const Fr *last = a.perm_products + (size_t)(a.perm_sets - 1) * len;
 
-    v = v * a.y + (one - load_fr(first + idx)) * l0;
-    const Fr zl = load_fr(last + idx);
-    v = v * a.y + (sqr<FrCfg>(zl) - zl) * load_fr(a.l_last + idx);
+    v = s_add(s_mul(v, y), s_mul(s_sub(one, s_load(first + idx, zs)), l0));
+    const S zl = s_load(last + idx, zs);
+    v = s_add(s_mul(v, y), s_mul(s_sub(s_mul(zl, zl), zl), s_load(a.l_last + idx, ls)));
     for (uint32_t s = 1; s < a.perm_sets; ++s) {
-        const Fr cur = load_fr(a.perm_products + (size_t)s * len + idx);
-        const Fr prev = load_fr(a.perm_products + (size_t)(s - 1) * len + r_last);
-        v = v * a.y + (cur - prev) * l0;
+        const S cur = s_load(a.perm_products + (size_t)s * len + idx, zs);
+        const S prev = s_load(a.perm_products + (size_t)(s - 1) * len + r_last, zs);
+        v = s_add(s_mul(v, y), s_mul(s_sub(cur, prev), l0));
     }
 
-    const Fr l_active = load_fr(a.l_active + idx);
-    Fr delta = a.delta_start * omega_pow(idx, a);
+    const S l_active = s_load(a.l_active + idx, ls);
+    const S beta = s_const(a.beta), gamma = s_const(a.gamma), dconst = s_const(a.delta_const);
+    S delta = s_mul(s_const(a.delta_start), omega_pow(idx, a));
     for (uint32_t s = 0; s < a.perm_sets; ++s) {
         const Fr *z = a.perm_products + (size_t)s * len;
-        Fr left = load_fr(z + r_next);
-        Fr right = load_fr(z + idx);
+        S left = s_load(z + r_next, zs);
+        S right = s_load(z + idx, zs);
         const uint32_t base = s * a.chunk_len;
         for (uint32_t j = 0; j < a.chunk_len && base + j < a.perm_cols; ++j) {
             const RawColumnRef c = a.perm_columns[base + j];
-            const Fr *col = column_base(c.kind, c.index, a);
+            bool scaled = false;
+            const Fr *col = column_base(c.kind, c.index, a, scaled);
             if (!col) continue;
-            const Fr x = load_fr(col + idx) + a.gamma;
-            const Fr sigma = load_fr(a.perm_cosets + (size_t)(base + j) * len + idx);
-            left = left * (x + a.beta * sigma);
-            right = right * (x + delta);
-            delta = delta * a.delta_const;
+            const S x = s_add(s_load(col + idx, scaled), gamma);
+            const S sigma = s_load(a.perm_cosets + (size_t)(base + j) * len + idx, cs);
+            left = s_mul(left, s_add(x, s_mul(beta, sigma)));
+            right = s_mul(right, s_add(x, delta));
+            delta = s_mul(delta, dconst);
         }
-        v = v * a.y + (left - right) * l_active;
+        v = s_add(s_mul(v, y), s_mul(s_sub(left, right), l_active));
     }
     return v;
 }
@@ -263,29 +328,32 @@ __device__ __forceinline__ Fr permutation_terms(Fr v, size_t idx, const NumArgs 
 // Lookup terms (numerator_lookup_kernel, bls12_381_g1_msm.cu:758-815;
 // evaluation.rs:3405-3466).
 template <bool kLds>
-__device__ __forceinline__ Fr lookup_terms(Fr v, size_t idx, const NumArgs &a, const Slots<kLds> &sl)
+__device__ __forceinline__ S lookup_terms(S v, size_t idx, const NumArgs &a, const Slots<kLds> &sl)
 {
     const size_t len = a.len;
     const size_t r_next = rotated(idx, 1, a.rot_scale, len);
     const size_t r_prev = rotated(idx, -1, a.rot_scale, len);
-    const Fr one = Fr::one();
-    const Fr l0 = load_fr(a.l0 + idx);
-    const Fr l_last = load_fr(a.l_last + idx);
-    const Fr l_active = load_fr(a.l_active + idx);
+    const bool ks = a.scaled & kScLookup, ls = a.scaled & kScLagrange;
+    const S one = s29_one();
+    const S y = s_const(a.y), beta = s_const(a.beta), gamma = s_const(a.gamma);
+    const S l0 = s_load(a.l0 + idx, ls);
+    const S l_last = s_load(a.l_last + idx, ls);
+    const S l_active = s_load(a.l_active + idx, ls);
     for (uint32_t n = 0; n < a.lookup_count; ++n) {
-        const Fr table_value = eval_graph(a.graphs[1 + n], idx, a, sl);
+        const S table_value = eval_graph(a.graphs[1 + n], idx, a, sl);
         const Fr *z = a.lookup_products + (size_t)n * len;
         const Fr *ai = a.lookup_inputs + (size_t)n * len;
         const Fr *st = a.lookup_tables + (size_t)n * len;
-        const Fr zi = load_fr(z + idx);
-        const Fr a_i = load_fr(ai + idx);
-        const Fr s_i = load_fr(st + idx);
-        const Fr a_minus_s = a_i - s_i;
-        v = v * a.y + (one - zi) * l0;
-        v = v * a.y + (sqr<FrCfg>(zi) - zi) * l_last;
-        v = v * a.y + (load_fr(z + r_next) * (a_i + a.beta) * (s_i + a.gamma) - zi * table_value) * l_active;
-        v = v * a.y + a_minus_s * l0;
-        v = v * a.y + a_minus_s * (a_i - load_fr(ai + r_prev)) * l_active;
+        const S zi = s_load(z + idx, ks);
+        const S a_i = s_load(ai + idx, ks);
+        const S s_i = s_load(st + idx, ks);
+        const S a_minus_s = s_sub(a_i, s_i);
+        v = s_add(s_mul(v, y), s_mul(s_sub(one, zi), l0));
+        v = s_add(s_mul(v, y), s_mul(s_sub(s_mul(zi, zi), zi), l_last));
+        const S lhs = s_mul(s_mul(s_load(z + r_next, ks), s_add(a_i, beta)), s_add(s_i, gamma));
+        v = s_add(s_mul(v, y), s_mul(s_sub(lhs, s_mul(zi, table_value)), l_active));
+        v = s_add(s_mul(v, y), s_mul(a_minus_s, l0));
+        v = s_add(s_mul(v, y), s_mul(s_mul(a_minus_s, s_sub(a_i, s_load(ai + r_prev, ks))), l_active));
     }
     return v;
 }
@@ -306,23 +374,30 @@ __global__ __launch_bounds__(kBlock, 4) void k_numerator(NumArgs a)
     const Slots<kLds> sl = slot_file<kLds>(a);
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < a.len; idx += stride) {
-        Fr v = a.run_custom ? eval_graph(a.graphs[0], idx, a, sl) : load_fr(a.initial + idx);
+        S v = a.run_custom ? eval_graph(a.graphs[0], idx, a, sl) : s_load(a.initial + idx, a.scaled & kScInitial);
         if (a.perm_sets != 0 && a.chunk_len != 0) v = permutation_terms(v, idx, a);
-        if (!kExtras) {
-            store_fr(a.out + idx, v);
-            continue;
+        if (kExtras) {
+            if (a.lookup_count != 0) v = lookup_terms(v, idx, a, sl);
+            const bool ts = a.scaled & kScTrash, fs = a.scaled & kScFixed;
+            const S y = s_const(a.y), one = s29_one();
+            for (uint32_t t = 0; t < a.trash_count; ++t) {
+                const uint32_t sel = a.trash_sel[t];
+                if (sel >= a.fixed_cols) continue;
+                const S compressed = eval_graph(a.graphs[1 + a.lookup_count + t], idx, a, sl);
+                const S q = s_load(a.fixed + (size_t)sel * a.len + idx, fs);
+                const S trash = s_load(a.trash_cosets + (size_t)t * a.len + idx, ts);
+                v = s_add(s_mul(v, y), s_sub(compressed, s_mul(s_sub(one, q), trash)));
+            }
         }
-        if (a.lookup_count != 0) v = lookup_terms(v, idx, a, sl);
-        for (uint32_t t = 0; t < a.trash_count; ++t) {
-            const uint32_t sel = a.trash_sel[t];
-            if (sel >= a.fixed_cols) continue;
-            const Fr compressed = eval_graph(a.graphs[1 + a.lookup_count + t], idx, a, sl);
-            const Fr q = load_fr(a.fixed + (size_t)sel * a.len + idx);
-            const Fr trash = load_fr(a.trash_cosets + (size_t)t * a.len + idx);
-            v = v * a.y + (compressed - (Fr::one() - q) * trash);
-        }
-        store_fr(a.out + idx, v);
+        store_fr(a.out + idx, s29_to_usual(v));
     }
+}
+
+// v[i] (usual form) -> v[i] 2^5 mod r: the scaled form, stored as 8 x 32 bits
+__global__ __launch_bounds__(kBlock) void k_to_scaled(Fr *v, size_t n)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) store_fr(v + i, fr29_to32(s29_from_usual(load_fr(v + i))));
 }
 
 // out[g * len + idx] = graph g at row idx (eval_graph_batch, bls12_381_g1_msm.cu:5434-5515)
@@ -333,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_graphs(NumArgs a)
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < a.len; idx += stride)
         for (uint32_t gi = 0; gi < a.graph_count; ++gi)
-            store_fr(a.out + (size_t)gi * a.len + idx, eval_graph(a.graphs[gi], idx, a, sl));
+            store_fr(a.out + (size_t)gi * a.len + idx, s29_to_usual(eval_graph(a.graphs[gi], idx, a, sl)));
 }
 
 // ---------------------------------------------------------------- host
@@ -346,6 +421,39 @@ struct UploadedGraph {
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// a constant in the scaled form (v 2^261 mod r: the usual representation
+// times 32), as the kernels take their challenges and graph constants
+Fr scaled_fr(const HostFr &h)
+{
+    const HostFr x = h * HostFr::from_u64(32);
+    Fr r;
+    std::memcpy(r.v, x.l, 32);
+    return r;
+}
+Fr scaled_fr(const RawFr &v) { return scaled_fr(to_hfr(v)); }
+
+// omega^(j 2^(11 l)) tables (rowpow.hpp layout) in the scaled form
+void build_scaled_omega_table(DevBuf<Fr> &tab, uint32_t &levels, const HostFr &omega, size_t len, hipStream_t st)
+{
+    uint32_t bits = 0;
+    while (bits < 64 && (1ull << bits) < len) ++bits;
+    levels = bits <= (uint32_t)kOmegaBits ? 1 : (bits + kOmegaBits - 1) / kOmegaBits;
+    std::vector<HostFr> h((size_t)levels * kOmegaTab);
+    const HostFr k32 = HostFr::from_u64(32);
+    HostFr step = omega;
+    for (uint32_t l = 0; l < levels; ++l) {
+        HostFr acc = HostFr::one();
+        for (uint32_t j = 0; j < kOmegaTab; ++j) {
+            h[(size_t)l * kOmegaTab + j] = acc * k32;
+            acc = acc * step;
+        }
+        step = acc;   // omega^(2048 * 2^(11 l))
+    }
+    tab.alloc(h.size());
+    MH_HIP(hipMemcpyAsync(tab.get(), h.data(), h.size() * 32, hipMemcpyHostToDevice, st));
+    MH_HIP(hipStreamSynchronize(st));
+}
 
 void check_graph(const RawGraph &raw)
 {
@@ -372,7 +480,10 @@ void upload_graph(UploadedGraph &u, const RawGraph &raw, hipStream_t st)
     const size_t opart = ocalc + align256(raw.calculations_len * sizeof(RawCalculation));
     const size_t total = opart + align256(raw.horner_parts_len * sizeof(RawValueSource));
     std::vector<uint8_t> host(total ? total : 1, 0);
-    if (raw.constants_len) std::memcpy(host.data() + oc, raw.constants, raw.constants_len * sizeof(RawFr));
+    for (size_t i = 0; i < raw.constants_len; ++i) {   // constants in the scaled form
+        const Fr c = scaled_fr(raw.constants[i]);
+        std::memcpy(host.data() + oc + i * sizeof(RawFr), c.v, sizeof(RawFr));
+    }
     if (raw.rotations_len) std::memcpy(host.data() + orot, raw.rotations, raw.rotations_len * sizeof(int32_t));
     if (raw.calculations_len)
         std::memcpy(host.data() + ocalc, raw.calculations, raw.calculations_len * sizeof(RawCalculation));
@@ -424,13 +535,13 @@ void launch_rows(NumArgs a, uint32_t slots, Kernel lds_kernel, Kernel global_ker
     if (slots <= kLdsSlots) {
         const size_t blocks = ceil_div(a.len, kBlock);
         MH_CHECK(blocks <= 0x7fffffffu, (int)hipErrorInvalidValue, "too many rows");
-        const size_t lds = (size_t)(slots ? slots : 1) * 32 * kBlock;
+        const size_t lds = (size_t)(slots ? slots : 1) * 36 * kBlock;
         hipLaunchKernelGGL(lds_kernel, dim3((unsigned)blocks), dim3(kBlock), lds, st, a);
     } else {
         const size_t blocks = std::min<size_t>(ceil_div(a.len, kBlock), kGlobalSlotBlocks);
         auto &sf = numerator_slot_file();
         a.slot_stride = blocks * kBlock;
-        sf.buf.ensure((size_t)slots * 8 * a.slot_stride);
+        sf.buf.ensure((size_t)slots * 9 * a.slot_stride);
         a.slots_global = sf.buf.get();
         hipLaunchKernelGGL(global_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
     }
@@ -461,6 +572,26 @@ const Fr *up(DevBuf<Fr> &buf, const RawFr *src, size_t n, hipStream_t st)
     return buf.get();
 }
 
+// A device buffer of usual-form values converted to the scaled form in place.
+void to_scaled(Fr *v, size_t n, hipStream_t st)
+{
+    if (n == 0 || !v) return;
+    hipLaunchKernelGGL(k_to_scaled, dim3((unsigned)ceil_div(n, kBlock)), dim3(kBlock), 0, st, v, n);
+    MH_HIP(hipGetLastError());
+}
+
+// Host challenges -> device buffer in the scaled form.
+const Fr *up_scaled(DevBuf<Fr> &buf, const RawFr *src, size_t n, hipStream_t st)
+{
+    if (n == 0 || !src) return nullptr;
+    std::vector<Fr> h(n);
+    for (size_t i = 0; i < n; ++i) h[i] = scaled_fr(src[i]);
+    buf.alloc(n);
+    MH_HIP(hipMemcpyAsync(buf.get(), h.data(), n * 32, hipMemcpyHostToDevice, st));
+    MH_HIP(hipStreamSynchronize(st));   // h goes out of scope
+    return buf.get();
+}
+
 template <class T>
 const T *up_array(DevBuf<T> &buf, const T *src, size_t n, hipStream_t st)
 {
@@ -488,6 +619,9 @@ struct NumeratorContext {
     int32_t rot_scale = 0, last_rotation = 0;
     uint32_t chunk_len = 0;
     HostFr omega{}, g_coset{}, delta_const{}, trace_omega{};
+    // fixed, l0 / l_last / l_active and perm_cosets (extended cosets) are
+    // held in the scaled form; fixed_lagrange and perm_lagrange (the trace
+    // domain, read by the grand products and trace graphs) in the usual form
     DevBuf<Fr> fixed, fixed_lagrange, l0, l_last, l_active, perm_cosets, perm_lagrange, omega_tab;
     DevBuf<RawColumnRef> perm_columns;
     DevBuf<uint32_t> trash_sel;
@@ -544,18 +678,18 @@ extern "C" RustError midnight_sppark_bls12_381_eval_numerator(
     a.fixed = up(d_fixed, fixed, fixed_cols * len, st);
     a.advice = up(d_advice, advice, advice_cols * len, st);
     a.instance = up(d_instance, instance, instance_cols * len, st);
-    a.challenges = up(d_ch, challenges, challenge_len, st);
+    a.challenges = up_scaled(d_ch, challenges, challenge_len, st);
     a.fixed_cols = (uint32_t)fixed_cols;
     a.advice_cols = (uint32_t)advice_cols;
     a.instance_cols = (uint32_t)instance_cols;
     a.challenge_len = (uint32_t)challenge_len;
-    a.beta = to_fr(beta);
-    a.gamma = to_fr(gamma);
-    a.theta = to_fr(theta);
-    a.trash = to_fr(trash_challenge);
-    a.y = to_fr(y);
-    a.delta_start = to_fr(delta_start);
-    a.delta_const = to_fr(delta_const);
+    a.beta = scaled_fr(beta);
+    a.gamma = scaled_fr(gamma);
+    a.theta = scaled_fr(theta);
+    a.trash = scaled_fr(trash_challenge);
+    a.y = scaled_fr(y);
+    a.delta_start = scaled_fr(delta_start);
+    a.delta_const = scaled_fr(delta_const);
     a.rot_scale = rot_scale;
     a.last_rotation = last_rotation;
     a.chunk_len = chunk_len;
@@ -574,7 +708,7 @@ extern "C" RustError midnight_sppark_bls12_381_eval_numerator(
     a.trash_count = (uint32_t)trash_count;
     a.trash_cosets = up(d_tc, trash_cosets, trash_count * len, st);
     a.trash_sel = up_array(d_sel, trash_selector_fixed_indices, trash_count, st);
-    build_omega_table(d_omega, a.omega_levels, to_hfr(omega), len, st);
+    build_scaled_omega_table(d_omega, a.omega_levels, to_hfr(omega), len, st);
     a.omega_tab = d_omega.get();
 
     GraphSet gs;
@@ -613,16 +747,16 @@ extern "C" RustError midnight_sppark_bls12_381_eval_graph_batch(
     a.fixed = up(d_fixed, fixed, fixed_cols * len, st);
     a.advice = up(d_advice, advice, advice_cols * len, st);
     a.instance = up(d_instance, instance, instance_cols * len, st);
-    a.challenges = up(d_ch, challenges, challenge_len, st);
+    a.challenges = up_scaled(d_ch, challenges, challenge_len, st);
     a.fixed_cols = (uint32_t)fixed_cols;
     a.advice_cols = (uint32_t)advice_cols;
     a.instance_cols = (uint32_t)instance_cols;
     a.challenge_len = (uint32_t)challenge_len;
-    a.beta = to_fr(beta);
-    a.gamma = to_fr(gamma);
-    a.theta = to_fr(theta);
-    a.trash = to_fr(trash_challenge);
-    a.y = to_fr(y);
+    a.beta = scaled_fr(beta);
+    a.gamma = scaled_fr(gamma);
+    a.theta = scaled_fr(theta);
+    a.trash = scaled_fr(trash_challenge);
+    a.y = scaled_fr(y);
     a.rot_scale = rot_scale;
     GraphSet gs;
     for (size_t i = 0; i < graph_count; ++i) gs.add(graphs[i], st);
@@ -682,8 +816,14 @@ extern "C" RustError midnight_sppark_bls12_381_numerator_context_create(
     up(c->l0, l0, len, st);
     up(c->l_last, l_last, len, st);
     up(c->l_active, l_active, len, st);
+    // the numerator reads the key's cosets in the scaled form (converted once here)
+    to_scaled(c->fixed.get(), fixed_cols * len, st);
+    to_scaled(c->l0.get(), len, st);
+    to_scaled(c->l_last.get(), len, st);
+    to_scaled(c->l_active.get(), len, st);
     up_array(c->perm_columns, permutation_columns, permutation_columns_len, st);
     up(c->perm_cosets, permutation_pk_cosets, permutation_pk_cosets_len * len, st);
+    to_scaled(c->perm_cosets.get(), permutation_pk_cosets_len * len, st);
     up(c->perm_lagrange, permutation_pk_lagrange, permutation_pk_cosets_len * input_len, st);
     up_array(c->trash_sel, trash_selector_fixed_indices, trash_count, st);
     c->graphs.add(custom_graph, st);
@@ -696,7 +836,7 @@ extern "C" RustError midnight_sppark_bls12_381_numerator_context_create(
     c->graphs.finish(st);
     c->lookup_input_graphs.finish(st);   // descriptor tables for trace_context_lookup_permute
     c->lookup_table_graphs.finish(st);
-    build_omega_table(c->omega_tab, c->omega_levels, c->omega, len, st);
+    build_scaled_omega_table(c->omega_tab, c->omega_levels, c->omega, len, st);
     MH_HIP(hipStreamSynchronize(st));
     *out = c.release();
     return ok();
@@ -714,12 +854,18 @@ extern "C" void midnight_sppark_bls12_381_numerator_context_free(void *context)
     delete c;
 }
 
-extern "C" RustError midnight_sppark_bls12_381_numerator_context_eval(
-    void **out, const void *raw_context, const void *raw_columns, size_t total_columns, size_t advice_offset,
-    size_t advice_cols, size_t instance_offset, size_t instance_cols, const RawFr *challenges, size_t challenge_len,
-    RawFr beta, RawFr gamma, RawFr theta, RawFr trash_challenge, RawFr y, size_t permutation_offset,
-    size_t permutation_sets, size_t lookup_product_offset, size_t lookup_input_offset, size_t lookup_table_offset,
-    size_t lookup_count, size_t trash_offset, size_t trash_count)
+namespace mh {
+
+// numerator_context_eval; `columns_scaled` tells that the column vector holds
+// its values in the scaled form (v 2^261 mod r) instead of the usual one (the
+// proof pipeline's cosets, written so by the NTT that makes them).
+RustError numerator_context_eval(void **out, const void *raw_context, const void *raw_columns, size_t total_columns,
+                                 size_t advice_offset, size_t advice_cols, size_t instance_offset,
+                                 size_t instance_cols, const RawFr *challenges, size_t challenge_len, RawFr beta,
+                                 RawFr gamma, RawFr theta, RawFr trash_challenge, RawFr y, size_t permutation_offset,
+                                 size_t permutation_sets, size_t lookup_product_offset, size_t lookup_input_offset,
+                                 size_t lookup_table_offset, size_t lookup_count, size_t trash_offset,
+                                 size_t trash_count, bool columns_scaled)
 {
     if (out == nullptr || raw_context == nullptr || (challenge_len != 0 && challenges == nullptr))
         return make_error((int)hipErrorInvalidValue, "invalid numerator evaluation arguments");
@@ -762,16 +908,16 @@ extern "C" RustError midnight_sppark_bls12_381_numerator_context_eval(
     a.advice_cols = (uint32_t)advice_cols;
     a.instance = view(instance_offset, instance_cols);
     a.instance_cols = (uint32_t)instance_cols;
-    a.challenges = up(d_ch, challenges, challenge_len, st);
+    a.challenges = up_scaled(d_ch, challenges, challenge_len, st);
     a.challenge_len = (uint32_t)challenge_len;
-    a.beta = to_fr(beta);
-    a.gamma = to_fr(gamma);
-    a.theta = to_fr(theta);
-    a.trash = to_fr(trash_challenge);
-    a.y = to_fr(y);
+    a.beta = scaled_fr(beta);
+    a.gamma = scaled_fr(gamma);
+    a.theta = scaled_fr(theta);
+    a.trash = scaled_fr(trash_challenge);
+    a.y = scaled_fr(y);
     const HostFr ds = to_hfr(beta) * c->g_coset;
-    std::memcpy(a.delta_start.v, ds.l, 32);
-    std::memcpy(a.delta_const.v, c->delta_const.l, 32);
+    a.delta_start = scaled_fr(ds);
+    a.delta_const = scaled_fr(c->delta_const);
     a.rot_scale = c->rot_scale;
     a.last_rotation = c->last_rotation;
     a.chunk_len = c->chunk_len;
@@ -793,11 +939,29 @@ extern "C" RustError midnight_sppark_bls12_381_numerator_context_eval(
     a.trash_cosets = view(trash_offset, trash_count);
     a.trash_sel = c->trash_sel.get();
     a.graphs = c->graphs.table.get();
+    a.scaled = kScFixed | kScCosets | kScLagrange |
+               (columns_scaled ? (kScAdvice | kScInstance | kScProducts | kScLookup | kScTrash) : 0u);
     run_numerator(a, c->graphs.max_slots, st);
     MH_HIP(hipStreamSynchronize(st));
     *out = dv.release();
     return ok();
     MH_CATCH
+}
+
+}  // namespace mh
+
+extern "C" RustError midnight_sppark_bls12_381_numerator_context_eval(
+    void **out, const void *raw_context, const void *raw_columns, size_t total_columns, size_t advice_offset,
+    size_t advice_cols, size_t instance_offset, size_t instance_cols, const RawFr *challenges, size_t challenge_len,
+    RawFr beta, RawFr gamma, RawFr theta, RawFr trash_challenge, RawFr y, size_t permutation_offset,
+    size_t permutation_sets, size_t lookup_product_offset, size_t lookup_input_offset, size_t lookup_table_offset,
+    size_t lookup_count, size_t trash_offset, size_t trash_count)
+{
+    return numerator_context_eval(out, raw_context, raw_columns, total_columns, advice_offset, advice_cols,
+                                  instance_offset, instance_cols, challenges, challenge_len, beta, gamma, theta,
+                                  trash_challenge, y, permutation_offset, permutation_sets, lookup_product_offset,
+                                  lookup_input_offset, lookup_table_offset, lookup_count, trash_offset, trash_count,
+                                  false);
 }
 
 // bls12_381_g1_msm.cu:5667-5736: the context's trash graphs over its
@@ -838,12 +1002,12 @@ extern "C" RustError midnight_sppark_bls12_381_trace_context_trash(void **out, c
     a.fixed = c->fixed_lagrange.get();
     a.advice = advice_cols ? advice->values.get() : nullptr;
     a.instance = instance_cols ? instance->values.get() : nullptr;
-    a.challenges = up(d_ch, challenges, challenge_len, st);
+    a.challenges = up_scaled(d_ch, challenges, challenge_len, st);
     a.fixed_cols = (uint32_t)c->fixed_cols;
     a.advice_cols = (uint32_t)advice_cols;
     a.instance_cols = (uint32_t)instance_cols;
     a.challenge_len = (uint32_t)challenge_len;
-    a.trash = to_fr(trash_challenge);
+    a.trash = scaled_fr(trash_challenge);
     a.rot_scale = 1;
     a.graphs = c->graphs.table.get() + 1 + c->lookup_count;   // custom, lookups, then trash
     a.graph_count = (uint32_t)c->trash_count;
@@ -901,12 +1065,12 @@ extern "C" RustError midnight_sppark_bls12_381_trace_context_lookup_permute(
     a.fixed = c->fixed_lagrange.get();
     a.advice = advice_cols ? advice->values.get() : nullptr;
     a.instance = instance_cols ? instance->values.get() : nullptr;
-    a.challenges = up(d_ch, challenges, challenge_len, st);
+    a.challenges = up_scaled(d_ch, challenges, challenge_len, st);
     a.fixed_cols = (uint32_t)c->fixed_cols;
     a.advice_cols = (uint32_t)advice_cols;
     a.instance_cols = (uint32_t)instance_cols;
     a.challenge_len = (uint32_t)challenge_len;
-    a.theta = to_fr(theta);
+    a.theta = scaled_fr(theta);
     a.rot_scale = 1;
     a.graph_count = (uint32_t)lookup_count;
     a.out = dv->values.get();

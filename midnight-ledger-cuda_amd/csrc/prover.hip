@@ -42,6 +42,7 @@
 #include "host_g1.hpp"
 #include "hostcrypto.hpp"
 #include "ntt.hpp"
+#include "numerator.hpp"
 #include "polyops.hpp"
 #include "rngdev.hpp"
 #include "runtime.hpp"
@@ -176,9 +177,11 @@ void ntt_batch(Fr *data, uint32_t log_n, size_t count, bool inverse, DevBuf<Fr> 
     ntt_run(j, st);
 }
 
-// coefficients (len n each, count polys) -> values on the extended coset
+// coefficients (len n each, count polys) -> values on the extended coset;
+// `scaled` writes them in the scaled form v 2^261 mod r (times 32), the form
+// the numerator computes in
 void to_extended(Fr *dst, const Fr *src, size_t n, size_t count, uint32_t ext_log_n, DevBuf<Fr> &scratch,
-                 hipStream_t st)
+                 hipStream_t st, bool scaled = false)
 {
     scratch.ensure(ntt_scratch_elems(ext_log_n, count));
     NttJob j;
@@ -193,6 +196,10 @@ void to_extended(Fr *dst, const Fr *src, size_t n, size_t count, uint32_t ext_lo
     j.in_len = n;
     j.first = kZeta;
     j.second = kZeta.sqr();
+    if (scaled) {
+        j.post_mul = true;
+        j.mul = HostFr::from_u64(32);
+    }
     ntt_run(j, st);
 }
 
@@ -521,13 +528,12 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
 
     // numerator on the extended coset of advice | products
     auto cols = new_vec((A + S) * N);
-    to_extended(cols->values.get(), pd + P_ADV * n, n, A, pk.ext_k, pk.scratch(), st);
-    to_extended(cols->values.get() + A * N, pd + P_Z * n, n, S, pk.ext_k, pk.scratch(), st);
+    to_extended(cols->values.get(), pd + P_ADV * n, n, A, pk.ext_k, pk.scratch(), st, true);
+    to_extended(cols->values.get() + A * N, pd + P_Z * n, n, S, pk.ext_k, pk.scratch(), st, true);
     MH_HIP(hipStreamSynchronize(st));
     void *hv = nullptr;
-    check(midnight_sppark_bls12_381_numerator_context_eval(&hv, pk.numctx, cols.get(), A + S, 0, A, 0, 0, nullptr,
-                                                           0, raw(beta), raw(gamma), raw(theta), raw(trash), raw(y),
-                                                           A, S, 0, 0, 0, 0, 0, 0));
+    check(numerator_context_eval(&hv, pk.numctx, cols.get(), A + S, 0, A, 0, 0, nullptr, 0, raw(beta), raw(gamma),
+                                 raw(theta), raw(trash), raw(y), A, S, 0, 0, 0, 0, 0, 0, true));
     cols.reset();
     std::unique_ptr<DeviceVec> h(reinterpret_cast<DeviceVec *>(hv));
     // h = numerator / (X^n - 1) on the coset, then coefficients (extended_to_coeff)
