@@ -127,6 +127,42 @@ __device__ __forceinline__ Fp28 fp28_mul(const Fp28 &a, const Fp28 &b)
     return r;
 }
 
+// (a b + c d) / R mod p with one Montgomery reduction for both products:
+// each row adds a_j b_i + c_j d_i, then m p.  For operands below 2p the
+// result is below (4p^2 + 4p^2) / 2^392 + p < 2p (loose), and an accumulator
+// collects at most 42 products below 2^56 (< 2^62).  Saves the second
+// product's reduction (14 of its 28 multiply-adds per row).
+__device__ __forceinline__ Fp28 fp28_mul2_lz(const Fp28 &a, const Fp28 &b, const Fp28 &c, const Fp28 &d)
+{
+    uint64_t t[14];
+#pragma unroll
+    for (int j = 0; j < 14; ++j) t[j] = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+#pragma unroll
+        for (int j = 0; j < 14; ++j) t[j] += (uint64_t)a.v[j] * b.v[i];
+#pragma unroll
+        for (int j = 0; j < 14; ++j) t[j] += (uint64_t)c.v[j] * d.v[i];
+        const uint32_t m = ((uint32_t)t[0] * kPinv28) & kM28;
+#pragma unroll
+        for (int j = 0; j < 14; ++j) t[j] += (uint64_t)m * p28(j);
+        const uint64_t carry = t[0] >> 28;
+#pragma unroll
+        for (int j = 0; j < 13; ++j) t[j] = t[j + 1];
+        t[0] += carry;
+        t[13] = 0;
+    }
+    Fp28 r;
+    uint64_t cc = 0;
+#pragma unroll
+    for (int j = 0; j < 14; ++j) {
+        cc += t[j];
+        r.v[j] = (uint32_t)cc & kM28;
+        cc >>= 28;
+    }
+    return r;
+}
+
 // Column-oriented form (finely integrated product scanning), kept for
 // squaring where the symmetric cross products halve the multiplies.
 __device__ __forceinline__ Fp28 fp28_mul_fips(const Fp28 &a, const Fp28 &b)
@@ -448,7 +484,8 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_affine_lz(const Xyzz28 &p, const Af
     Fp28 qq = fp28_mul<false>(p.x, pp);
     Xyzz28 r;
     r.x = fp28_sub_lz(fp28_sub_lz(fp28_sqr<false>(r_), ppp), fp28_add_lz(qq, qq));
-    r.y = fp28_sub_lz(fp28_mul<false>(r_, fp28_sub_lz(qq, r.x)), fp28_mul<false>(p.y, ppp));
+    // Y3 = R (Q - X3) - Y1 PPP = R (Q - X3) + (2p - Y1) PPP, one reduction
+    r.y = fp28_mul2_lz(r_, fp28_sub_lz(qq, r.x), fp28_sub_lz(fp28_zero(), p.y), ppp);
     r.zz = fp28_mul<false>(p.zz, pp);
     r.zzz = fp28_mul<false>(p.zzz, ppp);
     return r;
@@ -603,7 +640,8 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_lz(const Xyzz28 &p, const Xyzz28 &q
     Fp28 qq = fp28_mul<false>(u1, pp);
     Xyzz28 r;
     r.x = fp28_sub_lz(fp28_sub_lz(fp28_sqr<false>(r_), ppp), fp28_add_lz(qq, qq));
-    r.y = fp28_sub_lz(fp28_mul<false>(r_, fp28_sub_lz(qq, r.x)), fp28_mul<false>(s1, ppp));
+    // Y3 = R (Q - X3) - S1 PPP = R (Q - X3) + (2p - S1) PPP, one reduction
+    r.y = fp28_mul2_lz(r_, fp28_sub_lz(qq, r.x), fp28_sub_lz(fp28_zero(), s1), ppp);
     r.zz = fp28_mul<false>(fp28_mul<false>(p.zz, q.zz), pp);
     r.zzz = fp28_mul<false>(fp28_mul<false>(p.zzz, q.zzz), ppp);
     return r;
@@ -618,8 +656,9 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_lz(const Xyzz28 &p, const Xyzz28 &q
 // step and the quad broadcasts the four results.  Every lane of a quad holds
 // the same operands and results, so the quad takes every branch together,
 // and each product is the same Montgomery product of the same operands as
-// in xyzz28_add_lz (a square as a product of equal operands gives the same
-// limbs): the sum is limb-identical to the one-lane form.
+// in xyzz28_add_lz except Y3, which the one-lane form folds into one
+// reduction: the sum is the same point (a representative of the same
+// coordinates mod p) as the one-lane form's.
 
 // lane `kLane` of each quad -> all four lanes of the quad
 template <int kLane>
