@@ -598,6 +598,15 @@ RustError midnight_hip_plonk_vk(const void *pk, RawG1Projective *fixed_commitmen
  * any work is done (the size is fixed by the key). */
 RustError midnight_hip_plonk_prove(void *pk, const RawFr *advice, size_t advice_len, uint64_t rng_seed,
                                    uint8_t *proof, size_t proof_capacity, size_t *proof_len);
+/* Commitment stages from HIP graphs (default on): on each lane, the first
+ * proof of the key captures its commitment stages (inverse NTT + MSM batch of
+ * the advice, the permutation products; the MSMs of the random polynomial,
+ * the quotient pieces and the opening polynomials) and every later proof on
+ * that lane replays them; the lane keeps its polynomial arena and MSM
+ * engines for the key's lifetime.  Off: every stage is launched eagerly.
+ * Proof bytes are the same either way.  Not to be called while a proof of
+ * the key runs. */
+RustError midnight_hip_plonk_set_graphs(void *pk, bool enable);
 /* The length in bytes of every proof of this key. */
 RustError midnight_hip_plonk_proof_size(const void *pk, size_t *proof_len);
 /* Thread count of the reference prover whose draws proofs of this key

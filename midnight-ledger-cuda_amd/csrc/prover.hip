@@ -41,6 +41,8 @@
 #include "host_fr.hpp"
 #include "host_g1.hpp"
 #include "hostcrypto.hpp"
+#include "msm.hpp"
+#include "msm_context.hpp"
 #include "ntt.hpp"
 #include "numerator.hpp"
 #include "polyops.hpp"
@@ -205,6 +207,36 @@ void to_extended(Fr *dst, const Fr *src, size_t n, size_t count, uint32_t ext_lo
 
 }  // namespace
 
+// A commitment stage of the proof pipeline captured once per key and lane in
+// a HIP graph: the inverse NTT of the stage's polynomials in place (when the
+// stage has one), then the MSM batch of those polynomials against the SRS.
+// Every proof on the lane replays it: one launch instead of the NTT passes
+// and the MSM's ~20 kernels, and no read-back inside the MSM.  The stage owns
+// its MSM engine, so nothing the graph refers to is reallocated by other
+// calls.
+struct StageGraph {
+    MsmEngine engine;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    StageGraph() = default;
+    StageGraph(const StageGraph &) = delete;
+    StageGraph &operator=(const StageGraph &) = delete;
+    ~StageGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+// Per-lane state of the proofs of one key: the polynomial arena and the
+// [f | work] pair at fixed addresses (which the captured stages refer to),
+// the NTT scratch of the captured transforms, and the stages themselves.
+struct LaneProof {
+    std::unique_ptr<DeviceVec> polys, fv;
+    DevBuf<Fr> ntt_scratch;
+    StageGraph advice, products, random, pieces, opening;
+};
+
 struct PlonkKey {
     uint32_t lane = 0;
     uint32_t k = 0, ext_k = 0, degree = 0;
@@ -219,10 +251,13 @@ struct PlonkKey {
     std::vector<HostFr> t_inv, delta_powers;
     void *numctx = nullptr;
     uint32_t rng_threads = 1;   // thread count of the reference prover whose random draws are reproduced
+    bool graphs = true;         // commitment stages replayed from HIP graphs
     DevBuf<Fr> scratch_by_lane[kLaneCount];   // NTT scratch of proofs running on each lane
     DevBuf<Fr> &scratch() { return scratch_by_lane[current_lane()]; }
+    std::unique_ptr<LaneProof> lane_proofs[kLaneCount];
     ~PlonkKey()
     {
+        for (auto &lp : lane_proofs) lp.reset();
         if (numctx) midnight_sppark_bls12_381_numerator_context_free(numctx);
     }
 };
@@ -234,6 +269,47 @@ void commit(std::vector<RawG1Projective> &out, PlonkKey &pk, const DeviceVec &v,
     out.resize(count);
     check(midnight_sppark_bls12_381_g1_msm_context_device_batch(out.data(), pk.srs, &v, offset_polys * pk.n, pk.n,
                                                                 count));
+}
+
+// One captured commitment stage: `count` polynomials of n coefficients at
+// `base` (transformed from Lagrange values first when `intt`), committed into
+// out[0..count).  Captured on the first call, replayed after.
+void stage_commit(StageGraph &g, PlonkKey &pk, LaneProof &lp, Fr *base, size_t count, bool intt,
+                  std::vector<RawG1Projective> &out, hipStream_t st)
+{
+    const size_t n = pk.n;
+    out.resize(count);
+    if (!g.exec) {
+        MsmContext &ctx = *static_cast<MsmContext *>(pk.srs);
+        std::vector<MsmJob> jobs(count);
+        for (size_t i = 0; i < count; ++i) jobs[i] = MsmJob{base + i * n, 0, n};
+        // the same table / plain-window choice as the context's own batches
+        g.engine.chunk_len = ctx.chunk_len;
+        if (ctx.pc_nwin && n >= (size_t(1) << (ctx.pc_c - 1))) {
+            const PrecompTable pc{ctx.table.get(), ctx.npoints, ctx.pc_c, ctx.pc_nwin};
+            g.engine.prepare(ctx.points.get(), ctx.npoints, jobs, 0, st, &pc);
+        } else {
+            g.engine.prepare(ctx.points.get(), ctx.npoints, jobs, ctx.window_bits, st);
+        }
+        MH_CHECK(!intt || lp.ntt_scratch.len >= ntt_scratch_elems(pk.k, count), -1,
+                 "captured transform larger than the lane's scratch");
+        MH_HIP(hipStreamSynchronize(st));
+        MH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            if (intt) ntt_batch(base, pk.k, count, true, lp.ntt_scratch, st);
+            g.engine.enqueue(st, nullptr);
+        } catch (...) {
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(st, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            throw;
+        }
+        MH_HIP(hipStreamEndCapture(st, &g.graph));
+        MH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    }
+    MH_HIP(hipGraphLaunch(g.exec, st));
+    MH_HIP(hipStreamSynchronize(st));
+    g.engine.finish(out.data());
 }
 
 HostFr rotate(const PlonkKey &pk, const HostFr &x, int32_t rot)
@@ -479,8 +555,39 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
 
     // polys: advice | z sets | random | h pieces (coefficient form, n each)
     const size_t P_ADV = 0, P_Z = A, P_RAND = A + S, P_H = A + S + 1, NPOLY = A + S + 1 + pk.pieces;
-    auto polys = new_vec(NPOLY * n);
-    Fr *pd = polys->values.get();
+    // with graphs, the lane's arena at fixed addresses; else one per proof
+    LaneProof *lp = nullptr;
+    std::unique_ptr<DeviceVec> own_polys, own_fv;
+    if (pk.graphs) {
+        auto &slot = pk.lane_proofs[current_lane()];
+        if (!slot) {
+            slot = std::make_unique<LaneProof>();
+            slot->polys = new_vec(NPOLY * n);
+            slot->fv = new_vec(2 * n);
+            // sized once for every captured transform: graphs keep its address
+            slot->ntt_scratch.alloc(std::max<size_t>(ntt_scratch_elems(pk.k, std::max(A, S)), 1));
+        }
+        lp = slot.get();
+    } else {
+        own_polys = new_vec(NPOLY * n);
+        own_fv = new_vec(2 * n);
+    }
+    DeviceVec &polys = lp ? *lp->polys : *own_polys;
+    DeviceVec &fvec = lp ? *lp->fv : *own_fv;
+    Fr *pd = polys.values.get();
+    // commitments of `count` polynomials from index `first` of the arena
+    // (after their inverse NTT when `intt`), through the lane's captured
+    // stage when graphs are on
+    std::vector<RawG1Projective> com;
+    auto commit_stage = [&](StageGraph *stage, DeviceVec &vec, size_t first, size_t count, bool intt) {
+        Fr *base = vec.values.get() + first * n;
+        if (lp) {
+            stage_commit(*stage, pk, *lp, base, count, intt, com, st);
+            return;
+        }
+        if (intt) ntt_batch(base, pk.k, count, true, pk.scratch(), st);
+        commit(com, pk, vec, first, count);
+    };
     // the random polynomial's chunks, drawn on the device while the stream is still empty
     random_fr_streams_device(pd + P_RAND * n, voff.data(), vlen.data(), vseeds.data(), voff.size(), st);
 
@@ -497,9 +604,7 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
         MH_HIP(hipMemcpyAsync(al + c * n + pk.u, adv_blinds.data() + c * brows, brows * 32, hipMemcpyHostToDevice,
                               st));
     MH_HIP(hipMemcpyAsync(pd, al, A * n * 32, hipMemcpyDeviceToDevice, st));
-    ntt_batch(pd + P_ADV * n, pk.k, A, true, pk.scratch(), st);
-    std::vector<RawG1Projective> com;
-    commit(com, pk, *polys, P_ADV, A);
+    commit_stage(lp ? &lp->advice : nullptr, polys, P_ADV, A, true);
     for (auto &c : com) T.write_point(c);
 
     const HostFr theta = T.squeeze();
@@ -515,13 +620,12 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     std::unique_ptr<DeviceVec> z(reinterpret_cast<DeviceVec *>(zv));
     MH_HIP(hipMemcpyAsync(pd + P_Z * n, z->values.get(), S * n * 32, hipMemcpyDeviceToDevice, st));
     z.reset();
-    ntt_batch(pd + P_Z * n, pk.k, S, true, pk.scratch(), st);
-    commit(com, pk, *polys, P_Z, S);
+    commit_stage(lp ? &lp->products : nullptr, polys, P_Z, S, true);
     for (auto &c : com) T.write_point(c);
 
     const HostFr trash = T.squeeze();
 
-    commit(com, pk, *polys, P_RAND, 1);
+    commit_stage(lp ? &lp->random : nullptr, polys, P_RAND, 1, false);
     T.write_point(com[0]);
 
     const HostFr y = T.squeeze();
@@ -545,7 +649,7 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     fr_quotient_pieces(pd + P_H * n, h->values.get(), n, pk.pieces, lb.get(), st);
     MH_HIP(hipStreamSynchronize(st));
     h.reset();
-    commit(com, pk, *polys, P_H, pk.pieces);
+    commit_stage(lp ? &lp->pieces : nullptr, polys, P_H, pk.pieces, false);
     for (auto &c : com) T.write_point(c);
 
     const HostFr x = T.squeeze();
@@ -653,8 +757,7 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t a, size_t b) { return set_points[a].size() < set_points[b].size(); });
     // f = sum_i x2^i (q_i - r_i) / prod (X - p), i in sorted order
-    auto fv = new_vec(2 * n);   // [f | work]
-    Fr *f = fv->values.get(), *work = f + n;
+    Fr *f = fvec.values.get(), *work = f + n;   // [f | work]
     {
         std::vector<const Fr *> ps;
         std::vector<HostFr> sc;
@@ -671,7 +774,7 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
         }
         poly_lincomb(f, ps, sc, n, HostFr::zero(), st);
     }
-    commit(com, pk, *fv, 0, 1);
+    commit_stage(lp ? &lp->opening : nullptr, fvec, 0, 1, false);
     T.write_point(com[0]);
     const HostFr x3 = T.squeeze();
     std::vector<HostFr> qe(nsets);
@@ -701,7 +804,7 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     }
     poly_kate_divide(work, work, n, x3, st);
     MH_HIP(hipMemcpyAsync(f, work, n * 32, hipMemcpyDeviceToDevice, st));
-    commit(com, pk, *fv, 0, 1);
+    commit_stage(lp ? &lp->opening : nullptr, fvec, 0, 1, false);
     T.write_point(com[0]);
     MH_HIP(hipStreamSynchronize(st));
     return T.proof;
@@ -786,6 +889,21 @@ extern "C" RustError midnight_hip_plonk_set_rng_threads(void *pk, uint32_t threa
     if (!pk || threads == 0) return make_error((int)hipErrorInvalidValue, "invalid prover thread count");
     reinterpret_cast<PlonkKey *>(pk)->rng_threads = threads;
     return ok();
+}
+
+extern "C" RustError midnight_hip_plonk_set_graphs(void *pk, bool enable)
+{
+    if (!pk) return make_error((int)hipErrorInvalidValue, "null key");
+    auto *p = reinterpret_cast<PlonkKey *>(pk);
+    MH_TRY
+    for (auto &lp : p->lane_proofs)
+        if (lp) {
+            (void)hipStreamSynchronize(cur_stream());
+            lp.reset();
+        }
+    p->graphs = enable;
+    return ok();
+    MH_CATCH
 }
 
 extern "C" RustError midnight_hip_plonk_proof_size(const void *pk, size_t *proof_len)

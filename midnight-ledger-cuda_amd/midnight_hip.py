@@ -203,6 +203,7 @@ def _declare(L):
                                          ctypes.POINTER(c_size)]),
         "midnight_hip_plonk_proof_size": (E, [vp, ctypes.POINTER(c_size)]),
         "midnight_hip_plonk_set_rng_threads": (E, [vp, u32]),
+        "midnight_hip_plonk_set_graphs": (E, [vp, ctypes.c_bool]),
         "midnight_hip_srs_load": (E, [vpp, ctypes.c_char_p, c_size, u32]),
         "midnight_hip_srs_load_file": (E, [vpp, ctypes.c_char_p, u32]),
         "midnight_hip_srs_info": (E, [vp, ctypes.POINTER(u32), vpp, vpp]),
@@ -1141,6 +1142,9 @@ class PlonkKey:
         rp = np.zeros(4, dtype=np.uint64)
         check(lib().midnight_hip_plonk_vk(self.handle, P(fc) if self.nfixed else None, P(pc), P(rp)))
         return fc, pc, rp
+
+    def set_graphs(self, enable):
+        check(lib().midnight_hip_plonk_set_graphs(self.handle, enable))
 
     def set_rng_threads(self, threads):
         check(lib().midnight_hip_plonk_set_rng_threads(self.handle, threads))

@@ -139,3 +139,35 @@ def test_sample_random_poly_matches_oracle(H, O, log_n, threads):
         assert np.array_equal(vals[o:o + ln], want), (ci, o, ln)
     with pytest.raises(H.MidnightHipError):
         H.fr_sample_random_poly(log_n, threads, seeds[:-32])      # one seed short
+
+
+def test_gpu_proof_graph_replays_match_eager(H, O):
+    """Commitment stages captured in HIP graphs on a lane's first proof and
+    replayed after (with another key's proofs on the same lane in between)
+    give the same bytes as eager launches and as the CPU prover."""
+    c = S.Circuit(9, seed=31)
+    tau = 1357924680
+    key = PR.CpuKey(c, tau)
+    pk, srs = gpu_key(H, O, c, tau)
+    other = S.Circuit(8, seed=32)
+    pk2, srs2 = gpu_key(H, O, other, tau + 1)
+    adv = S.mont([v for col in c.cols[:5] for v in col])
+    adv2 = S.mont([v for col in other.cols[:5] for v in col])
+    graphs = []
+    for seed in (1, 2, 3, 1):
+        graphs.append(pk.prove(adv, seed))
+        pk2.prove(adv2, seed)
+    assert graphs[0] == graphs[3]
+    assert graphs[0] == PR.cpu_prove(key, c.cols[:5], 1)
+    pk.set_graphs(False)
+    eager = [pk.prove(adv, seed) for seed in (1, 2, 3)]
+    assert eager == graphs[:3]
+    pk.set_graphs(True)
+    assert pk.prove(adv, 2) == graphs[1]
+    try:
+        H.set_lane(3)
+        assert pk.prove(adv, 3) == graphs[2]        # a second lane captures its own stages
+    finally:
+        H.set_lane(0)
+    for x in (pk, srs, pk2, srs2):
+        x.free()
