@@ -475,6 +475,7 @@ def bench_prove(args, H):
     dt = (time.perf_counter() - t0) / args.steps
     msm_ms, msm_cnt = H.profile_read("msm_total")
     acc_ms, _ = H.profile_read("msm_accumulate")
+    stage_ms, _ = H.profile_read("commit_stage")
     num_ms, num_cnt = H.profile_read("numerator")
     H.profile_enable(False)
     peak_measured, _, _ = H.valu_peak()
@@ -491,8 +492,9 @@ def bench_prove(args, H):
         "config": {"workload": "full proof, k=%d (2^%d rows), 5 advice / 3 fixed, gate q(ab+c-d), 8-column "
                                "permutation, 4x extended domain" % (k, k), "proof_bytes": len(proof)},
         "p50_ms": round(lat[len(lat) // 2] * 1e3, 3),
-        "stage_ms_per_proof": {"msm_total": round(msm_ms / args.steps, 3),
-                               "msm_accumulate": round(acc_ms / args.steps, 3),
+        "stage_ms_per_proof": {"commit_stages_graph": round(stage_ms / args.steps, 3),
+                               "msm_total_eager": round(msm_ms / args.steps, 3),
+                               "msm_accumulate_eager": round(acc_ms / args.steps, 3),
                                "numerator": round(num_ms / args.steps, 3)},
         "roofline": {"bound": "valu", "kernel": "numerator", "achieved": round(num_macs / t_num / 1e12, 3),
                      "peak": round(peak / 1e12, 3), "peak_measured": round(peak_measured / 1e12, 3),

@@ -212,6 +212,41 @@ __device__ __forceinline__ Fr29 fr29_mul_loose(const Fr29 &a, const Fr29 &b)
     return r;
 }
 
+// (a b + c d) / 2^261 mod r for loose a, b, c, d with one reduction: each
+// row adds a_j b_i + c_j d_i, then m r; loose result (below (8r^2) / 2^261 +
+// r < 1.12 r), accumulators below 27 products of 2^58.
+__device__ __forceinline__ Fr29 fr29_mul2_loose(const Fr29 &a, const Fr29 &b, const Fr29 &c, const Fr29 &d)
+{
+    uint64_t t[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) t[j] = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) t[j] += (uint64_t)a.v[j] * b.v[i];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) t[j] += (uint64_t)c.v[j] * d.v[i];
+        const uint32_t m = (0u - (uint32_t)t[0]) & kM29;
+        t[0] += m;   // r_0 = 1
+#pragma unroll
+        for (int j = 1; j < 9; ++j) t[j] += (uint64_t)m * r29(j);
+        const uint64_t carry = t[0] >> 29;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) t[j] = t[j + 1];
+        t[0] += carry;
+        t[8] = 0;
+    }
+    Fr29 r;
+    uint64_t cc = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        cc += t[j];
+        r.v[j] = (uint32_t)cc & kM29;
+        cc >>= 29;
+    }
+    return r;
+}
+
 // a + b for loose a, b; loose result
 __device__ __forceinline__ Fr29 fr29_add_loose(const Fr29 &a, const Fr29 &b)
 {

@@ -122,6 +122,11 @@ __device__ __forceinline__ S s_zero()
 __device__ __forceinline__ S s_add(const S &a, const S &b) { return fr29_add_loose(a, b); }
 __device__ __forceinline__ S s_sub(const S &a, const S &b) { return fr29_sub_loose(a, b); }
 __device__ __forceinline__ S s_mul(const S &a, const S &b) { return fr29_mul_loose(a, b); }
+// v y + t l, the numerator's fold step, with one reduction for both products
+__device__ __forceinline__ S s_fold(const S &v, const S &y, const S &t, const S &l)
+{
+    return fr29_mul2_loose(v, y, t, l);
+}
 __device__ __forceinline__ S s_const(const Fr &c) { return fr29_from32(c); }   // already scaled
 __device__ __forceinline__ S s_load(const Fr *p, bool scaled)
 {
@@ -292,13 +297,13 @@ __device__ __forceinline__ S permutation_terms(S v, size_t idx, const NumArgs &a
     const Fr *first = a.perm_products;
     const Fr *last = a.perm_products + (size_t)(a.perm_sets - 1) * len;
 
-    v = s_add(s_mul(v, y), s_mul(s_sub(one, s_load(first + idx, zs)), l0));
+    v = s_fold(v, y, s_sub(one, s_load(first + idx, zs)), l0);
     const S zl = s_load(last + idx, zs);
-    v = s_add(s_mul(v, y), s_mul(s_sub(s_mul(zl, zl), zl), s_load(a.l_last + idx, ls)));
+    v = s_fold(v, y, s_sub(s_mul(zl, zl), zl), s_load(a.l_last + idx, ls));
     for (uint32_t s = 1; s < a.perm_sets; ++s) {
         const S cur = s_load(a.perm_products + (size_t)s * len + idx, zs);
         const S prev = s_load(a.perm_products + (size_t)(s - 1) * len + r_last, zs);
-        v = s_add(s_mul(v, y), s_mul(s_sub(cur, prev), l0));
+        v = s_fold(v, y, s_sub(cur, prev), l0);
     }
 
     const S l_active = s_load(a.l_active + idx, ls);
@@ -320,7 +325,7 @@ __device__ __forceinline__ S permutation_terms(S v, size_t idx, const NumArgs &a
             right = s_mul(right, s_add(x, delta));
             delta = s_mul(delta, dconst);
         }
-        v = s_add(s_mul(v, y), s_mul(s_sub(left, right), l_active));
+        v = s_fold(v, y, s_sub(left, right), l_active);
     }
     return v;
 }
@@ -348,12 +353,12 @@ __device__ __forceinline__ S lookup_terms(S v, size_t idx, const NumArgs &a, con
         const S a_i = s_load(ai + idx, ks);
         const S s_i = s_load(st + idx, ks);
         const S a_minus_s = s_sub(a_i, s_i);
-        v = s_add(s_mul(v, y), s_mul(s_sub(one, zi), l0));
-        v = s_add(s_mul(v, y), s_mul(s_sub(s_mul(zi, zi), zi), l_last));
+        v = s_fold(v, y, s_sub(one, zi), l0);
+        v = s_fold(v, y, s_sub(s_mul(zi, zi), zi), l_last);
         const S lhs = s_mul(s_mul(s_load(z + r_next, ks), s_add(a_i, beta)), s_add(s_i, gamma));
-        v = s_add(s_mul(v, y), s_mul(s_sub(lhs, s_mul(zi, table_value)), l_active));
-        v = s_add(s_mul(v, y), s_mul(a_minus_s, l0));
-        v = s_add(s_mul(v, y), s_mul(s_mul(a_minus_s, s_sub(a_i, s_load(ai + r_prev, ks))), l_active));
+        v = s_fold(v, y, s_sub(lhs, s_mul(zi, table_value)), l_active);
+        v = s_fold(v, y, a_minus_s, l0);
+        v = s_fold(v, y, s_mul(a_minus_s, s_sub(a_i, s_load(ai + r_prev, ks))), l_active);
     }
     return v;
 }

@@ -46,6 +46,7 @@
 #include "ntt.hpp"
 #include "numerator.hpp"
 #include "polyops.hpp"
+#include "profile.hpp"
 #include "rngdev.hpp"
 #include "runtime.hpp"
 
@@ -307,8 +308,12 @@ void stage_commit(StageGraph &g, PlonkKey &pk, LaneProof &lp, Fr *base, size_t c
         MH_HIP(hipStreamEndCapture(st, &g.graph));
         MH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
     }
+    ProfileScope prof;
+    prof.begin("commit_stage", st);
     MH_HIP(hipGraphLaunch(g.exec, st));
+    prof.end(st);
     MH_HIP(hipStreamSynchronize(st));
+    if (profiling()) prof.flush();
     g.engine.finish(out.data());
 }
 
