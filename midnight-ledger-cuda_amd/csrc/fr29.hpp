@@ -278,6 +278,70 @@ __device__ __forceinline__ Fr29 fr29_sub_loose(const Fr29 &a, const Fr29 &b)
 }
 
 // ---------------------------------------------------------------------------
+// Lazy values: limbs may exceed 29 bits (each below 2^31 where a value enters
+// a product, below 2^32 otherwise) and the value may exceed 2r; only the
+// integer's bound matters, tracked by the caller.  fr29_mul_loose accepts
+// such a value as either operand when its bound times the other's stays
+// below r 2^261 (every accumulator then stays below 2^64), and returns a
+// normalized value below a b / 2^261 + r.
+
+// a + b, no carries
+__device__ __forceinline__ Fr29 fr29_lazy_add(const Fr29 &a, const Fr29 &b)
+{
+    Fr29 r;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) r.v[j] = a.v[j] + b.v[j];
+    return r;
+}
+
+// 4r with 2^29 lent to every limb below the top (each of those limbs is at
+// least 2^29, the top one at least the top limb of any value below 2r), so
+// that a - b + 4r can be formed limb by limb for a normalized b < 2r
+__device__ __forceinline__ uint32_t r4x29_lent(int i)
+{
+    constexpr uint32_t O[9] = {0x20000004u, 0x3fffffdfu, 0x3e5bfefeu, 0x2d2017feu, 0x360154eeu,
+                               0x30101342u, 0x3483339cu, 0x2994cebdu, 0x01cfb69cu};
+    return O[i];
+}
+
+// a - b + 4r for b normalized and below 2r, no carries
+__device__ __forceinline__ Fr29 fr29_lazy_sub4r(const Fr29 &a, const Fr29 &b)
+{
+    Fr29 r;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) r.v[j] = a.v[j] + r4x29_lent(j) - b.v[j];
+    return r;
+}
+
+// carry propagation: limbs below 2^32, value below 2^261 -> limbs below 2^29
+__device__ __forceinline__ void fr29_norm(Fr29 &a)
+{
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t s = a.v[j] + c;
+        c = s >> 29;
+        a.v[j] = s & kM29;
+    }
+    a.v[8] += c;
+}
+
+// normalized x below 2^261 -> the same class below 3r: x - q r with
+// q = floor(x8 M / 2^48), M = floor(2^48 / (r8 + 1)), r8 the top limb of r,
+// so floor(x / r) - 2 <= q <= floor(x / r)
+__device__ __forceinline__ void fr29_reduce_3r(Fr29 &a)
+{
+    const uint32_t q = (uint32_t)(((uint64_t)a.v[8] * 0x2355091ull) >> 48);
+    int64_t carry = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int64_t t = (int64_t)a.v[j] - (int64_t)q * (int64_t)r29(j) + carry;
+        a.v[j] = (uint32_t)t & kM29;
+        carry = t >> 29;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The scaled form: a value v kept as v 2^261 mod r (fully reduced, 9 limbs),
 // the form in which fr29_mul is closed: mont29(a 2^261, b 2^261) = (a b)
 // 2^261.  Sums and differences of scaled values are scaled values.  Data in

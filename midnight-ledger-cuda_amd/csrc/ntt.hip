@@ -12,8 +12,8 @@
 //   (vendor/sppark-0.1.15/sppark/ntt/ntt.cuh:217-243) and best_fft
 //   (vendor/midnight-curves-0.3.1/src/fft.rs:30-112) limb for limb.
 // Inside a pass, a workgroup loads C columns of m = 2^t elements into LDS in
-// bit-reversed order, runs t radix-2 DIT stages with one barrier per stage,
-// and writes the C columns back; C is chosen so C*m*32 B <= 64 KiB.  Loads
+// bit-reversed order, runs t radix-2 DIT stages (in pairs, one barrier per
+// pair), and writes the C columns back; C is chosen so C*m <= 2048.  Loads
 // and stores walk the unit-stride dimension fastest, so every wave touches
 // whole runs of C (pass A) or m (pass B) consecutive elements.
 //
@@ -26,11 +26,22 @@
 //   post scale(+zeta): a_i *= scale * {1, first, second}[i % 3]
 //        (fr_ntt_zeta_postprocess_kernel, :1680-1700)
 //
-// Products of a data value by a constant (twiddles, zeta, scale) use the
-// radix-2^29 product (fr29.hpp, ~0.6x the instructions of the 32-bit one):
-// the twiddle tables and the constants are stored as c 2^261 mod r, so the
-// result comes out in the usual 2^256 form, limb-identical.  The one
-// data-by-data product (pre-multiplication by caller factors) stays 32-bit.
+// Arithmetic inside a pass is radix 2^29 (fr29.hpp): an element is repacked
+// into 9 x 29-bit limbs at the load and stays so in LDS (36 bytes), still
+// meaning a 2^256 mod r as in the usual form.  Twiddles and constants are
+// stored as c 2^261 mod r, so the radix-2^29 Montgomery product a c 2^256
+// 2^261 / 2^261 keeps the usual form.  Butterflies are lazy (Harvey): the
+// twiddled operand comes out of the product below 2r, the sum and the
+// difference (plus 4r) are formed limb by limb with no carries and no
+// conditional subtraction, and the carries are propagated once per stage
+// pair before the LDS store.  Every element's bound grows by at most 4r per
+// stage: below 2r at the load, below 46r after 11 stages, inside the 2^261
+// (70.6 r) the limbs hold and below what keeps every product's accumulators
+// under 2^64.  The first stage of a pass has only unit twiddles and skips
+// its products; every other stage multiplies, also by w^0 (in a stage's
+// first butterfly), which keeps the bound additive.  The store reduces once
+// to the canonical value and repacks to 8 x 32 bits, so results are limb
+// for limb those of the reference.
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -49,19 +60,40 @@ namespace {
 
 constexpr int kMaxLogLds = 11;     // largest single-pass sub-transform
 constexpr int kTwLog = 22;         // twiddle tables are built for 2^22
-constexpr int kThreads = 512;   // 8 waves: two 64 KiB workgroups per CU give 4 waves per SIMD
+constexpr int kThreads = 512;   // 8 waves: two 72 KiB workgroups per CU give 4 waves per SIMD
+constexpr uint32_t kLdsElems = 2048;               // elements of a pass held in LDS
+constexpr uint32_t kLdsElemBytes = 36;             // 9 x 29-bit limbs
+constexpr int kLdsBytesMax = kLdsElems * kLdsElemBytes;
 
-struct Tables {
-    // each stored as value * 2^261 mod r (the radix-2^29 product's form)
-    Fr *roots[2];   // omega_{2^11}^j, j < 1024
-    Fr *lo[2];      // omega_{2^22}^i, i < 2048
-    Fr *hi[2];      // omega_{2^22}^(2048 i), i < 2048
+// A table of n entries held as three planes: limbs 0-3 and 4-7 as 16-byte
+// words, limb 8 as a word, so an entry is two aligned 16-byte loads and one
+// 4-byte load.
+struct Tab29 {
+    const uint4 *p0, *p1;
+    const uint32_t *p2;
+    __device__ __forceinline__ Fr29 get(uint32_t i) const
+    {
+        const uint4 a = p0[i], b = p1[i];
+        Fr29 v;
+        v.v[0] = a.x; v.v[1] = a.y; v.v[2] = a.z; v.v[3] = a.w;
+        v.v[4] = b.x; v.v[5] = b.y; v.v[6] = b.z; v.v[7] = b.w;
+        v.v[8] = p2[i];
+        return v;
+    }
 };
 
-// The tables live in static device memory ([direction][table][entry], 32-bit
-// words): filled once per device by a copy, so no transform ever allocates.
+struct Tables {
+    // each entry c stored as c * 2^261 mod r in 29-bit limbs
+    Tab29 roots[2];   // omega_{2^11}^j, j < 1024
+    Tab29 lo[2];      // omega_{2^22}^i, i < 2048
+    Tab29 hi[2];      // omega_{2^22}^(2048 i), i < 2048
+};
+
+// The tables live in static device memory ([direction][table] blocks of
+// three planes, 32-bit words): filled once per device by a copy, so no
+// transform ever allocates.
 constexpr size_t kTwEntries = 1024 + 2048 + 2048;
-__device__ __align__(256) uint32_t g_twiddles[2 * kTwEntries * 8];
+__device__ __align__(256) uint32_t g_twiddles[2 * kTwEntries * 9];
 
 Tables g_tables[64];
 std::once_flag g_tables_once[64];
@@ -69,32 +101,67 @@ std::once_flag g_tables_once[64];
 // c -> c * 2^261 mod r, which is the usual (2^256) form of 32 c
 HostFr scaled29(const HostFr &c) { return c * HostFr::from_u64(32); }
 
+// the 9 x 29-bit limbs of a host value's stored integer
+void limbs29(const HostFr &h, uint32_t out[9])
+{
+    for (int j = 0; j < 9; ++j) {
+        const int bit = 29 * j, q = bit >> 6, sh = bit & 63;
+        uint64_t v = h.l[q] >> sh;
+        if (sh > 35 && q + 1 < 4) v |= h.l[q + 1] << (64 - sh);
+        out[j] = (uint32_t)v & 0x1fffffffu;
+    }
+}
+
+Fr29 to_dev29(const HostFr &h)
+{
+    Fr29 r;
+    limbs29(h, r.v);
+    return r;
+}
+
 const Tables &tables()
 {
     int dev = device_id();
     std::call_once(g_tables_once[dev], [dev]() {
         Tables &t = g_tables[dev];
-        Fr *base = nullptr;
+        uint32_t *base = nullptr;
         MH_HIP(hipGetSymbolAddress(reinterpret_cast<void **>(&base), HIP_SYMBOL(g_twiddles)));
-        std::vector<HostFr> host(2 * kTwEntries);
+        std::vector<uint32_t> host(2 * kTwEntries * 9);
+        // one table of n entries at word offset `off`: planes [n uint4][n uint4][n u32]
+        auto put = [&](size_t off, size_t n, const std::vector<HostFr> &vals, Tab29 &tab) {
+            uint32_t *w = host.data() + off;
+            for (size_t i = 0; i < n; ++i) {
+                uint32_t l[9];
+                limbs29(scaled29(vals[i]), l);
+                for (int j = 0; j < 4; ++j) {
+                    w[4 * i + j] = l[j];
+                    w[4 * n + 4 * i + j] = l[4 + j];
+                }
+                w[8 * n + i] = l[8];
+            }
+            tab.p0 = reinterpret_cast<const uint4 *>(base + off);
+            tab.p1 = reinterpret_cast<const uint4 *>(base + off + 4 * n);
+            tab.p2 = base + off + 8 * n;
+        };
         for (int dir = 0; dir < 2; ++dir) {
             HostFr w22 = HostFr::omega(kTwLog);
             if (dir) w22 = w22.inverse();
             HostFr w11 = w22;
             for (int i = kMaxLogLds; i < kTwLog; ++i) w11 = w11.sqr();
-            HostFr *roots = host.data() + dir * kTwEntries, *lo = roots + 1024, *hi = lo + 2048;
+            std::vector<HostFr> roots(1024), lo(2048), hi(2048);
             HostFr acc = HostFr::one();
-            for (int i = 0; i < 1024; ++i) { roots[i] = scaled29(acc); acc = acc * w11; }
+            for (int i = 0; i < 1024; ++i) { roots[i] = acc; acc = acc * w11; }
             acc = HostFr::one();
-            for (int i = 0; i < 2048; ++i) { lo[i] = scaled29(acc); acc = acc * w22; }
-            HostFr step = acc;   // w22^2048
+            for (int i = 0; i < 2048; ++i) { lo[i] = acc; acc = acc * w22; }
+            const HostFr step = acc;   // w22^2048
             acc = HostFr::one();
-            for (int i = 0; i < 2048; ++i) { hi[i] = scaled29(acc); acc = acc * step; }
-            t.roots[dir] = base + dir * kTwEntries;
-            t.lo[dir] = t.roots[dir] + 1024;
-            t.hi[dir] = t.lo[dir] + 2048;
+            for (int i = 0; i < 2048; ++i) { hi[i] = acc; acc = acc * step; }
+            const size_t off = (size_t)dir * kTwEntries * 9;
+            put(off, 1024, roots, t.roots[dir]);
+            put(off + 1024 * 9, 2048, lo, t.lo[dir]);
+            put(off + 3072 * 9, 2048, hi, t.hi[dir]);
         }
-        MH_HIP(hipMemcpy(base, host.data(), host.size() * 32, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(base, host.data(), host.size() * 4, hipMemcpyHostToDevice));
     });
     return g_tables[dev];
 }
@@ -105,7 +172,7 @@ struct PassArgs {
     uint64_t in_pstride, out_pstride;   // per polynomial (blockIdx.y)
     uint32_t in_es, in_cs;              // logical index = j*es + c*cs
     uint32_t out_es, out_cs;
-    uint32_t log_m, C;
+    uint32_t log_m, log_C;              // column length and columns per workgroup (log2)
     uint32_t log_n;                     // whole transform (mid twiddle)
     uint32_t twiddle;                   // multiply (c, k) by w_n^(c*k)
     uint32_t pre;                       // 0 none, 1 zeta + zero pad, 2 repeating factors
@@ -113,8 +180,8 @@ struct PassArgs {
     uint64_t in_len;                    // logical indices >= in_len read as zero
     const Fr *factors;
     uint64_t factor_len;
-    Fr first, second, scale;            // stored as c * 2^261 mod r
-    const Fr *roots, *lo, *hi;
+    Fr29 first, second, scale;          // stored as c * 2^261 mod r
+    Tab29 roots, lo, hi;
 };
 
 __device__ __forceinline__ uint32_t bitrev(uint32_t x, uint32_t bits)
@@ -122,64 +189,90 @@ __device__ __forceinline__ uint32_t bitrev(uint32_t x, uint32_t bits)
     return __builtin_bitreverse32(x) >> (32 - bits);
 }
 
-__device__ __forceinline__ Fr zeta_factor(const Fr &v, uint64_t idx, const Fr &first, const Fr &second)
-{
-    uint32_t c = (uint32_t)(idx % 3);
-    if (c == 1) return fr_mul_scaled(v, first);
-    if (c == 2) return fr_mul_scaled(v, second);
-    return v;
-}
-
-// LDS holds a pass's C * m elements as two planes of 16-byte halves
-// (element e: low half at plane[e], high half at plane[P + e]), so lanes
-// touching consecutive elements touch consecutive 16-byte words: a wave's
-// ds_read_b128 then covers whole bank rows instead of colliding 8 ways, as
-// contiguous 32-byte elements do.
-struct LdsPlanes {
-    uint4 *plane;
-    uint32_t P;
-    __device__ __forceinline__ Fr get(uint32_t e) const
+// LDS holds a pass's C * m elements as three planes (limbs 0-3 and 4-7 as
+// 16-byte words, limb 8 as a word), so lanes touching consecutive elements
+// touch consecutive words of each plane: a wave's ds_read_b128 covers whole
+// bank rows instead of colliding, as contiguous 36-byte elements would.
+struct Lds29 {
+    uint4 *p0, *p1;
+    uint32_t *p2;
+    __device__ __forceinline__ Lds29(uint32_t *mem, uint32_t total)
+        : p0(reinterpret_cast<uint4 *>(mem)), p1(reinterpret_cast<uint4 *>(mem) + total),
+          p2(mem + 8 * (size_t)total)
     {
-        const uint4 lo = plane[e], hi = plane[P + e];
-        Fr v;
-        v.v[0] = lo.x; v.v[1] = lo.y; v.v[2] = lo.z; v.v[3] = lo.w;
-        v.v[4] = hi.x; v.v[5] = hi.y; v.v[6] = hi.z; v.v[7] = hi.w;
+    }
+    __device__ __forceinline__ Fr29 get(uint32_t e) const
+    {
+        const uint4 a = p0[e], b = p1[e];
+        Fr29 v;
+        v.v[0] = a.x; v.v[1] = a.y; v.v[2] = a.z; v.v[3] = a.w;
+        v.v[4] = b.x; v.v[5] = b.y; v.v[6] = b.z; v.v[7] = b.w;
+        v.v[8] = p2[e];
         return v;
     }
-    __device__ __forceinline__ void put(uint32_t e, const Fr &v) const
+    __device__ __forceinline__ void put(uint32_t e, const Fr29 &v) const
     {
-        plane[e] = make_uint4(v.v[0], v.v[1], v.v[2], v.v[3]);
-        plane[P + e] = make_uint4(v.v[4], v.v[5], v.v[6], v.v[7]);
+        p0[e] = make_uint4(v.v[0], v.v[1], v.v[2], v.v[3]);
+        p1[e] = make_uint4(v.v[4], v.v[5], v.v[6], v.v[7]);
+        p2[e] = v.v[8];
     }
 };
 
-// one workgroup of a pass: columns [bx * C, bx * C + C) of polynomial by
-__device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32_t by, Fr *lds_mem)
+__device__ __forceinline__ Fr29 zeta29(const Fr29 &v, uint64_t idx, const Fr29 &first, const Fr29 &second)
 {
-    const uint32_t m = 1u << a.log_m;
-    const uint32_t C = a.C;
-    const uint32_t total = C * m;
-    const LdsPlanes lds{reinterpret_cast<uint4 *>(lds_mem), total};
-    const uint32_t c0 = bx * C;
+    const uint32_t c = (uint32_t)(idx % 3);
+    if (c == 1) return fr29_mul_loose(v, first);
+    if (c == 2) return fr29_mul_loose(v, second);
+    return v;
+}
+
+// one workgroup of a pass: columns [bx * C, bx * C + C) of polynomial by
+__device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32_t by, uint32_t *lds_mem)
+{
+    const uint32_t log_m = a.log_m, m = 1u << log_m;
+    const uint32_t log_C = a.log_C, C = 1u << log_C;
+    const uint32_t total = C << log_m;
+    const Lds29 lds(lds_mem, total);
+    const uint32_t c0 = bx << log_C;
     const Fr *in = a.in + (uint64_t)by * a.in_pstride;
     Fr *out = a.out + (uint64_t)by * a.out_pstride;
 
-    // load (bit-reversed within each column), applying the pre-operation
+    // load (bit-reversed within each column), applying the pre-operation;
+    // every element enters below 2r, limbs normalized.  A thread issues the
+    // loads of two elements before it converts either, so they are in flight
+    // together.
     const bool load_cfast = (a.in_cs == 1) && C > 1;
-    for (uint32_t f = threadIdx.x; f < total; f += blockDim.x) {
-        uint32_t c, j;
-        if (load_cfast) { c = f % C; j = f / C; }
-        else { j = f % m; c = f / m; }
-        uint64_t L = (uint64_t)j * a.in_es + (uint64_t)(c0 + c) * a.in_cs;
-        Fr v;
-        if (L < a.in_len) {
-            v = load_fr(in + L);
-            if (a.pre == 1) v = zeta_factor(v, L, a.first, a.second);
-            else if (a.pre == 2) v = v * a.factors[L % a.factor_len];
-        } else {
-            v = Fr::zero();
+    constexpr int kLoadUnroll = 2;
+    for (uint32_t f0 = threadIdx.x; f0 < total; f0 += kLoadUnroll * blockDim.x) {
+        Fr x[kLoadUnroll];
+        uint64_t Ls[kLoadUnroll];
+        uint32_t dst[kLoadUnroll];
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; ++u) {
+            const uint32_t f = f0 + u * blockDim.x;
+            uint32_t c, j;
+            if (load_cfast) { c = f & (C - 1); j = f >> log_C; }
+            else { j = f & (m - 1); c = f >> log_m; }
+            Ls[u] = (uint64_t)j * a.in_es + (uint64_t)(c0 + c) * a.in_cs;
+            dst[u] = (c << log_m) + bitrev(j, log_m);
+            if (f < total && Ls[u] < a.in_len) x[u] = load_fr(in + Ls[u]);
+            else x[u] = Fr{};
         }
-        lds.put(c * m + bitrev(j, a.log_m), v);
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; ++u) {
+            if (f0 + u * blockDim.x >= total) break;
+            Fr29 v;
+            if (Ls[u] < a.in_len) {
+                Fr xv = x[u];
+                if (a.pre == 2) xv = xv * a.factors[Ls[u] % a.factor_len];
+                v = fr29_from32(xv);
+                if (a.pre == 1) v = zeta29(v, Ls[u], a.first, a.second);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 9; ++i) v.v[i] = 0;
+            }
+            lds.put(dst[u], v);
+        }
     }
     __syncthreads();
 
@@ -190,75 +283,99 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
     // w_4h^(low+h) -- with one LDS round trip and one barrier for the pair.
     // An odd last stage is a radix-2 one.
     uint32_t s = 0;
-    for (; s + 1 < a.log_m; s += 2) {
+    for (; s + 1 < log_m; s += 2) {
         const uint32_t h = 1u << s;
-        const uint32_t quarter = m >> 2;
-        for (uint32_t g = threadIdx.x; g < C * quarter; g += blockDim.x) {
-            const uint32_t c = g / quarter, gg = g % quarter;
+        const uint32_t lq = log_m - 2;
+        for (uint32_t g = threadIdx.x; g < (C << lq); g += blockDim.x) {
+            const uint32_t c = g >> lq, gg = g & ((1u << lq) - 1);
             const uint32_t low = gg & (h - 1);
-            const uint32_t e = c * m + (((gg >> s) << (s + 2)) | low);
-            Fr x0 = lds.get(e), x1 = lds.get(e + h), x2 = lds.get(e + 2 * h), x3 = lds.get(e + 3 * h);
-            if (low) {
-                const Fr29 w = fr29_from32(a.roots[low << (kMaxLogLds - 1 - s)]);
-                x1 = fr_mul_scaled29(x1, w);
-                x3 = fr_mul_scaled29(x3, w);
+            const uint32_t e = (c << log_m) + (((gg >> s) << (s + 2)) | low);
+            const Fr29 x0 = lds.get(e), x2 = lds.get(e + 2 * h);
+            Fr29 t1 = lds.get(e + h), t3 = lds.get(e + 3 * h);
+            if (s) {
+                const Fr29 w = a.roots.get(low << (kMaxLogLds - 1 - s));
+                t1 = fr29_mul_loose(t1, w);
+                t3 = fr29_mul_loose(t3, w);
             }
-            const Fr y0 = x0 + x1, y1 = x0 - x1;
-            Fr y2 = x2 + x3, y3 = x2 - x3;
-            if (low) y2 = fr_mul_scaled(y2, a.roots[low << (kMaxLogLds - 2 - s)]);
-            y3 = fr_mul_scaled(y3, a.roots[(low + h) << (kMaxLogLds - 2 - s)]);
-            lds.put(e, y0 + y2);
-            lds.put(e + 2 * h, y0 - y2);
-            lds.put(e + h, y1 + y3);
-            lds.put(e + 3 * h, y1 - y3);
+            const Fr29 y0 = fr29_lazy_add(x0, t1), y1 = fr29_lazy_sub4r(x0, t1);
+            const Fr29 y2 = fr29_lazy_add(x2, t3), y3 = fr29_lazy_sub4r(x2, t3);
+            const Fr29 u2 = fr29_mul_loose(y2, a.roots.get(low << (kMaxLogLds - 2 - s)));
+            const Fr29 u3 = fr29_mul_loose(y3, a.roots.get((low + h) << (kMaxLogLds - 2 - s)));
+            Fr29 z0 = fr29_lazy_add(y0, u2), z2 = fr29_lazy_sub4r(y0, u2);
+            Fr29 z1 = fr29_lazy_add(y1, u3), z3 = fr29_lazy_sub4r(y1, u3);
+            fr29_norm(z0);
+            fr29_norm(z1);
+            fr29_norm(z2);
+            fr29_norm(z3);
+            lds.put(e, z0);
+            lds.put(e + 2 * h, z2);
+            lds.put(e + h, z1);
+            lds.put(e + 3 * h, z3);
         }
         __syncthreads();
     }
-    if (s < a.log_m) {
+    if (s < log_m) {
         const uint32_t h = 1u << s;
-        const uint32_t half = m >> 1;
-        for (uint32_t b = threadIdx.x; b < C * half; b += blockDim.x) {
-            const uint32_t c = b / half, bb = b % half;
+        const uint32_t lh = log_m - 1;
+        for (uint32_t b = threadIdx.x; b < (C << lh); b += blockDim.x) {
+            const uint32_t c = b >> lh, bb = b & ((1u << lh) - 1);
             const uint32_t low = bb & (h - 1);
-            const uint32_t e = c * m + (((bb >> s) << (s + 1)) | low);
-            Fr u = lds.get(e);
-            Fr v = lds.get(e + h);
-            if (low) v = fr_mul_scaled(v, a.roots[low << (kMaxLogLds - 1 - s)]);
-            lds.put(e, u + v);
-            lds.put(e + h, u - v);
+            const uint32_t e = (c << log_m) + (((bb >> s) << (s + 1)) | low);
+            const Fr29 u = lds.get(e);
+            Fr29 v = lds.get(e + h);
+            if (s) v = fr29_mul_loose(v, a.roots.get(low << (kMaxLogLds - 1 - s)));
+            Fr29 o0 = fr29_lazy_add(u, v), o1 = fr29_lazy_sub4r(u, v);
+            fr29_norm(o0);
+            fr29_norm(o1);
+            lds.put(e, o0);
+            lds.put(e + h, o1);
         }
         __syncthreads();
     }
 
-    // store, applying the mid twiddle and the post-operation
+    // store, applying the mid twiddle or the post-operation, reducing to
+    // the canonical value (elements are below 46r here)
     const bool store_cfast = (a.out_cs == 1) && C > 1;
     const uint32_t tw_shift = kTwLog - a.log_n;
     for (uint32_t f = threadIdx.x; f < total; f += blockDim.x) {
         uint32_t c, k;
-        if (store_cfast) { c = f % C; k = f / C; }
-        else { k = f % m; c = f / m; }
-        Fr v = lds.get(c * m + k);
+        if (store_cfast) { c = f & (C - 1); k = f >> log_C; }
+        else { k = f & (m - 1); c = f >> log_m; }
+        Fr29 v = lds.get((c << log_m) + k);
+        const uint64_t L = (uint64_t)k * a.out_es + (uint64_t)(c0 + c) * a.out_cs;
         if (a.twiddle) {
-            uint32_t e = ((c0 + c) * k) << tw_shift;
-            if (e) {
-                Fr w = a.lo[e & 2047];
-                if (e >> 11) w = fr_mul_scaled(w, a.hi[e >> 11]);   // stays in the 2^261 form
-                v = fr_mul_scaled(v, w);
-            }
+            // w_n^(c k) from the two tables (w^0 = 1 included); the product
+            // is below 2.4r
+            const uint32_t e = ((c0 + c) * k) << tw_shift;
+            Fr29 w = a.lo.get(e & 2047);
+            if (e >> 11) w = fr29_mul_loose(w, a.hi.get(e >> 11));
+            v = fr29_mul_loose(v, w);
+            fr29_reduce_2r(v);
+            fr29_reduce(v);
+        } else if (a.post) {
+            // below 1.7r after the scale, below 1.1r after a zeta factor
+            v = fr29_mul_loose(v, a.scale);
+            if (a.post == 2) v = zeta29(v, L, a.first, a.second);
+            fr29_reduce(v);
+        } else {
+            fr29_reduce_3r(v);
+            fr29_reduce_2r(v);
+            fr29_reduce(v);
         }
-        uint64_t L = (uint64_t)k * a.out_es + (uint64_t)(c0 + c) * a.out_cs;
-        if (a.post) {
-            v = fr_mul_scaled(v, a.scale);
-            if (a.post == 2) v = zeta_factor(v, L, a.first, a.second);
-        }
-        store_fr(out + L, v);
+        store_fr(out + L, fr29_to32(v));
     }
 }
 
+// Workgroups are dealt to the 8 XCDs round robin by blockIdx.x; neighbouring
+// column groups read the same 128-byte lines of a strided pass, so they are
+// given to blocks on one XCD (sharing its L2): block b takes group
+// (b % 8) * (gx / 8) + b / 8 when gx is a multiple of 8.
 __global__ void __launch_bounds__(kThreads) ntt_pass(PassArgs a)
 {
-    extern __shared__ Fr lds[];
-    pass_body(a, blockIdx.x, blockIdx.y, lds);
+    extern __shared__ uint32_t lds[];
+    const uint32_t b = blockIdx.x, gx = gridDim.x;
+    const uint32_t bx = (gx & 7) ? b : (b & 7) * (gx >> 3) + (b >> 3);
+    pass_body(a, bx, blockIdx.y, lds);
 }
 
 // A pass of every transform of a ragged set in one launch: descriptor d owns
@@ -270,7 +387,7 @@ struct RaggedDesc {
 
 __global__ void __launch_bounds__(kThreads) ntt_pass_ragged(const RaggedDesc *d, uint32_t nd)
 {
-    extern __shared__ Fr lds[];
+    extern __shared__ uint32_t lds[];
     uint32_t lo = 0, hi = nd;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) / 2;
@@ -283,26 +400,34 @@ __global__ void __launch_bounds__(kThreads) ntt_pass_ragged(const RaggedDesc *d,
     pass_body(a, local % gx, local / gx, lds);
 }
 
-uint32_t pick_cols(uint32_t log_m, uint32_t ncols)
+// a 2^11-point column holds 72 KiB of LDS, above the default per-launch
+// limit: raised once per device for both kernels
+std::once_flag g_lds_once[64];
+void lds_limit()
 {
-    uint32_t C = 1;
-    while (C * 2 <= ncols && ((C * 2) << log_m) <= 2048) C *= 2;
-    return C;
+    std::call_once(g_lds_once[device_id()], []() {
+        MH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(ntt_pass),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax));
+        MH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(ntt_pass_ragged),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax));
+    });
 }
+
+// log2 of the columns per workgroup: C * m <= 2048 elements (72 KiB)
+uint32_t pick_log_cols(uint32_t log_m, uint32_t ncols)
+{
+    uint32_t lc = 0;
+    while ((2u << lc) <= ncols && ((2u << lc) << log_m) <= kLdsElems) ++lc;
+    return lc;
+}
+
+size_t lds_bytes(const PassArgs &a) { return ((size_t)1 << (a.log_C + a.log_m)) * kLdsElemBytes; }
 
 void launch(const PassArgs &a, uint32_t ncols, uint32_t npolys, hipStream_t st)
 {
-    dim3 grid(ncols / a.C, npolys);
-    size_t lds = (size_t)a.C * (1u << a.log_m) * sizeof(Fr);
-    hipLaunchKernelGGL(ntt_pass, grid, dim3(kThreads), lds, st, a);
+    dim3 grid(ncols >> a.log_C, npolys);
+    hipLaunchKernelGGL(ntt_pass, grid, dim3(kThreads), lds_bytes(a), st, a);
     MH_HIP(hipGetLastError());
-}
-
-Fr to_dev(const HostFr &h)
-{
-    Fr r;
-    std::memcpy(r.v, h.l, 32);
-    return r;
 }
 
 }  // namespace
@@ -320,6 +445,7 @@ int job_passes(const NttJob &job, PassLaunch out[2])
     const uint32_t log_n = job.log_n;
     MH_CHECK(log_n <= 2 * kMaxLogLds, hipErrorInvalidValue, "NTT size above 2^22 is not supported");
     const Tables &T = tables();
+    lds_limit();
     const int dir = job.inverse ? 1 : 0;
     const uint64_t n = 1ull << log_n;
 
@@ -344,9 +470,9 @@ int job_passes(const NttJob &job, PassLaunch out[2])
     base.lo = T.lo[dir];
     base.hi = T.hi[dir];
     base.log_n = log_n;
-    base.first = to_dev(scaled29(job.first));
-    base.second = to_dev(scaled29(job.second));
-    base.scale = to_dev(scaled29(post_scale));
+    base.first = to_dev29(scaled29(job.first));
+    base.second = to_dev29(scaled29(job.second));
+    base.scale = to_dev29(scaled29(post_scale));
     base.factors = job.factors;
     base.factor_len = job.factor_len;
 
@@ -360,7 +486,7 @@ int job_passes(const NttJob &job, PassLaunch out[2])
         a.in_es = 1; a.in_cs = 0;
         a.out_es = 1; a.out_cs = 0;
         a.log_m = log_n;
-        a.C = 1;
+        a.log_C = 0;
         a.twiddle = 0;
         a.pre = job.pre;
         a.in_len = job.in_len;
@@ -381,7 +507,7 @@ int job_passes(const NttJob &job, PassLaunch out[2])
     A.in_es = n2; A.in_cs = 1;
     A.out_es = n2; A.out_cs = 1;
     A.log_m = t1;
-    A.C = pick_cols(t1, n2);
+    A.log_C = pick_log_cols(t1, n2);
     A.twiddle = 1;
     A.pre = job.pre;
     A.in_len = job.in_len;
@@ -396,7 +522,7 @@ int job_passes(const NttJob &job, PassLaunch out[2])
     B.in_es = 1; B.in_cs = n2;
     B.out_es = n1; B.out_cs = 1;
     B.log_m = t2;
-    B.C = pick_cols(t2, n1);
+    B.log_C = pick_log_cols(t2, n1);
     B.twiddle = 0;
     B.pre = 0;
     B.in_len = ~0ull;
@@ -467,12 +593,12 @@ void RaggedNtt::prepare(Fr *base, const size_t *offsets, const uint32_t *log_ns,
         PassLaunch p[2];
         const int np = job_passes(job, p);
         for (int s = 0; s < np; ++s) {
-            const uint32_t gx = p[s].ncols / p[s].a.C;
+            const uint32_t gx = p[s].ncols >> p[s].a.log_C;
             const uint64_t nb = (uint64_t)gx * p[s].npolys;
             MH_CHECK((uint64_t)im.blocks[s] + nb < (1ull << 31), hipErrorInvalidValue, "ragged NTT set too large");
             im.host[s].push_back(RaggedDesc{p[s].a, gx, im.blocks[s]});
             im.blocks[s] += (uint32_t)nb;
-            im.lds[s] = std::max<size_t>(im.lds[s], (size_t)p[s].a.C * (1u << p[s].a.log_m) * sizeof(Fr));
+            im.lds[s] = std::max<size_t>(im.lds[s], lds_bytes(p[s].a));
         }
     }
     for (int s = 0; s < 2; ++s) {
