@@ -36,6 +36,7 @@ struct Candidate {
     bool precomputed;
     uint32_t window;      // plain windows: width (0 = automatic)
     uint32_t chunk_len;
+    bool small = false;   // precomputed: the context's small table
 };
 
 }  // namespace
@@ -65,7 +66,8 @@ struct BatchPlan {
     {
         engine.chunk_len = c.chunk_len;
         if (c.precomputed) {
-            PrecompTable pc{ctx->table.get(), ctx->npoints, ctx->pc_c, ctx->pc_nwin};
+            const PrecompTable pc = c.small ? PrecompTable{ctx->small_table.get(), ctx->sm_n, ctx->sm_c, ctx->sm_nwin}
+                                            : PrecompTable{ctx->table.get(), ctx->npoints, ctx->pc_c, ctx->pc_nwin};
             engine.prepare(ctx->points.get(), ctx->npoints, jobs, 0, st, &pc);
         } else {
             engine.prepare(ctx->points.get(), ctx->npoints, jobs, c.window, st, nullptr);
@@ -142,16 +144,21 @@ extern "C" RustError midnight_hip_batch_plan_create(void **out, void *msm_contex
     // the MSM configuration: timed candidates, or the engine's own choice
     std::vector<Candidate> cands;
     const bool has_pc = ctx->pc_nwin != 0;
+    size_t max_end = 0;
+    for (const MsmJob &j : plan->jobs) max_end = std::max<size_t>(max_end, j.pt_off + j.n);
+    const bool has_small = ctx->sm_nwin != 0 && max_end <= ctx->sm_n;
+    PrecompTable pc;
     if (flags & kPlanAutotune) {
         if (has_pc) {
             cands.push_back({true, 0, 64});
             cands.push_back({true, 0, 32});
         }
+        if (has_small) cands.push_back({true, 0, 64, true});
         const uint32_t w = msm_auto_window(max_n);
         for (uint32_t c : {w - 1, w, w + 1})
             if (c >= 8 && c <= 16) cands.push_back({false, c, 64});
-    } else if (has_pc && max_n >= (size_t(1) << (ctx->pc_c - 1))) {
-        cands.push_back({true, 0, 64});
+    } else if (context_table(*ctx, max_n, max_end, pc)) {
+        cands.push_back({true, 0, 64, pc.table == ctx->small_table.get()});
     } else {
         cands.push_back({false, 0, 64});
     }

@@ -774,9 +774,9 @@ uint32_t rule_table_window(size_t n)
 
 // the per-window table of window c into `table`; false when its entries
 // would not fit 31-bit indices
-bool build_table(const MsmContext &ctx, uint32_t c, DevBuf<uint32_t> &table, hipStream_t st)
+bool build_table(const MsmContext &ctx, uint32_t c, DevBuf<uint32_t> &table, hipStream_t st, size_t rows = 0)
 {
-    const size_t n = ctx.npoints;
+    const size_t n = rows ? rows : ctx.npoints;
     const uint32_t nwin = (256 + c - 1) / c;
     if ((uint64_t)nwin * n >= (1ull << 31)) return false;   // entry indices are 31-bit
     table.alloc((size_t)nwin * n * kRow28);   // Affine28 rows
@@ -806,16 +806,35 @@ float time_table(MsmEngine &eng, const MsmContext &ctx, const std::vector<MsmJob
 
 }  // namespace
 
-// Builds the context's per-window table.  A config with window_bits set
-// gets that window (10..16; the reference's 17 and 18 are capped at 16, the
-// widest key range of the LDS digit sort).  Otherwise contexts of 2^16
-// points or more are autotuned here, once: a batch of four MSMs of uniform
-// scalars over the whole point set is timed with the rule's window and one bit either
-// side (within 10..16), each with accumulation chunks of 64 and 32 entries,
-// and the fastest table and chunk length are kept (the reference's
-// counterpart is its hand-measured per-k table, lib.rs:159-203).  MSM
-// results do not depend on the choice.
-void build_precomp(MsmContext &ctx, hipStream_t st)
+namespace {
+
+// The small table: window kSmallTableC over the points below 2^(pc_c - 1),
+// built for autotuned contexts whose main table is wider (a 2^20-point SRS
+// serving proofs of k = 10..14 is the case).  At c = 12 an MSM of 2^12
+// points has 22 x 2^12 entries in 2048 buckets, against 32 x 2^12 entries
+// in 32 x 128 buckets and a 256-bit host chain on plain 8-bit windows.
+constexpr uint32_t kSmallTableC = 12;
+
+void build_small_table(MsmContext &ctx, hipStream_t st)
+{
+    if (ctx.window_bits || !ctx.pc_nwin || ctx.pc_c <= kSmallTableC) return;
+    const size_t rows = std::min<size_t>(ctx.npoints, size_t(1) << (ctx.pc_c - 1));
+    if (!build_table(ctx, kSmallTableC, ctx.small_table, st, rows)) return;
+    ctx.sm_c = kSmallTableC;
+    ctx.sm_nwin = (256 + kSmallTableC - 1) / kSmallTableC;
+    ctx.sm_n = rows;
+}
+
+// The main per-window table.  A config with window_bits set gets that
+// window (10..16; the reference's 17 and 18 are capped at 16, the widest
+// key range of the LDS digit sort).  Otherwise contexts of 2^16 points or
+// more are autotuned here, once: a batch of four MSMs of uniform scalars
+// over the whole point set is timed with the rule's window and one bit
+// either side (within 10..16), each with accumulation chunks of 64 and 32
+// entries, and the fastest table and chunk length are kept (the
+// reference's counterpart is its hand-measured per-k table,
+// lib.rs:159-203).  MSM results do not depend on the choice.
+void build_precomp_main(MsmContext &ctx, hipStream_t st)
 {
     const size_t n = ctx.npoints;
     if (n < (size_t(1) << 10)) return;
@@ -858,6 +877,27 @@ void build_precomp(MsmContext &ctx, hipStream_t st)
     }
     ctx.tuned_ms = best;
     MH_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+void build_precomp(MsmContext &ctx, hipStream_t st)
+{
+    build_precomp_main(ctx, st);
+    build_small_table(ctx, st);
+}
+
+bool context_table(const MsmContext &ctx, size_t max_n, size_t max_end, PrecompTable &pc)
+{
+    if (ctx.pc_nwin && max_n >= (size_t(1) << (ctx.pc_c - 1))) {
+        pc = PrecompTable{ctx.table.get(), ctx.npoints, ctx.pc_c, ctx.pc_nwin};
+        return true;
+    }
+    if (ctx.sm_nwin && max_end <= ctx.sm_n) {
+        pc = PrecompTable{ctx.small_table.get(), ctx.sm_n, ctx.sm_c, ctx.sm_nwin};
+        return true;
+    }
+    return false;
 }
 
 // ---------------------------------------------------------------- engine
@@ -990,6 +1030,7 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
     uint64_t max_n = 0, total_pairs = 0;
     for (const auto &j : jobs) {
         MH_CHECK(j.pt_off + j.n <= npoints_total, (int)hipErrorInvalidValue, "MSM range exceeds the point set");
+        MH_CHECK(!pc || j.pt_off + j.n <= pc->n, (int)hipErrorInvalidValue, "MSM range exceeds the table");
         max_n = std::max<uint64_t>(max_n, j.n);
         total_pairs += j.n;
     }
@@ -1414,14 +1455,17 @@ void run_batched(MsmContext &ctx, MsmEngine &engine, const std::vector<MsmJob> &
             ++j;
         }
         std::vector<MsmJob> part(jobs.begin() + i, jobs.begin() + j);
-        size_t part_max = 0;
-        for (const auto &pj : part) part_max = std::max<size_t>(part_max, pj.n);
-        // the table's one bucket set has 2^(c-1) buckets; MSMs much shorter
-        // than that (a small circuit over a large SRS) spend less on plain
-        // windows sized for their own length
+        size_t part_max = 0, part_end = 0;
+        for (const auto &pj : part) {
+            part_max = std::max<size_t>(part_max, pj.n);
+            part_end = std::max<size_t>(part_end, pj.pt_off + pj.n);
+        }
+        // the main table's one bucket set has 2^(c-1) buckets; MSMs much
+        // shorter than that (a small circuit over a large SRS) take the small
+        // table when it covers them, else plain windows sized for their length
         engine.chunk_len = ctx.chunk_len;
-        if (ctx.pc_nwin && part_max >= (size_t(1) << (ctx.pc_c - 1))) {
-            PrecompTable pc{ctx.table.get(), ctx.npoints, ctx.pc_c, ctx.pc_nwin};
+        PrecompTable pc;
+        if (context_table(ctx, part_max, part_end, pc)) {
             engine.run(ctx.points.get(), ctx.npoints, part, 0, out + i, st, &pc);
         } else {
             engine.run(ctx.points.get(), ctx.npoints, part, engine_window(ctx), out + i, st);

@@ -19,6 +19,13 @@ struct MsmContext {
     // per-window copies [2^(c w)] P_i (w < pc_nwin), built once per context
     DevBuf<uint32_t> table;
     uint32_t pc_c = 0, pc_nwin = 0;
+    // A narrow-window table [2^(sm_c w)] P_i over the first sm_n points, for
+    // MSMs too short for the main table's 2^(pc_c - 1) buckets: its one
+    // bucket set stays small and the host finishes with sm_c doublings, where
+    // plain windows leave a doubling chain over all 256 bits.
+    DevBuf<uint32_t> small_table;
+    uint32_t sm_c = 0, sm_nwin = 0;
+    size_t sm_n = 0;
     uint32_t window_bits = 0;  // a caller config's window (0: autotuned table / automatic plain windows)
     uint32_t chunk_len = 64;   // accumulation chunk bound of this context's MSMs
     float tuned_ms = 0;        // autotuned MSM time at creation (0: not tuned)
@@ -45,6 +52,11 @@ std::unique_ptr<MsmContext> make_context(size_t npoints);
 // (no-op for small point sets, which use plain windows): the config's
 // window_bits when set, else autotuned for contexts of 2^16 points or more.
 void build_precomp(MsmContext &ctx, hipStream_t st);
+// The precomputed table for a batch whose longest MSM has max_n points and
+// whose points all lie below max_end: the main table when max_n reaches
+// half its bucket count, else the small table when it covers max_end.
+// False: plain windows.
+bool context_table(const MsmContext &ctx, size_t max_n, size_t max_end, PrecompTable &pc);
 void generator_multiples(uint32_t *out_points, const Fr *scalars, size_t n, hipStream_t st);
 
 }  // namespace mh

@@ -286,8 +286,8 @@ void stage_commit(StageGraph &g, PlonkKey &pk, LaneProof &lp, Fr *base, size_t c
         for (size_t i = 0; i < count; ++i) jobs[i] = MsmJob{base + i * n, 0, n};
         // the same table / plain-window choice as the context's own batches
         g.engine.chunk_len = ctx.chunk_len;
-        if (ctx.pc_nwin && n >= (size_t(1) << (ctx.pc_c - 1))) {
-            const PrecompTable pc{ctx.table.get(), ctx.npoints, ctx.pc_c, ctx.pc_nwin};
+        PrecompTable pc;
+        if (context_table(ctx, n, n, pc)) {
             g.engine.prepare(ctx.points.get(), ctx.npoints, jobs, 0, st, &pc);
         } else {
             g.engine.prepare(ctx.points.get(), ctx.npoints, jobs, ctx.window_bits, st);

@@ -319,6 +319,47 @@ def test_msm_short_on_large_precomputed_context(H, O):
     dv_t.free()
 
 
+def test_msm_small_table_on_wide_context(H, O):
+    """A context whose main table is wider than 12 bits (2^18 points) also
+    holds a 12-bit table over its first 2^(c-1) points: batches of short
+    MSMs within it (ragged, uniform and repeated scalars) take that table,
+    each against the closed form, and either side of the switch to the
+    main table."""
+    n = 1 << 18
+    ts = O.random_fr(n, 1210)
+    dv_t = H.DeviceVec.upload(ts)
+    ctx = H.MsmContext.from_multiples(dv_t)
+    w, _, _ = ctx.tuning()
+    assert w > 12
+    m = (1 << 12) - 1                                   # below every main window's switch
+    sizes = [1, 100, 1000, m, m]
+    sc = O.random_fr(sum(sizes), 1211)
+    sc[sizes[0] + sizes[1]:sum(sizes[:3])] = sc[sizes[0] + sizes[1]]     # one scalar 1000 times
+    dv = H.DeviceVec.upload(sc)
+    offs = np.cumsum([0] + sizes[:-1]).tolist()
+    out = ctx.device_multi(dv, offs, sizes)
+    assert ctx.last_stats()[1] == 12
+    for i, (o, m) in enumerate(zip(offs, sizes)):
+        assert O.g1_eq(out[i], O.g1_mul(O.generator(), O.inner_product(sc[o:o + m], ts[:m]))), m
+    got = ctx.device_batch(dv, sum(sizes[:3]), m, 2)
+    assert ctx.last_stats()[1] == 12
+    for i in range(2):
+        o = sum(sizes[:3]) + i * m
+        assert O.g1_eq(got[i], O.g1_mul(O.generator(), O.inner_product(sc[o:o + m], ts[:m]))), i
+    dv.free()
+    # either side of the switch to the main table at 2^(w-1) points
+    big = 1 << (w - 1)
+    sc2 = O.random_fr(big, 1212)
+    dv2 = H.DeviceVec.upload(sc2)
+    for m, window in ((big - 1, 12), (big, w)):
+        got2 = ctx.device_batch(dv2, 0, m, 1)[0]
+        assert ctx.last_stats()[1] == window, m
+        assert O.g1_eq(got2, O.g1_mul(O.generator(), O.inner_product(sc2[:m], ts[:m]))), m
+    dv2.free()
+    ctx.free()
+    dv_t.free()
+
+
 def _reference_points(O, n):
     """points (i + 1) G, as the reference's ordering tests use"""
     ts = np.stack([O.fr_from_int(i + 1) for i in range(n)])
