@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "devvec.hpp"
@@ -60,7 +61,38 @@ hipStream_t lane_stream(uint32_t lane)
     return s;
 }
 
-void *dev_alloc(size_t bytes, hipStream_t st)
+namespace {
+
+// Per-stream caches of freed blocks.  A block freed on a stream is handed
+// to the next allocation of its size class on the same stream: stream order
+// puts every use of the new owner after every use of the old one, as the
+// pool's own reuse would, without the pool's device-wide lock on every call
+// (a proof makes about 70 temporary allocations; twelve lanes proving at
+// once spent longer queued on that lock than on their kernels).  Blocks up
+// to kCacheBlockMax are kept, up to kCacheStreamMax bytes per stream; the
+// rest go back to the pool.  Cached memory stays with the process.
+constexpr size_t kCacheBlockMax = size_t(1) << 30;
+constexpr size_t kCacheStreamMax = size_t(8) << 30;
+
+// size classes in quarter steps of powers of two (at most 25% over)
+size_t size_class(size_t bytes)
+{
+    if (bytes <= 256) return 256;
+    int lg = 63 - __builtin_clzll(bytes - 1);   // 2^lg < bytes <= 2^(lg+1)
+    const size_t step = (size_t(1) << lg) >> 2;
+    return ((bytes + step - 1) / step) * step;
+}
+
+struct StreamCache {
+    std::unordered_multimap<size_t, void *> free;
+    size_t bytes = 0;
+};
+
+std::mutex g_cache_mu;
+std::unordered_map<hipStream_t, StreamCache> g_cache;
+std::unordered_map<void *, size_t> g_block_class;   // every live or cached block
+
+void pool_setup()
 {
     static std::once_flag pool_once[64];
     const int dev = device_id();
@@ -71,12 +103,69 @@ void *dev_alloc(size_t bytes, hipStream_t st)
             (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
         }
     });
+}
+
+}  // namespace
+
+void *dev_alloc(size_t bytes, hipStream_t st)
+{
+    const size_t cls = size_class(bytes);
+    {
+        std::lock_guard<std::mutex> g(g_cache_mu);
+        auto c = g_cache.find(st);
+        if (c != g_cache.end()) {
+            auto it = c->second.free.find(cls);
+            if (it != c->second.free.end()) {
+                void *p = it->second;
+                c->second.free.erase(it);
+                c->second.bytes -= cls;
+                return p;
+            }
+        }
+    }
+    pool_setup();
     void *p = nullptr;
-    MH_HIP(hipMallocAsync(&p, bytes, st));
+    if (hipMallocAsync(&p, cls, st) != hipSuccess) {
+        // out of memory: hand every cached block back to the pool, wait for
+        // the frees to complete, and try once more
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> g(g_cache_mu);
+            for (auto &c : g_cache) {
+                for (auto &blk : c.second.free) {
+                    g_block_class.erase(blk.second);
+                    (void)hipFreeAsync(blk.second, c.first);
+                }
+                c.second.free.clear();
+                c.second.bytes = 0;
+            }
+        }
+        MH_HIP(hipDeviceSynchronize());
+        MH_HIP(hipMallocAsync(&p, cls, st));
+    }
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    g_block_class[p] = cls;
     return p;
 }
 
-void dev_free(void *p, hipStream_t st) { (void)hipFreeAsync(p, st); }
+void dev_free(void *p, hipStream_t st)
+{
+    {
+        std::lock_guard<std::mutex> g(g_cache_mu);
+        auto b = g_block_class.find(p);
+        if (b != g_block_class.end()) {
+            const size_t cls = b->second;
+            StreamCache &c = g_cache[st];
+            if (cls <= kCacheBlockMax && c.bytes + cls <= kCacheStreamMax) {
+                c.free.emplace(cls, p);
+                c.bytes += cls;
+                return;
+            }
+            g_block_class.erase(b);
+        }
+    }
+    (void)hipFreeAsync(p, st);
+}
 
 // per-lane reusable staging/scratch buffers for the host-bridge entry points
 namespace {

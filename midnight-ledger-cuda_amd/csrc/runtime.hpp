@@ -68,7 +68,8 @@ int device_id();
 // stream.  hipFree would synchronise every stream of the device, so proofs
 // running on different lanes would stall each other at every temporary
 // buffer; the pool keeps freed memory for reuse (release threshold raised
-// once per device).
+// once per device).  In front of the pool, a per-stream cache hands a freed
+// block to the next allocation of its size class on the same stream.
 void *dev_alloc(size_t bytes, hipStream_t st);
 void dev_free(void *p, hipStream_t st);
 
