@@ -34,6 +34,32 @@ constexpr size_t kEvalFill = 256 * 4 * 2 * 64;   // lanes that give every SIMD t
 constexpr int kScanItems = 8;
 constexpr size_t kScanTile = (size_t)kBlock * kScanItems;
 
+constexpr uint32_t kPowPoints = 16;
+struct PowPoints {
+    Fr p[kPowPoints];
+};
+
+// one lane per entry of the power tables of up to kPowPoints points
+__global__ __launch_bounds__(kBlock) void k_omega_tables(Fr *tab, uint32_t levels, PowPoints pts, uint32_t npts)
+{
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t per = (size_t)levels * kOmegaTab;
+    if (t >= npts * per) return;
+    const uint32_t p = (uint32_t)(t / per), r = (uint32_t)(t % per);
+    const uint32_t l = r >> kOmegaBits, j = r & (kOmegaTab - 1);
+    Fr base = pts.p[0];
+#pragma unroll
+    for (uint32_t i = 1; i < kPowPoints; ++i)   // selected, not indexed: the argument stays in registers
+        if (p == i) base = pts.p[i];
+    for (uint32_t i = 0; i < kOmegaBits * l; ++i) base = base * base;
+    Fr acc = Fr::one();
+    for (int b = kOmegaBits - 1; b >= 0; --b) {
+        acc = acc * acc;
+        if ((j >> b) & 1u) acc = acc * base;
+    }
+    store_fr(tab + t, acc);
+}
+
 // partial[(poly * npts + pt) * nblk + block] = sum over the block's rows
 __global__ __launch_bounds__(kBlock) void k_eval(const Fr *polys, size_t len, uint32_t chunk, const Fr *tabs,
                                                  uint32_t levels, uint32_t npts, Fr *partial)
@@ -270,6 +296,19 @@ size_t eval_chunk(size_t len, size_t sums)
 
 }  // namespace
 
+void omega_tables_device(Fr *tab, uint32_t levels, const HostFr *points, size_t npts, hipStream_t st)
+{
+    const size_t per = (size_t)levels * kOmegaTab;
+    for (size_t p0 = 0; p0 < npts; p0 += kPowPoints) {
+        const size_t m = std::min<size_t>(kPowPoints, npts - p0);
+        PowPoints pts{};
+        for (size_t i = 0; i < m; ++i) std::memcpy(pts.p[i].v, points[p0 + i].l, 32);
+        hipLaunchKernelGGL(k_omega_tables, dim3((unsigned)ceil_div(m * per, kBlock)), dim3(kBlock), 0, st,
+                           tab + p0 * per, levels, pts, (uint32_t)m);
+        MH_HIP(hipGetLastError());
+    }
+}
+
 void poly_eval_many(HostFr *out, const Fr *polys, size_t len, size_t count, const HostFr *points, size_t npts,
                     hipStream_t st)
 {
@@ -334,15 +373,14 @@ void poly_kate_divide(Fr *q, const Fr *p, size_t len, const HostFr &a, hipStream
         hipLaunchKernelGGL(k_shift_down, dim3((unsigned)ceil_div(len, kBlock)), dim3(kBlock), 0, st, q,
                            (const Fr *)w.get(), len);
         MH_HIP(hipGetLastError());
-        MH_HIP(hipStreamSynchronize(st));
         return;
     }
     if (len < 2) {   // the quotient of a constant is zero
         MH_HIP(hipMemsetAsync(q, 0, 32, st));
-        MH_HIP(hipStreamSynchronize(st));
         return;
     }
-    // the power tables of a and a^-1, back to back (one upload)
+    // the power tables of a and a^-1, back to back (stream-ordered: the
+    // temporaries return to this stream's cache, nothing waits here)
     DevBuf<Fr> tabs, w(len);
     uint32_t levels = 1;
     const HostFr pts[2] = {a, a.inverse()};
@@ -360,7 +398,6 @@ void poly_kate_divide(Fr *q, const Fr *p, size_t len, const HostFr &a, hipStream
     hipLaunchKernelGGL(k_kate_finish, dim3(grid), dim3(kBlock), 0, st, q, (const Fr *)w.get(),
                        (const Fr *)totals.get(), len, inv_tab, levels);
     MH_HIP(hipGetLastError());
-    MH_HIP(hipStreamSynchronize(st));
 }
 
 void poly_eval_queries(HostFr *out, const std::vector<const Fr *> &polys, size_t len, const uint32_t *qpoly,

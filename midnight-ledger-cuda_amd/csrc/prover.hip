@@ -665,23 +665,37 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
         poly_eval_many(&v, poly, n, 1, &pt, 1, st);
         return v;
     };
-    for (auto &q : pk.advice_queries) T.write_scalar(eval1(pd + (P_ADV + q.first) * n, rotate(pk, x, q.second)));
+    // every evaluation the transcript takes after x, in its order, computed
+    // by one launch and read back once: advice and fixed queries, the random
+    // polynomial, the sigmas, then per product set z(x), z(x w) and (all but
+    // the last set) z(x w^-(bf+1))
     const Fr *fixed = pk.fixed_coeff->values.get();
-    for (auto &q : pk.fixed_queries) T.write_scalar(eval1(fixed + (size_t)q.first * n, rotate(pk, x, q.second)));
-    T.write_scalar(eval1(pd + P_RAND * n, x));
-    {
-        std::vector<HostFr> sv(pk.perm_cols);
-        poly_eval_many(sv.data(), pk.sigma_coeff->values.get(), n, pk.perm_cols, &x, 1, st);
-        for (auto &v : sv) T.write_scalar(v);
-    }
     const HostFr x_next = rotate(pk, x, 1), x_last = rotate(pk, x, -(int32_t)(pk.bf + 1));
-    for (size_t s = 0; s < S; ++s) {
-        const HostFr pts[3] = {x, x_next, x_last};
-        HostFr ev[3];
-        poly_eval_many(ev, pd + (P_Z + s) * n, n, 1, pts, s + 1 < S ? 3 : 2, st);
-        T.write_scalar(ev[0]);
-        T.write_scalar(ev[1]);
-        if (s + 1 < S) T.write_scalar(ev[2]);
+    {
+        std::vector<HostFr> epts;
+        std::vector<const Fr *> epolys;
+        std::vector<uint32_t> qpoly, qpt;
+        auto add = [&](const Fr *poly, const HostFr &pt) {
+            size_t pi = 0;
+            while (pi < epts.size() && !(epts[pi] == pt)) ++pi;
+            if (pi == epts.size()) epts.push_back(pt);
+            qpoly.push_back((uint32_t)epolys.size());
+            epolys.push_back(poly);
+            qpt.push_back((uint32_t)pi);
+        };
+        for (auto &q : pk.advice_queries) add(pd + (P_ADV + q.first) * n, rotate(pk, x, q.second));
+        for (auto &q : pk.fixed_queries) add(fixed + (size_t)q.first * n, rotate(pk, x, q.second));
+        add(pd + P_RAND * n, x);
+        for (size_t j = 0; j < pk.perm_cols; ++j) add(pk.sigma_coeff->values.get() + j * n, x);
+        for (size_t s = 0; s < S; ++s) {
+            add(pd + (P_Z + s) * n, x);
+            add(pd + (P_Z + s) * n, x_next);
+            if (s + 1 < S) add(pd + (P_Z + s) * n, x_last);
+        }
+        std::vector<HostFr> ev(qpoly.size());
+        poly_eval_queries(ev.data(), epolys, n, qpoly.data(), qpt.data(), qpoly.size(), epts.data(), epts.size(),
+                          st);
+        for (auto &v : ev) T.write_scalar(v);
     }
 
     // h(X) for the opening: sum_i x^((n-1) i) h_i(X) (vanishing/prover.rs:321-369)
@@ -783,10 +797,8 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     T.write_point(com[0]);
     const HostFr x3 = T.squeeze();
     std::vector<HostFr> qe(nsets);
-    for (size_t oi = 0; oi < nsets; ++oi) {
-        qe[oi] = eval1(qv->values.get() + order[oi] * n, x3);
-        T.write_scalar(qe[oi]);
-    }
+    poly_eval_many(qe.data(), qv->values.get(), n, nsets, &x3, 1, st);   // in set order
+    for (size_t oi = 0; oi < nsets; ++oi) T.write_scalar(qe[order[oi]]);
     const HostFr x4 = T.squeeze();
     {
         std::vector<const Fr *> ps;

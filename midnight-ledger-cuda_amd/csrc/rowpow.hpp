@@ -4,6 +4,7 @@
 // of one entry per level (one product per row up to 2^22 rows), and the
 // table (64 KB at 2^22) stays in cache.
 #pragma once
+#include <algorithm>
 #include <vector>
 
 #include "fields.hpp"
@@ -23,50 +24,32 @@ __device__ __forceinline__ Fr omega_pow_tab(size_t idx, const Fr *tab, uint32_t 
     return r;
 }
 
-inline void build_omega_table(DevBuf<Fr> &tab, uint32_t &levels, const HostFr &omega, size_t len, hipStream_t st)
+// tab[(p * levels + l) * kOmegaTab + j] = points[p]^(j * 2^(11 l)), computed
+// on the device (square-and-multiply per entry, no upload and no wait)
+void omega_tables_device(Fr *tab, uint32_t levels, const HostFr *points, size_t npts, hipStream_t st);
+
+inline uint32_t omega_levels(size_t len)
 {
     uint32_t bits = 0;
     while (bits < 64 && (1ull << bits) < len) ++bits;
-    levels = bits <= (uint32_t)kOmegaBits ? 1 : (bits + kOmegaBits - 1) / kOmegaBits;
-    std::vector<HostFr> h((size_t)levels * kOmegaTab);
-    HostFr step = omega;
-    for (uint32_t l = 0; l < levels; ++l) {
-        HostFr acc = HostFr::one();
-        for (uint32_t j = 0; j < kOmegaTab; ++j) {
-            h[(size_t)l * kOmegaTab + j] = acc;
-            acc = acc * step;
-        }
-        step = acc;   // omega^(2048 * 2^(11 l))
-    }
-    tab.alloc(h.size());
-    MH_HIP(hipMemcpyAsync(tab.get(), h.data(), h.size() * 32, hipMemcpyHostToDevice, st));
-    MH_HIP(hipStreamSynchronize(st));
+    return bits <= (uint32_t)kOmegaBits ? 1 : (bits + kOmegaBits - 1) / kOmegaBits;
+}
+
+inline void build_omega_table(DevBuf<Fr> &tab, uint32_t &levels, const HostFr &omega, size_t len, hipStream_t st)
+{
+    levels = omega_levels(len);
+    tab.alloc((size_t)levels * kOmegaTab);
+    omega_tables_device(tab.get(), levels, &omega, 1, st);
 }
 
 // The tables of npts points back to back (point p at p * levels *
-// kOmegaTab), with one upload and one synchronisation for all of them.
+// kOmegaTab).
 inline void build_omega_tables(DevBuf<Fr> &tab, uint32_t &levels, const HostFr *points, size_t npts, size_t len,
                                hipStream_t st)
 {
-    uint32_t bits = 0;
-    while (bits < 64 && (1ull << bits) < len) ++bits;
-    levels = bits <= (uint32_t)kOmegaBits ? 1 : (bits + kOmegaBits - 1) / kOmegaBits;
-    const size_t per = (size_t)levels * kOmegaTab;
-    std::vector<HostFr> h(std::max<size_t>(npts * per, 1));
-    for (size_t p = 0; p < npts; ++p) {
-        HostFr step = points[p];
-        for (uint32_t l = 0; l < levels; ++l) {
-            HostFr acc = HostFr::one();
-            for (uint32_t j = 0; j < kOmegaTab; ++j) {
-                h[p * per + (size_t)l * kOmegaTab + j] = acc;
-                acc = acc * step;
-            }
-            step = acc;
-        }
-    }
-    tab.alloc(h.size());
-    MH_HIP(hipMemcpyAsync(tab.get(), h.data(), h.size() * 32, hipMemcpyHostToDevice, st));
-    MH_HIP(hipStreamSynchronize(st));
+    levels = omega_levels(len);
+    tab.alloc(std::max<size_t>(npts * levels * kOmegaTab, 1));
+    omega_tables_device(tab.get(), levels, points, npts, st);
 }
 
 }  // namespace mh
