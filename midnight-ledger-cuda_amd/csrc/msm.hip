@@ -1060,7 +1060,14 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
     s.nkeys = g.keys();
     s.nent_max = (uint64_t)g.nwin * total_pairs;
     MH_CHECK(s.nent_max < (1ull << 32) && s.nkeys < (1ull << 31), (int)hipErrorInvalidValue, "MSM batch too large");
-    s.L = fill_group(s.nent_max, chunk_len);
+    // chunk length: enough lanes to fill the GPU, and at least the square
+    // root of the mean bucket length, which balances a bucket's serial
+    // chain (L mixed additions per chunk, then its ~mean/L chunk partials)
+    // on small batches that the fill rule would cut into four-entry chunks
+    uint32_t root = 1;
+    const uint64_t mean = s.nent_max / std::max<uint64_t>(s.nkeys, 1);
+    while ((uint64_t)root * root < mean) ++root;
+    s.L = std::max(fill_group(s.nent_max, chunk_len), std::min(root, chunk_len));
     s.nch_max = ceil_div(s.nent_max, s.L) + s.nkeys;
     MH_CHECK(s.nch_max < (1ull << 32), (int)hipErrorInvalidValue, "MSM batch too large");
     // partials per group in the bucket tree.  Sized for the worst case, the
