@@ -271,8 +271,31 @@ def bench_ntt(args, H):
         "hbm": {"achieved": round(bytes_alg / t_step / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(bytes_alg / t_step / HBM_PEAK, 4)},
     }
+    if args.cpu_seconds > 0:
+        log("ntt: timing the CPU reference FFT")
+        res["cpu_baseline"] = cpu_baseline_ntt(dv.download()[:n], log_n, args.cpu_seconds)
     dv.free()
     return res
+
+
+def cpu_baseline_ntt(values, log_n, seconds):
+    """The reference's best_fft (radix-2 with the parallel split of
+    fft.rs:30-112, restated in oracle/ntt.c) on the workload's first
+    polynomial, repeated for about `seconds` on all host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = cpu_threads()
+    oracle.fft(values[:1024], 10, threads=threads)   # warm-up (threads)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.fft(values, log_n, threads=threads)
+        reps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round((len(values) * reps) / dt / 1e6, 3), "unit": "Melem/s", "cores": threads, "kind": "port",
+            "sample": "%d x FFT of 2^%d values (the workload's first polynomial), best_fft restated in C, "
+                      "%d threads, %.1f s" % (reps, log_n, threads, dt)}
 
 
 def bench_batch(args, H):
@@ -349,9 +372,40 @@ def bench_batch(args, H):
         "one_at_a_time_ms": round(dt1 * 1e3, 3),
         "batched_speedup": round(dt1 / dt, 3),
     }
+    if args.cpu_seconds > 0:
+        log("batch: timing the CPU reference path on a sample of the instances")
+        res["cpu_baseline"] = cpu_baseline_batch(ctx, base, o_list, k_list, args.cpu_seconds)
     ctx.free()
     dv.free()
     return res
+
+
+def cpu_baseline_batch(ctx, base, o_list, k_list, seconds):
+    """The reference's CPU path per instance (best_fft inverse, then the
+    CPU MSM: Pippenger below 2^19 points, msm_best from 2^19) on the
+    instances in the batch's order until about `seconds` have passed; the
+    batch's instances/s is extrapolated by points (sample = the instances
+    done)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = cpu_threads()
+    pts = ctx.points(1 << max(k_list))
+    done_pts, done, t0 = 0, 0, time.perf_counter()
+    for o, k in zip(o_list, k_list):
+        n = 1 << k
+        coeffs = oracle.ifft(base[o:o + n], k, threads=threads)
+        oracle.msm(pts[:n], coeffs, threads=threads, best=k >= 19)
+        done_pts += n
+        done += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    total_pts = sum(1 << k for k in k_list)
+    est = dt * total_pts / done_pts
+    return {"value": round(len(k_list) / est, 3), "unit": "instances/s", "cores": threads, "kind": "port",
+            "sample": "%d of %d instances (%d of %d points, the smallest first) in %.1f s on %d threads: iFFT "
+                      "(best_fft) + CPU MSM restated in C; the whole batch extrapolated by points to %.1f s"
+                      % (done, len(k_list), done_pts, total_pts, dt, threads, est)}
 
 
 def bench_kzg(args, H):
@@ -399,8 +453,32 @@ def bench_kzg(args, H):
         "config": {"workload": "KZG commit: upload 2^%d Lagrange values, iNTT, MSM against [tau^i]G" % log_n},
         "p50_ms": round(lat[len(lat) // 2] * 1e3, 3), "p99_ms": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))] * 1e3, 3),
     }
+    if args.cpu_seconds > 0:
+        log("kzg: timing the CPU reference commit")
+        res["cpu_baseline"] = cpu_baseline_kzg(ctx.points(n), evals, log_n, args.cpu_seconds)
     ctx.free()
     return res
+
+
+def cpu_baseline_kzg(pts, evals, log_n, seconds):
+    """The reference's CPU commitment of Lagrange values: best_fft inverse
+    to coefficients, then the CPU MSM (blst-style Pippenger below 2^19
+    points, msm_best from 2^19), restated in C, on the same values,
+    repeated for about `seconds` on all host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = cpu_threads()
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        coeffs = oracle.ifft(evals, log_n, threads=threads)
+        oracle.msm(pts, coeffs, threads=threads, best=log_n >= 19)
+        reps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(reps / dt, 3), "unit": "commits/s", "cores": threads, "kind": "port",
+            "sample": "%d commitments of the workload's 2^%d values (iFFT + MSM restated in C), %d threads, %.1f s"
+                      % (reps, log_n, threads, dt)}
 
 
 MONT_ONE = np.array([0x00000001fffffffe, 0x5884b7fa00034802, 0x998c4fefecbc4ff5, 0x1824b159acc5056f],
