@@ -963,7 +963,7 @@ def main():
     ap.add_argument("--skewed", action="store_true")
     ap.add_argument("--chain-len", type=int, default=1024, help="poseidon: hashes in the chain")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--requests", type=int, default=200, help="serving: requests in the stream")
+    ap.add_argument("--requests", type=int, default=400, help="serving: requests in the stream")
     ap.add_argument("--slots", type=int, default=12, help="serving: concurrent proofs")
     ap.add_argument("--load", type=float, default=4.0, help="serving: offered load / one-at-a-time capacity")
     ap.add_argument("--hbm-budget-gb", type=int, default=64, help="serving: HBM admission budget")
