@@ -193,16 +193,25 @@ __device__ __forceinline__ uint32_t bitrev(uint32_t x, uint32_t bits)
 // 16-byte words, limb 8 as a word), so lanes touching consecutive elements
 // touch consecutive words of each plane: a wave's ds_read_b128 covers whole
 // bank rows instead of colliding, as contiguous 36-byte elements would.
+// Element e sits at slot e ^ (((e >> 3) ^ (e >> t)) & 7), t = log m - 3 (3
+// for short columns, where the swizzle vanishes): the bit-reversed load
+// (lanes 2^(log m - 3) elements apart) and the first stage pair (lanes four
+// elements apart) then spread each group of eight lanes over the bank rows
+// instead of stacking them on one.  The map permutes slots within aligned
+// groups of eight, so it is a bijection on the pass's slots.
 struct Lds29 {
     uint4 *p0, *p1;
     uint32_t *p2;
-    __device__ __forceinline__ Lds29(uint32_t *mem, uint32_t total)
+    uint32_t t;
+    __device__ __forceinline__ Lds29(uint32_t *mem, uint32_t total, uint32_t log_m)
         : p0(reinterpret_cast<uint4 *>(mem)), p1(reinterpret_cast<uint4 *>(mem) + total),
-          p2(mem + 8 * (size_t)total)
+          p2(mem + 8 * (size_t)total), t(log_m > 6 ? log_m - 3 : 3)
     {
     }
-    __device__ __forceinline__ Fr29 get(uint32_t e) const
+    __device__ __forceinline__ uint32_t slot(uint32_t e) const { return e ^ (((e >> 3) ^ (e >> t)) & 7u); }
+    __device__ __forceinline__ Fr29 get(uint32_t e0) const
     {
+        const uint32_t e = slot(e0);
         const uint4 a = p0[e], b = p1[e];
         Fr29 v;
         v.v[0] = a.x; v.v[1] = a.y; v.v[2] = a.z; v.v[3] = a.w;
@@ -210,8 +219,9 @@ struct Lds29 {
         v.v[8] = p2[e];
         return v;
     }
-    __device__ __forceinline__ void put(uint32_t e, const Fr29 &v) const
+    __device__ __forceinline__ void put(uint32_t e0, const Fr29 &v) const
     {
+        const uint32_t e = slot(e0);
         p0[e] = make_uint4(v.v[0], v.v[1], v.v[2], v.v[3]);
         p1[e] = make_uint4(v.v[4], v.v[5], v.v[6], v.v[7]);
         p2[e] = v.v[8];
@@ -232,7 +242,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
     const uint32_t log_m = a.log_m, m = 1u << log_m;
     const uint32_t log_C = a.log_C, C = 1u << log_C;
     const uint32_t total = C << log_m;
-    const Lds29 lds(lds_mem, total);
+    const Lds29 lds(lds_mem, total, log_m);
     const uint32_t c0 = bx << log_C;
     const Fr *in = a.in + (uint64_t)by * a.in_pstride;
     Fr *out = a.out + (uint64_t)by * a.out_pstride;
