@@ -92,12 +92,32 @@ __device__ __forceinline__ int32_t window_digit(const uint32_t k[9], uint32_t w,
     return (int32_t)raw;
 }
 
-__device__ __forceinline__ void canonical_scalar(const Fr *s, uint32_t k[9])
+// Canonical scalar, replaced by r - s when above (r - 1) / 2 (returns true:
+// the point is to be negated), so that it is below 2^254.
+__device__ __forceinline__ bool canonical_scalar(const Fr *s, uint32_t k[9])
 {
-    Fr v = fr_from_mont(load_fr(s));
+    constexpr uint32_t R[8] = {0x00000001u, 0xffffffffu, 0xfffe5bfeu, 0x53bda402u,
+                               0x09a1d805u, 0x3339d808u, 0x299d7d48u, 0x73eda753u};
+    constexpr uint32_t H[8] = {0x80000000u, 0x7fffffffu, 0x7fff2dffu, 0xa9ded201u,
+                               0x04d0ec02u, 0x199cec04u, 0x94cebea4u, 0x39f6d3a9u};   // (r - 1) / 2
+    const Fr v = fr_from_mont(load_fr(s));
+    // v > H  <=>  H - v borrows
+    uint32_t borrow = 0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) k[i] = v.v[i];
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t d = (uint64_t)H[i] - v.v[i] - borrow;
+        borrow = (uint32_t)(d >> 63);
+    }
+    const bool neg = borrow != 0;
+    uint32_t b2 = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t d = (uint64_t)R[i] - v.v[i] - b2;   // r - v
+        b2 = (uint32_t)(d >> 63);
+        k[i] = neg ? (uint32_t)d : v.v[i];
+    }
     k[8] = 0;
+    return neg;
 }
 
 // Counting sort through LDS.  A block takes a tile of one MSM's scalars and
@@ -116,7 +136,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_lds(const MsmDesc *descs,
                                                            uint32_t range_keys, uint32_t *counters, uint32_t *ent,
                                                            int scatter)
 {
-    constexpr uint32_t NW = (256 + C - 1) / C;
+    constexpr uint32_t NW = msm_windows(C);
     extern __shared__ uint32_t hist[];
     const uint32_t m = blockIdx.y;
     const MsmDesc d = descs[m];
@@ -130,7 +150,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_lds(const MsmDesc *descs,
     for (int round = 0; round < (scatter ? 2 : 1); ++round) {
         for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
             uint32_t k[9];
-            canonical_scalar(d.scalars + i, k);
+            const bool neg = canonical_scalar(d.scalars + i, k);
             uint32_t carry = 0;
 #pragma unroll
             for (uint32_t w = 0; w < NW; ++w) {
@@ -141,7 +161,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_lds(const MsmDesc *descs,
                 if (lkey >= range_keys) continue;
                 uint32_t r = atomicAdd(&hist[lkey], 1u);
                 if (round == 1)
-                    ent[r] = (d.pt_off + i + w * g.pstride) | (dg < 0 ? kSignBit : 0u);
+                    ent[r] = (d.pt_off + i + w * g.pstride) | ((dg < 0) != neg ? kSignBit : 0u);
             }
         }
         __syncthreads();
@@ -227,7 +247,7 @@ __global__ void __launch_bounds__(kPlaceThreads) k_place_coarse(const MsmDesc *d
                                                                uint32_t fshift, uint32_t nbins, uint32_t *ccursor,
                                                                uint64_t *tmp)
 {
-    constexpr uint32_t NW = (256 + C - 1) / C;
+    constexpr uint32_t NW = msm_windows(C);
     __shared__ uint32_t hist[kCoarseBins];
     __shared__ uint32_t delta[kCoarseBins];
     __shared__ uint64_t stage[kStage1];
@@ -244,7 +264,7 @@ __global__ void __launch_bounds__(kPlaceThreads) k_place_coarse(const MsmDesc *d
     const uint64_t mkw = (uint64_t)m * g.kw;
     uint32_t key[NW], val[NW];
     uint32_t k[9];
-    if (live) canonical_scalar(d.scalars + i, k);
+    const bool neg = live && canonical_scalar(d.scalars + i, k);
     uint32_t carry = 0;
 #pragma unroll
     for (uint32_t w = 0; w < NW; ++w) {
@@ -253,7 +273,7 @@ __global__ void __launch_bounds__(kPlaceThreads) k_place_coarse(const MsmDesc *d
         if (dg == 0) continue;
         const uint32_t b = (uint32_t)(dg < 0 ? -dg : dg) - 1;
         key[w] = (uint32_t)key_of(g, mkw, w, b);
-        val[w] = (d.pt_off + i + w * g.pstride) | (dg < 0 ? kSignBit : 0u);
+        val[w] = (d.pt_off + i + w * g.pstride) | ((dg < 0) != neg ? kSignBit : 0u);
         atomicAdd(&hist[key[w] >> fshift], 1u);
     }
     __syncthreads();
@@ -386,7 +406,7 @@ void launch_digits(bool scatter, uint32_t c, uint64_t max_n, hipStream_t st, con
 void launch_place_coarse(uint32_t c, uint64_t max_n, hipStream_t st, const MsmDesc *descs, const Geometry &g,
                          uint32_t fshift, uint32_t nbins, uint32_t *ccursor, uint64_t *tmp)
 {
-    const uint32_t nw = (256 + c - 1) / c;
+    const uint32_t nw = msm_windows(c);
     const uint32_t tile = std::min<uint32_t>(kPlaceThreads, kStage1 / nw);
     dim3 grid((uint32_t)ceil_div(max_n, tile), g.M);
     switch (c) {
@@ -777,7 +797,7 @@ uint32_t rule_table_window(size_t n)
 bool build_table(const MsmContext &ctx, uint32_t c, DevBuf<uint32_t> &table, hipStream_t st, size_t rows = 0)
 {
     const size_t n = rows ? rows : ctx.npoints;
-    const uint32_t nwin = (256 + c - 1) / c;
+    const uint32_t nwin = msm_windows(c);
     if ((uint64_t)nwin * n >= (1ull << 31)) return false;   // entry indices are 31-bit
     table.alloc((size_t)nwin * n * kRow28);   // Affine28 rows
     hipLaunchKernelGGL(k_precompute, dim3(grid1(n)), dim3(kBlock), 0, st, ctx.points.get(), table.get(),
@@ -821,7 +841,7 @@ void build_small_table(MsmContext &ctx, hipStream_t st)
     const size_t rows = std::min<size_t>(ctx.npoints, size_t(1) << (ctx.pc_c - 1));
     if (!build_table(ctx, kSmallTableC, ctx.small_table, st, rows)) return;
     ctx.sm_c = kSmallTableC;
-    ctx.sm_nwin = (256 + kSmallTableC - 1) / kSmallTableC;
+    ctx.sm_nwin = msm_windows(kSmallTableC);
     ctx.sm_n = rows;
 }
 
@@ -843,7 +863,7 @@ void build_precomp_main(MsmContext &ctx, hipStream_t st)
     if (wb || n < (size_t(1) << 16)) {
         if (build_table(ctx, c0, ctx.table, st)) {
             ctx.pc_c = c0;
-            ctx.pc_nwin = (256 + c0 - 1) / c0;
+            ctx.pc_nwin = msm_windows(c0);
         }
         return;
     }
@@ -859,7 +879,7 @@ void build_precomp_main(MsmContext &ctx, hipStream_t st)
     for (uint32_t c = std::max<uint32_t>(10, c0 - 1); c <= std::min<uint32_t>(16, c0 + 1); ++c) {
         DevBuf<uint32_t> table;
         if (!build_table(ctx, c, table, st)) continue;
-        const uint32_t nwin = (256 + c - 1) / c;
+        const uint32_t nwin = msm_windows(c);
         const PrecompTable pc{table.get(), n, c, nwin};
         bool keep = false;
         for (uint32_t L : {64u, 32u}) {
@@ -1045,7 +1065,7 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
         d_points = pc->table;
     } else {
         g.c = window_bits ? window_bits : auto_window(max_n);
-        g.nwin = (256 + g.c - 1) / g.c;
+        g.nwin = msm_windows(g.c);
         g.kw = g.nwin;
         g.pstride = 0;
     }

@@ -21,8 +21,14 @@ struct MsmJob {
     size_t n;
 };
 
+// Signed c-bit windows of a scalar: every scalar above (r - 1) / 2 is
+// replaced by r - s with its point negated, so scalars are below 2^254 and
+// ceil(255 / c) windows hold them with the last carry (one window fewer than
+// ceil(256 / c) at c = 15 and 17).
+__host__ __device__ constexpr uint32_t msm_windows(uint32_t c) { return (255 + c - 1) / c; }
+
 // Per-window copies of a point set: row w * n + i holds [2^(c w)] P_i
-// (RawG1Affine), w < nwin = ceil(256 / c).  With it every window of an MSM
+// (RawG1Affine), w < nwin = msm_windows(c).  With it every window of an MSM
 // feeds one shared bucket set, and the window fold disappears.
 struct PrecompTable {
     const uint32_t *table;
