@@ -692,6 +692,24 @@ struct PlaneLayout {
     uint32_t per_window;                 // chunks per window
     uint32_t first[kMaxWindow + 1];      // first chunk of plane k within a window
     uint32_t members[kMaxWindow];        // members of plane k
+    // planes over line sums (two-level weights, below): h low bits, and the
+    // number of line sums per MSM; h = 0 for planes over the buckets
+    uint32_t h, nlines;
+};
+
+// Two-level weights for a precomputed window's one bucket set of B = 2^(c-1)
+// buckets: weight x = hi 2^h + lo, and
+//   S = sum_x x B_x = sum_lo lo R_lo + 2^h sum_hi hi C_hi,
+//   R_lo = sum of the buckets with x mod 2^h = lo   (lo = 1 .. 2^h - 1),
+//   C_hi = sum of the buckets with x >> h = hi       (hi = 1 .. H, H = B >> h),
+// so each bucket is added twice (into its R and its C line) instead of
+// into ~c/2 bit planes; the planes then run over the 2^h - 1 + H line sums
+// (plane k < h over the R lines with bit k of lo, plane h + k over the C
+// lines with bit k of hi), and the host's sum_k 2^k Q_k is unchanged.
+struct LineLayout {
+    uint32_t h, H, nlines;
+    uint32_t chunks_r, chunks_c;         // chunks of G1 members per R line, per full C line
+    uint32_t per_msm;                    // chunks per MSM
 };
 
 template <int kLanes>
@@ -709,11 +727,65 @@ __global__ void __launch_bounds__(kBlock) k_plane_partials(const uint32_t *bucke
     uint32_t chunk = rem - pl.first[k];
     uint32_t e0 = chunk * G1, e1 = min(e0 + G1, pl.members[k]);
     const uint64_t stride = (uint64_t)g.M * g.kw;   // keys are bucket-major
-    const uint32_t lowmask = (1u << k) - 1u;
+    Xyzz28 acc = xyzz28_inf();
+    if (pl.h) {
+        // over the line sums of MSM mw: R lines 0 .. 2^h - 2, then C lines
+        const uint32_t kk = k < pl.h ? k : k - pl.h;
+        const uint32_t lowmask = (1u << kk) - 1u;
+        const uint64_t base = mw * pl.nlines + (k < pl.h ? 0u : (1u << pl.h) - 1u);
+        for (uint32_t e = e0; e < e1; ++e) {
+            const uint32_t x = ((e >> kk) << (kk + 1)) | (1u << kk) | (e & lowmask);
+            acc = msm_add<kLanes>(acc, load_xyzz28(bucket + (base + x - 1) * kXyzz28Words), sub);
+        }
+    } else {
+        const uint32_t lowmask = (1u << k) - 1u;
+        for (uint32_t e = e0; e < e1; ++e) {
+            uint32_t x = ((e >> k) << (k + 1)) | (1u << k) | (e & lowmask);
+            const uint64_t key = (uint64_t)(x - 1) * stride + mw;
+            // empty buckets are not written
+            if (cnt[key]) acc = msm_add<kLanes>(acc, load_xyzz28(bucket + key * kXyzz28Words), sub);
+        }
+    }
+    if (sub == 0) store_xyzz28(pp + id * kXyzz28Words, acc);
+}
+
+// Partial sums of the line sums (LineLayout) of a precomputed window's
+// buckets: chunk `id` adds at most G1 buckets of one R or C line.
+template <int kLanes>
+__global__ void __launch_bounds__(kBlock) k_line_partials(const uint32_t *bucket, const uint32_t *cnt, Geometry g,
+                                                          LineLayout ll, uint32_t G1, uint64_t total, uint32_t *pp)
+{
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t id = t / kLanes;
+    const uint32_t sub = (uint32_t)(t % kLanes);
+    if (id >= total) return;
+    const uint64_t m = id / ll.per_msm;
+    const uint32_t rem = (uint32_t)(id % ll.per_msm);
+    const uint32_t nr = ((1u << ll.h) - 1u) * ll.chunks_r;
+    uint32_t x0, step, e0, e1;   // weights x0 + step * e, e in [e0, e1)
+    if (rem < nr) {
+        const uint32_t lo = rem / ll.chunks_r + 1, chunk = rem % ll.chunks_r;
+        x0 = lo;
+        step = 1u << ll.h;
+        e0 = chunk * G1;
+        e1 = min(e0 + G1, ll.H);
+    } else if (rem - nr < (ll.H - 1) * ll.chunks_c) {
+        const uint32_t r2 = rem - nr;
+        const uint32_t hi = r2 / ll.chunks_c + 1, chunk = r2 % ll.chunks_c;
+        x0 = hi << ll.h;
+        step = 1;
+        e0 = chunk * G1;
+        e1 = min(e0 + G1, 1u << ll.h);
+    } else {   // C_H: the one bucket of weight B
+        x0 = ll.H << ll.h;
+        step = 1;
+        e0 = 0;
+        e1 = 1;
+    }
+    const uint64_t stride = (uint64_t)g.M * g.kw;   // keys are bucket-major
     Xyzz28 acc = xyzz28_inf();
     for (uint32_t e = e0; e < e1; ++e) {
-        uint32_t x = ((e >> k) << (k + 1)) | (1u << k) | (e & lowmask);
-        const uint64_t key = (uint64_t)(x - 1) * stride + mw;
+        const uint64_t key = (uint64_t)(x0 + step * e - 1) * stride + m;
         // empty buckets are not written
         if (cnt[key]) acc = msm_add<kLanes>(acc, load_xyzz28(bucket + key * kXyzz28Words), sub);
     }
@@ -927,7 +999,7 @@ struct MsmEngine::Buffers {
     DevBuf<uint64_t> tmp;                          // (key, entry) pairs between the placement passes
     DevBuf<uint32_t> ccursor, nch2, cst2, cmap2, t2;   // coarse bins: cursors, pass-2 chunks, map, total
     DevBuf<uint32_t> lc, lo, lmap;   // tree levels: groups per bucket, their offsets, lane -> bucket
-    DevBuf<uint32_t> part, part2, bucket, pp, pp2, pj;
+    DevBuf<uint32_t> part, part2, bucket, pp, pp2, pj, lines;
     DevBuf<uint32_t> seg_off, seg_len, out_off;
     DevBuf<MsmDesc> descs;
     // pinned host copies of the (window, plane) sums and the totals
@@ -958,6 +1030,13 @@ struct MsmEngine::State {
     uint32_t L = 0, G = 0, tree_levels = 0, G1 = 0;
     bool precomputed = false, ready = false;
     std::vector<Level> levels;
+    // two-level weights (precomputed windows of 13 bits or more): line sums
+    // first, then the planes over them
+    bool lines = false;
+    LineLayout ll{};
+    uint32_t G1l = 0;
+    uint64_t nppl = 0, nsegl = 0;
+    std::vector<Level> llevels;
     std::vector<MsmDesc> hd;
     std::vector<uint32_t> seg_off, seg_len, out_off;
     // the layout last uploaded and the device buffers it went to
@@ -1135,15 +1214,92 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
     }
     b.bucket.ensure(s.nkeys * kXyzz28Words);
 
-    // bit planes: S = sum_k 2^k Q_k, Q_k = sum of the buckets x with bit k of x
+    // bit planes: S = sum_k 2^k Q_k, Q_k = sum of the buckets x with bit k of
+    // x -- or, for a precomputed window of 13 bits or more, of the line sums
+    // of the two-level weights (LineLayout)
     const uint64_t nmw = (uint64_t)g.M * g.kw;
-    s.G1 = fill_group(nmw * ((uint64_t)(g.c - 1) * (g.B / 2) + 1), plane_group);
+    s.lines = g.kw == 1 && g.c >= 13;
+    s.seg_off.clear();
+    s.seg_len.clear();
+    s.out_off.clear();
+    // group passes within each segment until one partial per segment is
+    // left; returns the level list (descriptors appended to seg_*)
+    auto group_levels = [&](std::vector<uint32_t> offs, std::vector<uint32_t> lens, uint64_t in_total) {
+        std::vector<State::Level> out;
+        const uint64_t nseg = offs.size();
+        for (;;) {
+            const uint32_t maxlen = *std::max_element(lens.begin(), lens.end());
+            const bool last = maxlen <= plane_group2;
+            const uint32_t G =
+                last ? maxlen : (in_total >= (uint64_t)plane_group2 * kFillLanes / 8 ? plane_group2 : 4);
+            State::Level lv{s.seg_off.size(), 0, G};
+            for (uint64_t x = 0; x < nseg; ++x) {
+                s.seg_off.push_back(offs[x]);
+                s.seg_len.push_back(lens[x]);
+                s.out_off.push_back((uint32_t)lv.total_out);
+                offs[x] = (uint32_t)lv.total_out;
+                lens[x] = (lens[x] + G - 1) / G;
+                lv.total_out += lens[x];
+            }
+            out.push_back(lv);
+            in_total = lv.total_out;
+            if (last) break;
+        }
+        return out;
+    };
+    uint64_t pp_need = 0, pp2_need = 1;
     PlaneLayout &pl = s.pl;
     pl = PlaneLayout{};
     pl.c = g.c;
+    if (s.lines) {
+        LineLayout &ll = s.ll;
+        ll.h = g.c / 2;
+        ll.H = g.B >> ll.h;
+        ll.nlines = ((1u << ll.h) - 1u) + ll.H;
+        s.G1l = fill_group((uint64_t)g.M * 2 * g.B, plane_group);
+        ll.chunks_r = (ll.H + s.G1l - 1) / s.G1l;
+        ll.chunks_c = ((1u << ll.h) + s.G1l - 1) / s.G1l;
+        ll.per_msm = ((1u << ll.h) - 1u) * ll.chunks_r + (ll.H - 1) * ll.chunks_c + 1;
+        s.nppl = (uint64_t)g.M * ll.per_msm;
+        s.nsegl = (uint64_t)g.M * ll.nlines;
+        std::vector<uint32_t> offs(s.nsegl), lens(s.nsegl);
+        for (uint64_t x = 0; x < s.nsegl; ++x) {
+            const uint32_t m = (uint32_t)(x / ll.nlines), line = (uint32_t)(x % ll.nlines);
+            const uint32_t nr = (1u << ll.h) - 1u;
+            uint32_t first, len;
+            if (line < nr) {
+                first = line * ll.chunks_r;
+                len = ll.chunks_r;
+            } else if (line - nr < ll.H - 1) {
+                first = nr * ll.chunks_r + (line - nr) * ll.chunks_c;
+                len = ll.chunks_c;
+            } else {
+                first = nr * ll.chunks_r + (ll.H - 1) * ll.chunks_c;
+                len = 1;
+            }
+            offs[x] = m * ll.per_msm + first;
+            lens[x] = len;
+        }
+        s.llevels = group_levels(offs, lens, s.nppl);
+        pp_need = std::max(pp_need, s.nppl);
+        for (size_t i = 0; i < s.llevels.size(); ++i)
+            (i & 1 ? pp_need : pp2_need) = std::max(i & 1 ? pp_need : pp2_need, s.llevels[i].total_out);
+        b.lines.ensure(s.nsegl * kXyzz28Words);
+        // planes over the lines: k < h over R lines (2^(h-1) members), then
+        // over C lines (H / 2 members, one for the top plane)
+        pl.h = ll.h;
+        pl.nlines = ll.nlines;
+        for (uint32_t k = 0; k < g.c; ++k)
+            pl.members[k] = k < ll.h ? (1u << (ll.h - 1)) : (k + 1 < g.c ? ll.H / 2 : 1);
+        s.G1 = fill_group(nmw * ((uint64_t)g.c * (1u << (ll.h - 1))), plane_group);
+    } else {
+        s.llevels.clear();
+        s.nppl = s.nsegl = 0;
+        for (uint32_t k = 0; k < g.c; ++k) pl.members[k] = k + 1 < g.c ? g.B / 2 : 1;
+        s.G1 = fill_group(nmw * ((uint64_t)(g.c - 1) * (g.B / 2) + 1), plane_group);
+    }
     uint32_t acc = 0;
     for (uint32_t k = 0; k < g.c; ++k) {
-        pl.members[k] = k + 1 < g.c ? g.B / 2 : 1;
         pl.first[k] = acc;
         acc += (pl.members[k] + s.G1 - 1) / s.G1;
     }
@@ -1151,43 +1307,26 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
     for (uint32_t k = g.c + 1; k <= kMaxWindow; ++k) pl.first[k] = 0xffffffffu;
     pl.per_window = acc;
     s.npp = nmw * pl.per_window;
-    b.pp.ensure(s.npp * kXyzz28Words);
 
-    // group passes within each segment (msm, window, plane) until one
-    // partial per segment is left; the last pass converts to Jacobian
+    // segments (msm, window, plane); the last pass converts to Jacobian
     s.nseg = nmw * g.c;
-    s.levels.clear();
-    s.seg_off.clear();
-    s.seg_len.clear();
-    s.out_off.clear();
-    std::vector<uint32_t> offs(s.nseg), lens(s.nseg);
-    for (uint64_t x = 0; x < s.nseg; ++x) {
-        const uint32_t k = (uint32_t)(x % g.c);
-        offs[x] = (uint32_t)((x / g.c) * pl.per_window + pl.first[k]);
-        lens[x] = pl.first[k + 1] - pl.first[k];
-    }
-    uint64_t in_total = s.npp;
-    for (;;) {
-        const uint32_t maxlen = *std::max_element(lens.begin(), lens.end());
-        const bool last = maxlen <= plane_group2;
-        const uint32_t G = last ? maxlen : (in_total >= (uint64_t)plane_group2 * kFillLanes / 8 ? plane_group2 : 4);
-        State::Level lv{s.seg_off.size(), 0, G};
+    {
+        std::vector<uint32_t> offs(s.nseg), lens(s.nseg);
         for (uint64_t x = 0; x < s.nseg; ++x) {
-            s.seg_off.push_back(offs[x]);
-            s.seg_len.push_back(lens[x]);
-            s.out_off.push_back((uint32_t)lv.total_out);
-            offs[x] = (uint32_t)lv.total_out;
-            lens[x] = (lens[x] + G - 1) / G;
-            lv.total_out += lens[x];
+            const uint32_t k = (uint32_t)(x % g.c);
+            offs[x] = (uint32_t)((x / g.c) * pl.per_window + pl.first[k]);
+            lens[x] = pl.first[k + 1] - pl.first[k];
         }
-        s.levels.push_back(lv);
-        in_total = lv.total_out;
-        if (last) break;
+        s.levels = group_levels(offs, lens, s.npp);
     }
+    pp_need = std::max(pp_need, s.npp);
+    for (size_t i = 0; i < s.levels.size(); ++i)
+        (i & 1 ? pp_need : pp2_need) = std::max(i & 1 ? pp_need : pp2_need, s.levels[i].total_out);
+    b.pp.ensure(pp_need * kXyzz28Words);
+    b.pp2.ensure(pp2_need * kXyzz28Words);
     b.seg_off.ensure(s.seg_off.size());
     b.seg_len.ensure(s.seg_len.size());
     b.out_off.ensure(s.out_off.size());
-    b.pp2.ensure(std::max<uint64_t>(s.levels[0].total_out, 1) * kXyzz28Words);
     b.pj.ensure(s.nseg * 36);
 
     // uploads of the descriptors and the plane-pass layout
@@ -1329,29 +1468,48 @@ void MsmEngine::enqueue_back(hipStream_t st, uint32_t levels, ProfileScope *prof
                            b.bucket.get());
     MH_HIP(hipGetLastError());
 
-    // 7: bit-plane partial sums and group passes down to one per segment
+    // group passes of a segment set down to one value per segment (the
+    // last pass writes `final`, as Jacobian when `to_jac`)
+    auto group_passes = [&](const std::vector<State::Level> &lvls, uint64_t nseg, uint32_t *final, int to_jac) {
+        uint32_t *lin = b.pp.get(), *lout = b.pp2.get();
+        for (size_t i = 0; i < lvls.size(); ++i) {
+            const State::Level &lv = lvls[i];
+            const bool last = i + 1 == lvls.size();
+            if (quad(lv.total_out))
+                hipLaunchKernelGGL(k_group_sums<4>, dim3(grid1(4 * lv.total_out)), dim3(kBlock), 0, st, lin, nseg,
+                                   b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base,
+                                   lv.G, lv.total_out, last ? final : lout, last ? to_jac : 0);
+            else
+                hipLaunchKernelGGL(k_group_sums<1>, dim3(grid1(lv.total_out)), dim3(kBlock), 0, st, lin, nseg,
+                                   b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base,
+                                   lv.G, lv.total_out, last ? final : lout, last ? to_jac : 0);
+            MH_HIP(hipGetLastError());
+            std::swap(lin, lout);
+        }
+    };
+
+    // 7: two-level weights: line sums of the buckets, then planes over them
+    const uint32_t *plane_src = b.bucket.get();
+    if (s.lines) {
+        if (quad(s.nppl))
+            hipLaunchKernelGGL(k_line_partials<4>, dim3(grid1(4 * s.nppl)), dim3(kBlock), 0, st, b.bucket.get(),
+                               b.cnt.get(), g, s.ll, s.G1l, s.nppl, b.pp.get());
+        else
+            hipLaunchKernelGGL(k_line_partials<1>, dim3(grid1(s.nppl)), dim3(kBlock), 0, st, b.bucket.get(),
+                               b.cnt.get(), g, s.ll, s.G1l, s.nppl, b.pp.get());
+        MH_HIP(hipGetLastError());
+        group_passes(s.llevels, s.nsegl, b.lines.get(), 0);
+        plane_src = b.lines.get();
+    }
+    // bit-plane partial sums and group passes down to one per segment
     if (quad(s.npp))
-        hipLaunchKernelGGL(k_plane_partials<4>, dim3(grid1(4 * s.npp)), dim3(kBlock), 0, st, b.bucket.get(),
+        hipLaunchKernelGGL(k_plane_partials<4>, dim3(grid1(4 * s.npp)), dim3(kBlock), 0, st, plane_src,
                            b.cnt.get(), g, s.pl, s.G1, s.npp, b.pp.get());
     else
-        hipLaunchKernelGGL(k_plane_partials<1>, dim3(grid1(s.npp)), dim3(kBlock), 0, st, b.bucket.get(), b.cnt.get(),
+        hipLaunchKernelGGL(k_plane_partials<1>, dim3(grid1(s.npp)), dim3(kBlock), 0, st, plane_src, b.cnt.get(),
                            g, s.pl, s.G1, s.npp, b.pp.get());
     MH_HIP(hipGetLastError());
-    uint32_t *lin = b.pp.get(), *lout = b.pp2.get();
-    for (size_t i = 0; i < s.levels.size(); ++i) {
-        const State::Level &lv = s.levels[i];
-        const bool last = i + 1 == s.levels.size();
-        if (quad(lv.total_out))
-            hipLaunchKernelGGL(k_group_sums<4>, dim3(grid1(4 * lv.total_out)), dim3(kBlock), 0, st, lin, s.nseg,
-                               b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base,
-                               lv.G, lv.total_out, last ? b.pj.get() : lout, last ? 1 : 0);
-        else
-            hipLaunchKernelGGL(k_group_sums<1>, dim3(grid1(lv.total_out)), dim3(kBlock), 0, st, lin, s.nseg,
-                               b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base,
-                               lv.G, lv.total_out, last ? b.pj.get() : lout, last ? 1 : 0);
-        MH_HIP(hipGetLastError());
-        std::swap(lin, lout);
-    }
+    group_passes(s.levels, s.nseg, b.pj.get(), 1);
     MH_HIP(hipMemcpyAsync(b.h_pj, b.pj.get(), s.nseg * 144, hipMemcpyDeviceToHost, st));
     MH_HIP(hipMemcpyAsync(b.h_totals, b.totals.get(), 16, hipMemcpyDeviceToHost, st));
     if (prof) prof->end(st);
