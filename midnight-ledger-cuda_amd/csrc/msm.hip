@@ -855,7 +855,7 @@ namespace {
 
 // the table window of the fixed rule: c = log2 n - 4, at most 16: wider
 // windows cut the entries (n * ceil(256/c)) but multiply the digit passes of
-// the LDS sort (2^(c-1) / 16384 key ranges) and the bucket reduction; at
+// the LDS sort (2^(c-1) / 32768 key ranges) and the bucket reduction; at
 // 2^24 points c = 20 spent as long sorting and reducing as accumulating.
 uint32_t rule_table_window(size_t n)
 {
@@ -906,6 +906,7 @@ namespace {
 // points has 22 x 2^12 entries in 2048 buckets, against 32 x 2^12 entries
 // in 32 x 128 buckets and a 256-bit host chain on plain 8-bit windows.
 constexpr uint32_t kSmallTableC = 12;
+constexpr uint32_t kMaxTableC = 17;   // widest autotuned or configured table window
 
 void build_small_table(MsmContext &ctx, hipStream_t st)
 {
@@ -918,20 +919,22 @@ void build_small_table(MsmContext &ctx, hipStream_t st)
 }
 
 // The main per-window table.  A config with window_bits set gets that
-// window (10..16; the reference's 17 and 18 are capped at 16, the widest
-// key range of the LDS digit sort).  Otherwise contexts of 2^16 points or
-// more are autotuned here, once: a batch of four MSMs of uniform scalars
-// over the whole point set is timed with the rule's window and one bit
-// either side (within 10..16), each with accumulation chunks of 64 and 32
-// entries, and the fastest table and chunk length are kept (the
-// reference's counterpart is its hand-measured per-k table,
-// lib.rs:159-203).  MSM results do not depend on the choice.
+// window (10..17; the reference's 18 is capped at 17: two key ranges of the
+// LDS digit sort).  Otherwise contexts of 2^16 points or more are autotuned
+// here, once: a batch of four MSMs of uniform scalars over the whole point
+// set is timed with the rule's window and one bit either side (within
+// 10..17), each with accumulation chunks of 64 and 32 entries, and the
+// fastest table and chunk length are kept (the reference's counterpart is
+// its hand-measured per-k table, lib.rs:159-203).  At 17 bits the scalars'
+// negation above (r - 1) / 2 saves a window (15 instead of 16) and the
+// two-level bucket weights keep its 2^16 buckets' reduction short.  MSM
+// results do not depend on the choice.
 void build_precomp_main(MsmContext &ctx, hipStream_t st)
 {
     const size_t n = ctx.npoints;
     if (n < (size_t(1) << 10)) return;
     const uint32_t wb = ctx.window_bits;
-    const uint32_t c0 = wb ? std::min<uint32_t>(16, std::max<uint32_t>(10, wb)) : rule_table_window(n);
+    const uint32_t c0 = wb ? std::min<uint32_t>(kMaxTableC, std::max<uint32_t>(10, wb)) : rule_table_window(n);
     if (wb || n < (size_t(1) << 16)) {
         if (build_table(ctx, c0, ctx.table, st)) {
             ctx.pc_c = c0;
@@ -948,7 +951,7 @@ void build_precomp_main(MsmContext &ctx, hipStream_t st)
     const std::vector<MsmJob> jobs(4, MsmJob{sc.get(), 0, n});
     MsmEngine eng;
     float best = 1e30f;
-    for (uint32_t c = std::max<uint32_t>(10, c0 - 1); c <= std::min<uint32_t>(16, c0 + 1); ++c) {
+    for (uint32_t c = std::max<uint32_t>(10, c0 - 1); c <= std::min<uint32_t>(kMaxTableC, c0 + 1); ++c) {
         DevBuf<uint32_t> table;
         if (!build_table(ctx, c, table, st)) continue;
         const uint32_t nwin = msm_windows(c);

@@ -420,7 +420,7 @@ def test_scrambled_msm_waits_for_delayed_mask_producer(H, O):
 def test_msm_context_window_bits_and_autotune(H, O):
     """A context made without a config is autotuned at creation (window of
     the size rule or one bit either side, chunks of 64 or 32); a caller's
-    window_bits is used as given (tables capped at 16, plain windows as
+    window_bits is used as given (tables capped at 17, plain windows as
     given, .cu:50-110 validity); every choice gives the closed-form MSM."""
     n = 1 << 16
     ts = O.random_fr(n, 71)
@@ -433,12 +433,12 @@ def test_msm_context_window_bits_and_autotune(H, O):
     got = auto.batch(sc, n, 2)
     assert all(O.g1_eq(a, b) for a, b in zip(got, want))
     auto.free()
-    for wb in (10, 13, 17):
+    for wb in (10, 13, 17, 18):
         cfg = H.default_config(n)
         cfg.window_bits = wb
         ctx = H.MsmContext(pts, config=cfg)
         w, L, ms = ctx.tuning()
-        assert w == min(wb, 16) and ms == 0
+        assert w == min(wb, 17) and ms == 0
         got = ctx.batch(sc, n, 2)
         assert all(O.g1_eq(a, b) for a, b in zip(got, want)), wb
         # a short MSM runs plain windows of the configured width
