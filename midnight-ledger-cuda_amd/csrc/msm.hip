@@ -509,8 +509,12 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(const uint32_t *pts, cons
     if (j >= totals[1]) return;
     const uint32_t b = cmap[j];
     const uint32_t r = j - co[b];
-    const uint32_t start = off[b] + r * L;
-    const uint32_t end = min(start + L, off[b] + cnt[b]);
+    // the bucket's cnt entries cut into its q = ceil(cnt / L) chunks at
+    // equal lengths (+-1, at most L): no short last chunk leaves its lane
+    // idle while the others finish
+    const uint32_t qb = q[b], cb = cnt[b];
+    const uint32_t start = off[b] + (uint32_t)(((uint64_t)r * cb) / qb);
+    const uint32_t end = off[b] + (uint32_t)(((uint64_t)(r + 1) * cb) / qb);
     // each point is converted to radix 2^28 (fully reduced: the rare doubling
     // path takes it as canonical) and added in loose coordinates, as in the
     // precomputed pass
@@ -551,8 +555,12 @@ __global__ void __launch_bounds__(kBlock) k_accumulate28(const uint32_t *pts, co
     if (j >= totals[1]) return;
     const uint32_t b = cmap[j];
     const uint32_t r = j - co[b];
-    const uint32_t start = off[b] + r * L;
-    const uint32_t end = min(start + L, off[b] + cnt[b]);
+    // the bucket's cnt entries cut into its q = ceil(cnt / L) chunks at
+    // equal lengths (+-1, at most L): no short last chunk leaves its lane
+    // idle while the others finish
+    const uint32_t qb = q[b], cb = cnt[b];
+    const uint32_t start = off[b] + (uint32_t)(((uint64_t)r * cb) / qb);
+    const uint32_t end = off[b] + (uint32_t)(((uint64_t)(r + 1) * cb) / qb);
     Xyzz28 acc = xyzz28_inf();
     // the next row is in flight while the current one is added
     Affine28 nxt = fetch_point28(pts, ent[start]);
