@@ -869,7 +869,9 @@ uint32_t rule_table_window(size_t n)
 {
     int lg = 0;
     while ((size_t(1) << lg) < n) ++lg;
-    return (uint32_t)std::min(16, std::max(10, lg - 4));
+    uint32_t c = (uint32_t)std::min(16, std::max(10, lg - 4));
+    while (!full_top_window(c)) ++c;
+    return c;
 }
 
 // the per-window table of window c into `table`; false when its entries
@@ -910,10 +912,12 @@ namespace {
 
 // The small table: window kSmallTableC over the points below 2^(pc_c - 1),
 // built for autotuned contexts whose main table is wider (a 2^20-point SRS
-// serving proofs of k = 10..14 is the case).  At c = 12 an MSM of 2^12
-// points has 22 x 2^12 entries in 2048 buckets, against 32 x 2^12 entries
-// in 32 x 128 buckets and a 256-bit host chain on plain 8-bit windows.
-constexpr uint32_t kSmallTableC = 12;
+// serving proofs of k = 10..15 is the case).  At c = 13 an MSM of 2^12
+// points has 20 x 2^12 entries in 4096 buckets, against 32 x 2^12 entries
+// in 32 x 128 buckets and a 256-bit host chain on plain 8-bit windows.  (At
+// c = 12 the top window holds two bits of the scalar: its 2^12 digits fall
+// into four buckets, whose chunk trees then set the MSM's latency.)
+constexpr uint32_t kSmallTableC = 13;
 constexpr uint32_t kMaxTableC = 17;   // widest autotuned or configured table window
 
 void build_small_table(MsmContext &ctx, hipStream_t st)
@@ -959,7 +963,15 @@ void build_precomp_main(MsmContext &ctx, hipStream_t st)
     const std::vector<MsmJob> jobs(4, MsmJob{sc.get(), 0, n});
     MsmEngine eng;
     float best = 1e30f;
-    for (uint32_t c = std::max<uint32_t>(10, c0 - 1); c <= std::min<uint32_t>(kMaxTableC, c0 + 1); ++c) {
+    // widths within two bits of the rule's whose top window is not
+    // degenerate, nearest first, at most three
+    std::vector<uint32_t> cands;
+    for (uint32_t d = 0; d <= 2 && cands.size() < 3; ++d)
+        for (uint32_t c : {c0 - d, c0 + d})
+            if (c >= 10 && c <= kMaxTableC && full_top_window(c) && cands.size() < 3 &&
+                std::find(cands.begin(), cands.end(), c) == cands.end())
+                cands.push_back(c);
+    for (uint32_t c : cands) {
         DevBuf<uint32_t> table;
         if (!build_table(ctx, c, table, st)) continue;
         const uint32_t nwin = msm_windows(c);

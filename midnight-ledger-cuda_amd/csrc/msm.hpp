@@ -27,6 +27,13 @@ struct MsmJob {
 // ceil(256 / c) at c = 15 and 17).
 __host__ __device__ constexpr uint32_t msm_windows(uint32_t c) { return (255 + c - 1) / c; }
 
+// Whether the top window of width c holds at least four bits of a scalar
+// (scalars are below 2^254).  With a precomputed table every window feeds
+// one bucket set, so a top window of one or two bits (c = 9, 11, 12, 14)
+// sends all of its digits to the lowest few buckets, whose long chunk
+// trees then set the latency of the whole MSM.
+__host__ __device__ constexpr bool full_top_window(uint32_t c) { return 254 >= c * (msm_windows(c) - 1) + 4; }
+
 // Per-window copies of a point set: row w * n + i holds [2^(c w)] P_i
 // (RawG1Affine), w < nwin = msm_windows(c).  With it every window of an MSM
 // feeds one shared bucket set, and the window fold disappears.

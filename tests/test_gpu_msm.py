@@ -320,8 +320,8 @@ def test_msm_short_on_large_precomputed_context(H, O):
 
 
 def test_msm_small_table_on_wide_context(H, O):
-    """A context whose main table is wider than 12 bits (2^18 points) also
-    holds a 12-bit table over its first 2^(c-1) points: batches of short
+    """A context whose main table is wider than 13 bits (2^18 points) also
+    holds a 13-bit table over its first 2^(c-1) points: batches of short
     MSMs within it (ragged, uniform and repeated scalars) take that table,
     each against the closed form, and either side of the switch to the
     main table."""
@@ -330,7 +330,7 @@ def test_msm_small_table_on_wide_context(H, O):
     dv_t = H.DeviceVec.upload(ts)
     ctx = H.MsmContext.from_multiples(dv_t)
     w, _, _ = ctx.tuning()
-    assert w > 12
+    assert w > 13
     m = (1 << 12) - 1                                   # below every main window's switch
     sizes = [1, 100, 1000, m, m]
     sc = O.random_fr(sum(sizes), 1211)
@@ -338,11 +338,11 @@ def test_msm_small_table_on_wide_context(H, O):
     dv = H.DeviceVec.upload(sc)
     offs = np.cumsum([0] + sizes[:-1]).tolist()
     out = ctx.device_multi(dv, offs, sizes)
-    assert ctx.last_stats()[1] == 12
+    assert ctx.last_stats()[1] == 13
     for i, (o, m) in enumerate(zip(offs, sizes)):
         assert O.g1_eq(out[i], O.g1_mul(O.generator(), O.inner_product(sc[o:o + m], ts[:m]))), m
     got = ctx.device_batch(dv, sum(sizes[:3]), m, 2)
-    assert ctx.last_stats()[1] == 12
+    assert ctx.last_stats()[1] == 13
     for i in range(2):
         o = sum(sizes[:3]) + i * m
         assert O.g1_eq(got[i], O.g1_mul(O.generator(), O.inner_product(sc[o:o + m], ts[:m]))), i
@@ -351,7 +351,7 @@ def test_msm_small_table_on_wide_context(H, O):
     big = 1 << (w - 1)
     sc2 = O.random_fr(big, 1212)
     dv2 = H.DeviceVec.upload(sc2)
-    for m, window in ((big - 1, 12), (big, w)):
+    for m, window in ((big - 1, 13), (big, w)):
         got2 = ctx.device_batch(dv2, 0, m, 1)[0]
         assert ctx.last_stats()[1] == window, m
         assert O.g1_eq(got2, O.g1_mul(O.generator(), O.inner_product(sc2[:m], ts[:m]))), m
@@ -418,8 +418,9 @@ def test_scrambled_msm_waits_for_delayed_mask_producer(H, O):
 
 
 def test_msm_context_window_bits_and_autotune(H, O):
-    """A context made without a config is autotuned at creation (window of
-    the size rule or one bit either side, chunks of 64 or 32); a caller's
+    """A context made without a config is autotuned at creation (windows
+    within two bits of the size rule's whose top window holds at least four
+    bits, chunks of 64 or 32); a caller's
     window_bits is used as given (tables capped at 17, plain windows as
     given, .cu:50-110 validity); every choice gives the closed-form MSM."""
     n = 1 << 16
@@ -428,7 +429,7 @@ def test_msm_context_window_bits_and_autotune(H, O):
     sc = O.random_fr(2 * n, 72)
     auto = H.MsmContext(pts)
     w, L, ms = auto.tuning()
-    assert w in (11, 12, 13) and L in (32, 64) and ms > 0
+    assert w in (13, 15) and L in (32, 64) and ms > 0
     want = [closed_form(O, ts, sc[i * n:(i + 1) * n]) for i in range(2)]
     got = auto.batch(sc, n, 2)
     assert all(O.g1_eq(a, b) for a, b in zip(got, want))
