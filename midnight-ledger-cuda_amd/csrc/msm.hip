@@ -620,6 +620,53 @@ __device__ __forceinline__ Xyzz28 msm_add(const Xyzz28 &p, const Xyzz28 &q, uint
     else return xyzz28_add_lz(p, q);
 }
 
+// the point held by the lane `kMask` places away (xor, kMask < 32), through
+// the LDS crossbar's swizzle (no LDS memory)
+template <int kMask>
+__device__ __forceinline__ Xyzz28 xyzz28_swizzle_xor(const Xyzz28 &p)
+{
+    constexpr int kPattern = (kMask << 10) | 0x1f;
+    Xyzz28 r;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        r.x.v[i] = (uint32_t)__builtin_amdgcn_ds_swizzle((int)p.x.v[i], kPattern);
+        r.y.v[i] = (uint32_t)__builtin_amdgcn_ds_swizzle((int)p.y.v[i], kPattern);
+        r.zz.v[i] = (uint32_t)__builtin_amdgcn_ds_swizzle((int)p.zz.v[i], kPattern);
+        r.zzz.v[i] = (uint32_t)__builtin_amdgcn_ds_swizzle((int)p.zzz.v[i], kPattern);
+    }
+    return r;
+}
+
+// Sum of the n points get(0..n-1) on a group of kLanes lanes: one lane, a
+// quad, or (kLanes = 16) four quads, each adding every fourth point, then
+// two rounds that add the quads' sums pairwise (lanes 4 and 8 apart): a
+// chain of ceil(n / 4) + 2 quad additions instead of n.  `sub` is the lane's
+// place in its group; every lane of a group passes the same n, and groups
+// leave a kernel only as a whole (the exchanges read within the group).
+// Points x with keep(x) false are skipped (empty buckets).
+template <int kLanes, class Get, class Keep>
+__device__ __forceinline__ Xyzz28 group_sum(uint32_t n, uint32_t sub, Get get, Keep keep)
+{
+    Xyzz28 acc = xyzz28_inf();
+    if constexpr (kLanes == 16) {
+        const uint32_t lane = sub & 3;
+        for (uint32_t x = sub >> 2; x < n; x += 4)
+            if (keep(x)) acc = xyzz28_add_quad(acc, get(x), lane);
+        acc = xyzz28_add_quad(acc, xyzz28_swizzle_xor<4>(acc), lane);
+        acc = xyzz28_add_quad(acc, xyzz28_swizzle_xor<8>(acc), lane);
+    } else {
+        for (uint32_t x = 0; x < n; ++x)
+            if (keep(x)) acc = msm_add<kLanes>(acc, get(x), sub);
+    }
+    return acc;
+}
+
+template <int kLanes, class Get>
+__device__ __forceinline__ Xyzz28 group_sum(uint32_t n, uint32_t sub, Get get)
+{
+    return group_sum<kLanes>(n, sub, get, [](uint32_t) { return true; });
+}
+
 template <int kLanes>
 __global__ void __launch_bounds__(kBlock) k_tree_level(const uint32_t *q, const uint32_t *co, const uint32_t *lmap,
                                                        const uint32_t *lo, const uint32_t *totals, uint64_t div,
@@ -632,11 +679,11 @@ __global__ void __launch_bounds__(kBlock) k_tree_level(const uint32_t *q, const 
     const uint32_t r = j - lo[b];
     const uint32_t qprev = level_len(q[b], div);
     const uint32_t s0 = r * G, s1 = min(s0 + G, qprev);
-    const uint64_t base = co[b];
-    Xyzz28 acc = xyzz28_inf();
-    for (uint32_t x = s0; x < s1; ++x)
-        acc = msm_add<kLanes>(acc, load_xyzz28(in + (base + x) * kXyzz28Words), sub);
-    if (sub == 0) store_xyzz28(out + (base + r) * kXyzz28Words, acc);
+    const uint64_t base = co[b] + s0;
+    const Xyzz28 acc = group_sum<kLanes>(s1 - s0, sub, [&](uint32_t x) {
+        return load_xyzz28(in + (base + x) * kXyzz28Words);
+    });
+    if (sub == 0) store_xyzz28(out + (co[b] + r) * kXyzz28Words, acc);
 }
 
 // Bucket b after `levels` levels: the levels that shrank it are the first
@@ -660,8 +707,9 @@ __global__ void __launch_bounds__(kBlock) k_tree_final(const uint32_t *q, const 
     }
     const uint32_t *src = (lv & 1) ? part2 : part;
     const uint64_t base = co[b];
-    Xyzz28 acc = xyzz28_inf();
-    for (uint32_t x = 0; x < rem; ++x) acc = msm_add<kLanes>(acc, load_xyzz28(src + (base + x) * kXyzz28Words), sub);
+    const Xyzz28 acc = group_sum<kLanes>(rem, sub, [&](uint32_t x) {
+        return load_xyzz28(src + (base + x) * kXyzz28Words);
+    });
     if (sub == 0) store_xyzz28(bucket + b * kXyzz28Words, acc);
 }
 
@@ -735,24 +783,28 @@ __global__ void __launch_bounds__(kBlock) k_plane_partials(const uint32_t *bucke
     uint32_t chunk = rem - pl.first[k];
     uint32_t e0 = chunk * G1, e1 = min(e0 + G1, pl.members[k]);
     const uint64_t stride = (uint64_t)g.M * g.kw;   // keys are bucket-major
-    Xyzz28 acc = xyzz28_inf();
+    Xyzz28 acc;
     if (pl.h) {
         // over the line sums of MSM mw: R lines 0 .. 2^h - 2, then C lines
         const uint32_t kk = k < pl.h ? k : k - pl.h;
         const uint32_t lowmask = (1u << kk) - 1u;
         const uint64_t base = mw * pl.nlines + (k < pl.h ? 0u : (1u << pl.h) - 1u);
-        for (uint32_t e = e0; e < e1; ++e) {
+        acc = group_sum<kLanes>(e1 - e0, sub, [&](uint32_t i) {
+            const uint32_t e = e0 + i;
             const uint32_t x = ((e >> kk) << (kk + 1)) | (1u << kk) | (e & lowmask);
-            acc = msm_add<kLanes>(acc, load_xyzz28(bucket + (base + x - 1) * kXyzz28Words), sub);
-        }
+            return load_xyzz28(bucket + (base + x - 1) * kXyzz28Words);
+        });
     } else {
         const uint32_t lowmask = (1u << k) - 1u;
-        for (uint32_t e = e0; e < e1; ++e) {
-            uint32_t x = ((e >> k) << (k + 1)) | (1u << k) | (e & lowmask);
-            const uint64_t key = (uint64_t)(x - 1) * stride + mw;
-            // empty buckets are not written
-            if (cnt[key]) acc = msm_add<kLanes>(acc, load_xyzz28(bucket + key * kXyzz28Words), sub);
-        }
+        const auto key = [&](uint32_t i) {
+            const uint32_t e = e0 + i;
+            const uint32_t x = ((e >> k) << (k + 1)) | (1u << k) | (e & lowmask);
+            return (uint64_t)(x - 1) * stride + mw;
+        };
+        // empty buckets are not written
+        acc = group_sum<kLanes>(
+            e1 - e0, sub, [&](uint32_t i) { return load_xyzz28(bucket + key(i) * kXyzz28Words); },
+            [&](uint32_t i) { return cnt[key(i)] != 0; });
     }
     if (sub == 0) store_xyzz28(pp + id * kXyzz28Words, acc);
 }
@@ -791,12 +843,11 @@ __global__ void __launch_bounds__(kBlock) k_line_partials(const uint32_t *bucket
         e1 = 1;
     }
     const uint64_t stride = (uint64_t)g.M * g.kw;   // keys are bucket-major
-    Xyzz28 acc = xyzz28_inf();
-    for (uint32_t e = e0; e < e1; ++e) {
-        const uint64_t key = (uint64_t)(x0 + step * e - 1) * stride + m;
-        // empty buckets are not written
-        if (cnt[key]) acc = msm_add<kLanes>(acc, load_xyzz28(bucket + key * kXyzz28Words), sub);
-    }
+    const auto key = [&](uint32_t i) { return (uint64_t)(x0 + step * (e0 + i) - 1) * stride + m; };
+    // empty buckets are not written
+    const Xyzz28 acc = group_sum<kLanes>(
+        e1 - e0, sub, [&](uint32_t i) { return load_xyzz28(bucket + key(i) * kXyzz28Words); },
+        [&](uint32_t i) { return cnt[key(i)] != 0; });
     if (sub == 0) store_xyzz28(pp + id * kXyzz28Words, acc);
 }
 
@@ -820,8 +871,9 @@ __global__ void __launch_bounds__(kBlock) k_group_sums(const uint32_t *in, uint6
     }
     uint32_t local = (uint32_t)(id - out_off[lo]);
     uint32_t s0 = seg_off[lo] + local * G, s1 = min(s0 + G, seg_off[lo] + seg_len[lo]);
-    Xyzz28 acc = xyzz28_inf();
-    for (uint32_t x = s0; x < s1; ++x) acc = msm_add<kLanes>(acc, load_xyzz28(in + (uint64_t)x * kXyzz28Words), sub);
+    const Xyzz28 acc = group_sum<kLanes>(s1 - s0, sub, [&](uint32_t x) {
+        return load_xyzz28(in + (uint64_t)(s0 + x) * kXyzz28Words);
+    });
     if (sub) return;
     // the last pass converts (fully reduced) to the 32-bit Jacobian form
     if (to_jac) store_jacobian(out + id * 36, xyzz_to_jacobian(xyzz28_to_std(acc)));
@@ -1455,11 +1507,13 @@ void MsmEngine::enqueue_back(hipStream_t st, uint32_t levels, ProfileScope *prof
     State &s = *st_;
     const Geometry &g = s.g;
     const uint64_t nkeys = s.nkeys;
-    // passes of fewer sums than this run each sum on a quad of lanes
-    const auto quad = [this](uint64_t sums) { return sums < quad_below; };
+    // lanes per sum of a reduction pass: passes of fewer sums than
+    // group16_below run each sum on four quads, of fewer than quad_below on
+    // one quad, else on one lane
+    const auto width = [this](uint64_t sums) { return sums < group16_below ? 16 : sums < quad_below ? 4 : 1; };
     levels = std::min(levels, s.tree_levels);
 
-    // bucket tree: dense levels, then one lane per bucket
+    // bucket tree: dense levels, then one sum per bucket
     uint32_t *tin = b.part.get(), *tout = b.part2.get();
     uint64_t div = 1;
     for (uint32_t l = 0; l < levels; ++l) {
@@ -1471,25 +1525,21 @@ void MsmEngine::enqueue_back(hipStream_t st, uint32_t levels, ProfileScope *prof
                            (const uint32_t *)b.lo.get(), nkeys, b.lmap.get(), b.totals.get());
         MH_HIP(hipGetLastError());
         const uint64_t lanes = s.nch_max / (div * s.G) + nkeys;   // bound on the level's groups
-        if (quad(lanes))
-            hipLaunchKernelGGL(k_tree_level<4>, dim3(grid1(4 * lanes)), dim3(kBlock), 0, st, b.q.get(), b.co.get(),
-                               b.lmap.get(), b.lo.get(), b.totals.get(), div, s.G, (const uint32_t *)tin, tout);
-        else
-            hipLaunchKernelGGL(k_tree_level<1>, dim3(grid1(lanes)), dim3(kBlock), 0, st, b.q.get(), b.co.get(),
-                               b.lmap.get(), b.lo.get(), b.totals.get(), div, s.G, (const uint32_t *)tin, tout);
+        const int w = width(lanes);
+        auto *tl = w == 16 ? k_tree_level<16> : w == 4 ? k_tree_level<4> : k_tree_level<1>;
+        hipLaunchKernelGGL(tl, dim3(grid1(w * lanes)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), b.lmap.get(),
+                           b.lo.get(), b.totals.get(), div, s.G, (const uint32_t *)tin, tout);
         MH_HIP(hipGetLastError());
         div *= s.G;
         std::swap(tin, tout);
     }
-    if (quad(nkeys))
-        hipLaunchKernelGGL(k_tree_final<4>, dim3(grid1(4 * nkeys)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), nkeys,
-                           s.G, levels, (const uint32_t *)b.part.get(), (const uint32_t *)b.part2.get(),
-                           b.bucket.get());
-    else
-        hipLaunchKernelGGL(k_tree_final<1>, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), nkeys,
-                           s.G, levels, (const uint32_t *)b.part.get(), (const uint32_t *)b.part2.get(),
-                           b.bucket.get());
-    MH_HIP(hipGetLastError());
+    {
+        const int w = width(nkeys);
+        auto *tf = w == 16 ? k_tree_final<16> : w == 4 ? k_tree_final<4> : k_tree_final<1>;
+        hipLaunchKernelGGL(tf, dim3(grid1(w * nkeys)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), nkeys, s.G, levels,
+                           (const uint32_t *)b.part.get(), (const uint32_t *)b.part2.get(), b.bucket.get());
+        MH_HIP(hipGetLastError());
+    }
 
     // group passes of a segment set down to one value per segment (the
     // last pass writes `final`, as Jacobian when `to_jac`)
@@ -1498,14 +1548,11 @@ void MsmEngine::enqueue_back(hipStream_t st, uint32_t levels, ProfileScope *prof
         for (size_t i = 0; i < lvls.size(); ++i) {
             const State::Level &lv = lvls[i];
             const bool last = i + 1 == lvls.size();
-            if (quad(lv.total_out))
-                hipLaunchKernelGGL(k_group_sums<4>, dim3(grid1(4 * lv.total_out)), dim3(kBlock), 0, st, lin, nseg,
-                                   b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base,
-                                   lv.G, lv.total_out, last ? final : lout, last ? to_jac : 0);
-            else
-                hipLaunchKernelGGL(k_group_sums<1>, dim3(grid1(lv.total_out)), dim3(kBlock), 0, st, lin, nseg,
-                                   b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base,
-                                   lv.G, lv.total_out, last ? final : lout, last ? to_jac : 0);
+            const int w = width(lv.total_out);
+            auto *gs = w == 16 ? k_group_sums<16> : w == 4 ? k_group_sums<4> : k_group_sums<1>;
+            hipLaunchKernelGGL(gs, dim3(grid1(w * lv.total_out)), dim3(kBlock), 0, st, lin, nseg,
+                               b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base, lv.G,
+                               lv.total_out, last ? final : lout, last ? to_jac : 0);
             MH_HIP(hipGetLastError());
             std::swap(lin, lout);
         }
@@ -1514,24 +1561,22 @@ void MsmEngine::enqueue_back(hipStream_t st, uint32_t levels, ProfileScope *prof
     // 7: two-level weights: line sums of the buckets, then planes over them
     const uint32_t *plane_src = b.bucket.get();
     if (s.lines) {
-        if (quad(s.nppl))
-            hipLaunchKernelGGL(k_line_partials<4>, dim3(grid1(4 * s.nppl)), dim3(kBlock), 0, st, b.bucket.get(),
-                               b.cnt.get(), g, s.ll, s.G1l, s.nppl, b.pp.get());
-        else
-            hipLaunchKernelGGL(k_line_partials<1>, dim3(grid1(s.nppl)), dim3(kBlock), 0, st, b.bucket.get(),
-                               b.cnt.get(), g, s.ll, s.G1l, s.nppl, b.pp.get());
+        const int w = width(s.nppl);
+        auto *lp = w == 16 ? k_line_partials<16> : w == 4 ? k_line_partials<4> : k_line_partials<1>;
+        hipLaunchKernelGGL(lp, dim3(grid1(w * s.nppl)), dim3(kBlock), 0, st, b.bucket.get(), b.cnt.get(), g, s.ll,
+                           s.G1l, s.nppl, b.pp.get());
         MH_HIP(hipGetLastError());
         group_passes(s.llevels, s.nsegl, b.lines.get(), 0);
         plane_src = b.lines.get();
     }
     // bit-plane partial sums and group passes down to one per segment
-    if (quad(s.npp))
-        hipLaunchKernelGGL(k_plane_partials<4>, dim3(grid1(4 * s.npp)), dim3(kBlock), 0, st, plane_src,
-                           b.cnt.get(), g, s.pl, s.G1, s.npp, b.pp.get());
-    else
-        hipLaunchKernelGGL(k_plane_partials<1>, dim3(grid1(s.npp)), dim3(kBlock), 0, st, plane_src, b.cnt.get(),
-                           g, s.pl, s.G1, s.npp, b.pp.get());
-    MH_HIP(hipGetLastError());
+    {
+        const int w = width(s.npp);
+        auto *pp = w == 16 ? k_plane_partials<16> : w == 4 ? k_plane_partials<4> : k_plane_partials<1>;
+        hipLaunchKernelGGL(pp, dim3(grid1(w * s.npp)), dim3(kBlock), 0, st, plane_src, b.cnt.get(), g, s.pl, s.G1,
+                           s.npp, b.pp.get());
+        MH_HIP(hipGetLastError());
+    }
     group_passes(s.levels, s.nseg, b.pj.get(), 1);
     MH_HIP(hipMemcpyAsync(b.h_pj, b.pj.get(), s.nseg * 144, hipMemcpyDeviceToHost, st));
     MH_HIP(hipMemcpyAsync(b.h_totals, b.totals.get(), 16, hipMemcpyDeviceToHost, st));
