@@ -799,6 +799,7 @@ extern "C" RustError midnight_hip_fr_device_vec_ntt_zeta_noalloc(void *device_ve
     job.in = dv->values.get();
     job.out = dv->values.get();
     job.scratch = ntt_scratch_elems(log_n, 1) ? scr : nullptr;
+    job.mid_table = false;   // nothing allocated, not even a twiddle table
     job.in_pstride = job.out_pstride = size_t(1) << log_n;
     job.log_n = log_n;
     job.count = 1;

@@ -182,6 +182,8 @@ struct PassArgs {
     uint64_t factor_len;
     Fr29 first, second, scale;          // stored as c * 2^261 mod r
     Tab29 roots, lo, hi;
+    Tab29 mid;                          // w_n^(c k) at c * m + k (with 1/n for inverses), when has_mid
+    uint32_t has_mid;
 };
 
 __device__ __forceinline__ uint32_t bitrev(uint32_t x, uint32_t bits)
@@ -354,13 +356,27 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
         Fr29 v = lds.get((c << log_m) + k);
         const uint64_t L = (uint64_t)k * a.out_es + (uint64_t)(c0 + c) * a.out_cs;
         if (a.twiddle) {
-            // w_n^(c k) from the two tables (w^0 = 1 included); the product
-            // is below 2.4r
-            const uint32_t e = ((c0 + c) * k) << tw_shift;
-            Fr29 w = a.lo.get(e & 2047);
-            if (e >> 11) w = fr29_mul_loose(w, a.hi.get(e >> 11));
+            // w_n^(c k) from the transform's table, or formed from the two
+            // small tables (w^0 = 1 included); the product is below 2.4r
+            Fr29 w;
+            if (a.has_mid) {
+                w = a.mid.get(((c0 + c) << log_m) + k);
+            } else {
+                const uint32_t e = ((c0 + c) * k) << tw_shift;
+                w = a.lo.get(e & 2047);
+                if (e >> 11) w = fr29_mul_loose(w, a.hi.get(e >> 11));
+            }
             v = fr29_mul_loose(v, w);
             fr29_reduce_2r(v);
+            fr29_reduce(v);
+        } else if (a.post == 3) {
+            // the zeta pattern alone (below 1.1r after a factor, else < 46r)
+            if (L % 3) {
+                v = zeta29(v, L, a.first, a.second);
+            } else {
+                fr29_reduce_3r(v);
+                fr29_reduce_2r(v);
+            }
             fr29_reduce(v);
         } else if (a.post) {
             // below 1.7r after the scale, below 1.1r after a zeta factor
@@ -440,7 +456,70 @@ void launch(const PassArgs &a, uint32_t ncols, uint32_t npolys, hipStream_t st)
     MH_HIP(hipGetLastError());
 }
 
+// The four-step twiddles of a two-pass transform of 2^log_n: entry
+// c * n1 + k = w_n^(c k) (c < n2 the column, k < n1 its row), times 1/n for
+// the inverse direction, stored scaled in the three planes of Tab29.
+__global__ void __launch_bounds__(256) k_mid_table(uint32_t log_n, uint32_t log_m, Tab29 lo, Tab29 hi, Fr29 factor,
+                                                   uint32_t has_factor, uint4 *p0, uint4 *p1, uint32_t *p2)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >> log_n) return;
+    const uint32_t c = i >> log_m, k = i & ((1u << log_m) - 1u);
+    const uint32_t e = (c * k) << (kTwLog - log_n);
+    Fr29 w = lo.get(e & 2047);
+    if (e >> 11) w = fr29_mul_loose(w, hi.get(e >> 11));
+    if (has_factor) w = fr29_mul_loose(w, factor);
+    fr29_reduce_2r(w);
+    fr29_reduce(w);
+    p0[i] = make_uint4(w.v[0], w.v[1], w.v[2], w.v[3]);
+    p1[i] = make_uint4(w.v[4], w.v[5], w.v[6], w.v[7]);
+    p2[i] = w.v[8];
+}
+
+// Tables per (device, log_n, direction), built on first use and kept until
+// the process ends (2^22: 151 MB per direction).  A stream under graph
+// capture gets none when the table is not built yet (the pass then forms
+// its twiddles from the small tables): building allocates and waits.
+struct MidEntry {
+    bool built = false;
+    Tab29 tab{};
+};
+std::mutex g_mid_mu;
+MidEntry g_mid[64][2 * kMaxLogLds + 1][2];
+
+const Tab29 *mid_table(uint32_t log_n, int dir, hipStream_t st)
+{
+    const int dev = device_id();
+    std::lock_guard<std::mutex> g(g_mid_mu);
+    MidEntry &m = g_mid[dev][log_n][dir];
+    if (m.built) return &m.tab;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    MH_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) return nullptr;
+    const Tables &T = tables();
+    const size_t n = size_t(1) << log_n;
+    uint32_t *mem = nullptr;
+    MH_HIP(hipMalloc(reinterpret_cast<void **>(&mem), n * kLdsElemBytes));
+    uint4 *p0 = reinterpret_cast<uint4 *>(mem), *p1 = p0 + n;
+    uint32_t *p2 = mem + 8 * n;
+    const HostFr ninv = HostFr::from_u64(n).inverse();
+    hipLaunchKernelGGL(k_mid_table, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, log_n,
+                       (uint32_t)((log_n + 1) / 2), T.lo[dir], T.hi[dir], to_dev29(scaled29(ninv)), (uint32_t)dir,
+                       p0, p1, p2);
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipStreamSynchronize(st));
+    m.tab = Tab29{p0, p1, p2};
+    m.built = true;
+    return &m.tab;
+}
+
 }  // namespace
+
+void ntt_warm_tables(uint32_t log_n, hipStream_t st)
+{
+    if (log_n <= kMaxLogLds || log_n > 2 * kMaxLogLds) return;
+    for (int dir = 0; dir < 2; ++dir) (void)mid_table(log_n, dir, st);
+}
 
 namespace {
 
@@ -450,7 +529,7 @@ struct PassLaunch {
 };
 
 // the one (log n <= 11) or two passes of a job
-int job_passes(const NttJob &job, PassLaunch out[2])
+int job_passes(const NttJob &job, PassLaunch out[2], hipStream_t st)
 {
     const uint32_t log_n = job.log_n;
     MH_CHECK(log_n <= 2 * kMaxLogLds, hipErrorInvalidValue, "NTT size above 2^22 is not supported");
@@ -458,22 +537,23 @@ int job_passes(const NttJob &job, PassLaunch out[2])
     lds_limit();
     const int dir = job.inverse ? 1 : 0;
     const uint64_t n = 1ull << log_n;
+    // a two-pass transform reads its four-step twiddles (with 1/n for an
+    // inverse) from its table when there is one
+    const Tab29 *mid = (log_n > kMaxLogLds && job.mid_table) ? mid_table(log_n, dir, st) : nullptr;
 
-    // post: inverse includes 1/n; ntt_zeta multiplies by scale and the zeta pattern
-    HostFr post_scale = HostFr::one();
-    bool has_post = false;
-    if (job.inverse) {
-        post_scale = HostFr::from_u64(n).inverse();
-        has_post = true;
-    }
-    if (job.post_zeta) {
-        post_scale = post_scale * job.scale;
-        has_post = true;
-    }
-    if (job.post_mul) {
-        post_scale = post_scale * job.mul;
-        has_post = true;
-    }
+    // post: inverse includes 1/n (unless the table holds it); ntt_zeta
+    // multiplies by scale and the zeta pattern.  Factors equal to one are
+    // not multiplied.
+    const HostFr one = HostFr::one();
+    HostFr post_scale = one;
+    if (job.inverse && !mid) post_scale = HostFr::from_u64(n).inverse();
+    if (job.post_zeta) post_scale = post_scale * job.scale;
+    if (job.post_mul) post_scale = post_scale * job.mul;
+    const bool post_pattern = job.post_zeta && !(job.first == one && job.second == one);
+    const bool has_post = post_pattern || !(post_scale == one);
+    const uint32_t post_mode = !has_post ? 0 : (post_scale == one ? 3 : (post_pattern ? 2 : 1));
+    // the zeta pre-operation with unit factors is the zero padding alone
+    const uint32_t pre = (job.pre == 1 && job.first == one && job.second == one) ? 0 : job.pre;
 
     PassArgs base{};
     base.roots = T.roots[dir];
@@ -498,9 +578,9 @@ int job_passes(const NttJob &job, PassLaunch out[2])
         a.log_m = log_n;
         a.log_C = 0;
         a.twiddle = 0;
-        a.pre = job.pre;
+        a.pre = pre;
         a.in_len = job.in_len;
-        a.post = has_post ? (job.post_zeta ? 2 : 1) : 0;
+        a.post = post_mode;
         out[0] = PassLaunch{a, 1, job.count};
         return 1;
     }
@@ -519,9 +599,13 @@ int job_passes(const NttJob &job, PassLaunch out[2])
     A.log_m = t1;
     A.log_C = pick_log_cols(t1, n2);
     A.twiddle = 1;
-    A.pre = job.pre;
+    A.pre = pre;
     A.in_len = job.in_len;
     A.post = 0;
+    if (mid) {
+        A.mid = *mid;
+        A.has_mid = 1;
+    }
     out[0] = PassLaunch{A, n2, job.count};
 
     PassArgs B = base;
@@ -536,7 +620,7 @@ int job_passes(const NttJob &job, PassLaunch out[2])
     B.twiddle = 0;
     B.pre = 0;
     B.in_len = ~0ull;
-    B.post = has_post ? (job.post_zeta ? 2 : 1) : 0;
+    B.post = post_mode;
     out[1] = PassLaunch{B, n1, job.count};
     return 2;
 }
@@ -546,7 +630,7 @@ int job_passes(const NttJob &job, PassLaunch out[2])
 void ntt_run(const NttJob &job, hipStream_t st)
 {
     PassLaunch p[2];
-    const int np = job_passes(job, p);
+    const int np = job_passes(job, p, st);
     for (int i = 0; i < np; ++i) launch(p[i].a, p[i].ncols, p[i].npolys, st);
 }
 
@@ -601,7 +685,7 @@ void RaggedNtt::prepare(Fr *base, const size_t *offsets, const uint32_t *log_ns,
         job.scratch = im.scratch.get() + soff;
         soff += ntt_scratch_elems(log_n, r.count);
         PassLaunch p[2];
-        const int np = job_passes(job, p);
+        const int np = job_passes(job, p, st);
         for (int s = 0; s < np; ++s) {
             const uint32_t gx = p[s].ncols >> p[s].a.log_C;
             const uint64_t nb = (uint64_t)gx * p[s].npolys;

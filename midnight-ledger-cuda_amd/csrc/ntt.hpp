@@ -28,9 +28,16 @@ struct NttJob {
     HostFr first = HostFr::one(), second = HostFr::one();
     bool post_mul = false;       // also multiply every output by `mul`
     HostFr mul = HostFr::one();
+    // Two-pass transforms read their four-step twiddles from a table per
+    // size and direction, built (allocated) on first use; false forms them
+    // from two small static tables instead (no allocation).
+    bool mid_table = true;
 };
 
 void ntt_run(const NttJob &job, hipStream_t st);
+// Builds the four-step twiddle tables of 2^log_n (both directions) now, so
+// that transforms captured in a graph later find them.
+void ntt_warm_tables(uint32_t log_n, hipStream_t st);
 size_t ntt_scratch_elems(uint32_t log_n, size_t count);
 
 // In-place transforms of mixed sizes (polynomial i: 2^log_ns[i] values at

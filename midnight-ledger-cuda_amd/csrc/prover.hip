@@ -352,6 +352,10 @@ std::unique_ptr<PlonkKey> plonk_keygen(uint32_t k, void *srs, size_t srs_points,
     pk->ext_k = ek;
     pk->ext_n = size_t(1) << ek;
     pk->pieces = degree - 1;
+    // the transforms of both sizes find their twiddle tables built, also
+    // the ones captured in the commitment stages' graphs
+    ntt_warm_tables(k, st);
+    ntt_warm_tables(ek, st);
     pk->omega = HostFr::omega(k);
     pk->omega_inv = pk->omega.inverse();
     pk->ext_omega = HostFr::omega(ek);
