@@ -88,8 +88,9 @@ struct NumArgs {
     const RawColumnRef *perm_columns;
     const Fr *perm_cosets;
     uint32_t perm_sets, perm_cols;
-    const Fr *omega_tab;      // scaled
-    uint32_t omega_levels;
+    const Fr *omega_tab;      // scaled; level 0 times delta_start when omega_has_delta
+    uint32_t omega_levels, omega_has_delta;
+    uint32_t out_scaled;      // write the scaled form (fully reduced) instead of the usual one
     uint32_t lookup_count, trash_count;
     const Fr *lookup_products, *lookup_inputs, *lookup_tables;
     const Fr *trash_cosets;
@@ -308,7 +309,8 @@ __device__ __forceinline__ S permutation_terms(S v, size_t idx, const NumArgs &a
 
     const S l_active = s_load(a.l_active + idx, ls);
     const S beta = s_const(a.beta), gamma = s_const(a.gamma), dconst = s_const(a.delta_const);
-    S delta = s_mul(s_const(a.delta_start), omega_pow(idx, a));
+    // delta_start omega^idx (the table's first level may already hold delta_start)
+    S delta = a.omega_has_delta ? omega_pow(idx, a) : s_mul(s_const(a.delta_start), omega_pow(idx, a));
     for (uint32_t s = 0; s < a.perm_sets; ++s) {
         const Fr *z = a.perm_products + (size_t)s * len;
         S left = s_load(z + r_next, zs);
@@ -323,7 +325,7 @@ __device__ __forceinline__ S permutation_terms(S v, size_t idx, const NumArgs &a
             const S sigma = s_load(a.perm_cosets + (size_t)(base + j) * len + idx, cs);
             left = s_mul(left, s_add(x, s_mul(beta, sigma)));
             right = s_mul(right, s_add(x, delta));
-            delta = s_mul(delta, dconst);
+            if (base + j + 1 < a.perm_cols) delta = s_mul(delta, dconst);   // (the last column's is unused)
         }
         v = s_fold(v, y, s_sub(left, right), l_active);
     }
@@ -394,8 +396,27 @@ __global__ __launch_bounds__(kBlock, 4) void k_numerator(NumArgs a)
                 v = s_add(s_mul(v, y), s_sub(compressed, s_mul(s_sub(one, q), trash)));
             }
         }
-        store_fr(a.out + idx, s29_to_usual(v));
+        if (a.out_scaled) {
+            fr29_reduce(v);
+            store_fr(a.out + idx, fr29_to32(v));
+        } else {
+            store_fr(a.out + idx, s29_to_usual(v));
+        }
     }
+}
+
+// dst[i] = src[i] * factor for the first n0 entries, src[i] after them (all
+// scaled): the omega table with delta_start folded into its first level
+__global__ __launch_bounds__(kBlock) void k_scale_prefix(Fr *dst, const Fr *src, size_t n0, size_t n, Fr factor)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    S v = fr29_from32(load_fr(src + i));
+    if (i < n0) {
+        v = s_mul(v, s_const(factor));
+        fr29_reduce(v);
+    }
+    store_fr(dst + i, fr29_to32(v));
 }
 
 // v[i] (usual form) -> v[i] 2^5 mod r: the scaled form, stored as 8 x 32 bits
@@ -439,7 +460,9 @@ Fr scaled_fr(const HostFr &h)
 Fr scaled_fr(const RawFr &v) { return scaled_fr(to_hfr(v)); }
 
 // omega^(j 2^(11 l)) tables (rowpow.hpp layout) in the scaled form
-void build_scaled_omega_table(DevBuf<Fr> &tab, uint32_t &levels, const HostFr &omega, size_t len, hipStream_t st)
+// (level 0 times `first` when given: delta_start folded in)
+void build_scaled_omega_table(DevBuf<Fr> &tab, uint32_t &levels, const HostFr &omega, size_t len, hipStream_t st,
+                              const HostFr *first = nullptr)
 {
     uint32_t bits = 0;
     while (bits < 64 && (1ull << bits) < len) ++bits;
@@ -449,8 +472,9 @@ void build_scaled_omega_table(DevBuf<Fr> &tab, uint32_t &levels, const HostFr &o
     HostFr step = omega;
     for (uint32_t l = 0; l < levels; ++l) {
         HostFr acc = HostFr::one();
+        const HostFr f = (l == 0 && first) ? k32 * *first : k32;
         for (uint32_t j = 0; j < kOmegaTab; ++j) {
-            h[(size_t)l * kOmegaTab + j] = acc * k32;
+            h[(size_t)l * kOmegaTab + j] = acc * f;
             acc = acc * step;
         }
         step = acc;   // omega^(2048 * 2^(11 l))
@@ -713,7 +737,11 @@ extern "C" RustError midnight_sppark_bls12_381_eval_numerator(
     a.trash_count = (uint32_t)trash_count;
     a.trash_cosets = up(d_tc, trash_cosets, trash_count * len, st);
     a.trash_sel = up_array(d_sel, trash_selector_fixed_indices, trash_count, st);
-    build_scaled_omega_table(d_omega, a.omega_levels, to_hfr(omega), len, st);
+    {
+        const HostFr ds = to_hfr(delta_start);
+        build_scaled_omega_table(d_omega, a.omega_levels, to_hfr(omega), len, st, &ds);
+        a.omega_has_delta = 1;
+    }
     a.omega_tab = d_omega.get();
 
     GraphSet gs;
@@ -870,7 +898,7 @@ RustError numerator_context_eval(void **out, const void *raw_context, const void
                                  RawFr gamma, RawFr theta, RawFr trash_challenge, RawFr y, size_t permutation_offset,
                                  size_t permutation_sets, size_t lookup_product_offset, size_t lookup_input_offset,
                                  size_t lookup_table_offset, size_t lookup_count, size_t trash_offset,
-                                 size_t trash_count, bool columns_scaled)
+                                 size_t trash_count, bool columns_scaled, bool out_scaled)
 {
     if (out == nullptr || raw_context == nullptr || (challenge_len != 0 && challenges == nullptr))
         return make_error((int)hipErrorInvalidValue, "invalid numerator evaluation arguments");
@@ -934,8 +962,19 @@ RustError numerator_context_eval(void **out, const void *raw_context, const void
     a.perm_columns = c->perm_columns.get();
     a.perm_cols = (uint32_t)c->perm_cols;
     a.perm_cosets = c->perm_cosets.get();
-    a.omega_tab = c->omega_tab.get();
+    // the context's omega table with this evaluation's delta_start folded
+    // into its first level (one product per row fewer)
+    DevBuf<Fr> d_omega(c->omega_levels * kOmegaTab);
+    {
+        const size_t n = (size_t)c->omega_levels * kOmegaTab;
+        hipLaunchKernelGGL(k_scale_prefix, dim3((unsigned)ceil_div(n, kBlock)), dim3(kBlock), 0, st, d_omega.get(),
+                           (const Fr *)c->omega_tab.get(), (size_t)kOmegaTab, n, a.delta_start);
+        MH_HIP(hipGetLastError());
+    }
+    a.omega_tab = d_omega.get();
     a.omega_levels = c->omega_levels;
+    a.omega_has_delta = 1;
+    a.out_scaled = out_scaled ? 1u : 0u;
     a.lookup_count = (uint32_t)lookup_count;
     a.lookup_products = view(lookup_product_offset, lookup_count);
     a.lookup_inputs = view(lookup_input_offset, lookup_count);
@@ -966,7 +1005,7 @@ extern "C" RustError midnight_sppark_bls12_381_numerator_context_eval(
                                   instance_offset, instance_cols, challenges, challenge_len, beta, gamma, theta,
                                   trash_challenge, y, permutation_offset, permutation_sets, lookup_product_offset,
                                   lookup_input_offset, lookup_table_offset, lookup_count, trash_offset, trash_count,
-                                  false);
+                                  false, false);
 }
 
 // bls12_381_g1_msm.cu:5667-5736: the context's trash graphs over its

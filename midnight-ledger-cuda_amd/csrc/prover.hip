@@ -249,7 +249,7 @@ struct PlonkKey {
     std::unique_ptr<DeviceVec> fixed_coeff, sigma_coeff;  // coefficient form
     std::vector<RawG1Projective> fixed_com, sigma_com;
     HostFr omega, omega_inv, ext_omega, repr;
-    std::vector<HostFr> t_inv, delta_powers;
+    std::vector<HostFr> t_inv, t_inv_scaled, delta_powers;
     void *numctx = nullptr;
     uint32_t rng_threads = 1;   // thread count of the reference prover whose random draws are reproduced
     bool graphs = true;         // commitment stages replayed from HIP graphs
@@ -366,6 +366,9 @@ std::unique_ptr<PlonkKey> plonk_keygen(uint32_t k, void *srs, size_t srs_points,
             pk->t_inv.push_back((cur - HostFr::one()).inverse());
             cur = cur * step;
         }
+        // for a numerator written in the scaled form (32 times its value)
+        const HostFr inv32 = HostFr::from_u64(32).inverse();
+        for (const HostFr &t : pk->t_inv) pk->t_inv_scaled.push_back(t * inv32);
     }
     // blinding factors (plonk/circuit.rs:2502-2530): max(3, queries per advice column) + 1 + 1
     size_t maxq = 1;
@@ -646,13 +649,15 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     MH_HIP(hipStreamSynchronize(st));
     void *hv = nullptr;
     check(numerator_context_eval(&hv, pk.numctx, cols.get(), A + S, 0, A, 0, 0, nullptr, 0, raw(beta), raw(gamma),
-                                 raw(theta), raw(trash), raw(y), A, S, 0, 0, 0, 0, 0, 0, true));
+                                 raw(theta), raw(trash), raw(y), A, S, 0, 0, 0, 0, 0, 0, true, true));
     cols.reset();
     std::unique_ptr<DeviceVec> h(reinterpret_cast<DeviceVec *>(hv));
-    // h = numerator / (X^n - 1) on the coset, then coefficients (extended_to_coeff)
+    // h = numerator / (X^n - 1) on the coset, then coefficients
+    // (extended_to_coeff); the numerator is 32 times its value (scaled form),
+    // which the division's factors take out
     check(midnight_sppark_bls12_381_fr_device_vec_divide_ntt_zeta(
-        h.get(), pk.ext_k, reinterpret_cast<const RawFr *>(pk.t_inv.data()), pk.t_inv.size(), raw(HostFr::one()),
-        raw(kZeta.sqr()), raw(kZeta)));
+        h.get(), pk.ext_k, reinterpret_cast<const RawFr *>(pk.t_inv_scaled.data()), pk.t_inv_scaled.size(),
+        raw(HostFr::one()), raw(kZeta.sqr()), raw(kZeta)));
     DevBuf<Fr> lb(limb_blinds.size());
     MH_HIP(hipMemcpyAsync(lb.get(), limb_blinds.data(), limb_blinds.size() * 32, hipMemcpyHostToDevice, st));
     fr_quotient_pieces(pd + P_H * n, h->values.get(), n, pk.pieces, lb.get(), st);
