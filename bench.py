@@ -522,16 +522,19 @@ def bench_prove(args, H):
     permutation, extended domain 4x (2^22).  A step is one
     midnight_hip_plonk_prove call: advice upload, commitments, permutation
     products, numerator and quotient on the 2^22 coset, evaluations and the
-    KZG multi-open, returning the proof bytes.  Key and SRS are resident."""
+    KZG multi-open, returning the proof bytes.  Key and SRS are resident;
+    the SRS has 2^max(k, --srs-log) points (--srs-log 20: the serving
+    configuration, small circuits over the 2^20-point SRS)."""
     import numerator_graph as G
     k = args.log_n
+    srs_log = max(k, args.srs_log)
     rng = np.random.default_rng(args.seed)
-    log("building the circuit, SRS and key for k = %d" % k)
+    log("building the circuit, SRS (2^%d points) and key for k = %d" % (srs_log, k))
     tau = random_canonical(1, rng)
     tdv = device_fr(H, tau)
     tau_m = tdv.download()[0]
     tdv.free()
-    pw = H.DeviceVec.upload(H.fr_shifted_prefix_product(np.tile(tau_m, (1 << k, 1)), MONT_ONE))
+    pw = H.DeviceVec.upload(H.fr_shifted_prefix_product(np.tile(tau_m, (1 << srs_log, 1)), MONT_ONE))
     srs = H.MsmContext.from_multiples(pw)
     pw.free()
     gate = ("prod", ("fixed", 0, 0), ("sum", ("sum", ("prod", ("advice", 0, 0), ("advice", 1, 0)),
@@ -568,7 +571,8 @@ def bench_prove(args, H):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic satisfying circuit of the benchmark family; SRS [tau^i]G built on the device",
         "config": {"workload": "full proof, k=%d (2^%d rows), 5 advice / 3 fixed, gate q(ab+c-d), 8-column "
-                               "permutation, 4x extended domain" % (k, k), "proof_bytes": len(proof)},
+                               "permutation, 4x extended domain" % (k, k), "proof_bytes": len(proof),
+                   "srs_points": 1 << srs_log},
         "p50_ms": round(lat[len(lat) // 2] * 1e3, 3),
         "stage_ms_per_proof": {"commit_stages_graph": round(stage_ms / args.steps, 3),
                                "msm_total_eager": round(msm_ms / args.steps, 3),
@@ -959,6 +963,7 @@ def main():
     ap.add_argument("--workload", default="msm", choices=["msm", "ntt", "batch", "quotient", "kzg", "serve", "prove",
                                                          "poseidon"])
     ap.add_argument("--log-n", type=int, default=20)
+    ap.add_argument("--srs-log", type=int, default=0, help="prove: SRS of 2^max(log-n, srs-log) points")
     ap.add_argument("--msms-per-step", type=int, default=4)
     ap.add_argument("--skewed", action="store_true")
     ap.add_argument("--chain-len", type=int, default=1024, help="poseidon: hashes in the chain")

@@ -156,7 +156,7 @@ extern "C" RustError midnight_hip_batch_plan_create(void **out, void *msm_contex
         if (has_small) cands.push_back({true, 0, 64, true});
         const uint32_t w = msm_auto_window(max_n);
         for (uint32_t c : {w - 1, w, w + 1})
-            if (c >= 8 && c <= 16 && full_top_window(c)) cands.push_back({false, c, 64});
+            if (c >= 8 && c <= 16) cands.push_back({false, c, 64});
     } else if (context_table(*ctx, max_n, max_end, pc)) {
         cands.push_back({true, 0, 64, pc.table == ctx->small_table.get()});
     } else {

@@ -921,9 +921,7 @@ uint32_t rule_table_window(size_t n)
 {
     int lg = 0;
     while ((size_t(1) << lg) < n) ++lg;
-    uint32_t c = (uint32_t)std::min(16, std::max(10, lg - 4));
-    while (!full_top_window(c)) ++c;
-    return c;
+    return (uint32_t)std::min(16, std::max(10, lg - 4));
 }
 
 // the per-window table of window c into `table`; false when its entries
@@ -998,7 +996,11 @@ void build_precomp_main(MsmContext &ctx, hipStream_t st)
     const size_t n = ctx.npoints;
     if (n < (size_t(1) << 10)) return;
     const uint32_t wb = ctx.window_bits;
-    const uint32_t c0 = wb ? std::min<uint32_t>(kMaxTableC, std::max<uint32_t>(10, wb)) : rule_table_window(n);
+    uint32_t c0 = wb ? std::min<uint32_t>(kMaxTableC, std::max<uint32_t>(10, wb)) : rule_table_window(n);
+    // untuned contexts avoid the widths whose top window is degenerate
+    // (tuned ones time them: one batch of full-length MSMs may still gain)
+    if (!wb && n < (size_t(1) << 16))
+        while (!full_top_window(c0)) ++c0;
     if (wb || n < (size_t(1) << 16)) {
         if (build_table(ctx, c0, ctx.table, st)) {
             ctx.pc_c = c0;
@@ -1015,15 +1017,7 @@ void build_precomp_main(MsmContext &ctx, hipStream_t st)
     const std::vector<MsmJob> jobs(4, MsmJob{sc.get(), 0, n});
     MsmEngine eng;
     float best = 1e30f;
-    // widths within two bits of the rule's whose top window is not
-    // degenerate, nearest first, at most three
-    std::vector<uint32_t> cands;
-    for (uint32_t d = 0; d <= 2 && cands.size() < 3; ++d)
-        for (uint32_t c : {c0 - d, c0 + d})
-            if (c >= 10 && c <= kMaxTableC && full_top_window(c) && cands.size() < 3 &&
-                std::find(cands.begin(), cands.end(), c) == cands.end())
-                cands.push_back(c);
-    for (uint32_t c : cands) {
+    for (uint32_t c = std::max<uint32_t>(10, c0 - 1); c <= std::min<uint32_t>(kMaxTableC, c0 + 1); ++c) {
         DevBuf<uint32_t> table;
         if (!build_table(ctx, c, table, st)) continue;
         const uint32_t nwin = msm_windows(c);
