@@ -456,6 +456,37 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles(const uint32_t *in, uint3
     if (threadIdx.x == kBlock - 1 && sums) sums[blockIdx.x] = s[kBlock - 1];
 }
 
+// Exclusive scan of at most kSmallScan values in one workgroup: each thread
+// sums a run of consecutive values, the runs' totals are scanned in LDS, and
+// each thread writes its run's prefixes.  One launch where the tiled scan
+// takes three (a small proof's bucket and chunk scans).
+constexpr uint32_t kSmallScanThreads = 1024;
+constexpr uint64_t kSmallScan = 32768;
+__global__ void __launch_bounds__(kSmallScanThreads) k_scan_small(const uint32_t *in, uint32_t *out, uint32_t *total,
+                                                                 uint64_t n)
+{
+    __shared__ uint32_t s[kSmallScanThreads];
+    const uint32_t per = (uint32_t)((n + kSmallScanThreads - 1) / kSmallScanThreads);
+    const uint64_t b0 = (uint64_t)threadIdx.x * per, b1 = min(b0 + per, n);
+    uint32_t t = 0;
+    for (uint64_t i = b0; i < b1; ++i) t += in[i];
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t off = 1; off < kSmallScanThreads; off <<= 1) {
+        const uint32_t add = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - t;
+    for (uint64_t i = b0; i < b1; ++i) {
+        const uint32_t v = in[i];
+        out[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == kSmallScanThreads - 1 && total) *total = s[kSmallScanThreads - 1];
+}
+
 __global__ void __launch_bounds__(kBlock) k_scan_add(uint32_t *out, const uint32_t *sums, uint64_t n)
 {
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
@@ -1144,6 +1175,11 @@ uint64_t scan_tmp_elems(uint64_t n)
 // allocation, no read-back), tmp holds scan_tmp_elems(n)
 void scan_launch(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp, hipStream_t st)
 {
+    if (n > kScanTile && n <= kSmallScan) {
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kSmallScanThreads), 0, st, in, out, tmp, n);
+        MH_HIP(hipGetLastError());
+        return;
+    }
     const auto sizes = scan_levels(n);
     std::vector<uint32_t *> sums(sizes.size(), nullptr), sscan(sizes.size(), nullptr);
     uint64_t o = 0;

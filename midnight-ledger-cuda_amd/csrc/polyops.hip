@@ -21,6 +21,7 @@
 
 #include "devvec.hpp"
 #include "fields.hpp"
+#include "fr29.hpp"
 #include "polyops.hpp"
 #include "rowpow.hpp"
 #include "runtime.hpp"
@@ -35,11 +36,18 @@ constexpr int kScanItems = 8;
 constexpr size_t kScanTile = (size_t)kBlock * kScanItems;
 
 constexpr uint32_t kPowPoints = 16;
+constexpr uint32_t kPowLevels = 3;   // tables for lengths up to 2^33
+
+// the level bases x^(2048^l) of up to kPowPoints points, in the scaled form
+// (x 2^261 mod r), from the host
 struct PowPoints {
-    Fr p[kPowPoints];
+    Fr b[kPowPoints][kPowLevels];
 };
 
-// one lane per entry of the power tables of up to kPowPoints points
+// One lane per entry of the power tables of up to kPowPoints points: entry
+// j of level l is base_l^j, an 11-bit square-and-multiply chain of
+// radix-2^29 products in the scaled form, then one product back to the
+// usual form.
 __global__ __launch_bounds__(kBlock) void k_omega_tables(Fr *tab, uint32_t levels, PowPoints pts, uint32_t npts)
 {
     const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -47,17 +55,19 @@ __global__ __launch_bounds__(kBlock) void k_omega_tables(Fr *tab, uint32_t level
     if (t >= npts * per) return;
     const uint32_t p = (uint32_t)(t / per), r = (uint32_t)(t % per);
     const uint32_t l = r >> kOmegaBits, j = r & (kOmegaTab - 1);
-    Fr base = pts.p[0];
+    Fr bsel = pts.b[0][0];
 #pragma unroll
-    for (uint32_t i = 1; i < kPowPoints; ++i)   // selected, not indexed: the argument stays in registers
-        if (p == i) base = pts.p[i];
-    for (uint32_t i = 0; i < kOmegaBits * l; ++i) base = base * base;
-    Fr acc = Fr::one();
+    for (uint32_t i = 0; i < kPowPoints; ++i)   // selected, not indexed: the argument stays in registers
+#pragma unroll
+        for (uint32_t q = 0; q < kPowLevels; ++q)
+            if (p == i && l == q) bsel = pts.b[i][q];
+    const Fr29 base = fr29_from32(bsel);
+    Fr29 acc = s29_one();
     for (int b = kOmegaBits - 1; b >= 0; --b) {
-        acc = acc * acc;
-        if ((j >> b) & 1u) acc = acc * base;
+        acc = fr29_mul(acc, acc);
+        if ((j >> b) & 1u) acc = fr29_mul(acc, base);
     }
-    store_fr(tab + t, acc);
+    store_fr(tab + t, s29_to_usual(acc));
 }
 
 // partial[(poly * npts + pt) * nblk + block] = sum over the block's rows
@@ -301,8 +311,17 @@ void omega_tables_device(Fr *tab, uint32_t levels, const HostFr *points, size_t 
     const size_t per = (size_t)levels * kOmegaTab;
     for (size_t p0 = 0; p0 < npts; p0 += kPowPoints) {
         const size_t m = std::min<size_t>(kPowPoints, npts - p0);
+        MH_CHECK(levels <= kPowLevels, (int)hipErrorInvalidValue, "polynomial too long for the power tables");
         PowPoints pts{};
-        for (size_t i = 0; i < m; ++i) std::memcpy(pts.p[i].v, points[p0 + i].l, 32);
+        const HostFr k32 = HostFr::from_u64(32);
+        for (size_t i = 0; i < m; ++i) {
+            HostFr b = points[p0 + i];
+            for (uint32_t l = 0; l < levels; ++l) {
+                const HostFr sc = b * k32;   // scaled form
+                std::memcpy(pts.b[i][l].v, sc.l, 32);
+                for (uint32_t q = 0; q < kOmegaBits; ++q) b = b.sqr();
+            }
+        }
         hipLaunchKernelGGL(k_omega_tables, dim3((unsigned)ceil_div(m * per, kBlock)), dim3(kBlock), 0, st,
                            tab + p0 * per, levels, pts, (uint32_t)m);
         MH_HIP(hipGetLastError());
