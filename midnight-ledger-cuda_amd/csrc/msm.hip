@@ -296,29 +296,6 @@ __global__ void __launch_bounds__(kPlaceThreads) k_place_coarse(const MsmDesc *d
     }
 }
 
-// per coarse bin: its pass-2 chunk count, and the bin's cursor = its first
-// bucket's start
-__global__ void k_bin_setup(const uint32_t *off, const uint32_t *cnt, uint64_t nkeys, uint32_t fshift, uint32_t nbins,
-                            uint32_t *ccursor, uint32_t *nch2)
-{
-    const uint32_t cb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cb >= nbins) return;
-    const uint64_t k0 = (uint64_t)cb << fshift, kl = min<uint64_t>(k0 + (1ull << fshift), nkeys) - 1;
-    const uint32_t bs = off[k0], be = off[kl] + cnt[kl];
-    ccursor[cb] = bs;
-    nch2[cb] = (be - bs + kFineChunk - 1) / kFineChunk;
-}
-
-// pass-2 block -> coarse bin map; t2[0] = pass-2 blocks
-__global__ void k_fine_map(const uint32_t *nch2, const uint32_t *cst2, uint32_t nbins, uint32_t *cmap2, uint32_t *t2)
-{
-    const uint32_t cb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cb >= nbins) return;
-    const uint32_t n = nch2[cb], o = cst2[cb];
-    for (uint32_t x = 0; x < n; ++x) cmap2[o + x] = cb;
-    if (cb == nbins - 1) t2[0] = o + n;
-}
-
 __global__ void __launch_bounds__(kPlaceThreads) k_place_fine(const uint64_t *tmp, const uint32_t *off,
                                                              const uint32_t *cnt, const uint32_t *cmap2,
                                                              const uint32_t *cst2, const uint32_t *t2,
@@ -485,6 +462,141 @@ __global__ void __launch_bounds__(kSmallScanThreads) k_scan_small(const uint32_t
         run += v;
     }
     if (threadIdx.x == kSmallScanThreads - 1 && total) *total = s[kSmallScanThreads - 1];
+}
+
+// Exclusive prefix of each thread's t over the block (kSmallScanThreads
+// threads); *total = the sum.  `s` is the block's LDS array, reusable after.
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t t, uint32_t *s, uint32_t *total)
+{
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t off = 1; off < kSmallScanThreads; off <<= 1) {
+        const uint32_t add = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    const uint32_t r = s[threadIdx.x] - t;
+    *total = s[kSmallScanThreads - 1];
+    __syncthreads();
+    return r;
+}
+
+// The planning steps of batches of at most kSmallScan bucket keys, each in
+// one workgroup (one launch where the multi-block form takes two to four);
+// thread t owns the keys [t per, (t + 1) per).
+__device__ __forceinline__ void small_run(uint64_t n, uint64_t &b0, uint64_t &b1)
+{
+    const uint64_t per = (n + kSmallScanThreads - 1) / kSmallScanThreads;
+    b0 = min<uint64_t>((uint64_t)threadIdx.x * per, n);
+    b1 = min<uint64_t>(b0 + per, n);
+}
+
+// bucket offsets: off = exclusive scan of cnt, cursor = off
+__global__ void __launch_bounds__(kSmallScanThreads) k_offsets_small(const uint32_t *cnt, uint32_t *off,
+                                                                    uint32_t *cursor, uint64_t nkeys)
+{
+    __shared__ uint32_t s[kSmallScanThreads];
+    uint64_t b0, b1;
+    small_run(nkeys, b0, b1);
+    uint32_t t = 0;
+    for (uint64_t b = b0; b < b1; ++b) t += cnt[b];
+    uint32_t total;
+    uint32_t run = block_exclusive(t, s, &total);
+    for (uint64_t b = b0; b < b1; ++b) {
+        off[b] = run;
+        cursor[b] = run;
+        run += cnt[b];
+    }
+}
+
+// chunks: q = ceil(cnt / L), co = exclusive scan of q, the chunk -> bucket
+// map, totals[0] entries, totals[1] chunks, totals[3] the fullest bucket's
+// chunk count
+__global__ void __launch_bounds__(kSmallScanThreads) k_chunk_plan_small(const uint32_t *cnt, const uint32_t *off,
+                                                                       uint32_t *q, uint32_t *co, uint32_t *cmap,
+                                                                       uint32_t *totals, uint64_t nkeys, uint32_t L)
+{
+    __shared__ uint32_t s[kSmallScanThreads];
+    __shared__ uint32_t smax;
+    if (threadIdx.x == 0) smax = 0;
+    uint64_t b0, b1;
+    small_run(nkeys, b0, b1);
+    uint32_t t = 0, mx = 0;
+    for (uint64_t b = b0; b < b1; ++b) {
+        const uint32_t v = (cnt[b] + L - 1) / L;
+        q[b] = v;
+        t += v;
+        mx = max(mx, v);
+    }
+    __syncthreads();
+    atomicMax(&smax, mx);
+    uint32_t total;
+    uint32_t run = block_exclusive(t, s, &total);
+    for (uint64_t b = b0; b < b1; ++b) {
+        const uint32_t v = q[b];
+        co[b] = run;
+        for (uint32_t j = 0; j < v; ++j) cmap[run + j] = (uint32_t)b;
+        run += v;
+    }
+    if (threadIdx.x == 0) {
+        totals[0] = off[nkeys - 1] + cnt[nkeys - 1];
+        totals[1] = total;
+        totals[3] = smax;
+    }
+}
+
+// one tree level's plan: groups of G partials of the buckets that hold more
+// than G at this level (inputs ceil(q / div)), their exclusive offsets lo,
+// the lane -> bucket map and totals[2] = lanes of the level
+__global__ void __launch_bounds__(kSmallScanThreads) k_level_plan_small(const uint32_t *q, uint64_t nkeys,
+                                                                       uint64_t div, uint32_t G, uint32_t *lo,
+                                                                       uint32_t *lmap, uint32_t *totals)
+{
+    __shared__ uint32_t s[kSmallScanThreads];
+    uint64_t b0, b1;
+    small_run(nkeys, b0, b1);
+    const auto groups = [&](uint64_t b) {
+        const uint32_t qprev = (uint32_t)(((uint64_t)q[b] + div - 1) / div);
+        return qprev > G ? (qprev + G - 1) / G : 0u;
+    };
+    uint32_t t = 0;
+    for (uint64_t b = b0; b < b1; ++b) t += groups(b);
+    uint32_t total;
+    uint32_t run = block_exclusive(t, s, &total);
+    for (uint64_t b = b0; b < b1; ++b) {
+        const uint32_t n = groups(b);
+        lo[b] = run;
+        for (uint32_t j = 0; j < n; ++j) lmap[run + j] = (uint32_t)b;
+        run += n;
+    }
+    if (threadIdx.x == 0) totals[2] = total;
+}
+
+// placement bins (at most kCoarseBins, one per thread): each bin's first
+// entry (its coarse cursor), its pass-2 blocks, their exclusive offsets cst2,
+// the pass-2 block -> bin map and t2[0] = pass-2 blocks
+__global__ void __launch_bounds__(kSmallScanThreads) k_bin_plan(const uint32_t *off, const uint32_t *cnt,
+                                                               uint64_t nkeys, uint32_t fshift, uint32_t nbins,
+                                                               uint32_t *ccursor, uint32_t *cst2, uint32_t *cmap2,
+                                                               uint32_t *t2)
+{
+    __shared__ uint32_t s[kSmallScanThreads];
+    const uint32_t cb = threadIdx.x;
+    uint32_t n = 0;
+    if (cb < nbins) {
+        const uint64_t k0 = (uint64_t)cb << fshift, kl = min<uint64_t>(k0 + (1ull << fshift), nkeys) - 1;
+        const uint32_t bs = off[k0], be = off[kl] + cnt[kl];
+        ccursor[cb] = bs;
+        n = (be - bs + kFineChunk - 1) / kFineChunk;
+    }
+    uint32_t total;
+    const uint32_t o = block_exclusive(n, s, &total);
+    if (cb < nbins) {
+        cst2[cb] = o;
+        for (uint32_t x = 0; x < n; ++x) cmap2[o + x] = cb;
+    }
+    if (cb == 0) t2[0] = total;
 }
 
 __global__ void __launch_bounds__(kBlock) k_scan_add(uint32_t *out, const uint32_t *sums, uint64_t n)
@@ -1097,7 +1209,7 @@ bool context_table(const MsmContext &ctx, size_t max_n, size_t max_end, PrecompT
 struct MsmEngine::Buffers {
     DevBuf<uint32_t> cnt, off, cursor, q, co, cmap, ent, scan_tmp, totals;
     DevBuf<uint64_t> tmp;                          // (key, entry) pairs between the placement passes
-    DevBuf<uint32_t> ccursor, nch2, cst2, cmap2, t2;   // coarse bins: cursors, pass-2 chunks, map, total
+    DevBuf<uint32_t> ccursor, cst2, cmap2, t2;   // coarse bins: cursors, pass-2 block offsets, map, total
     DevBuf<uint32_t> lc, lo, lmap;   // tree levels: groups per bucket, their offsets, lane -> bucket
     DevBuf<uint32_t> part, part2, bucket, pp, pp2, pj, lines;
     DevBuf<uint32_t> seg_off, seg_len, out_off;
@@ -1304,7 +1416,6 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
         s.fine_blocks = ceil_div(s.nent_max, kFineChunk) + s.nbins;
         b.tmp.ensure(std::max<uint64_t>(s.nent_max, 1));
         b.ccursor.ensure(s.nbins);
-        b.nch2.ensure(s.nbins);
         b.cst2.ensure(s.nbins);
         b.cmap2.ensure(s.fine_blocks);
         b.t2.ensure(1);
@@ -1478,17 +1589,21 @@ void MsmEngine::enqueue_front(hipStream_t st, ProfileScope *prof)
                        b.cnt.get(), nkeys, b.totals.get());
     MH_HIP(hipGetLastError());
     if (s.max_n) launch_digits(false, g.c, s.max_n, st, b.descs.get(), g, b.cnt.get(), nullptr);
-    scan_launch(b.cnt.get(), b.off.get(), nkeys, b.scan_tmp.get(), st);
-    hipLaunchKernelGGL(k_copy_u32, dim3((uint32_t)std::min<uint64_t>(grid1(nkeys), 4096)), dim3(kBlock), 0, st,
-                       b.cursor.get(), (const uint32_t *)b.off.get(), nkeys);
-    MH_HIP(hipGetLastError());
-    if (s.max_n && s.two_pass) {
-        hipLaunchKernelGGL(k_bin_setup, dim3(grid1(s.nbins)), dim3(kBlock), 0, st, (const uint32_t *)b.off.get(),
-                           (const uint32_t *)b.cnt.get(), nkeys, s.fshift, s.nbins, b.ccursor.get(), b.nch2.get());
+    const bool small = nkeys <= kSmallScan;
+    if (small) {
+        hipLaunchKernelGGL(k_offsets_small, dim3(1), dim3(kSmallScanThreads), 0, st, (const uint32_t *)b.cnt.get(),
+                           b.off.get(), b.cursor.get(), nkeys);
         MH_HIP(hipGetLastError());
-        scan_launch(b.nch2.get(), b.cst2.get(), s.nbins, b.scan_tmp.get(), st);
-        hipLaunchKernelGGL(k_fine_map, dim3(grid1(s.nbins)), dim3(kBlock), 0, st, (const uint32_t *)b.nch2.get(),
-                           (const uint32_t *)b.cst2.get(), s.nbins, b.cmap2.get(), b.t2.get());
+    } else {
+        scan_launch(b.cnt.get(), b.off.get(), nkeys, b.scan_tmp.get(), st);
+        hipLaunchKernelGGL(k_copy_u32, dim3((uint32_t)std::min<uint64_t>(grid1(nkeys), 4096)), dim3(kBlock), 0, st,
+                           b.cursor.get(), (const uint32_t *)b.off.get(), nkeys);
+        MH_HIP(hipGetLastError());
+    }
+    if (s.max_n && s.two_pass) {
+        hipLaunchKernelGGL(k_bin_plan, dim3(1), dim3(kSmallScanThreads), 0, st, (const uint32_t *)b.off.get(),
+                           (const uint32_t *)b.cnt.get(), nkeys, s.fshift, s.nbins, b.ccursor.get(), b.cst2.get(),
+                           b.cmap2.get(), b.t2.get());
         MH_HIP(hipGetLastError());
         launch_place_coarse(g.c, s.max_n, st, b.descs.get(), g, s.fshift, s.nbins, b.ccursor.get(), b.tmp.get());
         hipLaunchKernelGGL(k_place_fine, dim3((uint32_t)s.fine_blocks), dim3(kPlaceThreads), 0, st,
@@ -1502,16 +1617,23 @@ void MsmEngine::enqueue_front(hipStream_t st, ProfileScope *prof)
     }
 
     // 4-6: chunked accumulation into buckets
-    hipLaunchKernelGGL(k_chunk_counts, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.cnt.get(), b.q.get(), nkeys, s.L,
-                       b.totals.get());
-    MH_HIP(hipGetLastError());
-    scan_launch(b.q.get(), b.co.get(), nkeys, b.scan_tmp.get(), st);
-    hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, st, b.cnt.get(), b.off.get(), b.q.get(), b.co.get(), nkeys,
-                       b.totals.get());
-    MH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_chunk_map, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), b.cmap.get(),
-                       nkeys);
-    MH_HIP(hipGetLastError());
+    if (small) {
+        hipLaunchKernelGGL(k_chunk_plan_small, dim3(1), dim3(kSmallScanThreads), 0, st, (const uint32_t *)b.cnt.get(),
+                           (const uint32_t *)b.off.get(), b.q.get(), b.co.get(), b.cmap.get(), b.totals.get(), nkeys,
+                           s.L);
+        MH_HIP(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(k_chunk_counts, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.cnt.get(), b.q.get(), nkeys,
+                           s.L, b.totals.get());
+        MH_HIP(hipGetLastError());
+        scan_launch(b.q.get(), b.co.get(), nkeys, b.scan_tmp.get(), st);
+        hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, st, b.cnt.get(), b.off.get(), b.q.get(), b.co.get(),
+                           nkeys, b.totals.get());
+        MH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_chunk_map, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), b.co.get(), b.cmap.get(),
+                           nkeys);
+        MH_HIP(hipGetLastError());
+    }
     if (prof) prof->begin("msm_accumulate", st);
     if (s.precomputed) {
         if (s.nch_max < quad_below)
@@ -1547,13 +1669,19 @@ void MsmEngine::enqueue_back(hipStream_t st, uint32_t levels, ProfileScope *prof
     uint32_t *tin = b.part.get(), *tout = b.part2.get();
     uint64_t div = 1;
     for (uint32_t l = 0; l < levels; ++l) {
-        hipLaunchKernelGGL(k_level_counts, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), nkeys, div, s.G,
-                           b.lc.get());
-        MH_HIP(hipGetLastError());
-        scan_launch(b.lc.get(), b.lo.get(), nkeys, b.scan_tmp.get(), st);
-        hipLaunchKernelGGL(k_level_map, dim3(grid1(nkeys)), dim3(kBlock), 0, st, (const uint32_t *)b.lc.get(),
-                           (const uint32_t *)b.lo.get(), nkeys, b.lmap.get(), b.totals.get());
-        MH_HIP(hipGetLastError());
+        if (nkeys <= kSmallScan) {
+            hipLaunchKernelGGL(k_level_plan_small, dim3(1), dim3(kSmallScanThreads), 0, st, (const uint32_t *)b.q.get(),
+                               nkeys, div, s.G, b.lo.get(), b.lmap.get(), b.totals.get());
+            MH_HIP(hipGetLastError());
+        } else {
+            hipLaunchKernelGGL(k_level_counts, dim3(grid1(nkeys)), dim3(kBlock), 0, st, b.q.get(), nkeys, div, s.G,
+                               b.lc.get());
+            MH_HIP(hipGetLastError());
+            scan_launch(b.lc.get(), b.lo.get(), nkeys, b.scan_tmp.get(), st);
+            hipLaunchKernelGGL(k_level_map, dim3(grid1(nkeys)), dim3(kBlock), 0, st, (const uint32_t *)b.lc.get(),
+                               (const uint32_t *)b.lo.get(), nkeys, b.lmap.get(), b.totals.get());
+            MH_HIP(hipGetLastError());
+        }
         const uint64_t lanes = s.nch_max / (div * s.G) + nkeys;   // bound on the level's groups
         const int w = width(lanes);
         auto *tl = w == 16 ? k_tree_level<16> : w == 4 ? k_tree_level<4> : k_tree_level<1>;
