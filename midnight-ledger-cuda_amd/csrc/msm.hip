@@ -484,12 +484,14 @@ __device__ __forceinline__ uint32_t block_exclusive(uint32_t t, uint32_t *s, uin
 
 // The planning steps of batches of at most kSmallScan bucket keys, each in
 // one workgroup (one launch where the multi-block form takes two to four);
-// thread t owns the keys [t per, (t + 1) per).
-__device__ __forceinline__ void small_run(uint64_t n, uint64_t &b0, uint64_t &b1)
+// thread t owns the keys [t per, (t + 1) per), at most kSmallPer, whose
+// values it keeps in registers (the loads of a run are issued together).
+constexpr uint32_t kSmallPer = kSmallScan / kSmallScanThreads;
+__device__ __forceinline__ void small_run(uint64_t n, uint64_t &b0, uint32_t &len)
 {
     const uint64_t per = (n + kSmallScanThreads - 1) / kSmallScanThreads;
     b0 = min<uint64_t>((uint64_t)threadIdx.x * per, n);
-    b1 = min<uint64_t>(b0 + per, n);
+    len = (uint32_t)(min<uint64_t>(b0 + per, n) - b0);
 }
 
 // bucket offsets: off = exclusive scan of cnt, cursor = off
@@ -497,16 +499,24 @@ __global__ void __launch_bounds__(kSmallScanThreads) k_offsets_small(const uint3
                                                                     uint32_t *cursor, uint64_t nkeys)
 {
     __shared__ uint32_t s[kSmallScanThreads];
-    uint64_t b0, b1;
-    small_run(nkeys, b0, b1);
-    uint32_t t = 0;
-    for (uint64_t b = b0; b < b1; ++b) t += cnt[b];
+    uint64_t b0;
+    uint32_t len;
+    small_run(nkeys, b0, len);
+    uint32_t v[kSmallPer], t = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kSmallPer; ++i) {
+        v[i] = i < len ? cnt[b0 + i] : 0u;
+        t += v[i];
+    }
     uint32_t total;
     uint32_t run = block_exclusive(t, s, &total);
-    for (uint64_t b = b0; b < b1; ++b) {
-        off[b] = run;
-        cursor[b] = run;
-        run += cnt[b];
+#pragma unroll
+    for (uint32_t i = 0; i < kSmallPer; ++i) {
+        if (i < len) {
+            off[b0 + i] = run;
+            cursor[b0 + i] = run;
+        }
+        run += v[i];
     }
 }
 
@@ -520,24 +530,30 @@ __global__ void __launch_bounds__(kSmallScanThreads) k_chunk_plan_small(const ui
     __shared__ uint32_t s[kSmallScanThreads];
     __shared__ uint32_t smax;
     if (threadIdx.x == 0) smax = 0;
-    uint64_t b0, b1;
-    small_run(nkeys, b0, b1);
-    uint32_t t = 0, mx = 0;
-    for (uint64_t b = b0; b < b1; ++b) {
-        const uint32_t v = (cnt[b] + L - 1) / L;
-        q[b] = v;
-        t += v;
-        mx = max(mx, v);
+    uint64_t b0;
+    uint32_t len;
+    small_run(nkeys, b0, len);
+    uint32_t v[kSmallPer], t = 0, mx = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kSmallPer; ++i) v[i] = i < len ? cnt[b0 + i] : 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < kSmallPer; ++i) {
+        v[i] = (v[i] + L - 1) / L;
+        if (i < len) q[b0 + i] = v[i];
+        t += v[i];
+        mx = max(mx, v[i]);
     }
     __syncthreads();
     atomicMax(&smax, mx);
     uint32_t total;
     uint32_t run = block_exclusive(t, s, &total);
-    for (uint64_t b = b0; b < b1; ++b) {
-        const uint32_t v = q[b];
-        co[b] = run;
-        for (uint32_t j = 0; j < v; ++j) cmap[run + j] = (uint32_t)b;
-        run += v;
+#pragma unroll
+    for (uint32_t i = 0; i < kSmallPer; ++i) {
+        if (i < len) {
+            co[b0 + i] = run;
+            for (uint32_t j = 0; j < v[i]; ++j) cmap[run + j] = (uint32_t)(b0 + i);
+        }
+        run += v[i];
     }
     if (threadIdx.x == 0) {
         totals[0] = off[nkeys - 1] + cnt[nkeys - 1];
@@ -554,21 +570,31 @@ __global__ void __launch_bounds__(kSmallScanThreads) k_level_plan_small(const ui
                                                                        uint32_t *lmap, uint32_t *totals)
 {
     __shared__ uint32_t s[kSmallScanThreads];
-    uint64_t b0, b1;
-    small_run(nkeys, b0, b1);
-    const auto groups = [&](uint64_t b) {
-        const uint32_t qprev = (uint32_t)(((uint64_t)q[b] + div - 1) / div);
-        return qprev > G ? (qprev + G - 1) / G : 0u;
-    };
-    uint32_t t = 0;
-    for (uint64_t b = b0; b < b1; ++b) t += groups(b);
+    // no bucket holds more than G partials at this level (the fullest
+    // bucket's chunk count is totals[3]): no groups, nothing else to write
+    if ((totals[3] + div - 1) / div <= G) {
+        if (threadIdx.x == 0) totals[2] = 0;
+        return;
+    }
+    uint64_t b0;
+    uint32_t len;
+    small_run(nkeys, b0, len);
+    uint32_t v[kSmallPer], t = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kSmallPer; ++i) {
+        const uint32_t qprev = i < len ? (uint32_t)(((uint64_t)q[b0 + i] + div - 1) / div) : 0u;
+        v[i] = qprev > G ? (qprev + G - 1) / G : 0u;
+        t += v[i];
+    }
     uint32_t total;
     uint32_t run = block_exclusive(t, s, &total);
-    for (uint64_t b = b0; b < b1; ++b) {
-        const uint32_t n = groups(b);
-        lo[b] = run;
-        for (uint32_t j = 0; j < n; ++j) lmap[run + j] = (uint32_t)b;
-        run += n;
+#pragma unroll
+    for (uint32_t i = 0; i < kSmallPer; ++i) {
+        if (i < len) {
+            lo[b0 + i] = run;
+            for (uint32_t j = 0; j < v[i]; ++j) lmap[run + j] = (uint32_t)(b0 + i);
+        }
+        run += v[i];
     }
     if (threadIdx.x == 0) totals[2] = total;
 }

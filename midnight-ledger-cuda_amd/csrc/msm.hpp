@@ -86,10 +86,11 @@ public:
     // a third of the latency per addition for a third more issue, a gain
     // while the pass leaves SIMDs without a wave (0 = never).
     uint64_t quad_below = 3 * 16384;
-    // Reduction passes of fewer sums than this give each sum four quads
+    // Reduction passes of fewer sums than this (a batch of five MSMs over
+    // a 4096-bucket table included) give each sum four quads
     // (each adds every fourth partial, then the quads' sums are added
     // pairwise): about a quarter of a quad's chain for a third more issue.
-    uint64_t group16_below = 12288;
+    uint64_t group16_below = 24576;
 
     // statistics of the last run (for throughput / roofline reporting)
     uint64_t last_entries = 0;    // nonzero signed digits = bucket additions
