@@ -70,6 +70,23 @@ __global__ void k_blind_pieces(Fr *pieces, const Fr *blinds, size_t poly_len, si
     store_fr(next, load_fr(next) - b);
 }
 
+// k_quotient_pieces and k_blind_pieces in one pass, the blinds by value
+struct PieceBlinds {
+    Fr b[kMaxArgBlinds];
+};
+__global__ void k_quotient_pieces_blinded(Fr *out, const Fr *q, size_t poly_len, size_t pieces, PieceBlinds bl)
+{
+    const size_t total = poly_len * pieces;
+    for (size_t f = blockIdx.x * (size_t)blockDim.x + threadIdx.x; f < total; f += (size_t)gridDim.x * blockDim.x) {
+        const size_t piece = f / poly_len, idx = f % poly_len;
+        Fr v;
+        if (idx == poly_len - 1) v = piece + 1 < pieces ? bl.b[piece] : Fr::zero();
+        else v = load_fr(q + piece * (poly_len - 1) + idx);
+        if (idx == 0 && piece > 0) v = v - bl.b[piece - 1];
+        store_fr(out + f, v);
+    }
+}
+
 __global__ void k_fill(Fr *v, Fr x, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -153,6 +170,18 @@ void fr_quotient_pieces(Fr *out, const Fr *quotient, size_t poly_len, size_t pie
     MH_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_blind_pieces, dim3((unsigned)ceil_div(pieces, kBlock)), dim3(kBlock), 0, st, out, blinds,
                        poly_len, pieces);
+    MH_HIP(hipGetLastError());
+}
+
+void fr_quotient_pieces(Fr *out, const Fr *quotient, size_t poly_len, size_t pieces, const Fr *host_blinds,
+                        size_t nblinds, hipStream_t st)
+{
+    MH_CHECK(nblinds + 1 == pieces && nblinds <= kMaxArgBlinds, (int)hipErrorInvalidValue,
+             "quotient piece blinds do not match the piece count");
+    PieceBlinds bl{};
+    for (size_t i = 0; i < nblinds; ++i) bl.b[i] = host_blinds[i];
+    hipLaunchKernelGGL(k_quotient_pieces_blinded, grid_for(poly_len * pieces), dim3(kBlock), 0, st, out, quotient,
+                       poly_len, pieces, bl);
     MH_HIP(hipGetLastError());
 }
 

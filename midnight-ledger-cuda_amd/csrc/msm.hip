@@ -1021,11 +1021,14 @@ __global__ void __launch_bounds__(kBlock) k_line_partials(const uint32_t *bucket
 }
 
 // Sum groups of G consecutive partials within segments of `seg` partials;
-// output segment length ceil(seg / G).  Results as Jacobian when `to_jac`.
+// output segment length ceil(seg / G).  Results as Jacobian when `to_jac`;
+// with `totals`, the batch totals (4 words) are copied after the results, so
+// that one read-back brings both.
 template <int kLanes>
 __global__ void __launch_bounds__(kBlock) k_group_sums(const uint32_t *in, uint64_t nseg, const uint32_t *seg_off,
                                                        const uint32_t *seg_len, const uint32_t *out_off, uint32_t G,
-                                                       uint64_t total_out, uint32_t *out, int to_jac)
+                                                       uint64_t total_out, uint32_t *out, int to_jac,
+                                                       const uint32_t *totals)
 {
     const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t id = t / kLanes;
@@ -1045,6 +1048,8 @@ __global__ void __launch_bounds__(kBlock) k_group_sums(const uint32_t *in, uint6
     });
     if (sub) return;
     // the last pass converts (fully reduced) to the 32-bit Jacobian form
+    if (totals && id == 0)
+        for (int i = 0; i < 4; ++i) out[total_out * 36 + i] = totals[i];
     if (to_jac) store_jacobian(out + id * 36, xyzz_to_jacobian(xyzz28_to_std(acc)));
     else store_xyzz28(out + id * kXyzz28Words, acc);
 }
@@ -1569,7 +1574,7 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
     b.seg_off.ensure(s.seg_off.size());
     b.seg_len.ensure(s.seg_len.size());
     b.out_off.ensure(s.out_off.size());
-    b.pj.ensure(s.nseg * 36);
+    b.pj.ensure(s.nseg * 36 + 4);
 
     // uploads of the descriptors and the plane-pass layout
     s.hd.resize(jobs.size());
@@ -1594,7 +1599,7 @@ void MsmEngine::prepare(const uint32_t *d_points, size_t npoints_total, const st
         if (b.h_pj) MH_HIP(hipHostFree(b.h_pj));
         b.h_pj = nullptr;
         b.h_pj_cap = 0;
-        MH_HIP(hipHostMalloc(&b.h_pj, s.nseg * 144));
+        MH_HIP(hipHostMalloc(&b.h_pj, s.nseg * 144 + 16));   // + the batch totals
         b.h_pj_cap = s.nseg;
     }
     if (!b.h_totals) MH_HIP(hipHostMalloc(reinterpret_cast<void **>(&b.h_totals), 16));
@@ -1736,7 +1741,8 @@ void MsmEngine::enqueue_back(hipStream_t st, uint32_t levels, ProfileScope *prof
             auto *gs = w == 16 ? k_group_sums<16> : w == 4 ? k_group_sums<4> : k_group_sums<1>;
             hipLaunchKernelGGL(gs, dim3(grid1(w * lv.total_out)), dim3(kBlock), 0, st, lin, nseg,
                                b.seg_off.get() + lv.base, b.seg_len.get() + lv.base, b.out_off.get() + lv.base, lv.G,
-                               lv.total_out, last ? final : lout, last ? to_jac : 0);
+                               lv.total_out, last ? final : lout, last ? to_jac : 0,
+                               last && to_jac ? (const uint32_t *)b.totals.get() : nullptr);
             MH_HIP(hipGetLastError());
             std::swap(lin, lout);
         }
@@ -1762,8 +1768,8 @@ void MsmEngine::enqueue_back(hipStream_t st, uint32_t levels, ProfileScope *prof
         MH_HIP(hipGetLastError());
     }
     group_passes(s.levels, s.nseg, b.pj.get(), 1);
-    MH_HIP(hipMemcpyAsync(b.h_pj, b.pj.get(), s.nseg * 144, hipMemcpyDeviceToHost, st));
-    MH_HIP(hipMemcpyAsync(b.h_totals, b.totals.get(), 16, hipMemcpyDeviceToHost, st));
+    // the (window, plane) sums and, after them, the batch totals
+    MH_HIP(hipMemcpyAsync(b.h_pj, b.pj.get(), s.nseg * 144 + 16, hipMemcpyDeviceToHost, st));
     if (prof) prof->end(st);
 }
 
@@ -1779,7 +1785,7 @@ void MsmEngine::finish(RawG1Projective *out)
     const State &s = *st_;
     MH_CHECK(s.ready, -1, "MSM engine used before prepare");
     const Geometry &g = s.g;
-    last_entries = b.h_totals[0];
+    last_entries = static_cast<const uint32_t *>(b.h_pj)[s.nseg * 36];
     const HJac *hj = static_cast<const HJac *>(b.h_pj);
     // 8: host: one doubling chain per MSM over its (window, plane) sums,
     // exponent e = c*w + k

@@ -612,9 +612,10 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     adv_lag.values.alloc(A * n);
     Fr *al = adv_lag.values.get();
     MH_HIP(hipMemcpyAsync(al, advice, A * n * 32, hipMemcpyHostToDevice, st));
-    for (size_t c = 0; c < A && brows; ++c)
-        MH_HIP(hipMemcpyAsync(al + c * n + pk.u, adv_blinds.data() + c * brows, brows * 32, hipMemcpyHostToDevice,
-                              st));
+    // the blinded rows u..n of every column in one strided copy
+    if (brows)
+        MH_HIP(hipMemcpy2DAsync(al + pk.u, n * 32, adv_blinds.data(), brows * 32, brows * 32, A,
+                                hipMemcpyHostToDevice, st));
     MH_HIP(hipMemcpyAsync(pd, al, A * n * 32, hipMemcpyDeviceToDevice, st));
     commit_stage(lp ? &lp->advice : nullptr, polys, P_ADV, A, true);
     for (auto &c : com) T.write_point(c);
@@ -658,9 +659,15 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     check(midnight_sppark_bls12_381_fr_device_vec_divide_ntt_zeta(
         h.get(), pk.ext_k, reinterpret_cast<const RawFr *>(pk.t_inv_scaled.data()), pk.t_inv_scaled.size(),
         raw(HostFr::one()), raw(kZeta.sqr()), raw(kZeta)));
-    DevBuf<Fr> lb(limb_blinds.size());
-    MH_HIP(hipMemcpyAsync(lb.get(), limb_blinds.data(), limb_blinds.size() * 32, hipMemcpyHostToDevice, st));
-    fr_quotient_pieces(pd + P_H * n, h->values.get(), n, pk.pieces, lb.get(), st);
+    if (limb_blinds.size() <= kMaxArgBlinds) {
+        fr_quotient_pieces(pd + P_H * n, h->values.get(), n, pk.pieces,
+                           reinterpret_cast<const Fr *>(limb_blinds.data()), limb_blinds.size(), st);
+    } else {
+        DevBuf<Fr> lb(limb_blinds.size());
+        MH_HIP(hipMemcpyAsync(lb.get(), limb_blinds.data(), limb_blinds.size() * 32, hipMemcpyHostToDevice, st));
+        fr_quotient_pieces(pd + P_H * n, h->values.get(), n, pk.pieces, lb.get(), st);
+        MH_HIP(hipStreamSynchronize(st));   // lb goes out of scope
+    }
     MH_HIP(hipStreamSynchronize(st));
     h.reset();
     commit_stage(lp ? &lp->pieces : nullptr, polys, P_H, pk.pieces, false);
