@@ -134,6 +134,12 @@ RustError midnight_sppark_bls12_381_g1_msm_context_device_scrambled_batch(
  * returns before it has run. */
 void midnight_sppark_test_mask_delay(uint32_t delay_ms);
 RustError midnight_sppark_test_delayed_scale(void *raw_device, RawFr scale, uint32_t delay_ms);
+/* Test hook (no counterpart in the reference): caps the bucket-tree levels
+ * of the calling thread's MSMs that read their fullest bucket back (the
+ * context and one-shot entry points); the last reduction pass then sums
+ * all the partials deeper buckets still hold, which tests that pass on
+ * buckets of any depth.  0xffffffff lifts the cap. */
+void midnight_hip_test_msm_tree_levels(uint32_t max_levels);
 /* replaces bls12_381_g1_msm.cu:6488.  In-place natural-order NTT of 2^log_n
  * values; forward uses omega = ROOT_OF_UNITY^(2^(32-log_n)), inverse uses
  * omega^-1 and includes the 1/n factor. */

@@ -1812,6 +1812,12 @@ void MsmEngine::finish(RawG1Projective *out)
     }
 }
 
+namespace {
+// test hook: a cap on the bucket-tree levels of the calling thread's eager
+// MSMs (the last pass then sums deeper buckets' partials itself)
+thread_local uint32_t t_tree_level_cap = ~0u;
+}  // namespace
+
 void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::vector<MsmJob> &jobs,
                     uint32_t window_bits, RawG1Projective *out, hipStream_t st, const PrecompTable *pc)
 {
@@ -1828,7 +1834,7 @@ void MsmEngine::run(const uint32_t *d_points, size_t npoints_total, const std::v
     const State &s = *st_;
     uint32_t levels = 0;
     for (uint32_t rem = b.h_totals[3]; rem > s.G; rem = (rem + s.G - 1) / s.G) ++levels;
-    enqueue_back(st, levels, pr);
+    enqueue_back(st, std::min(levels, t_tree_level_cap), pr);
     MH_HIP(hipStreamSynchronize(st));
     prof.flush();
     finish(out);
@@ -2106,6 +2112,8 @@ void delay_stream(hipStream_t st, uint32_t ms)
 }  // namespace
 
 extern "C" void midnight_sppark_test_mask_delay(uint32_t delay_ms) { t_mask_delay_ms = delay_ms; }
+
+extern "C" void midnight_hip_test_msm_tree_levels(uint32_t max_levels) { t_tree_level_cap = max_levels; }
 
 // scale a device vector in place behind a delay of delay_ms on the lane's
 // stream; returns at once (the scaling is still pending)

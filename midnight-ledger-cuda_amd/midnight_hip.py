@@ -221,6 +221,7 @@ def _declare(L):
         "midnight_hip_proof_server_wait": (E, [vp, ctypes.c_uint64, ctypes.c_char_p, c_size, ctypes.POINTER(c_size),
                                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "midnight_hip_proof_server_free": (None, [vp]),
+        "midnight_hip_test_msm_tree_levels": (None, [u32]),
         "midnight_hip_profile_enable": (None, [ctypes.c_int]),
         "midnight_hip_profile_read": (
             ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]),

@@ -99,6 +99,33 @@ def test_msm_srs_fixture_identity(H, O):
     assert O.g1_eq(a, O.msm(g, c))
 
 
+def test_msm_bucket_tree_capped_levels(H, O):
+    """Buckets deeper than the launched tree levels: every scalar of a batch
+    equal (each window's digit bucket holds every point), run with the tree
+    capped at 0 and 1 levels and uncapped; the last reduction pass sums what
+    the levels leave, and every result is the closed form."""
+    n, m = 4096, 3
+    ts = O.random_fr(n, 1601)
+    dv_t = H.DeviceVec.upload(ts)
+    ctx = H.MsmContext.from_multiples(dv_t)
+    dv_t.free()
+    sc = O.random_fr(n * m, 1602)
+    sc[:n] = sc[0]
+    sc[2 * n:] = sc[2 * n]
+    dv = H.DeviceVec.upload(sc)
+    want = [closed_form(O, ts, sc[i * n:(i + 1) * n]) for i in range(m)]
+    try:
+        for cap in (0, 1, 0xffffffff):
+            H.lib().midnight_hip_test_msm_tree_levels(cap)
+            got = ctx.device_batch(dv, 0, n, m)
+            for i in range(m):
+                assert O.g1_eq(got[i], want[i]), (cap, i)
+    finally:
+        H.lib().midnight_hip_test_msm_tree_levels(0xffffffff)
+    dv.free()
+    ctx.free()
+
+
 def test_msm_context_batch_variants(H, O):
     n, m = 3000, 4
     ts = O.random_fr(n, 41)
