@@ -313,6 +313,28 @@ __device__ __forceinline__ Fr29 fr29_lazy_sub4r(const Fr29 &a, const Fr29 &b)
     return r;
 }
 
+// 8r with 2^30 lent to every limb below the top (each of those limbs is at
+// least 2^30 - 2, the top one at least twice the top limb of any value
+// below 2r), so that a - b + 8r can be formed limb by limb for b the lazy
+// sum of two normalized values below 2r
+__device__ __forceinline__ uint32_t r8x29_lent(int i)
+{
+    constexpr uint32_t O[9] = {0x40000008u, 0x5fffffbeu, 0x5cb7fdfdu, 0x5a402ffdu, 0x4c02a9dcu,
+                               0x40202685u, 0x49066739u, 0x53299d7bu, 0x039f6d38u};
+    return O[i];
+}
+
+// a - b + 8r for b = x + y, x and y normalized and below 2r (limbs below
+// 2^30, value below 4r), no carries; a's limbs below 2^30 keep the result's
+// below 2^32
+__device__ __forceinline__ Fr29 fr29_lazy_sub8r(const Fr29 &a, const Fr29 &b)
+{
+    Fr29 r;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) r.v[j] = a.v[j] + r8x29_lent(j) - b.v[j];
+    return r;
+}
+
 // carry propagation: limbs below 2^32, value below 2^261 -> limbs below 2^29
 __device__ __forceinline__ void fr29_norm(Fr29 &a)
 {
@@ -326,10 +348,12 @@ __device__ __forceinline__ void fr29_norm(Fr29 &a)
     a.v[8] += c;
 }
 
-// normalized x below 2^261 -> the same class below 3r: x - q r with
-// q = floor(x8 M / 2^48), M = floor(2^48 / (r8 + 1)), r8 the top limb of r,
-// so floor(x / r) - 2 <= q <= floor(x / r)
-__device__ __forceinline__ void fr29_reduce_3r(Fr29 &a)
+// normalized x below 2^261 -> the same class below 2r: x - q r with
+// q = floor(x8 M / 2^48), M = floor(2^48 / (r8 + 1)), r8 the top limb of r.
+// x8 M / 2^48 is below x / r and short of it by at most (x / r) / r8 plus
+// 2^-23 (the two floors), under 10^-5 for x / r < 70.6, so
+// floor(x / r) - 1 <= q <= floor(x / r)
+__device__ __forceinline__ void fr29_reduce_top(Fr29 &a)
 {
     const uint32_t q = (uint32_t)(((uint64_t)a.v[8] * 0x2355091ull) >> 48);
     int64_t carry = 0;

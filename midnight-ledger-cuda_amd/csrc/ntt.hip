@@ -34,12 +34,15 @@
 // twiddled operand comes out of the product below 2r, the sum and the
 // difference (plus 4r) are formed limb by limb with no carries and no
 // conditional subtraction, and the carries are propagated once per stage
-// pair before the LDS store.  Every element's bound grows by at most 4r per
-// stage: below 2r at the load, below 46r after 11 stages, inside the 2^261
-// (70.6 r) the limbs hold and below what keeps every product's accumulators
-// under 2^64.  The first stage of a pass has only unit twiddles and skips
-// its products; every other stage multiplies, also by w^0 (in a stage's
-// first butterfly), which keeps the bound additive.  The store reduces once
+// pair before the LDS store.  The first stage of a pass has only unit
+// twiddles and skips its products, and so does the second stage's w_4^0
+// half (its operand, below 4r, enters unmultiplied and is subtracted with
+// 8r added): elements are below 2r at the load and below 12r after the
+// first pair.  Every later
+// stage multiplies, also by w^0 (in a stage's first butterfly), which keeps
+// the bound growing by at most 4r per stage: below 48r after 11 stages,
+// inside the 2^261 (70.6 r) the limbs hold and below what keeps every
+// product's accumulators under 2^64.  The store reduces once
 // to the canonical value and repacks to 8 x 32 bits, so results are limb
 // for limb those of the reference.
 #include <algorithm>
@@ -293,22 +296,45 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
     // stage s on (e, e+h) and (e+2h, e+3h) -- one twiddle w_2h^low for both
     // -- and stage s+1 on (e, e+2h) and (e+h, e+3h) -- twiddles w_4h^low and
     // w_4h^(low+h) -- with one LDS round trip and one barrier for the pair.
-    // An odd last stage is a radix-2 one.
+    // The first pair (h = 1, low = 0) has only the product by w_4; an odd
+    // last stage is a radix-2 one.
+    const uint32_t lq = log_m - 2;
     uint32_t s = 0;
+    if (log_m >= 2) {
+        for (uint32_t g = threadIdx.x; g < (C << lq); g += blockDim.x) {
+            const uint32_t c = g >> lq, gg = g & ((1u << lq) - 1);
+            const uint32_t e = (c << log_m) + (gg << 2);
+            const Fr29 x0 = lds.get(e), t1 = lds.get(e + 1);
+            const Fr29 x2 = lds.get(e + 2), t3 = lds.get(e + 3);
+            const Fr29 y0 = fr29_lazy_add(x0, t1), y1 = fr29_lazy_sub4r(x0, t1);
+            const Fr29 y2 = fr29_lazy_add(x2, t3), y3 = fr29_lazy_sub4r(x2, t3);
+            const Fr29 u3 = fr29_mul_loose(y3, a.roots.get(1u << (kMaxLogLds - 2)));
+            // w_4^0 = 1: y2 (below 4r, limbs below 2^30) enters as it is
+            Fr29 z0 = fr29_lazy_add(y0, y2), z2 = fr29_lazy_sub8r(y0, y2);
+            Fr29 z1 = fr29_lazy_add(y1, u3), z3 = fr29_lazy_sub4r(y1, u3);
+            fr29_norm(z0);
+            fr29_norm(z1);
+            fr29_norm(z2);
+            fr29_norm(z3);
+            lds.put(e, z0);
+            lds.put(e + 2, z2);
+            lds.put(e + 1, z1);
+            lds.put(e + 3, z3);
+        }
+        __syncthreads();
+        s = 2;
+    }
     for (; s + 1 < log_m; s += 2) {
         const uint32_t h = 1u << s;
-        const uint32_t lq = log_m - 2;
         for (uint32_t g = threadIdx.x; g < (C << lq); g += blockDim.x) {
             const uint32_t c = g >> lq, gg = g & ((1u << lq) - 1);
             const uint32_t low = gg & (h - 1);
             const uint32_t e = (c << log_m) + (((gg >> s) << (s + 2)) | low);
             const Fr29 x0 = lds.get(e), x2 = lds.get(e + 2 * h);
             Fr29 t1 = lds.get(e + h), t3 = lds.get(e + 3 * h);
-            if (s) {
-                const Fr29 w = a.roots.get(low << (kMaxLogLds - 1 - s));
-                t1 = fr29_mul_loose(t1, w);
-                t3 = fr29_mul_loose(t3, w);
-            }
+            const Fr29 w = a.roots.get(low << (kMaxLogLds - 1 - s));
+            t1 = fr29_mul_loose(t1, w);
+            t3 = fr29_mul_loose(t3, w);
             const Fr29 y0 = fr29_lazy_add(x0, t1), y1 = fr29_lazy_sub4r(x0, t1);
             const Fr29 y2 = fr29_lazy_add(x2, t3), y3 = fr29_lazy_sub4r(x2, t3);
             const Fr29 u2 = fr29_mul_loose(y2, a.roots.get(low << (kMaxLogLds - 2 - s)));
@@ -346,7 +372,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
     }
 
     // store, applying the mid twiddle or the post-operation, reducing to
-    // the canonical value (elements are below 46r here)
+    // the canonical value (elements are below 48r here)
     const bool store_cfast = (a.out_cs == 1) && C > 1;
     const uint32_t tw_shift = kTwLog - a.log_n;
     for (uint32_t f = threadIdx.x; f < total; f += blockDim.x) {
@@ -356,8 +382,9 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
         Fr29 v = lds.get((c << log_m) + k);
         const uint64_t L = (uint64_t)k * a.out_es + (uint64_t)(c0 + c) * a.out_cs;
         if (a.twiddle) {
-            // w_n^(c k) from the transform's table, or formed from the two
-            // small tables (w^0 = 1 included); the product is below 2.4r
+            // w_n^(c k) from the transform's table (below r), or formed from
+            // the two small tables (w^0 = 1 included, below 1.06r); the
+            // product is below 48r 1.06r / 2^261 + r < 1.8r
             Fr29 w;
             if (a.has_mid) {
                 w = a.mid.get(((c0 + c) << log_m) + k);
@@ -367,16 +394,11 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
                 if (e >> 11) w = fr29_mul_loose(w, a.hi.get(e >> 11));
             }
             v = fr29_mul_loose(v, w);
-            fr29_reduce_2r(v);
             fr29_reduce(v);
         } else if (a.post == 3) {
-            // the zeta pattern alone (below 1.1r after a factor, else < 46r)
-            if (L % 3) {
-                v = zeta29(v, L, a.first, a.second);
-            } else {
-                fr29_reduce_3r(v);
-                fr29_reduce_2r(v);
-            }
+            // the zeta pattern alone (below 1.1r after a factor, else < 48r)
+            if (L % 3) v = zeta29(v, L, a.first, a.second);
+            else fr29_reduce_top(v);
             fr29_reduce(v);
         } else if (a.post) {
             // below 1.7r after the scale, below 1.1r after a zeta factor
@@ -384,8 +406,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
             if (a.post == 2) v = zeta29(v, L, a.first, a.second);
             fr29_reduce(v);
         } else {
-            fr29_reduce_3r(v);
-            fr29_reduce_2r(v);
+            fr29_reduce_top(v);
             fr29_reduce(v);
         }
         store_fr(out + L, fr29_to32(v));
