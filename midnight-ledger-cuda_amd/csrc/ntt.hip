@@ -13,7 +13,8 @@
 //   (vendor/midnight-curves-0.3.1/src/fft.rs:30-112) limb for limb.
 // Inside a pass, a workgroup loads C columns of m = 2^t elements into LDS in
 // bit-reversed order, runs t radix-2 DIT stages (in pairs, one barrier per
-// pair), and writes the C columns back; C is chosen so C*m <= 2048.  Loads
+// pair; the first pair during the load, an odd last stage during the
+// store), and writes the C columns back; C is chosen so C*m <= 2048.  Loads
 // and stores walk the unit-stride dimension fastest, so every wave touches
 // whole runs of C (pass A) or m (pass B) consecutive elements.
 //
@@ -252,60 +253,54 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
     const Fr *in = a.in + (uint64_t)by * a.in_pstride;
     Fr *out = a.out + (uint64_t)by * a.out_pstride;
 
-    // load (bit-reversed within each column), applying the pre-operation;
-    // every element enters below 2r, limbs normalized.  A thread issues the
-    // loads of two elements before it converts either, so they are in flight
-    // together.
-    const bool load_cfast = (a.in_cs == 1) && C > 1;
-    constexpr int kLoadUnroll = 2;
-    for (uint32_t f0 = threadIdx.x; f0 < total; f0 += kLoadUnroll * blockDim.x) {
-        Fr x[kLoadUnroll];
-        uint64_t Ls[kLoadUnroll];
-        uint32_t dst[kLoadUnroll];
+    // an input element with the pre-operation: below 2r, limbs normalized
+    // (zero at and past in_len)
+    auto load_pre = [&](const Fr &x, uint64_t L) {
+        Fr29 v;
+        if (L < a.in_len) {
+            Fr xv = x;
+            if (a.pre == 2) xv = xv * a.factors[L % a.factor_len];
+            v = fr29_from32(xv);
+            if (a.pre == 1) v = zeta29(v, L, a.first, a.second);
+        } else {
 #pragma unroll
-        for (int u = 0; u < kLoadUnroll; ++u) {
-            const uint32_t f = f0 + u * blockDim.x;
-            uint32_t c, j;
-            if (load_cfast) { c = f & (C - 1); j = f >> log_C; }
-            else { j = f & (m - 1); c = f >> log_m; }
-            Ls[u] = (uint64_t)j * a.in_es + (uint64_t)(c0 + c) * a.in_cs;
-            dst[u] = (c << log_m) + bitrev(j, log_m);
-            if (f < total && Ls[u] < a.in_len) x[u] = load_fr(in + Ls[u]);
-            else x[u] = Fr{};
+            for (int i = 0; i < 9; ++i) v.v[i] = 0;
         }
-#pragma unroll
-        for (int u = 0; u < kLoadUnroll; ++u) {
-            if (f0 + u * blockDim.x >= total) break;
-            Fr29 v;
-            if (Ls[u] < a.in_len) {
-                Fr xv = x[u];
-                if (a.pre == 2) xv = xv * a.factors[Ls[u] % a.factor_len];
-                v = fr29_from32(xv);
-                if (a.pre == 1) v = zeta29(v, Ls[u], a.first, a.second);
-            } else {
-#pragma unroll
-                for (int i = 0; i < 9; ++i) v.v[i] = 0;
-            }
-            lds.put(dst[u], v);
-        }
-    }
-    __syncthreads();
+        return v;
+    };
 
     // DIT stages, two at a time: a thread takes the four elements
     // e, e + h, e + 2h, e + 3h (h = 2^s, bits s and s+1 of e clear), does
     // stage s on (e, e+h) and (e+2h, e+3h) -- one twiddle w_2h^low for both
     // -- and stage s+1 on (e, e+2h) and (e+h, e+3h) -- twiddles w_4h^low and
     // w_4h^(low+h) -- with one LDS round trip and one barrier for the pair.
-    // The first pair (h = 1, low = 0) has only the product by w_4; an odd
-    // last stage is a radix-2 one.
+    // The first pair is fused with the load and an odd last stage (radix 2)
+    // with the store.
+    const bool load_cfast = (a.in_cs == 1) && C > 1;
     const uint32_t lq = log_m - 2;
     uint32_t s = 0;
     if (log_m >= 2) {
-        for (uint32_t g = threadIdx.x; g < (C << lq); g += blockDim.x) {
-            const uint32_t c = g >> lq, gg = g & ((1u << lq) - 1);
-            const uint32_t e = (c << log_m) + (gg << 2);
-            const Fr29 x0 = lds.get(e), t1 = lds.get(e + 1);
-            const Fr29 x2 = lds.get(e + 2), t3 = lds.get(e + 3);
+        // Load and first pair (h = 1, low = 0: only the product by w_4).
+        // Group gg of a column holds LDS positions 4gg .. 4gg + 3, which the
+        // bit-reversed order fills with the natural indices J, J + m/2,
+        // J + m/4, J + 3m/4, J = bitrev(gg) over log m - 2 bits.  A thread
+        // takes one J (lanes walk J, or the columns when those are
+        // unit-stride), issues its four loads together, runs both stages in
+        // registers and writes the group once.
+        for (uint32_t f = threadIdx.x; f < (C << lq); f += blockDim.x) {
+            uint32_t c, J;
+            if (load_cfast) { c = f & (C - 1); J = f >> log_C; }
+            else { J = f & ((1u << lq) - 1); c = f >> lq; }
+            Fr x[4];
+            uint64_t Ls[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t j = J + ((uint32_t)(((i & 1) << 1) | (i >> 1)) << lq);
+                Ls[i] = (uint64_t)j * a.in_es + (uint64_t)(c0 + c) * a.in_cs;
+                x[i] = Ls[i] < a.in_len ? load_fr(in + Ls[i]) : Fr{};
+            }
+            const Fr29 x0 = load_pre(x[0], Ls[0]), t1 = load_pre(x[1], Ls[1]);
+            const Fr29 x2 = load_pre(x[2], Ls[2]), t3 = load_pre(x[3], Ls[3]);
             const Fr29 y0 = fr29_lazy_add(x0, t1), y1 = fr29_lazy_sub4r(x0, t1);
             const Fr29 y2 = fr29_lazy_add(x2, t3), y3 = fr29_lazy_sub4r(x2, t3);
             const Fr29 u3 = fr29_mul_loose(y3, a.roots.get(1u << (kMaxLogLds - 2)));
@@ -316,14 +311,25 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
             fr29_norm(z1);
             fr29_norm(z2);
             fr29_norm(z3);
+            const uint32_t e = (c << log_m) + ((lq ? bitrev(J, lq) : 0u) << 2);
             lds.put(e, z0);
             lds.put(e + 2, z2);
             lds.put(e + 1, z1);
             lds.put(e + 3, z3);
         }
-        __syncthreads();
         s = 2;
+    } else {
+        // one or two elements per column: load in bit-reversed order
+        for (uint32_t f = threadIdx.x; f < total; f += blockDim.x) {
+            uint32_t c, j;
+            if (load_cfast) { c = f & (C - 1); j = f >> log_C; }
+            else { j = f & (m - 1); c = f >> log_m; }
+            const uint64_t L = (uint64_t)j * a.in_es + (uint64_t)(c0 + c) * a.in_cs;
+            const Fr x = L < a.in_len ? load_fr(in + L) : Fr{};
+            lds.put((c << log_m) + (log_m ? bitrev(j, log_m) : 0u), load_pre(x, L));
+        }
     }
+    __syncthreads();
     for (; s + 1 < log_m; s += 2) {
         const uint32_t h = 1u << s;
         for (uint32_t g = threadIdx.x; g < (C << lq); g += blockDim.x) {
@@ -352,34 +358,13 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
         }
         __syncthreads();
     }
-    if (s < log_m) {
-        const uint32_t h = 1u << s;
-        const uint32_t lh = log_m - 1;
-        for (uint32_t b = threadIdx.x; b < (C << lh); b += blockDim.x) {
-            const uint32_t c = b >> lh, bb = b & ((1u << lh) - 1);
-            const uint32_t low = bb & (h - 1);
-            const uint32_t e = (c << log_m) + (((bb >> s) << (s + 1)) | low);
-            const Fr29 u = lds.get(e);
-            Fr29 v = lds.get(e + h);
-            if (s) v = fr29_mul_loose(v, a.roots.get(low << (kMaxLogLds - 1 - s)));
-            Fr29 o0 = fr29_lazy_add(u, v), o1 = fr29_lazy_sub4r(u, v);
-            fr29_norm(o0);
-            fr29_norm(o1);
-            lds.put(e, o0);
-            lds.put(e + h, o1);
-        }
-        __syncthreads();
-    }
 
     // store, applying the mid twiddle or the post-operation, reducing to
-    // the canonical value (elements are below 48r here)
+    // the canonical value (elements are below 48r here; limbs below 2^31,
+    // normalized unless they come from the fused last stage)
     const bool store_cfast = (a.out_cs == 1) && C > 1;
     const uint32_t tw_shift = kTwLog - a.log_n;
-    for (uint32_t f = threadIdx.x; f < total; f += blockDim.x) {
-        uint32_t c, k;
-        if (store_cfast) { c = f & (C - 1); k = f >> log_C; }
-        else { k = f & (m - 1); c = f >> log_m; }
-        Fr29 v = lds.get((c << log_m) + k);
+    auto store_one = [&](uint32_t c, uint32_t k, Fr29 v, bool lazy) {
         const uint64_t L = (uint64_t)k * a.out_es + (uint64_t)(c0 + c) * a.out_cs;
         if (a.twiddle) {
             // w_n^(c k) from the transform's table (below r), or formed from
@@ -397,8 +382,12 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
             fr29_reduce(v);
         } else if (a.post == 3) {
             // the zeta pattern alone (below 1.1r after a factor, else < 48r)
-            if (L % 3) v = zeta29(v, L, a.first, a.second);
-            else fr29_reduce_top(v);
+            if (L % 3) {
+                v = zeta29(v, L, a.first, a.second);
+            } else {
+                if (lazy) fr29_norm(v);
+                fr29_reduce_top(v);
+            }
             fr29_reduce(v);
         } else if (a.post) {
             // below 1.7r after the scale, below 1.1r after a zeta factor
@@ -406,10 +395,35 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, uint32_t bx, uint32
             if (a.post == 2) v = zeta29(v, L, a.first, a.second);
             fr29_reduce(v);
         } else {
+            if (lazy) fr29_norm(v);
             fr29_reduce_top(v);
             fr29_reduce(v);
         }
         store_fr(out + L, fr29_to32(v));
+    };
+    if (s < log_m) {
+        // last stage (radix 2, h = m/2) and store: butterfly k of a column
+        // pairs k and k + m/2 (twiddle w_m^k, none when m = 2), and both
+        // results go out from registers
+        const uint32_t lh = log_m - 1, half = 1u << lh;
+        for (uint32_t f = threadIdx.x; f < (C << lh); f += blockDim.x) {
+            uint32_t c, k;
+            if (store_cfast) { c = f & (C - 1); k = f >> log_C; }
+            else { k = f & (half - 1); c = f >> lh; }
+            const uint32_t e = (c << log_m) + k;
+            const Fr29 u = lds.get(e);
+            Fr29 v = lds.get(e + half);
+            if (s) v = fr29_mul_loose(v, a.roots.get(k << (kMaxLogLds - 1 - s)));
+            store_one(c, k, fr29_lazy_add(u, v), true);
+            store_one(c, k + half, fr29_lazy_sub4r(u, v), true);
+        }
+    } else {
+        for (uint32_t f = threadIdx.x; f < total; f += blockDim.x) {
+            uint32_t c, k;
+            if (store_cfast) { c = f & (C - 1); k = f >> log_C; }
+            else { k = f & (m - 1); c = f >> log_m; }
+            store_one(c, k, lds.get((c << log_m) + k), false);
+        }
     }
 }
 
