@@ -128,6 +128,14 @@ __device__ __forceinline__ S s_fold(const S &v, const S &y, const S &t, const S 
 {
     return fr29_mul2_loose(v, y, t, l);
 }
+// v y + (p - q) l for loose p, q: the difference stays lazy (p - q + 4r,
+// below 6r, limbs below 2^31), so each of the product's rows adds at most
+// 2^58 + 2^60 + 2^58 and its accumulators stay below 54 x 2^58 < 2^64; the
+// result is below (2r r + 6r r) / 2^261 + r < 1.12r, loose
+__device__ __forceinline__ S s_fold_diff(const S &v, const S &y, const S &p, const S &q, const S &l)
+{
+    return fr29_mul2_loose(v, y, fr29_lazy_sub4r(p, q), l);
+}
 __device__ __forceinline__ S s_const(const Fr &c) { return fr29_from32(c); }   // already scaled
 __device__ __forceinline__ S s_load(const Fr *p, bool scaled)
 {
@@ -298,13 +306,13 @@ __device__ __forceinline__ S permutation_terms(S v, size_t idx, const NumArgs &a
     const Fr *first = a.perm_products;
     const Fr *last = a.perm_products + (size_t)(a.perm_sets - 1) * len;
 
-    v = s_fold(v, y, s_sub(one, s_load(first + idx, zs)), l0);
+    v = s_fold_diff(v, y, one, s_load(first + idx, zs), l0);
     const S zl = s_load(last + idx, zs);
-    v = s_fold(v, y, s_sub(s_mul(zl, zl), zl), s_load(a.l_last + idx, ls));
+    v = s_fold_diff(v, y, s_mul(zl, zl), zl, s_load(a.l_last + idx, ls));
     for (uint32_t s = 1; s < a.perm_sets; ++s) {
         const S cur = s_load(a.perm_products + (size_t)s * len + idx, zs);
         const S prev = s_load(a.perm_products + (size_t)(s - 1) * len + r_last, zs);
-        v = s_fold(v, y, s_sub(cur, prev), l0);
+        v = s_fold_diff(v, y, cur, prev, l0);
     }
 
     const S l_active = s_load(a.l_active + idx, ls);
@@ -321,13 +329,18 @@ __device__ __forceinline__ S permutation_terms(S v, size_t idx, const NumArgs &a
             bool scaled = false;
             const Fr *col = column_base(c.kind, c.index, a, scaled);
             if (!col) continue;
-            const S x = s_add(s_load(col + idx, scaled), gamma);
+            // Sums that only feed a product stay lazy (limbs added, no
+            // carries, no reduction): x + gamma < 2r with limbs below 2^30,
+            // x + gamma + beta sigma < 3.1r and x + gamma + delta < 4r with
+            // limbs below 2^31, each times a loose factor (< 2r) under the
+            // product's r 2^261 limit, so the products stay loose.
+            const S x = fr29_lazy_add(s_load(col + idx, scaled), gamma);
             const S sigma = s_load(a.perm_cosets + (size_t)(base + j) * len + idx, cs);
-            left = s_mul(left, s_add(x, s_mul(beta, sigma)));
-            right = s_mul(right, s_add(x, delta));
+            left = s_mul(left, fr29_lazy_add(x, s_mul(beta, sigma)));
+            right = s_mul(right, fr29_lazy_add(x, delta));
             if (base + j + 1 < a.perm_cols) delta = s_mul(delta, dconst);   // (the last column's is unused)
         }
-        v = s_fold(v, y, s_sub(left, right), l_active);
+        v = s_fold_diff(v, y, left, right, l_active);
     }
     return v;
 }
@@ -355,10 +368,10 @@ __device__ __forceinline__ S lookup_terms(S v, size_t idx, const NumArgs &a, con
         const S a_i = s_load(ai + idx, ks);
         const S s_i = s_load(st + idx, ks);
         const S a_minus_s = s_sub(a_i, s_i);
-        v = s_fold(v, y, s_sub(one, zi), l0);
-        v = s_fold(v, y, s_sub(s_mul(zi, zi), zi), l_last);
+        v = s_fold_diff(v, y, one, zi, l0);
+        v = s_fold_diff(v, y, s_mul(zi, zi), zi, l_last);
         const S lhs = s_mul(s_mul(s_load(z + r_next, ks), s_add(a_i, beta)), s_add(s_i, gamma));
-        v = s_fold(v, y, s_sub(lhs, s_mul(zi, table_value)), l_active);
+        v = s_fold_diff(v, y, lhs, s_mul(zi, table_value), l_active);
         v = s_fold(v, y, a_minus_s, l0);
         v = s_fold(v, y, s_mul(a_minus_s, s_sub(a_i, s_load(ai + r_prev, ks))), l_active);
     }
