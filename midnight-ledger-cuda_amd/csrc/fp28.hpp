@@ -491,6 +491,135 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_affine_lz(const Xyzz28 &p, const Af
     return r;
 }
 
+// ---------------------------------------------------------------------------
+// Lazy forms for the accumulation's madd: limb-wise sums and differences with
+// no carries and no conditional correction, for values that only feed a
+// product (whose 64-bit accumulators take operand limbs below 2^29.6 here:
+// each row then adds at most 2^57.6 + 2^57 + 2^56 < 2^58.5, fourteen rows
+// stay below 2^62.3) or that are normalized and reduced at once.
+
+// k p with 2^s lent to every limb below the top (the top limb gives what
+// the limb below it borrowed), so that a subtrahend with limbs below 2^s
+// never drives a limb negative: 2p and 4p with 2^28 lent, 8p with 2^30
+__device__ __forceinline__ uint32_t p2x28_lent(int i)
+{
+    constexpr uint32_t L[14] = {0x1fff5556u, 0x1fdffffeu, 0x17ffff72u, 0x1fffd629u, 0x1c483d56u,
+                                0x141ed61du, 0x1ece61a4u, 0x1e70a256u, 0x18ee9708u, 0x19759aebu,
+                                0x174f6c85u, 0x1cd34962u, 0x13d472feu, 0x00034021u};
+    return L[i];
+}
+__device__ __forceinline__ uint32_t p4x28_lent(int i)
+{
+    constexpr uint32_t L[14] = {0x1ffeaaacu, 0x1fbffffeu, 0x1ffffee6u, 0x1fffac53u, 0x18907aaeu,
+                                0x183dac3cu, 0x1d9cc349u, 0x1ce144aeu, 0x11dd2e12u, 0x12eb35d8u,
+                                0x1e9ed90cu, 0x19a692c5u, 0x17a8e5feu, 0x00068043u};
+    return L[i];
+}
+__device__ __forceinline__ uint32_t p8x30_lent(int i)
+{
+    constexpr uint32_t L[14] = {0x4ffd5558u, 0x4f7ffffbu, 0x4ffffdcbu, 0x4fff58a5u, 0x4120f55bu,
+                                0x407b5877u, 0x4b398691u, 0x49c2895bu, 0x43ba5c23u, 0x45d66baeu,
+                                0x4d3db216u, 0x434d2589u, 0x4f51cbfbu, 0x000d0084u};
+    return L[i];
+}
+
+// 2p - y for canonical y (top limb below that of 2p minus one): the
+// negation, below 2p with limbs below 2^29, as a product operand
+__device__ __forceinline__ Fp28 fp28_lazy_neg2p(const Fp28 &y)
+{
+    Fp28 r;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) r.v[i] = p2x28_lent(i) - y.v[i];
+    return r;
+}
+
+// a - b + 4p for a, b loose (below 2p, limbs below 2^28): below 6p, limbs
+// below 2^29.6, as a product operand
+__device__ __forceinline__ Fp28 fp28_lazy_sub4p(const Fp28 &a, const Fp28 &b)
+{
+    Fp28 r;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) r.v[i] = a.v[i] + p4x28_lent(i) - b.v[i];
+    return r;
+}
+
+// normalized x (limbs below 2^28, value below 2^392) -> the same class below
+// 2p: x - q p with q = floor(x13 M / 2^48), M = floor(2^48 / (p13 + 1)), p13
+// the top limb of p.  x13 M / 2^48 is below x / p and short of it by at
+// most (x / p) / p13 + 2^-16, so q is floor(x / p) or one less for x / p
+// up to ~1000.
+__device__ __forceinline__ void fp28_reduce_top(Fp28 &a)
+{
+    const uint32_t q = (uint32_t)(((uint64_t)a.v[13] * 0x9d8307deull) >> 48);
+    int64_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        const int64_t t = (int64_t)a.v[i] - (int64_t)q * (int64_t)p28(i) + carry;
+        a.v[i] = i < 13 ? (uint32_t)t & kM28 : (uint32_t)t;
+        carry = t >> 28;   // arithmetic: the borrow into the next limb
+    }
+}
+
+// X3 = r2 - ppp - 2 qq for loose r2, ppp, qq: formed limb-wise with 8p
+// added (below 10p, limbs below 2^31), then carried and reduced by a
+// quotient from the top limb: loose (below 2p, limbs below 2^28)
+__device__ __forceinline__ Fp28 fp28_x3_lz(const Fp28 &r2, const Fp28 &ppp, const Fp28 &qq)
+{
+    Fp28 r;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) r.v[i] = r2.v[i] + p8x30_lent(i) - ppp.v[i] - 2 * qq.v[i];
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 13; ++i) {
+        const uint32_t s = r.v[i] + c;
+        c = s >> 28;
+        r.v[i] = s & kM28;
+    }
+    r.v[13] += c;
+    fp28_reduce_top(r);
+    return r;
+}
+
+// madd-2008-s as xyzz28_add_affine_lz, for q or -q (neg): the negated y
+// enters the one product that takes it as 2p - y, unreduced; X3 is formed
+// lazily and reduced once, and Y3's two differences stay lazy inside its
+// two-product reduction.  The rare exceptional paths take -q exactly.
+__device__ __forceinline__ Xyzz28 xyzz28_add_signed_lz(const Xyzz28 &p, const Affine28 &q, bool neg)
+{
+    if (q.is_inf()) return p;
+    if (p.is_inf()) {
+        Affine28 c = q;
+        if (neg) c.y = fp28_neg(q.y);
+        return xyzz28_from_affine(c);
+    }
+    const Fp28 ny = fp28_lazy_neg2p(q.y);
+    Fp28 qy;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) qy.v[i] = neg ? ny.v[i] : q.y.v[i];
+    Fp28 u2 = fp28_mul<false>(q.x, p.zz);
+    Fp28 s2 = fp28_mul<false>(qy, p.zzz);
+    Fp28 pp_ = fp28_sub_lz(u2, p.x);
+    Fp28 r_ = fp28_sub_lz(s2, p.y);
+    if (fp28_is_zero_lz(pp_)) {
+        if (fp28_is_zero_lz(r_)) {
+            Affine28 c = q;
+            if (neg) c.y = fp28_neg(q.y);
+            return xyzz28_dbl_affine(c);
+        }
+        return xyzz28_inf();
+    }
+    Fp28 pp = fp28_sqr<false>(pp_);
+    Fp28 ppp = fp28_mul<false>(pp_, pp);
+    Fp28 qq = fp28_mul<false>(p.x, pp);
+    Xyzz28 r;
+    r.x = fp28_x3_lz(fp28_sqr<false>(r_), ppp, qq);
+    // Y3 = R (Q - X3) - Y1 PPP = R (Q - X3 + 4p) + (4p - Y1) PPP
+    r.y = fp28_mul2_lz(r_, fp28_lazy_sub4p(qq, r.x), fp28_lazy_sub4p(fp28_zero(), p.y), ppp);
+    r.zz = fp28_mul<false>(p.zz, pp);
+    r.zzz = fp28_mul<false>(p.zzz, ppp);
+    return r;
+}
+
 __device__ __forceinline__ Affine28 load_affine28(const uint32_t *p)
 {
     const uint4 *q = reinterpret_cast<const uint4 *>(p);

@@ -732,12 +732,26 @@ __global__ void __launch_bounds__(kBlock) k_accumulate28(const uint32_t *pts, co
     const uint32_t end = off[b] + (uint32_t)(((uint64_t)(r + 1) * cb) / qb);
     Xyzz28 acc = xyzz28_inf();
     // the next row is in flight while the current one is added
-    Affine28 nxt = fetch_point28(pts, ent[start]);
-    for (uint32_t x = start; x < end; ++x) {
-        Affine28 cur = nxt;
-        if (x + 1 < end) nxt = fetch_point28(pts, ent[x + 1]);
-        if constexpr (kLanes == 4) acc = xyzz28_add_affine_quad(acc, cur, sub);
-        else acc = xyzz28_add_affine_lz(acc, cur);
+    if constexpr (kLanes == 4) {
+        Affine28 nxt = fetch_point28(pts, ent[start]);
+        for (uint32_t x = start; x < end; ++x) {
+            Affine28 cur = nxt;
+            if (x + 1 < end) nxt = fetch_point28(pts, ent[x + 1]);
+            acc = xyzz28_add_affine_quad(acc, cur, sub);
+        }
+    } else {
+        // the digit's sign goes into the addition (a lazy 2p - y)
+        uint32_t e = ent[start];
+        Affine28 nxt = load_affine28(pts + (uint64_t)(e & ~kSignBit) * kRow28);
+        for (uint32_t x = start; x < end; ++x) {
+            const Affine28 cur = nxt;
+            const bool neg = e & kSignBit;
+            if (x + 1 < end) {
+                e = ent[x + 1];
+                nxt = load_affine28(pts + (uint64_t)(e & ~kSignBit) * kRow28);
+            }
+            acc = xyzz28_add_signed_lz(acc, cur, neg);
+        }
     }
     if (sub) return;
     // the reduction kernels take the loose radix-2^28 sums as they are
