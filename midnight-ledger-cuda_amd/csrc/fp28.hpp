@@ -463,34 +463,6 @@ __device__ __forceinline__ bool fp28_is_zero_lz(const Fp28 &a)
     return z == 0 || e == 0;
 }
 
-// madd-2008-s on loose coordinates: p (accumulator, loose, identity when
-// zz == 0 exactly) + q (affine, reduced).  The identity is only ever made
-// by xyzz28_inf, and zz of a sum is zz * PP with PP nonzero, so the exact
-// zero test on zz stays valid.
-__device__ __forceinline__ Xyzz28 xyzz28_add_affine_lz(const Xyzz28 &p, const Affine28 &q)
-{
-    if (q.is_inf()) return p;
-    if (p.is_inf()) return xyzz28_from_affine(q);
-    Fp28 u2 = fp28_mul<false>(q.x, p.zz);
-    Fp28 s2 = fp28_mul<false>(q.y, p.zzz);
-    Fp28 pp_ = fp28_sub_lz(u2, p.x);
-    Fp28 r_ = fp28_sub_lz(s2, p.y);
-    if (fp28_is_zero_lz(pp_)) {
-        if (fp28_is_zero_lz(r_)) return xyzz28_dbl_affine(q);
-        return xyzz28_inf();
-    }
-    Fp28 pp = fp28_sqr<false>(pp_);
-    Fp28 ppp = fp28_mul<false>(pp_, pp);
-    Fp28 qq = fp28_mul<false>(p.x, pp);
-    Xyzz28 r;
-    r.x = fp28_sub_lz(fp28_sub_lz(fp28_sqr<false>(r_), ppp), fp28_add_lz(qq, qq));
-    // Y3 = R (Q - X3) - Y1 PPP = R (Q - X3) + (2p - Y1) PPP, one reduction
-    r.y = fp28_mul2_lz(r_, fp28_sub_lz(qq, r.x), fp28_sub_lz(fp28_zero(), p.y), ppp);
-    r.zz = fp28_mul<false>(p.zz, pp);
-    r.zzz = fp28_mul<false>(p.zzz, ppp);
-    return r;
-}
-
 // ---------------------------------------------------------------------------
 // Lazy forms for the accumulation's madd: limb-wise sums and differences with
 // no carries and no conditional correction, for values that only feed a
@@ -580,10 +552,13 @@ __device__ __forceinline__ Fp28 fp28_x3_lz(const Fp28 &r2, const Fp28 &ppp, cons
     return r;
 }
 
-// madd-2008-s as xyzz28_add_affine_lz, for q or -q (neg): the negated y
-// enters the one product that takes it as 2p - y, unreduced; X3 is formed
-// lazily and reduced once, and Y3's two differences stay lazy inside its
-// two-product reduction.  The rare exceptional paths take -q exactly.
+// madd-2008-s on loose coordinates: p (accumulator, loose, identity when
+// zz == 0 exactly) + q or -q (neg; q affine, reduced).  The identity is
+// only ever made by xyzz28_inf, and zz of a sum is zz * PP with PP nonzero,
+// so the exact zero test on zz stays valid.  The negated y enters the one
+// product that takes it as 2p - y, unreduced; X3 is formed lazily and
+// reduced once, and Y3's two differences stay lazy inside its two-product
+// reduction.  The rare exceptional paths take -q exactly.
 __device__ __forceinline__ Xyzz28 xyzz28_add_signed_lz(const Xyzz28 &p, const Affine28 &q, bool neg)
 {
     if (q.is_inf()) return p;
@@ -768,9 +743,10 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_lz(const Xyzz28 &p, const Xyzz28 &q
     Fp28 ppp = fp28_mul<false>(pp_, pp);
     Fp28 qq = fp28_mul<false>(u1, pp);
     Xyzz28 r;
-    r.x = fp28_sub_lz(fp28_sub_lz(fp28_sqr<false>(r_), ppp), fp28_add_lz(qq, qq));
-    // Y3 = R (Q - X3) - S1 PPP = R (Q - X3) + (2p - S1) PPP, one reduction
-    r.y = fp28_mul2_lz(r_, fp28_sub_lz(qq, r.x), fp28_sub_lz(fp28_zero(), s1), ppp);
+    r.x = fp28_x3_lz(fp28_sqr<false>(r_), ppp, qq);
+    // Y3 = R (Q - X3) - S1 PPP = R (Q - X3 + 4p) + (4p - S1) PPP, one
+    // reduction, the differences lazy
+    r.y = fp28_mul2_lz(r_, fp28_lazy_sub4p(qq, r.x), fp28_lazy_sub4p(fp28_zero(), s1), ppp);
     r.zz = fp28_mul<false>(fp28_mul<false>(p.zz, q.zz), pp);
     r.zzz = fp28_mul<false>(fp28_mul<false>(p.zzz, q.zzz), ppp);
     return r;
@@ -836,25 +812,37 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_quad(const Xyzz28 &p, const Xyzz28 
     const Fp28 ppp = fp28_quad_bcast<0>(t), qq = fp28_quad_bcast<1>(t);
     Xyzz28 r;
     r.zz = fp28_quad_bcast<2>(t);
-    r.x = fp28_sub_lz(fp28_sub_lz(r2, ppp), fp28_add_lz(qq, qq));
-    // R (Q - x3), s1 PPP, zzz3 = zzz1 zzz2 PPP
-    t = fp28_mul<false>(fp28_sel4(sub, r_, s1, zzz12, zzz12), fp28_sel4(sub, fp28_sub_lz(qq, r.x), ppp, ppp, ppp));
+    r.x = fp28_x3_lz(r2, ppp, qq);
+    // R (Q - x3), s1 PPP, zzz3 = zzz1 zzz2 PPP (Q - x3 lazy, plus 4p)
+    t = fp28_mul<false>(fp28_sel4(sub, r_, s1, zzz12, zzz12), fp28_sel4(sub, fp28_lazy_sub4p(qq, r.x), ppp, ppp, ppp));
     r.y = fp28_sub_lz(fp28_quad_bcast<0>(t), fp28_quad_bcast<1>(t));
     r.zzz = fp28_quad_bcast<2>(t);
     return r;
 }
 
-// xyzz28_add_affine_lz on a quad: its ten products in four steps
+// xyzz28_add_signed_lz on a quad: its ten products in four steps
 // (u2 s2 | PP R^2 | PPP Q zz3 | R(Q-X3) y1PPP zzz3)
-__device__ __forceinline__ Xyzz28 xyzz28_add_affine_quad(const Xyzz28 &p, const Affine28 &q, uint32_t sub)
+__device__ __forceinline__ Xyzz28 xyzz28_add_affine_quad(const Xyzz28 &p, const Affine28 &q, bool neg, uint32_t sub)
 {
     if (q.is_inf()) return p;
-    if (p.is_inf()) return xyzz28_from_affine(q);
-    Fp28 t = fp28_mul<false>(fp28_sel4(sub, q.x, q.y, q.x, q.y), fp28_sel4(sub, p.zz, p.zzz, p.zz, p.zzz));
+    if (p.is_inf()) {
+        Affine28 c = q;
+        if (neg) c.y = fp28_neg(q.y);
+        return xyzz28_from_affine(c);
+    }
+    const Fp28 ny = fp28_lazy_neg2p(q.y);
+    Fp28 qy;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) qy.v[i] = neg ? ny.v[i] : q.y.v[i];
+    Fp28 t = fp28_mul<false>(fp28_sel4(sub, q.x, qy, q.x, qy), fp28_sel4(sub, p.zz, p.zzz, p.zz, p.zzz));
     const Fp28 pp_ = fp28_sub_lz(fp28_quad_bcast<0>(t), p.x);
     const Fp28 r_ = fp28_sub_lz(fp28_quad_bcast<1>(t), p.y);
     if (fp28_is_zero_lz(pp_)) {
-        if (fp28_is_zero_lz(r_)) return xyzz28_dbl_affine(q);
+        if (fp28_is_zero_lz(r_)) {
+            Affine28 c = q;
+            if (neg) c.y = fp28_neg(q.y);
+            return xyzz28_dbl_affine(c);
+        }
         return xyzz28_inf();
     }
     const Fp28 sq = fp28_sel4(sub, pp_, r_, pp_, r_);
@@ -864,8 +852,8 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_affine_quad(const Xyzz28 &p, const 
     const Fp28 ppp = fp28_quad_bcast<0>(t), qq = fp28_quad_bcast<1>(t);
     Xyzz28 r;
     r.zz = fp28_quad_bcast<2>(t);
-    r.x = fp28_sub_lz(fp28_sub_lz(r2, ppp), fp28_add_lz(qq, qq));
-    t = fp28_mul<false>(fp28_sel4(sub, r_, p.y, p.zzz, p.zzz), fp28_sel4(sub, fp28_sub_lz(qq, r.x), ppp, ppp, ppp));
+    r.x = fp28_x3_lz(r2, ppp, qq);
+    t = fp28_mul<false>(fp28_sel4(sub, r_, p.y, p.zzz, p.zzz), fp28_sel4(sub, fp28_lazy_sub4p(qq, r.x), ppp, ppp, ppp));
     r.y = fp28_sub_lz(fp28_quad_bcast<0>(t), fp28_quad_bcast<1>(t));
     r.zzz = fp28_quad_bcast<2>(t);
     return r;

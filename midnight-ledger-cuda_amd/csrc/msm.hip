@@ -662,13 +662,6 @@ __global__ void k_chunk_map(const uint32_t *q, const uint32_t *co, uint32_t *cma
     for (uint32_t j = 0; j < n; ++j) cmap[o + j] = (uint32_t)b;
 }
 
-__device__ __forceinline__ Affine fetch_point(const uint32_t *pts, uint32_t e)
-{
-    Affine p = load_affine(pts + (uint64_t)(e & ~kSignBit) * 24);
-    if (e & kSignBit) p = affine_neg(p);
-    return p;
-}
-
 __global__ void __launch_bounds__(kBlock) k_accumulate(const uint32_t *pts, const uint32_t *ent, const uint32_t *cnt,
                                                        const uint32_t *off, const uint32_t *q, const uint32_t *co,
                                                        const uint32_t *cmap, const uint32_t *totals, uint32_t L,
@@ -685,15 +678,19 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(const uint32_t *pts, cons
     const uint32_t start = off[b] + (uint32_t)(((uint64_t)r * cb) / qb);
     const uint32_t end = off[b] + (uint32_t)(((uint64_t)(r + 1) * cb) / qb);
     // each point is converted to radix 2^28 (fully reduced: the rare doubling
-    // path takes it as canonical) and added in loose coordinates, as in the
-    // precomputed pass
+    // path takes it as canonical) and added in loose coordinates with the
+    // digit's sign, as in the precomputed pass
     Xyzz28 acc = xyzz28_inf();
     uint32_t e = ent[start];
-    Affine nxt = fetch_point(pts, e);
+    Affine nxt = load_affine(pts + (uint64_t)(e & ~kSignBit) * 24);
     for (uint32_t x = start; x < end; ++x) {
-        Affine cur = nxt;
-        if (x + 1 < end) nxt = fetch_point(pts, ent[x + 1]);
-        acc = xyzz28_add_affine_lz(acc, affine28_from_std(cur));
+        const Affine cur = nxt;
+        const bool neg = e & kSignBit;
+        if (x + 1 < end) {
+            e = ent[x + 1];
+            nxt = load_affine(pts + (uint64_t)(e & ~kSignBit) * 24);
+        }
+        acc = xyzz28_add_signed_lz(acc, affine28_from_std(cur), neg);
     }
     if (q[b] == 1) store_xyzz28(bucket + (uint64_t)b * kXyzz28Words, acc);
     else store_xyzz28(part + (uint64_t)j * kXyzz28Words, acc);
@@ -704,13 +701,6 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(const uint32_t *pts, cons
 // line; unpadded, six rows in eight straddle two lines): the bucket sum runs
 // in Fp28 and is converted once at the end.
 constexpr uint32_t kRow28 = 32;
-
-__device__ __forceinline__ Affine28 fetch_point28(const uint32_t *pts, uint32_t e)
-{
-    Affine28 p = load_affine28(pts + (uint64_t)(e & ~kSignBit) * kRow28);
-    if (e & kSignBit) p.y = fp28_neg(p.y);
-    return p;
-}
 
 // kLanes = 4: each chunk on a lane quad (small MSMs that cannot fill the GPU)
 template <int kLanes>
@@ -731,27 +721,19 @@ __global__ void __launch_bounds__(kBlock) k_accumulate28(const uint32_t *pts, co
     const uint32_t start = off[b] + (uint32_t)(((uint64_t)r * cb) / qb);
     const uint32_t end = off[b] + (uint32_t)(((uint64_t)(r + 1) * cb) / qb);
     Xyzz28 acc = xyzz28_inf();
-    // the next row is in flight while the current one is added
-    if constexpr (kLanes == 4) {
-        Affine28 nxt = fetch_point28(pts, ent[start]);
-        for (uint32_t x = start; x < end; ++x) {
-            Affine28 cur = nxt;
-            if (x + 1 < end) nxt = fetch_point28(pts, ent[x + 1]);
-            acc = xyzz28_add_affine_quad(acc, cur, sub);
+    // the next row is in flight while the current one is added; the
+    // digit's sign goes into the addition (a lazy 2p - y)
+    uint32_t e = ent[start];
+    Affine28 nxt = load_affine28(pts + (uint64_t)(e & ~kSignBit) * kRow28);
+    for (uint32_t x = start; x < end; ++x) {
+        const Affine28 cur = nxt;
+        const bool neg = e & kSignBit;
+        if (x + 1 < end) {
+            e = ent[x + 1];
+            nxt = load_affine28(pts + (uint64_t)(e & ~kSignBit) * kRow28);
         }
-    } else {
-        // the digit's sign goes into the addition (a lazy 2p - y)
-        uint32_t e = ent[start];
-        Affine28 nxt = load_affine28(pts + (uint64_t)(e & ~kSignBit) * kRow28);
-        for (uint32_t x = start; x < end; ++x) {
-            const Affine28 cur = nxt;
-            const bool neg = e & kSignBit;
-            if (x + 1 < end) {
-                e = ent[x + 1];
-                nxt = load_affine28(pts + (uint64_t)(e & ~kSignBit) * kRow28);
-            }
-            acc = xyzz28_add_signed_lz(acc, cur, neg);
-        }
+        if constexpr (kLanes == 4) acc = xyzz28_add_affine_quad(acc, cur, neg, sub);
+        else acc = xyzz28_add_signed_lz(acc, cur, neg);
     }
     if (sub) return;
     // the reduction kernels take the loose radix-2^28 sums as they are
