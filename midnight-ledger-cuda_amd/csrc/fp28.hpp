@@ -515,6 +515,15 @@ __device__ __forceinline__ Fp28 fp28_lazy_sub4p(const Fp28 &a, const Fp28 &b)
     return r;
 }
 
+// u - x for loose u, x is 0 mod p only if it is 0, p or -p, whose low 28
+// bits are 0, p_0 and 2^28 - p_0: any other low difference rules it out
+// (the exact test runs for about three pairs in 2^28)
+__device__ __forceinline__ bool fp28_may_equal(const Fp28 &u, const Fp28 &x)
+{
+    const uint32_t d = (u.v[0] - x.v[0]) & kM28;
+    return d == 0 || d == 0xfffaaabu || d == 0x0005555u;
+}
+
 // normalized x (limbs below 2^28, value below 2^392) -> the same class below
 // 2p: x - q p with q = floor(x13 M / 2^48), M = floor(2^48 / (p13 + 1)), p13
 // the top limb of p.  x13 M / 2^48 is below x / p and short of it by at
@@ -573,16 +582,17 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_signed_lz(const Xyzz28 &p, const Af
     for (int i = 0; i < 14; ++i) qy.v[i] = neg ? ny.v[i] : q.y.v[i];
     Fp28 u2 = fp28_mul<false>(q.x, p.zz);
     Fp28 s2 = fp28_mul<false>(qy, p.zzz);
-    Fp28 pp_ = fp28_sub_lz(u2, p.x);
-    Fp28 r_ = fp28_sub_lz(s2, p.y);
-    if (fp28_is_zero_lz(pp_)) {
-        if (fp28_is_zero_lz(r_)) {
+    if (fp28_may_equal(u2, p.x) && fp28_is_zero_lz(fp28_sub_lz(u2, p.x))) {
+        if (fp28_is_zero_lz(fp28_sub_lz(s2, p.y))) {
             Affine28 c = q;
             if (neg) c.y = fp28_neg(q.y);
             return xyzz28_dbl_affine(c);
         }
         return xyzz28_inf();
     }
+    // PP_ and R stay lazy (plus 4p: below 6p, limbs below 2^29.6)
+    const Fp28 pp_ = fp28_lazy_sub4p(u2, p.x);
+    const Fp28 r_ = fp28_lazy_sub4p(s2, p.y);
     Fp28 pp = fp28_sqr<false>(pp_);
     Fp28 ppp = fp28_mul<false>(pp_, pp);
     Fp28 qq = fp28_mul<false>(p.x, pp);
@@ -733,12 +743,12 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_lz(const Xyzz28 &p, const Xyzz28 &q
     Fp28 u2 = fp28_mul<false>(q.x, p.zz);
     Fp28 s1 = fp28_mul<false>(p.y, q.zzz);
     Fp28 s2 = fp28_mul<false>(q.y, p.zzz);
-    Fp28 pp_ = fp28_sub_lz(u2, u1);
-    Fp28 r_ = fp28_sub_lz(s2, s1);
-    if (fp28_is_zero_lz(pp_)) {
-        if (fp28_is_zero_lz(r_)) return xyzz28_dbl_lz(p);
+    if (fp28_may_equal(u2, u1) && fp28_is_zero_lz(fp28_sub_lz(u2, u1))) {
+        if (fp28_is_zero_lz(fp28_sub_lz(s2, s1))) return xyzz28_dbl_lz(p);
         return xyzz28_inf();
     }
+    const Fp28 pp_ = fp28_lazy_sub4p(u2, u1);
+    const Fp28 r_ = fp28_lazy_sub4p(s2, s1);
     Fp28 pp = fp28_sqr<false>(pp_);
     Fp28 ppp = fp28_mul<false>(pp_, pp);
     Fp28 qq = fp28_mul<false>(u1, pp);
@@ -797,12 +807,13 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_quad(const Xyzz28 &p, const Xyzz28 
     // u1 = x1 zz2, u2 = x2 zz1, s1 = y1 zzz2, s2 = y2 zzz1
     Fp28 t = fp28_mul<false>(fp28_sel4(sub, p.x, q.x, p.y, q.y), fp28_sel4(sub, q.zz, p.zz, q.zzz, p.zzz));
     const Fp28 u1 = fp28_quad_bcast<0>(t), s1 = fp28_quad_bcast<2>(t);
-    const Fp28 pp_ = fp28_sub_lz(fp28_quad_bcast<1>(t), u1);
-    const Fp28 r_ = fp28_sub_lz(fp28_quad_bcast<3>(t), s1);
-    if (fp28_is_zero_lz(pp_)) {
-        if (fp28_is_zero_lz(r_)) return xyzz28_dbl_lz(p);
+    const Fp28 u2 = fp28_quad_bcast<1>(t), s2 = fp28_quad_bcast<3>(t);
+    if (fp28_may_equal(u2, u1) && fp28_is_zero_lz(fp28_sub_lz(u2, u1))) {
+        if (fp28_is_zero_lz(fp28_sub_lz(s2, s1))) return xyzz28_dbl_lz(p);
         return xyzz28_inf();
     }
+    const Fp28 pp_ = fp28_lazy_sub4p(u2, u1);
+    const Fp28 r_ = fp28_lazy_sub4p(s2, s1);
     // PP = P^2, R^2, zz1 zz2, zzz1 zzz2
     t = fp28_mul<false>(fp28_sel4(sub, pp_, r_, p.zz, p.zzz), fp28_sel4(sub, pp_, r_, q.zz, q.zzz));
     const Fp28 pp = fp28_quad_bcast<0>(t), r2 = fp28_quad_bcast<1>(t);
@@ -835,16 +846,17 @@ __device__ __forceinline__ Xyzz28 xyzz28_add_affine_quad(const Xyzz28 &p, const 
 #pragma unroll
     for (int i = 0; i < 14; ++i) qy.v[i] = neg ? ny.v[i] : q.y.v[i];
     Fp28 t = fp28_mul<false>(fp28_sel4(sub, q.x, qy, q.x, qy), fp28_sel4(sub, p.zz, p.zzz, p.zz, p.zzz));
-    const Fp28 pp_ = fp28_sub_lz(fp28_quad_bcast<0>(t), p.x);
-    const Fp28 r_ = fp28_sub_lz(fp28_quad_bcast<1>(t), p.y);
-    if (fp28_is_zero_lz(pp_)) {
-        if (fp28_is_zero_lz(r_)) {
+    const Fp28 u2 = fp28_quad_bcast<0>(t), s2 = fp28_quad_bcast<1>(t);
+    if (fp28_may_equal(u2, p.x) && fp28_is_zero_lz(fp28_sub_lz(u2, p.x))) {
+        if (fp28_is_zero_lz(fp28_sub_lz(s2, p.y))) {
             Affine28 c = q;
             if (neg) c.y = fp28_neg(q.y);
             return xyzz28_dbl_affine(c);
         }
         return xyzz28_inf();
     }
+    const Fp28 pp_ = fp28_lazy_sub4p(u2, p.x);
+    const Fp28 r_ = fp28_lazy_sub4p(s2, p.y);
     const Fp28 sq = fp28_sel4(sub, pp_, r_, pp_, r_);
     t = fp28_mul<false>(sq, sq);
     const Fp28 pp = fp28_quad_bcast<0>(t), r2 = fp28_quad_bcast<1>(t);
