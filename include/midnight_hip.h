@@ -574,9 +574,9 @@ RustError midnight_hip_g1_msm_context_last_stats(const void *context, uint64_t *
                                                  uint32_t *window_bits);
 
 /* ---- proof pipeline (create_proof of plonk/prover.rs:1019-1076 for circuits
- * made of custom gates and a copy permutation; no lookups, trashcans or
- * instance columns).  For provers that keep a circuit's key and the SRS
- * resident and want whole proofs from one call. ---- */
+ * made of custom gates, a copy permutation, lookups, trashcans and
+ * non-committed instance columns).  For provers that keep a circuit's key
+ * and the SRS resident and want whole proofs from one call. ---- */
 
 /* Proving key.  srs_context: an MSM context whose first srs_points points
  * are [tau^i]G (>= 2^k); not owned, must outlive the key.  fixed: fixed_cols
@@ -593,6 +593,33 @@ RustError midnight_hip_plonk_keygen(void **out, uint32_t k, void *srs_context, s
                                     size_t fixed_query_count, const RawColumnRef *perm_columns,
                                     size_t perm_cols, const uint32_t *perm_mapping, RawGraph gate,
                                     const char *description);
+/* The arguments of a key beyond gates and the copy permutation.
+ * instance_cols: non-committed instance columns (compute_instances,
+ * plonk/prover.rs:1237-1330: their lengths and values go into the transcript;
+ * they are neither committed nor opened).  Per lookup
+ * (plonk/lookup/prover.rs): the compression of its input expressions and of
+ * its table expressions (Horner in theta from zero, evaluation.rs:2743-2766)
+ * and the numerator's lookup graph (input + beta)(table + gamma)
+ * (evaluation.rs:2768-2797).  Per trashcan (plonk/trash/prover.rs): the graph
+ * of its constraint expressions folded in the trash challenge and the fixed
+ * column of its selector.  Graphs read advice at rotations the key queries. */
+typedef struct RawPlonkArguments {
+    size_t instance_cols;
+    size_t lookup_count;
+    const RawGraph *lookup_input_graphs;
+    const RawGraph *lookup_table_graphs;
+    const RawGraph *lookup_graphs;
+    size_t trash_count;
+    const RawGraph *trash_graphs;
+    const uint32_t *trash_selectors;
+} RawPlonkArguments;
+/* midnight_hip_plonk_keygen for a circuit with the arguments above (null:
+ * none).  Permuted columns may name instance columns (kind 2). */
+RustError midnight_hip_plonk_keygen_with_arguments(
+    void **out, uint32_t k, void *srs_context, size_t srs_points, uint32_t degree, const RawFr *fixed,
+    size_t fixed_cols, size_t advice_cols, const uint32_t *advice_queries, size_t advice_query_count,
+    const uint32_t *fixed_queries, size_t fixed_query_count, const RawColumnRef *perm_columns, size_t perm_cols,
+    const uint32_t *perm_mapping, RawGraph gate, const char *description, const RawPlonkArguments *arguments);
 void midnight_hip_plonk_key_free(void *pk);
 /* Verifying data: fixed and permutation commitments, transcript representative. */
 RustError midnight_hip_plonk_vk(const void *pk, RawG1Projective *fixed_commitments,
@@ -604,6 +631,16 @@ RustError midnight_hip_plonk_vk(const void *pk, RawG1Projective *fixed_commitmen
  * any work is done (the size is fixed by the key). */
 RustError midnight_hip_plonk_prove(void *pk, const RawFr *advice, size_t advice_len, uint64_t rng_seed,
                                    uint8_t *proof, size_t proof_capacity, size_t *proof_len);
+/* The same with the proof's instance values: instance_cols (the key's count)
+ * columns of instance_lens[c] values each, concatenated in column order; a
+ * column may hold at most 2^k - blinding - 1 values (Error::InstanceTooLarge
+ * of compute_instances), the rest of its rows are zero.  A lookup input
+ * missing from its table fails the proof (code -1) as
+ * permute_expression_pair does. */
+RustError midnight_hip_plonk_prove_with_instances(void *pk, const RawFr *advice, size_t advice_len,
+                                                  const RawFr *instance, const size_t *instance_lens,
+                                                  size_t instance_cols, uint64_t rng_seed, uint8_t *proof,
+                                                  size_t proof_capacity, size_t *proof_len);
 /* Commitment stages from HIP graphs (default on): on each lane, the first
  * proof of the key captures its commitment stages (inverse NTT + MSM batch of
  * the advice, the permutation products; the MSMs of the random polynomial,

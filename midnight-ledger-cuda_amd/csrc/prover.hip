@@ -1,17 +1,22 @@
 // Proof pipeline: key generation and create_proof for PLONK circuits made of
-// custom gates and a copy permutation (the benchmark circuit family: 5
-// advice and 3 fixed columns, gate q(ab + c - d), 8 permuted columns),
-// device-resident from the advice upload to the last opening commitment.
+// custom gates, a copy permutation, lookups, trashcans and (non-committed)
+// instance columns, device-resident from the advice upload to the last
+// opening commitment.  The benchmark circuit family has 5 advice and 3 fixed
+// columns, gate q(ab + c - d) and 8 permuted columns.
 //
 // Stage order, transcript and blinding draws follow the reference prover
 // (vendor/midnight-proofs-0.8.1/src/plonk/prover.rs:68-575 compute_trace,
-// :769-1010 finalise_proof, :1931-2080 evaluations and queries;
-// permutation/prover.rs:327-530, :545-712; vanishing/prover.rs:41-406;
-// poly/kzg/mod.rs:1978-2570 multi_open; transcript/{mod.rs,
-// implementors.rs}):
-//   vk repr -> advice (blinded rows u..n from the RNG) -> commit ->
-//   theta, beta, gamma -> permutation products (blinds from the RNG) ->
-//   commit -> trash challenge -> random polynomial (ChaCha20 seeded from
+// :769-1010 finalise_proof, :1237-1330 compute_instances, :1931-2080
+// evaluations and queries; lookup/prover.rs:737-1072 commit_permuted_batch,
+// :1075-1340 products, :1341-1572 evaluate / open, :1573-1690
+// permute_expression_pair; trash/prover.rs; permutation/prover.rs:327-530,
+// :545-712; vanishing/prover.rs:41-406; poly/kzg/mod.rs:1978-2570
+// multi_open; transcript/{mod.rs, implementors.rs}):
+//   vk repr -> instance values (length, values) -> advice (blinded rows
+//   u..n from the RNG) -> commit -> theta -> lookup permuted input / table
+//   (blinds from the RNG) -> commit -> beta, gamma -> permutation products,
+//   lookup products (blinds from the RNG) -> commit -> trash challenge ->
+//   trash polynomials -> commit -> random polynomial (ChaCha20 seeded from
 //   the RNG) -> commit -> y -> numerator on the extended coset -> h ->
 //   pieces (telescoping blinds) -> commit -> x -> evaluations -> multi-open
 //   (x1, x2, f, x3, q evaluations, x4, pi).
@@ -27,6 +32,7 @@
 // (midnight_hip_plonk_set_rng_threads, default 1).  Commitments are computed
 // from coefficients with the monomial SRS (commit_lagrange with the Lagrange
 // SRS gives the same points).
+
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -235,7 +241,7 @@ struct StageGraph {
 struct LaneProof {
     std::unique_ptr<DeviceVec> polys, fv;
     DevBuf<Fr> ntt_scratch;
-    StageGraph advice, products, random, pieces, opening;
+    StageGraph advice, lookups, products, trash, random, pieces, opening;
 };
 
 struct PlonkKey {
@@ -243,6 +249,7 @@ struct PlonkKey {
     uint32_t k = 0, ext_k = 0, degree = 0;
     size_t n = 0, ext_n = 0, bf = 0, u = 0;
     size_t fixed_cols = 0, advice_cols = 0, perm_cols = 0, chunk = 0, sets = 0, pieces = 0;
+    size_t instance_cols = 0, lookups = 0, trashes = 0;
     void *srs = nullptr;                                   // MSM context (not owned), >= n points
     std::vector<std::pair<uint32_t, int32_t>> advice_queries, fixed_queries;
     std::vector<RawColumnRef> perm_columns;
@@ -334,7 +341,8 @@ std::unique_ptr<PlonkKey> plonk_keygen(uint32_t k, void *srs, size_t srs_points,
                                        const std::vector<std::pair<uint32_t, int32_t>> &advice_queries,
                                        const std::vector<std::pair<uint32_t, int32_t>> &fixed_queries,
                                        const RawColumnRef *perm_columns, size_t perm_cols, const uint32_t *mapping,
-                                       const RawGraph &gate, const std::string &description)
+                                       const RawGraph &gate, const std::string &description,
+                                       const RawPlonkArguments &args)
 {
     hipStream_t st = cur_stream();
     auto pk = std::make_unique<PlonkKey>();
@@ -383,6 +391,15 @@ std::unique_ptr<PlonkKey> plonk_keygen(uint32_t k, void *srs, size_t srs_points,
     pk->fixed_cols = fixed_cols;
     pk->advice_cols = advice_cols;
     pk->perm_cols = perm_cols;
+    pk->instance_cols = args.instance_cols;
+    pk->lookups = args.lookup_count;
+    pk->trashes = args.trash_count;
+    for (size_t j = 0; j < perm_cols; ++j)
+        MH_CHECK(perm_columns[j].kind <= 2 &&
+                     perm_columns[j].index < (perm_columns[j].kind == 0   ? fixed_cols
+                                              : perm_columns[j].kind == 1 ? advice_cols
+                                                                          : args.instance_cols),
+                 (int)hipErrorInvalidValue, "permuted column out of range");
     pk->chunk = degree - 2;
     pk->sets = (perm_cols + pk->chunk - 1) / pk->chunk;
     pk->advice_queries = advice_queries;
@@ -453,8 +470,9 @@ std::unique_ptr<PlonkKey> plonk_keygen(uint32_t k, void *srs, size_t srs_points,
     check(midnight_sppark_bls12_381_numerator_context_create(
         &pk->numctx, N, n, fixed_cols ? h_fixed_ext.data() : nullptr, fixed_cols ? h_fixed_lag.data() : nullptr,
         fixed_cols, h_lag_ext.data(), h_lag_ext.data() + N, h_lag_ext.data() + 2 * N, perm_columns, perm_cols,
-        h_sigma_ext.data(), reinterpret_cast<const RawFr *>(sigma.data()), perm_cols, nullptr, 0, gate, nullptr,
-        nullptr, nullptr, 0, nullptr, raw(pk->ext_omega), raw(pk->omega), raw(kZeta), raw(kDelta),
+        h_sigma_ext.data(), reinterpret_cast<const RawFr *>(sigma.data()), perm_cols, args.trash_selectors,
+        args.trash_count, gate, args.lookup_graphs, args.lookup_input_graphs, args.lookup_table_graphs,
+        args.lookup_count, args.trash_graphs, raw(pk->ext_omega), raw(pk->omega), raw(kZeta), raw(kDelta),
         (int32_t)(1u << (ek - k)), -(int32_t)(pk->bf + 1), (uint32_t)pk->chunk));
 
     // verifying-key representative (plonk/mod.rs:371-405): Blake2b-512
@@ -495,8 +513,9 @@ std::unique_ptr<PlonkKey> plonk_keygen(uint32_t k, void *srs, size_t srs_points,
 // coefficient-form polynomials and opening temporaries, and MSM scratch.
 size_t plonk_footprint(const PlonkKey &pk)
 {
-    const size_t polys = pk.advice_cols + pk.sets + 1 + pk.pieces;
-    return 32 * (pk.ext_n * (pk.advice_cols + pk.sets + 2) + pk.n * (2 * polys + 12)) + 256 * pk.n;
+    const size_t polys = pk.advice_cols + pk.sets + 1 + pk.pieces + 3 * pk.lookups + pk.trashes + pk.instance_cols;
+    const size_t ext_cols = pk.advice_cols + pk.instance_cols + pk.sets + 3 * pk.lookups + pk.trashes + 2;
+    return 32 * (pk.ext_n * ext_cols + pk.n * (2 * polys + 12 + 4 * pk.lookups + pk.trashes)) + 256 * pk.n;
 }
 
 size_t plonk_advice_len(const PlonkKey &pk) { return pk.advice_cols * pk.n; }
@@ -506,17 +525,28 @@ size_t plonk_advice_len(const PlonkKey &pk) { return pk.advice_cols * pk.n; }
 // them (queries at distinct rotations of x are distinct points).
 size_t plonk_proof_size(const PlonkKey &pk)
 {
-    const size_t A = pk.advice_cols, S = pk.sets;
+    const size_t A = pk.advice_cols, S = pk.sets, L = pk.lookups, TR = pk.trashes;
     const int32_t last = -(int32_t)(pk.bf + 1);
-    // (polynomial id, rotation) in the prover's query order
+    // (polynomial id, rotation) in the prover's query order; ids: advice,
+    // permutation sets, lookup products, permuted inputs, permuted tables,
+    // trash polynomials, fixed columns, sigmas, h, the random polynomial
+    const long z0 = (long)A, lz0 = z0 + (long)S, li0 = lz0 + (long)L, lt0 = li0 + (long)L, tr0 = lt0 + (long)L;
+    const long fixed0 = tr0 + (long)TR, sigma0 = fixed0 + (long)pk.fixed_cols, hid = sigma0 + (long)pk.perm_cols;
     std::vector<std::pair<long, int32_t>> q;
     for (auto &a : pk.advice_queries) q.emplace_back((long)a.first, a.second);
     for (size_t s = 0; s < S; ++s) {
-        q.emplace_back((long)(A + s), 0);
-        q.emplace_back((long)(A + s), 1);
+        q.emplace_back(z0 + (long)s, 0);
+        q.emplace_back(z0 + (long)s, 1);
     }
-    for (size_t s = S - 1; s-- > 0;) q.emplace_back((long)(A + s), last);
-    const long fixed0 = (long)(A + S), sigma0 = fixed0 + (long)pk.fixed_cols, hid = sigma0 + (long)pk.perm_cols;
+    for (size_t s = S - 1; s-- > 0;) q.emplace_back(z0 + (long)s, last);
+    for (size_t l = 0; l < L; ++l) {
+        q.emplace_back(lz0 + (long)l, 0);
+        q.emplace_back(li0 + (long)l, 0);
+        q.emplace_back(lt0 + (long)l, 0);
+        q.emplace_back(li0 + (long)l, -1);
+        q.emplace_back(lz0 + (long)l, 1);
+    }
+    for (size_t t = 0; t < TR; ++t) q.emplace_back(tr0 + (long)t, 0);
     for (auto &f : pk.fixed_queries) q.emplace_back(fixed0 + f.first, f.second);
     for (size_t j = 0; j < pk.perm_cols; ++j) q.emplace_back(sigma0 + (long)j, 0);
     q.emplace_back(hid, 0);
@@ -526,9 +556,9 @@ size_t plonk_proof_size(const PlonkKey &pk)
     for (auto &e : q) rots[e.first].insert(((int64_t)e.second % n + n) % n);
     std::set<std::set<int64_t>> sets;
     for (auto &kv : rots) sets.insert(kv.second);
-    const size_t points = A + S + 1 + pk.pieces + 2;
+    const size_t points = A + 2 * L + S + L + TR + 1 + pk.pieces + 2;
     const size_t scalars = pk.advice_queries.size() + pk.fixed_queries.size() + 1 + pk.perm_cols + 3 * S - 1 +
-                           sets.size();
+                           5 * L + TR + sets.size();
     return 48 * points + 32 * scalars;
 }
 
@@ -543,10 +573,12 @@ struct Query {
 
 }  // namespace
 
-std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t seed)
+std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, const RawFr *instance,
+                                 const size_t *instance_lens, uint64_t seed)
 {
     hipStream_t st = cur_stream();
     const size_t n = pk.n, N = pk.ext_n, A = pk.advice_cols, S = pk.sets;
+    const size_t I = pk.instance_cols, L = pk.lookups, TR = pk.trashes;
     Transcript T;
 
     // every draw of the prover's RNG, in the reference's order
@@ -554,8 +586,17 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     const size_t brows = n - pk.u;
     std::vector<HostFr> adv_blinds(A * brows);   // unusable rows u..n of each advice column
     for (auto &b : adv_blinds) b = rng.random_fr();
+    // per lookup, the permuted input's then the permuted table's rows u..n
+    // (lookup/prover.rs:827-838)
+    std::vector<HostFr> lk_in_blinds(L * brows), lk_tab_blinds(L * brows);
+    for (size_t l = 0; l < L; ++l) {
+        for (size_t j = 0; j < brows; ++j) lk_in_blinds[l * brows + j] = rng.random_fr();
+        for (size_t j = 0; j < brows; ++j) lk_tab_blinds[l * brows + j] = rng.random_fr();
+    }
     std::vector<HostFr> perm_blinds(S * pk.bf);
     for (auto &b : perm_blinds) b = rng.random_fr();
+    std::vector<HostFr> lk_prod_blinds(L * pk.bf);
+    for (auto &b : lk_prod_blinds) b = rng.random_fr();
     // vanishing random polynomial: one ChaCha20Rng seed per chunk of the
     // reference prover's thread pool (vanishing/prover.rs:47-66)
     std::vector<size_t> voff, vlen;
@@ -565,8 +606,11 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     std::vector<HostFr> limb_blinds(pk.pieces - 1);
     for (auto &b : limb_blinds) b = rng.random_fr();
 
-    // polys: advice | z sets | random | h pieces (coefficient form, n each)
-    const size_t P_ADV = 0, P_Z = A, P_RAND = A + S, P_H = A + S + 1, NPOLY = A + S + 1 + pk.pieces;
+    // polys (coefficient form, n each): advice | lookup permuted input and
+    // table, interleaved per lookup | permutation sets | lookup products |
+    // trash | random | h pieces | instance
+    const size_t P_ADV = 0, P_LP = A, P_Z = P_LP + 2 * L, P_LZ = P_Z + S, P_TR = P_LZ + L, P_RAND = P_TR + TR,
+                 P_H = P_RAND + 1, P_INST = P_H + pk.pieces, NPOLY = P_INST + I;
     // with graphs, the lane's arena at fixed addresses; else one per proof
     LaneProof *lp = nullptr;
     std::unique_ptr<DeviceVec> own_polys, own_fv;
@@ -577,7 +621,8 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
             slot->polys = new_vec(NPOLY * n);
             slot->fv = new_vec(2 * n);
             // sized once for every captured transform: graphs keep its address
-            slot->ntt_scratch.alloc(std::max<size_t>(ntt_scratch_elems(pk.k, std::max(A, S)), 1));
+            const size_t widest = std::max({A, S + L, 2 * L, TR, size_t(1)});
+            slot->ntt_scratch.alloc(ntt_scratch_elems(pk.k, widest));
         }
         lp = slot.get();
     } else {
@@ -605,7 +650,35 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
 
     T.common_scalar(pk.repr);
 
-    // advice: Lagrange values (kept for the permutation), coefficients, commitments
+    // instance columns (compute_instances, prover.rs:1255-1286): each
+    // column's length and values into the transcript; Lagrange values
+    // (zero beyond the length) for the arguments, coefficients for the
+    // numerator
+    std::unique_ptr<DeviceVec> inst_lag;
+    if (I) {
+        std::vector<RawFr> iv(I * n);
+        std::memset(iv.data(), 0, I * n * 32);
+        size_t off = 0;
+        for (size_t c = 0; c < I; ++c) {
+            const size_t len = instance_lens ? instance_lens[c] : 0;
+            MH_CHECK(len <= pk.u, (int)hipErrorInvalidValue, "instance column longer than the usable rows");
+            T.common_scalar(HostFr::from_u64(len));
+            for (size_t i = 0; i < len; ++i) {
+                HostFr v;
+                std::memcpy(v.l, instance[off + i].limbs, 32);
+                T.common_scalar(v);
+                iv[c * n + i] = instance[off + i];
+            }
+            off += len;
+        }
+        inst_lag = new_vec(I * n);
+        MH_HIP(hipMemcpyAsync(inst_lag->values.get(), iv.data(), I * n * 32, hipMemcpyHostToDevice, st));
+        MH_HIP(hipMemcpyAsync(pd + P_INST * n, inst_lag->values.get(), I * n * 32, hipMemcpyDeviceToDevice, st));
+        ntt_batch(pd + P_INST * n, pk.k, I, true, pk.scratch(), st);
+        MH_HIP(hipStreamSynchronize(st));   // iv goes out of scope
+    }
+
+    // advice: Lagrange values (kept for the arguments), coefficients, commitments
     DeviceVec adv_lag;
     adv_lag.lane = current_lane();
     adv_lag.len = A * n;
@@ -621,36 +694,95 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     for (auto &c : com) T.write_point(c);
 
     const HostFr theta = T.squeeze();
+
+    // lookups: compressed inputs and tables (theta), permute_expression_pair
+    // with the blinds in rows u..n, then the permuted input and table of
+    // each lookup committed in that order (lookup/prover.rs:737-1072)
+    std::unique_ptr<DeviceVec> ltrace;   // [compressed inputs | tables | permuted inputs | tables]
+    if (L) {
+        void *lv = nullptr;
+        check(midnight_sppark_bls12_381_trace_context_lookup_permute(
+            &lv, pk.numctx, &adv_lag, A, inst_lag.get(), I, nullptr, 0, raw(theta), pk.u,
+            reinterpret_cast<const RawFr *>(lk_in_blinds.data()), reinterpret_cast<const RawFr *>(lk_tab_blinds.data()),
+            brows));
+        ltrace.reset(reinterpret_cast<DeviceVec *>(lv));
+        const Fr *tr = ltrace->values.get();
+        for (size_t l = 0; l < L; ++l) {
+            MH_HIP(hipMemcpyAsync(pd + (P_LP + 2 * l) * n, tr + (2 * L + l) * n, n * 32, hipMemcpyDeviceToDevice, st));
+            MH_HIP(hipMemcpyAsync(pd + (P_LP + 2 * l + 1) * n, tr + (3 * L + l) * n, n * 32, hipMemcpyDeviceToDevice,
+                                  st));
+        }
+        commit_stage(lp ? &lp->lookups : nullptr, polys, P_LP, 2 * L, true);
+        for (auto &c : com) T.write_point(c);
+    }
+
     const HostFr beta = T.squeeze();
     const HostFr gamma = T.squeeze();
 
-    // permutation products from the resident Lagrange advice
+    // permutation products from the resident Lagrange advice / instance, then
+    // the lookup products (lookup/prover.rs:1075-1340); committed together,
+    // permutation sets first
     void *zv = nullptr;
     check(midnight_sppark_bls12_381_trace_context_permutation(
-        &zv, pk.numctx, &adv_lag, A, nullptr, 0, raw(beta), raw(gamma),
+        &zv, pk.numctx, &adv_lag, A, inst_lag.get(), I, raw(beta), raw(gamma),
         reinterpret_cast<const RawFr *>(pk.delta_powers.data()), pk.perm_cols,
         reinterpret_cast<const RawFr *>(perm_blinds.data()), perm_blinds.size(), pk.bf));
     std::unique_ptr<DeviceVec> z(reinterpret_cast<DeviceVec *>(zv));
     MH_HIP(hipMemcpyAsync(pd + P_Z * n, z->values.get(), S * n * 32, hipMemcpyDeviceToDevice, st));
+    if (L) {
+        void *lzv = nullptr;
+        check(midnight_sppark_bls12_381_fr_lookup_products_device(
+            &lzv, ltrace.get(), n, L, raw(beta), raw(gamma), reinterpret_cast<const RawFr *>(lk_prod_blinds.data()),
+            pk.bf));
+        std::unique_ptr<DeviceVec> lz(reinterpret_cast<DeviceVec *>(lzv));
+        MH_HIP(hipMemcpyAsync(pd + P_LZ * n, lz->values.get(), L * n * 32, hipMemcpyDeviceToDevice, st));
+        MH_HIP(hipStreamSynchronize(st));
+    }
     z.reset();
-    commit_stage(lp ? &lp->products : nullptr, polys, P_Z, S, true);
+    ltrace.reset();
+    commit_stage(lp ? &lp->products : nullptr, polys, P_Z, S + L, true);
     for (auto &c : com) T.write_point(c);
 
     const HostFr trash = T.squeeze();
+
+    // trash polynomials: each trashcan's constraints folded in the trash
+    // challenge on the Lagrange domain, unblinded (trash/prover.rs)
+    if (TR) {
+        void *tv = nullptr;
+        check(midnight_sppark_bls12_381_trace_context_trash(&tv, pk.numctx, &adv_lag, A, inst_lag.get(), I, nullptr,
+                                                            0, raw(trash)));
+        std::unique_ptr<DeviceVec> tvec(reinterpret_cast<DeviceVec *>(tv));
+        MH_HIP(hipMemcpyAsync(pd + P_TR * n, tvec->values.get(), TR * n * 32, hipMemcpyDeviceToDevice, st));
+        MH_HIP(hipStreamSynchronize(st));
+        tvec.reset();
+        commit_stage(lp ? &lp->trash : nullptr, polys, P_TR, TR, true);
+        for (auto &c : com) T.write_point(c);
+    }
 
     commit_stage(lp ? &lp->random : nullptr, polys, P_RAND, 1, false);
     T.write_point(com[0]);
 
     const HostFr y = T.squeeze();
 
-    // numerator on the extended coset of advice | products
-    auto cols = new_vec((A + S) * N);
-    to_extended(cols->values.get(), pd + P_ADV * n, n, A, pk.ext_k, pk.scratch(), st, true);
-    to_extended(cols->values.get() + A * N, pd + P_Z * n, n, S, pk.ext_k, pk.scratch(), st, true);
+    // numerator on the extended coset of advice | instance | permutation
+    // products | lookup products | permuted inputs | permuted tables | trash
+    const size_t C_INST = A, C_Z = C_INST + I, C_LZ = C_Z + S, C_LI = C_LZ + L, C_LT = C_LI + L, C_TR = C_LT + L,
+                 NCOL = C_TR + TR;
+    auto cols = new_vec(NCOL * N);
+    Fr *cv = cols->values.get();
+    to_extended(cv, pd + P_ADV * n, n, A, pk.ext_k, pk.scratch(), st, true);
+    if (I) to_extended(cv + C_INST * N, pd + P_INST * n, n, I, pk.ext_k, pk.scratch(), st, true);
+    to_extended(cv + C_Z * N, pd + P_Z * n, n, S + L, pk.ext_k, pk.scratch(), st, true);
+    for (size_t l = 0; l < L; ++l) {
+        to_extended(cv + (C_LI + l) * N, pd + (P_LP + 2 * l) * n, n, 1, pk.ext_k, pk.scratch(), st, true);
+        to_extended(cv + (C_LT + l) * N, pd + (P_LP + 2 * l + 1) * n, n, 1, pk.ext_k, pk.scratch(), st, true);
+    }
+    if (TR) to_extended(cv + C_TR * N, pd + P_TR * n, n, TR, pk.ext_k, pk.scratch(), st, true);
     MH_HIP(hipStreamSynchronize(st));
     void *hv = nullptr;
-    check(numerator_context_eval(&hv, pk.numctx, cols.get(), A + S, 0, A, 0, 0, nullptr, 0, raw(beta), raw(gamma),
-                                 raw(theta), raw(trash), raw(y), A, S, 0, 0, 0, 0, 0, 0, true, true));
+    check(numerator_context_eval(&hv, pk.numctx, cols.get(), NCOL, 0, A, C_INST, I, nullptr, 0, raw(beta),
+                                 raw(gamma), raw(theta), raw(trash), raw(y), C_Z, S, C_LZ, C_LI, C_LT, L, C_TR, TR,
+                                 true, true));
     cols.reset();
     std::unique_ptr<DeviceVec> h(reinterpret_cast<DeviceVec *>(hv));
     // h = numerator / (X^n - 1) on the coset, then coefficients
@@ -683,10 +815,12 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     };
     // every evaluation the transcript takes after x, in its order, computed
     // by one launch and read back once: advice and fixed queries, the random
-    // polynomial, the sigmas, then per product set z(x), z(x w) and (all but
-    // the last set) z(x w^-(bf+1))
+    // polynomial, the sigmas, per product set z(x), z(x w) and (all but the
+    // last set) z(x w^-(bf+1)), per lookup its product at x and x w, permuted
+    // input at x and x w^-1 and permuted table at x (lookup/prover.rs:
+    // 1341-1525), per trashcan its polynomial at x
     const Fr *fixed = pk.fixed_coeff->values.get();
-    const HostFr x_next = rotate(pk, x, 1), x_last = rotate(pk, x, -(int32_t)(pk.bf + 1));
+    const HostFr x_next = rotate(pk, x, 1), x_last = rotate(pk, x, -(int32_t)(pk.bf + 1)), x_prev = rotate(pk, x, -1);
     {
         std::vector<HostFr> epts;
         std::vector<const Fr *> epolys;
@@ -708,6 +842,14 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
             add(pd + (P_Z + s) * n, x_next);
             if (s + 1 < S) add(pd + (P_Z + s) * n, x_last);
         }
+        for (size_t l = 0; l < L; ++l) {
+            add(pd + (P_LZ + l) * n, x);
+            add(pd + (P_LZ + l) * n, x_next);
+            add(pd + (P_LP + 2 * l) * n, x);
+            add(pd + (P_LP + 2 * l) * n, x_prev);
+            add(pd + (P_LP + 2 * l + 1) * n, x);
+        }
+        for (size_t t = 0; t < TR; ++t) add(pd + (P_TR + t) * n, x);
         std::vector<HostFr> ev(qpoly.size());
         poly_eval_queries(ev.data(), epolys, n, qpoly.data(), qpt.data(), qpoly.size(), epts.data(), epts.size(),
                           st);
@@ -736,6 +878,14 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
         queries.push_back({pd + (P_Z + s) * n, x_next});
     }
     for (size_t s = S - 1; s-- > 0;) queries.push_back({pd + (P_Z + s) * n, x_last});
+    for (size_t l = 0; l < L; ++l) {   // lookup/prover.rs:1527-1572
+        queries.push_back({pd + (P_LZ + l) * n, x});
+        queries.push_back({pd + (P_LP + 2 * l) * n, x});
+        queries.push_back({pd + (P_LP + 2 * l + 1) * n, x});
+        queries.push_back({pd + (P_LP + 2 * l) * n, x_prev});
+        queries.push_back({pd + (P_LZ + l) * n, x_next});
+    }
+    for (size_t t = 0; t < TR; ++t) queries.push_back({pd + (P_TR + t) * n, x});
     for (auto &q : pk.fixed_queries) queries.push_back({fixed + (size_t)q.first * n, rotate(pk, x, q.second)});
     for (size_t j = 0; j < pk.perm_cols; ++j) queries.push_back({pk.sigma_coeff->values.get() + j * n, x});
     queries.push_back({hpoly->values.get(), x});
@@ -843,21 +993,34 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t see
     return T.proof;
 }
 
+// A proof with no instance values (every instance column of the key empty).
+std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t seed)
+{
+    return plonk_prove(pk, advice, nullptr, nullptr, seed);
+}
+
 }  // namespace mh
 
 using namespace mh;
 
-extern "C" RustError midnight_hip_plonk_keygen(void **out, uint32_t k, void *srs_context, size_t srs_points,
-                                               uint32_t degree, const RawFr *fixed, size_t fixed_cols,
-                                               size_t advice_cols, const uint32_t *advice_queries,
-                                               size_t advice_query_count, const uint32_t *fixed_queries,
-                                               size_t fixed_query_count, const RawColumnRef *perm_columns,
-                                               size_t perm_cols, const uint32_t *perm_mapping, RawGraph gate,
-                                               const char *description)
+extern "C" RustError midnight_hip_plonk_keygen_with_arguments(
+    void **out, uint32_t k, void *srs_context, size_t srs_points, uint32_t degree, const RawFr *fixed,
+    size_t fixed_cols, size_t advice_cols, const uint32_t *advice_queries, size_t advice_query_count,
+    const uint32_t *fixed_queries, size_t fixed_query_count, const RawColumnRef *perm_columns, size_t perm_cols,
+    const uint32_t *perm_mapping, RawGraph gate, const char *description, const RawPlonkArguments *arguments)
 {
     if (!out || !srs_context || k < 3 || k > 26 || (fixed_cols && !fixed) || !perm_columns || !perm_mapping ||
         (advice_query_count && !advice_queries) || (fixed_query_count && !fixed_queries))
         return make_error((int)hipErrorInvalidValue, "invalid keygen arguments");
+    RawPlonkArguments args{};
+    if (arguments) args = *arguments;
+    if ((args.lookup_count &&
+         (!args.lookup_input_graphs || !args.lookup_table_graphs || !args.lookup_graphs)) ||
+        (args.trash_count && (!args.trash_graphs || !args.trash_selectors)))
+        return make_error((int)hipErrorInvalidValue, "invalid keygen arguments");
+    for (size_t t = 0; t < args.trash_count; ++t)
+        if (args.trash_selectors[t] >= fixed_cols)
+            return make_error((int)hipErrorInvalidValue, "trash selector column out of range");
     *out = nullptr;
     MH_TRY
     std::vector<std::pair<uint32_t, int32_t>> aq, fq;
@@ -870,10 +1033,24 @@ extern "C" RustError midnight_hip_plonk_keygen(void **out, uint32_t k, void *srs
         fq.emplace_back(fixed_queries[2 * i], (int32_t)fixed_queries[2 * i + 1]);
     }
     auto pk = plonk_keygen(k, srs_context, srs_points, degree, fixed, fixed_cols, advice_cols, aq, fq, perm_columns,
-                           perm_cols, perm_mapping, gate, description ? description : "");
+                           perm_cols, perm_mapping, gate, description ? description : "", args);
     *out = pk.release();
     return ok();
     MH_CATCH
+}
+
+extern "C" RustError midnight_hip_plonk_keygen(void **out, uint32_t k, void *srs_context, size_t srs_points,
+                                               uint32_t degree, const RawFr *fixed, size_t fixed_cols,
+                                               size_t advice_cols, const uint32_t *advice_queries,
+                                               size_t advice_query_count, const uint32_t *fixed_queries,
+                                               size_t fixed_query_count, const RawColumnRef *perm_columns,
+                                               size_t perm_cols, const uint32_t *perm_mapping, RawGraph gate,
+                                               const char *description)
+{
+    return midnight_hip_plonk_keygen_with_arguments(out, k, srs_context, srs_points, degree, fixed, fixed_cols,
+                                                    advice_cols, advice_queries, advice_query_count, fixed_queries,
+                                                    fixed_query_count, perm_columns, perm_cols, perm_mapping, gate,
+                                                    description, nullptr);
 }
 
 extern "C" void midnight_hip_plonk_key_free(void *pk)
@@ -898,23 +1075,40 @@ extern "C" RustError midnight_hip_plonk_vk(const void *pk, RawG1Projective *fixe
     return ok();
 }
 
-extern "C" RustError midnight_hip_plonk_prove(void *pk, const RawFr *advice, size_t advice_len, uint64_t rng_seed,
-                                              uint8_t *proof, size_t proof_capacity, size_t *proof_len)
+extern "C" RustError midnight_hip_plonk_prove_with_instances(void *pk, const RawFr *advice, size_t advice_len,
+                                                             const RawFr *instance, const size_t *instance_lens,
+                                                             size_t instance_cols, uint64_t rng_seed, uint8_t *proof,
+                                                             size_t proof_capacity, size_t *proof_len)
 {
     if (!pk || !advice || !proof_len) return make_error((int)hipErrorInvalidValue, "invalid prove arguments");
     auto *p = reinterpret_cast<PlonkKey *>(pk);
     if (advice_len != p->advice_cols * p->n)
         return make_error((int)hipErrorInvalidValue, "advice length does not match the key");
+    if (instance_cols != p->instance_cols || (instance_cols && !instance_lens))
+        return make_error((int)hipErrorInvalidValue, "instance columns do not match the key");
+    size_t total = 0;
+    for (size_t c = 0; c < instance_cols; ++c) total += instance_lens[c];
+    if (total && !instance) return make_error((int)hipErrorInvalidValue, "null instance values");
     MH_TRY
     // the length is fixed by the key: a short buffer is rejected before any work
     const size_t want = plonk_proof_size(*p);
     *proof_len = want;
     if (!proof || proof_capacity < want) return make_error((int)hipErrorInvalidValue, "proof buffer too small");
-    const std::vector<uint8_t> bytes = plonk_prove(*p, advice, rng_seed);
+    const std::vector<uint8_t> bytes = plonk_prove(*p, advice, instance, instance_lens, rng_seed);
     MH_CHECK(bytes.size() == want, -1, "proof length differs from the key's proof size");
     std::memcpy(proof, bytes.data(), bytes.size());
     return ok();
     MH_CATCH
+}
+
+extern "C" RustError midnight_hip_plonk_prove(void *pk, const RawFr *advice, size_t advice_len, uint64_t rng_seed,
+                                              uint8_t *proof, size_t proof_capacity, size_t *proof_len)
+{
+    if (!pk) return make_error((int)hipErrorInvalidValue, "invalid prove arguments");
+    const size_t I = reinterpret_cast<const PlonkKey *>(pk)->instance_cols;
+    std::vector<size_t> empty(I, 0);
+    return midnight_hip_plonk_prove_with_instances(pk, advice, advice_len, nullptr, I ? empty.data() : nullptr, I,
+                                                   rng_seed, proof, proof_capacity, proof_len);
 }
 
 extern "C" RustError midnight_hip_plonk_set_rng_threads(void *pk, uint32_t threads)

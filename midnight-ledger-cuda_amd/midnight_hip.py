@@ -197,6 +197,11 @@ def _declare(L):
         "midnight_hip_plonk_keygen": (
             E, [vpp, u32, vp, c_size, u32, vp, c_size, c_size, vp, c_size, vp, c_size, vp, c_size, vp, RawGraph,
                 ctypes.c_char_p]),
+        "midnight_hip_plonk_keygen_with_arguments": (
+            E, [vpp, u32, vp, c_size, u32, vp, c_size, c_size, vp, c_size, vp, c_size, vp, c_size, vp, RawGraph,
+                ctypes.c_char_p, vp]),
+        "midnight_hip_plonk_prove_with_instances": (
+            E, [vp, u64p, c_size, vp, vp, c_size, ctypes.c_uint64, ctypes.c_char_p, c_size, ctypes.POINTER(c_size)]),
         "midnight_hip_plonk_key_free": (None, [vp]),
         "midnight_hip_plonk_vk": (E, [vp, u64p, u64p, u64p]),
         "midnight_hip_plonk_prove": (E, [vp, u64p, c_size, ctypes.c_uint64, ctypes.c_char_p, c_size,
@@ -1113,12 +1118,24 @@ class Srs:
 
 # ---------------------------------------------------------------- proof pipeline
 
+class RawPlonkArguments(ctypes.Structure):
+    """include/midnight_hip.h RawPlonkArguments"""
+    _fields_ = [("instance_cols", ctypes.c_size_t), ("lookup_count", ctypes.c_size_t),
+                ("lookup_input_graphs", ctypes.c_void_p), ("lookup_table_graphs", ctypes.c_void_p),
+                ("lookup_graphs", ctypes.c_void_p), ("trash_count", ctypes.c_size_t),
+                ("trash_graphs", ctypes.c_void_p), ("trash_selectors", ctypes.c_void_p)]
+
+
 class PlonkKey:
-    """Proving key of a gates + copy-permutation circuit, resident on the
-    device (midnight_hip_plonk_keygen); proofs with prove()."""
+    """Proving key of a PLONK circuit (custom gates, copy permutation and
+    optionally lookups, trashcans and instance columns), resident on the
+    device (midnight_hip_plonk_keygen_with_arguments); proofs with prove().
+
+    lookups: list of (input compression graph, table compression graph,
+    numerator lookup graph); trash: list of (graph, selector fixed column)."""
 
     def __init__(self, k, srs, degree, fixed, advice_cols, advice_queries, fixed_queries, perm_columns,
-                 perm_mapping, gate, description=b""):
+                 perm_mapping, gate, description=b"", instance_cols=0, lookups=(), trash=()):
         n = 1 << k
         fx = _fr_array(fixed) if fixed is not None and len(fixed) else None
         aq = np.ascontiguousarray([(c, r & 0xFFFFFFFF) for c, r in advice_queries], dtype=np.uint32).reshape(-1)
@@ -1127,10 +1144,19 @@ class PlonkKey:
         mp = np.ascontiguousarray(perm_mapping, dtype=np.uint32)
         assert mp.size == 2 * len(perm_columns) * n
         h = ctypes.c_void_p()
-        check(lib().midnight_hip_plonk_keygen(
+        lookups, trash = list(lookups), list(trash)
+        li_arr, li_ptr = _graphs([lk[0] for lk in lookups])
+        lt_arr, lt_ptr = _graphs([lk[1] for lk in lookups])
+        lg_arr, lg_ptr = _graphs([lk[2] for lk in lookups])
+        tg_arr, tg_ptr = _graphs([t[0] for t in trash])
+        tsel = np.ascontiguousarray([t[1] for t in trash], dtype=np.uint32)
+        args = RawPlonkArguments(instance_cols, len(lookups), li_ptr, lt_ptr, lg_ptr, len(trash), tg_ptr,
+                                 tsel.ctypes.data if len(trash) else None)
+        check(lib().midnight_hip_plonk_keygen_with_arguments(
             ctypes.byref(h), k, srs.handle, srs.npoints, degree, _ptr(fx), 0 if fx is None else fx.shape[0] // n,
             advice_cols, _ptr(aq), len(advice_queries), _ptr(fq), len(fixed_queries), col_ptr, len(perm_columns),
-            mp.ctypes.data, gate.as_raw(), description))
+            mp.ctypes.data, gate.as_raw(), description, ctypes.byref(args)))
+        self.instance_cols = instance_cols
         self.handle = h
         self.k, self.n = k, n
         self.advice_cols = advice_cols
@@ -1155,13 +1181,24 @@ class PlonkKey:
         check(lib().midnight_hip_plonk_proof_size(self.handle, ctypes.byref(ln)))
         return ln.value
 
-    def prove(self, advice, seed, capacity=None):
+    def prove(self, advice, seed, capacity=None, instance=None):
+        """instance: list of instance_cols arrays of Montgomery values (or
+        None: every column empty)"""
         a = _fr_array(advice)
         if capacity is None:
             capacity = self.proof_size()
         buf = ctypes.create_string_buffer(max(capacity, 1))
         ln = c_size()
-        check(lib().midnight_hip_plonk_prove(self.handle, P(a), a.shape[0], seed, buf, capacity, ctypes.byref(ln)))
+        if instance is None and self.instance_cols == 0:
+            check(lib().midnight_hip_plonk_prove(self.handle, P(a), a.shape[0], seed, buf, capacity,
+                                                 ctypes.byref(ln)))
+            return buf.raw[:ln.value]
+        cols = [np.ascontiguousarray(c, dtype=np.uint64).reshape(-1, 4) for c in (instance or [])]
+        lens = np.ascontiguousarray([c.shape[0] for c in cols] or [0] * self.instance_cols, dtype=np.uint64)
+        vals = np.concatenate(cols) if cols and sum(c.shape[0] for c in cols) else None
+        check(lib().midnight_hip_plonk_prove_with_instances(
+            self.handle, P(a), a.shape[0], None if vals is None else vals.ctypes.data,
+            lens.ctypes.data if len(lens) else None, len(lens), seed, buf, capacity, ctypes.byref(ln)))
         return buf.raw[:ln.value]
 
     def free(self):

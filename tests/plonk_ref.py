@@ -114,6 +114,32 @@ class Transcript:
 
 # ------------------------------------------------------------------ keygen
 
+class Spec:
+    """The constraint system of a circuit family: column counts, queries,
+    gate, lookup and trash expressions, permuted columns, key description."""
+
+    def __init__(self, args=False):
+        if args:
+            self.advice, self.fixed, self.instance = S.ADVICE_X, S.FIXED_X, S.INSTANCE_X
+            self.perm = S.PERM_COLUMNS_X
+            self.aq, self.fq, self.iq = S.ADVICE_QUERIES_X, S.FIXED_QUERIES_X, S.INSTANCE_QUERIES_X
+            self.gates = S.gate_expressions_x()
+            self.lookups = S.lookup_expressions_x()
+            self.trash = S.trash_expressions_x()
+            self.description = S.DESCRIPTION_X
+        else:
+            self.advice, self.fixed, self.instance = S.ADVICE, S.FIXED, 0
+            self.perm = S.PERM_COLUMNS
+            self.aq, self.fq, self.iq = S.ADVICE_QUERIES, S.FIXED_QUERIES, []
+            self.gates = [S.gate_expression()]
+            self.lookups, self.trash = [], []
+            self.description = S.DESCRIPTION
+
+    @staticmethod
+    def of(circuit):
+        return Spec(isinstance(circuit, S.ArgCircuit))
+
+
 class CpuKey:
     def __init__(self, circuit, tau_int, threads=None):
         c = circuit
@@ -131,7 +157,8 @@ class CpuKey:
         self.zeta = S.fr_const("zeta")
         self.delta = S.fr_const("delta")
         m = lambda col: col if isinstance(col, np.ndarray) else S.mont(col)
-        self.fixed_lag = [m(col) for col in c.cols[5:]]
+        self.spec = Spec.of(c)
+        self.fixed_lag = [m(col) for col in getattr(c, "fixed_values", c.cols[5:])]
         self.sigma_lag = [m(col) for col in c.sigma]
         self.fixed_coeff = [O.ifft(v, k, threads) for v in self.fixed_lag]
         self.sigma_coeff = [O.ifft(v, k, threads) for v in self.sigma_lag]
@@ -147,8 +174,14 @@ class CpuKey:
             hv.update(len(coms).to_bytes(4, "little"))
             for p in coms:
                 hv.update(O.to_affine(p).astype("<u8").tobytes())
-        hv.update(S.DESCRIPTION)
+        hv.update(self.spec.description)
         self.repr = int.from_bytes(hv.digest(), "little") % R
+        sp = self.spec
+        self.custom_graph = G.RawGraphBuffers(G.custom_gates_graph(sp.gates))
+        self.compress_graphs = [(G.RawGraphBuffers(G.compression_graph(i)), G.RawGraphBuffers(G.compression_graph(t)))
+                                for i, t in sp.lookups]
+        self.lookup_graphs = [G.RawGraphBuffers(G.lookup_graph(i, t)) for i, t in sp.lookups]
+        self.trash_graphs = [G.RawGraphBuffers(G.trash_graph(cs)) for _, cs in sp.trash]
 
 
 class VerifierKey:
@@ -156,15 +189,16 @@ class VerifierKey:
     polynomials: sizes, domain constants and the SRS trapdoor (for the GPU
     keys of circuits too large for CpuKey)."""
 
-    def __init__(self, k, tau_int):
+    def __init__(self, k, tau_int, args=False):
         self.k, self.n = k, 1 << k
+        self.spec = Spec(args)
         self.tau = tau_int
         self.bf = S.BLINDING
         self.u = self.n - self.bf - 1
         self.pieces = S.DEGREE - 1
         self.omega = S.omega_int(k)
         self.delta = S.fr_const("delta")
-        self.nsets = (len(S.PERM_COLUMNS) + S.CHUNK - 1) // S.CHUNK
+        self.nsets = (len(self.spec.perm) + S.CHUNK - 1) // S.CHUNK
 
 
 # ------------------------------------------------------------------ prover
@@ -216,49 +250,122 @@ def sample_random(rng, n, rng_threads):
     return np.concatenate([O.ChaCha(seed_bytes=sd).random_fr(ln) for sd, (_, ln) in zip(seeds, chunks)])
 
 
-def cpu_prove(key, advice_cols, seed, threads=None, rng_threads=1):
-    """advice_cols: list of 5 lists of n ints (Lagrange values); rng_threads:
-    the thread count of the reference prover whose random draws to follow."""
-    c, n, k, A = key.c, key.n, key.k, S.ADVICE
-    Sn = c.nsets
+def _perm_columns(key, adv, inst_lag):
+    """the permuted columns' Lagrange values, in permutation order"""
+    n = key.n
+    out = []
+    for kind, idx in key.spec.perm:
+        if kind == 0:
+            out.append(key.fixed_lag[idx])
+        elif kind == 1:
+            out.append(adv[idx * n:(idx + 1) * n])
+        else:
+            out.append(inst_lag[idx * n:(idx + 1) * n])
+    return np.concatenate(out)
+
+
+def cpu_prove(key, advice_cols, seed, threads=None, rng_threads=1, instance=None):
+    """advice_cols: list of advice columns of n ints (Lagrange values), or
+    their Montgomery limbs column after column; instance: list of instance
+    columns (ints) of the key's instance count; rng_threads: the thread count
+    of the reference prover whose random draws to follow."""
+    c, n, k = key.c, key.n, key.k
+    sp = key.spec
+    A, L, TR, I = sp.advice, len(sp.lookups), len(sp.trash), sp.instance
+    Sn = (len(sp.perm) + S.CHUNK - 1) // S.CHUNK
+    u, bf = key.u, key.bf
     rng = O.ChaCha(seed=seed)
     if isinstance(advice_cols, np.ndarray):     # Montgomery limbs, column after column
         adv = np.array(advice_cols, dtype=np.uint64, copy=True).reshape(A * n, 4)
     else:
         adv = S.mont([v for col in advice_cols for v in col]).copy()
     for col in range(A):
-        adv[col * n + key.u:(col + 1) * n] = rng.random_fr(n - key.u)
-    perm_blinds = rng.random_fr(Sn * key.bf)
+        adv[col * n + u:(col + 1) * n] = rng.random_fr(n - u)
+    lk_blinds = [(rng.random_fr(n - u), rng.random_fr(n - u)) for _ in range(L)]
+    perm_blinds = rng.random_fr(Sn * bf)
+    lk_prod_blinds = rng.random_fr(L * bf) if L else None
     rand_poly = sample_random(rng, n, rng_threads)
     limb_blinds = rng.random_fr(key.pieces - 1)
 
     T = Transcript()
     T.common_scalar(key.repr)
+    # instance columns (compute_instances, prover.rs:1255-1286)
+    instance = instance if instance is not None else [[] for _ in range(I)]
+    assert len(instance) == I
+    inst_lag = None
+    if I:
+        vals = []
+        for col in instance:
+            assert len(col) <= u, "instance column longer than the usable rows"
+            T.common_scalar(len(col))
+            for v in col:
+                T.common_scalar(v % R)
+            vals += [v % R for v in col] + [0] * (n - len(col))
+        inst_lag = S.mont(vals)
+    fixed_all = np.concatenate(key.fixed_lag)
     adv_coeff = [O.ifft(adv[i * n:(i + 1) * n], k, threads) for i in range(A)]
     for p in adv_coeff:
         T.write_point(O.msm(key.srs, p, threads))
-    theta, beta, gamma = T.squeeze(), T.squeeze(), T.squeeze()
+    theta = T.squeeze()
 
-    cols = np.concatenate([adv] + key.fixed_lag)
-    z_lag = O.permutation_products(cols, np.concatenate(key.sigma_lag), n, S.CHUNK, to_fr(beta), to_fr(gamma),
-                                   to_fr(key.omega), S.mont(c.delta_pows), perm_blinds, key.bf, threads)
+    # lookups (lookup/prover.rs:737-1072, permute_expression_pair :1573-1690)
+    ci_all, ct_all, pin_all, ptab_all = [], [], [], []
+    for l in range(L):
+        gi, gt = key.compress_graphs[l]
+        ci = O.graph_rows(n, gi, theta=to_fr(theta), fixed=fixed_all, advice=adv, instance=inst_lag,
+                          threads=threads)
+        ct = O.graph_rows(n, gt, theta=to_fr(theta), fixed=fixed_all, advice=adv, instance=inst_lag,
+                          threads=threads)
+        perm = O.lookup_permute(ci, ct, u)
+        if perm is None:
+            raise ValueError("lookup input not in its table")
+        ci_all.append(ci)
+        ct_all.append(ct)
+        pin_all.append(np.concatenate([perm[0], lk_blinds[l][0]]))
+        ptab_all.append(np.concatenate([perm[1], lk_blinds[l][1]]))
+    lk_in_coeff = [O.ifft(v, k, threads) for v in pin_all]
+    lk_tab_coeff = [O.ifft(v, k, threads) for v in ptab_all]
+    for l in range(L):
+        T.write_point(O.msm(key.srs, lk_in_coeff[l], threads))
+        T.write_point(O.msm(key.srs, lk_tab_coeff[l], threads))
+    beta, gamma = T.squeeze(), T.squeeze()
+
+    z_lag = O.permutation_products(_perm_columns(key, adv, inst_lag), np.concatenate(key.sigma_lag), n, S.CHUNK,
+                                   to_fr(beta), to_fr(gamma), to_fr(key.omega),
+                                   S.mont([pow(key.delta, j, R) for j in range(len(sp.perm))]), perm_blinds, bf,
+                                   threads)
     z_coeff = [O.ifft(z_lag[s * n:(s + 1) * n], k, threads) for s in range(Sn)]
-    for p in z_coeff:
+    lz_coeff = []
+    if L:
+        lz = O.lookup_products(np.concatenate(ci_all), np.concatenate(ct_all), np.concatenate(pin_all),
+                               np.concatenate(ptab_all), n, L, to_fr(beta), to_fr(gamma), lk_prod_blinds, bf)
+        lz_coeff = [O.ifft(lz[l * n:(l + 1) * n], k, threads) for l in range(L)]
+    for p in z_coeff + lz_coeff:
         T.write_point(O.msm(key.srs, p, threads))
     trash = T.squeeze()
+    tr_coeff = []
+    for t in range(TR):
+        tv = O.graph_rows(n, key.trash_graphs[t], trash_challenge=to_fr(trash), fixed=fixed_all, advice=adv,
+                          instance=inst_lag, threads=threads)
+        tr_coeff.append(O.ifft(tv, k, threads))
+        T.write_point(O.msm(key.srs, tr_coeff[-1], threads))
     T.write_point(O.msm(key.srs, rand_poly, threads))
     y = T.squeeze()
 
     N = key.N
-    adv_ext = np.concatenate([O.coeff_to_extended(p, key.dom, threads) for p in adv_coeff])
-    z_ext = np.concatenate([O.coeff_to_extended(p, key.dom, threads) for p in z_coeff])
-    nu = O.numerator(N, fixed=key.fixed_ext, advice=adv_ext, beta=to_fr(beta), gamma=to_fr(gamma),
+    ext = lambda ps: np.concatenate([O.coeff_to_extended(p, key.dom, threads) for p in ps]) if ps else None
+    nu = O.numerator(N, fixed=key.fixed_ext, advice=ext(adv_coeff),
+                     instance=ext([O.ifft(inst_lag[i * n:(i + 1) * n], k, threads) for i in range(I)]),
+                     beta=to_fr(beta), gamma=to_fr(gamma),
                      theta=to_fr(theta), trash_challenge=to_fr(trash), y=to_fr(y),
                      omega=np.array(key.dom.fr("extended_omega")), g_coset=np.array(key.dom.fr("g_coset")),
-                     rot_scale=N // n, last_rotation=-(key.bf + 1), chunk_len=S.CHUNK, l0=key.l0_ext,
-                     l_last=key.l_last_ext, l_active=key.l_active_ext, permutation_products=z_ext,
-                     permutation_columns=S.PERM_COLUMNS, permutation_pk_cosets=key.sigma_ext,
-                     custom_graph=c.custom_graph(), threads=threads)
+                     rot_scale=N // n, last_rotation=-(bf + 1), chunk_len=S.CHUNK, l0=key.l0_ext,
+                     l_last=key.l_last_ext, l_active=key.l_active_ext, permutation_products=ext(z_coeff),
+                     permutation_columns=sp.perm, permutation_pk_cosets=key.sigma_ext,
+                     lookup_product_cosets=ext(lz_coeff), lookup_input_cosets=ext(lk_in_coeff),
+                     lookup_table_cosets=ext(lk_tab_coeff), lookup_graphs=key.lookup_graphs,
+                     trash_cosets=ext(tr_coeff), trash_selector_fixed_indices=[sel for sel, _ in sp.trash],
+                     trash_graphs=key.trash_graphs, custom_graph=key.custom_graph, threads=threads)
     # h truncated to (n - 1) pieces of n - 1 coefficients, each padded to n with
     # the blinding telescope (vanishing/prover.rs:155-182, 304-318)
     h = O.extended_to_coeff(O.mul_repeating(nu, key.t_evals), key.dom, threads)
@@ -273,28 +380,40 @@ def cpu_prove(key, advice_cols, seed, threads=None, rng_threads=1):
         T.write_point(O.msm(key.srs, p, threads))
     x = T.squeeze()
 
-    for col in range(A):
-        T.write_scalar(_eval(adv_coeff[col], x))
-    for p in key.fixed_coeff:
-        T.write_scalar(_eval(p, x))
+    for col, rot in sp.aq:
+        T.write_scalar(_eval(adv_coeff[col], _rot(key, x, rot)))
+    for col, rot in sp.fq:
+        T.write_scalar(_eval(key.fixed_coeff[col], _rot(key, x, rot)))
     T.write_scalar(_eval(rand_poly, x))
     for p in key.sigma_coeff:
         T.write_scalar(_eval(p, x))
-    x_next, x_last = _rot(key, x, 1), _rot(key, x, -(key.bf + 1))
+    x_next, x_last, x_prev = _rot(key, x, 1), _rot(key, x, -(bf + 1)), _rot(key, x, -1)
     for s in range(Sn):
         T.write_scalar(_eval(z_coeff[s], x))
         T.write_scalar(_eval(z_coeff[s], x_next))
         if s + 1 < Sn:
             T.write_scalar(_eval(z_coeff[s], x_last))
+    for l in range(L):      # lookup/prover.rs:1341-1525
+        T.write_scalar(_eval(lz_coeff[l], x))
+        T.write_scalar(_eval(lz_coeff[l], x_next))
+        T.write_scalar(_eval(lk_in_coeff[l], x))
+        T.write_scalar(_eval(lk_in_coeff[l], x_prev))
+        T.write_scalar(_eval(lk_tab_coeff[l], x))
+    for t in range(TR):
+        T.write_scalar(_eval(tr_coeff[t], x))
     sf = pow(x, n - 1, R)
     h_poly = _lincomb(h_pieces, [pow(sf, i, R) for i in range(key.pieces)], n)
 
-    queries = [(("adv", i), adv_coeff[i], x) for i in range(A)]
+    queries = [(("adv", col), adv_coeff[col], _rot(key, x, rot)) for col, rot in sp.aq]
     for s in range(Sn):
         queries += [(("z", s), z_coeff[s], x), (("z", s), z_coeff[s], x_next)]
     for s in range(Sn - 2, -1, -1):
         queries.append((("z", s), z_coeff[s], x_last))
-    queries += [(("fixed", i), p, x) for i, p in enumerate(key.fixed_coeff)]
+    for l in range(L):      # lookup/prover.rs:1527-1572
+        queries += [(("lz", l), lz_coeff[l], x), (("li", l), lk_in_coeff[l], x), (("lt", l), lk_tab_coeff[l], x),
+                    (("li", l), lk_in_coeff[l], x_prev), (("lz", l), lz_coeff[l], x_next)]
+    queries += [(("trash", t), tr_coeff[t], x) for t in range(TR)]
+    queries += [(("fixed", col), key.fixed_coeff[col], _rot(key, x, rot)) for col, rot in sp.fq]
     queries += [(("sigma", j), p, x) for j, p in enumerate(key.sigma_coeff)]
     queries += [(("h",), h_poly, x), (("rand",), rand_poly, x)]
     _multi_open_prove(T, key, queries, n, threads)
@@ -366,68 +485,134 @@ def _lagrange_at(key, x, i):
     return wi * (pow(x, n, R) - 1) * pow(n * (x - wi) % R, R - 2, R) % R
 
 
-def verify(vk, proof, key):
-    """vk: (fixed commitments, permutation commitments, transcript repr int).
-    Returns True when the proof verifies."""
+def _expr_at(e, ev, key):
+    """an expression tree (numerator_graph's tags) at x from the evaluations:
+    ev(kind, column, rotation)"""
+    tag = e[0]
+    if tag == "constant":
+        return e[1] % R
+    if tag in ("fixed", "advice", "instance"):
+        return ev(tag, e[1], e[2])
+    if tag == "neg":
+        return -_expr_at(e[1], ev, key) % R
+    if tag == "sum":
+        return (_expr_at(e[1], ev, key) + _expr_at(e[2], ev, key)) % R
+    if tag == "prod":
+        return _expr_at(e[1], ev, key) * _expr_at(e[2], ev, key) % R
+    if tag == "scaled":
+        return _expr_at(e[1], ev, key) * e[2] % R
+    raise ValueError("unknown expression tag %r" % (tag,))
+
+
+def verify(vk, proof, key, instance=None):
+    """vk: (fixed commitments, permutation commitments, transcript repr int);
+    instance: the proof's instance columns (ints).  Returns True when the
+    proof verifies."""
     fixed_com, sigma_com, repr_int = vk
-    n, A, Sn, bf = key.n, S.ADVICE, key.nsets, key.bf
+    sp = key.spec
+    n, A, bf = key.n, sp.advice, key.bf
+    L, TR, I = len(sp.lookups), len(sp.trash), sp.instance
+    Sn = (len(sp.perm) + S.CHUNK - 1) // S.CHUNK
+    instance = instance if instance is not None else [[] for _ in range(I)]
+    if len(instance) != I or any(len(col) > key.u for col in instance):
+        return False
     try:
         T = Transcript(proof)
         T.common_scalar(repr_int)
+        for col in instance:
+            T.common_scalar(len(col))
+            for v in col:
+                T.common_scalar(v % R)
         adv_com = [T.read_point() for _ in range(A)]
-        theta, beta, gamma = T.squeeze(), T.squeeze(), T.squeeze()
+        theta = T.squeeze()
+        lk_perm_com = [(T.read_point(), T.read_point()) for _ in range(L)]
+        beta, gamma = T.squeeze(), T.squeeze()
         z_com = [T.read_point() for _ in range(Sn)]
+        lz_com = [T.read_point() for _ in range(L)]
         trash = T.squeeze()
+        tr_com = [T.read_point() for _ in range(TR)]
         rand_com = T.read_point()
         y = T.squeeze()
         h_com = [T.read_point() for _ in range(key.pieces)]
         x = T.squeeze()
-        adv_ev = [T.read_scalar() for _ in range(A)]
-        fixed_ev = [T.read_scalar() for _ in range(S.FIXED)]
+        adv_ev = [T.read_scalar() for _ in sp.aq]
+        fixed_ev = [T.read_scalar() for _ in sp.fq]
         rand_ev = T.read_scalar()
-        sigma_ev = [T.read_scalar() for _ in range(len(S.PERM_COLUMNS))]
+        sigma_ev = [T.read_scalar() for _ in range(len(sp.perm))]
         z_ev = []
         for s in range(Sn):
             zx, zn = T.read_scalar(), T.read_scalar()
             zl = T.read_scalar() if s + 1 < Sn else None
             z_ev.append((zx, zn, zl))
+        lk_ev = [tuple(T.read_scalar() for _ in range(5)) for _ in range(L)]
+        tr_ev = [T.read_scalar() for _ in range(TR)]
     except ValueError:
         return False
 
-    # numerator at x from the evaluations
+    def ev(kind, col, rot):
+        if kind == "advice":
+            return adv_ev[sp.aq.index((col, rot))]
+        if kind == "fixed":
+            return fixed_ev[sp.fq.index((col, rot))]
+        pt = _rot(key, x, rot)      # instance: interpolated from the values
+        return sum(v * _lagrange_at(key, pt, i) for i, v in enumerate(instance[col])) % R
+
+    # numerator at x from the evaluations, folded in y as evaluate_numerator folds it
     l0 = _lagrange_at(key, x, 0)
     l_last = _lagrange_at(key, x, key.u)
     l_blind = sum(_lagrange_at(key, x, i) for i in range(n - bf, n)) % R
-    a, b, c, d = adv_ev[:4]
-    q = fixed_ev[0]
-    v = q * (a * b + c - d) % R
+    l_active = (1 - l_last - l_blind) % R
+    v = 0
+    for g in sp.gates:
+        v = (v * y + _expr_at(g, ev, key)) % R
     v = (v * y + (1 - z_ev[0][0]) * l0) % R
     zl = z_ev[-1][0]
     v = (v * y + (zl * zl - zl) * l_last) % R
     for s in range(1, Sn):
         v = (v * y + (z_ev[s][0] - z_ev[s - 1][2]) * l0) % R
-    col_ev = adv_ev + fixed_ev
+    col_ev = [ev({0: "fixed", 1: "advice", 2: "instance"}[kind], idx, 0) for kind, idx in sp.perm]
     cur_delta = beta * x % R
     for s in range(Sn):
         left, right = z_ev[s][1], z_ev[s][0]
-        for j in range(s * S.CHUNK, min((s + 1) * S.CHUNK, len(S.PERM_COLUMNS))):
+        for j in range(s * S.CHUNK, min((s + 1) * S.CHUNK, len(sp.perm))):
             left = left * (col_ev[j] + beta * sigma_ev[j] + gamma) % R
             right = right * (col_ev[j] + cur_delta + gamma) % R
             cur_delta = cur_delta * key.delta % R
-        v = (v * y + (left - right) * (1 - l_last - l_blind)) % R
+        v = (v * y + (left - right) * l_active) % R
+    for l, (inputs, tables) in enumerate(sp.lookups):       # lookup/verifier.rs
+        zx, zn, ai, ai_prev, si = lk_ev[l]
+        comp = lambda es: sum(_expr_at(e, ev, key) * pow(theta, len(es) - 1 - j, R) for j, e in enumerate(es)) % R
+        v = (v * y + (1 - zx) * l0) % R
+        v = (v * y + (zx * zx - zx) * l_last) % R
+        lhs = zn * (ai + beta) * (si + gamma)
+        rhs = zx * (comp(inputs) + beta) * (comp(tables) + gamma)
+        v = (v * y + (lhs - rhs) * l_active) % R
+        v = (v * y + (ai - si) * l0) % R
+        v = (v * y + (ai - si) * (ai - ai_prev) * l_active) % R
+    for t, (sel, cs) in enumerate(sp.trash):                # trash.rs Evaluated::expressions
+        comp = 0
+        for e in cs:
+            comp = (comp * trash + _expr_at(e, ev, key)) % R
+        v = (v * y + comp - (1 - ev("fixed", sel, 0)) * tr_ev[t]) % R
     h_ev = v * pow((pow(x, n, R) - 1) % R, R - 2, R) % R
 
     sf = pow(x, n - 1, R)
     h_comb = O.g1j()
     for i, cm in enumerate(h_com):
         h_comb = g_add(h_comb, g_mul(cm, pow(sf, i, R)))
-    x_next, x_last = _rot(key, x, 1), _rot(key, x, -(bf + 1))
-    queries = [(("adv", i), adv_com[i], x, adv_ev[i]) for i in range(A)]
+    x_next, x_last, x_prev = _rot(key, x, 1), _rot(key, x, -(bf + 1)), _rot(key, x, -1)
+    queries = [(("adv", col), adv_com[col], _rot(key, x, rot), adv_ev[i]) for i, (col, rot) in enumerate(sp.aq)]
     for s in range(Sn):
         queries += [(("z", s), z_com[s], x, z_ev[s][0]), (("z", s), z_com[s], x_next, z_ev[s][1])]
     for s in range(Sn - 2, -1, -1):
         queries.append((("z", s), z_com[s], x_last, z_ev[s][2]))
-    queries += [(("fixed", i), cm, x, fixed_ev[i]) for i, cm in enumerate(fixed_com)]
+    for l in range(L):
+        zx, zn, ai, ai_prev, si = lk_ev[l]
+        queries += [(("lz", l), lz_com[l], x, zx), (("li", l), lk_perm_com[l][0], x, ai),
+                    (("lt", l), lk_perm_com[l][1], x, si), (("li", l), lk_perm_com[l][0], x_prev, ai_prev),
+                    (("lz", l), lz_com[l], x_next, zn)]
+    queries += [(("trash", t), tr_com[t], x, tr_ev[t]) for t in range(TR)]
+    queries += [(("fixed", col), fixed_com[col], _rot(key, x, rot), fixed_ev[i]) for i, (col, rot) in enumerate(sp.fq)]
     queries += [(("sigma", j), cm, x, sigma_ev[j]) for j, cm in enumerate(sigma_com)]
     queries += [(("h",), h_comb, x, h_ev), (("rand",), rand_com, x, rand_ev)]
     return _multi_open_verify(T, key, queries)

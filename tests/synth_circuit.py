@@ -64,7 +64,7 @@ def gate_graph():
 
 
 class Circuit:
-    def __init__(self, k, seed=1):
+    def __init__(self, k, seed=1, reserved=()):
         rnd = random.Random(seed)
         self.k = k
         n = self.n = 1 << k
@@ -93,7 +93,7 @@ class Circuit:
         rnd.shuffle(free_targets)
         for _ in range(max(4, u // 3)):
             src = (rnd.randrange(4), rnd.randrange(u))
-            if src in used:
+            if src in used or src in reserved:
                 continue
             size = rnd.randrange(1, 4)
             members = [src]
@@ -115,6 +115,7 @@ class Circuit:
                 cycles.append(cyc)
         self.cols = cols
         self.cycles = cycles
+        self.rnd = rnd
 
         # sigma mapping; cells outside cycles map to themselves
         mapping = {}
@@ -313,6 +314,133 @@ def mapping(c):
     n = c.n
     m = np.zeros((len(PERM_COLUMNS), n, 2), dtype=np.uint32)
     m[:, :, 0] = np.arange(len(PERM_COLUMNS), dtype=np.uint32)[:, None]
+    m[:, :, 1] = np.arange(n, dtype=np.uint32)[None, :]
+    for cyc in c.cycles:
+        for t, (j, i) in enumerate(cyc):
+            jj, ii = cyc[(t + 1) % len(cyc)]
+            m[j, i] = (jj, ii)
+    return m.reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# The family with every argument of the prover: lookups, a trashcan and an
+# instance column next to the gate and the copy permutation.
+
+ADVICE_X = 7       # a b c d e f g
+FIXED_X = 7        # q s1 s2 t1 t2 qp qt
+INSTANCE_X = 1     # p
+PERM_COLUMNS_X = PERM_COLUMNS + [(2, 0)]
+ADVICE_QUERIES_X = [(c, 0) for c in range(ADVICE_X)]
+FIXED_QUERIES_X = [(c, 0) for c in range(FIXED_X)]
+INSTANCE_QUERIES_X = [(0, 0)]
+DESCRIPTION_X = (b"synthetic PLONK circuit with arguments: advice a b c d e f g, fixed q s1 s2 t1 t2 qp qt, "
+                 b"instance p, gates q*(a*b+c-d) and qp*(a-p), lookups (f, g) in (t1, t2) and e in s1, "
+                 b"trashcan qt: (d-a*b-c, e*(d-a*b-c)), copy permutation over a b c d e q s1 s2 p, degree 5")
+
+
+def _adv(i):
+    return ("advice", i, 0)
+
+
+def _fix(i):
+    return ("fixed", i, 0)
+
+
+def _sub(x, y):
+    return ("sum", x, ("neg", y))
+
+
+def gate_expressions_x():
+    a, b, c, d = (_adv(i) for i in range(4))
+    return [("prod", _fix(0), ("sum", ("sum", ("prod", a, b), c), ("neg", d))),
+            ("prod", _fix(5), _sub(a, ("instance", 0, 0)))]
+
+
+def lookup_expressions_x():
+    """(input expressions, table expressions) per lookup"""
+    return [([_adv(5), _adv(6)], [_fix(3), _fix(4)]),
+            ([("prod", _fix(0), _adv(4))], [_fix(1)])]
+
+
+def trash_expressions_x():
+    """(selector fixed column, constraint expressions) per trashcan"""
+    r = _sub(_sub(_adv(3), ("prod", _adv(0), _adv(1))), _adv(2))
+    return [(6, [r, ("prod", _adv(4), r)])]
+
+
+class ArgCircuit(Circuit):
+    """Circuit plus: advice f, g holding (t1, t2) table rows (lookup 0,
+    inputs (f, g) in table (t1, t2)); lookup 1, q * e in table s1 (e's free
+    cells copy s1 values); a trashcan with selector qt on odd usable rows
+    whose constraints vanish there; an instance column p of `m` public values
+    equal to a's first rows, tied to them by the gate qp * (a - p) and by
+    copy constraints (the instance column is the ninth permuted column)."""
+
+    def __init__(self, k, seed=1, m=6):
+        super().__init__(k, seed, reserved={(0, r) for r in range(m)})
+        rnd, n, u = self.rnd, self.n, self.u
+        rv = lambda: rnd.randrange(R)
+        a, b, c, d, e, q, s1, s2 = self.cols
+        # lookup 1 (q * e in s1): every usable row of e holds an s1 value.
+        # e's cells copied from a..d in cycles hold those values instead, so
+        # s1 takes them at the same rows (s1 itself is in cycles too: its
+        # copied cells already equal their source)
+        in_cycle = {cell for cyc in self.cycles for cell in cyc}
+        taken = set()     # rows of s1 whose value an e cell needs
+        for i in range(u):
+            if (4, i) not in in_cycle:
+                continue
+            if (6, i) not in in_cycle and i not in taken:
+                j = i
+            else:
+                j = next(r for r in range(u) if (6, r) not in in_cycle and r not in taken)
+            s1[j] = e[i]
+            taken.add(j)
+        for i in range(u):
+            if (4, i) not in in_cycle:
+                e[i] = s1[rnd.randrange(u)]
+        t1 = [rv() if i < u else 0 for i in range(n)]
+        t2 = [rv() if i < u else 0 for i in range(n)]
+        rows = [rnd.randrange(u) for _ in range(n)]
+        f = [t1[rows[i]] if i < u else rv() for i in range(n)]
+        g = [t2[rows[i]] if i < u else rv() for i in range(n)]
+        qp = [1 if i < m else 0 for i in range(n)]
+        qt = [1 if (i < u and i % 2 == 1) else 0 for i in range(n)]
+        self.instance = [[a[i] for i in range(m)]]
+        self.cols = [a, b, c, d, e, q, s1, s2]
+        self.advice_values = [a, b, c, d, e, f, g]
+        self.fixed_values = [q, s1, s2, t1, t2, qp, qt]
+        for r in range(m):
+            self.cycles.append([(0, r), (8, r)])
+        # sigma over the nine permuted columns; cells outside cycles map to themselves
+        mapping = {}
+        for cyc in self.cycles:
+            for t, cell in enumerate(cyc):
+                mapping[cell] = cyc[(t + 1) % len(cyc)]
+        ncol = len(PERM_COLUMNS_X)
+        self.delta_pows = [pow(self.delta, j, R) for j in range(ncol)]
+        self.sigma = [[self.delta_pows[mapping.get((j, i), (j, i))[0]] *
+                       self.omega_pows[mapping.get((j, i), (j, i))[1]] % R for i in range(n)]
+                      for j in range(ncol)]
+
+    def custom_graph(self):
+        return G.RawGraphBuffers(G.custom_gates_graph(gate_expressions_x()))
+
+    def lookup_graphs(self):
+        """(input compression, table compression, numerator graph) per lookup"""
+        return [(G.RawGraphBuffers(G.compression_graph(i)), G.RawGraphBuffers(G.compression_graph(t)),
+                 G.RawGraphBuffers(G.lookup_graph(i, t))) for i, t in lookup_expressions_x()]
+
+    def trash_graphs(self):
+        """(graph, selector fixed column) per trashcan"""
+        return [(G.RawGraphBuffers(G.trash_graph(cs)), sel) for sel, cs in trash_expressions_x()]
+
+
+def mapping_x(c):
+    """mapping() over the nine permuted columns of ArgCircuit"""
+    n = c.n
+    m = np.zeros((len(PERM_COLUMNS_X), n, 2), dtype=np.uint32)
+    m[:, :, 0] = np.arange(len(PERM_COLUMNS_X), dtype=np.uint32)[:, None]
     m[:, :, 1] = np.arange(n, dtype=np.uint32)[None, :]
     for cyc in c.cycles:
         for t, (j, i) in enumerate(cyc):

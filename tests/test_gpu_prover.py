@@ -171,3 +171,71 @@ def test_gpu_proof_graph_replays_match_eager(H, O):
         H.set_lane(0)
     for x in (pk, srs, pk2, srs2):
         x.free()
+
+
+def gpu_arg_key(H, O, c, tau_int, extra_srs=0):
+    """the GPU key of an ArgCircuit (lookups, trashcan, instance column)"""
+    n = c.n
+    powers = H.fr_shifted_prefix_product(np.tile(PR.to_fr(tau_int), (n + extra_srs, 1)), O.fr_from_int(1))
+    dv = H.DeviceVec.upload(powers)
+    srs = H.MsmContext.from_multiples(dv)
+    dv.free()
+    fixed = S.mont([v for col in c.fixed_values for v in col])
+    pk = H.PlonkKey(c.k, srs, S.DEGREE, fixed, S.ADVICE_X, S.ADVICE_QUERIES_X, S.FIXED_QUERIES_X,
+                    S.PERM_COLUMNS_X, S.mapping_x(c), c.custom_graph(), S.DESCRIPTION_X,
+                    instance_cols=S.INSTANCE_X, lookups=c.lookup_graphs(), trash=c.trash_graphs())
+    return pk, srs
+
+
+@pytest.mark.parametrize("k,seed", [(7, 5), (10, 8), (12, 2)])
+def test_gpu_proof_with_lookups_trash_instance(H, O, k, seed):
+    """lookups (two, one of two expressions), a trashcan and an instance
+    column tied to advice by a gate and copy constraints: GPU proof bytes
+    equal to the CPU prover's, accepted by the verifier with the instance and
+    rejected with other instance values; graph replays equal eager proofs."""
+    c = S.ArgCircuit(k, seed=seed)
+    tau = 1357911 + k
+    key = PR.CpuKey(c, tau)
+    pk, srs = gpu_arg_key(H, O, c, tau)
+    fc, pc, rp = pk.vk()
+    assert PR.to_int(rp) == key.repr
+    adv = S.mont([v for col in c.advice_values for v in col])
+    inst = [S.mont(col) for col in c.instance]
+    print("arg proof k=%d" % k, flush=True)
+    gpu = pk.prove(adv, seed, instance=inst)
+    assert len(gpu) == pk.proof_size()
+    cpu = PR.cpu_prove(key, c.advice_values, seed, instance=c.instance)
+    assert gpu == cpu
+    vk = (fc, pc, PR.to_int(rp))
+    assert PR.verify(vk, gpu, key, c.instance)
+    assert not PR.verify(vk, gpu, key, [[(v + 1) % S.R for v in c.instance[0]]])
+    assert pk.prove(adv, seed, instance=inst) == gpu          # replayed stages
+    pk.set_graphs(False)
+    assert pk.prove(adv, seed, instance=inst) == gpu          # eager
+    pk.free()
+    srs.free()
+
+
+def test_gpu_proof_lookup_input_missing(H, O):
+    """a lookup input absent from its table fails the proof with code -1
+    (permute_expression_pair's ConstraintSystemFailure); instance arguments
+    that do not fit the key are rejected before any work"""
+    c = S.ArgCircuit(7, seed=6)
+    pk, srs = gpu_arg_key(H, O, c, 4242)
+    adv_cols = [list(x) for x in c.advice_values]
+    adv_cols[6][3] = (adv_cols[6][3] + 1) % S.R
+    adv = S.mont([v for col in adv_cols for v in col])
+    inst = [S.mont(col) for col in c.instance]
+    with pytest.raises(H.MidnightHipError) as e:
+        pk.prove(adv, 1, instance=inst)
+    assert e.value.code == -1
+    good = S.mont([v for col in c.advice_values for v in col])
+    with pytest.raises(H.MidnightHipError):
+        pk.prove(good, 1, instance=inst + inst)                  # two columns for a key of one
+    with pytest.raises(H.MidnightHipError):
+        pk.prove(good, 1, instance=[S.mont(list(range(c.u + 1)))])   # longer than the usable rows
+    # the key still proves after the failures
+    key = PR.CpuKey(c, 4242)
+    assert pk.prove(good, 1, instance=inst) == PR.cpu_prove(key, c.advice_values, 1, instance=c.instance)
+    pk.free()
+    srs.free()

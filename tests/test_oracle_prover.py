@@ -176,3 +176,72 @@ def test_cpu_prover_on_fast_circuit():
     row = int(np.nonzero((fc.mapping.reshape(8, fc.n, 2)[4, :, 0] == 0))[0][0])   # a copied cell of e
     bad[4, row] = O.fr_op("add", bad[4, row], O.fr_from_int(1))
     assert not PR.verify(vk, PR.cpu_prove(key, bad.reshape(-1, 4), 4), key)
+
+
+# ---- lookups, a trashcan and an instance column (plonk/lookup, plonk/trash,
+# compute_instances of plonk/prover.rs:1237-1330)
+
+@pytest.fixture(scope="module")
+def arg_setup():
+    c = S.ArgCircuit(7, seed=3)
+    key = PR.CpuKey(c, 24681357)
+    return c, key
+
+
+def test_arg_circuit_satisfies_its_constraints(arg_setup):
+    """the witness the verifier's numerator is built against: gates, copy
+    cycles (instance cells included), both lookups and the trashcan hold on
+    every usable row"""
+    c, _ = arg_setup
+    a, b, cc, d, e, f, g = c.advice_values
+    q, s1, s2, t1, t2, qp, qt = c.fixed_values
+    p = c.instance[0] + [0] * (c.n - len(c.instance[0]))
+    cols = c.cols + [p]
+    for i in range(c.u):
+        assert q[i] * (a[i] * b[i] + cc[i] - d[i]) % R == 0
+        assert qp[i] * (a[i] - p[i]) % R == 0
+        assert qt[i] * (d[i] - a[i] * b[i] - cc[i]) % R == 0
+    for cyc in c.cycles:
+        assert len({cols[j][i] for j, i in cyc}) == 1
+    table0 = {(t1[i], t2[i]) for i in range(c.u)}
+    assert all((f[i], g[i]) in table0 for i in range(c.u))
+    table1 = set(s1[:c.u])
+    assert all(q[i] * e[i] % R in table1 for i in range(c.u))
+
+
+def test_arg_circuit_cpu_proof_verifies(arg_setup):
+    c, key = arg_setup
+    proof = PR.cpu_prove(key, c.advice_values, 42, instance=c.instance)
+    A, F, L, TR, Sn = S.ADVICE_X, S.FIXED_X, 2, 1, 3
+    # commitments: advice, permuted pairs, permutation sets, lookup products,
+    # trash, random, pieces, f, pi; evaluations: advice, fixed, random,
+    # sigmas, sets, lookups, trash, then one q(x3) per point set
+    points = A + 2 * L + Sn + L + TR + 1 + 4 + 2
+    scalars = A + F + 1 + 9 + (3 * Sn - 1) + 5 * L + TR
+    assert (len(proof) - 48 * points - 32 * scalars) % 32 == 0
+    vk = (key.fixed_com, key.sigma_com, key.repr)
+    assert PR.verify(vk, proof, key, c.instance)
+    assert PR.cpu_prove(key, c.advice_values, 42, instance=c.instance) == proof
+    # other instance values: rejected (the gate qp (a - p) and the copies fail)
+    other = [[(c.instance[0][0] + 1) % R] + c.instance[0][1:]]
+    assert not PR.verify(vk, proof, key, other)
+    assert not PR.verify(vk, proof, key, [c.instance[0][:-1]])
+    for pos in (10, 48 * 8 + 3, len(proof) - 60):
+        bad = bytearray(proof)
+        bad[pos] ^= 1
+        assert not PR.verify(vk, bytes(bad), key, c.instance), pos
+
+
+def test_arg_circuit_wrong_witness(arg_setup):
+    c, key = arg_setup
+    vk = (key.fixed_com, key.sigma_com, key.repr)
+    adv = [list(x) for x in c.advice_values]
+    adv[5][4] = (adv[5][4] + 1) % R             # lookup input (f, g) not in (t1, t2)
+    with pytest.raises(ValueError):
+        PR.cpu_prove(key, adv, 42, instance=c.instance)
+    adv = [list(x) for x in c.advice_values]
+    adv[3][5] = (adv[3][5] + 1) % R             # gate and trashcan (row 5: qt = 1) fail
+    assert not PR.verify(vk, PR.cpu_prove(key, adv, 42, instance=c.instance), key, c.instance)
+    # the instance passed to the prover differs from the cells it is tied to
+    wrong = [[(v + 1) % R for v in c.instance[0]]]
+    assert not PR.verify(vk, PR.cpu_prove(key, c.advice_values, 42, instance=wrong), key, wrong)
