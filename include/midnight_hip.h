@@ -594,9 +594,14 @@ RustError midnight_hip_plonk_keygen(void **out, uint32_t k, void *srs_context, s
                                     size_t perm_cols, const uint32_t *perm_mapping, RawGraph gate,
                                     const char *description);
 /* The arguments of a key beyond gates and the copy permutation.
- * instance_cols: non-committed instance columns (compute_instances,
- * plonk/prover.rs:1237-1330: their lengths and values go into the transcript;
- * they are neither committed nor opened).  Per lookup
+ * instance_cols: instance columns (compute_instances,
+ * plonk/prover.rs:1237-1330), the first committed_instances of them
+ * committed (nb_committed_instances: the commitment of each such column's
+ * Lagrange polynomial goes into the transcript, its instance queries are
+ * evaluated after x and opened; instance_queries lists (column, rotation
+ * as int32 bits) pairs in the constraint system's order), the others
+ * non-committed (their lengths and values go into the transcript; neither
+ * committed nor opened).  Per lookup
  * (plonk/lookup/prover.rs): the compression of its input expressions and of
  * its table expressions (Horner in theta from zero, evaluation.rs:2743-2766)
  * and the numerator's lookup graph (input + beta)(table + gamma)
@@ -605,6 +610,9 @@ RustError midnight_hip_plonk_keygen(void **out, uint32_t k, void *srs_context, s
  * column of its selector.  Graphs read advice at rotations the key queries. */
 typedef struct RawPlonkArguments {
     size_t instance_cols;
+    size_t committed_instances;
+    const uint32_t *instance_queries;
+    size_t instance_query_count;
     size_t lookup_count;
     const RawGraph *lookup_input_graphs;
     const RawGraph *lookup_table_graphs;

@@ -109,6 +109,12 @@ public:
         fr_to_bytes(f, b);
         absorb(b, 32);
     }
+    void common_point(const RawG1Projective &p)
+    {
+        uint8_t b[48];
+        g1_compress(p, b);
+        absorb(b, 48);
+    }
     void write_scalar(const HostFr &f)
     {
         uint8_t b[32];
@@ -249,7 +255,8 @@ struct PlonkKey {
     uint32_t k = 0, ext_k = 0, degree = 0;
     size_t n = 0, ext_n = 0, bf = 0, u = 0;
     size_t fixed_cols = 0, advice_cols = 0, perm_cols = 0, chunk = 0, sets = 0, pieces = 0;
-    size_t instance_cols = 0, lookups = 0, trashes = 0;
+    size_t instance_cols = 0, committed = 0, lookups = 0, trashes = 0;
+    std::vector<std::pair<uint32_t, int32_t>> committed_queries;   // instance queries of committed columns
     void *srs = nullptr;                                   // MSM context (not owned), >= n points
     std::vector<std::pair<uint32_t, int32_t>> advice_queries, fixed_queries;
     std::vector<RawColumnRef> perm_columns;
@@ -392,6 +399,10 @@ std::unique_ptr<PlonkKey> plonk_keygen(uint32_t k, void *srs, size_t srs_points,
     pk->advice_cols = advice_cols;
     pk->perm_cols = perm_cols;
     pk->instance_cols = args.instance_cols;
+    pk->committed = args.committed_instances;
+    for (size_t i = 0; i < args.instance_query_count; ++i)
+        if (args.instance_queries[2 * i] < args.committed_instances)
+            pk->committed_queries.emplace_back(args.instance_queries[2 * i], (int32_t)args.instance_queries[2 * i + 1]);
     pk->lookups = args.lookup_count;
     pk->trashes = args.trash_count;
     for (size_t j = 0; j < perm_cols; ++j)
@@ -532,8 +543,10 @@ size_t plonk_proof_size(const PlonkKey &pk)
     // trash polynomials, fixed columns, sigmas, h, the random polynomial
     const long z0 = (long)A, lz0 = z0 + (long)S, li0 = lz0 + (long)L, lt0 = li0 + (long)L, tr0 = lt0 + (long)L;
     const long fixed0 = tr0 + (long)TR, sigma0 = fixed0 + (long)pk.fixed_cols, hid = sigma0 + (long)pk.perm_cols;
+    const long inst0 = hid + 2;
     std::vector<std::pair<long, int32_t>> q;
     for (auto &a : pk.advice_queries) q.emplace_back((long)a.first, a.second);
+    for (auto &c : pk.committed_queries) q.emplace_back(inst0 + (long)c.first, c.second);
     for (size_t s = 0; s < S; ++s) {
         q.emplace_back(z0 + (long)s, 0);
         q.emplace_back(z0 + (long)s, 1);
@@ -557,8 +570,8 @@ size_t plonk_proof_size(const PlonkKey &pk)
     std::set<std::set<int64_t>> sets;
     for (auto &kv : rots) sets.insert(kv.second);
     const size_t points = A + 2 * L + S + L + TR + 1 + pk.pieces + 2;
-    const size_t scalars = pk.advice_queries.size() + pk.fixed_queries.size() + 1 + pk.perm_cols + 3 * S - 1 +
-                           5 * L + TR + sets.size();
+    const size_t scalars = pk.committed_queries.size() + pk.advice_queries.size() + pk.fixed_queries.size() + 1 +
+                           pk.perm_cols + 3 * S - 1 + 5 * L + TR + sets.size();
     return 48 * points + 32 * scalars;
 }
 
@@ -650,10 +663,11 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, const RawFr 
 
     T.common_scalar(pk.repr);
 
-    // instance columns (compute_instances, prover.rs:1255-1286): each
-    // column's length and values into the transcript; Lagrange values
-    // (zero beyond the length) for the arguments, coefficients for the
-    // numerator
+    // instance columns (compute_instances, prover.rs:1255-1286): Lagrange
+    // values (zero beyond each column's length) for the arguments,
+    // coefficients for the numerator and the openings; into the transcript
+    // go the commitment of each committed column, then the length and values
+    // of each other column
     std::unique_ptr<DeviceVec> inst_lag;
     if (I) {
         std::vector<RawFr> iv(I * n);
@@ -662,13 +676,7 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, const RawFr 
         for (size_t c = 0; c < I; ++c) {
             const size_t len = instance_lens ? instance_lens[c] : 0;
             MH_CHECK(len <= pk.u, (int)hipErrorInvalidValue, "instance column longer than the usable rows");
-            T.common_scalar(HostFr::from_u64(len));
-            for (size_t i = 0; i < len; ++i) {
-                HostFr v;
-                std::memcpy(v.l, instance[off + i].limbs, 32);
-                T.common_scalar(v);
-                iv[c * n + i] = instance[off + i];
-            }
+            for (size_t i = 0; i < len; ++i) iv[c * n + i] = instance[off + i];
             off += len;
         }
         inst_lag = new_vec(I * n);
@@ -676,6 +684,23 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, const RawFr 
         MH_HIP(hipMemcpyAsync(pd + P_INST * n, inst_lag->values.get(), I * n * 32, hipMemcpyDeviceToDevice, st));
         ntt_batch(pd + P_INST * n, pk.k, I, true, pk.scratch(), st);
         MH_HIP(hipStreamSynchronize(st));   // iv goes out of scope
+        if (pk.committed) {
+            commit(com, pk, polys, P_INST, pk.committed);
+            for (auto &c : com) T.common_point(c);
+        }
+        off = 0;
+        for (size_t c = 0; c < I; ++c) {
+            const size_t len = instance_lens ? instance_lens[c] : 0;
+            if (c >= pk.committed) {
+                T.common_scalar(HostFr::from_u64(len));
+                for (size_t i = 0; i < len; ++i) {
+                    HostFr v;
+                    std::memcpy(v.l, instance[off + i].limbs, 32);
+                    T.common_scalar(v);
+                }
+            }
+            off += len;
+        }
     }
 
     // advice: Lagrange values (kept for the arguments), coefficients, commitments
@@ -833,6 +858,7 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, const RawFr 
             epolys.push_back(poly);
             qpt.push_back((uint32_t)pi);
         };
+        for (auto &q : pk.committed_queries) add(pd + (P_INST + q.first) * n, rotate(pk, x, q.second));
         for (auto &q : pk.advice_queries) add(pd + (P_ADV + q.first) * n, rotate(pk, x, q.second));
         for (auto &q : pk.fixed_queries) add(fixed + (size_t)q.first * n, rotate(pk, x, q.second));
         add(pd + P_RAND * n, x);
@@ -873,6 +899,7 @@ std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, const RawFr 
     // queries (compute_queries, prover.rs:1993-2080)
     std::vector<Query> queries;
     for (auto &q : pk.advice_queries) queries.push_back({pd + (P_ADV + q.first) * n, rotate(pk, x, q.second)});
+    for (auto &q : pk.committed_queries) queries.push_back({pd + (P_INST + q.first) * n, rotate(pk, x, q.second)});
     for (size_t s = 0; s < S; ++s) {
         queries.push_back({pd + (P_Z + s) * n, x});
         queries.push_back({pd + (P_Z + s) * n, x_next});
@@ -1021,6 +1048,11 @@ extern "C" RustError midnight_hip_plonk_keygen_with_arguments(
     for (size_t t = 0; t < args.trash_count; ++t)
         if (args.trash_selectors[t] >= fixed_cols)
             return make_error((int)hipErrorInvalidValue, "trash selector column out of range");
+    if (args.committed_instances > args.instance_cols || (args.instance_query_count && !args.instance_queries))
+        return make_error((int)hipErrorInvalidValue, "invalid instance arguments");
+    for (size_t i = 0; i < args.instance_query_count; ++i)
+        if (args.instance_queries[2 * i] >= args.instance_cols)
+            return make_error((int)hipErrorInvalidValue, "instance query column out of range");
     *out = nullptr;
     MH_TRY
     std::vector<std::pair<uint32_t, int32_t>> aq, fq;

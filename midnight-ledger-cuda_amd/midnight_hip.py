@@ -1120,7 +1120,9 @@ class Srs:
 
 class RawPlonkArguments(ctypes.Structure):
     """include/midnight_hip.h RawPlonkArguments"""
-    _fields_ = [("instance_cols", ctypes.c_size_t), ("lookup_count", ctypes.c_size_t),
+    _fields_ = [("instance_cols", ctypes.c_size_t), ("committed_instances", ctypes.c_size_t),
+                ("instance_queries", ctypes.c_void_p), ("instance_query_count", ctypes.c_size_t),
+                ("lookup_count", ctypes.c_size_t),
                 ("lookup_input_graphs", ctypes.c_void_p), ("lookup_table_graphs", ctypes.c_void_p),
                 ("lookup_graphs", ctypes.c_void_p), ("trash_count", ctypes.c_size_t),
                 ("trash_graphs", ctypes.c_void_p), ("trash_selectors", ctypes.c_void_p)]
@@ -1135,7 +1137,8 @@ class PlonkKey:
     numerator lookup graph); trash: list of (graph, selector fixed column)."""
 
     def __init__(self, k, srs, degree, fixed, advice_cols, advice_queries, fixed_queries, perm_columns,
-                 perm_mapping, gate, description=b"", instance_cols=0, lookups=(), trash=()):
+                 perm_mapping, gate, description=b"", instance_cols=0, lookups=(), trash=(), committed_instances=0,
+                 instance_queries=()):
         n = 1 << k
         fx = _fr_array(fixed) if fixed is not None and len(fixed) else None
         aq = np.ascontiguousarray([(c, r & 0xFFFFFFFF) for c, r in advice_queries], dtype=np.uint32).reshape(-1)
@@ -1150,7 +1153,9 @@ class PlonkKey:
         lg_arr, lg_ptr = _graphs([lk[2] for lk in lookups])
         tg_arr, tg_ptr = _graphs([t[0] for t in trash])
         tsel = np.ascontiguousarray([t[1] for t in trash], dtype=np.uint32)
-        args = RawPlonkArguments(instance_cols, len(lookups), li_ptr, lt_ptr, lg_ptr, len(trash), tg_ptr,
+        iq = np.ascontiguousarray([(c, r & 0xFFFFFFFF) for c, r in instance_queries], dtype=np.uint32).reshape(-1)
+        args = RawPlonkArguments(instance_cols, committed_instances, iq.ctypes.data if len(iq) else None,
+                                 len(instance_queries), len(lookups), li_ptr, lt_ptr, lg_ptr, len(trash), tg_ptr,
                                  tsel.ctypes.data if len(trash) else None)
         check(lib().midnight_hip_plonk_keygen_with_arguments(
             ctypes.byref(h), k, srs.handle, srs.npoints, degree, _ptr(fx), 0 if fx is None else fx.shape[0] // n,

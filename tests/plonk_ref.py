@@ -78,6 +78,9 @@ class Transcript:
     def common_scalar(self, f_int):
         self.absorb(f_int.to_bytes(32, "little"))
 
+    def common_point(self, jac):
+        self.absorb(bytes(O.compress(O.to_affine(jac))))
+
     def write_scalar(self, f_int):
         b = f_int.to_bytes(32, "little")
         self.absorb(b)
@@ -118,7 +121,8 @@ class Spec:
     """The constraint system of a circuit family: column counts, queries,
     gate, lookup and trash expressions, permuted columns, key description."""
 
-    def __init__(self, args=False):
+    def __init__(self, args=False, committed=0):
+        self.committed = committed
         if args:
             self.advice, self.fixed, self.instance = S.ADVICE_X, S.FIXED_X, S.INSTANCE_X
             self.perm = S.PERM_COLUMNS_X
@@ -137,7 +141,7 @@ class Spec:
 
     @staticmethod
     def of(circuit):
-        return Spec(isinstance(circuit, S.ArgCircuit))
+        return Spec(isinstance(circuit, S.ArgCircuit), getattr(circuit, "committed", 0))
 
 
 class CpuKey:
@@ -189,9 +193,9 @@ class VerifierKey:
     polynomials: sizes, domain constants and the SRS trapdoor (for the GPU
     keys of circuits too large for CpuKey)."""
 
-    def __init__(self, k, tau_int, args=False):
+    def __init__(self, k, tau_int, args=False, committed=0):
         self.k, self.n = k, 1 << k
-        self.spec = Spec(args)
+        self.spec = Spec(args, committed)
         self.tau = tau_int
         self.bf = S.BLINDING
         self.u = self.n - self.bf - 1
@@ -292,16 +296,21 @@ def cpu_prove(key, advice_cols, seed, threads=None, rng_threads=1, instance=None
     # instance columns (compute_instances, prover.rs:1255-1286)
     instance = instance if instance is not None else [[] for _ in range(I)]
     assert len(instance) == I
-    inst_lag = None
+    inst_lag, inst_coeff = None, []
     if I:
         vals = []
         for col in instance:
             assert len(col) <= u, "instance column longer than the usable rows"
-            T.common_scalar(len(col))
-            for v in col:
-                T.common_scalar(v % R)
             vals += [v % R for v in col] + [0] * (n - len(col))
         inst_lag = S.mont(vals)
+        inst_coeff = [O.ifft(inst_lag[i * n:(i + 1) * n], k, threads) for i in range(I)]
+        for i, col in enumerate(instance):
+            if i < sp.committed:
+                T.common_point(O.msm(key.srs, inst_coeff[i], threads))
+            else:
+                T.common_scalar(len(col))
+                for v in col:
+                    T.common_scalar(v % R)
     fixed_all = np.concatenate(key.fixed_lag)
     adv_coeff = [O.ifft(adv[i * n:(i + 1) * n], k, threads) for i in range(A)]
     for p in adv_coeff:
@@ -355,7 +364,7 @@ def cpu_prove(key, advice_cols, seed, threads=None, rng_threads=1, instance=None
     N = key.N
     ext = lambda ps: np.concatenate([O.coeff_to_extended(p, key.dom, threads) for p in ps]) if ps else None
     nu = O.numerator(N, fixed=key.fixed_ext, advice=ext(adv_coeff),
-                     instance=ext([O.ifft(inst_lag[i * n:(i + 1) * n], k, threads) for i in range(I)]),
+                     instance=ext(inst_coeff),
                      beta=to_fr(beta), gamma=to_fr(gamma),
                      theta=to_fr(theta), trash_challenge=to_fr(trash), y=to_fr(y),
                      omega=np.array(key.dom.fr("extended_omega")), g_coset=np.array(key.dom.fr("g_coset")),
@@ -380,6 +389,9 @@ def cpu_prove(key, advice_cols, seed, threads=None, rng_threads=1, instance=None
         T.write_point(O.msm(key.srs, p, threads))
     x = T.squeeze()
 
+    committed_q = [(col, rot) for col, rot in sp.iq if col < sp.committed]
+    for col, rot in committed_q:
+        T.write_scalar(_eval(inst_coeff[col], _rot(key, x, rot)))
     for col, rot in sp.aq:
         T.write_scalar(_eval(adv_coeff[col], _rot(key, x, rot)))
     for col, rot in sp.fq:
@@ -405,6 +417,7 @@ def cpu_prove(key, advice_cols, seed, threads=None, rng_threads=1, instance=None
     h_poly = _lincomb(h_pieces, [pow(sf, i, R) for i in range(key.pieces)], n)
 
     queries = [(("adv", col), adv_coeff[col], _rot(key, x, rot)) for col, rot in sp.aq]
+    queries += [(("inst", col), inst_coeff[col], _rot(key, x, rot)) for col, rot in committed_q]
     for s in range(Sn):
         queries += [(("z", s), z_coeff[s], x), (("z", s), z_coeff[s], x_next)]
     for s in range(Sn - 2, -1, -1):
@@ -519,10 +532,17 @@ def verify(vk, proof, key, instance=None):
     try:
         T = Transcript(proof)
         T.common_scalar(repr_int)
-        for col in instance:
-            T.common_scalar(len(col))
-            for v in col:
-                T.common_scalar(v % R)
+        # a committed column's commitment [p(tau)]G, p its Lagrange polynomial
+        inst_com = [g_mul(jac_of_affine(O.generator()),
+                          sum(v * _lagrange_at(key, key.tau, i) for i, v in enumerate(col)) % R)
+                    for col in instance[:sp.committed]]
+        for i, col in enumerate(instance):
+            if i < sp.committed:
+                T.common_point(inst_com[i])
+            else:
+                T.common_scalar(len(col))
+                for v in col:
+                    T.common_scalar(v % R)
         adv_com = [T.read_point() for _ in range(A)]
         theta = T.squeeze()
         lk_perm_com = [(T.read_point(), T.read_point()) for _ in range(L)]
@@ -535,6 +555,8 @@ def verify(vk, proof, key, instance=None):
         y = T.squeeze()
         h_com = [T.read_point() for _ in range(key.pieces)]
         x = T.squeeze()
+        committed_q = [(col, rot) for col, rot in sp.iq if col < sp.committed]
+        inst_ev = [T.read_scalar() for _ in committed_q]
         adv_ev = [T.read_scalar() for _ in sp.aq]
         fixed_ev = [T.read_scalar() for _ in sp.fq]
         rand_ev = T.read_scalar()
@@ -554,6 +576,8 @@ def verify(vk, proof, key, instance=None):
             return adv_ev[sp.aq.index((col, rot))]
         if kind == "fixed":
             return fixed_ev[sp.fq.index((col, rot))]
+        if col < sp.committed:
+            return inst_ev[committed_q.index((col, rot))]
         pt = _rot(key, x, rot)      # instance: interpolated from the values
         return sum(v * _lagrange_at(key, pt, i) for i, v in enumerate(instance[col])) % R
 
@@ -602,6 +626,7 @@ def verify(vk, proof, key, instance=None):
         h_comb = g_add(h_comb, g_mul(cm, pow(sf, i, R)))
     x_next, x_last, x_prev = _rot(key, x, 1), _rot(key, x, -(bf + 1)), _rot(key, x, -1)
     queries = [(("adv", col), adv_com[col], _rot(key, x, rot), adv_ev[i]) for i, (col, rot) in enumerate(sp.aq)]
+    queries += [(("inst", col), inst_com[col], _rot(key, x, rot), inst_ev[i]) for i, (col, rot) in enumerate(committed_q)]
     for s in range(Sn):
         queries += [(("z", s), z_com[s], x, z_ev[s][0]), (("z", s), z_com[s], x_next, z_ev[s][1])]
     for s in range(Sn - 2, -1, -1):

@@ -374,10 +374,13 @@ class ArgCircuit(Circuit):
     cells copy s1 values); a trashcan with selector qt on odd usable rows
     whose constraints vanish there; an instance column p of `m` public values
     equal to a's first rows, tied to them by the gate qp * (a - p) and by
-    copy constraints (the instance column is the ninth permuted column)."""
+    copy constraints (the instance column is the ninth permuted column);
+    with committed = 1 that column is a committed instance
+    (nb_committed_instances of create_proof)."""
 
-    def __init__(self, k, seed=1, m=6):
+    def __init__(self, k, seed=1, m=6, committed=0):
         super().__init__(k, seed, reserved={(0, r) for r in range(m)})
+        self.committed = committed      # leading instance columns that are committed
         rnd, n, u = self.rnd, self.n, self.u
         rv = lambda: rnd.randrange(R)
         a, b, c, d, e, q, s1, s2 = self.cols

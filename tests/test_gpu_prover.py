@@ -183,17 +183,19 @@ def gpu_arg_key(H, O, c, tau_int, extra_srs=0):
     fixed = S.mont([v for col in c.fixed_values for v in col])
     pk = H.PlonkKey(c.k, srs, S.DEGREE, fixed, S.ADVICE_X, S.ADVICE_QUERIES_X, S.FIXED_QUERIES_X,
                     S.PERM_COLUMNS_X, S.mapping_x(c), c.custom_graph(), S.DESCRIPTION_X,
-                    instance_cols=S.INSTANCE_X, lookups=c.lookup_graphs(), trash=c.trash_graphs())
+                    instance_cols=S.INSTANCE_X, lookups=c.lookup_graphs(), trash=c.trash_graphs(),
+                    committed_instances=c.committed, instance_queries=S.INSTANCE_QUERIES_X)
     return pk, srs
 
 
-@pytest.mark.parametrize("k,seed", [(7, 5), (10, 8), (12, 2)])
-def test_gpu_proof_with_lookups_trash_instance(H, O, k, seed):
+@pytest.mark.parametrize("k,seed,committed", [(7, 5, 0), (10, 8, 0), (12, 2, 0), (7, 4, 1), (11, 9, 1)])
+def test_gpu_proof_with_lookups_trash_instance(H, O, k, seed, committed):
     """lookups (two, one of two expressions), a trashcan and an instance
-    column tied to advice by a gate and copy constraints: GPU proof bytes
-    equal to the CPU prover's, accepted by the verifier with the instance and
-    rejected with other instance values; graph replays equal eager proofs."""
-    c = S.ArgCircuit(k, seed=seed)
+    column tied to advice by a gate and copy constraints, non-committed or
+    committed (nb_committed_instances = 1): GPU proof bytes equal to the CPU
+    prover's, accepted by the verifier with the instance and rejected with
+    other instance values; graph replays equal eager proofs."""
+    c = S.ArgCircuit(k, seed=seed, committed=committed)
     tau = 1357911 + k
     key = PR.CpuKey(c, tau)
     pk, srs = gpu_arg_key(H, O, c, tau)
@@ -201,7 +203,7 @@ def test_gpu_proof_with_lookups_trash_instance(H, O, k, seed):
     assert PR.to_int(rp) == key.repr
     adv = S.mont([v for col in c.advice_values for v in col])
     inst = [S.mont(col) for col in c.instance]
-    print("arg proof k=%d" % k, flush=True)
+    print("arg proof k=%d committed=%d" % (k, committed), flush=True)
     gpu = pk.prove(adv, seed, instance=inst)
     assert len(gpu) == pk.proof_size()
     cpu = PR.cpu_prove(key, c.advice_values, seed, instance=c.instance)

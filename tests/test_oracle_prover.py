@@ -245,3 +245,25 @@ def test_arg_circuit_wrong_witness(arg_setup):
     # the instance passed to the prover differs from the cells it is tied to
     wrong = [[(v + 1) % R for v in c.instance[0]]]
     assert not PR.verify(vk, PR.cpu_prove(key, c.advice_values, 42, instance=wrong), key, wrong)
+
+
+def test_arg_circuit_committed_instance():
+    """nb_committed_instances = 1: the instance column's commitment goes
+    into the transcript instead of its values, its query is evaluated after
+    x (before the advice) and opened; the verifier computes the commitment
+    from the public values"""
+    c = S.ArgCircuit(7, seed=3, committed=1)
+    key = PR.CpuKey(c, 24681357)
+    proof = PR.cpu_prove(key, c.advice_values, 42, instance=c.instance)
+    vk = (key.fixed_com, key.sigma_com, key.repr)
+    assert PR.verify(vk, proof, key, c.instance)
+    other = [[(c.instance[0][0] + 1) % R] + c.instance[0][1:]]
+    assert not PR.verify(vk, proof, key, other)
+    # the same witness proved with the column not committed: the advice
+    # commitments agree, theta and everything after it differ
+    c0 = S.ArgCircuit(7, seed=3, committed=0)
+    key0 = PR.CpuKey(c0, 24681357)
+    proof0 = PR.cpu_prove(key0, c0.advice_values, 42, instance=c0.instance)
+    assert len(proof) == len(proof0) + 32
+    assert proof[:48 * 7] == proof0[:48 * 7] and proof[48 * 7:48 * 8] != proof0[48 * 7:48 * 8]
+    assert not PR.verify(vk, proof0, key, c.instance)
