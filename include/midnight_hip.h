@@ -702,6 +702,13 @@ RustError midnight_hip_proof_server_create(void **out, uint32_t slots, uint64_t 
 /* Queues one proof (the advice is copied); *job_id identifies it. */
 RustError midnight_hip_proof_server_submit(void *server, void *pk, const RawFr *advice, size_t advice_len,
                                            uint64_t seed, uint64_t *job_id);
+/* The same with the proof's instance values (instance_cols columns of
+ * instance_lens[c] values, concatenated, as midnight_hip_plonk_prove_with_instances
+ * takes them); the values are copied at submission. */
+RustError midnight_hip_proof_server_submit_with_instances(void *server, void *pk, const RawFr *advice,
+                                                          size_t advice_len, const RawFr *instance,
+                                                          const size_t *instance_lens, size_t instance_cols,
+                                                          uint64_t seed, uint64_t *job_id);
 /* Waits for a job: proof bytes, and its time queued and proving (ms).  A
  * null or too small proof buffer is an error that sets *proof_len and keeps
  * the finished job, so the caller can wait again with a large enough one. */

@@ -530,6 +530,7 @@ size_t plonk_footprint(const PlonkKey &pk)
 }
 
 size_t plonk_advice_len(const PlonkKey &pk) { return pk.advice_cols * pk.n; }
+size_t plonk_instance_cols(const PlonkKey &pk) { return pk.instance_cols; }
 
 // Length of every proof of this key: the commitments and evaluations that
 // plonk_prove writes, with the multi-open's point sets formed as it forms

@@ -31,9 +31,11 @@
 namespace mh {
 
 struct PlonkKey;
-std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, uint64_t seed);
+std::vector<uint8_t> plonk_prove(PlonkKey &pk, const RawFr *advice, const RawFr *instance,
+                                 const size_t *instance_lens, uint64_t seed);
 size_t plonk_footprint(const PlonkKey &pk);
 size_t plonk_advice_len(const PlonkKey &pk);
+size_t plonk_instance_cols(const PlonkKey &pk);
 
 namespace {
 
@@ -45,7 +47,8 @@ double now_ms()
 struct ProofJob {
     uint64_t id = 0;
     PlonkKey *key = nullptr;
-    std::vector<RawFr> advice;
+    std::vector<RawFr> advice, instance;
+    std::vector<size_t> instance_lens;   // one per instance column of the key
     uint64_t seed = 0;
     size_t footprint = 0;
     double t_submit = 0, t_start = 0, t_end = 0;
@@ -106,7 +109,8 @@ struct ProofServer {
             int state = 2, code = 0;
             std::string err;
             try {
-                proof = plonk_prove(*job->key, job->advice.data(), job->seed);
+                proof = plonk_prove(*job->key, job->advice.data(), job->instance.data(), job->instance_lens.data(),
+                                    job->seed);
             } catch (const hip_error &e) {
                 code = e.code;
                 err = e.what();
@@ -155,18 +159,28 @@ extern "C" RustError midnight_hip_proof_server_create(void **out, uint32_t slots
     }
 }
 
-extern "C" RustError midnight_hip_proof_server_submit(void *server, void *pk, const RawFr *advice, size_t advice_len,
-                                                      uint64_t seed, uint64_t *job_id)
+extern "C" RustError midnight_hip_proof_server_submit_with_instances(void *server, void *pk, const RawFr *advice,
+                                                                     size_t advice_len, const RawFr *instance,
+                                                                     const size_t *instance_lens,
+                                                                     size_t instance_cols, uint64_t seed,
+                                                                     uint64_t *job_id)
 {
     if (!server || !pk || !advice || !job_id) return make_error((int)hipErrorInvalidValue, "invalid submit arguments");
     auto *s = reinterpret_cast<ProofServer *>(server);
     auto *key = reinterpret_cast<PlonkKey *>(pk);
     if (advice_len != plonk_advice_len(*key))
         return make_error((int)hipErrorInvalidValue, "advice length does not match the key");
+    if (instance_cols != plonk_instance_cols(*key) || (instance_cols && !instance_lens))
+        return make_error((int)hipErrorInvalidValue, "instance columns do not match the key");
+    size_t total = 0;
+    for (size_t c = 0; c < instance_cols; ++c) total += instance_lens[c];
+    if (total && !instance) return make_error((int)hipErrorInvalidValue, "null instance values");
     try {
         auto job = std::make_shared<ProofJob>();
         job->key = key;
         job->advice.assign(advice, advice + advice_len);
+        if (total) job->instance.assign(instance, instance + total);
+        job->instance_lens.assign(instance_lens, instance_lens + instance_cols);
         job->seed = seed;
         job->footprint = plonk_footprint(*key);
         {
@@ -182,6 +196,16 @@ extern "C" RustError midnight_hip_proof_server_submit(void *server, void *pk, co
     } catch (const std::exception &e) {
         return make_error(-1, e.what());
     }
+}
+
+extern "C" RustError midnight_hip_proof_server_submit(void *server, void *pk, const RawFr *advice, size_t advice_len,
+                                                      uint64_t seed, uint64_t *job_id)
+{
+    if (!pk) return make_error((int)hipErrorInvalidValue, "invalid submit arguments");
+    const size_t I = plonk_instance_cols(*reinterpret_cast<PlonkKey *>(pk));
+    std::vector<size_t> empty(I, 0);
+    return midnight_hip_proof_server_submit_with_instances(server, pk, advice, advice_len, nullptr,
+                                                          I ? empty.data() : nullptr, I, seed, job_id);
 }
 
 // Blocks until the job is finished; copies the proof and its queueing and

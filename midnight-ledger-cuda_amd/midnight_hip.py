@@ -223,6 +223,8 @@ def _declare(L):
         "midnight_hip_proof_server_create": (E, [vpp, u32, ctypes.c_uint64]),
         "midnight_hip_proof_server_submit": (E, [vp, vp, u64p, c_size, ctypes.c_uint64,
                                                  ctypes.POINTER(ctypes.c_uint64)]),
+        "midnight_hip_proof_server_submit_with_instances": (
+            E, [vp, vp, u64p, c_size, vp, vp, c_size, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
         "midnight_hip_proof_server_wait": (E, [vp, ctypes.c_uint64, ctypes.c_char_p, c_size, ctypes.POINTER(c_size),
                                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "midnight_hip_proof_server_free": (None, [vp]),
@@ -1226,11 +1228,20 @@ class ProofServer:
         check(lib().midnight_hip_proof_server_create(ctypes.byref(h), slots, hbm_budget_bytes))
         self.handle = h
 
-    def submit(self, pk, advice, seed):
+    def submit(self, pk, advice, seed, instance=None):
+        """instance: as PlonkKey.prove takes it (None: every column empty)"""
         a = _fr_array(advice)
         jid = ctypes.c_uint64()
-        check(lib().midnight_hip_proof_server_submit(self.handle, pk.handle, P(a), a.shape[0], seed,
-                                                     ctypes.byref(jid)))
+        if instance is None:
+            check(lib().midnight_hip_proof_server_submit(self.handle, pk.handle, P(a), a.shape[0], seed,
+                                                         ctypes.byref(jid)))
+            return jid.value
+        cols = [np.ascontiguousarray(c, dtype=np.uint64).reshape(-1, 4) for c in instance]
+        lens = np.ascontiguousarray([c.shape[0] for c in cols], dtype=np.uint64)
+        vals = np.concatenate(cols) if sum(c.shape[0] for c in cols) else None
+        check(lib().midnight_hip_proof_server_submit_with_instances(
+            self.handle, pk.handle, P(a), a.shape[0], None if vals is None else vals.ctypes.data,
+            lens.ctypes.data if len(lens) else None, len(lens), seed, ctypes.byref(jid)))
         return jid.value
 
     def wait(self, job, capacity=1 << 16):

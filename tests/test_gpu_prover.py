@@ -241,3 +241,35 @@ def test_gpu_proof_lookup_input_missing(H, O):
     assert pk.prove(good, 1, instance=inst) == PR.cpu_prove(key, c.advice_values, 1, instance=c.instance)
     pk.free()
     srs.free()
+
+
+def test_server_mixes_argument_circuits(H, O):
+    """the proof server with keys of lookups, a trashcan and an instance
+    column (committed and not) next to a benchmark-family key: every served
+    proof equals the one made alone and the CPU prover's, and verifies"""
+    circuits = [S.ArgCircuit(8, seed=21, committed=0), S.ArgCircuit(9, seed=22, committed=1)]
+    keys = [gpu_arg_key(H, O, c, 777 + c.k) for c in circuits]
+    base = S.Circuit(9, seed=23)
+    bkey = gpu_key(H, O, base, 555)
+    adv = [S.mont([v for col in c.advice_values for v in col]) for c in circuits]
+    inst = [[S.mont(col) for col in c.instance] for c in circuits]
+    badv = S.mont([v for col in base.cols[:5] for v in col])
+    alone = [pk.prove(adv[i], 5, instance=inst[i]) for i, (pk, _) in enumerate(keys)]
+    for i, c in enumerate(circuits):
+        cpu_key = PR.CpuKey(c, 777 + c.k)
+        assert alone[i] == PR.cpu_prove(cpu_key, c.advice_values, 5, instance=c.instance)
+        assert PR.verify((cpu_key.fixed_com, cpu_key.sigma_com, cpu_key.repr), alone[i], cpu_key, c.instance)
+    balone = bkey[0].prove(badv, 5)
+    server = H.ProofServer(4, 8 << 30)
+    jobs = []
+    for rep in range(3):
+        for i, (pk, _) in enumerate(keys):
+            jobs.append((i, server.submit(pk, adv[i], 5, instance=inst[i])))
+        jobs.append((-1, server.submit(bkey[0], badv, 5)))
+    for i, jid in jobs:
+        proof, _, _ = server.wait(jid)
+        assert proof == (balone if i < 0 else alone[i]), i
+    server.free()
+    for pk, srs in keys + [bkey]:
+        pk.free()
+        srs.free()
