@@ -516,6 +516,42 @@ def synthetic_circuit(H, k, seed):
     return np.concatenate([q, s1, s2]), np.concatenate([a, b, c, d, e]), mapping.reshape(-1)
 
 
+def synthetic_argument_circuit(H, k, seed, m=6):
+    """The benchmark circuit plus every argument of the prover: advice f, g
+    holding random rows of fixed tables t1, t2 (lookup (f, g) in (t1, t2)),
+    lookup q * e in s1 (e's copied cells' values placed in s1 at their rows,
+    its other cells taking s1 values), a trashcan with selector qt on odd
+    usable rows over (d - ab - c, e (d - ab - c)), and an instance column p
+    of m values equal to a's first rows, tied by the gate qp (a - p).
+    Returns (fixed, advice, instance, mapping)."""
+    n = 1 << k
+    u = n - 6
+    fixed, adv, mapping = synthetic_circuit(H, k, seed)
+    q, s1, s2 = (fixed[i * n:(i + 1) * n].copy() for i in range(3))
+    a, b, c, d, e = (adv[i * n:(i + 1) * n].copy() for i in range(5))
+    rng = np.random.default_rng(seed + 1)
+    m_ = mapping.reshape(8, n, 2)
+    copied = np.nonzero(m_[4, :u, 0] == 0)[0]         # e cells copying a
+    s1[copied] = e[copied]
+    free = np.setdiff1d(np.arange(u), copied)
+    e[free] = s1[rng.integers(0, u, free.size)]
+    dv = device_fr(H, random_canonical(2 * n, rng))
+    t = dv.download()
+    dv.free()
+    t1, t2 = t[:n].copy(), t[n:].copy()
+    t1[u:] = 0
+    t2[u:] = 0
+    rows = rng.integers(0, u, n)
+    f, g = t1[rows].copy(), t2[rows].copy()
+    qp = np.zeros((n, 4), dtype=np.uint64)
+    qp[:m] = MONT_ONE
+    qt = np.zeros((n, 4), dtype=np.uint64)
+    qt[1:u:2] = MONT_ONE
+    inst = a[:m].copy()
+    return (np.concatenate([q, s1, s2, t1, t2, qp, qt]), np.concatenate([a, b, c, d, e, f, g]), [inst],
+            mapping)
+
+
 def bench_prove(args, H):
     """One full proof of the benchmark circuit at k = log_n (default 20):
     5 advice and 3 fixed columns, gate q(ab + c - d), 8-column copy
@@ -541,17 +577,35 @@ def bench_prove(args, H):
                                              ("advice", 2, 0)), ("neg", ("advice", 3, 0))))
     graph = G.RawGraphBuffers(G.custom_gates_graph([gate]))
     perm = [(1, i) for i in range(5)] + [(0, i) for i in range(3)]
-    fixed, adv, mapping = synthetic_circuit(H, k, 2000 + k)
-    pk = H.PlonkKey(k, srs, 5, fixed, 5, [(c, 0) for c in range(5)], [(c, 0) for c in range(3)], perm, mapping,
-                    graph, b"benchmark circuit family")
+    inst = None
+    if args.arguments:
+        A_, F_ = ("advice", 0, 0), ("fixed", 0, 0)
+        adv_ = lambda i: ("advice", i, 0)
+        fix_ = lambda i: ("fixed", i, 0)
+        sub_ = lambda x, y: ("sum", x, ("neg", y))
+        gates = [gate, ("prod", fix_(5), sub_(A_, ("instance", 0, 0)))]
+        graph = G.RawGraphBuffers(G.custom_gates_graph(gates))
+        lk = [([adv_(5), adv_(6)], [fix_(3), fix_(4)]), ([("prod", F_, adv_(4))], [fix_(1)])]
+        lookups = [(G.RawGraphBuffers(G.compression_graph(i)), G.RawGraphBuffers(G.compression_graph(t)),
+                    G.RawGraphBuffers(G.lookup_graph(i, t))) for i, t in lk]
+        r_ = sub_(sub_(adv_(3), ("prod", adv_(0), adv_(1))), adv_(2))
+        trash = [(G.RawGraphBuffers(G.trash_graph([r_, ("prod", adv_(4), r_)])), 6)]
+        fixed, adv, inst, mapping = synthetic_argument_circuit(H, k, 2000 + k)
+        pk = H.PlonkKey(k, srs, 5, fixed, 7, [(c, 0) for c in range(7)], [(c, 0) for c in range(7)], perm,
+                        mapping, graph, b"benchmark circuit family with lookups, a trashcan and an instance",
+                        instance_cols=1, lookups=lookups, trash=trash)
+    else:
+        fixed, adv, mapping = synthetic_circuit(H, k, 2000 + k)
+        pk = H.PlonkKey(k, srs, 5, fixed, 5, [(c, 0) for c in range(5)], [(c, 0) for c in range(3)], perm,
+                        mapping, graph, b"benchmark circuit family")
     for i in range(args.warmup):
-        pk.prove(adv, 100 + i)
+        pk.prove(adv, 100 + i, instance=inst)
     H.profile_enable(True)
     lat = []
     t0 = time.perf_counter()
     for i in range(args.steps):
         t1 = time.perf_counter()
-        proof = pk.prove(adv, 200 + i)
+        proof = pk.prove(adv, 200 + i, instance=inst)
         lat.append(time.perf_counter() - t1)
     dt = (time.perf_counter() - t0) / args.steps
     msm_ms, msm_cnt = H.profile_read("msm_total")
@@ -570,8 +624,11 @@ def bench_prove(args, H):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic satisfying circuit of the benchmark family; SRS [tau^i]G built on the device",
-        "config": {"workload": "full proof, k=%d (2^%d rows), 5 advice / 3 fixed, gate q(ab+c-d), 8-column "
-                               "permutation, 4x extended domain" % (k, k), "proof_bytes": len(proof),
+        "config": {"workload": ("full proof, k=%d (2^%d rows), 5 advice / 3 fixed, gate q(ab+c-d), 8-column "
+                                "permutation, 4x extended domain" % (k, k)) +
+                               (" + advice f g, fixed t1 t2 qp qt, instance p: lookups (f, g) in (t1, t2) and "
+                                "q e in s1, trashcan qt, gate qp (a - p)" if args.arguments else ""),
+                   "proof_bytes": len(proof),
                    "srs_points": 1 << srs_log},
         "p50_ms": round(lat[len(lat) // 2] * 1e3, 3),
         "stage_ms_per_proof": {"commit_stages_graph": round(stage_ms / args.steps, 3),
@@ -582,7 +639,11 @@ def bench_prove(args, H):
                      "peak": round(peak / 1e12, 3), "peak_measured": round(peak_measured / 1e12, 3),
                      "unit": "TMAC/s", "frac": round(num_macs / t_num / peak, 4), "traffic": None},
     }
-    if args.cpu_seconds > 0:
+    if args.arguments:
+        # the numerator's product count above is the benchmark circuit's; the
+        # lookup and trash terms add products it does not count
+        res["roofline"] = None
+    if args.cpu_seconds > 0 and not args.arguments:
         res["cpu_baseline"] = cpu_baseline_prove(args.cpu_seconds)
     pk.free()
     srs.free()
@@ -966,6 +1027,8 @@ def main():
     ap.add_argument("--srs-log", type=int, default=0, help="prove: SRS of 2^max(log-n, srs-log) points")
     ap.add_argument("--msms-per-step", type=int, default=4)
     ap.add_argument("--skewed", action="store_true")
+    ap.add_argument("--arguments", action="store_true",
+                    help="prove: the circuit with two lookups, a trashcan and an instance column")
     ap.add_argument("--chain-len", type=int, default=1024, help="poseidon: hashes in the chain")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--requests", type=int, default=400, help="serving: requests in the stream")
