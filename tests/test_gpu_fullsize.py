@@ -12,6 +12,9 @@
   inverse NTT and its MSM the closed form [sum c_i t_i]G.
 - 2^20 -> 2^22 coset transform and the vanishing division / extended_to_coeff
   at 2^22 (poly/domain.rs:325-443) against the oracle.
+- A circuit with two lookups, a trashcan and a committed instance column at
+  2^15 rows: accepted by the designated verifier, rejected tampered or with
+  other instance values.
 """
 import numpy as np
 import pytest
@@ -162,3 +165,29 @@ def test_coset_and_vanishing_division_2p22(H, O):
     assert not back[n:].any()
     ext.free()
     src.free()
+
+
+def test_argument_circuit_k15_accepted_by_verifier(H, O):
+    """the circuit with two lookups, a trashcan and a committed instance
+    column at 2^15 rows: the GPU proof is accepted by the designated verifier
+    (its numerator from the expression trees), rejected tampered or with
+    other instance values"""
+    k = 15
+    tau = 0x1badc0ffee15
+    print("args k=15: circuit", flush=True)
+    c = S.ArgCircuit(k, seed=1515, committed=1)
+    srs = _srs(H, O, 1 << k, tau)
+    print("args k=15: keygen", flush=True)
+    pk = H.PlonkKey(k, srs, S.DEGREE, S.mont([v for col in c.fixed_values for v in col]), S.ADVICE_X,
+                    S.ADVICE_QUERIES_X, S.FIXED_QUERIES_X, S.PERM_COLUMNS_X, S.mapping_x(c), c.custom_graph(),
+                    S.DESCRIPTION_X, instance_cols=S.INSTANCE_X, lookups=c.lookup_graphs(), trash=c.trash_graphs(),
+                    committed_instances=1, instance_queries=S.INSTANCE_QUERIES_X)
+    print("args k=15: prove", flush=True)
+    proof = pk.prove(S.mont([v for col in c.advice_values for v in col]), 15, instance=[S.mont(c.instance[0])])
+    assert len(proof) == pk.proof_size()
+    vkey = PR.VerifierKey(k, tau, args=True, committed=1)
+    assert PR.verify(_vk(pk), proof, vkey, c.instance)
+    assert not PR.verify(_vk(pk), _tampered(proof), vkey, c.instance)
+    assert not PR.verify(_vk(pk), proof, vkey, [[(c.instance[0][0] + 1) % S.R] + c.instance[0][1:]])
+    pk.free()
+    srs.free()
